@@ -1,0 +1,201 @@
+/* astro_abi.h -- C ABI of the MI355X (gfx950) AStrO hot-path library
+ * (misc_amd/libastro_hip.so).
+ *
+ * Two layers:
+ *
+ * 1. Routine-for-routine replacements of the reference's Fortran solvers and
+ *    book-keeping, with the reference's argument lists and index conventions
+ *    (1-based values, (0:n) prefix ranges as n+1 ints, column value 0 = unused
+ *    slot, column-major arrays).  All pointers are HOST pointers; each call is
+ *    synchronous.  A Fortran host binds them with ISO_C_BINDING (scalars by
+ *    VALUE, arrays by reference).  Every function returns an astro_status.
+ *
+ * 2. A device-resident model context (astro_ctx_*) that keeps mesh, matrices,
+ *    profile factor and Krylov workspace in HBM between calls, for the
+ *    assemble -> profile -> factor -> GMRES sequence of one solve, without
+ *    host round trips.
+ *
+ * No torch types cross this boundary; the library allocates device memory
+ * with hipMalloc and owns it until the matching free/destroy call.
+ */
+#ifndef ASTRO_ABI_H
+#define ASTRO_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ASTRO_OK = 0,
+    ASTRO_ERR_ARG = 1,          /* invalid argument or size */
+    ASTRO_ERR_DEVICE = 2,       /* HIP runtime error */
+    ASTRO_ERR_ZERO_PIVOT = 3,   /* D(i) == 0 in the LDL^T factor */
+    ASTRO_ERR_NO_CONVERGE = 4,  /* GMRES hit its cycle limit */
+    ASTRO_ERR_TIMEOUT = 5,      /* a device-side bounded wait expired */
+    ASTRO_ERR_UNSUPPORTED = 6   /* element type / option not implemented */
+} astro_status;
+
+/* Last error message of the calling thread ("" if none). */
+const char* astro_last_error(void);
+/* Library version string. */
+const char* astro_version(void);
+
+/* ---- Layer 1: reference routine replacements --------------------------- */
+
+/* In-place row-skyline LDL^T.  Replaces getSparseLDLFact
+ * (Working/AStrO_solvers.f90:10-78): ALTri(ASize) holds rows i = 1..n as
+ * columns i-len+1..i with the diagonal last, ARange(0:n) the prefix ends.  On
+ * return the strict lower part holds L (unit diagonal implicit) and the
+ * diagonal holds D. */
+int astro_getSparseLDLFact(double* ALTri, int64_t ASize, const int32_t* ARange, int32_t n);
+
+/* Solve A x = b with the factor of astro_getSparseLDLFact.  Replaces
+ * solveSparseLDLFact (Working/AStrO_solvers.f90:83-126); zVec (Ldim) receives
+ * the forward-substitution intermediate z = L^-1 b as in the reference. */
+int astro_solveSparseLDLFact(double* soln, const double* rhs, const double* Lmat, int64_t LSize,
+                             const int32_t* LRange, int32_t Ldim, double* zVec);
+
+/* Left-preconditioned GMRES(numVecs) on (A + C^T C) x = b with the factored
+ * profile P as preconditioner.  Replaces gMRes (Working/AStrO_solvers.f90:128-297);
+ * xVec is the initial guess on entry.  The stopping rule is |P^-1 (b - A x)| <=
+ * max(rtol * |P^-1 (b - A x0)|, atol); the reference's rule is atol = 1e-12,
+ * rtol = 0 (pass rtol <= 0 to keep it).  At most maxIt/numVecs + 1 restart
+ * cycles, as in the reference. */
+int astro_gMRes(double* xVec, const double* Amat, const int32_t* Acols, const int32_t* ARange, int64_t ASize,
+                int32_t ADim, const double* Cmat, const int32_t* Ccols, const int32_t* CRange, int64_t CSize,
+                int32_t CDim, const double* Pmat, const int32_t* PRange, int64_t PSize, const double* bVec,
+                int32_t numVecs, int32_t maxIt, double rtol, double atol);
+
+/* Profile from CSR plus the penalty C^T C.  Replaces convertToLTri
+ * (Working/AStrO_bookKeeping.f90:754-792); lTri is overwritten. */
+int astro_convertToLTri(double* lTri, const int32_t* lTRange, int64_t lTSize, const double* mainMat,
+                        const int32_t* mainCols, const int32_t* mainRange, int64_t mainSize, int32_t mainDim,
+                        const double* cMat, const int32_t* cCols, const int32_t* cRange, int64_t cSize,
+                        int32_t cDim);
+
+/* ---- Host symbolic analysis (no device work) ----------------------------- */
+
+typedef struct astro_prep astro_prep;
+
+/* ---- Layer 2: device-resident model context ----------------------------- */
+
+typedef struct astro_ctx astro_ctx;
+
+/* Model description (host pointers, copied at create).  Mirrors the model
+ * arrays of Working/AStrO_globalData.f90:5-58 after readModelInput:
+ *   nodeList(3,numNodes), elementList(8,numEls) 1-based with 0 padding,
+ *   elementType(numEls) (8 = brick8), per-element engineering constants
+ *   eprops(9,numEls) = (E1,E2,E3,nu12,nu13,nu23,G12,G13,G23), density(numEls),
+ *   conductivity cond(6,numEls) = (k11,k22,k33,k12,k13,k23) (may be NULL),
+ *   constraint terms (eqn, dof 1..7, coef, node-or-set) with the node lists of
+ *   set terms, and nodal loads (node id, 7 components). */
+typedef struct {
+    int32_t numNodes, numEls;
+    const double* nodeList;
+    const int32_t* elementList;
+    const int32_t* elementType;
+    const double* eprops;
+    const double* density;
+    const double* cond;
+    int32_t numMpcTerms;
+    const int32_t* mpcEqn;
+    const int32_t* mpcDof;
+    const double* mpcCoef;
+    const int32_t* mpcIsSet;
+    const int32_t* mpcNodeStart; /* numMpcTerms + 1 */
+    const int32_t* mpcNodes;
+    int32_t numMPC;
+    const double* mpcRHS;
+    int32_t numLoads;
+    const int32_t* loadNode;
+    const double* loadVal; /* 7 per load */
+} astro_model;
+
+/* Options of the analysis (solver parameters of the *solve command). */
+typedef struct {
+    int32_t maxBW;            /* <= 0: reference rule from numNodes */
+    int32_t literalMaxBW;     /* 1: 27*N*N in 32-bit wraparound as the reference; 0: 64-bit */
+    int32_t thermLiteralD8;   /* 1: thermal scatter never fills a row's last slot, as the reference */
+    int32_t numVecs;          /* GMRES restart length; <= 0: reference rule (n/2 below 200, else 100) */
+    double rtol;              /* GMRES relative tolerance on |P^-1 r| (default 1e-12 when <= 0) */
+    int32_t maxCycles;        /* GMRES restart cycles cap; <= 0: 1000 */
+} astro_options;
+
+/* Host-only analysisPrep (Working/AStrO_bookKeeping.f90:794-1039): level
+ * ordering, DOF numbering, CSR ranges and first-touch columns, profile ranges,
+ * constraint matrices.  Needs no GPU.  Arrays are read with
+ * astro_prep_get_i32 / astro_prep_get_f64 under the names of
+ * astro_ctx_get_i32 plus the double arrays "elMPCMat", "elMPCRHS". */
+int astro_prep_create(const astro_model* model, const astro_options* opts, astro_prep** out);
+int astro_prep_destroy(astro_prep* prep);
+int64_t astro_prep_scalar(const astro_prep* prep, const char* name);
+int astro_prep_get_i32(const astro_prep* prep, const char* name, int32_t* host, int64_t count);
+int astro_prep_get_f64(const astro_prep* prep, const char* name, double* host, int64_t count);
+
+/* Create a context on the current HIP device: symbolic analysis on the host
+ * (analysisPrep, Working/AStrO_bookKeeping.f90:794-1039) and upload. */
+int astro_ctx_create(const astro_model* model, const astro_options* opts, astro_ctx** out);
+int astro_ctx_destroy(astro_ctx* ctx);
+
+/* Scalars: "elMatDim", "elMatSize", "elMatLTSize", "numSolidNds", "solverMaxBW",
+ * "elMPCDim", "thermMatSize", "thermMatLTSize", "gmresSteps", "gmresCycles",
+ * "numColors".  Returns -1 for an unknown name. */
+int64_t astro_ctx_scalar(const astro_ctx* ctx, const char* name);
+double astro_ctx_scalar_f(const astro_ctx* ctx, const char* name);
+
+/* Copy a context array to host memory: int arrays "currentRank", "nDofIndex",
+ * "elMatRange", "elMatCols", "elMatLTRange", "elMPCMatRange", "elMPCMatCols",
+ * "thermMatRange", "thermMatCols", "thermMatLTRange"; double arrays
+ * "elasticMat", "elMatLT", "elMPCMat", "elMPCRHS", "elasticLoad", "nodeDisp",
+ * "thermMat", "thermMatLT", "thermalLoad", "nodeTemp".  count is the element
+ * count of the host buffer; it must equal the array length. */
+int astro_ctx_get_i32(astro_ctx* ctx, const char* name, int32_t* host, int64_t count);
+int astro_ctx_get_f64(astro_ctx* ctx, const char* name, double* host, int64_t count);
+/* Overwrite a double array of the context from host memory. */
+int astro_ctx_set_f64(astro_ctx* ctx, const char* name, const double* host, int64_t count);
+
+/* Elastic hot path, all on the device, asynchronous on the context stream
+ * unless stated: */
+/* getElasticSolnLoad(1) for brick8 at u = 0 (Working/AStrO_commandFunctions.f90:260-351):
+ * element tangents scattered into the CSR values.  massFactor adds
+ * massFactor * M_e (Newmark: 1/(beta dt^2)). */
+int astro_ctx_assemble_elastic(astro_ctx* ctx, double massFactor);
+/* scaleElasticMPC (Working/AStrO_commandFunctions.f90:537-566). */
+int astro_ctx_scale_mpc(astro_ctx* ctx);
+/* convertToLTri for the elastic profile. */
+int astro_ctx_build_profile(astro_ctx* ctx);
+/* getSparseLDLFact on the elastic profile (in place).  Synchronous; returns
+ * ASTRO_ERR_ZERO_PIVOT if a pivot is zero. */
+int astro_ctx_factor(astro_ctx* ctx);
+/* stepSolve for a linear static elastic step: RHS (nodal + constraint loads,
+ * Working/AStrO_commandFunctions.f90:353-535), GMRES from delDisp, nodeDisp +=
+ * delDisp.  Synchronous. */
+int astro_ctx_solve(astro_ctx* ctx);
+/* Whole solve() sequence: assemble, scale, profile, factor, solve. */
+int astro_ctx_static_solve(astro_ctx* ctx);
+
+/* Thermal twins (Working/AStrO_commandFunctions.f90:16-74, 229-258, 689-703,
+ * 763-769). */
+int astro_ctx_assemble_thermal(astro_ctx* ctx);
+int astro_ctx_thermal_solve(astro_ctx* ctx);
+
+/* Preconditioner apply on device vectors of length elMatDim:
+ * x = P^-1 b via the two triangular sweeps. */
+int astro_ctx_precond_apply(astro_ctx* ctx, const double* d_b, double* d_x);
+/* Operator apply y = (A + C^T C) x on device vectors. */
+int astro_ctx_operator_apply(astro_ctx* ctx, const double* d_x, double* d_y);
+
+/* Wait for all work queued on the context stream. */
+int astro_ctx_sync(astro_ctx* ctx);
+/* Per-phase device times (ms) of the last astro_ctx_static_solve, measured
+ * with events on the context stream: "assemble", "scale", "profile",
+ * "factor", "solve". */
+double astro_ctx_phase_ms(const astro_ctx* ctx, const char* phase);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTRO_ABI_H */

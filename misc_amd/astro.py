@@ -1,0 +1,355 @@
+"""Python binding of the MI355X hot-path library (misc_amd/libastro_hip.so).
+
+The functions keep the names, argument meaning and index conventions of the
+reference routines they replace (Working/AStrO_solvers.f90,
+Working/AStrO_bookKeeping.f90): 1-based column values with 0 for an unused
+slot, (0:n) range arrays of n + 1 entries, rows of a skyline profile holding
+columns i-len+1..i with the diagonal last.  Errors raise AstroError with the
+library's status and message.  There is no fallback: if the library is not
+built, importing the binding raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libastro_hip.so")
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "zero pivot", 4: "no convergence",
+          5: "device wait timeout", 6: "unsupported"}
+
+# Exported symbols (include/astro_abi.h); checked by the CPU tests.
+EXPORTS = [
+    "astro_last_error", "astro_version", "astro_getSparseLDLFact", "astro_solveSparseLDLFact", "astro_gMRes",
+    "astro_convertToLTri", "astro_ctx_create", "astro_ctx_destroy", "astro_ctx_scalar", "astro_ctx_scalar_f",
+    "astro_ctx_get_i32", "astro_ctx_get_f64", "astro_ctx_set_f64", "astro_ctx_assemble_elastic",
+    "astro_ctx_scale_mpc", "astro_ctx_build_profile", "astro_ctx_factor", "astro_ctx_solve",
+    "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_precond_apply",
+    "astro_ctx_operator_apply", "astro_ctx_sync", "astro_ctx_phase_ms", "astro_prep_create", "astro_prep_destroy",
+    "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64",
+]
+
+
+class AstroError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64 = C.c_int64
+_i32 = C.c_int32
+_lib = None
+
+
+class _Model(C.Structure):
+    _fields_ = [("numNodes", C.c_int32), ("numEls", C.c_int32), ("nodeList", C.c_void_p),
+                ("elementList", C.c_void_p), ("elementType", C.c_void_p), ("eprops", C.c_void_p),
+                ("density", C.c_void_p), ("cond", C.c_void_p), ("numMpcTerms", C.c_int32), ("mpcEqn", C.c_void_p),
+                ("mpcDof", C.c_void_p), ("mpcCoef", C.c_void_p), ("mpcIsSet", C.c_void_p),
+                ("mpcNodeStart", C.c_void_p), ("mpcNodes", C.c_void_p), ("numMPC", C.c_int32),
+                ("mpcRHS", C.c_void_p), ("numLoads", C.c_int32), ("loadNode", C.c_void_p),
+                ("loadVal", C.c_void_p)]
+
+
+class _Options(C.Structure):
+    _fields_ = [("maxBW", C.c_int32), ("literalMaxBW", C.c_int32), ("thermLiteralD8", C.c_int32),
+                ("numVecs", C.c_int32), ("rtol", C.c_double), ("maxCycles", C.c_int32)]
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the library; raises if it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    L.astro_last_error.restype = C.c_char_p
+    L.astro_version.restype = C.c_char_p
+    L.astro_getSparseLDLFact.argtypes = [_dp, _i64, _ip, _i32]
+    L.astro_solveSparseLDLFact.argtypes = [_dp, _dp, _dp, _i64, _ip, _i32, _dp]
+    L.astro_gMRes.argtypes = [_dp, _dp, _ip, _ip, _i64, _i32, _dp, _ip, _ip, _i64, _i32, _dp, _ip, _i64, _dp, _i32,
+                              _i32, C.c_double, C.c_double]
+    L.astro_convertToLTri.argtypes = [_dp, _ip, _i64, _dp, _ip, _ip, _i64, _i32, _dp, _ip, _ip, _i64, _i32]
+    L.astro_ctx_create.argtypes = [C.POINTER(_Model), C.POINTER(_Options), C.POINTER(C.c_void_p)]
+    L.astro_ctx_destroy.argtypes = [C.c_void_p]
+    L.astro_ctx_scalar.restype = C.c_int64
+    L.astro_ctx_scalar.argtypes = [C.c_void_p, C.c_char_p]
+    L.astro_ctx_scalar_f.restype = C.c_double
+    L.astro_ctx_scalar_f.argtypes = [C.c_void_p, C.c_char_p]
+    L.astro_ctx_get_i32.argtypes = [C.c_void_p, C.c_char_p, _ip, _i64]
+    L.astro_ctx_get_f64.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
+    L.astro_ctx_set_f64.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
+    L.astro_ctx_assemble_elastic.argtypes = [C.c_void_p, C.c_double]
+    for f in ("astro_ctx_scale_mpc", "astro_ctx_build_profile", "astro_ctx_factor", "astro_ctx_solve",
+              "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_sync"):
+        getattr(L, f).argtypes = [C.c_void_p]
+    L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.astro_ctx_operator_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.astro_prep_create.argtypes = [C.POINTER(_Model), C.POINTER(_Options), C.POINTER(C.c_void_p)]
+    L.astro_prep_destroy.argtypes = [C.c_void_p]
+    L.astro_prep_scalar.restype = C.c_int64
+    L.astro_prep_scalar.argtypes = [C.c_void_p, C.c_char_p]
+    L.astro_prep_get_i32.argtypes = [C.c_void_p, C.c_char_p, _ip, _i64]
+    L.astro_prep_get_f64.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
+    L.astro_ctx_phase_ms.restype = C.c_double
+    L.astro_ctx_phase_ms.argtypes = [C.c_void_p, C.c_char_p]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise AstroError(rc, load_library().astro_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Reference routine replacements (host arrays, synchronous)
+
+def getSparseLDLFact(ALTri, ARange, n):
+    """In-place LDL^T of a row-skyline profile (Working/AStrO_solvers.f90:10-78).
+    Returns the factored array (a new array; the input is not modified)."""
+    A = _f64(ALTri).copy()
+    _check(load_library().astro_getSparseLDLFact(A, A.size, _i(ARange), int(n)))
+    return A
+
+
+def solveSparseLDLFact(Lmat, LRange, rhs):
+    """x = (L D L^T)^-1 rhs (Working/AStrO_solvers.f90:83-126); returns (soln, zVec)."""
+    rhs = _f64(rhs)
+    n = rhs.size
+    x = np.zeros(n)
+    z = np.zeros(n)
+    L = _f64(Lmat)
+    _check(load_library().astro_solveSparseLDLFact(x, rhs, L, L.size, _i(LRange), n, z))
+    return x, z
+
+
+def gMRes(xVec, Amat, Acols, ARange, Cmat, Ccols, CRange, Pmat, PRange, bVec, numVecs, maxIt, rtol=1e-12,
+          atol=0.0):
+    """Preconditioned GMRES(numVecs) (Working/AStrO_solvers.f90:128-297).  xVec is
+    the initial guess; returns the solution.  rtol <= 0 and atol <= 0 select the
+    reference's absolute 1e-12 stop."""
+    x = _f64(xVec).copy()
+    A, Ac, Ar = _f64(Amat), _i(Acols), _i(ARange)
+    cdim = len(CRange) - 1 if CRange is not None and len(CRange) > 1 else 0
+    Cm = _f64(Cmat) if cdim else np.zeros(1)
+    Cc = _i(Ccols) if cdim else np.zeros(1, np.int32)
+    Cr = _i(CRange) if cdim else np.zeros(1, np.int32)
+    P = _f64(Pmat)
+    _check(load_library().astro_gMRes(x, A, Ac, Ar, A.size, x.size, Cm, Cc, Cr, Cm.size, cdim, P, _i(PRange), P.size,
+                                      _f64(bVec), int(numVecs), int(maxIt), float(rtol), float(atol)))
+    return x
+
+
+def convertToLTri(lTRange, mainMat, mainCols, mainRange, cMat=None, cCols=None, cRange=None):
+    """Profile from CSR plus penalty C^T C (Working/AStrO_bookKeeping.f90:754-792)."""
+    lTRange = _i(lTRange)
+    n = lTRange.size - 1
+    lt = np.zeros(int(lTRange[-1]))
+    M = _f64(mainMat)
+    cdim = len(cRange) - 1 if cRange is not None and len(cRange) > 1 else 0
+    Cm = _f64(cMat) if cdim else np.zeros(1)
+    Cc = _i(cCols) if cdim else np.zeros(1, np.int32)
+    Cr = _i(cRange) if cdim else np.zeros(1, np.int32)
+    _check(load_library().astro_convertToLTri(lt, lTRange, lt.size, M, _i(mainCols), _i(mainRange), M.size, n, Cm,
+                                              Cc, Cr, Cm.size, cdim))
+    return lt
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class Options:
+    maxBW: int = 0
+    literalMaxBW: bool = True
+    thermLiteralD8: bool = True
+    numVecs: int = 0
+    rtol: float = 1e-12
+    maxCycles: int = 1000
+
+
+def _marshal(model, o: "Options"):
+    keep = []
+
+    def ptr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    M = _Model()
+    M.numNodes = model.num_nodes
+    M.numEls = model.num_els
+    M.nodeList = ptr(model.nodes.reshape(-1), np.float64)
+    M.elementList = ptr(model.elements.reshape(-1), np.int32)
+    M.elementType = ptr(model.etype, np.int32)
+    M.eprops = ptr(model.eprops.reshape(-1), np.float64)
+    M.density = ptr(model.density, np.float64)
+    M.cond = ptr(model.cond.reshape(-1), np.float64)
+    if model.mpc:
+        M.numMpcTerms = len(model.mpc)
+        M.mpcEqn = ptr([t.eqn for t in model.mpc], np.int32)
+        M.mpcDof = ptr([t.dof for t in model.mpc], np.int32)
+        M.mpcCoef = ptr([t.coef for t in model.mpc], np.float64)
+        M.mpcIsSet = ptr([1 if t.is_set else 0 for t in model.mpc], np.int32)
+        starts = np.zeros(len(model.mpc) + 1, np.int32)
+        starts[1:] = np.cumsum([len(t.nodes) for t in model.mpc])
+        M.mpcNodeStart = ptr(starts, np.int32)
+        M.mpcNodes = ptr(np.concatenate([t.nodes for t in model.mpc]), np.int32)
+        M.numMPC = len(model.mpc_rhs)
+        M.mpcRHS = ptr(model.mpc_rhs, np.float64)
+    M.numLoads = int(model.load_nodes.size)
+    if M.numLoads:
+        M.loadNode = ptr(model.load_nodes, np.int32)
+        M.loadVal = ptr(model.load_vals.reshape(-1), np.float64)
+    O = _Options(int(o.maxBW), 1 if o.literalMaxBW else 0, 1 if o.thermLiteralD8 else 0, int(o.numVecs),
+                 float(o.rtol), int(o.maxCycles))
+    return M, O, keep
+
+
+class Prep:
+    """Host-only analysisPrep of the library (no device needed)."""
+
+    def __init__(self, model, options: Optional[Options] = None):
+        L = load_library()
+        self.model = model
+        M, O, keep = _marshal(model, options or Options())
+        h = C.c_void_p()
+        _check(L.astro_prep_create(C.byref(M), C.byref(O), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load_library().astro_prep_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def scalar(self, name):
+        return int(load_library().astro_prep_scalar(self.h, name.encode()))
+
+    def geti(self, name):
+        n = self.scalar
+        N = self.model.num_nodes
+        sizes = {"currentRank": N, "nDofIndex": 6 * N, "elMatRange": n("elMatDim") + 1,
+                 "elMatLTRange": n("elMatDim") + 1, "elMatCols": n("elMatSize"), "elMPCMatRange": n("elMPCDim") + 1,
+                 "elMPCMatCols": n("elMPCSize"), "thermMatRange": N + 1, "thermMatLTRange": N + 1,
+                 "thermMatCols": n("thermMatSize"), "thermMPCMatRange": n("thermMPCDim") + 1}
+        out = np.zeros(sizes[name], np.int32)
+        _check(load_library().astro_prep_get_i32(self.h, name.encode(), out, out.size))
+        return out
+
+    def getd(self, name):
+        sizes = {"elMPCMat": self.scalar("elMPCSize"), "elMPCRHS": self.scalar("elMPCDim")}
+        out = np.zeros(sizes[name])
+        _check(load_library().astro_prep_get_f64(self.h, name.encode(), out, out.size))
+        return out
+
+
+class Context:
+    """Device-resident model: the reference's global state of one job in HBM."""
+
+    def __init__(self, model, options: Optional[Options] = None):
+        L = load_library()
+        self.model = model
+        M, O, keep = _marshal(model, options or Options())
+        h = C.c_void_p()
+        _check(L.astro_ctx_create(C.byref(M), C.byref(O), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().astro_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def scalar(self, name):
+        return int(load_library().astro_ctx_scalar(self.h, name.encode()))
+
+    def scalar_f(self, name):
+        return float(load_library().astro_ctx_scalar_f(self.h, name.encode()))
+
+    def _len(self, name):
+        n = self.scalar
+        sizes = {
+            "currentRank": self.model.num_nodes, "nDofIndex": 6 * self.model.num_nodes,
+            "elMatRange": n("elMatDim") + 1, "elMatLTRange": n("elMatDim") + 1, "elMatCols": n("elMatSize"),
+            "elasticMat": n("elMatSize"), "elMatLT": n("elMatLTSize"), "elasticLoad": n("elMatDim"),
+            "nodeDisp": n("elMatDim"), "delDisp": n("elMatDim"), "thermMatRange": self.model.num_nodes + 1,
+            "thermMatLTRange": self.model.num_nodes + 1, "thermMatCols": n("thermMatSize"),
+            "thermMat": n("thermMatSize"), "thermMatLT": n("thermMatLTSize"), "thermalLoad": self.model.num_nodes,
+            "nodeTemp": self.model.num_nodes,
+        }
+        return sizes[name]
+
+    def geti(self, name, count=None):
+        count = self._len(name) if count is None else count
+        out = np.zeros(count, np.int32)
+        _check(load_library().astro_ctx_get_i32(self.h, name.encode(), out, count))
+        return out
+
+    def getd(self, name, count=None):
+        count = self._len(name) if count is None else count
+        out = np.zeros(count)
+        _check(load_library().astro_ctx_get_f64(self.h, name.encode(), out, count))
+        return out
+
+    def getd_sized(self, name, count):
+        out = np.zeros(count)
+        _check(load_library().astro_ctx_get_f64(self.h, name.encode(), out, count))
+        return out
+
+    def setd(self, name, values):
+        v = _f64(values)
+        _check(load_library().astro_ctx_set_f64(self.h, name.encode(), v, v.size))
+
+    def assemble(self, mass_factor=0.0):
+        _check(load_library().astro_ctx_assemble_elastic(self.h, float(mass_factor)))
+
+    def scale_mpc(self):
+        _check(load_library().astro_ctx_scale_mpc(self.h))
+
+    def build_profile(self):
+        _check(load_library().astro_ctx_build_profile(self.h))
+
+    def factor(self):
+        _check(load_library().astro_ctx_factor(self.h))
+
+    def solve(self):
+        _check(load_library().astro_ctx_solve(self.h))
+
+    def static_solve(self):
+        _check(load_library().astro_ctx_static_solve(self.h))
+
+    def assemble_thermal(self):
+        _check(load_library().astro_ctx_assemble_thermal(self.h))
+
+    def thermal_solve(self):
+        _check(load_library().astro_ctx_thermal_solve(self.h))
+
+    def sync(self):
+        _check(load_library().astro_ctx_sync(self.h))
+
+    def phase_ms(self, phase):
+        return float(load_library().astro_ctx_phase_ms(self.h, phase.encode()))

@@ -1,0 +1,1210 @@
+// astro_abi.hip -- host orchestration and the C ABI (include/astro_abi.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/astro_abi.h"
+#include "astro_kernels.h"
+#include "astro_prep.h"
+
+namespace astro {
+
+static thread_local std::string g_lastError;
+
+static int fail(int code, const std::string& msg) {
+    g_lastError = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ASTRO_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+#define ASTRO_TRY(expr)            \
+    do {                           \
+        int s_ = (expr);           \
+        if (s_ != ASTRO_OK) return s_; \
+    } while (0)
+
+// Device buffer owned by RAII.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return ASTRO_OK;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(ASTRO_ERR_DEVICE, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes: " +
+                                              hipGetErrorString(e));
+        }
+        return ASTRO_OK;
+    }
+    int upload(const T* host, size_t count, hipStream_t s) {
+        ASTRO_TRY(alloc(count));
+        if (count) HIP_TRY(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+        return ASTRO_OK;
+    }
+    int upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+};
+
+static inline int grid_for(int64_t n, int block, int cap = 65536) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+// ---------------------------------------------------------------------------
+// Factor + sweeps of one row-skyline profile.
+struct DeviceProfile {
+    int n = 0;
+    int64_t size = 0;
+    std::vector<int> hS;  // first column per row
+    DBuf<double> lt;
+    DBuf<int64_t> rowoff;
+    DBuf<int> sfirst;
+    // factor
+    int nBlocks = 0, panelsPerBlock = 0, splitK = 1, scratchRows = 64, maxTiles = 1;
+    DBuf<PanelInfo> panels;
+    DBuf<double> scratch;
+    DBuf<int> zeroPivot;
+    // sweeps
+    int numTiles = 0;
+    DBuf<SweepTile> fwdTiles, bwdTiles;
+    DBuf<int> fwdFlags, bwdFlags, ticket, timeout;
+    int fwdEpoch = 0, bwdEpoch = 0;
+    DBuf<double> z;
+    long spinLimit = 4000000;
+
+    int setup(const int* LRange, int dim, hipStream_t s) {
+        n = dim;
+        size = LRange[n];
+        std::vector<int64_t> off(n);
+        hS.resize(n);
+        for (int r = 0; r < n; ++r) {
+            int len = LRange[r + 1] - LRange[r];
+            if (len < 1 || len > r + 1) return fail(ASTRO_ERR_ARG, "profile row " + std::to_string(r + 1) + " has invalid length");
+            hS[r] = r - len + 1;
+            off[r] = (int64_t)LRange[r + 1] - 1 - r;
+        }
+        ASTRO_TRY(rowoff.upload(off, s));
+        ASTRO_TRY(sfirst.upload(hS, s));
+        // independent diagonal blocks: a boundary b has no row >= b reaching
+        // below b; small blocks are merged so launches stay few
+        std::vector<int> sufMin(n + 1, INT32_MAX);
+        for (int r = n - 1; r >= 0; --r) sufMin[r] = std::min(sufMin[r + 1], hS[r]);
+        std::vector<int> bounds = {0};
+        const int minBlock = 16384;
+        for (int b = 1; b < n; ++b)
+            if (sufMin[b] >= b && b - bounds.back() >= minBlock && n - b >= minBlock && bounds.size() < 64)
+                bounds.push_back(b);
+        bounds.push_back(n);
+        nBlocks = (int)bounds.size() - 1;
+        // panels
+        panelsPerBlock = 0;
+        for (int k = 0; k < nBlocks; ++k)
+            panelsPerBlock = std::max(panelsPerBlock, (bounds[k + 1] - bounds[k] + 63) / 64);
+        std::vector<PanelInfo> hp((size_t)nBlocks * panelsPerBlock, PanelInfo{-1, -1, -1, 0});
+        scratchRows = 64;
+        for (int k = 0; k < nBlocks; ++k) {
+            const int b0 = bounds[k], b1 = bounds[k + 1];
+            const int np = (b1 - b0 + 63) / 64;
+            std::vector<int> maxRow(np, -1);
+            for (int r = b0; r < b1; ++r) {
+                int pk = (hS[r] - b0) / 64;
+                maxRow[pk] = std::max(maxRow[pk], r);
+            }
+            int run = -1;
+            for (int q = 0; q < np; ++q) {
+                run = std::max(run, maxRow[q]);
+                PanelInfo P;
+                P.p = b0 + 64 * q;
+                P.pe = std::min(P.p + 64, b1);
+                P.iend = std::max(run, P.pe - 1) + 1;
+                int sm = INT32_MAX;
+                for (int j = P.p; j < P.pe; ++j) sm = std::min(sm, hS[j]);
+                P.smin = sm;
+                hp[(size_t)k * panelsPerBlock + q] = P;
+                scratchRows = std::max(scratchRows, ((P.iend - P.p) + 63) / 64 * 64);
+            }
+        }
+        maxTiles = scratchRows / 64;
+        splitK = std::max(1, std::min(16, 1024 / std::max(1, nBlocks * maxTiles)));
+        ASTRO_TRY(panels.upload(hp, s));
+        ASTRO_TRY(scratch.alloc((size_t)splitK * nBlocks * scratchRows * 64));
+        ASTRO_TRY(zeroPivot.alloc(1));
+        // sweep tiles, chains interleaved in ticket order
+        std::vector<std::vector<int>> chainTiles(nBlocks);  // tile start rows per chain
+        int maxCount = 0;
+        for (int k = 0; k < nBlocks; ++k) {
+            for (int r = bounds[k]; r < bounds[k + 1]; r += 64) chainTiles[k].push_back(r);
+            maxCount = std::max(maxCount, (int)chainTiles[k].size());
+        }
+        numTiles = 0;
+        for (auto& c : chainTiles) numTiles += (int)c.size();
+        std::vector<SweepTile> f, bw;
+        std::vector<std::vector<int>> fIdx(nBlocks), bIdx(nBlocks);
+        for (int k = 0; k < nBlocks; ++k) {
+            fIdx[k].assign(chainTiles[k].size(), -1);
+            bIdx[k].assign(chainTiles[k].size(), -1);
+        }
+        for (int q = 0; q < maxCount; ++q)
+            for (int k = 0; k < nBlocks; ++k)
+                if (q < (int)chainTiles[k].size()) {
+                    fIdx[k][q] = (int)f.size();
+                    int r0 = chainTiles[k][q], r1 = std::min(r0 + 64, bounds[k + 1]);
+                    f.push_back(SweepTile{r0, r1, k, 0});
+                }
+        for (int q = 0; q < maxCount; ++q)
+            for (int k = 0; k < nBlocks; ++k) {
+                int cnt = (int)chainTiles[k].size();
+                int t = cnt - 1 - q;
+                if (t >= 0) {
+                    bIdx[k][t] = (int)bw.size();
+                    int r0 = chainTiles[k][t], r1 = std::min(r0 + 64, bounds[k + 1]);
+                    bw.push_back(SweepTile{r0, r1, k, 0});
+                }
+            }
+        for (int k = 0; k < nBlocks; ++k) {
+            const int b0 = bounds[k], b1 = bounds[k + 1];
+            const int cnt = (int)chainTiles[k].size();
+            // forward: first tile holding the smallest first column of the tile's rows
+            for (int t = 0; t < cnt; ++t) {
+                int r0 = b0 + 64 * t, r1 = std::min(r0 + 64, b1), sm = r0;
+                for (int r = r0; r < r1; ++r) sm = std::min(sm, hS[r]);
+                int td = (sm - b0) / 64;
+                f[fIdx[k][t]].depFirst = (td < t) ? fIdx[k][td] : fIdx[k][t];
+            }
+            // backward: last tile holding a row whose profile reaches the tile
+            std::vector<int> maxRowByTile(cnt, -1);
+            for (int r = b0; r < b1; ++r) {
+                int tc = (hS[r] - b0) / 64;
+                maxRowByTile[tc] = std::max(maxRowByTile[tc], r);
+            }
+            int run = -1;
+            for (int t = 0; t < cnt; ++t) {
+                run = std::max(run, maxRowByTile[t]);
+                int td = (run - b0) / 64;
+                bw[bIdx[k][t]].depFirst = (td > t) ? bIdx[k][td] : bIdx[k][t];
+            }
+        }
+        ASTRO_TRY(fwdTiles.upload(f, s));
+        ASTRO_TRY(bwdTiles.upload(bw, s));
+        ASTRO_TRY(fwdFlags.alloc(numTiles));
+        ASTRO_TRY(bwdFlags.alloc(numTiles));
+        HIP_TRY(hipMemsetAsync(fwdFlags.p, 0, sizeof(int) * numTiles, s));
+        HIP_TRY(hipMemsetAsync(bwdFlags.p, 0, sizeof(int) * numTiles, s));
+        fwdEpoch = bwdEpoch = 0;
+        ASTRO_TRY(ticket.alloc(1));
+        ASTRO_TRY(timeout.alloc(1));
+        HIP_TRY(hipMemsetAsync(timeout.p, 0, sizeof(int), s));
+        ASTRO_TRY(z.alloc(n));
+        ASTRO_TRY(lt.alloc(size));
+        return ASTRO_OK;
+    }
+
+    // Left-looking blocked factor, two launches per panel.  Synchronous.
+    int factor(hipStream_t s) {
+        HIP_TRY(hipMemsetAsync(zeroPivot.p, 0x7f, sizeof(int), s));
+        dim3 gu(maxTiles * splitK, nBlocks), gf(maxTiles, nBlocks);
+        for (int q = 0; q < panelsPerBlock; ++q) {
+            hipLaunchKernelGGL(k_ldl_update, gu, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK, lt.p,
+                               rowoff.p, sfirst.p, scratch.p, scratchRows);
+            hipLaunchKernelGGL(k_ldl_factor_panel, gf, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK,
+                               lt.p, rowoff.p, sfirst.p, scratch.p, scratchRows, zeroPivot.p);
+        }
+        HIP_TRY(hipGetLastError());
+        int zp = 0;
+        HIP_TRY(hipMemcpyAsync(&zp, zeroPivot.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (zp != 0x7f7f7f7f) return fail(ASTRO_ERR_ZERO_PIVOT, "zero pivot at profile row " + std::to_string(zp));
+        return ASTRO_OK;
+    }
+
+    // x = P^-1 b (forward then backward sweep); z keeps L^-1 b.  Async.
+    int apply(const double* b, double* x, hipStream_t s) {
+        ++fwdEpoch;
+        hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
+        hipLaunchKernelGGL(k_sweep_forward, dim3(numTiles), dim3(256), 0, s, fwdTiles.p, numTiles, lt.p, rowoff.p,
+                           sfirst.p, b, z.p, fwdFlags.p, ticket.p, fwdEpoch, spinLimit, timeout.p);
+        ++bwdEpoch;
+        hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
+        hipLaunchKernelGGL(k_sweep_backward, dim3(numTiles), dim3(256), 0, s, bwdTiles.p, numTiles, lt.p,
+                           rowoff.p, sfirst.p, z.p, x, bwdFlags.p, ticket.p, bwdEpoch, spinLimit, timeout.p);
+        HIP_TRY(hipGetLastError());
+        return ASTRO_OK;
+    }
+
+    int check_timeout(hipStream_t s) {
+        int t = 0;
+        HIP_TRY(hipMemcpyAsync(&t, timeout.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (t) return fail(ASTRO_ERR_TIMEOUT, "triangular sweep wait exceeded its spin limit");
+        return ASTRO_OK;
+    }
+};
+
+// CSR operator with optional penalty rows (device arrays, 1-based columns).
+struct DeviceOperator {
+    int n = 0, cdim = 0;
+    const double* val = nullptr;
+    const int* cols = nullptr;
+    const int* range = nullptr;
+    const double* cmat = nullptr;
+    const int* ccols = nullptr;
+    const int* crange = nullptr;
+    int apply(const double* x, double* y, hipStream_t s) const {
+        hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val, cols,
+                           range, n, x, y);
+        if (cdim > 0)
+            hipLaunchKernelGGL(k_mpc_apply, dim3(grid_for(cdim, 256)), dim3(256), 0, s, cmat, ccols, crange, cdim, x,
+                               y);
+        HIP_TRY(hipGetLastError());
+        return ASTRO_OK;
+    }
+};
+
+// Restarted, left-preconditioned GMRES with classical Gram-Schmidt and
+// re-orthogonalisation (CGS2) in place of the reference's modified
+// Gram-Schmidt: the same Arnoldi relation, two multi-vector passes per step
+// instead of j dependent dot/axpy pairs.  Standard Givens rotations and a
+// beta*e1 right-hand side; a cycle ends when the rotated residual estimate
+// meets the target.  Working/AStrO_solvers.f90:128-297 is the reference form.
+struct Gmres {
+    int n = 0, m = 0;
+    DBuf<double> V, t, t2, r, partial, hdev;
+    double* hHost = nullptr;  // pinned
+    int nblk = 0;
+    int steps = 0, cycles = 0;
+    double res0 = 0, res = 0;
+    ~Gmres() {
+        if (hHost) (void)hipHostFree(hHost);
+    }
+    int setup(int dim, int numVecs) {
+        n = dim;
+        m = numVecs;
+        ASTRO_TRY(V.alloc((size_t)(m + 1) * n));
+        ASTRO_TRY(t.alloc(n));
+        ASTRO_TRY(t2.alloc(n));
+        ASTRO_TRY(r.alloc(n));
+        nblk = std::min(1024, std::max(1, n / 2048));
+        ASTRO_TRY(partial.alloc((size_t)nblk * (m + 8)));
+        ASTRO_TRY(hdev.alloc(3 * (m + 8)));
+        if (!hHost) HIP_TRY(hipHostMalloc(&hHost, sizeof(double) * 3 * (m + 8)));
+        return ASTRO_OK;
+    }
+    // out[0..nq) = V[0..nq)^T w  (device)
+    void multidot(const double* Vb, int nq, const double* w, double* out, hipStream_t s) {
+        hipLaunchKernelGGL(k_multidot, dim3(nblk, (nq + 7) / 8), dim3(256), 0, s, Vb, (int64_t)n, nq, w, n,
+                           partial.p, m + 8);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial.p, nblk, m + 8, nq, out);
+    }
+    double* Vcol(int q) { return V.p + (size_t)q * n; }
+
+    int solve(const DeviceOperator& A, DeviceProfile& P, const double* b, double* x, double rtol, double atol,
+              int maxCycles, hipStream_t s) {
+        steps = cycles = 0;
+        auto residual = [&](double& out) -> int {
+            ASTRO_TRY(A.apply(x, t.p, s));
+            hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b, t.p, t2.p, n, 1.0, -1.0);
+            ASTRO_TRY(P.apply(t2.p, r.p, s));
+            multidot(r.p, 1, r.p, hdev.p, s);
+            HIP_TRY(hipMemcpyAsync(hHost, hdev.p, sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            ASTRO_TRY(P.check_timeout(s));
+            out = std::sqrt(hHost[0]);
+            return ASTRO_OK;
+        };
+        ASTRO_TRY(residual(res));
+        res0 = res;
+        const double target = std::max(rtol * res0, atol);
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+        auto h = [&](int i, int j) -> double& { return H[(size_t)i + (size_t)(m + 1) * j]; };
+        while (res > target && cycles < maxCycles) {
+            hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, r.p, r.p, Vcol(0), n,
+                               1.0 / res, 0.0);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = res;
+            int k = 0;
+            for (int j = 0; j < m; ++j) {
+                double* w = Vcol(j + 1);
+                ASTRO_TRY(A.apply(Vcol(j), t.p, s));
+                ASTRO_TRY(P.apply(t.p, w, s));
+                // CGS2: two projections against V[0..j]
+                double* h1 = hdev.p;
+                double* h2 = hdev.p + (m + 8);
+                double* hn = hdev.p + 2 * (m + 8);
+                multidot(V.p, j + 1, w, h1, s);
+                hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, V.p, (int64_t)n,
+                                   j + 1, h1, w, n, -1.0);
+                multidot(V.p, j + 1, w, h2, s);
+                hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, V.p, (int64_t)n,
+                                   j + 1, h2, w, n, -1.0);
+                multidot(w, 1, w, hn, s);
+                HIP_TRY(hipMemcpyAsync(hHost, hdev.p, sizeof(double) * 3 * (m + 8), hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                ASTRO_TRY(P.check_timeout(s));
+                const double hnorm = std::sqrt(hHost[2 * (m + 8)]);
+                for (int i = 0; i <= j; ++i) h(i, j) = hHost[i] + hHost[(m + 8) + i];
+                h(j + 1, j) = hnorm;
+                ++steps;
+                k = j + 1;
+                // Givens
+                for (int i = 0; i < j; ++i) {
+                    double a = cs[i] * h(i, j) + sn[i] * h(i + 1, j);
+                    double bb = -sn[i] * h(i, j) + cs[i] * h(i + 1, j);
+                    h(i, j) = a;
+                    h(i + 1, j) = bb;
+                }
+                const double rr = std::hypot(h(j, j), h(j + 1, j));
+                cs[j] = rr == 0.0 ? 1.0 : h(j, j) / rr;
+                sn[j] = rr == 0.0 ? 0.0 : h(j + 1, j) / rr;
+                h(j, j) = rr;
+                h(j + 1, j) = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                if (hnorm == 0.0 || std::fabs(g[j + 1]) <= target) break;
+                hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, w, w, w, n, 1.0 / hnorm,
+                                   0.0);
+            }
+            for (int i = k - 1; i >= 0; --i) {
+                double v = g[i];
+                for (int q = i + 1; q < k; ++q) v -= h(i, q) * y[q];
+                y[i] = v / h(i, i);
+            }
+            HIP_TRY(hipMemcpyAsync(hdev.p, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, V.p, (int64_t)n, k,
+                               hdev.p, x, n, 1.0);
+            ++cycles;
+            ASTRO_TRY(residual(res));
+        }
+        if (res > target) return fail(ASTRO_ERR_NO_CONVERGE, "GMRES did not reach the tolerance in " +
+                                                                 std::to_string(cycles) + " cycles");
+        return ASTRO_OK;
+    }
+};
+
+// Host LU inverse without pivoting of a 6x6 compliance matrix (column-major),
+// the procedure of r_GetInvLU (Working/AStrO_r_designPropertyFunctions.f90:996-1073).
+static void compliance_inverse(const double* ep, double* C) {
+    double S[36] = {0};
+    auto s = [&](int r, int c) -> double& { return S[r + 6 * c]; };
+    s(0, 0) = 1.0 / ep[0];
+    s(0, 1) = -ep[3] / ep[0];
+    s(0, 2) = -ep[4] / ep[0];
+    s(1, 1) = 1.0 / ep[1];
+    s(1, 2) = -ep[5] / ep[1];
+    s(2, 2) = 1.0 / ep[2];
+    s(3, 3) = 1.0 / ep[6];
+    s(4, 4) = 1.0 / ep[7];
+    s(5, 5) = 1.0 / ep[8];
+    for (int i = 1; i < 6; ++i)
+        for (int j = 0; j < i; ++j) s(i, j) = s(j, i);
+    for (int i = 0; i < 6; ++i) {
+        for (int j = i; j < 6; ++j)
+            for (int k = 0; k < i; ++k) s(i, j) -= s(i, k) * s(k, j);
+        double f = 1.0 / s(i, i);
+        for (int j = i + 1; j < 6; ++j) {
+            for (int k = 0; k < i; ++k) s(j, i) -= s(j, k) * s(k, i);
+            s(j, i) *= f;
+        }
+    }
+    for (int c = 0; c < 6; ++c) {
+        double z[6], x[6];
+        for (int i = 0; i < 6; ++i) {
+            z[i] = (i == c) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) z[i] -= s(i, k) * z[k];
+        }
+        for (int i = 5; i >= 0; --i) {
+            double dp = 0;
+            for (int k = i + 1; k < 6; ++k) dp += s(i, k) * x[k];
+            x[i] = (z[i] - dp) / s(i, i);
+        }
+        for (int i = 0; i < 6; ++i) C[i + 6 * c] = x[i];
+    }
+}
+
+}  // namespace astro
+
+using namespace astro;
+
+// ===========================================================================
+struct astro_prep {
+    Prep p;
+};
+
+// Context
+struct astro_ctx {
+    hipStream_t s = nullptr;
+    Prep prep;
+    astro_options opt{};
+    int numVecs = 100;
+    // model on device
+    DBuf<double> X, matC, elScale, elRho, elCond;
+    DBuf<int> conn, elMat, rank, perm;
+    DBuf<uint8_t> elPos, thPos;
+    std::vector<int> colorStart;
+    // elastic system
+    DBuf<int> csrRange, csrCols, mpcRange, mpcCols, loadDofs;
+    DBuf<double> csrVal, mpcMat, mpcRhs, loadVals, load, delDisp, nodeDisp;
+    DBuf<unsigned long long> maxBits;
+    DeviceProfile prof;
+    DeviceOperator op;
+    Gmres gm;
+    // thermal system
+    DBuf<int> tRange, tCols, tMpcRange, tMpcCols;
+    DBuf<double> tVal, tMpcMat, tMpcRhs, tLoad, tDel, nodeTemp;
+    DeviceProfile tprof;
+    DeviceOperator top;
+    Gmres tgm;
+    bool thermalReady = false;
+    // timing
+    hipEvent_t ev[8] = {};
+    double phaseMs[6] = {0, 0, 0, 0, 0, 0};
+    ~astro_ctx() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+extern "C" {
+
+const char* astro_last_error(void) { return g_lastError.c_str(); }
+const char* astro_version(void) { return "astro-mi355x 0.1 (gfx950, FP64)"; }
+
+static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
+    hipStream_t s = c->s;
+    const Prep& p = c->prep;
+    const int E = m->numEls, N = m->numNodes;
+    ASTRO_TRY(c->X.upload(m->nodeList, 3 * (size_t)N, s));
+    ASTRO_TRY(c->conn.upload(m->elementList, 8 * (size_t)E, s));
+    // materials: rows of constants normalised by E1 are deduplicated; the
+    // element keeps E1 as a scale (C is linear in the moduli at fixed ratios)
+    std::vector<double> mats;
+    std::vector<int> elMat(E);
+    std::vector<double> scale(E), rho(E, 0.0), cond(6 * (size_t)E, 0.0);
+    std::vector<std::vector<double>> keys;
+    for (int e = 0; e < E; ++e) {
+        const double* ep = m->eprops ? m->eprops + 9 * (size_t)e : nullptr;
+        std::vector<double> k(9, 0.0);
+        double E1 = 1.0;
+        if (ep) {
+            E1 = ep[0];
+            for (int i = 0; i < 9; ++i) k[i] = ep[i];
+            for (int i : {0, 1, 2, 6, 7, 8}) k[i] /= E1;
+        }
+        int id = -1;
+        for (size_t q = keys.size(); q-- > 0;)
+            if (keys[q] == k) {
+                id = (int)q;
+                break;
+            }
+        if (id < 0) {
+            if (keys.size() > 4096) return fail(ASTRO_ERR_UNSUPPORTED, "more than 4096 distinct material ratios");
+            id = (int)keys.size();
+            keys.push_back(k);
+            double C[36];
+            if (ep) {
+                compliance_inverse(k.data(), C);
+            } else {
+                std::fill(C, C + 36, 0.0);
+            }
+            mats.insert(mats.end(), C, C + 36);
+        }
+        elMat[e] = id;
+        scale[e] = E1;
+        if (m->density) rho[e] = m->density[e];
+        if (m->cond)
+            for (int i = 0; i < 6; ++i) cond[6 * (size_t)e + i] = m->cond[6 * (size_t)e + i];
+    }
+    ASTRO_TRY(c->matC.upload(mats, s));
+    ASTRO_TRY(c->elMat.upload(elMat, s));
+    ASTRO_TRY(c->elScale.upload(scale, s));
+    ASTRO_TRY(c->elRho.upload(rho, s));
+    ASTRO_TRY(c->elCond.upload(cond, s));
+    ASTRO_TRY(c->rank.upload(p.currentRank, s));
+    ASTRO_TRY(c->elPos.upload(p.elPos, s));
+    ASTRO_TRY(c->thPos.upload(p.thPos, s));
+    ASTRO_TRY(c->perm.upload(p.colorPerm, s));
+    c->colorStart = p.colorStart;
+    const int n = p.elMatDim;
+    ASTRO_TRY(c->csrRange.upload(p.elMatRange, s));
+    ASTRO_TRY(c->csrCols.upload(p.elMatCols, s));
+    ASTRO_TRY(c->csrVal.alloc(p.elMatCols.size()));
+    ASTRO_TRY(c->mpcRange.upload(p.elMPCMatRange, s));
+    ASTRO_TRY(c->mpcCols.upload(p.elMPCMatCols, s));
+    ASTRO_TRY(c->mpcMat.upload(p.elMPCMat, s));
+    ASTRO_TRY(c->mpcRhs.upload(p.elMPCRHS, s));
+    // nodal loads expanded to (dof, value) pairs (node-set loads arrive expanded)
+    std::vector<int> ld;
+    std::vector<double> lv;
+    for (int l = 0; l < m->numLoads; ++l) {
+        int nd = m->loadNode[l];
+        if (nd < 1 || nd > N) return fail(ASTRO_ERR_ARG, "load node out of range");
+        for (int q = 0; q < 6; ++q) {
+            int d = p.nDofIndex[6 * (size_t)(nd - 1) + q];
+            double v = m->loadVal[7 * (size_t)l + q];
+            if (d != 0 && v != 0.0) {
+                ld.push_back(d);
+                lv.push_back(v);
+            }
+        }
+    }
+    ASTRO_TRY(c->loadDofs.upload(ld, s));
+    ASTRO_TRY(c->loadVals.upload(lv, s));
+    ASTRO_TRY(c->load.alloc(n));
+    ASTRO_TRY(c->delDisp.alloc(n));
+    ASTRO_TRY(c->nodeDisp.alloc(n));
+    HIP_TRY(hipMemsetAsync(c->delDisp.p, 0, sizeof(double) * n, s));
+    HIP_TRY(hipMemsetAsync(c->nodeDisp.p, 0, sizeof(double) * n, s));
+    ASTRO_TRY(c->maxBits.alloc(1));
+    ASTRO_TRY(c->prof.setup(p.elMatLTRange.data(), n, s));
+    c->op.n = n;
+    c->op.cdim = p.elMPCDim;
+    c->op.val = c->csrVal.p;
+    c->op.cols = c->csrCols.p;
+    c->op.range = c->csrRange.p;
+    c->op.cmat = c->mpcMat.p;
+    c->op.ccols = c->mpcCols.p;
+    c->op.crange = c->mpcRange.p;
+    ASTRO_TRY(c->gm.setup(n, c->numVecs));
+    return ASTRO_OK;
+}
+
+static ModelDesc to_desc(const astro_model* m) {
+    ModelDesc d;
+    d.numNodes = m->numNodes;
+    d.numEls = m->numEls;
+    d.nodeList = m->nodeList;
+    d.elementList = m->elementList;
+    d.elementType = m->elementType;
+    d.numMpcTerms = m->numMpcTerms;
+    d.mpcEqn = m->mpcEqn;
+    d.mpcDof = m->mpcDof;
+    d.mpcCoef = m->mpcCoef;
+    d.mpcIsSet = m->mpcIsSet;
+    d.mpcNodeStart = m->mpcNodeStart;
+    d.mpcNodes = m->mpcNodes;
+    d.numMPC = m->numMPC;
+    d.mpcRHS = m->mpcRHS;
+    return d;
+}
+
+int astro_ctx_create(const astro_model* m, const astro_options* o, astro_ctx** out) {
+    if (!m || !out) return fail(ASTRO_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<astro_ctx> c(new astro_ctx);
+    if (o) c->opt = *o;
+    if (c->opt.rtol <= 0) c->opt.rtol = 1e-12;
+    if (c->opt.maxCycles <= 0) c->opt.maxCycles = 1000;
+    HIP_TRY(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+    ModelDesc d = to_desc(m);
+    std::string err = analysis_prep(d, c->opt.maxBW, c->opt.literalMaxBW != 0, c->opt.thermLiteralD8 != 0, c->prep);
+    if (!err.empty()) return fail(ASTRO_ERR_ARG, err);
+    for (int e = 0; e < m->numEls; ++e)
+        if (m->elementType[e] != 8) return fail(ASTRO_ERR_UNSUPPORTED, "device path supports brick8 elements only");
+    const int n = c->prep.elMatDim;
+    c->numVecs = c->opt.numVecs > 0 ? c->opt.numVecs : (n < 200 ? n / 2 : 100);
+    if (c->numVecs < 1 || c->numVecs > 255) return fail(ASTRO_ERR_ARG, "numVecs must be in [1, 255]");
+    ASTRO_TRY(ctx_upload_model(c.get(), m));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    *out = c.release();
+    return ASTRO_OK;
+}
+
+int astro_prep_create(const astro_model* m, const astro_options* o, astro_prep** out) {
+    if (!m || !out) return fail(ASTRO_ERR_ARG, "null argument");
+    *out = nullptr;
+    astro_options opt{};
+    if (o) opt = *o;
+    std::unique_ptr<astro_prep> p(new astro_prep);
+    std::string err = analysis_prep(to_desc(m), opt.maxBW, opt.literalMaxBW != 0, opt.thermLiteralD8 != 0, p->p);
+    if (!err.empty()) return fail(ASTRO_ERR_ARG, err);
+    *out = p.release();
+    return ASTRO_OK;
+}
+
+int astro_prep_destroy(astro_prep* p) {
+    delete p;
+    return ASTRO_OK;
+}
+
+int64_t astro_prep_scalar(const astro_prep* pp, const char* name) {
+    if (!pp || !name) return -1;
+    const Prep& p = pp->p;
+    std::string s(name);
+    if (s == "elMatDim") return p.elMatDim;
+    if (s == "elMatSize") return (int64_t)p.elMatCols.size();
+    if (s == "elMatLTSize") return p.elMatLTRange.empty() ? 0 : p.elMatLTRange.back();
+    if (s == "numSolidNds") return p.numSolidNds;
+    if (s == "solverMaxBW") return p.solverMaxBW;
+    if (s == "elMPCDim") return p.elMPCDim;
+    if (s == "elMPCSize") return (int64_t)p.elMPCMat.size();
+    if (s == "thermMPCDim") return p.thermMPCDim;
+    if (s == "thermMatSize") return (int64_t)p.thermMatCols.size();
+    if (s == "thermMatLTSize") return p.thermMatLTRange.empty() ? 0 : p.thermMatLTRange.back();
+    if (s == "numColors") return (int64_t)p.colorStart.size() - 1;
+    return -1;
+}
+
+static const std::vector<int>* prep_int_array(const Prep& p, const std::string& s) {
+    if (s == "currentRank") return &p.currentRank;
+    if (s == "nDofIndex") return &p.nDofIndex;
+    if (s == "elMatRange") return &p.elMatRange;
+    if (s == "elMatCols") return &p.elMatCols;
+    if (s == "elMatLTRange") return &p.elMatLTRange;
+    if (s == "elMPCMatRange") return &p.elMPCMatRange;
+    if (s == "elMPCMatCols") return &p.elMPCMatCols;
+    if (s == "thermMatRange") return &p.thermMatRange;
+    if (s == "thermMatCols") return &p.thermMatCols;
+    if (s == "thermMatLTRange") return &p.thermMatLTRange;
+    if (s == "thermMPCMatRange") return &p.thermMPCMatRange;
+    if (s == "thermMPCMatCols") return &p.thermMPCMatCols;
+    return nullptr;
+}
+
+int astro_prep_get_i32(const astro_prep* pp, const char* name, int32_t* host, int64_t count) {
+    if (!pp || !name || !host) return fail(ASTRO_ERR_ARG, "null argument");
+    const std::vector<int>* v = prep_int_array(pp->p, name);
+    if (!v) return fail(ASTRO_ERR_ARG, std::string("unknown int array ") + name);
+    if ((int64_t)v->size() != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
+    if (count) std::memcpy(host, v->data(), sizeof(int) * count);
+    return ASTRO_OK;
+}
+
+int astro_prep_get_f64(const astro_prep* pp, const char* name, double* host, int64_t count) {
+    if (!pp || !name || !host) return fail(ASTRO_ERR_ARG, "null argument");
+    std::string s(name);
+    const std::vector<double>* v = nullptr;
+    if (s == "elMPCMat") v = &pp->p.elMPCMat;
+    if (s == "elMPCRHS") v = &pp->p.elMPCRHS;
+    if (s == "thermMPCMat") v = &pp->p.thermMPCMat;
+    if (s == "thermMPCRHS") v = &pp->p.thermMPCRHS;
+    if (!v) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
+    if ((int64_t)v->size() != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
+    if (count) std::memcpy(host, v->data(), sizeof(double) * count);
+    return ASTRO_OK;
+}
+
+int astro_ctx_destroy(astro_ctx* c) {
+    if (c) {
+        (void)hipStreamSynchronize(c->s);
+        delete c;
+    }
+    return ASTRO_OK;
+}
+
+int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
+    if (!c || !name) return -1;
+    std::string s(name);
+    const Prep& p = c->prep;
+    if (s == "elMatDim") return p.elMatDim;
+    if (s == "elMatSize") return (int64_t)p.elMatCols.size();
+    if (s == "elMatLTSize") return p.elMatLTRange.empty() ? 0 : p.elMatLTRange.back();
+    if (s == "numSolidNds") return p.numSolidNds;
+    if (s == "solverMaxBW") return p.solverMaxBW;
+    if (s == "elMPCDim") return p.elMPCDim;
+    if (s == "thermMatSize") return (int64_t)p.thermMatCols.size();
+    if (s == "thermMatLTSize") return p.thermMatLTRange.empty() ? 0 : p.thermMatLTRange.back();
+    if (s == "gmresSteps") return c->gm.steps;
+    if (s == "gmresCycles") return c->gm.cycles;
+    if (s == "thermGmresSteps") return c->tgm.steps;
+    if (s == "numColors") return (int64_t)c->colorStart.size() - 1;
+    if (s == "profileBlocks") return c->prof.nBlocks;
+    if (s == "numVecs") return c->numVecs;
+    return -1;
+}
+
+double astro_ctx_scalar_f(const astro_ctx* c, const char* name) {
+    if (!c || !name) return NAN;
+    std::string s(name);
+    if (s == "gmresRes") return c->gm.res;
+    if (s == "gmresRes0") return c->gm.res0;
+    if (s == "thermGmresRes") return c->tgm.res;
+    return NAN;
+}
+
+static const std::vector<int>* host_int_array(const astro_ctx* c, const std::string& s) {
+    const Prep& p = c->prep;
+    if (s == "currentRank") return &p.currentRank;
+    if (s == "nDofIndex") return &p.nDofIndex;
+    if (s == "elMatRange") return &p.elMatRange;
+    if (s == "elMatCols") return &p.elMatCols;
+    if (s == "elMatLTRange") return &p.elMatLTRange;
+    if (s == "elMPCMatRange") return &p.elMPCMatRange;
+    if (s == "elMPCMatCols") return &p.elMPCMatCols;
+    if (s == "thermMatRange") return &p.thermMatRange;
+    if (s == "thermMatCols") return &p.thermMatCols;
+    if (s == "thermMatLTRange") return &p.thermMatLTRange;
+    return nullptr;
+}
+
+int astro_ctx_get_i32(astro_ctx* c, const char* name, int32_t* host, int64_t count) {
+    if (!c || !name || !host) return fail(ASTRO_ERR_ARG, "null argument");
+    const std::vector<int>* v = host_int_array(c, name);
+    if (!v) return fail(ASTRO_ERR_ARG, std::string("unknown int array ") + name);
+    if ((int64_t)v->size() != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
+    std::memcpy(host, v->data(), sizeof(int) * count);
+    return ASTRO_OK;
+}
+
+static double* dev_double_array(astro_ctx* c, const std::string& s, int64_t& len) {
+    auto pick = [&](DBuf<double>& b, int64_t l) {
+        len = l;
+        return b.p;
+    };
+    const Prep& p = c->prep;
+    if (s == "elasticMat") return pick(c->csrVal, (int64_t)p.elMatCols.size());
+    if (s == "elMatLT") return pick(c->prof.lt, c->prof.size);
+    if (s == "elMPCMat") return pick(c->mpcMat, (int64_t)p.elMPCMat.size());
+    if (s == "elMPCRHS") return pick(c->mpcRhs, (int64_t)p.elMPCRHS.size());
+    if (s == "elasticLoad") return pick(c->load, p.elMatDim);
+    if (s == "nodeDisp") return pick(c->nodeDisp, p.elMatDim);
+    if (s == "delDisp") return pick(c->delDisp, p.elMatDim);
+    if (s == "thermMat") return pick(c->tVal, (int64_t)p.thermMatCols.size());
+    if (s == "thermMatLT") return pick(c->tprof.lt, c->tprof.size);
+    if (s == "thermalLoad") return pick(c->tLoad, p.numNodes);
+    if (s == "nodeTemp") return pick(c->nodeTemp, p.numNodes);
+    len = -1;
+    return nullptr;
+}
+
+int astro_ctx_get_f64(astro_ctx* c, const char* name, double* host, int64_t count) {
+    if (!c || !name || !host) return fail(ASTRO_ERR_ARG, "null argument");
+    int64_t len = 0;
+    double* d = dev_double_array(c, name, len);
+    if (len < 0) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
+    if (len != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
+    if (!d) return fail(ASTRO_ERR_ARG, std::string("array not built yet: ") + name);
+    HIP_TRY(hipMemcpyAsync(host, d, sizeof(double) * count, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+int astro_ctx_set_f64(astro_ctx* c, const char* name, const double* host, int64_t count) {
+    if (!c || !name || !host) return fail(ASTRO_ERR_ARG, "null argument");
+    int64_t len = 0;
+    double* d = dev_double_array(c, name, len);
+    if (len < 0 || !d) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
+    if (len != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
+    HIP_TRY(hipMemcpyAsync(d, host, sizeof(double) * count, hipMemcpyHostToDevice, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+int astro_ctx_assemble_elastic(astro_ctx* c, double massFactor) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    const int64_t nnz = (int64_t)c->prep.elMatCols.size();
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, c->s, c->csrVal.p, nnz, 0.0);
+    const int nS = c->prep.numSolidNds;
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_brick8_tangent, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
+                           c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, massFactor, c->rank.p,
+                           nS, c->csrRange.p, c->elPos.p, c->csrVal.p);
+    }
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+int astro_ctx_scale_mpc(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    const int64_t nnz = (int64_t)c->prep.elMatCols.size();
+    HIP_TRY(hipMemsetAsync(c->maxBits.p, 0, sizeof(unsigned long long), c->s));
+    hipLaunchKernelGGL(k_max_abs, dim3(grid_for(nnz, 256, 2048)), dim3(256), 0, c->s, c->csrVal.p, nnz, c->maxBits.p);
+    const int cd = c->prep.elMPCDim;
+    if (cd > 0)
+        hipLaunchKernelGGL(k_scale_mpc, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcRhs.p,
+                           c->mpcRange.p, cd, c->maxBits.p);
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+int astro_ctx_build_profile(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    DeviceProfile& P = c->prof;
+    const int n = c->prep.elMatDim;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, c->s, P.lt.p, P.size, 0.0);
+    hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, c->s,
+                       c->csrVal.p, c->csrCols.p, c->csrRange.p, n, P.sfirst.p, P.rowoff.p, P.lt.p);
+    const int cd = c->prep.elMPCDim;
+    if (cd > 0)
+        hipLaunchKernelGGL(k_penalty_to_profile, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p,
+                           c->mpcCols.p, c->mpcRange.p, cd, P.sfirst.p, P.rowoff.p, P.lt.p);
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+int astro_ctx_factor(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    return c->prof.factor(c->s);
+}
+
+static int ctx_rhs(astro_ctx* c) {
+    const int n = c->prep.elMatDim;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->load.p, (int64_t)n, 0.0);
+    const int nl = (int)c->loadDofs.n;
+    if (nl > 0)
+        hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
+                           nl, c->load.p);
+    const int cd = c->prep.elMPCDim;
+    if (cd > 0)
+        hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
+                           c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p);
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+int astro_ctx_solve(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    ASTRO_TRY(ctx_rhs(c));
+    const int n = c->prep.elMatDim;
+    ASTRO_TRY(c->gm.solve(c->op, c->prof, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s));
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->nodeDisp.p, c->delDisp.p,
+                       c->nodeDisp.p, n, 1.0, 1.0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+int astro_ctx_static_solve(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    HIP_TRY(hipEventRecord(c->ev[0], c->s));
+    ASTRO_TRY(astro_ctx_assemble_elastic(c, 0.0));
+    HIP_TRY(hipEventRecord(c->ev[1], c->s));
+    ASTRO_TRY(astro_ctx_scale_mpc(c));
+    HIP_TRY(hipEventRecord(c->ev[2], c->s));
+    ASTRO_TRY(astro_ctx_build_profile(c));
+    HIP_TRY(hipEventRecord(c->ev[3], c->s));
+    ASTRO_TRY(astro_ctx_factor(c));
+    HIP_TRY(hipEventRecord(c->ev[4], c->s));
+    ASTRO_TRY(astro_ctx_solve(c));
+    HIP_TRY(hipEventRecord(c->ev[5], c->s));
+    HIP_TRY(hipEventSynchronize(c->ev[5]));
+    for (int k = 0; k < 5; ++k) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+        c->phaseMs[k] = ms;
+    }
+    return ASTRO_OK;
+}
+
+double astro_ctx_phase_ms(const astro_ctx* c, const char* phase) {
+    if (!c || !phase) return NAN;
+    static const char* names[] = {"assemble", "scale", "profile", "factor", "solve"};
+    for (int k = 0; k < 5; ++k)
+        if (!std::strcmp(phase, names[k])) return c->phaseMs[k];
+    return NAN;
+}
+
+// ---- thermal ----------------------------------------------------------------
+static int ctx_thermal_setup(astro_ctx* c) {
+    if (c->thermalReady) return ASTRO_OK;
+    const Prep& p = c->prep;
+    hipStream_t s = c->s;
+    ASTRO_TRY(c->tRange.upload(p.thermMatRange, s));
+    ASTRO_TRY(c->tCols.upload(p.thermMatCols, s));
+    ASTRO_TRY(c->tVal.alloc(p.thermMatCols.size()));
+    ASTRO_TRY(c->tMpcRange.upload(p.thermMPCMatRange, s));
+    ASTRO_TRY(c->tMpcCols.upload(p.thermMPCMatCols, s));
+    ASTRO_TRY(c->tMpcMat.upload(p.thermMPCMat, s));
+    ASTRO_TRY(c->tMpcRhs.upload(p.thermMPCRHS, s));
+    const int N = p.numNodes;
+    ASTRO_TRY(c->tLoad.alloc(N));
+    ASTRO_TRY(c->tDel.alloc(N));
+    ASTRO_TRY(c->nodeTemp.alloc(N));
+    HIP_TRY(hipMemsetAsync(c->tDel.p, 0, sizeof(double) * N, s));
+    HIP_TRY(hipMemsetAsync(c->nodeTemp.p, 0, sizeof(double) * N, s));
+    ASTRO_TRY(c->tprof.setup(p.thermMatLTRange.data(), N, s));
+    c->top.n = N;
+    c->top.cdim = p.thermMPCDim;
+    c->top.val = c->tVal.p;
+    c->top.cols = c->tCols.p;
+    c->top.range = c->tRange.p;
+    c->top.cmat = c->tMpcMat.p;
+    c->top.ccols = c->tMpcCols.p;
+    c->top.crange = c->tMpcRange.p;
+    int nv = c->opt.numVecs > 0 ? c->opt.numVecs : (N < 200 ? N / 2 : 100);
+    ASTRO_TRY(c->tgm.setup(N, nv));
+    c->thermalReady = true;
+    return ASTRO_OK;
+}
+
+int astro_ctx_assemble_thermal(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    ASTRO_TRY(ctx_thermal_setup(c));
+    const int64_t nnz = (int64_t)c->prep.thermMatCols.size();
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, c->s, c->tVal.p, nnz, 0.0);
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_brick8_conduct, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
+                           c->conn.p, c->X.p, c->elCond.p, c->rank.p, c->tRange.p, c->thPos.p, c->tVal.p);
+    }
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+// Thermal solve: scaleThermalMPC, convertToLTri, factor, RHS (constraint load),
+// GMRES, nodeTemp += delta.  Working/AStrO_commandFunctions.f90:763-769, 689-703.
+int astro_ctx_thermal_solve(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    ASTRO_TRY(ctx_thermal_setup(c));
+    hipStream_t s = c->s;
+    const Prep& p = c->prep;
+    const int64_t nnz = (int64_t)p.thermMatCols.size();
+    const int N = p.numNodes, cd = p.thermMPCDim;
+    HIP_TRY(hipMemsetAsync(c->maxBits.p, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_max_abs, dim3(grid_for(nnz, 256, 2048)), dim3(256), 0, s, c->tVal.p, nnz, c->maxBits.p);
+    if (cd > 0)
+        hipLaunchKernelGGL(k_scale_mpc, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p, c->tMpcRhs.p,
+                           c->tMpcRange.p, cd, c->maxBits.p);
+    DeviceProfile& P = c->tprof;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, s, P.lt.p, P.size, 0.0);
+    hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)N * 32, 256, 1 << 30)), dim3(256), 0, s, c->tVal.p,
+                       c->tCols.p, c->tRange.p, N, P.sfirst.p, P.rowoff.p, P.lt.p);
+    if (cd > 0)
+        hipLaunchKernelGGL(k_penalty_to_profile, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p,
+                           c->tMpcCols.p, c->tMpcRange.p, cd, P.sfirst.p, P.rowoff.p, P.lt.p);
+    HIP_TRY(hipGetLastError());
+    ASTRO_TRY(P.factor(s));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->tLoad.p, (int64_t)N, 0.0);
+    if (cd > 0)
+        hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p, c->tMpcCols.p,
+                           c->tMpcRange.p, c->tMpcRhs.p, cd, c->nodeTemp.p, c->tLoad.p);
+    ASTRO_TRY(c->tgm.solve(c->top, P, c->tLoad.p, c->tDel.p, c->opt.rtol, 0.0, c->opt.maxCycles, s));
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->nodeTemp.p, c->tDel.p,
+                       c->nodeTemp.p, N, 1.0, 1.0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return ASTRO_OK;
+}
+
+int astro_ctx_precond_apply(astro_ctx* c, const double* d_b, double* d_x) {
+    if (!c || !d_b || !d_x) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(c->prof.apply(d_b, d_x, c->s));
+    return c->prof.check_timeout(c->s);
+}
+
+int astro_ctx_operator_apply(astro_ctx* c, const double* d_x, double* d_y) {
+    if (!c || !d_x || !d_y) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(c->op.apply(d_x, d_y, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+int astro_ctx_sync(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+// ===========================================================================
+// Layer 1: reference routine replacements on host arrays.
+
+static int check_range(const int32_t* R, int32_t n, int64_t size, const char* what) {
+    if (n < 0) return fail(ASTRO_ERR_ARG, std::string(what) + ": negative dimension");
+    if (R[0] != 0) return fail(ASTRO_ERR_ARG, std::string(what) + ": range(0) must be 0");
+    for (int i = 1; i <= n; ++i)
+        if (R[i] < R[i - 1]) return fail(ASTRO_ERR_ARG, std::string(what) + ": ranges not monotone");
+    if ((int64_t)R[n] > size) return fail(ASTRO_ERR_ARG, std::string(what) + ": range end exceeds array size");
+    return ASTRO_OK;
+}
+
+int astro_getSparseLDLFact(double* ALTri, int64_t ASize, const int32_t* ARange, int32_t n) {
+    if (!ALTri || !ARange) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(check_range(ARange, n, ASize, "getSparseLDLFact"));
+    if (n == 0) return ASTRO_OK;
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc;
+    {
+        DeviceProfile P;
+        rc = P.setup(ARange, n, s);
+        if (rc == ASTRO_OK && hipMemcpyAsync(P.lt.p, ALTri, sizeof(double) * P.size, hipMemcpyHostToDevice, s) != hipSuccess)
+            rc = fail(ASTRO_ERR_DEVICE, "upload of ALTri failed");
+        if (rc == ASTRO_OK) rc = P.factor(s);
+        if (rc == ASTRO_OK || rc == ASTRO_ERR_ZERO_PIVOT) {
+            if (hipMemcpyAsync(ALTri, P.lt.p, sizeof(double) * P.size, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                rc = fail(ASTRO_ERR_DEVICE, "download of the factor failed");
+        }
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int astro_solveSparseLDLFact(double* soln, const double* rhs, const double* Lmat, int64_t LSize,
+                             const int32_t* LRange, int32_t Ldim, double* zVec) {
+    if (!soln || !rhs || !Lmat || !LRange) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(check_range(LRange, Ldim, LSize, "solveSparseLDLFact"));
+    if (Ldim == 0) return ASTRO_OK;
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc;
+    {
+        DeviceProfile P;
+        DBuf<double> b, x;
+        rc = P.setup(LRange, Ldim, s);
+        if (rc == ASTRO_OK) rc = b.upload(rhs, Ldim, s);
+        if (rc == ASTRO_OK) rc = x.alloc(Ldim);
+        if (rc == ASTRO_OK && hipMemcpyAsync(P.lt.p, Lmat, sizeof(double) * P.size, hipMemcpyHostToDevice, s) != hipSuccess)
+            rc = fail(ASTRO_ERR_DEVICE, "upload failed");
+        if (rc == ASTRO_OK) rc = P.apply(b.p, x.p, s);
+        if (rc == ASTRO_OK) rc = P.check_timeout(s);
+        if (rc == ASTRO_OK) {
+            if (hipMemcpyAsync(soln, x.p, sizeof(double) * Ldim, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                (zVec && hipMemcpyAsync(zVec, P.z.p, sizeof(double) * Ldim, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+                hipStreamSynchronize(s) != hipSuccess)
+                rc = fail(ASTRO_ERR_DEVICE, "download failed");
+        }
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int astro_gMRes(double* xVec, const double* Amat, const int32_t* Acols, const int32_t* ARange, int64_t ASize,
+                int32_t ADim, const double* Cmat, const int32_t* Ccols, const int32_t* CRange, int64_t CSize,
+                int32_t CDim, const double* Pmat, const int32_t* PRange, int64_t PSize, const double* bVec,
+                int32_t numVecs, int32_t maxIt, double rtol, double atol) {
+    if (!xVec || !Amat || !Acols || !ARange || !Pmat || !PRange || !bVec) return fail(ASTRO_ERR_ARG, "null argument");
+    if (numVecs < 1 || numVecs > 255) return fail(ASTRO_ERR_ARG, "numVecs must be in [1, 255]");
+    ASTRO_TRY(check_range(ARange, ADim, ASize, "gMRes A"));
+    ASTRO_TRY(check_range(PRange, ADim, PSize, "gMRes P"));
+    if (CDim > 0) {
+        if (!Cmat || !Ccols || !CRange) return fail(ASTRO_ERR_ARG, "null constraint arrays");
+        ASTRO_TRY(check_range(CRange, CDim, CSize, "gMRes C"));
+    }
+    for (int64_t q = 0; q < ARange[ADim]; ++q)
+        if (Acols[q] < 0 || Acols[q] > ADim) return fail(ASTRO_ERR_ARG, "A column out of range");
+    for (int64_t q = 0; CDim > 0 && q < CRange[CDim]; ++q)
+        if (Ccols[q] < 0 || Ccols[q] > ADim) return fail(ASTRO_ERR_ARG, "C column out of range");
+    if (ADim == 0) return ASTRO_OK;
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc;
+    {
+        DeviceProfile P;
+        DBuf<double> av, cv, b, x;
+        DBuf<int> ac, ar, cc, cr;
+        Gmres gm;
+        DeviceOperator op;
+        rc = P.setup(PRange, ADim, s);
+        if (rc == ASTRO_OK && hipMemcpyAsync(P.lt.p, Pmat, sizeof(double) * P.size, hipMemcpyHostToDevice, s) != hipSuccess)
+            rc = fail(ASTRO_ERR_DEVICE, "upload failed");
+        if (rc == ASTRO_OK) rc = av.upload(Amat, ARange[ADim], s);
+        if (rc == ASTRO_OK) rc = ac.upload(Acols, ARange[ADim], s);
+        if (rc == ASTRO_OK) rc = ar.upload(ARange, ADim + 1, s);
+        if (rc == ASTRO_OK && CDim > 0) rc = cv.upload(Cmat, CRange[CDim], s);
+        if (rc == ASTRO_OK && CDim > 0) rc = cc.upload(Ccols, CRange[CDim], s);
+        if (rc == ASTRO_OK && CDim > 0) rc = cr.upload(CRange, CDim + 1, s);
+        if (rc == ASTRO_OK) rc = b.upload(bVec, ADim, s);
+        if (rc == ASTRO_OK) rc = x.upload(xVec, ADim, s);
+        if (rc == ASTRO_OK) rc = gm.setup(ADim, numVecs);
+        if (rc == ASTRO_OK) {
+            op.n = ADim;
+            op.cdim = CDim > 0 ? CDim : 0;
+            op.val = av.p;
+            op.cols = ac.p;
+            op.range = ar.p;
+            op.cmat = cv.p;
+            op.ccols = cc.p;
+            op.crange = cr.p;
+            const int maxCycles = maxIt / numVecs + 1;
+            if (rtol <= 0 && atol <= 0) atol = (double)1e-12f;
+            rc = gm.solve(op, P, b.p, x.p, rtol > 0 ? rtol : 0.0, atol, maxCycles, s);
+        }
+        if (rc == ASTRO_OK || rc == ASTRO_ERR_NO_CONVERGE) {
+            if (hipMemcpyAsync(xVec, x.p, sizeof(double) * ADim, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                rc = fail(ASTRO_ERR_DEVICE, "download failed");
+        }
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int astro_convertToLTri(double* lTri, const int32_t* lTRange, int64_t lTSize, const double* mainMat,
+                        const int32_t* mainCols, const int32_t* mainRange, int64_t mainSize, int32_t mainDim,
+                        const double* cMat, const int32_t* cCols, const int32_t* cRange, int64_t cSize,
+                        int32_t cDim) {
+    if (!lTri || !lTRange || !mainMat || !mainCols || !mainRange) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(check_range(lTRange, mainDim, lTSize, "convertToLTri profile"));
+    ASTRO_TRY(check_range(mainRange, mainDim, mainSize, "convertToLTri CSR"));
+    if (cDim > 0) {
+        if (!cMat || !cCols || !cRange) return fail(ASTRO_ERR_ARG, "null constraint arrays");
+        ASTRO_TRY(check_range(cRange, cDim, cSize, "convertToLTri C"));
+        for (int64_t q = 0; q < cRange[cDim]; ++q)
+            if (cCols[q] < 0 || cCols[q] > mainDim) return fail(ASTRO_ERR_ARG, "C column out of range");
+    }
+    if (mainDim == 0) return ASTRO_OK;
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc = ASTRO_OK;
+    {
+        std::vector<int> sf(mainDim);
+        std::vector<int64_t> off(mainDim);
+        for (int r = 0; r < mainDim; ++r) {
+            int len = lTRange[r + 1] - lTRange[r];
+            if (len < 1 || len > r + 1) rc = fail(ASTRO_ERR_ARG, "invalid profile row length");
+            sf[r] = r - len + 1;
+            off[r] = (int64_t)lTRange[r + 1] - 1 - r;
+        }
+        DBuf<double> lt, mv, cv;
+        DBuf<int> mc, mr, cc, cr, sfd;
+        DBuf<int64_t> offd;
+        const int64_t sz = lTRange[mainDim];
+        if (rc == ASTRO_OK) rc = lt.alloc(sz);
+        if (rc == ASTRO_OK) rc = sfd.upload(sf, s);
+        if (rc == ASTRO_OK) rc = offd.upload(off, s);
+        if (rc == ASTRO_OK) rc = mv.upload(mainMat, mainRange[mainDim], s);
+        if (rc == ASTRO_OK) rc = mc.upload(mainCols, mainRange[mainDim], s);
+        if (rc == ASTRO_OK) rc = mr.upload(mainRange, mainDim + 1, s);
+        if (rc == ASTRO_OK && cDim > 0) rc = cv.upload(cMat, cRange[cDim], s);
+        if (rc == ASTRO_OK && cDim > 0) rc = cc.upload(cCols, cRange[cDim], s);
+        if (rc == ASTRO_OK && cDim > 0) rc = cr.upload(cRange, cDim + 1, s);
+        if (rc == ASTRO_OK) {
+            hipLaunchKernelGGL(k_fill, dim3(grid_for(sz, 256, 16384)), dim3(256), 0, s, lt.p, sz, 0.0);
+            hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)mainDim * 32, 256, 1 << 30)), dim3(256), 0, s,
+                               mv.p, mc.p, mr.p, mainDim, sfd.p, offd.p, lt.p);
+            if (cDim > 0)
+                hipLaunchKernelGGL(k_penalty_to_profile, dim3(grid_for(cDim, 256)), dim3(256), 0, s, cv.p, cc.p, cr.p,
+                                   cDim, sfd.p, offd.p, lt.p);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(lTri, lt.p, sizeof(double) * sz, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                rc = fail(ASTRO_ERR_DEVICE, "convertToLTri device error");
+            for (int64_t q = sz; q < lTSize; ++q) lTri[q] = 0.0;
+        }
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,765 @@
+// astro_kernels.hip -- device kernels of the AStrO hot path for gfx950 (CDNA4).
+//
+// All arithmetic is FP64.  Index conventions inside the kernels are 0-based;
+// the reference's 1-based column numbers are converted where CSR columns are
+// read.  A row-skyline profile row r covers columns s[r]..r and its entry
+// (r, c) lives at lt[rowoff[r] + c].
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "astro_kernels.h"
+
+namespace astro {
+
+// ---------------------------------------------------------------------------
+// Element tangents: brick8, 2x2x2 Gauss, B^T C B (+ massFactor * rho N^T N),
+// one 64-lane wavefront per element, lane (a, b) owning the 3x3 block of node
+// pair (a, b).  Working/AStrO_r_elementEqns.f90:2206-2533 (solid branch at
+// u = 0) with shape functions :27-36, :110-135, Jacobian :580-620, B columns of
+// r_getNLStrain :649-662 and the mass of r_getElRum :2628-2653.
+//
+// Elements of one launch share no node (colour classes), so the scatter into
+// the CSR values is a plain read-modify-write.  The slot of column (node b,
+// component d) in the rows of node a is elPos[e][a][b][d]; rows of node a start
+// at csrRange[dof(a, c) - 1] with dof(a, c) = rank(a) + c * nS.
+__global__ __launch_bounds__(64) void k_brick8_tangent(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const int* __restrict__ elMat, const double* __restrict__ elScale, const double* __restrict__ matC,
+    const double* __restrict__ elRho, double massFactor, const int* __restrict__ rank, int nS,
+    const int* __restrict__ csrRange, const uint8_t* __restrict__ elPos, double* __restrict__ val) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[8][3];
+    __shared__ double sNx[8][8][3];
+    __shared__ double sN[8][8];
+    __shared__ double sW[8];
+    __shared__ int sNode[8];
+    if (lane < 8) {
+        int nd = conn[8 * (size_t)e + lane] - 1;
+        sNode[lane] = nd;
+        sX[lane][0] = X[3 * (size_t)nd];
+        sX[lane][1] = X[3 * (size_t)nd + 1];
+        sX[lane][2] = X[3 * (size_t)nd + 2];
+    }
+    __syncthreads();
+    if (lane < 8) {
+        const double g = 0.57735026918962576451;  // 1/sqrt(3)
+        const int gp = lane;
+        const double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
+        double Ns[8][3];
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const double sg0 = ((n + 1) & 2) ? 1.0 : -1.0;  // nodes 2,3,6,7 at +s1
+            const double sg1 = (n & 2) ? 1.0 : -1.0;        // nodes 3,4,7,8 at +s2
+            const double sg2 = (n & 4) ? 1.0 : -1.0;        // nodes 5-8 at +s3
+            const double f0 = 1.0 + sg0 * s0, f1 = 1.0 + sg1 * s1, f2 = 1.0 + sg2 * s2;
+            sN[gp][n] = 0.125 * f0 * f1 * f2;
+            Ns[n][0] = 0.125 * sg0 * f1 * f2;
+            Ns[n][1] = 0.125 * sg1 * f0 * f2;
+            Ns[n][2] = 0.125 * sg2 * f0 * f1;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) J[i][j] += sX[n][i] * Ns[n][j];
+        }
+        const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
+        const double di = 1.0 / det;
+        double I[3][3];
+        I[0][0] = di * a11;
+        I[1][0] = di * a12;
+        I[2][0] = di * a13;
+        I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+        I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+        I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+        I[0][2] = di * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+        I[1][2] = di * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
+        I[2][2] = di * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) sNx[gp][n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j] + Ns[n][2] * I[2][j];
+        sW[gp] = det;  // Gauss weight 1
+    }
+    __syncthreads();
+    const int a = lane >> 3, b = lane & 7;
+    const double sc = elScale[e];
+    const double* Cm = matC + 36 * (size_t)elMat[e];
+    double C[6][6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) C[r][c] = sc * Cm[r + 6 * c];
+    double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double mf = massFactor * elRho[e];
+#pragma unroll 2
+    for (int g = 0; g < 8; ++g) {
+        const double w = sW[g];
+        const double ax = sNx[g][a][0], ay = sNx[g][a][1], az = sNx[g][a][2];
+        const double bx = sNx[g][b][0], by = sNx[g][b][1], bz = sNx[g][b][2];
+        double D[6][3];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            D[r][0] = C[r][0] * bx + C[r][3] * by + C[r][4] * bz;
+            D[r][1] = C[r][1] * by + C[r][3] * bx + C[r][5] * bz;
+            D[r][2] = C[r][2] * bz + C[r][4] * bx + C[r][5] * by;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            K[0][d] += w * (ax * D[0][d] + ay * D[3][d] + az * D[4][d]);
+            K[1][d] += w * (ay * D[1][d] + ax * D[3][d] + az * D[5][d]);
+            K[2][d] += w * (az * D[2][d] + ax * D[4][d] + ay * D[5][d]);
+        }
+        if (mf != 0.0) {
+            const double m = mf * w * sN[g][a] * sN[g][b];
+            K[0][0] += m;
+            K[1][1] += m;
+            K[2][2] += m;
+        }
+    }
+    const int ra = rank[sNode[a]];
+    const uint8_t* pos = elPos + 192 * (size_t)e + 3 * lane;
+    const int p0 = pos[0], p1 = pos[1], p2 = pos[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int row = ra + c * nS;  // 1-based DOF
+        double* v = val + csrRange[row - 1];
+        v[p0] += K[c][0];
+        v[p1] += K[c][1];
+        v[p2] += K[c][2];
+    }
+}
+
+// Brick8 conductivity (Working/AStrO_r_elementEqns.f90:2062-2124) scattered
+// into the thermal CSR by node rank; slot 255 marks a column the reference's
+// probe never stores (its `.lt.` bound, Working/AStrO_commandFunctions.f90:57).
+__global__ __launch_bounds__(64) void k_brick8_conduct(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ elCond, const int* __restrict__ rank, const int* __restrict__ csrRange,
+    const uint8_t* __restrict__ thPos, double* __restrict__ val) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[8][3];
+    __shared__ double sNx[8][8][3];
+    __shared__ double sW[8];
+    __shared__ int sNode[8];
+    if (lane < 8) {
+        int nd = conn[8 * (size_t)e + lane] - 1;
+        sNode[lane] = nd;
+        for (int c = 0; c < 3; ++c) sX[lane][c] = X[3 * (size_t)nd + c];
+    }
+    __syncthreads();
+    if (lane < 8) {
+        const double g = 0.57735026918962576451;
+        const int gp = lane;
+        const double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
+        double Ns[8][3];
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int n = 0; n < 8; ++n) {
+            const double sg0 = ((n + 1) & 2) ? 1.0 : -1.0, sg1 = (n & 2) ? 1.0 : -1.0, sg2 = (n & 4) ? 1.0 : -1.0;
+            const double f0 = 1.0 + sg0 * s0, f1 = 1.0 + sg1 * s1, f2 = 1.0 + sg2 * s2;
+            Ns[n][0] = 0.125 * sg0 * f1 * f2;
+            Ns[n][1] = 0.125 * sg1 * f0 * f2;
+            Ns[n][2] = 0.125 * sg2 * f0 * f1;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) J[i][j] += sX[n][i] * Ns[n][j];
+        }
+        const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
+        const double di = 1.0 / det;
+        double I[3][3];
+        I[0][0] = di * a11;
+        I[1][0] = di * a12;
+        I[2][0] = di * a13;
+        I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+        I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+        I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+        I[0][2] = di * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+        I[1][2] = di * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
+        I[2][2] = di * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+        for (int n = 0; n < 8; ++n)
+            for (int j = 0; j < 3; ++j) sNx[gp][n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j] + Ns[n][2] * I[2][j];
+        sW[gp] = det;
+    }
+    __syncthreads();
+    const int a = lane >> 3, b = lane & 7;
+    const double* kc = elCond + 6 * (size_t)e;
+    const double k00 = kc[0], k11 = kc[1], k22 = kc[2], k01 = kc[3], k02 = kc[4], k12 = kc[5];
+    double acc = 0.0;
+    for (int g = 0; g < 8; ++g) {
+        const double ax = sNx[g][a][0], ay = sNx[g][a][1], az = sNx[g][a][2];
+        const double bx = sNx[g][b][0], by = sNx[g][b][1], bz = sNx[g][b][2];
+        const double kbx = k00 * bx + k01 * by + k02 * bz;
+        const double kby = k01 * bx + k11 * by + k12 * bz;
+        const double kbz = k02 * bx + k12 * by + k22 * bz;
+        acc += sW[g] * (ax * kbx + ay * kby + az * kbz);
+    }
+    const int slot = thPos[64 * (size_t)e + lane];
+    if (slot != 255) {
+        const int row = rank[sNode[a]];
+        val[csrRange[row - 1] + slot] += acc;
+    }
+}
+
+// max |v| over a vector, combined into *out as the bit pattern of a
+// non-negative double (ordered like an unsigned integer).
+__global__ void k_max_abs(const double* __restrict__ v, int64_t n, unsigned long long* __restrict__ out) {
+    double m = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(v[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    __shared__ double sm[16];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sm[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, sm[i]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(r));
+    }
+}
+
+// Constraint rows normalised, then scaled by 100 * max|K|.
+// Working/AStrO_commandFunctions.f90:537-566.
+__global__ void k_scale_mpc(double* __restrict__ cmat, double* __restrict__ rhs, const int* __restrict__ crange,
+                            int cdim, const unsigned long long* __restrict__ maxBits) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cdim) return;
+    const double maxMat = __longlong_as_double((long long)*maxBits);
+    const int a = crange[r], b = crange[r + 1];
+    double mag = 0.0;
+    for (int s = a; s < b; ++s) mag += cmat[s] * cmat[s];
+    mag = sqrt(mag);
+    const double inv = 1.0 / mag, f = 100.0 * maxMat;
+    for (int s = a; s < b; ++s) cmat[s] = f * (inv * cmat[s]);
+    rhs[r] = f * (inv * rhs[r]);
+}
+
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// In-profile CSR entries into the (zeroed) profile: row r keeps columns
+// s[r] <= c <= r.  Working/AStrO_bookKeeping.f90:764-774.  32 lanes per row.
+__global__ __launch_bounds__(256) void k_csr_to_profile(const double* __restrict__ val, const int* __restrict__ cols,
+                                                        const int* __restrict__ range, int n,
+                                                        const int* __restrict__ sfirst,
+                                                        const int64_t* __restrict__ rowoff, double* __restrict__ lt) {
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const int l = threadIdx.x & 31;
+    if (row >= n) return;
+    const int s = sfirst[row];
+    const int64_t off = rowoff[row];
+    for (int q = range[row] + l; q < range[row + 1]; q += 32) {
+        const int c = cols[q] - 1;
+        if (c >= s && c <= row) lt[off + c] += val[q];
+    }
+}
+
+// Penalty C^T C into the profile (Working/AStrO_bookKeeping.f90:776-790); one
+// thread per constraint row, atomics because rows may share a DOF.
+__global__ void k_penalty_to_profile(const double* __restrict__ cmat, const int* __restrict__ ccols,
+                                     const int* __restrict__ crange, int cdim, const int* __restrict__ sfirst,
+                                     const int64_t* __restrict__ rowoff, double* __restrict__ lt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cdim) return;
+    for (int s2 = crange[r]; s2 < crange[r + 1]; ++s2) {
+        const int i4 = ccols[s2] - 1;
+        if (i4 < 0) continue;
+        for (int s3 = crange[r]; s3 < crange[r + 1]; ++s3) {
+            const int i5 = ccols[s3] - 1;
+            if (i5 >= sfirst[i4] && i5 <= i4) atomicAdd(&lt[rowoff[i4] + i5], cmat[s2] * cmat[s3]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked left-looking LDL^T on the row-skyline profile.
+//
+// Panel P = columns [p, pe), pe <= p + 64, of one independent block.  Rows
+// [p, iend) may reach into P.  Step U: W(i, j) = sum_{k < p} L(i,k) D(k) L(j,k)
+// for 64-row tiles of those rows and j in P, split over splitK ranges of k,
+// partial sums to scratch (deterministic).  Step FT: every tile re-forms the
+// updated diagonal block A(P,P) - W, factors it in LDS (tile 0 writes it
+// back), then solves its own rows: L(i,P) = (A(i,P) - W(i,P)) L_PP^-T D_P^-1.
+// Profile masking keeps every entry with c < s[r] an implicit zero; the LDL^T
+// of an envelope matrix stays inside the envelope, as the reference's
+// in-place row factor (Working/AStrO_solvers.f90:10-78) relies on.
+
+__global__ __launch_bounds__(256) void k_ldl_update(const PanelInfo* __restrict__ panels, int panelIdx, int numPanelsPerBlock,
+                                                    int splitK, double* __restrict__ lt,
+                                                    const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
+                                                    double* __restrict__ scratch, int scratchRows) {
+    const int blk = blockIdx.y;
+    const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
+    if (P.p < 0) return;
+    const int tile = blockIdx.x / splitK, sp = blockIdx.x % splitK;
+    const int r0 = P.p + 64 * tile;
+    if (r0 >= P.iend) return;
+    const int nr = min(64, P.iend - r0);
+    const int np = P.pe - P.p;
+    __shared__ double As[16][64 + 1];
+    __shared__ double Bs[16][64 + 1];
+    __shared__ double Dk[16];
+    __shared__ int sMin[2];
+    const int tid = threadIdx.x;
+    // k range: the tile's and the panel's first columns bound it from below
+    if (tid < 64) {
+        int v = tid < nr ? sfirst[r0 + tid] : 0x7fffffff;
+        for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+        if (tid == 0) sMin[0] = v;
+    }
+    __syncthreads();
+    const int kLo = max(sMin[0], P.smin), kHi = P.p;
+    const int total = max(0, kHi - kLo);
+    int chunk = (total + splitK - 1) / splitK;
+    chunk = (chunk + 15) & ~15;
+    const int kb = kLo + sp * chunk, ke = min(kHi, kb + chunk);
+    const int tr = tid >> 4, tc = tid & 15;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        __syncthreads();
+        if (tid < 16) {
+            const int k = k0 + tid;
+            Dk[tid] = k < ke ? lt[rowoff[k] + k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int id = q * 256 + tid;
+            const int rr = id >> 4, kk = id & 15;
+            const int k = k0 + kk;
+            double va = 0.0, vb = 0.0;
+            if (k < ke) {
+                if (rr < nr) {
+                    const int i = r0 + rr;
+                    if (k >= sfirst[i]) va = lt[rowoff[i] + k];
+                }
+                if (rr < np) {
+                    const int j = P.p + rr;
+                    if (k >= sfirst[j]) vb = lt[rowoff[j] + k];
+                }
+            }
+            As[kk][rr] = va;
+            Bs[kk][rr] = vb;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            const double d = Dk[kk];
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[kk][tr * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = d * Bs[kk][tc * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+        }
+    }
+    double* out = scratch + (((size_t)sp * gridDim.y + blk) * scratchRows + (r0 - P.p)) * 64;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int rr = tr * 4 + i;
+        if (rr < nr) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) out[(size_t)rr * 64 + tc * 4 + j] = acc[i][j];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __restrict__ panels, int panelIdx,
+                                                          int numPanelsPerBlock, int splitK, double* __restrict__ lt,
+                                                          const int64_t* __restrict__ rowoff,
+                                                          const int* __restrict__ sfirst,
+                                                          const double* __restrict__ scratch, int scratchRows,
+                                                          int* __restrict__ zeroPivot) {
+    const int blk = blockIdx.y;
+    const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
+    if (P.p < 0) return;
+    const int tile = blockIdx.x;
+    const int r0 = P.p + 64 * tile;
+    if (r0 >= P.iend) return;
+    const int nr = min(64, P.iend - r0);
+    const int np = P.pe - P.p;
+    __shared__ double Dg[64][65];
+    __shared__ double Y[64][65];
+    const int tid = threadIdx.x;
+    const size_t nb = gridDim.y;
+    auto wsum = [&](int rowInPanel, int j) {
+        double s = 0.0;
+        for (int q = 0; q < splitK; ++q) s += scratch[(((size_t)q * nb + blk) * scratchRows + rowInPanel) * 64 + j];
+        return s;
+    };
+    // updated diagonal block (lower triangle; zeros outside the profile)
+    for (int id = tid; id < 64 * 64; id += 256) {
+        const int i = id >> 6, j = id & 63;
+        double v = 0.0;
+        if (i < np && j <= i) {
+            const int gi = P.p + i, gj = P.p + j;
+            if (gj >= sfirst[gi]) v = lt[rowoff[gi] + gj] - wsum(i, j);
+        }
+        Dg[i][j] = v;
+    }
+    __syncthreads();
+    for (int j = 0; j < np; ++j) {
+        const double d = Dg[j][j];
+        if (tid == 0 && d == 0.0 && tile == 0) atomicMin(zeroPivot, P.p + j + 1);
+        const double inv = 1.0 / d;
+        for (int i = j + 1 + tid; i < np; i += 256) Dg[i][j] *= inv;
+        __syncthreads();
+        // trailing update Dg[i][k] -= l_ij d l_kj, j < k <= i < np
+        const int m = np - j - 1;
+        const int npair = m * (m + 1) / 2;
+        for (int q = tid; q < npair; q += 256) {
+            // map q -> (ii, kk) with 0 <= kk <= ii < m
+            int ii = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+            while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
+            while (ii * (ii + 1) / 2 > q) --ii;
+            const int kk = q - ii * (ii + 1) / 2;
+            const int i = j + 1 + ii, k = j + 1 + kk;
+            Dg[i][k] -= Dg[i][j] * d * Dg[k][j];
+        }
+        __syncthreads();
+    }
+    if (tile == 0) {
+        for (int id = tid; id < 64 * 64; id += 256) {
+            const int i = id >> 6, j = id & 63;
+            if (i < np && j <= i) {
+                const int gi = P.p + i, gj = P.p + j;
+                if (gj >= sfirst[gi]) lt[rowoff[gi] + gj] = Dg[i][j];
+            }
+        }
+        return;
+    }
+    // rows below the panel: y = L_PP^-1 a, L(i, j) = y_j / D_j; 4 lanes per row
+    const int row = tid >> 2, q = tid & 3;
+    const int gi = r0 + row;
+    const bool live = row < nr;
+    const int si = live ? sfirst[gi] : 0x7fffffff;
+    for (int j = 0; j < np; ++j) {
+        double part = 0.0;
+        for (int k = q; k < j; k += 4) part += Dg[j][k] * Y[row][k];
+        part += __shfl_xor(part, 1);
+        part += __shfl_xor(part, 2);
+        if ((j & 3) == q) {
+            const int gj = P.p + j;
+            double aij = 0.0;
+            if (live && gj >= si) aij = lt[rowoff[gi] + gj] - wsum(r0 - P.p + row, j);
+            Y[row][j] = aij - part;
+        }
+    }
+    // (each lane only reads the Y entries it wrote itself: k == q mod 4)
+    for (int j = q; j < np; j += 4) {
+        const int gj = P.p + j;
+        if (live && gj >= si) lt[rowoff[gi] + gj] = Y[row][j] / Dg[j][j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular sweeps on the factored profile, one 64-row tile per workgroup in
+// dependency order.  Tiles are handed out by an atomic ticket, so a tile is
+// only ever processed after every tile it depends on has been claimed by a
+// running workgroup; waits on their completion flags are bounded (spinLimit),
+// after which the kernel records a timeout and exits.
+//
+// Forward (z = L^-1 b, Working/AStrO_solvers.f90:95-107): tile rows need z of
+// the earlier tiles their profiles reach.  Backward (x = D^-1 z - L^T x,
+// :109-123): tile columns need x of the later tiles whose rows reach them.
+
+__device__ __forceinline__ bool wait_flag(const int* flag, int epoch, long spinLimit, int* timeout) {
+    long it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++it > spinLimit) {
+            atomicExch(timeout, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
+                                                       const double* __restrict__ lt,
+                                                       const int64_t* __restrict__ rowoff,
+                                                       const int* __restrict__ sfirst, const double* __restrict__ b,
+                                                       double* __restrict__ z, int* __restrict__ flags,
+                                                       int* __restrict__ ticket, int epoch, long spinLimit,
+                                                       int* __restrict__ timeout) {
+    __shared__ int sT;
+    __shared__ double Ld[64][65];
+    __shared__ double part[4][64];
+    __shared__ double acc[64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) sT = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int t = sT;
+    if (t >= numTiles) return;
+    const SweepTile T = tiles[t];
+    const int r0 = T.r0, nr = T.r1 - T.r0;
+    // diagonal block rows (independent of z): Ld[i][c] = L(r0+i, r0+c), c < i
+    for (int id = tid; id < 64 * 64; id += 256) {
+        const int i = id >> 6, c = id & 63;
+        double v = 0.0;
+        if (i < nr && c < i) {
+            const int gi = r0 + i, gc = r0 + c;
+            if (gc >= sfirst[gi]) v = lt[rowoff[gi] + gc];
+        }
+        Ld[i][c] = v;
+    }
+    // earlier tiles: wave w owns rows w*16 .. w*16+15, lane = column offset
+    double p[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) p[r] = 0.0;
+    bool ok = true;
+    for (int J = T.depFirst; J < t && ok; ++J) {
+        const SweepTile D = tiles[J];
+        if (D.chain != T.chain) continue;
+        if (tid == 0) ok = wait_flag(&flags[J], epoch, spinLimit, timeout);
+        ok = __syncthreads_or(!ok) == 0;
+        if (!ok) break;
+        const int c = D.r0 + lane;
+        const double zc = c < D.r1 ? z[c] : 0.0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = w * 16 + r;
+            if (i < nr && c < D.r1) {
+                const int gi = r0 + i;
+                if (c >= sfirst[gi]) p[r] += lt[rowoff[gi] + c] * zc;
+            }
+        }
+    }
+    if (!ok) return;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        double v = p[r];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) part[w][r] = v;
+    }
+    __syncthreads();
+    if (w == 0) {
+        // part[w'][r] holds row w'*16 + r
+        const int i = lane;
+        double a = 0.0;
+        if (i < nr) a = b[r0 + i] - part[i >> 4][i & 15];
+        for (int c = 0; c < nr; ++c) {
+            const double zc = __shfl(a, c);
+            if (i > c) a -= Ld[i][c] * zc;
+        }
+        if (i < nr) z[r0 + i] = a;
+        __threadfence();
+        if (i == 0) __hip_atomic_store(&flags[t], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    (void)acc;
+}
+
+__global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restrict__ tiles, int numTiles,
+                                                        const double* __restrict__ lt,
+                                                        const int64_t* __restrict__ rowoff,
+                                                        const int* __restrict__ sfirst, const double* __restrict__ z,
+                                                        double* __restrict__ x, int* __restrict__ flags,
+                                                        int* __restrict__ ticket, int epoch, long spinLimit,
+                                                        int* __restrict__ timeout) {
+    __shared__ int sT;
+    __shared__ double Ld[64][65];
+    __shared__ double part[4][64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) sT = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int t = sT;
+    if (t >= numTiles) return;
+    const SweepTile T = tiles[t];
+    const int r0 = T.r0, nr = T.r1 - T.r0;
+    // Ld[c][i] = L(r0+c, r0+i) for i < c (row c of the diagonal block)
+    for (int id = tid; id < 64 * 64; id += 256) {
+        const int c = id >> 6, i = id & 63;
+        double v = 0.0;
+        if (c < nr && i < c) {
+            const int gc = r0 + c, gi = r0 + i;
+            if (gi >= sfirst[gc]) v = lt[rowoff[gc] + gi];
+        }
+        Ld[c][i] = v;
+    }
+    // later tiles (processed earlier): lane = column i of this tile, wave w
+    // takes rows j == w (mod 4) of each later tile
+    double p = 0.0;
+    bool ok = true;
+    const int ci = r0 + lane;
+    for (int J = T.depFirst; J < t && ok; ++J) {
+        const SweepTile D = tiles[J];
+        if (D.chain != T.chain) continue;
+        if (tid == 0) ok = wait_flag(&flags[J], epoch, spinLimit, timeout);
+        ok = __syncthreads_or(!ok) == 0;
+        if (!ok) break;
+        for (int j = D.r0 + w; j < D.r1; j += 4) {
+            const int sj = sfirst[j];
+            if (lane < nr && ci >= sj) p += lt[rowoff[j] + ci] * x[j];
+        }
+    }
+    if (!ok) return;
+    part[w][lane] = p;
+    __syncthreads();
+    if (w == 0) {
+        const int i = lane;
+        double a = 0.0;
+        if (i < nr) {
+            const int gi = r0 + i;
+            a = z[gi] / lt[rowoff[gi] + gi] - (part[0][i] + part[1][i] + part[2][i] + part[3][i]);
+        }
+        for (int c = nr - 1; c >= 0; --c) {
+            const double xc = __shfl(a, c);
+            if (i < c) a -= Ld[c][i] * xc;
+        }
+        if (i < nr) x[r0 + i] = a;
+        __threadfence();
+        if (i == 0) __hip_atomic_store(&flags[t], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_reset_ticket(int* ticket) { *ticket = 0; }
+
+// ---------------------------------------------------------------------------
+// Operator y = A x + C^T (C x) with the reference's CSR (1-based columns,
+// 0 = unused slot).  Working/AStrO_solvers.f90:143-171.  32 lanes per row.
+__global__ __launch_bounds__(256) void k_spmv(const double* __restrict__ val, const int* __restrict__ cols,
+                                              const int* __restrict__ range, int n, const double* __restrict__ x,
+                                              double* __restrict__ y) {
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const int l = threadIdx.x & 31;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int q = range[row] + l; q < range[row + 1]; q += 32) {
+        const int c = cols[q];
+        if (c) s += val[q] * x[c - 1];
+    }
+    for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
+    if (l == 0) y[row] = s;
+}
+
+__global__ void k_mpc_apply(const double* __restrict__ cmat, const int* __restrict__ ccols,
+                            const int* __restrict__ crange, int cdim, const double* __restrict__ x,
+                            double* __restrict__ y) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cdim) return;
+    double t = 0.0;
+    for (int s = crange[r]; s < crange[r + 1]; ++s)
+        if (ccols[s]) t += cmat[s] * x[ccols[s] - 1];
+    for (int s = crange[r]; s < crange[r + 1]; ++s)
+        if (ccols[s]) atomicAdd(&y[ccols[s] - 1], cmat[s] * t);
+}
+
+// RHS pieces: nodal loads and the constraint load C^T (rhs - C u).
+// Working/AStrO_commandFunctions.f90:368-392, 506-535.
+__global__ void k_nodal_loads(const int* __restrict__ dofs, const double* __restrict__ vals, int count,
+                              double* __restrict__ load) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    atomicAdd(&load[dofs[k] - 1], vals[k]);
+}
+
+__global__ void k_constraint_load(const double* __restrict__ cmat, const int* __restrict__ ccols,
+                                  const int* __restrict__ crange, const double* __restrict__ crhs, int cdim,
+                                  const double* __restrict__ u, double* __restrict__ load) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cdim) return;
+    double t = crhs[r];
+    for (int s = crange[r]; s < crange[r + 1]; ++s)
+        if (ccols[s]) t -= cmat[s] * u[ccols[s] - 1];
+    for (int s = crange[r]; s < crange[r + 1]; ++s)
+        if (ccols[s]) atomicAdd(&load[ccols[s] - 1], cmat[s] * t);
+}
+
+// ---------------------------------------------------------------------------
+// Krylov vector kernels.  Multi-dot: partial[blk][q] = sum over the block's
+// rows of V[q][i] * w[i] for q in [q0, q0 + nq); the host-side launcher sums
+// the partials in a second fixed-order pass (deterministic).
+__global__ __launch_bounds__(256) void k_multidot(const double* __restrict__ V, int64_t ldv, int nqTotal,
+                                                  const double* __restrict__ w, int n, double* __restrict__ partial,
+                                                  int pstride) {
+    const int tid = threadIdx.x;
+    const int q0 = blockIdx.y * 8, nq = min(8, nqTotal - q0);
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+    const int rowsPerBlock = (n + gridDim.x - 1) / gridDim.x;
+    const int i0 = blockIdx.x * rowsPerBlock, i1 = min(n, i0 + rowsPerBlock);
+    for (int i = i0 + tid; i < i1; i += 256) {
+        const double wi = w[i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nq) acc[k] += V[(size_t)(q0 + k) * ldv + i] * wi;
+    }
+    __shared__ double red[4][8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if ((tid & 63) == 0) red[tid >> 6][k] = v;
+    }
+    __syncthreads();
+    if (tid < nq) partial[(size_t)blockIdx.x * pstride + q0 + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int nblk, int pstride, int nq,
+                               double* __restrict__ out) {
+    const int q = threadIdx.x;
+    if (q >= nq) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * pstride + q];
+    out[q] = s;
+}
+
+// w -= sum_q V[q] h[q]   (h on the device)
+__global__ void k_sub_combination(const double* __restrict__ V, int64_t ldv, int nq, const double* __restrict__ h,
+                                  double* __restrict__ w, int n, double sign) {
+    __shared__ double sh[256];
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) sh[q] = h[q];
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < nq; ++q) s += V[(size_t)q * ldv + i] * sh[q];
+        w[i] += sign * s;
+    }
+}
+
+// y = alpha * x  (alpha read from device memory as 1 / *den when den != null)
+__global__ void k_scale_copy(const double* __restrict__ x, double* __restrict__ y, int n,
+                             const double* __restrict__ den) {
+    const double a = 1.0 / *den;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[i];
+}
+
+__global__ void k_axpby(const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ out,
+                        int n, double a, double b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = a * x[i] + b * y[i];
+}
+
+// CSR max |value| entry check helper: count entries with column 0 (unused slots)
+__global__ void k_count_zero_cols(const int* __restrict__ cols, int64_t nnz, int* __restrict__ out) {
+    int c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        c += cols[i] == 0;
+    if (c) atomicAdd(out, c);
+}
+
+}  // namespace astro

@@ -1,0 +1,170 @@
+"""Synthetic structured models for the workloads of the hot path.
+
+A :class:`Model` holds exactly what the reference's model file carries for the
+linear-static, thermal and Newmark cases this package solves
+(Working/InputTemplate.yaml): node coordinates, element connectivity and type,
+per-element material constants, multi-point constraints and nodal loads.
+Numbering follows the reference's conventions: node and element ids are
+1-based, connectivity rows hold 8 node ids padded with 0
+(Working/AStrO_globalData.f90 ``elementList(8, numEls)``), element type codes
+are those of Working/AStrO_input.f90:429-452 (8 = brick8, 41 = shell4).
+
+Node id = i + NX*(j + NY*k) + 1 with x fastest; elements are ordered x, y, z.
+Brick connectivity (i,j,k), (i+1,j,k), (i+1,j+1,k), (i,j+1,k) then the same
+four at k+1, which gives detJ > 0 with the shape functions of
+Working/AStrO_r_elementEqns.f90:27-35.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+BRICK8 = 8
+SHELL4 = 41
+
+
+@dataclass
+class MPCTerm:
+    """One ``[eqn, node|set, dof, coef]`` line of the constraints block."""
+
+    eqn: int
+    nodes: np.ndarray  # 1-based node ids
+    dof: int
+    coef: float
+    is_set: bool
+    set_name: str = ""
+
+
+@dataclass
+class Model:
+    name: str
+    nodes: np.ndarray            # (N, 3) float64
+    elements: np.ndarray         # (E, 8) int32, 1-based, 0-padded
+    etype: np.ndarray            # (E,) int32
+    eprops: np.ndarray           # (E, 9) E1,E2,E3,nu12,nu13,nu23,G12,G13,G23
+    density: np.ndarray          # (E,)
+    cond: np.ndarray             # (E, 6) k11,k22,k33,k12,k13,k23
+    mpc: List[MPCTerm] = field(default_factory=list)
+    mpc_rhs: List[float] = field(default_factory=list)
+    load_nodes: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    load_vals: np.ndarray = field(default_factory=lambda: np.zeros((0, 7)))
+    node_sets: dict = field(default_factory=dict)
+    colors: Optional[np.ndarray] = None  # (E,) element colour, conflict-free scatter
+
+    @property
+    def num_nodes(self) -> int:
+        return int(self.nodes.shape[0])
+
+    @property
+    def num_els(self) -> int:
+        return int(self.elements.shape[0])
+
+
+def iso_props(E, nu, G=None):
+    """Engineering constants of an isotropic material, G = E / (2 (1 + nu))
+    as Working/AStrO_input.f90:807-809 derives it when only E and nu are given."""
+    if G is None:
+        G = E / (2.0 * (1.0 + nu))
+    return np.array([E, E, E, nu, nu, nu, G, G, G], dtype=np.float64)
+
+
+def brick_block(nx, ny, nz, lx, ly, lz):
+    """Nodes and brick8 connectivity of an nx*ny*nz structured block."""
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    i = np.arange(NX)
+    j = np.arange(NY)
+    k = np.arange(NZ)
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    nodes = np.stack([I.ravel() * (lx / nx), J.ravel() * (ly / ny), K.ravel() * (lz / nz)], axis=1)
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    ek, ej, ei = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ei, ej, ek = ei.ravel(), ej.ravel(), ek.ravel()
+
+    def nid(a, b, c):
+        return a + NX * (b + NY * c) + 1
+
+    conn = np.stack(
+        [nid(ei, ej, ek), nid(ei + 1, ej, ek), nid(ei + 1, ej + 1, ek), nid(ei, ej + 1, ek),
+         nid(ei, ej, ek + 1), nid(ei + 1, ej, ek + 1), nid(ei + 1, ej + 1, ek + 1), nid(ei, ej + 1, ek + 1)],
+        axis=1).astype(np.int32)
+    colors = ((ei & 1) | ((ej & 1) << 1) | ((ek & 1) << 2)).astype(np.int32)
+    return nodes, conn, colors
+
+
+def face_pressure_loads(nx, ny, nz, lx, ly, lz, pressure):
+    """Consistent nodal forces of a uniform pressure on the top face z = lz of a
+    brick block with bilinear faces: p * hx * hy / 4 from every face touching the
+    node (interior 1, edge 1/2, corner 1/4 of p * hx * hy), pointing -z."""
+    NX, NY = nx + 1, ny + 1
+    hx, hy = lx / nx, ly / ny
+    wx = np.full(NX, 1.0)
+    wx[0] = wx[-1] = 0.5
+    wy = np.full(NY, 1.0)
+    wy[0] = wy[-1] = 0.5
+    W = np.outer(wy, wx).ravel()  # j slow, i fast
+    ids = (np.arange(NX * NY) + NX * NY * nz + 1).astype(np.int32)
+    vals = np.zeros((NX * NY, 7))
+    vals[:, 2] = -pressure * hx * hy * W
+    return ids, vals
+
+
+def clamp_set_mpc(node_ids, dofs, set_name, first_eqn=1):
+    """One constraint equation per DOF naming a node set: the reference expands
+    it into one single-DOF row per set member
+    (Working/AStrO_bookKeeping.f90:585-601)."""
+    terms = []
+    for q, d in enumerate(dofs):
+        terms.append(MPCTerm(eqn=first_eqn + q, nodes=np.asarray(node_ids, np.int32), dof=d, coef=1.0,
+                             is_set=True, set_name=set_name))
+    return terms
+
+
+def cantilever_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, E=200e9, nu=0.3, pressure=1e6,
+                     e_field=None, name=None):
+    """Linear-static brick8 block clamped on z = 0 with a uniform top-face pressure
+    written as consistent nodal loads.  ``e_field`` optionally gives one Young's
+    modulus per element (same nu, G = E / (2 (1 + nu)))."""
+    nodes, conn, colors = brick_block(nx, ny, nz, lx, ly, lz)
+    E_el = np.full(conn.shape[0], float(E)) if e_field is None else np.asarray(e_field, np.float64)
+    eprops = np.stack([iso_props(e, nu) for e in E_el]) if e_field is not None else \
+        np.tile(iso_props(E, nu), (conn.shape[0], 1))
+    NX, NY = nx + 1, ny + 1
+    bottom = np.arange(1, NX * NY + 1, dtype=np.int32)
+    m = Model(name=name or f"brick{nx}x{ny}x{nz}", nodes=nodes, elements=conn,
+              etype=np.full(conn.shape[0], BRICK8, np.int32), eprops=eprops,
+              density=np.zeros(conn.shape[0]), cond=np.zeros((conn.shape[0], 6)), colors=colors)
+    m.mpc = clamp_set_mpc(bottom, [1, 2, 3], "bottom")
+    m.mpc_rhs = [0.0, 0.0, 0.0]
+    m.node_sets["bottom"] = bottom
+    ids, vals = face_pressure_loads(nx, ny, nz, lx, ly, lz, pressure)
+    m.load_nodes, m.load_vals = ids, vals
+    return m
+
+
+def thermal_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, k=50.0, t_top=100.0, t_bottom=0.0, name=None):
+    """Steady conduction on a brick8 block: temperature fixed on the top and
+    bottom faces through dof-7 constraint equations on node sets."""
+    nodes, conn, colors = brick_block(nx, ny, nz, lx, ly, lz)
+    E = conn.shape[0]
+    cond = np.tile(np.array([k, k, k, 0.0, 0.0, 0.0]), (E, 1))
+    NX, NY = nx + 1, ny + 1
+    bottom = np.arange(1, NX * NY + 1, dtype=np.int32)
+    top = bottom + NX * NY * nz
+    m = Model(name=name or f"thermal{nx}x{ny}x{nz}", nodes=nodes, elements=conn,
+              etype=np.full(E, BRICK8, np.int32), eprops=np.tile(iso_props(200e9, 0.3), (E, 1)),
+              density=np.zeros(E), cond=cond, colors=colors)
+    m.mpc = [MPCTerm(eqn=1, nodes=top, dof=7, coef=1.0, is_set=True, set_name="top"),
+             MPCTerm(eqn=2, nodes=bottom, dof=7, coef=1.0, is_set=True, set_name="bottom")]
+    m.mpc_rhs = [t_top, t_bottom]
+    m.node_sets["top"] = top
+    m.node_sets["bottom"] = bottom
+    return m
+
+
+def uniform_e_field(num_els, lo, hi, seed):
+    """Young's modulus per element drawn i.i.d. uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, size=num_els)

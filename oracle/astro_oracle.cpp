@@ -1,0 +1,1293 @@
+// astro_oracle.cpp -- CPU FP64 restatement of the AStrO finite-element hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
+// compared against; it is never linked into, loaded by or called from the
+// product path (misc_amd/).  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may use it.
+//
+// Every routine restates one reference routine (Fortran, andersone35/Misc,
+// directory Working/) and cites it as file:line.  Index conventions follow the
+// reference exactly: node ids, DOF numbers and CSR/profile column numbers are
+// 1-based values, range arrays are (0:n) prefix offsets stored as n+1 ints, and
+// a CSR column value of 0 marks an unused slot.  Arrays passed in from the
+// caller are plain C pointers, column-major where the reference is.
+//
+// Pinning of each part (what checks it besides being a restatement) is listed
+// in tests/test_oracle_*.py; the literal reference compiled from source
+// (oracle/_ref, built by oracle/Makefile) supplies golden vectors for the
+// defect-free linear-static brick path.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <complex>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Literal constants.  Working/AStrO_constantVals.f90:6-17 and the single
+// precision literals of Working/AStrO_solvers.f90:184 (1e-12) and :336
+// (1.570796326794897, read as a default-kind REAL).
+const double kOneRt3 = 1.0 / std::sqrt(3.0);
+const double kGmresTol = (double)1e-12f;
+const double kHalfPiSingle = (double)1.570796326794897f;
+
+// Element profile (integration points, weights, DOF table) for the element
+// types this oracle evaluates.  Working/AStrO_r_elementEqns.f90:334-449.
+struct ElProfile {
+    int numNds = 0, dofPerNd = 0, numIntDof = 0, numIntPts = 0;
+    int dofTable[2][33];  // [0]=variable (1-based), [1]=local node (1-based)
+    double intPts[8][3];
+    double ipWt[8];
+};
+
+void element_profile(int eType, ElProfile& p) {
+    p = ElProfile{};
+    if (eType == 8) {
+        p.numNds = 8; p.dofPerNd = 3; p.numIntDof = 0; p.numIntPts = 8;
+        // GP order: s1 fastest, s3 slowest; :363-376
+        int k = 0;
+        for (int c = 0; c < 2; ++c)
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a) {
+                    p.intPts[k][0] = kOneRt3 * (a ? 1.0 : -1.0);
+                    p.intPts[k][1] = kOneRt3 * (b ? 1.0 : -1.0);
+                    p.intPts[k][2] = kOneRt3 * (c ? 1.0 : -1.0);
+                    p.ipWt[k] = 1.0;
+                    ++k;
+                }
+    } else {
+        std::fprintf(stderr, "oracle: element type %d not supported\n", eType);
+        std::abort();
+    }
+    // component-major DOF order: :440-447
+    int i3 = 0;
+    for (int v = 1; v <= p.dofPerNd; ++v)
+        for (int n = 1; n <= p.numNds; ++n) {
+            p.dofTable[0][i3] = v;
+            p.dofTable[1][i3] = n;
+            ++i3;
+        }
+}
+
+// Trilinear brick shape functions and natural derivatives.
+// Working/AStrO_r_elementEqns.f90:27-36 (N) and :110-135 (dN/ds).
+void brick_shape(const double s[3], double N[8], double Ns[8][3]) {
+    static const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                 {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    for (int n = 0; n < 8; ++n) {
+        double f0 = 1.0 + sg[n][0] * s[0];
+        double f1 = 1.0 + sg[n][1] * s[1];
+        double f2 = 1.0 + sg[n][2] * s[2];
+        N[n] = 0.125 * f0 * f1 * f2;
+        Ns[n][0] = 0.125 * sg[n][0] * f1 * f2;
+        Ns[n][1] = 0.125 * sg[n][1] * f0 * f2;
+        Ns[n][2] = 0.125 * sg[n][2] * f0 * f1;
+    }
+}
+
+// Determinant and inverse by adjugate.  Working/AStrO_r_elementEqns.f90:212-238.
+// A, I are 3x3 column-major (A[r + 3*c]).
+double det_inv3(const double A[9], double I[9]) {
+    auto a = [&](int r, int c) { return A[(r - 1) + 3 * (c - 1)]; };
+    double adj11 = a(2, 2) * a(3, 3) - a(2, 3) * a(3, 2);
+    double adj12 = a(2, 3) * a(3, 1) - a(2, 1) * a(3, 3);
+    double adj13 = a(2, 1) * a(3, 2) - a(2, 2) * a(3, 1);
+    double D = a(1, 1) * adj11 + a(1, 2) * adj12 + a(1, 3) * adj13;
+    double dinv = 1.0 / D;
+    auto I_ = [&](int r, int c) -> double& { return I[(r - 1) + 3 * (c - 1)]; };
+    I_(1, 1) = dinv * adj11;
+    I_(2, 1) = dinv * adj12;
+    I_(3, 1) = dinv * adj13;
+    I_(1, 2) = dinv * (a(1, 3) * a(3, 2) - a(1, 2) * a(3, 3));
+    I_(2, 2) = dinv * (a(1, 1) * a(3, 3) - a(1, 3) * a(3, 1));
+    I_(3, 2) = dinv * (a(1, 2) * a(3, 1) - a(1, 1) * a(3, 2));
+    I_(1, 3) = dinv * (a(1, 2) * a(2, 3) - a(1, 3) * a(2, 2));
+    I_(2, 3) = dinv * (a(1, 3) * a(2, 1) - a(1, 1) * a(2, 3));
+    I_(3, 3) = dinv * (a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1));
+    return D;
+}
+
+// Integration point data for a brick: Jacobian dx/ds = locNds * Ns, its
+// determinant and Nx = Ns * ds/dx.  Working/AStrO_r_elementEqns.f90:580-620.
+// xyz is 3x8 column-major (node coordinates in the section frame).
+double brick_ipdata(const double* xyz, const double s[3], double N[8], double Nx[8][3]) {
+    double Ns[8][3];
+    brick_shape(s, N, Ns);
+    double J[9] = {0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0;
+            for (int n = 0; n < 8; ++n) acc += xyz[i + 3 * n] * Ns[n][j];
+            J[i + 3 * j] = acc;
+        }
+    double Ji[9];
+    double detJ = det_inv3(J, Ji);
+    for (int n = 0; n < 8; ++n)
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0;
+            for (int k = 0; k < 3; ++k) acc += Ns[n][k] * Ji[k + 3 * j];
+            Nx[n][j] = acc;
+        }
+    return detJ;
+}
+
+// Doolittle LU without pivoting on rows/cols [st, en] (0-based, inclusive) of
+// a column-major n x n matrix.  Working/AStrO_r_designPropertyFunctions.f90:1015-1039.
+void lu_factor(double* A, int n, int st, int en) {
+    auto a = [&](int r, int c) -> double& { return A[r + (size_t)n * c]; };
+    for (int i = st; i <= en; ++i) {
+        for (int j = i; j <= en; ++j)
+            for (int k = st; k < i; ++k) a(i, j) -= a(i, k) * a(k, j);
+        double f = 1.0 / a(i, i);
+        for (int j = i + 1; j <= en; ++j) {
+            for (int k = st; k < i; ++k) a(j, i) -= a(j, k) * a(k, i);
+            a(j, i) = f * a(j, i);
+        }
+    }
+}
+
+// Forward/back substitution with the factor of lu_factor.
+// Working/AStrO_r_designPropertyFunctions.f90:1042-1073.
+void lu_solve(double* x, const double* LU, const double* b, int n, int st, int en) {
+    std::vector<double> z(n, 0.0);
+    auto lu = [&](int r, int c) { return LU[r + (size_t)n * c]; };
+    z[st] = b[st];
+    for (int i = st + 1; i <= en; ++i) {
+        z[i] = b[i];
+        for (int k = st; k < i; ++k) z[i] -= lu(i, k) * z[k];
+    }
+    x[en] = z[en] / lu(en, en);
+    for (int i = en - 1; i >= st; --i) {
+        double f = 1.0 / lu(i, i);
+        double dp = 0;
+        for (int k = i + 1; k <= en; ++k) dp += lu(i, k) * x[k];
+        x[i] = f * (z[i] - dp);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// a4: material stiffness C = S^-1 from 9 engineering constants
+// (E1,E2,E3,nu12,nu13,nu23,G12,G13,G23) by LU inverse without pivoting.
+// Working/AStrO_r_designPropertyFunctions.f90:8-80 (compliance branch :43-77)
+// and r_GetInvLU :996-1013.  C is 6x6 column-major, Voigt order
+// (11,22,33,12,13,23) with engineering shears.
+void orc_material_stiffness(const double* ep, double* C) {
+    double S[36] = {0};
+    auto s = [&](int r, int c) -> double& { return S[(r - 1) + 6 * (c - 1)]; };
+    s(1, 1) = 1.0 / ep[0];
+    s(1, 2) = -ep[3] / ep[0];
+    s(1, 3) = -ep[4] / ep[0];
+    s(2, 2) = 1.0 / ep[1];
+    s(2, 3) = -ep[5] / ep[1];
+    s(3, 3) = 1.0 / ep[2];
+    s(4, 4) = 1.0 / ep[6];
+    s(5, 5) = 1.0 / ep[7];
+    s(6, 6) = 1.0 / ep[8];
+    for (int i = 2; i <= 6; ++i)
+        for (int j = 1; j < i; ++j) s(i, j) = s(j, i);
+    lu_factor(S, 6, 0, 5);
+    double f[6];
+    for (int c = 0; c < 6; ++c) {
+        for (int k = 0; k < 6; ++k) f[k] = 0.0;
+        f[c] = 1.0;
+        lu_solve(C + 6 * c, S, f, 6, 0, 5);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a10 (solid branch at u = 0): brick8 tangent dR/dU = sum_gp detJ*w * B^T C B.
+// Working/AStrO_r_elementEqns.f90:2424-2475 (B columns from r_getNLStrain
+// :649-662 with orient = identity) and :2512-2529 (contraction).  The
+// geometric-stress term (:2466-2473) is zero at u = 0, T = 0 and is omitted.
+// xyz: 3x8 column-major section-frame coordinates.  Ke: 24x24 column-major,
+// DOF order component-major [u1..u8, v1..v8, w1..w8].
+void orc_brick8_stiffness(const double* xyz, const double* C, double* Ke) {
+    ElProfile p;
+    element_profile(8, p);
+    std::memset(Ke, 0, sizeof(double) * 576);
+    for (int g = 0; g < 8; ++g) {
+        double N[8], Nx[8][3];
+        double detJ = brick_ipdata(xyz, p.intPts[g], N, Nx);
+        double B[6][24];
+        for (int d = 0; d < 24; ++d) {
+            int var = p.dofTable[0][d], nd = p.dofTable[1][d] - 1;
+            double o[3] = {var == 1 ? 1.0 : 0.0, var == 2 ? 1.0 : 0.0, var == 3 ? 1.0 : 0.0};
+            B[0][d] = o[0] * Nx[nd][0];
+            B[1][d] = o[1] * Nx[nd][1];
+            B[2][d] = o[2] * Nx[nd][2];
+            B[3][d] = o[0] * Nx[nd][1] + o[1] * Nx[nd][0];
+            B[4][d] = o[0] * Nx[nd][2] + o[2] * Nx[nd][0];
+            B[5][d] = o[1] * Nx[nd][2] + o[2] * Nx[nd][1];
+        }
+        double wd = detJ * p.ipWt[g];
+        double CB[6][24];
+        for (int r = 0; r < 6; ++r)
+            for (int d = 0; d < 24; ++d) {
+                double acc = 0;
+                for (int k = 0; k < 6; ++k) acc += C[r + 6 * k] * B[k][d];
+                CB[r][d] = wd * acc;
+            }
+        for (int j = 0; j < 24; ++j)
+            for (int i = 0; i < 24; ++i) {
+                double acc = 0;
+                for (int k = 0; k < 6; ++k) acc += B[k][i] * CB[k][j];
+                Ke[i + 24 * j] += acc;
+            }
+    }
+}
+
+// a11 (solid branch): consistent mass rho * sum_gp detJ*w * N^T N per
+// component.  Working/AStrO_r_elementEqns.f90:2628-2653.  Me: 24x24.
+void orc_brick8_mass(const double* xyz, double rho, double* Me) {
+    ElProfile p;
+    element_profile(8, p);
+    std::memset(Me, 0, sizeof(double) * 576);
+    for (int g = 0; g < 8; ++g) {
+        double N[8], Nx[8][3];
+        double detJ = brick_ipdata(xyz, p.intPts[g], N, Nx);
+        double f = rho * detJ * p.ipWt[g];
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 8; ++a)
+                for (int b = 0; b < 8; ++b) Me[(c * 8 + a) + 24 * (c * 8 + b)] += f * N[a] * N[b];
+    }
+}
+
+// a13: brick8 conductivity sum_gp detJ*w * Nx k Nx^T.
+// Working/AStrO_r_elementEqns.f90:2062-2124.  k: 3x3 column-major; Kt: 8x8.
+void orc_brick8_conductivity(const double* xyz, const double* k, double* Kt) {
+    ElProfile p;
+    element_profile(8, p);
+    std::memset(Kt, 0, sizeof(double) * 64);
+    for (int g = 0; g < 8; ++g) {
+        double N[8], Nx[8][3];
+        double detJ = brick_ipdata(xyz, p.intPts[g], N, Nx);
+        double wd = detJ * p.ipWt[g];
+        double kNx[8][3];
+        for (int a = 0; a < 8; ++a)
+            for (int j = 0; j < 3; ++j) {
+                double acc = 0;
+                for (int m = 0; m < 3; ++m) acc += Nx[a][m] * k[m + 3 * j];
+                kNx[a][j] = wd * acc;
+            }
+        for (int b = 0; b < 8; ++b)
+            for (int a = 0; a < 8; ++a) {
+                double acc = 0;
+                for (int m = 0; m < 3; ++m) acc += Nx[a][m] * kNx[b][m];
+                Kt[a + 8 * b] += acc;
+            }
+    }
+}
+
+// Volume sum_gp detJ*w of one brick (used by the closed-form checks).
+double orc_brick8_volume(const double* xyz) {
+    ElProfile p;
+    element_profile(8, p);
+    double v = 0;
+    for (int g = 0; g < 8; ++g) {
+        double N[8], Nx[8][3];
+        v += brick_ipdata(xyz, p.intPts[g], N, Nx) * p.ipWt[g];
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// a20: profile from CSR + penalty.  Working/AStrO_bookKeeping.f90:754-792.
+// lTri is zeroed first (:764).
+void orc_convert_ltri(double* lTri, const int* lTRange, long long lTSize, const double* mainMat,
+                      const int* mainCols, const int* mainRange, int mainDim, const double* cMat,
+                      const int* cCols, const int* cRange, int cDim) {
+    std::memset(lTri, 0, sizeof(double) * (size_t)lTSize);
+    for (int i1 = 1; i1 <= mainDim; ++i1) {
+        long long minCol = (long long)i1 - lTRange[i1] + lTRange[i1 - 1] + 1;
+        for (int i2 = mainRange[i1 - 1] + 1; i2 <= mainRange[i1]; ++i2) {
+            int i3 = mainCols[i2 - 1];
+            if (i3 >= minCol && i3 <= i1) {
+                long long i4 = (long long)lTRange[i1] - i1 + i3;
+                lTri[i4 - 1] += mainMat[i2 - 1];
+            }
+        }
+    }
+    for (int i1 = 1; i1 <= cDim; ++i1)
+        for (int i2 = cRange[i1 - 1] + 1; i2 <= cRange[i1]; ++i2) {
+            int i4 = cCols[i2 - 1];
+            if (i4 == 0) continue;
+            long long minCol = (long long)i4 - lTRange[i4] + lTRange[i4 - 1] + 1;
+            for (int i3 = cRange[i1 - 1] + 1; i3 <= cRange[i1]; ++i3) {
+                int i5 = cCols[i3 - 1];
+                if (i5 >= minCol && i5 <= i4) {
+                    long long i6 = (long long)lTRange[i4] - i4 + i5;
+                    lTri[i6 - 1] += cMat[i2 - 1] * cMat[i3 - 1];
+                }
+            }
+        }
+}
+
+// a21: in-place row-skyline LDL^T.  Working/AStrO_solvers.f90:10-78.
+// ARange is the (0:n) prefix array; row i holds columns i-len+1..i with the
+// diagonal last.  After the call the strict lower part holds L (unit
+// diagonal implicit) and the diagonal holds D.  Returns 0, or the 1-based row
+// of the first zero pivot (the reference divides by it regardless).
+int orc_ldl_factor(double* A, long long ASize, const int* ARange, int n) {
+    (void)ASize;
+    std::vector<long long> rowSt(n + 1);
+    int maxBW = 0;
+    for (int i = 1; i <= n; ++i) {
+        int len = ARange[i] - ARange[i - 1];
+        rowSt[i] = (long long)i - len + 1;
+        if (len > maxBW) maxBW = len;
+    }
+    std::vector<double> diag(n + 1, 0.0), dL(maxBW + 1, 0.0);
+    int bad = 0;
+    for (int i1 = 1; i1 <= n; ++i1) {
+        long long i3 = ARange[i1];          // diagonal (1-based)
+        long long i2 = ARange[i1 - 1] + 1;  // first entry of row
+        // dL(maxBW - k) = D(col) * L(i1, col), right-aligned: :42-48
+        int i5 = maxBW;
+        long long col = i1 - 1;
+        for (long long i4 = i3 - 1; i4 >= i2; --i4) {
+            dL[i5] = diag[col] * A[i4 - 1];
+            --i5;
+            --col;
+        }
+        // pivot: :50-57
+        long long c = i1 - i3 + i2;
+        double d = A[i3 - 1];
+        for (long long i4 = i2; i4 <= i3 - 1; ++i4) {
+            d -= diag[c] * A[i4 - 1] * A[i4 - 1];
+            ++c;
+        }
+        diag[i1] = d;
+        A[i3 - 1] = d;
+        if (d == 0.0 && !bad) bad = i1;
+        double fact = 1.0 / d;
+        // column i1 of L below the pivot: :58-73
+        int imax = std::min(n, i1 + maxBW);
+        for (int r = i1 + 1; r <= imax; ++r) {
+            long long stCol = std::max(rowSt[i1], rowSt[r]);
+            long long rowLen = i1 - stCol;
+            if (rowLen >= 0) {
+                long long j3 = maxBW - rowLen + 1;
+                long long j4 = ARange[r] - (r - stCol);
+                long long j5 = j4 + rowLen;
+                double v = A[j5 - 1];
+                for (long long k = 0; k <= rowLen - 1; ++k) v -= dL[j3 + k] * A[j4 + k - 1];
+                A[j5 - 1] = v * fact;
+            }
+        }
+    }
+    return bad;
+}
+
+// a22: z = L^-1 b (row dots), then x = D^-1 z - L^T x by probing later rows.
+// Working/AStrO_solvers.f90:83-126.
+void orc_ldl_solve(double* soln, const double* rhs, const double* L, const int* LRange, int n,
+                   double* z) {
+    int maxBW = 0;
+    for (int i1 = 1; i1 <= n; ++i1) {
+        long long i2 = LRange[i1 - 1] + 1;
+        long long rowLen = LRange[i1] - i2;
+        if (rowLen > maxBW) maxBW = (int)rowLen;
+        long long stCol = i1 - rowLen;
+        double v = rhs[i1 - 1];
+        for (long long k = 0; k <= rowLen - 1; ++k) v -= L[i2 + k - 1] * z[stCol + k - 1];
+        z[i1 - 1] = v;
+    }
+    for (int i1 = n; i1 >= 1; --i1) {
+        double dt = 0;
+        double d = L[LRange[i1] - 1];
+        int imax = std::min(n, i1 + maxBW);
+        for (int i2 = i1 + 1; i2 <= imax; ++i2) {
+            long long rowLen = LRange[i2] - LRange[i2 - 1];
+            long long stCol = i2 - rowLen + 1;
+            if (stCol <= i1) {
+                long long i3 = LRange[i2] + i1 - i2;
+                dt += L[i3 - 1] * soln[i2 - 1];
+            }
+        }
+        dt *= d;
+        soln[i1 - 1] = (z[i1 - 1] - dt) / d;
+    }
+}
+
+// Givens QR of the (m+1) x m Hessenberg matrix with angles stored in the
+// sub-diagonal (triDiag = 2 branch).  Working/AStrO_solvers.f90:299-353.
+static void r_factor_hessenberg(double* H, int rowDim, int colDim) {
+    auto h = [&](int r, int c) -> double& { return H[(r - 1) + (size_t)rowDim * (c - 1)]; };
+    for (int i1 = 1; i1 <= colDim; ++i1) {
+        int maxRow = std::min(i1 + 1, rowDim);
+        for (int i2 = i1 + 1; i2 <= maxRow; ++i2) {
+            if (std::fabs(h(i2, i1)) > 1e-10) {
+                double theta = (h(i1, i1) == 0.0) ? kHalfPiSingle : std::atan(h(i2, i1) / h(i1, i1));
+                double st = std::sin(theta), ct = std::cos(theta);
+                for (int i3 = i1; i3 <= colDim; ++i3) {
+                    double a1 = ct * h(i1, i3) + st * h(i2, i3);
+                    double a2 = -st * h(i1, i3) + ct * h(i2, i3);
+                    h(i1, i3) = a1;
+                    h(i2, i3) = a2;
+                }
+                h(i2, i1) = theta;
+            }
+        }
+    }
+}
+
+// Apply the stored rotations to b, then back-substitute R.
+// Working/AStrO_solvers.f90:355-404 (triDiag = 2).
+static void r_solve_hessenberg(double* soln, const double* H, double* b, int rowDim, int colDim) {
+    auto h = [&](int r, int c) { return H[(r - 1) + (size_t)rowDim * (c - 1)]; };
+    for (int i1 = 1; i1 <= colDim; ++i1) {
+        int maxRow = std::min(i1 + 1, rowDim);
+        for (int i2 = i1 + 1; i2 <= maxRow; ++i2) {
+            if (std::fabs(h(i2, i1)) > 1e-10) {
+                double st = std::sin(h(i2, i1)), ct = std::cos(h(i2, i1));
+                double r1 = ct * b[i1 - 1] + st * b[i2 - 1];
+                double r2 = -st * b[i1 - 1] + ct * b[i2 - 1];
+                b[i1 - 1] = r1;
+                b[i2 - 1] = r2;
+            }
+        }
+    }
+    for (int i1 = colDim; i1 >= 1; --i1) {
+        double v = b[i1 - 1];
+        for (int i2 = i1 + 1; i2 <= colDim; ++i2) v -= h(i1, i2) * soln[i2 - 1];
+        soln[i1 - 1] = v / h(i1, i1);
+    }
+}
+
+// y = A x + C^T (C x) with the reference's 0-means-empty CSR convention.
+// Working/AStrO_solvers.f90:143-171 (same loops at :189-215, :253-281).
+static void apply_operator(double* y, const double* x, const double* Amat, const int* Acols,
+                           const int* ARange, int n, const double* Cmat, const int* Ccols,
+                           const int* CRange, int cDim, std::vector<double>& t1) {
+    t1.assign(std::max(cDim, 1), 0.0);
+    for (int i1 = 1; i1 <= cDim; ++i1)
+        for (int i2 = CRange[i1 - 1] + 1; i2 <= CRange[i1]; ++i2) {
+            int c = Ccols[i2 - 1];
+            if (c != 0) t1[i1 - 1] += Cmat[i2 - 1] * x[c - 1];
+        }
+    for (int i = 0; i < n; ++i) y[i] = 0.0;
+    for (int i1 = 1; i1 <= cDim; ++i1)
+        for (int i2 = CRange[i1 - 1] + 1; i2 <= CRange[i1]; ++i2) {
+            int c = Ccols[i2 - 1];
+            if (c != 0) y[c - 1] += Cmat[i2 - 1] * t1[i1 - 1];
+        }
+    for (int i1 = 1; i1 <= n; ++i1) {
+        double acc = y[i1 - 1];
+        for (int i2 = ARange[i1 - 1] + 1; i2 <= ARange[i1]; ++i2) {
+            int c = Acols[i2 - 1];
+            if (c != 0) acc += Amat[i2 - 1] * x[c - 1];
+        }
+        y[i1 - 1] = acc;
+    }
+}
+
+// a23: left-preconditioned restarted GMRES(m) with modified Gram-Schmidt.
+// Working/AStrO_solvers.f90:128-297.  xVec is the initial guess on entry and
+// the solution on exit.  Returns the number of restart cycles run; *finalRes
+// receives the last preconditioned residual norm |P^-1 (b - A x)|, *stepsOut
+// the number of Arnoldi steps.
+//
+// mode 0 (literal): every cycle runs all m Arnoldi steps, the least-squares
+//   right-hand side is V^T r (:238-243), the stop test is the absolute
+//   |P^-1 r| > 1e-12 (single-precision literal, :184), at most maxIt/m + 1
+//   cycles (:181).
+// mode 1 (intended): the same operator, preconditioner, MGS and Givens
+//   rotations, but the least-squares right-hand side is beta*e1 and a cycle ends
+//   as soon as the rotated residual estimate drops below rtol * |P^-1 r0|; the
+//   solve stops when the true preconditioned residual does.  Once a literal
+//   cycle converges before step m its Arnoldi basis loses orthogonality and the
+//   V^T r projection turns the update into rounding noise; mode 1 is the
+//   numerically stable form of the same method.
+int orc_gmres_mode(double* xVec, const double* Amat, const int* Acols, const int* ARange, int n,
+                   const double* Cmat, const int* Ccols, const int* CRange, int cDim, const double* Pmat,
+                   const int* PRange, const double* bVec, int numVecs, int maxIt, int mode, double rtol,
+                   double* finalRes, int* stepsOut) {
+    int m = numVecs;
+    std::vector<double> z(n), r(n), t1, t2(n), tmp(n);
+    std::vector<double> V((size_t)n * (m + 1));
+    std::vector<double> H((size_t)(m + 1) * m);
+    auto hh = [&](int rr, int cc) -> double& { return H[(rr - 1) + (size_t)(m + 1) * (cc - 1)]; };
+    apply_operator(t2.data(), xVec, Amat, Acols, ARange, n, Cmat, Ccols, CRange, cDim, t1);
+    for (int i = 0; i < n; ++i) tmp[i] = bVec[i] - t2[i];
+    orc_ldl_solve(r.data(), tmp.data(), Pmat, PRange, n, z.data());
+    double res = 0;
+    for (int i = 0; i < n; ++i) res += r[i] * r[i];
+    res = std::sqrt(res);
+    const double res0 = res;
+    const double stopAt = mode == 0 ? kGmresTol : rtol * res0;
+    int maxOutIt = maxIt / numVecs + 1;
+    int cycles = 0, steps = 0;
+    while (res > stopAt && cycles < maxOutIt) {
+        for (int i = 0; i < n; ++i) V[i] = (1.0 / res) * r[i];
+        std::fill(H.begin(), H.end(), 0.0);
+        // running Givens state for the mode-1 residual estimate
+        std::vector<double> cs(m + 1), sn(m + 1), gg(m + 1, 0.0);
+        gg[0] = res;
+        int k = 0;
+        for (int i5 = 2; i5 <= m + 1; ++i5) {
+            double* vprev = &V[(size_t)n * (i5 - 2)];
+            double* vnew = &V[(size_t)n * (i5 - 1)];
+            apply_operator(t2.data(), vprev, Amat, Acols, ARange, n, Cmat, Ccols, CRange, cDim, t1);
+            orc_ldl_solve(vnew, t2.data(), Pmat, PRange, n, z.data());
+            for (int i1 = 1; i1 <= i5 - 1; ++i1) {
+                const double* vi = &V[(size_t)n * (i1 - 1)];
+                double dp = 0;
+                for (int q = 0; q < n; ++q) dp += vi[q] * vnew[q];
+                for (int q = 0; q < n; ++q) vnew[q] -= dp * vi[q];
+                hh(i1, i5 - 1) = dp;
+            }
+            double dp = 0;
+            for (int q = 0; q < n; ++q) dp += vnew[q] * vnew[q];
+            dp = std::sqrt(dp);
+            for (int q = 0; q < n; ++q) vnew[q] = (1.0 / dp) * vnew[q];
+            hh(i5, i5 - 1) = dp;
+            ++steps;
+            k = i5 - 1;
+            if (mode == 1) {
+                // estimate only (H itself is rotated below exactly as the reference does)
+                int j = k;
+                double hcol_j = hh(j, j), hsub = hh(j + 1, j);
+                std::vector<double> col(j + 1);
+                for (int q = 1; q <= j + 1; ++q) col[q - 1] = hh(q, j);
+                for (int q = 1; q < j; ++q) {
+                    double a = cs[q] * col[q - 1] + sn[q] * col[q];
+                    double b = -sn[q] * col[q - 1] + cs[q] * col[q];
+                    col[q - 1] = a;
+                    col[q] = b;
+                }
+                (void)hcol_j;
+                (void)hsub;
+                double rr = std::hypot(col[j - 1], col[j]);
+                cs[j] = rr == 0.0 ? 1.0 : col[j - 1] / rr;
+                sn[j] = rr == 0.0 ? 0.0 : col[j] / rr;
+                gg[j] = -sn[j] * gg[j - 1];
+                gg[j - 1] = cs[j] * gg[j - 1];
+                if (std::fabs(gg[j]) <= stopAt) break;
+            }
+        }
+        int mk = (mode == 1) ? k : m;  // columns actually built
+        // QR of the leading (mk+1) x mk block, angles stored in the sub-diagonal
+        std::vector<double> Hk((size_t)(mk + 1) * mk);
+        for (int c = 1; c <= mk; ++c)
+            for (int rr = 1; rr <= mk + 1; ++rr) Hk[(rr - 1) + (size_t)(mk + 1) * (c - 1)] = hh(rr, c);
+        r_factor_hessenberg(Hk.data(), mk + 1, mk);
+        std::vector<double> g(mk + 1, 0.0), y(mk, 0.0);
+        if (mode == 0) {
+            for (int i2 = 1; i2 <= mk + 1; ++i2) {
+                const double* vi = &V[(size_t)n * (i2 - 1)];
+                double acc = 0;
+                for (int i1 = 0; i1 < n; ++i1) acc += vi[i1] * r[i1];
+                g[i2 - 1] = acc;
+            }
+        } else {
+            g[0] = res;
+        }
+        r_solve_hessenberg(y.data(), Hk.data(), g.data(), mk + 1, mk);
+        for (int i1 = 0; i1 < n; ++i1) {
+            double acc = xVec[i1];
+            for (int i2 = 0; i2 < mk; ++i2) acc += V[i1 + (size_t)n * i2] * y[i2];
+            xVec[i1] = acc;
+        }
+        apply_operator(t2.data(), xVec, Amat, Acols, ARange, n, Cmat, Ccols, CRange, cDim, t1);
+        for (int i = 0; i < n; ++i) tmp[i] = bVec[i] - t2[i];
+        orc_ldl_solve(r.data(), tmp.data(), Pmat, PRange, n, z.data());
+        res = 0;
+        for (int i = 0; i < n; ++i) res += r[i] * r[i];
+        res = std::sqrt(res);
+        ++cycles;
+    }
+    if (finalRes) *finalRes = res;
+    if (stepsOut) *stepsOut = steps;
+    return cycles;
+}
+
+int orc_gmres(double* xVec, const double* Amat, const int* Acols, const int* ARange, int n,
+              const double* Cmat, const int* Ccols, const int* CRange, int cDim, const double* Pmat,
+              const int* PRange, const double* bVec, int numVecs, int maxIt, double* finalRes,
+              int* stepsOut) {
+    return orc_gmres_mode(xVec, Amat, Acols, ARange, n, Cmat, Ccols, CRange, cDim, Pmat, PRange, bVec,
+                          numVecs, maxIt, 0, 0.0, finalRes, stepsOut);
+}
+
+// Plain CSR operator y = A x + C^T C x (exported for the SpMV parity tests).
+void orc_apply_operator(double* y, const double* x, const double* Amat, const int* Acols,
+                        const int* ARange, int n, const double* Cmat, const int* Ccols,
+                        const int* CRange, int cDim) {
+    std::vector<double> t1;
+    apply_operator(y, x, Amat, Acols, ARange, n, Cmat, Ccols, CRange, cDim, t1);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Model-level bookkeeping and assembly (a16, a17, a19, a24).
+// ===========================================================================
+
+struct OrcModel {
+    int numNodes = 0, numEls = 0;
+    std::vector<double> nodeList;   // 3*N
+    std::vector<int> elementList;   // 8*E (1-based node ids, 0 = unused)
+    std::vector<int> elementType;   // E
+    std::vector<double> elC;        // 36*E
+    std::vector<double> elDen;      // E
+    std::vector<double> elCond;     // 9*E
+    // constraints (Working/AStrO_input.f90 constraints block, after set expansion
+    // of the node names): term t has equation mpcEqn, dof, coef and either a
+    // plain node (isSet = 0, one node) or a node set (isSet = 1, node list)
+    std::vector<int> mpcEqn, mpcDof, mpcIsSet, mpcNodeStart, mpcNodes;
+    std::vector<double> mpcCoef, mpcRHS;
+    int numMPC = 0;
+    // nodal loads after set expansion: node id and 7 components
+    std::vector<int> loadNode;
+    std::vector<double> loadVal;
+    // bookkeeping
+    int solverMaxBW = 0, numSolidNds = 0, numSectionNds = 0;
+    std::vector<int> nCRange, nodalConn, currentRank, originalRank, nDofIndex;
+    int elMatDim = 0;
+    std::vector<int> elMatRange, elMatCols, elMatLTRange;
+    std::vector<double> elasticMat, elMatLT, elasticLoad, delDisp, nodeDisp;
+    std::vector<int> thermMatRange, thermMatCols, thermMatLTRange;
+    std::vector<double> thermMat, thermMatLT, thermalLoad, delTemp;
+    int elMPCDim = 0;
+    std::vector<int> elMPCMatRange, elMPCMatCols;
+    std::vector<double> elMPCMat, elMPCRHS;
+    int thermMPCDim = 0;
+    std::vector<int> thermMPCMatRange, thermMPCMatCols;
+    std::vector<double> thermMPCMat, thermMPCRHS;
+    int gmresCycles = 0, gmresSteps = 0;
+    double gmresRes = 0;
+    std::vector<double> timings;  // seconds per phase of the last static solve
+};
+
+namespace {
+
+int ndof_of_type(int eT) { return (eT == 41 || eT == 3 || eT == 2) ? 6 : 3; }
+
+// levelSortNodes: breadth-first ordering from the first node of minimum
+// connectivity, neighbours taken in nodalConn insertion order; when the queue
+// empties, restart at the nearest unvisited node of the same DOF type.
+// Working/AStrO_bookKeeping.f90:233-331.  (The block-split branch at :272-296
+// needs nextAvail - blockSt == solverBlockDim == numNodes and cannot fire.)
+void level_sort(OrcModel& m, int& nextAvail, int stopLength, int ndof, std::vector<int>& inserted) {
+    const int N = m.numNodes;
+    auto ndofOf = [&](int nd) { return m.nDofIndex[6 * (nd - 1)]; };
+    auto nearest = [&](int from) {
+        double best = 1e100;
+        int st = 0;
+        for (int i4 = 1; i4 <= N; ++i4) {
+            if (inserted[i4] == 0 && ndofOf(i4) == ndof) {
+                double v0 = m.nodeList[3 * (i4 - 1)] - m.nodeList[3 * (from - 1)];
+                double v1 = m.nodeList[3 * (i4 - 1) + 1] - m.nodeList[3 * (from - 1) + 1];
+                double v2 = m.nodeList[3 * (i4 - 1) + 2] - m.nodeList[3 * (from - 1) + 2];
+                double d = std::sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                if (d < best) {
+                    best = d;
+                    st = i4;
+                }
+            }
+        }
+        return st;
+    };
+    int minCon = 1000, stNd = 0;
+    for (int i1 = 1; i1 <= N; ++i1)
+        if (ndofOf(i1) == ndof) {
+            int c = m.nCRange[i1] - m.nCRange[i1 - 1];
+            if (c < minCon) {
+                stNd = i1;
+                minCon = c;
+            }
+        }
+    m.originalRank[nextAvail] = stNd;
+    int currentNd = stNd;
+    int currentInd = nextAvail;
+    ++nextAvail;
+    inserted[stNd] = 1;
+    while (nextAvail <= stopLength) {
+        for (int i1 = m.nCRange[currentNd - 1] + 1; i1 <= m.nCRange[currentNd]; ++i1) {
+            int i2 = m.nodalConn[i1 - 1];
+            if (i2 != 0 && inserted[i2] == 0) {
+                m.originalRank[nextAvail] = i2;
+                ++nextAvail;
+                inserted[i2] = 1;
+            }
+        }
+        ++currentInd;
+        if (currentInd == nextAvail && nextAvail <= stopLength) {
+            int from = m.originalRank[nextAvail - 1];
+            int st = nearest(from);
+            m.originalRank[nextAvail] = st;
+            currentInd = nextAvail;
+            inserted[st] = 1;
+            ++nextAvail;
+        }
+        currentNd = m.originalRank[currentInd];
+    }
+}
+
+// Linear-probe insert of (col, val) into CSR row `row` (1-based), adding to an
+// existing slot with the same column.  `strictLast` reproduces the thermal
+// scatter's `.lt.` bound that never fills a row's last slot
+// (Working/AStrO_commandFunctions.f90:57); the elastic scatter uses `.le.`
+// (:334).
+inline void probe_add(std::vector<double>& val, std::vector<int>& cols, const std::vector<int>& range,
+                      int row, int col, double v, bool strictLast) {
+    int end = strictLast ? range[row] - 1 : range[row];
+    for (int s = range[row - 1] + 1; s <= end; ++s) {
+        int c = cols[s - 1];
+        if (c == 0) {
+            val[s - 1] = v;
+            cols[s - 1] = col;
+            return;
+        }
+        if (c == col) {
+            val[s - 1] += v;
+            return;
+        }
+    }
+}
+
+// buildElasticMPC / buildThermalMPC share this expansion: equation blocks sized
+// by the largest node set among their terms, one row per set member.
+// Working/AStrO_bookKeeping.f90:474-612 (elastic) and :614-752 (thermal).
+void build_mpc(OrcModel& m, bool thermal, int& dim, std::vector<int>& range, std::vector<int>& cols,
+               std::vector<double>& mat, std::vector<double>& rhs) {
+    int T = (int)m.mpcEqn.size();
+    std::vector<int> nEqns(m.numMPC + 1, 0), nTrms(m.numMPC + 1, 0);
+    auto use = [&](int t) { return thermal ? m.mpcDof[t] == 7 : m.mpcDof[t] <= 6; };
+    for (int t = 0; t < T; ++t) {
+        if (!use(t)) continue;
+        int e = m.mpcEqn[t];
+        nTrms[e] += 1;
+        if (!m.mpcIsSet[t]) {
+            if (nEqns[e] < 1) nEqns[e] = 1;
+        } else {
+            int sz = m.mpcNodeStart[t + 1] - m.mpcNodeStart[t];
+            if (sz > nEqns[e]) nEqns[e] = sz;
+        }
+    }
+    dim = 0;
+    long long size = 0;
+    for (int e = 1; e <= m.numMPC; ++e) {
+        dim += nEqns[e];
+        size += (long long)nEqns[e] * nTrms[e];
+    }
+    mat.assign(size, 0.0);
+    cols.assign(size, 0);
+    rhs.assign(dim, 0.0);
+    range.assign(dim + 1, 0);
+    for (int e = 2; e <= m.numMPC; ++e) nEqns[e] += nEqns[e - 1];
+    for (int e = 1; e <= m.numMPC; ++e)
+        for (int r = nEqns[e - 1] + 1; r <= nEqns[e]; ++r) range[r] = nTrms[e];
+    for (int r = 2; r <= dim; ++r) range[r] += range[r - 1];
+    auto dofcol = [&](int node, int dof) {
+        return thermal ? m.currentRank[node - 1] : m.nDofIndex[6 * (node - 1) + dof - 1];
+    };
+    auto insert = [&](int row, int col, double coef) {
+        for (int s = range[row - 1] + 1; s <= range[row]; ++s) {
+            if (cols[s - 1] == 0) {
+                mat[s - 1] = coef;
+                cols[s - 1] = col;
+                return;
+            }
+            if (cols[s - 1] == col) return;
+        }
+    };
+    for (int t = 0; t < T; ++t) {
+        if (!use(t)) continue;
+        int e = m.mpcEqn[t];
+        int i3 = nEqns[e - 1] + 1;
+        int a = m.mpcNodeStart[t], b = m.mpcNodeStart[t + 1];
+        if (!m.mpcIsSet[t] || b - a == 1) {
+            int col = dofcol(m.mpcNodes[a], m.mpcDof[t]);
+            for (; i3 <= nEqns[e]; ++i3) {
+                insert(i3, col, m.mpcCoef[t]);
+                rhs[i3 - 1] = m.mpcRHS[e - 1];
+            }
+        } else {
+            for (int k = a; k < b; ++k) {
+                int col = dofcol(m.mpcNodes[k], m.mpcDof[t]);
+                insert(i3, col, m.mpcCoef[t]);
+                rhs[i3 - 1] = m.mpcRHS[e - 1];
+                ++i3;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+OrcModel* orc_create(int numNodes, const double* nodeList, int numEls, const int* elementList,
+                     const int* elementType) {
+    auto* m = new OrcModel;
+    m->numNodes = numNodes;
+    m->numEls = numEls;
+    m->nodeList.assign(nodeList, nodeList + 3 * (size_t)numNodes);
+    m->elementList.assign(elementList, elementList + 8 * (size_t)numEls);
+    m->elementType.assign(elementType, elementType + numEls);
+    m->elC.assign(36 * (size_t)numEls, 0.0);
+    m->elDen.assign(numEls, 0.0);
+    m->elCond.assign(9 * (size_t)numEls, 0.0);
+    return m;
+}
+
+void orc_free(OrcModel* m) { delete m; }
+
+// Per-element properties: eprops (9*E engineering constants), den (E),
+// cond6 (6*E: k11,k22,k33,k12,k13,k23).  Any pointer may be null.
+// Working/AStrO_r_designPropertyFunctions.f90:8-80, 83-127, 129-162.
+void orc_set_properties(OrcModel* m, const double* eprops, const double* den, const double* cond6) {
+    for (int e = 0; e < m->numEls; ++e) {
+        if (eprops) orc_material_stiffness(eprops + 9 * (size_t)e, &m->elC[36 * (size_t)e]);
+        if (den) m->elDen[e] = den[e];
+        if (cond6) {
+            const double* k = cond6 + 6 * (size_t)e;
+            double* K = &m->elCond[9 * (size_t)e];
+            K[0] = k[0]; K[4] = k[1]; K[8] = k[2];
+            K[3] = K[1] = k[3];
+            K[6] = K[2] = k[4];
+            K[7] = K[5] = k[5];
+        }
+    }
+}
+
+void orc_set_mpc(OrcModel* m, int nTerms, const int* eqn, const int* dof, const double* coef,
+                 const int* isSet, const int* nodeStart, const int* nodes, int numMPC,
+                 const double* rhs) {
+    m->mpcEqn.assign(eqn, eqn + nTerms);
+    m->mpcDof.assign(dof, dof + nTerms);
+    m->mpcCoef.assign(coef, coef + nTerms);
+    m->mpcIsSet.assign(isSet, isSet + nTerms);
+    m->mpcNodeStart.assign(nodeStart, nodeStart + nTerms + 1);
+    m->mpcNodes.assign(nodes, nodes + nodeStart[nTerms]);
+    m->numMPC = numMPC;
+    m->mpcRHS.assign(rhs, rhs + numMPC);
+}
+
+void orc_set_nodal_loads(OrcModel* m, int nLoads, const int* node, const double* val7) {
+    m->loadNode.assign(node, node + nLoads);
+    m->loadVal.assign(val7, val7 + 7 * (size_t)nLoads);
+}
+
+// solverMaxBW as the *solve command sets it: smallest b with b^3 >= 27*N*N,
+// the product formed in 32-bit two's-complement arithmetic as the reference's
+// default INTEGER does (it wraps above N = 8918 nodes, and a negative result
+// leaves b = 1).  Working/AStrO_runJob.f90:107-114.
+int orc_solver_max_bw(int numNodes) {
+    uint32_t p = (uint32_t)27u * (uint32_t)numNodes;
+    p = p * (uint32_t)numNodes;
+    int32_t nnd2 = (int32_t)p;
+    int bw = 1;
+    long long b3 = 1;
+    while (b3 < nnd2) {
+        ++bw;
+        b3 = (long long)bw * bw * bw;
+    }
+    return bw;
+}
+
+// analysisPrep: adjacency, level ordering, DOF numbering, CSR capacity and
+// profile lengths, MPC matrices.  Working/AStrO_bookKeeping.f90:794-1039,
+// allocateElasticMat :333-411, allocateThermalMat :413-472.
+// maxBW <= 0 selects the reference's own rule (orc_solver_max_bw).
+void orc_analysis_prep(OrcModel* mp, int maxBW) {
+    OrcModel& m = *mp;
+    const int N = m.numNodes, E = m.numEls;
+    m.solverMaxBW = maxBW > 0 ? maxBW : orc_solver_max_bw(N);
+    m.currentRank.assign(N, 0);
+    m.originalRank.assign(N + 2, 0);  // 1-based, one spare slot
+    m.nDofIndex.assign(6 * (size_t)N, 0);
+    m.nCRange.assign(N + 1, 0);
+    for (int e = 0; e < E; ++e) {
+        int nd = ndof_of_type(m.elementType[e]);
+        for (int a = 0; a < 8; ++a) {
+            int i3 = m.elementList[8 * (size_t)e + a];
+            if (i3 == 0) continue;
+            for (int b = 0; b < 8; ++b) {
+                int i5 = m.elementList[8 * (size_t)e + b];
+                if (i5 != 0 && i5 != i3) m.nCRange[i3] += 1;
+            }
+            m.nDofIndex[6 * (size_t)(i3 - 1)] = nd;
+        }
+    }
+    m.numSolidNds = 0;
+    for (int i = 1; i <= N; ++i)
+        if (m.nDofIndex[6 * (size_t)(i - 1)] == 3) ++m.numSolidNds;
+    m.numSectionNds = N - m.numSolidNds;
+    for (int i = 2; i <= N; ++i) m.nCRange[i] += m.nCRange[i - 1];
+    m.nodalConn.assign(m.nCRange[N], 0);
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < 8; ++a) {
+            int i3 = m.elementList[8 * (size_t)e + a];
+            if (i3 == 0) continue;
+            for (int b = 0; b < 8; ++b) {
+                int i5 = m.elementList[8 * (size_t)e + b];
+                if (i5 == 0 || i5 == i3) continue;
+                for (int s = m.nCRange[i3 - 1] + 1; s <= m.nCRange[i3]; ++s) {
+                    if (m.nodalConn[s - 1] == 0) {
+                        m.nodalConn[s - 1] = i5;
+                        break;
+                    }
+                    if (m.nodalConn[s - 1] == i5) break;
+                }
+            }
+        }
+    std::vector<int> inserted(N + 1, 0);
+    int nextAvail = 1;
+    if (m.numSolidNds > 0) level_sort(m, nextAvail, m.numSolidNds, 3, inserted);
+    if (nextAvail <= N) level_sort(m, nextAvail, N, 6, inserted);
+    for (int i = 1; i <= N; ++i) m.currentRank[m.originalRank[i] - 1] = i;
+    for (int i = 1; i <= N; ++i) {
+        int* nd = &m.nDofIndex[6 * (size_t)(i - 1)];
+        int r = m.currentRank[i - 1];
+        if (nd[0] == 3) {
+            for (int c = 0; c < 3; ++c) nd[c] = r + c * m.numSolidNds;
+        } else if (nd[0] == 6) {
+            for (int c = 0; c < 6; ++c) nd[c] = r + c * m.numSectionNds + 2 * m.numSolidNds;
+        }
+    }
+    m.elMatDim = 3 * m.numSolidNds + 6 * m.numSectionNds;
+    const int n = m.elMatDim;
+    // allocateElasticMat
+    m.elMatRange.assign(n + 1, 0);
+    m.elMatLTRange.assign(n + 1, 0);
+    auto ndi = [&](int c, int node) { return m.nDofIndex[6 * (size_t)(node - 1) + c - 1]; };
+    for (int i1 = 1; i1 <= N; ++i1)
+        for (int i2 = 1; i2 <= 6; ++i2) {
+            int i3 = ndi(i2, i1);
+            if (i3 == 0) continue;
+            for (int i4 = 1; i4 <= 6; ++i4)
+                if (ndi(i4, i1) != 0) m.elMatRange[i3] += 1;
+            for (int s = m.nCRange[i1 - 1] + 1; s <= m.nCRange[i1]; ++s) {
+                int i5 = m.nodalConn[s - 1];
+                if (i5 == 0) continue;
+                for (int i6 = 1; i6 <= 6; ++i6)
+                    if (ndi(i6, i5) != 0) m.elMatRange[i3] += 1;
+            }
+        }
+    for (int i = 2; i <= n; ++i) m.elMatRange[i] += m.elMatRange[i - 1];
+    m.elasticMat.assign(m.elMatRange[n], 0.0);
+    m.elMatCols.assign(m.elMatRange[n], 0);
+    for (int i1 = 1; i1 <= N; ++i1)
+        for (int i2 = 1; i2 <= 6; ++i2) {
+            int i3 = ndi(i2, i1);
+            if (i3 == 0) continue;
+            m.elMatLTRange[i3] = 1;
+            for (int s = m.nCRange[i1 - 1] + 1; s <= m.nCRange[i1]; ++s) {
+                int i5 = m.nodalConn[s - 1];
+                if (i5 == 0) continue;
+                int i6 = m.currentRank[i1 - 1] - m.currentRank[i5 - 1];
+                if (i6 >= 0 && i6 <= m.solverMaxBW) {
+                    int i7 = ndi(i2, i5);
+                    if (i7 != 0 && i7 <= i3) {
+                        int i8 = i3 - i7 + 1;
+                        if (i8 > m.elMatLTRange[i3]) m.elMatLTRange[i3] = i8;
+                    }
+                }
+            }
+        }
+    for (int i = 2; i <= n; ++i) m.elMatLTRange[i] += m.elMatLTRange[i - 1];
+    // allocateThermalMat (indices by node rank)
+    m.thermMatRange.assign(N + 1, 0);
+    m.thermMatLTRange.assign(N + 1, 0);
+    for (int i1 = 1; i1 <= N; ++i1) {
+        int i3 = m.currentRank[i1 - 1];
+        m.thermMatRange[i3] += 1;
+        for (int s = m.nCRange[i1 - 1] + 1; s <= m.nCRange[i1]; ++s)
+            if (m.nodalConn[s - 1] != 0) m.thermMatRange[i3] += 1;
+    }
+    for (int i = 2; i <= N; ++i) m.thermMatRange[i] += m.thermMatRange[i - 1];
+    m.thermMat.assign(m.thermMatRange[N], 0.0);
+    m.thermMatCols.assign(m.thermMatRange[N], 0);
+    for (int i1 = 1; i1 <= N; ++i1) {
+        int i3 = m.currentRank[i1 - 1];
+        m.thermMatLTRange[i3] = 1;
+        for (int s = m.nCRange[i1 - 1] + 1; s <= m.nCRange[i1]; ++s) {
+            int i5 = m.nodalConn[s - 1];
+            if (i5 == 0) continue;
+            int i6 = m.currentRank[i1 - 1] - m.currentRank[i5 - 1];
+            if (i6 >= 0 && i6 <= m.solverMaxBW && i6 + 1 > m.thermMatLTRange[i3])
+                m.thermMatLTRange[i3] = i6 + 1;
+        }
+    }
+    for (int i = 2; i <= N; ++i) m.thermMatLTRange[i] += m.thermMatLTRange[i - 1];
+    build_mpc(m, false, m.elMPCDim, m.elMPCMatRange, m.elMPCMatCols, m.elMPCMat, m.elMPCRHS);
+    build_mpc(m, true, m.thermMPCDim, m.thermMPCMatRange, m.thermMPCMatCols, m.thermMPCMat,
+              m.thermMPCRHS);
+    m.elasticLoad.assign(n, 0.0);
+    m.delDisp.assign(n, 0.0);
+    m.nodeDisp.assign(n, 0.0);
+    m.thermalLoad.assign(N, 0.0);
+    m.delTemp.assign(N, 0.0);
+}
+
+// Element coordinates (3x8 column-major) of element e (0-based).
+// Working/AStrO_r_designPropertyFunctions.f90:945-968 with no DV offsets,
+// rotated by the identity section frame (solids with default orientation).
+void orc_element_coords(const OrcModel* m, int e, double* xyz) {
+    for (int a = 0; a < 8; ++a) {
+        int nd = m->elementList[8 * (size_t)e + a];
+        for (int c = 0; c < 3; ++c) xyz[c + 3 * a] = nd > 0 ? m->nodeList[3 * (size_t)(nd - 1) + c] : 1.0;
+    }
+}
+
+// a16: element loop + linear-probe CSR scatter (buildMat = 1 path; at u = 0
+// the internal force is zero, so the residual scatter adds nothing).
+// Working/AStrO_commandFunctions.f90:260-351.  `massFactor` adds
+// massFactor * M_e to each tangent (the dynamic branch of r_getElRu,
+// Working/AStrO_r_elementEqns.f90:2684-2696, c1*c2 = 1/(beta*dt^2)).
+void orc_assemble_elastic(OrcModel* mp, double massFactor) {
+    OrcModel& m = *mp;
+    std::fill(m.elasticMat.begin(), m.elasticMat.end(), 0.0);
+    std::vector<double> Ke(576), Me(576);
+    double xyz[24];
+    for (int e = 0; e < m.numEls; ++e) {
+        if (m.elementType[e] != 8) {
+            std::fprintf(stderr, "oracle: assemble supports brick8 only\n");
+            std::abort();
+        }
+        orc_element_coords(&m, e, xyz);
+        orc_brick8_stiffness(xyz, &m.elC[36 * (size_t)e], Ke.data());
+        if (massFactor != 0.0) {
+            orc_brick8_mass(xyz, m.elDen[e], Me.data());
+            for (int k = 0; k < 576; ++k) Ke[k] += massFactor * Me[k];
+        }
+        ElProfile p;
+        element_profile(8, p);
+        for (int i = 0; i < 24; ++i) {
+            int kGi = m.nDofIndex[6 * (size_t)(m.elementList[8 * (size_t)e + p.dofTable[1][i] - 1] - 1) +
+                                  p.dofTable[0][i] - 1];
+            for (int j = 0; j < 24; ++j) {
+                int kGj = m.nDofIndex[6 * (size_t)(m.elementList[8 * (size_t)e + p.dofTable[1][j] - 1] - 1) +
+                                      p.dofTable[0][j] - 1];
+                probe_add(m.elasticMat, m.elMatCols, m.elMatRange, kGi, kGj, Ke[i + 24 * j], false);
+            }
+        }
+    }
+}
+
+// a17: thermal element loop + scatter, indices by node rank.
+// Working/AStrO_commandFunctions.f90:16-74.  literalD8 = 1 keeps the
+// reference's `.lt.` probe bound that never fills the last slot of a row (:57).
+void orc_assemble_thermal(OrcModel* mp, int literalD8) {
+    OrcModel& m = *mp;
+    std::fill(m.thermMat.begin(), m.thermMat.end(), 0.0);
+    std::fill(m.thermMatCols.begin(), m.thermMatCols.end(), 0);
+    double xyz[24], Kt[64];
+    for (int e = 0; e < m.numEls; ++e) {
+        orc_element_coords(&m, e, xyz);
+        orc_brick8_conductivity(xyz, &m.elCond[9 * (size_t)e], Kt);
+        for (int a = 0; a < 8; ++a) {
+            int kGi = m.currentRank[m.elementList[8 * (size_t)e + a] - 1];
+            for (int b = 0; b < 8; ++b) {
+                int kGj = m.currentRank[m.elementList[8 * (size_t)e + b] - 1];
+                probe_add(m.thermMat, m.thermMatCols, m.thermMatRange, kGi, kGj, Kt[a + 8 * b],
+                          literalD8 != 0);
+            }
+        }
+    }
+}
+
+// a19: nodal loads (Working/AStrO_commandFunctions.f90:368-392), then the
+// constraint load C^T (rhs - C u) (:506-535).  Resets elasticLoad first, as
+// stepSolve does (:718-723), with the residual term zero at u = 0.
+void orc_elastic_rhs(OrcModel* mp) {
+    OrcModel& m = *mp;
+    std::fill(m.elasticLoad.begin(), m.elasticLoad.end(), 0.0);
+    for (size_t l = 0; l < m.loadNode.size(); ++l)
+        for (int c = 0; c < 6; ++c) {
+            int d = m.nDofIndex[6 * (size_t)(m.loadNode[l] - 1) + c];
+            if (d != 0) m.elasticLoad[d - 1] += m.loadVal[7 * l + c];
+        }
+    std::vector<double> t(m.elMPCDim);
+    for (int i = 1; i <= m.elMPCDim; ++i) {
+        double v = m.elMPCRHS[i - 1];
+        for (int s = m.elMPCMatRange[i - 1] + 1; s <= m.elMPCMatRange[i]; ++s) {
+            int c = m.elMPCMatCols[s - 1];
+            if (c != 0) v -= m.elMPCMat[s - 1] * m.nodeDisp[c - 1];
+        }
+        t[i - 1] = v;
+    }
+    for (int i = 1; i <= m.elMPCDim; ++i)
+        for (int s = m.elMPCMatRange[i - 1] + 1; s <= m.elMPCMatRange[i]; ++s) {
+            int c = m.elMPCMatCols[s - 1];
+            if (c != 0) m.elasticLoad[c - 1] += m.elMPCMat[s - 1] * t[i - 1];
+        }
+}
+
+// Thermal right-hand side: nodal heat loads (component 7) and the constraint
+// load; Working/AStrO_commandFunctions.f90:76-251 (nodal branch) and the
+// thermal twin of getElasticConstraintLoad.
+void orc_thermal_rhs(OrcModel* mp) {
+    OrcModel& m = *mp;
+    std::fill(m.thermalLoad.begin(), m.thermalLoad.end(), 0.0);
+    for (size_t l = 0; l < m.loadNode.size(); ++l) {
+        int r = m.currentRank[m.loadNode[l] - 1];
+        m.thermalLoad[r - 1] += m.loadVal[7 * l + 6];
+    }
+    std::vector<double> t(m.thermMPCDim);
+    for (int i = 1; i <= m.thermMPCDim; ++i) t[i - 1] = m.thermMPCRHS[i - 1];  // T = 0 initially
+    for (int i = 1; i <= m.thermMPCDim; ++i)
+        for (int s = m.thermMPCMatRange[i - 1] + 1; s <= m.thermMPCMatRange[i]; ++s) {
+            int c = m.thermMPCMatCols[s - 1];
+            if (c != 0) m.thermalLoad[c - 1] += m.thermMPCMat[s - 1] * t[i - 1];
+        }
+}
+
+// scaleElasticMPC / scaleThermalMPC: normalise every constraint row, then
+// scale rows and rhs by 100 * max|K_ij|.  Working/AStrO_commandFunctions.f90:537-566
+// (thermal twin :228-258).
+static void scale_mpc(const std::vector<double>& K, std::vector<double>& C, const std::vector<int>& range,
+                      std::vector<double>& rhs, int dim) {
+    double maxMat = 0;
+    for (double v : K) maxMat = std::max(maxMat, std::fabs(v));
+    for (int i = 1; i <= dim; ++i) {
+        double mag = 0;
+        for (int s = range[i - 1] + 1; s <= range[i]; ++s) mag += C[s - 1] * C[s - 1];
+        mag = std::sqrt(mag);
+        for (int s = range[i - 1] + 1; s <= range[i]; ++s) C[s - 1] = (1.0 / mag) * C[s - 1];
+        rhs[i - 1] = (1.0 / mag) * rhs[i - 1];
+    }
+    for (double& v : C) v = 100.0 * maxMat * v;
+    for (double& v : rhs) v = 100.0 * maxMat * v;
+}
+
+void orc_scale_elastic_mpc(OrcModel* m) {
+    scale_mpc(m->elasticMat, m->elMPCMat, m->elMPCMatRange, m->elMPCRHS, m->elMPCDim);
+}
+void orc_scale_thermal_mpc(OrcModel* m) {
+    scale_mpc(m->thermMat, m->thermMPCMat, m->thermMPCMatRange, m->thermMPCRHS, m->thermMPCDim);
+}
+
+void orc_build_elastic_profile(OrcModel* mp) {
+    OrcModel& m = *mp;
+    m.elMatLT.assign(m.elMatLTRange[m.elMatDim], 0.0);
+    orc_convert_ltri(m.elMatLT.data(), m.elMatLTRange.data(), (long long)m.elMatLT.size(), m.elasticMat.data(),
+                     m.elMatCols.data(), m.elMatRange.data(), m.elMatDim, m.elMPCMat.data(),
+                     m.elMPCMatCols.data(), m.elMPCMatRange.data(), m.elMPCDim);
+}
+void orc_build_thermal_profile(OrcModel* mp) {
+    OrcModel& m = *mp;
+    m.thermMatLT.assign(m.thermMatLTRange[m.numNodes], 0.0);
+    orc_convert_ltri(m.thermMatLT.data(), m.thermMatLTRange.data(), (long long)m.thermMatLT.size(),
+                     m.thermMat.data(), m.thermMatCols.data(), m.thermMatRange.data(), m.numNodes,
+                     m.thermMPCMat.data(), m.thermMPCMatCols.data(), m.thermMPCMatRange.data(),
+                     m.thermMPCDim);
+}
+
+int orc_factor_elastic(OrcModel* m) {
+    return orc_ldl_factor(m->elMatLT.data(), (long long)m->elMatLT.size(), m->elMatLTRange.data(), m->elMatDim);
+}
+int orc_factor_thermal(OrcModel* m) {
+    return orc_ldl_factor(m->thermMatLT.data(), (long long)m->thermMatLT.size(), m->thermMatLTRange.data(),
+                          m->numNodes);
+}
+
+// numVecs rule of stepSolve: n/2 below 200 unknowns, else 100.
+// Working/AStrO_commandFunctions.f90:711-715 (thermal :690-694).
+int orc_num_vecs(int n) { return n < 200 ? n / 2 : 100; }
+
+// stepSolve (linear static elastic branch): RHS, GMRES from the stored delDisp,
+// nodeDisp += delDisp.  Working/AStrO_commandFunctions.f90:681-749.  mode 0
+// keeps the reference's GMRES (maxIt = n); mode 1 is the stable form with a
+// relative tolerance rtol and no practical cycle cap.
+void orc_solve_elastic(OrcModel* mp, int numVecs, int mode, double rtol) {
+    OrcModel& m = *mp;
+    orc_elastic_rhs(mp);
+    if (numVecs <= 0) numVecs = orc_num_vecs(m.elMatDim);
+    int maxIt = mode == 0 ? m.elMatDim : 1000 * numVecs;
+    m.gmresCycles = orc_gmres_mode(m.delDisp.data(), m.elasticMat.data(), m.elMatCols.data(), m.elMatRange.data(),
+                              m.elMatDim, m.elMPCMat.data(), m.elMPCMatCols.data(), m.elMPCMatRange.data(),
+                              m.elMPCDim, m.elMatLT.data(), m.elMatLTRange.data(), m.elasticLoad.data(), numVecs,
+                              maxIt, mode, rtol, &m.gmresRes, &m.gmresSteps);
+    for (int i = 0; i < m.elMatDim; ++i) m.nodeDisp[i] += m.delDisp[i];
+}
+
+void orc_solve_thermal(OrcModel* mp, int numVecs, int mode, double rtol) {
+    OrcModel& m = *mp;
+    orc_thermal_rhs(mp);
+    if (numVecs <= 0) numVecs = orc_num_vecs(m.numNodes);
+    int maxIt = mode == 0 ? m.numNodes : 1000 * numVecs;
+    m.gmresCycles = orc_gmres_mode(m.delTemp.data(), m.thermMat.data(), m.thermMatCols.data(), m.thermMatRange.data(),
+                              m.numNodes, m.thermMPCMat.data(), m.thermMPCMatCols.data(),
+                              m.thermMPCMatRange.data(), m.thermMPCDim, m.thermMatLT.data(),
+                              m.thermMatLTRange.data(), m.thermalLoad.data(), numVecs, maxIt, mode, rtol,
+                              &m.gmresRes, &m.gmresSteps);
+}
+
+// Array getters for the Python side of the tests.
+const int* orc_geti(OrcModel* m, const char* name, long long* len) {
+    std::string s(name);
+    const std::vector<int>* v = nullptr;
+    if (s == "currentRank") v = &m->currentRank;
+    else if (s == "originalRank") v = &m->originalRank;
+    else if (s == "nDofIndex") v = &m->nDofIndex;
+    else if (s == "nCRange") v = &m->nCRange;
+    else if (s == "nodalConn") v = &m->nodalConn;
+    else if (s == "elMatRange") v = &m->elMatRange;
+    else if (s == "elMatCols") v = &m->elMatCols;
+    else if (s == "elMatLTRange") v = &m->elMatLTRange;
+    else if (s == "elMPCMatRange") v = &m->elMPCMatRange;
+    else if (s == "elMPCMatCols") v = &m->elMPCMatCols;
+    else if (s == "thermMatRange") v = &m->thermMatRange;
+    else if (s == "thermMatCols") v = &m->thermMatCols;
+    else if (s == "thermMatLTRange") v = &m->thermMatLTRange;
+    else if (s == "thermMPCMatRange") v = &m->thermMPCMatRange;
+    else if (s == "thermMPCMatCols") v = &m->thermMPCMatCols;
+    if (!v) { *len = -1; return nullptr; }
+    *len = (long long)v->size();
+    return v->data();
+}
+
+double* orc_getd(OrcModel* m, const char* name, long long* len) {
+    std::string s(name);
+    std::vector<double>* v = nullptr;
+    if (s == "elasticMat") v = &m->elasticMat;
+    else if (s == "elMatLT") v = &m->elMatLT;
+    else if (s == "elasticLoad") v = &m->elasticLoad;
+    else if (s == "delDisp") v = &m->delDisp;
+    else if (s == "nodeDisp") v = &m->nodeDisp;
+    else if (s == "elMPCMat") v = &m->elMPCMat;
+    else if (s == "elMPCRHS") v = &m->elMPCRHS;
+    else if (s == "thermMat") v = &m->thermMat;
+    else if (s == "thermMatLT") v = &m->thermMatLT;
+    else if (s == "thermalLoad") v = &m->thermalLoad;
+    else if (s == "delTemp") v = &m->delTemp;
+    else if (s == "thermMPCMat") v = &m->thermMPCMat;
+    else if (s == "thermMPCRHS") v = &m->thermMPCRHS;
+    else if (s == "elC") v = &m->elC;
+    if (!v) { *len = -1; return nullptr; }
+    *len = (long long)v->size();
+    return v->data();
+}
+
+int orc_geti_scalar(OrcModel* m, const char* name) {
+    std::string s(name);
+    if (s == "elMatDim") return m->elMatDim;
+    if (s == "solverMaxBW") return m->solverMaxBW;
+    if (s == "numSolidNds") return m->numSolidNds;
+    if (s == "elMPCDim") return m->elMPCDim;
+    if (s == "thermMPCDim") return m->thermMPCDim;
+    if (s == "gmresCycles") return m->gmresCycles;
+    if (s == "gmresSteps") return m->gmresSteps;
+    return -1;
+}
+
+double orc_getd_scalar(OrcModel* m, const char* name) {
+    std::string s(name);
+    if (s == "gmresRes") return m->gmresRes;
+    return NAN;
+}
+
+}  // extern "C"
