@@ -173,6 +173,9 @@ int astro_ctx_factor(astro_ctx* ctx);
  * Working/AStrO_commandFunctions.f90:353-535), GMRES from delDisp, nodeDisp +=
  * delDisp.  Synchronous. */
 int astro_ctx_solve(astro_ctx* ctx);
+/* Zero delDisp and nodeDisp (the state a fresh job starts from,
+ * Working/AStrO_bookKeeping.f90:960, Working/AStrO_commandFunctions.f90:653-679). */
+int astro_ctx_reset(astro_ctx* ctx);
 /* Whole solve() sequence: assemble, scale, profile, factor, solve. */
 int astro_ctx_static_solve(astro_ctx* ctx);
 
@@ -186,6 +189,13 @@ int astro_ctx_thermal_solve(astro_ctx* ctx);
 int astro_ctx_precond_apply(astro_ctx* ctx, const double* d_b, double* d_x);
 /* Operator apply y = (A + C^T C) x on device vectors. */
 int astro_ctx_operator_apply(astro_ctx* ctx, const double* d_x, double* d_y);
+
+/* Per-launch timing of the preconditioner sweeps with events on the context
+ * stream (on = 1 resets the totals).  astro_ctx_kernel_stat names:
+ * "sweep_fwd_ms", "sweep_bwd_ms" (summed device time), "sweep_fwd_count",
+ * "sweep_bwd_count", "profile_tiles", "panels_per_block", "split_k". */
+int astro_ctx_set_timing(astro_ctx* ctx, int on);
+double astro_ctx_kernel_stat(const astro_ctx* ctx, const char* name);
 
 /* Wait for all work queued on the context stream. */
 int astro_ctx_sync(astro_ctx* ctx);

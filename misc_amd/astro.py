@@ -32,7 +32,8 @@ EXPORTS = [
     "astro_ctx_scale_mpc", "astro_ctx_build_profile", "astro_ctx_factor", "astro_ctx_solve",
     "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_precond_apply",
     "astro_ctx_operator_apply", "astro_ctx_sync", "astro_ctx_phase_ms", "astro_prep_create", "astro_prep_destroy",
-    "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64",
+    "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64", "astro_ctx_set_timing",
+    "astro_ctx_kernel_stat", "astro_ctx_reset",
 ]
 
 
@@ -90,7 +91,8 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_set_f64.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
     L.astro_ctx_assemble_elastic.argtypes = [C.c_void_p, C.c_double]
     for f in ("astro_ctx_scale_mpc", "astro_ctx_build_profile", "astro_ctx_factor", "astro_ctx_solve",
-              "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_sync"):
+              "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_sync",
+              "astro_ctx_reset"):
         getattr(L, f).argtypes = [C.c_void_p]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_ctx_operator_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -100,6 +102,9 @@ def load_library(path: str = LIB_PATH):
     L.astro_prep_scalar.argtypes = [C.c_void_p, C.c_char_p]
     L.astro_prep_get_i32.argtypes = [C.c_void_p, C.c_char_p, _ip, _i64]
     L.astro_prep_get_f64.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
+    L.astro_ctx_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.astro_ctx_kernel_stat.restype = C.c_double
+    L.astro_ctx_kernel_stat.argtypes = [C.c_void_p, C.c_char_p]
     L.astro_ctx_phase_ms.restype = C.c_double
     L.astro_ctx_phase_ms.argtypes = [C.c_void_p, C.c_char_p]
     _lib = L
@@ -339,6 +344,9 @@ class Context:
     def solve(self):
         _check(load_library().astro_ctx_solve(self.h))
 
+    def reset(self):
+        _check(load_library().astro_ctx_reset(self.h))
+
     def static_solve(self):
         _check(load_library().astro_ctx_static_solve(self.h))
 
@@ -350,6 +358,12 @@ class Context:
 
     def sync(self):
         _check(load_library().astro_ctx_sync(self.h))
+
+    def set_timing(self, on: bool):
+        _check(load_library().astro_ctx_set_timing(self.h, 1 if on else 0))
+
+    def kernel_stat(self, name):
+        return float(load_library().astro_ctx_kernel_stat(self.h, name.encode()))
 
     def phase_ms(self, phase):
         return float(load_library().astro_ctx_phase_ms(self.h, phase.encode()))

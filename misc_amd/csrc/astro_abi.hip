@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -98,6 +99,42 @@ struct DeviceProfile {
     int fwdEpoch = 0, bwdEpoch = 0;
     DBuf<double> z;
     long spinLimit = 4000000;
+    // optional per-launch timing of the two sweeps (events on the launch stream)
+    bool timing = false;
+    std::vector<hipEvent_t> evPool;
+    size_t evUsed = 0;
+    double fwdMs = 0, bwdMs = 0;
+    long fwdCount = 0, bwdCount = 0;
+    std::vector<int> evKind;  // 0 = forward, 1 = backward, per event pair
+
+    ~DeviceProfile() {
+        for (auto e : evPool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t ev_next() {
+        if (evUsed >= evPool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            evPool.push_back(e);
+        }
+        return evPool[evUsed++];
+    }
+    // Fold recorded event pairs into the totals (stream must be idle).
+    void collect() {
+        for (size_t k = 0; k + 1 < evUsed; k += 2) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, evPool[k], evPool[k + 1]) == hipSuccess) {
+                if (evKind[k / 2] == 0) {
+                    fwdMs += ms;
+                    ++fwdCount;
+                } else {
+                    bwdMs += ms;
+                    ++bwdCount;
+                }
+            }
+        }
+        evUsed = 0;
+        evKind.clear();
+    }
 
     int setup(const int* LRange, int dim, hipStream_t s) {
         n = dim;
@@ -152,7 +189,7 @@ struct DeviceProfile {
             }
         }
         maxTiles = scratchRows / 64;
-        splitK = std::max(1, std::min(16, 1024 / std::max(1, nBlocks * maxTiles)));
+        splitK = std::max(1, std::min(16, (1536 + nBlocks * maxTiles - 1) / std::max(1, nBlocks * maxTiles)));
         ASTRO_TRY(panels.upload(hp, s));
         ASTRO_TRY(scratch.alloc((size_t)splitK * nBlocks * scratchRows * 64));
         ASTRO_TRY(zeroPivot.alloc(1));
@@ -176,7 +213,7 @@ struct DeviceProfile {
                 if (q < (int)chainTiles[k].size()) {
                     fIdx[k][q] = (int)f.size();
                     int r0 = chainTiles[k][q], r1 = std::min(r0 + 64, bounds[k + 1]);
-                    f.push_back(SweepTile{r0, r1, k, 0});
+                    f.push_back(SweepTile{r0, r1, k, 0, -1});
                 }
         for (int q = 0; q < maxCount; ++q)
             for (int k = 0; k < nBlocks; ++k) {
@@ -185,7 +222,7 @@ struct DeviceProfile {
                 if (t >= 0) {
                     bIdx[k][t] = (int)bw.size();
                     int r0 = chainTiles[k][t], r1 = std::min(r0 + 64, bounds[k + 1]);
-                    bw.push_back(SweepTile{r0, r1, k, 0});
+                    bw.push_back(SweepTile{r0, r1, k, 0, -1});
                 }
             }
         for (int k = 0; k < nBlocks; ++k) {
@@ -197,6 +234,7 @@ struct DeviceProfile {
                 for (int r = r0; r < r1; ++r) sm = std::min(sm, hS[r]);
                 int td = (sm - b0) / 64;
                 f[fIdx[k][t]].depFirst = (td < t) ? fIdx[k][td] : fIdx[k][t];
+                f[fIdx[k][t]].depLast = (td < t) ? fIdx[k][t - 1] : -1;
             }
             // backward: last tile holding a row whose profile reaches the tile
             std::vector<int> maxRowByTile(cnt, -1);
@@ -209,6 +247,7 @@ struct DeviceProfile {
                 run = std::max(run, maxRowByTile[t]);
                 int td = (run - b0) / 64;
                 bw[bIdx[k][t]].depFirst = (td > t) ? bIdx[k][td] : bIdx[k][t];
+                bw[bIdx[k][t]].depLast = (td > t) ? bIdx[k][t + 1] : -1;
             }
         }
         ASTRO_TRY(fwdTiles.upload(f, s));
@@ -230,9 +269,15 @@ struct DeviceProfile {
     int factor(hipStream_t s) {
         HIP_TRY(hipMemsetAsync(zeroPivot.p, 0x7f, sizeof(int), s));
         dim3 gu(maxTiles * splitK, nBlocks), gf(maxTiles, nBlocks);
+        const char* ev = std::getenv("ASTRO_LDL_UPDATE");
+        const bool useFma = ev && std::strcmp(ev, "fma") == 0;
         for (int q = 0; q < panelsPerBlock; ++q) {
-            hipLaunchKernelGGL(k_ldl_update, gu, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK, lt.p,
-                               rowoff.p, sfirst.p, scratch.p, scratchRows);
+            if (useFma)
+                hipLaunchKernelGGL(k_ldl_update_fma, gu, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK, lt.p,
+                                   rowoff.p, sfirst.p, scratch.p, scratchRows);
+            else
+                hipLaunchKernelGGL(k_ldl_update_mfma, gu, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK,
+                                   lt.p, rowoff.p, sfirst.p, scratch.p, scratchRows);
             hipLaunchKernelGGL(k_ldl_factor_panel, gf, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK,
                                lt.p, rowoff.p, sfirst.p, scratch.p, scratchRows, zeroPivot.p);
         }
@@ -248,12 +293,22 @@ struct DeviceProfile {
     int apply(const double* b, double* x, hipStream_t s) {
         ++fwdEpoch;
         hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
+        if (timing) {
+            evKind.push_back(0);
+            HIP_TRY(hipEventRecord(ev_next(), s));
+        }
         hipLaunchKernelGGL(k_sweep_forward, dim3(numTiles), dim3(256), 0, s, fwdTiles.p, numTiles, lt.p, rowoff.p,
                            sfirst.p, b, z.p, fwdFlags.p, ticket.p, fwdEpoch, spinLimit, timeout.p);
+        if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         ++bwdEpoch;
         hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
+        if (timing) {
+            evKind.push_back(1);
+            HIP_TRY(hipEventRecord(ev_next(), s));
+        }
         hipLaunchKernelGGL(k_sweep_backward, dim3(numTiles), dim3(256), 0, s, bwdTiles.p, numTiles, lt.p,
                            rowoff.p, sfirst.p, z.p, x, bwdFlags.p, ticket.p, bwdEpoch, spinLimit, timeout.p);
+        if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
     }
@@ -262,6 +317,7 @@ struct DeviceProfile {
         int t = 0;
         HIP_TRY(hipMemcpyAsync(&t, timeout.p, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        if (timing) collect();
         if (t) return fail(ASTRO_ERR_TIMEOUT, "triangular sweep wait exceeded its spin limit");
         return ASTRO_OK;
     }
@@ -327,6 +383,8 @@ struct Gmres {
     int solve(const DeviceOperator& A, DeviceProfile& P, const double* b, double* x, double rtol, double atol,
               int maxCycles, hipStream_t s) {
         steps = cycles = 0;
+        const char* ev = std::getenv("ASTRO_VERBOSE");
+        const bool verbose = ev && ev[0] == '1';
         auto residual = [&](double& out) -> int {
             ASTRO_TRY(A.apply(x, t.p, s));
             hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b, t.p, t2.p, n, 1.0, -1.0);
@@ -400,6 +458,8 @@ struct Gmres {
                                hdev.p, x, n, 1.0);
             ++cycles;
             ASTRO_TRY(residual(res));
+            if (verbose)
+                std::fprintf(stderr, "gmres: cycle %d steps %d res %.3e (target %.3e)\n", cycles, steps, res, target);
         }
         if (res > target) return fail(ASTRO_ERR_NO_CONVERGE, "GMRES did not reach the tolerance in " +
                                                                  std::to_string(cycles) + " cycles");
@@ -893,6 +953,15 @@ int astro_ctx_solve(astro_ctx* c) {
     return ASTRO_OK;
 }
 
+int astro_ctx_reset(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    const int n = c->prep.elMatDim;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->delDisp.p, (int64_t)n, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->nodeDisp.p, (int64_t)n, 0.0);
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
 int astro_ctx_static_solve(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     HIP_TRY(hipEventRecord(c->ev[0], c->s));
@@ -1017,6 +1086,30 @@ int astro_ctx_operator_apply(astro_ctx* c, const double* d_x, double* d_y) {
     ASTRO_TRY(c->op.apply(d_x, d_y, c->s));
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
+}
+
+int astro_ctx_set_timing(astro_ctx* c, int on) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->s));
+    c->prof.collect();
+    c->prof.timing = on != 0;
+    c->prof.fwdMs = c->prof.bwdMs = 0;
+    c->prof.fwdCount = c->prof.bwdCount = 0;
+    return ASTRO_OK;
+}
+
+double astro_ctx_kernel_stat(const astro_ctx* c, const char* name) {
+    if (!c || !name) return NAN;
+    std::string s(name);
+    const DeviceProfile& P = c->prof;
+    if (s == "sweep_fwd_ms") return P.fwdMs;
+    if (s == "sweep_bwd_ms") return P.bwdMs;
+    if (s == "sweep_fwd_count") return (double)P.fwdCount;
+    if (s == "sweep_bwd_count") return (double)P.bwdCount;
+    if (s == "profile_tiles") return (double)P.numTiles;
+    if (s == "panels_per_block") return (double)P.panelsPerBlock;
+    if (s == "split_k") return (double)P.splitK;
+    return NAN;
 }
 
 int astro_ctx_sync(astro_ctx* c) {

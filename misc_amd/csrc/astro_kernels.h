@@ -17,8 +17,11 @@ struct PanelInfo {
 // One 64-row tile of a triangular sweep in processing (ticket) order: rows
 // [r0, r1) of chain `chain` (an independent diagonal block); depFirst is the
 // ticket index of the first tile it depends on (== own index if none).
+// depLast: ticket index of the immediately neighbouring tile of the chain
+// (preceding for the forward sweep, following for the backward sweep) when
+// the tile depends on it, else -1; it is handled last, on the critical path.
 struct SweepTile {
-    int r0, r1, chain, depFirst;
+    int r0, r1, chain, depFirst, depLast;
 };
 
 __global__ void k_brick8_tangent(const int* perm, int count, const int* conn, const double* X, const int* elMat,
@@ -34,8 +37,12 @@ __global__ void k_csr_to_profile(const double* val, const int* cols, const int* 
                                  const int64_t* rowoff, double* lt);
 __global__ void k_penalty_to_profile(const double* cmat, const int* ccols, const int* crange, int cdim,
                                      const int* sfirst, const int64_t* rowoff, double* lt);
-__global__ void k_ldl_update(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK, double* lt,
-                             const int64_t* rowoff, const int* sfirst, double* scratch, int scratchRows);
+__global__ void k_ldl_update_mfma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
+                                  double* lt, const int64_t* rowoff, const int* sfirst, double* scratch,
+                                  int scratchRows);
+__global__ void k_ldl_update_fma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
+                                 double* lt, const int64_t* rowoff, const int* sfirst, double* scratch,
+                                 int scratchRows);
 __global__ void k_ldl_factor_panel(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
                                    double* lt, const int64_t* rowoff, const int* sfirst, const double* scratch,
                                    int scratchRows, int* zeroPivot);

@@ -301,7 +301,104 @@ __global__ void k_penalty_to_profile(const double* __restrict__ cmat, const int*
 // of an envelope matrix stays inside the envelope, as the reference's
 // in-place row factor (Working/AStrO_solvers.f90:10-78) relies on.
 
-__global__ __launch_bounds__(256) void k_ldl_update(const PanelInfo* __restrict__ panels, int panelIdx, int numPanelsPerBlock,
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// Step U on FP64 matrix cores: each of the 4 waves owns 16 tile rows x 64
+// panel columns as four 16x16 MFMA accumulators (v_mfma_f64_16x16x4f64);
+// k is staged through LDS in chunks of 32.  Operand lane layouts (measured on
+// gfx950): A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k, D(i, j) in
+// lane j + 16 (i % 4), register i / 4.
+__global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __restrict__ panels, int panelIdx, int numPanelsPerBlock,
+                                                    int splitK, double* __restrict__ lt,
+                                                    const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
+                                                    double* __restrict__ scratch, int scratchRows) {
+    const int blk = blockIdx.y;
+    const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
+    if (P.p < 0) return;
+    const int tile = blockIdx.x / splitK, sp = blockIdx.x % splitK;
+    const int r0 = P.p + 64 * tile;
+    if (r0 >= P.iend) return;
+    const int nr = min(64, P.iend - r0);
+    const int np = P.pe - P.p;
+    constexpr int KC = 32;
+    __shared__ double As[KC][64 + 1];
+    __shared__ double Bs[KC][64 + 1];
+    __shared__ double Dk[KC];
+    __shared__ int sMin;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid < 64) {
+        int v = tid < nr ? sfirst[r0 + tid] : 0x7fffffff;
+        for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+        if (tid == 0) sMin = v;
+    }
+    __syncthreads();
+    const int kLo = max(sMin, P.smin), kHi = P.p;
+    const int total = max(0, kHi - kLo);
+    int chunk = (total + splitK - 1) / splitK;
+    chunk = (chunk + KC - 1) / KC * KC;
+    const int kb = kLo + sp * chunk, ke = min(kHi, kb + chunk);
+    // per-thread staging rows (fixed across chunks)
+    const int srow = tid >> 2, sk = (tid & 3) * 8;  // 64 rows x 4 groups of 8 k
+    const bool aLive = srow < nr, bLive = srow < np;
+    const int aRow = r0 + srow, bRow = P.p + srow;
+    const int aS = aLive ? sfirst[aRow] : 0x7fffffff, bS = bLive ? sfirst[bRow] : 0x7fffffff;
+    const int64_t aOff = aLive ? rowoff[aRow] : 0, bOff = bLive ? rowoff[bRow] : 0;
+    double4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
+    // register double buffer: the next chunk's operands are fetched while the
+    // current one runs on the matrix cores
+    double ra[8], rb[8], rd = 0.0;
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + sk + q;
+            const bool kv = k < ke;
+            ra[q] = (kv && k >= aS) ? lt[aOff + k] : 0.0;
+            rb[q] = (kv && k >= bS) ? lt[bOff + k] : 0.0;
+        }
+        if (tid < KC) {
+            const int k = k0 + tid;
+            rd = k < ke ? lt[rowoff[k] + k] : 0.0;
+        }
+    };
+    if (kb < ke) fetch(kb);
+    for (int k0 = kb; k0 < ke; k0 += KC) {
+        __syncthreads();
+        if (tid < KC) Dk[tid] = rd;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            As[sk + q][srow] = ra[q];
+            Bs[sk + q][srow] = rb[q];
+        }
+        __syncthreads();
+        if (k0 + KC < ke) fetch(k0 + KC);
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            const double a = As[kk][w * 16 + (lane & 15)];
+            const double d = Dk[kk];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const double bv = d * Bs[kk][cb * 16 + (lane & 15)];
+                acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[cb], 0, 0, 0);
+            }
+        }
+    }
+    double* out = scratch + (((size_t)sp * gridDim.y + blk) * scratchRows + (r0 - P.p)) * 64;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = w * 16 + 4 * v + (lane >> 4);
+            const int j = cb * 16 + (lane & 15);
+            if (i < nr) out[(size_t)i * 64 + j] = acc[cb][v];
+        }
+}
+
+// Step U with plain FP64 FMAs (4x4 register tiles per thread); kept beside
+// the matrix-core form as its cross-check.
+__global__ __launch_bounds__(256) void k_ldl_update_fma(const PanelInfo* __restrict__ panels, int panelIdx, int numPanelsPerBlock,
                                                     int splitK, double* __restrict__ lt,
                                                     const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
                                                     double* __restrict__ scratch, int scratchRows) {
@@ -421,25 +518,25 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
         Dg[i][j] = v;
     }
     __syncthreads();
-    for (int j = 0; j < np; ++j) {
-        const double d = Dg[j][j];
-        if (tid == 0 && d == 0.0 && tile == 0) atomicMin(zeroPivot, P.p + j + 1);
-        const double inv = 1.0 / d;
-        for (int i = j + 1 + tid; i < np; i += 256) Dg[i][j] *= inv;
-        __syncthreads();
-        // trailing update Dg[i][k] -= l_ij d l_kj, j < k <= i < np
-        const int m = np - j - 1;
-        const int npair = m * (m + 1) / 2;
-        for (int q = tid; q < npair; q += 256) {
-            // map q -> (ii, kk) with 0 <= kk <= ii < m
-            int ii = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
-            while ((ii + 1) * (ii + 2) / 2 <= q) ++ii;
-            while (ii * (ii + 1) / 2 > q) --ii;
-            const int kk = q - ii * (ii + 1) / 2;
-            const int i = j + 1 + ii, k = j + 1 + kk;
-            Dg[i][k] -= Dg[i][j] * d * Dg[k][j];
+    // right-looking factor of the 64x64 block: column j is scaled into colL,
+    // then thread (i = tid / 4, q = tid % 4) updates row i at columns
+    // k = j+1+q, j+5+q, ... <= i
+    {
+        __shared__ double colL[64];
+        const int fi = tid >> 2, fq = tid & 3;
+        for (int j = 0; j < np; ++j) {
+            const double d = Dg[j][j];
+            if (tid == 0 && d == 0.0 && tile == 0) atomicMin(zeroPivot, P.p + j + 1);
+            if (tid < 64) colL[tid] = (tid > j && tid < np) ? Dg[tid][j] / d : 0.0;
+            __syncthreads();
+            if (fi > j && fi < np) {
+                const double l = colL[fi];
+                const double f = l * d;
+                for (int k = j + 1 + fq; k <= fi; k += 4) Dg[fi][k] -= f * colL[k];
+                if (fq == 0) Dg[fi][j] = l;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (tile == 0) {
         for (int id = tid; id < 64 * 64; id += 256) {
@@ -451,22 +548,24 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
         }
         return;
     }
-    // rows below the panel: y = L_PP^-1 a, L(i, j) = y_j / D_j; 4 lanes per row
+    // rows below the panel: y = L_PP^-1 a, L(i, j) = y_j / D_j; 4 lanes per row.
+    // The updated a(i, P) is gathered first (all loads in flight together).
     const int row = tid >> 2, q = tid & 3;
     const int gi = r0 + row;
     const bool live = row < nr;
     const int si = live ? sfirst[gi] : 0x7fffffff;
+    for (int j = q; j < 64; j += 4) {
+        const int gj = P.p + j;
+        double aij = 0.0;
+        if (live && j < np && gj >= si) aij = lt[rowoff[gi] + gj] - wsum(r0 - P.p + row, j);
+        Y[row][j] = aij;
+    }
     for (int j = 0; j < np; ++j) {
         double part = 0.0;
         for (int k = q; k < j; k += 4) part += Dg[j][k] * Y[row][k];
         part += __shfl_xor(part, 1);
         part += __shfl_xor(part, 2);
-        if ((j & 3) == q) {
-            const int gj = P.p + j;
-            double aij = 0.0;
-            if (live && gj >= si) aij = lt[rowoff[gi] + gj] - wsum(r0 - P.p + row, j);
-            Y[row][j] = aij - part;
-        }
+        if ((j & 3) == q) Y[row][j] -= part;
     }
     // (each lane only reads the Y entries it wrote itself: k == q mod 4)
     for (int j = q; j < np; j += 4) {
@@ -486,9 +585,33 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
 // the earlier tiles their profiles reach.  Backward (x = D^-1 z - L^T x,
 // :109-123): tile columns need x of the later tiles whose rows reach them.
 
+// Broadcast lane `src` (wave-uniform) of a double with two v_readlane.
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Cross-workgroup hand-off without cache-maintenance fences: the values a
+// tile publishes and its flag are written with agent-scope relaxed atomic
+// stores and read with agent-scope relaxed atomic loads, which go to the
+// device-coherent level instead of the per-XCD L2.  The producer waits for
+// its value stores to complete (vmcnt 0) before it stores the flag, so a
+// consumer that has seen the flag reads the values.  Agent-scope
+// acquire/release would instead write back and invalidate the whole L2 of the
+// XCD at every hand-off.
+__device__ __forceinline__ double coherent_load(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void coherent_store(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stores_complete() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ bool wait_flag(const int* flag, int epoch, long spinLimit, int* timeout) {
     long it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
         if (++it > spinLimit) {
             atomicExch(timeout, 1);
             return false;
@@ -498,6 +621,59 @@ __device__ __forceinline__ bool wait_flag(const int* flag, int epoch, long spinL
     return true;
 }
 
+// Per-wave incremental wait: lane l looks at ticket from + l; tickets of other
+// chains count as done.  Returns the first ticket in [from, last) whose flag
+// is not yet published (last if all are), spinning only while `from` itself
+// is pending.  Returns -1 if the bounded wait expired.
+__device__ __forceinline__ int wave_poll(const int* flags, const SweepTile* tiles, int chain, int from, int last,
+                                         int epoch, long spinLimit, int* timeout) {
+    const int lane = threadIdx.x & 63;
+    long it = 0;
+    while (true) {
+        const int J = from + lane;
+        bool done = true;
+        if (J < last && tiles[J].chain == chain)
+            done = __hip_atomic_load(&flags[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+        const unsigned long long pend = __ballot(!done);
+        if (pend == 0ull) {
+            const int nxt = from + 64;
+            if (nxt >= last) return last;
+            from = nxt;
+            continue;
+        }
+        const int firstPending = from + (int)__builtin_ctzll(pend);
+        if (firstPending > from) return firstPending;
+        if (++it > spinLimit) {
+            if (lane == 0) atomicExch(timeout, 1);
+            return -1;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Wait until every tile of `chain` with ticket in [first, last) has published
+// `epoch`: the 64 lanes of wave 0 poll distinct flags, then one barrier.
+// Returns false (for every thread) if a wait timed out.
+__device__ __forceinline__ bool wait_range(const int* flags, const SweepTile* tiles, int chain, int first, int last,
+                                           int epoch, long spinLimit, int* timeout) {
+    bool ok = true;
+    if (threadIdx.x < 64) {
+        for (int J = first + (int)threadIdx.x; J < last && ok; J += 64)
+            if (tiles[J].chain == chain) ok = wait_flag(&flags[J], epoch, spinLimit, timeout);
+    }
+    return __syncthreads_or(!ok) == 0;
+}
+
+__device__ __forceinline__ void publish_flag(int* flag, int epoch) {
+    __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Forward sweep tile: phase 1 accumulates the contributions of all earlier
+// tiles except the immediately preceding one (wave w owns rows w*16..+15,
+// lane = column, partials reduced across lanes once); phase 2 waits for the
+// preceding tile, applies its 64x64 coupling block (prefetched into LDS) and
+// solves the unit-lower diagonal block, so only LDS work and one coherent
+// load/store pair sit on the critical path between consecutive tiles.
 __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
                                                        const double* __restrict__ lt,
                                                        const int64_t* __restrict__ rowoff,
@@ -506,9 +682,10 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
                                                        int* __restrict__ ticket, int epoch, long spinLimit,
                                                        int* __restrict__ timeout) {
     __shared__ int sT;
-    __shared__ double Ld[64][65];
-    __shared__ double part[4][64];
-    __shared__ double acc[64];
+    __shared__ double Ld[64][65];   // Ld[i][c] = L(r0+i, r0+c), c < i
+    __shared__ double Lp[64][65];   // Lp[i][c] = L(r0+i, p0+c), preceding tile columns
+    __shared__ double part[4][16];
+    __shared__ double zp[64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
@@ -516,39 +693,55 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
     if (t >= numTiles) return;
     const SweepTile T = tiles[t];
     const int r0 = T.r0, nr = T.r1 - T.r0;
-    // diagonal block rows (independent of z): Ld[i][c] = L(r0+i, r0+c), c < i
+    const bool hasPrev = T.depLast >= 0;
+    const SweepTile Pv = hasPrev ? tiles[T.depLast] : T;
+    const int p0 = Pv.r0, np = hasPrev ? Pv.r1 - Pv.r0 : 0;
     for (int id = tid; id < 64 * 64; id += 256) {
         const int i = id >> 6, c = id & 63;
-        double v = 0.0;
-        if (i < nr && c < i) {
-            const int gi = r0 + i, gc = r0 + c;
-            if (gc >= sfirst[gi]) v = lt[rowoff[gi] + gc];
+        double v = 0.0, u = 0.0;
+        if (i < nr) {
+            const int gi = r0 + i, si = sfirst[gi];
+            const int64_t off = rowoff[gi];
+            if (c < i && r0 + c >= si) v = lt[off + r0 + c];
+            if (c < np && p0 + c >= si) u = lt[off + p0 + c];
         }
         Ld[i][c] = v;
+        Lp[i][c] = u;
     }
-    // earlier tiles: wave w owns rows w*16 .. w*16+15, lane = column offset
+    int rs[16];
+    int64_t ro[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = w * 16 + r;
+        rs[r] = i < nr ? sfirst[r0 + i] : 0x7fffffff;
+        ro[r] = i < nr ? rowoff[r0 + i] : 0;
+    }
     double p[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) p[r] = 0.0;
-    bool ok = true;
-    for (int J = T.depFirst; J < t && ok; ++J) {
-        const SweepTile D = tiles[J];
-        if (D.chain != T.chain) continue;
-        if (tid == 0) ok = wait_flag(&flags[J], epoch, spinLimit, timeout);
-        ok = __syncthreads_or(!ok) == 0;
-        if (!ok) break;
-        const int c = D.r0 + lane;
-        const double zc = c < D.r1 ? z[c] : 0.0;
+    const int lastFar = hasPrev ? T.depLast : t;  // phase 1 covers tickets [depFirst, lastFar)
+    // The far columns are the rows of earlier tiles of this chain, contiguous
+    // from the first far tile's r0 up to the preceding tile's r0.  Each wave
+    // streams the columns of the tiles already published and re-polls at the
+    // first pending one; no block-wide barrier.
+    if (T.depFirst < lastFar) {
+        const int cEnd = p0;
+        int c0 = tiles[T.depFirst].r0;
+        int J = T.depFirst;
+        while (c0 < cEnd) {
+            const int pend = wave_poll(flags, tiles, T.chain, J, lastFar, epoch, spinLimit, timeout);
+            if (pend < 0) return;
+            const int cStop = pend >= lastFar ? cEnd : tiles[pend].r0;
+            for (int c = c0 + lane; c < cStop; c += 64) {
+                const double zc = coherent_load(&z[c]);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int i = w * 16 + r;
-            if (i < nr && c < D.r1) {
-                const int gi = r0 + i;
-                if (c >= sfirst[gi]) p[r] += lt[rowoff[gi] + c] * zc;
+                for (int r = 0; r < 16; ++r)
+                    if (c >= rs[r]) p[r] += lt[ro[r] + c] * zc;
             }
+            c0 = cStop;
+            J = pend;
         }
     }
-    if (!ok) return;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         double v = p[r];
@@ -557,21 +750,39 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
     }
     __syncthreads();
     if (w == 0) {
-        // part[w'][r] holds row w'*16 + r
         const int i = lane;
         double a = 0.0;
         if (i < nr) a = b[r0 + i] - part[i >> 4][i & 15];
-        for (int c = 0; c < nr; ++c) {
-            const double zc = __shfl(a, c);
-            if (i > c) a -= Ld[i][c] * zc;
+        if (hasPrev) {
+            bool ok = true;
+            if (i == 0) ok = wait_flag(&flags[T.depLast], epoch, spinLimit, timeout);
+            ok = __shfl(ok ? 1 : 0, 0) != 0;
+            if (!ok) return;
+            zp[i] = i < np ? coherent_load(&z[p0 + i]) : 0.0;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            double s2 = 0.0;
+            for (int c = 0; c < np; ++c) s2 += Lp[i][c] * zp[c];
+            a -= s2;
         }
-        if (i < nr) z[r0 + i] = a;
-        __threadfence();
-        if (i == 0) __hip_atomic_store(&flags[t], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        double lrow[64];
+#pragma unroll
+        for (int c = 0; c < 64; ++c) lrow[c] = Ld[i][c];
+#pragma unroll
+        for (int c = 0; c < 64; ++c) {
+            const double zc = lane_bcast(a, c);
+            if (i > c) a -= lrow[c] * zc;
+        }
+        if (i < nr) coherent_store(&z[r0 + i], a);
+        stores_complete();
+        if (i == 0) publish_flag(&flags[t], epoch);
     }
-    (void)acc;
 }
 
+// Backward sweep tile (columns [r0, r1)): phase 1 takes the rows of all later
+// tiles except the immediately following one; phase 2 waits for that tile and
+// applies its rows' block (prefetched into LDS), then solves L^T on the
+// diagonal block.
 __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restrict__ tiles, int numTiles,
                                                         const double* __restrict__ lt,
                                                         const int64_t* __restrict__ rowoff,
@@ -580,8 +791,10 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                                                         int* __restrict__ ticket, int epoch, long spinLimit,
                                                         int* __restrict__ timeout) {
     __shared__ int sT;
-    __shared__ double Ld[64][65];
+    __shared__ double Ld[64][65];   // Ld[c][i] = L(r0+c, r0+i), i < c
+    __shared__ double Ln[64][65];   // Ln[j][i] = L(q0+j, r0+i), rows of the following tile
     __shared__ double part[4][64];
+    __shared__ double xn[64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
@@ -589,33 +802,65 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     if (t >= numTiles) return;
     const SweepTile T = tiles[t];
     const int r0 = T.r0, nr = T.r1 - T.r0;
-    // Ld[c][i] = L(r0+c, r0+i) for i < c (row c of the diagonal block)
+    const bool hasNext = T.depLast >= 0;
+    const SweepTile Nx = hasNext ? tiles[T.depLast] : T;
+    const int q0 = Nx.r0, nq = hasNext ? Nx.r1 - Nx.r0 : 0;
     for (int id = tid; id < 64 * 64; id += 256) {
         const int c = id >> 6, i = id & 63;
-        double v = 0.0;
-        if (c < nr && i < c) {
-            const int gc = r0 + c, gi = r0 + i;
-            if (gi >= sfirst[gc]) v = lt[rowoff[gc] + gi];
+        double v = 0.0, u = 0.0;
+        if (i < nr) {
+            if (c < nr && i < c) {
+                const int gc = r0 + c;
+                if (r0 + i >= sfirst[gc]) v = lt[rowoff[gc] + r0 + i];
+            }
+            if (c < nq) {
+                const int gj = q0 + c;
+                if (r0 + i >= sfirst[gj]) u = lt[rowoff[gj] + r0 + i];
+            }
         }
         Ld[c][i] = v;
+        Ln[c][i] = u;
     }
-    // later tiles (processed earlier): lane = column i of this tile, wave w
-    // takes rows j == w (mod 4) of each later tile
     double p = 0.0;
-    bool ok = true;
     const int ci = r0 + lane;
-    for (int J = T.depFirst; J < t && ok; ++J) {
-        const SweepTile D = tiles[J];
-        if (D.chain != T.chain) continue;
-        if (tid == 0) ok = wait_flag(&flags[J], epoch, spinLimit, timeout);
-        ok = __syncthreads_or(!ok) == 0;
-        if (!ok) break;
-        for (int j = D.r0 + w; j < D.r1; j += 4) {
-            const int sj = sfirst[j];
-            if (lane < nr && ci >= sj) p += lt[rowoff[j] + ci] * x[j];
+    const bool lv = lane < nr;
+    const int lastFar = hasNext ? T.depLast : t;
+    // Far rows: from the tile after the following one up to the last tile
+    // holding a row that reaches this tile.  Published tiles form a top-down
+    // prefix of that range (tiles finish in descending order); each wave
+    // streams the published rows and re-polls, without block barriers.
+    if (T.depFirst < lastFar) {
+        const int jLow = Nx.r1;
+        int j1 = tiles[T.depFirst].r1;  // rows [jStop, j1) still to do, top-down
+        int J = T.depFirst;
+        while (j1 > jLow) {
+            const int pend = wave_poll(flags, tiles, T.chain, J, lastFar, epoch, spinLimit, timeout);
+            if (pend < 0) return;
+            const int jStop = pend >= lastFar ? jLow : tiles[pend].r1;
+            // wave w takes 16-row groups jb = jStop + 16 w + 64 q; lanes 0..15
+            // fetch the group's first columns, offsets and x, broadcast below
+            for (int jb = jStop + 16 * w; jb < j1; jb += 64) {
+                int sjl = 0x7fffffff;
+                long long rol = 0;
+                double xl = 0.0;
+                if (lane < 16 && jb + lane < j1) {
+                    sjl = sfirst[jb + lane];
+                    rol = rowoff[jb + lane];
+                    xl = coherent_load(&x[jb + lane]);
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int sj = __builtin_amdgcn_readlane(sjl, r);
+                    const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
+                                         (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
+                    const double xj = lane_bcast(xl, r);
+                    if (lv && ci >= sj) p += lt[ro + ci] * xj;
+                }
+            }
+            j1 = jStop;
+            J = pend;
         }
     }
-    if (!ok) return;
     part[w][lane] = p;
     __syncthreads();
     if (w == 0) {
@@ -625,13 +870,29 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
             const int gi = r0 + i;
             a = z[gi] / lt[rowoff[gi] + gi] - (part[0][i] + part[1][i] + part[2][i] + part[3][i]);
         }
-        for (int c = nr - 1; c >= 0; --c) {
-            const double xc = __shfl(a, c);
-            if (i < c) a -= Ld[c][i] * xc;
+        if (hasNext) {
+            bool ok = true;
+            if (i == 0) ok = wait_flag(&flags[T.depLast], epoch, spinLimit, timeout);
+            ok = __shfl(ok ? 1 : 0, 0) != 0;
+            if (!ok) return;
+            xn[i] = i < nq ? coherent_load(&x[q0 + i]) : 0.0;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            double s2 = 0.0;
+            for (int j = 0; j < nq; ++j) s2 += Ln[j][i] * xn[j];
+            a -= s2;
         }
-        if (i < nr) x[r0 + i] = a;
-        __threadfence();
-        if (i == 0) __hip_atomic_store(&flags[t], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        double lcol[64];
+#pragma unroll
+        for (int c = 0; c < 64; ++c) lcol[c] = Ld[c][i];
+#pragma unroll
+        for (int c = 63; c >= 0; --c) {
+            const double xc = lane_bcast(a, c);
+            if (i < c) a -= lcol[c] * xc;
+        }
+        if (i < nr) coherent_store(&x[r0 + i], a);
+        stores_complete();
+        if (i == 0) publish_flag(&flags[t], epoch);
     }
 }
 
