@@ -304,7 +304,8 @@ class Context:
             "nodeDisp": n("elMatDim"), "delDisp": n("elMatDim"), "thermMatRange": self.model.num_nodes + 1,
             "thermMatLTRange": self.model.num_nodes + 1, "thermMatCols": n("thermMatSize"),
             "thermMat": n("thermMatSize"), "thermMatLT": n("thermMatLTSize"), "thermalLoad": self.model.num_nodes,
-            "nodeTemp": self.model.num_nodes,
+            "nodeTemp": self.model.num_nodes, "elMPCMatRange": n("elMPCDim") + 1,
+            "elMPCMatCols": n("elMPCSize"), "elMPCMat": n("elMPCSize"), "elMPCRHS": n("elMPCDim"),
         }
         return sizes[name]
 

@@ -789,6 +789,7 @@ int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
     if (s == "numSolidNds") return p.numSolidNds;
     if (s == "solverMaxBW") return p.solverMaxBW;
     if (s == "elMPCDim") return p.elMPCDim;
+    if (s == "elMPCSize") return (int64_t)p.elMPCMat.size();
     if (s == "thermMatSize") return (int64_t)p.thermMatCols.size();
     if (s == "thermMatLTSize") return p.thermMatLTRange.empty() ? 0 : p.thermMatLTRange.back();
     if (s == "gmresSteps") return c->gm.steps;

@@ -6,7 +6,8 @@
 ! of the hot path as raw little-endian binary so that tests can compare the
 ! oracle restatement and the HIP library against it.
 !
-! usage: astro_ref_driver <model.yaml> <out-prefix> <elastic|thermal>
+! usage: astro_ref_driver <model.yaml> <out-prefix> <elastic|thermal|prep>
+! (prep stops after analysisPrep and its dumps)
 !
 ! Solver parameters follow the *solve block: solverBlockDim = numNodes and
 ! solverMaxBW from 27*numNodes*numNodes in default INTEGER arithmetic
@@ -98,6 +99,11 @@ program astro_ref_driver
         call dumpi(elMPCMatRange, elMPCDim+1, 'elMPCMatRange0')
         call dumpi(elMPCMatCols, elMPCSize, 'elMPCMatCols')
         call dumpd(elMPCMat, elMPCSize, 'elMPCMat0')
+    endif
+
+    if (trim(mode) .eq. 'prep') then
+        close(lfUnit)
+        stop
     endif
 
     call loadInitialState()

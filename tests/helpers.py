@@ -1,0 +1,58 @@
+"""Shared helpers of the test suite (model builders, fixture loading, error
+metrics)."""
+
+import glob
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def golden(kind):
+    """All fixtures of one kind ("elastic", "thermal", "prep") as (dims, data)."""
+    out = []
+    for f in sorted(glob.glob(os.path.join(GOLDEN, f"ref_{kind}_*.npz"))):
+        d = dict(np.load(f, allow_pickle=False))
+        out.append((tuple(int(v) for v in d["dims"]), d))
+    return out
+
+
+def entry_rel(a, b, floor=0.0):
+    """max |a - b| / max(|b|, floor) over all entries."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.maximum(np.abs(b), floor)
+    ok = den > 0
+    return float(np.max(np.abs(a - b)[ok] / den[ok])) if ok.any() else float(np.max(np.abs(a - b)))
+
+
+def rel_norm(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+def csr_dense(vals, cols, rng, n):
+    A = np.zeros((n, n))
+    for i in range(n):
+        for s in range(rng[i], rng[i + 1]):
+            if cols[s]:
+                A[i, cols[s] - 1] += vals[s]
+    return A
+
+
+def csr_sparse(vals, cols, rng, n, m=None):
+    import scipy.sparse as sp
+
+    m = n if m is None else m
+    rows = np.repeat(np.arange(len(rng) - 1), np.diff(rng))
+    msk = np.asarray(cols) > 0
+    return sp.csr_matrix((np.asarray(vals)[msk], (rows[msk], np.asarray(cols)[msk] - 1)), shape=(len(rng) - 1, m))
+
+
+def profile_dense(lt, rng, n):
+    A = np.zeros((n, n))
+    for i in range(n):
+        ln = rng[i + 1] - rng[i]
+        A[i, i - ln + 1:i + 1] = lt[rng[i]:rng[i + 1]]
+    return A
