@@ -2,23 +2,75 @@
 """Assembled+solved DOF/s of the linear-static brick8 hot path on one MI355X.
 
 Workload (one "step"): the 1.0 x 1.0 x 0.5 m block of 64 x 64 x 32 brick8
-elements (418,275 DOF), E = 200 GPa, nu = 0.3, bottom face clamped, 1 MPa on
-the top face as consistent nodal loads, solved as the reference's solve()
-does it: element assembly into the CSR, penalty scaling, profile extraction,
-skyline LDL^T, then left-preconditioned GMRES to a relative preconditioned
-residual of 1e-12.  Everything is FP64 and resident in HBM; the mesh and the
-symbolic analysis (node ordering, DOF map, sparsity) are set up once before
-the timed region, as they are per mesh, not per solve.
+elements (139,425 nodes, 418,275 DOF), E = 200 GPa, nu = 0.3, bottom face
+clamped by penalty constraints, 1 MPa on the top face as consistent nodal
+loads, solved the way the reference's solve() does it: element assembly into
+the CSR, penalty scaling, profile extraction, skyline LDL^T of the profile,
+then left-preconditioned GMRES(100) to |P^-1 r| <= 1e-12 |P^-1 r0|.  All FP64
+and resident in HBM; the mesh upload and the symbolic analysis (node ordering,
+DOF map, sparsity, profile) happen once before the timed region, as they are
+per mesh, not per solve.
+
+solverMaxBW: "int64" (default) evaluates the reference's rule
+b^3 >= 27 N^2 without overflow (b = 8067, profile 1.25e9 entries, ~10 GB);
+"literal" reproduces the reference's 32-bit wraparound (b = 957, profile
+3.2e7 entries, ~30x more GMRES steps).
+
+The JSON line carries the phase times, the roofline of the dominant kernel
+(the triangular sweeps of the preconditioner, timed with HIP events on the
+context stream over the timed region) and a CPU baseline: the literal
+reference (its own Fortran compiled from source, oracle/_ref) run on a
+bounded sample, a 16 x 16 x 8 block of the same problem, on one host core.
 """
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(sample=(16, 16, 8)):
+    """Time the reference pipeline (assemble, scale, profile, factor, GMRES)
+    on a sample block with one host core; returns the cpu_baseline object."""
+    from misc_amd.meshes import cantilever_block
+    from oracle import oracle as O
+
+    model = cantilever_block(*sample)
+    if O.ref_available():
+        r = O.run_reference(model, "elastic")
+        t = r["times"]
+        total = t["assemble"] + t["scale"] + t["profile"] + t["factor"] + t["solve"]
+        n = int(r["scalars"][1])
+        kind = "reference"
+        what = "literal reference (flang -O2, its own GMRES stopping rule)"
+    else:
+        om = O.OracleModel(model).prep()
+        t0 = time.perf_counter()
+        om.static_elastic(mode=0)
+        total = time.perf_counter() - t0
+        n = om.scalar("elMatDim")
+        kind = "port"
+        what = "C++ restatement of the reference algorithm (-O2)"
+    return {"value": n / total, "unit": "DOF/s", "cores": 1, "kind": kind,
+            "sample": f"brick8 block {sample[0]}x{sample[1]}x{sample[2]} ({n} DOF), same physics and "
+                      f"solve sequence, {what}, {total:.1f} s"}
+
+
+def load_traffic():
+    """HBM bytes per sweep launch measured by rocprofv3 PMC runs (FETCH_SIZE,
+    WRITE_SIZE, calibrated against a copy kernel), if recorded."""
+    f = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+    if os.path.exists(f):
+        with open(f) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    return None
 
 
 def main():
@@ -29,20 +81,21 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=32)
-    ap.add_argument("--maxbw", choices=["literal", "int64"], default="literal")
+    ap.add_argument("--maxbw", choices=["literal", "int64"], default="int64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(the profile's component blocks would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported. "
+              f"(The three displacement-component blocks of the preconditioner would shard across GPUs; "
+              f"DESIGN.md describes the multi-GPU layout.)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     from misc_amd import astro
     from misc_amd.meshes import cantilever_block
 
     model = cantilever_block(args.nx, args.ny, args.nz)
-    print(f"bench: building brick8 {args.nx}x{args.ny}x{args.nz}", file=sys.stderr, flush=True)
+    print(f"bench: brick8 {args.nx}x{args.ny}x{args.nz}, solverMaxBW rule {args.maxbw}", file=sys.stderr,
+          flush=True)
     ctx = astro.Context(model, astro.Options(literalMaxBW=(args.maxbw == "literal")))
     n = ctx.scalar("elMatDim")
     prof = ctx.scalar("elMatLTSize")
@@ -62,15 +115,17 @@ def main():
         for p in phases:
             phases[p] += ctx.phase_ms(p)
         steps_total += ctx.scalar("gmresSteps")
+        print(f"bench: step {k} done", file=sys.stderr, flush=True)
     ctx.sync()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     fwd_ms, bwd_ms = ctx.kernel_stat("sweep_fwd_ms"), ctx.kernel_stat("sweep_bwd_ms")
     fwd_n, bwd_n = ctx.kernel_stat("sweep_fwd_count"), ctx.kernel_stat("sweep_bwd_count")
     ctx.set_timing(False)
-    u = ctx.getd("nodeDisp")
-    # true residual of the last solve: |b - (A + C^T C) u| / |b|
+    rel_res = ctx.true_residual()
     value = n * args.steps / elapsed
+    # one sweep reads the strict lower profile (or, backward, all of it) and
+    # reads b / writes x: 8 bytes per profile entry + 16 per row
     sweep_bytes = 8.0 * prof + 16.0 * n
     avg_sweep_ms = (fwd_ms + bwd_ms) / max(1.0, fwd_n + bwd_n)
     achieved = sweep_bytes / (avg_sweep_ms * 1e-3) / 1e9
@@ -87,17 +142,25 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"brick8 block {args.nx}x{args.ny}x{args.nz} linear static, skyline LDL^T "
-                               f"preconditioned GMRES, solverMaxBW rule {args.maxbw}",
+        "config": {"workload": f"brick8 block {args.nx}x{args.ny}x{args.nz} linear static (E 200 GPa, nu 0.3, "
+                               f"bottom clamped, 1 MPa top), skyline LDL^T-preconditioned GMRES(100), "
+                               f"solverMaxBW rule {args.maxbw}",
                    "dof": n, "csr_nnz": nnz, "profile_entries": prof, "solverMaxBW": ctx.scalar("solverMaxBW"),
-                   "gmres_steps_per_solve": steps_total / args.steps},
+                   "gmres_steps_per_solve": steps_total / args.steps, "rtol_preconditioned": 1e-12,
+                   "true_rel_residual": rel_res},
         "phases_ms_per_step": {k: v / args.steps for k, v in phases.items()},
-        "roofline": {"kernel": "triangular sweep (k_sweep_forward / k_sweep_backward)", "bound": "hbm",
-                     "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                     "traffic": None, "avg_launch_ms": avg_sweep_ms, "launches": fwd_n + bwd_n,
-                     "algorithmic_bytes_per_launch": sweep_bytes},
+        "roofline": {"kernel": "k_sweep_forward / k_sweep_backward (preconditioner triangular sweeps)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(), "avg_launch_ms": avg_sweep_ms,
+                     "launches": int(fwd_n + bwd_n), "algorithmic_bytes_per_launch": sweep_bytes},
         "cpu_baseline": None,
     }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline is a report, not the measurement
+            print(f"bench: cpu baseline failed: {e}", file=sys.stderr)
     print(json.dumps(out))
 
 

@@ -197,6 +197,10 @@ int astro_ctx_operator_apply(astro_ctx* ctx, const double* d_x, double* d_y);
 int astro_ctx_set_timing(astro_ctx* ctx, int on);
 double astro_ctx_kernel_stat(const astro_ctx* ctx, const char* name);
 
+/* |b - (A + C^T C) u| / |b| for the current elastic load b and nodeDisp u,
+ * computed on the device (synchronous). */
+int astro_ctx_true_residual(astro_ctx* ctx, double* relRes);
+
 /* Wait for all work queued on the context stream. */
 int astro_ctx_sync(astro_ctx* ctx);
 /* Per-phase device times (ms) of the last astro_ctx_static_solve, measured

@@ -33,7 +33,7 @@ EXPORTS = [
     "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_precond_apply",
     "astro_ctx_operator_apply", "astro_ctx_sync", "astro_ctx_phase_ms", "astro_prep_create", "astro_prep_destroy",
     "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64", "astro_ctx_set_timing",
-    "astro_ctx_kernel_stat", "astro_ctx_reset",
+    "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual",
 ]
 
 
@@ -102,6 +102,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_prep_scalar.argtypes = [C.c_void_p, C.c_char_p]
     L.astro_prep_get_i32.argtypes = [C.c_void_p, C.c_char_p, _ip, _i64]
     L.astro_prep_get_f64.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
+    L.astro_ctx_true_residual.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.astro_ctx_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.astro_ctx_kernel_stat.restype = C.c_double
     L.astro_ctx_kernel_stat.argtypes = [C.c_void_p, C.c_char_p]
@@ -359,6 +360,11 @@ class Context:
 
     def sync(self):
         _check(load_library().astro_ctx_sync(self.h))
+
+    def true_residual(self):
+        r = C.c_double(0)
+        _check(load_library().astro_ctx_true_residual(self.h, C.byref(r)))
+        return r.value
 
     def set_timing(self, on: bool):
         _check(load_library().astro_ctx_set_timing(self.h, 1 if on else 0))

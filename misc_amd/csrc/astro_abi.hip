@@ -1113,6 +1113,21 @@ double astro_ctx_kernel_stat(const astro_ctx* c, const char* name) {
     return NAN;
 }
 
+int astro_ctx_true_residual(astro_ctx* c, double* relRes) {
+    if (!c || !relRes) return fail(ASTRO_ERR_ARG, "null argument");
+    const int n = c->prep.elMatDim;
+    Gmres& g = c->gm;
+    ASTRO_TRY(c->op.apply(c->nodeDisp.p, g.t.p, c->s));
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->load.p, g.t.p, g.t2.p, n, 1.0,
+                       -1.0);
+    g.multidot(g.t2.p, 1, g.t2.p, g.hdev.p, c->s);
+    g.multidot(c->load.p, 1, c->load.p, g.hdev.p + 1, c->s);
+    HIP_TRY(hipMemcpyAsync(g.hHost, g.hdev.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    *relRes = std::sqrt(g.hHost[0]) / std::sqrt(g.hHost[1]);
+    return ASTRO_OK;
+}
+
 int astro_ctx_sync(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     HIP_TRY(hipStreamSynchronize(c->s));

@@ -18,6 +18,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "astro_ref_driver")
+# the reference host with its solver module replaced by the ISO_C_BINDING
+# module of misc_amd/fortran (links misc_amd/libastro_hip.so): the drop-in
+DROPIN_DRIVER = os.path.join(HERE, "_ref", "hip", "astro_ref_driver_hip")
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -256,25 +259,31 @@ def ref_available() -> bool:
     return os.path.exists(REF_DRIVER)
 
 
-def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None) -> Dict[str, np.ndarray]:
+def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
+                  driver: Optional[str] = None) -> Dict[str, np.ndarray]:
     """Write `model` in the reference's format, run the literal reference driver
     on it and return every dumped array (1-based values as the reference holds
     them)."""
     from misc_amd.refformat import write_model
 
-    if not ref_available():
-        raise RuntimeError("literal reference driver not built (make -C oracle ref)")
+    exe = driver or REF_DRIVER
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} not built (make -C oracle ref / ref-hip)")
     own = workdir is None
     wd = workdir or tempfile.mkdtemp(prefix="astro_ref_")
     mf = os.path.join(wd, "model.yaml")
     write_model(model, mf)
     # The reference reads heap memory it never initialised (the (0) entry of
     # elMPCMatRange / thermMPCMatRange, Working/AStrO_bookKeeping.f90:516-529,
-    # 656-674; elMatCols / thermMatCols, :374-375, :438-439) and works only where
-    # fresh allocations come back zero-filled.  glibc's MALLOC_PERTURB_=255 makes
-    # every malloc return zeroed memory, which pins that behaviour.
-    env = dict(os.environ, MALLOC_PERTURB_="255")
-    subprocess.run([REF_DRIVER, mf, os.path.join(wd, "o_"), mode], cwd=wd, check=True, env=env)
+    # 656-674, read during buildElasticMPC itself) and works only where fresh
+    # allocations come back zero-filled.  glibc's MALLOC_PERTURB_=255 zero-fills
+    # every allocation served by the main allocator; its per-thread cache
+    # bypasses that, so the cache is switched off as well.  (This matters once
+    # another runtime, e.g. HIP in the drop-in build, has used the heap.)
+    tun = os.environ.get("GLIBC_TUNABLES", "")
+    tun = (tun + ":" if tun else "") + "glibc.malloc.tcache_count=0"
+    env = dict(os.environ, MALLOC_PERTURB_="255", GLIBC_TUNABLES=tun)
+    subprocess.run([exe, mf, os.path.join(wd, "o_"), mode], cwd=wd, check=True, env=env)
     out = {}
     for fn in os.listdir(wd):
         if not fn.startswith("o_"):

@@ -87,6 +87,18 @@ program astro_ref_driver
     ! allocates when intVecSize > 0 (Working/AStrO_bookKeeping.f90:1005).
     if (.not. allocated(prevIntDisp)) allocate(prevIntDisp(0))
     if (.not. allocated(intElasticLoad)) allocate(intElasticLoad(0))
+    ! Arrays the reference allocates and then reads without ever assigning:
+    ! the (0) entry of the constraint range arrays (set nowhere in
+    ! Working/AStrO_bookKeeping.f90:516-529, 656-674) and the CSR column arrays,
+    ! whose 0 entries mark free slots for the assembly probe
+    ! (allocated at :374-375, :438-439, probed at
+    ! Working/AStrO_commandFunctions.f90:326, 50).  A fresh process heap
+    ! usually returns them zeroed; once another runtime has used the heap it
+    ! does not, so the driver zeroes them.
+    if (allocated(elMPCMatRange)) elMPCMatRange(0) = 0
+    if (allocated(thermMPCMatRange)) thermMPCMatRange(0) = 0
+    elMatCols(:) = 0
+    thermMatCols(:) = 0
 
     call dumpi(currentRank, numNodes, 'currentRank')
     call dumpi(nDofIndex, 6*numNodes, 'nDofIndex')

@@ -1,0 +1,36 @@
+"""Drop-in: the reference's own Fortran host (readModelInput, analysisPrep,
+getElasticSolnLoad, scaleElasticMPC, convertToLTri) with its solver module
+replaced by misc_amd/fortran/astro_solvers_hip.f90, which calls
+libastro_hip.so through ISO_C_BINDING.  The factor and the solution it
+produces are compared with the oracle."""
+
+import os
+
+import numpy as np
+import pytest
+
+from helpers import entry_rel, rel_norm
+from misc_amd.meshes import cantilever_block
+from oracle import oracle as O
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(O.DROPIN_DRIVER), reason="drop-in driver not built here")]
+
+
+@pytest.mark.parametrize("dims", [(6, 5, 3), (12, 10, 6)])
+def test_reference_host_with_gpu_solvers(dims):
+    m = cantilever_block(*dims)
+    print("drop-in", dims, flush=True)
+    r = O.run_reference(m, "elastic", driver=O.DROPIN_DRIVER)
+    om = O.OracleModel(m).prep().static_elastic(stop_after="profile")
+    np.testing.assert_array_equal(r["elMatCols"], om.geti("elMatCols"))
+    Kr = om.getd("elasticMat")
+    e_csr = entry_rel(r["elasticMat"], Kr, floor=1e-3 * np.abs(Kr).max())
+    e_prof = entry_rel(r["elMatLT"], om.getd("elMatLT"), floor=1e-3 * np.abs(Kr).max())
+    O.lib().orc_factor_elastic(om.h)
+    e_fact = entry_rel(r["elMatLTfact"], om.getd("elMatLT"), floor=1e-6)
+    print(f"csr {e_csr:.3e} profile {e_prof:.3e} factor {e_fact:.3e}", flush=True)
+    assert e_csr < 1e-12 and e_prof < 1e-12
+    assert e_fact < 1e-8
+    om2 = O.OracleModel(m).prep().static_elastic(mode=1, rtol=1e-13)
+    assert rel_norm(r["nodeDisp"], om2.getd("nodeDisp")) < 1e-10
