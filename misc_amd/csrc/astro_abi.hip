@@ -292,6 +292,9 @@ struct DeviceProfile {
     // x = P^-1 b (forward then backward sweep); z keeps L^-1 b.  Async.
     int apply(const double* b, double* x, hipStream_t s) {
         ++fwdEpoch;
+        // the sweeps poll the neighbouring tile's values: mark them unproduced
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, z.p, (int64_t)n);
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, x, (int64_t)n);
         hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
         if (timing) {
             evKind.push_back(0);

@@ -33,6 +33,7 @@ __global__ void k_max_abs(const double* v, int64_t n, unsigned long long* out);
 __global__ void k_scale_mpc(double* cmat, double* rhs, const int* crange, int cdim, const unsigned long long* maxBits);
 __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_fill_i32(int* p, int64_t n, int v);
+__global__ void k_fill_sentinel(double* p, int64_t n);
 __global__ void k_csr_to_profile(const double* val, const int* cols, const int* range, int n, const int* sfirst,
                                  const int64_t* rowoff, double* lt);
 __global__ void k_penalty_to_profile(const double* cmat, const int* ccols, const int* crange, int cdim,

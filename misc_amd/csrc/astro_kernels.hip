@@ -11,6 +11,10 @@
 
 namespace astro {
 
+// Values not yet produced in the current sweep hold this signalling-NaN bit
+// pattern; arithmetic never yields it.
+#define ASTRO_SENTINEL 0x7ff4d1c3a5b6c7e9LL
+
 // ---------------------------------------------------------------------------
 // Element tangents: brick8, 2x2x2 Gauss, B^T C B (+ massFactor * rho N^T N),
 // one 64-lane wavefront per element, lane (a, b) owning the 3x3 block of node
@@ -245,6 +249,12 @@ __global__ void k_scale_mpc(double* __restrict__ cmat, double* __restrict__ rhs,
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_sentinel(double* __restrict__ p, int64_t n) {
+    const double v = __longlong_as_double(ASTRO_SENTINEL);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
@@ -527,12 +537,19 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
         for (int j = 0; j < np; ++j) {
             const double d = Dg[j][j];
             if (tid == 0 && d == 0.0 && tile == 0) atomicMin(zeroPivot, P.p + j + 1);
-            if (tid < 64) colL[tid] = (tid > j && tid < np) ? Dg[tid][j] / d : 0.0;
+            const double rinv = 1.0 / d;
+            if (tid < 64) colL[tid] = (tid > j && tid < np) ? Dg[tid][j] * rinv : 0.0;
             __syncthreads();
             if (fi > j && fi < np) {
                 const double l = colL[fi];
                 const double f = l * d;
-                for (int k = j + 1 + fq; k <= fi; k += 4) Dg[fi][k] -= f * colL[k];
+                // columns k = j+1+fq+4m <= fi, unrolled with predication so
+                // the LDS loads of all 16 candidates are in flight together
+#pragma unroll
+                for (int m4 = 0; m4 < 16; ++m4) {
+                    const int k = j + 1 + fq + 4 * m4;
+                    if (k <= fi) Dg[fi][k] -= f * colL[k];
+                }
                 if (fq == 0) Dg[fi][j] = l;
             }
             __syncthreads();
@@ -561,8 +578,14 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
         Y[row][j] = aij;
     }
     for (int j = 0; j < np; ++j) {
-        double part = 0.0;
-        for (int k = q; k < j; k += 4) part += Dg[j][k] * Y[row][k];
+        double pa = 0.0, pb = 0.0;
+#pragma unroll
+        for (int m4 = 0; m4 < 16; m4 += 2) {
+            const int k0 = q + 4 * m4, k1 = k0 + 4;
+            if (k0 < j) pa += Dg[j][k0] * Y[row][k0];
+            if (k1 < j) pb += Dg[j][k1] * Y[row][k1];
+        }
+        double part = pa + pb;
         part += __shfl_xor(part, 1);
         part += __shfl_xor(part, 2);
         if ((j & 3) == q) Y[row][j] -= part;
@@ -598,7 +621,8 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 // stores and read with agent-scope relaxed atomic loads, which go to the
 // device-coherent level instead of the per-XCD L2.  The producer waits for
 // its value stores to complete (vmcnt 0) before it stores the flag, so a
-// consumer that has seen the flag reads the values.  Agent-scope
+// consumer that has seen the flag reads the values; 64-bit stores are
+// single-copy atomic, so a polled value is either the sentinel or final.  Agent-scope
 // acquire/release would instead write back and invalidate the whole L2 of the
 // XCD at every hand-off.
 __device__ __forceinline__ double coherent_load(const double* p) {
@@ -609,59 +633,66 @@ __device__ __forceinline__ void coherent_store(double* p, double v) {
 }
 __device__ __forceinline__ void stores_complete() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-__device__ __forceinline__ bool wait_flag(const int* flag, int epoch, long spinLimit, int* timeout) {
-    long it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++it > spinLimit) {
-            atomicExch(timeout, 1);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return true;
-}
+// Each sweep starts with its output vector filled with ASTRO_SENTINEL: a
+// consumer polls the values themselves (one 64-bit load per lane), so the
+// neighbouring tile's hand-off needs no separate flag round trip.  Flags
+// (stored after the tile's value stores have completed) still tell which
+// whole tiles are done, for bulk catch-up.
 
-// Per-wave incremental wait: lane l looks at ticket from + l; tickets of other
-// chains count as done.  Returns the first ticket in [from, last) whose flag
-// is not yet published (last if all are), spinning only while `from` itself
-// is pending.  Returns -1 if the bounded wait expired.
-__device__ __forceinline__ int wave_poll(const int* flags, const SweepTile* tiles, int chain, int from, int last,
-                                         int epoch, long spinLimit, int* timeout) {
+// Poll v[base + lane] (lane < cnt) until every lane holds a produced value;
+// lanes >= cnt return 0.  `backoff` (wave-uniform, 0..12) adds that many
+// ~1000-cycle sleeps between polls: waiters whose own result is many tiles
+// away poll rarely, which keeps the few freshly written lines every waiter
+// wants from being hammered.  Returns false if the bounded wait expired.
+__device__ __forceinline__ bool poll_values(const double* v, int base, int cnt, int backoff, long spinLimit,
+                                            int* timeout, double& out) {
     const int lane = threadIdx.x & 63;
     long it = 0;
     while (true) {
+        out = lane < cnt ? coherent_load(&v[base + lane]) : 0.0;
+        const bool pending = lane < cnt && __double_as_longlong(out) == ASTRO_SENTINEL;
+        if (__ballot(pending) == 0ull) return true;
+        if (++it > spinLimit) {
+            if (lane == 0) atomicExch(timeout, 1);
+            return false;
+        }
+        for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(16);
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Non-spinning scan: the first ticket in [from, last) of `chain` whose flag
+// does not hold `epoch` (last if none).  Tickets of other chains count as
+// done.
+__device__ __forceinline__ int first_pending(const int* flags, const SweepTile* tiles, int chain, int from, int last,
+                                             int epoch) {
+    const int lane = threadIdx.x & 63;
+    while (from < last) {
         const int J = from + lane;
         bool done = true;
         if (J < last && tiles[J].chain == chain)
             done = __hip_atomic_load(&flags[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
         const unsigned long long pend = __ballot(!done);
-        if (pend == 0ull) {
-            const int nxt = from + 64;
-            if (nxt >= last) return last;
-            from = nxt;
-            continue;
-        }
-        const int firstPending = from + (int)__builtin_ctzll(pend);
-        if (firstPending > from) return firstPending;
-        if (++it > spinLimit) {
-            if (lane == 0) atomicExch(timeout, 1);
-            return -1;
-        }
-        __builtin_amdgcn_s_sleep(1);
+        if (pend) return from + (int)__builtin_ctzll(pend);
+        from += 64;
     }
+    return last;
 }
 
-// Wait until every tile of `chain` with ticket in [first, last) has published
-// `epoch`: the 64 lanes of wave 0 poll distinct flags, then one barrier.
-// Returns false (for every thread) if a wait timed out.
-__device__ __forceinline__ bool wait_range(const int* flags, const SweepTile* tiles, int chain, int first, int last,
-                                           int epoch, long spinLimit, int* timeout) {
-    bool ok = true;
-    if (threadIdx.x < 64) {
-        for (int J = first + (int)threadIdx.x; J < last && ok; J += 64)
-            if (tiles[J].chain == chain) ok = wait_flag(&flags[J], epoch, spinLimit, timeout);
-    }
-    return __syncthreads_or(!ok) == 0;
+// Next ticket after J of the same chain, or `last`.
+__device__ __forceinline__ int next_in_chain(const SweepTile* tiles, int chain, int J, int last) {
+    int K = J + 1;
+    while (K < last && tiles[K].chain != chain) ++K;
+    return K;
+}
+
+// Waiters closer than this many tiles (of their chain) to the tile they
+// wait for follow the chain tile by tile without back-off.
+#define ASTRO_NEAR_TILES 4
+
+__device__ __forceinline__ int backoff_for(int chainDist) {
+    const int b = chainDist - ASTRO_NEAR_TILES;
+    return b < 0 ? 0 : (b > 12 ? 12 : b);
 }
 
 __device__ __forceinline__ void publish_flag(int* flag, int epoch) {
@@ -685,7 +716,6 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
     __shared__ double Ld[64][65];   // Ld[i][c] = L(r0+i, r0+c), c < i
     __shared__ double Lp[64][65];   // Lp[i][c] = L(r0+i, p0+c), preceding tile columns
     __shared__ double part[4][16];
-    __shared__ double zp[64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
@@ -700,10 +730,14 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
         const int i = id >> 6, c = id & 63;
         double v = 0.0, u = 0.0;
         if (i < nr) {
+            // loads at indices clamped into row gi's stored range, then
+            // selects: the loads of all columns are in flight together
             const int gi = r0 + i, si = sfirst[gi];
             const int64_t off = rowoff[gi];
-            if (c < i && r0 + c >= si) v = lt[off + r0 + c];
-            if (c < np && p0 + c >= si) u = lt[off + p0 + c];
+            const double vv = lt[off + min(max(r0 + c, si), gi)];
+            const double uu = lt[off + min(max(p0 + c, si), gi)];
+            v = (c < i && r0 + c >= si) ? vv : 0.0;
+            u = (c < np && p0 + c >= si) ? uu : 0.0;
         }
         Ld[i][c] = v;
         Lp[i][c] = u;
@@ -728,18 +762,45 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
         const int cEnd = p0;
         int c0 = tiles[T.depFirst].r0;
         int J = T.depFirst;
-        while (c0 < cEnd) {
-            const int pend = wave_poll(flags, tiles, T.chain, J, lastFar, epoch, spinLimit, timeout);
-            if (pend < 0) return;
-            const int cStop = pend >= lastFar ? cEnd : tiles[pend].r0;
-            for (int c = c0 + lane; c < cStop; c += 64) {
-                const double zc = coherent_load(&z[c]);
+        bool front = false;
+        while (c0 < cEnd && J < lastFar) {
+            if (!front) {
+                // bulk: every column of the published prefix of far tiles
+                const int pend = first_pending(flags, tiles, T.chain, J, lastFar, epoch);
+                const int cStop = pend >= lastFar ? cEnd : tiles[pend].r0;
+                for (int c = c0 + lane; c < cStop; c += 64) {
+                    const double zc = coherent_load(&z[c]);
+                    // unconditional loads (a column left of a row's first
+                    // stored column addresses earlier rows' entries, still
+                    // inside the profile), then a select: no per-row branch
+                    // and wait
+                    double lv[16];
 #pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    if (c >= rs[r]) p[r] += lt[ro[r] + c] * zc;
+                    for (int r = 0; r < 16; ++r) lv[r] = lt[ro[r] + c];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) p[r] += (c >= rs[r] ? lv[r] : 0.0) * zc;
+                }
+                c0 = cStop;
+                J = pend;
+                if (pend >= lastFar) break;
             }
-            c0 = cStop;
-            J = pend;
+            // tile J is not known to be done: load this wave's coupling
+            // entries to its columns first, then wait for its values
+            const SweepTile Q = tiles[J];
+            const int c = Q.r0 + lane;
+            double lv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) lv[r] = lt[ro[r] + c];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) lv[r] = (c < Q.r1 && c >= rs[r]) ? lv[r] : 0.0;
+            const int dist = (r0 - Q.r0) >> 6;
+            double zc;
+            if (!poll_values(z, Q.r0, Q.r1 - Q.r0, backoff_for(dist), spinLimit, timeout, zc)) return;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) p[r] += lv[r] * zc;
+            c0 = Q.r1;
+            J = next_in_chain(tiles, T.chain, J, lastFar);
+            front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
 #pragma unroll
@@ -754,16 +815,14 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
         double a = 0.0;
         if (i < nr) a = b[r0 + i] - part[i >> 4][i & 15];
         if (hasPrev) {
-            bool ok = true;
-            if (i == 0) ok = wait_flag(&flags[T.depLast], epoch, spinLimit, timeout);
-            ok = __shfl(ok ? 1 : 0, 0) != 0;
-            if (!ok) return;
-            zp[i] = i < np ? coherent_load(&z[p0 + i]) : 0.0;
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            double s2 = 0.0;
-            for (int c = 0; c < np; ++c) s2 += Lp[i][c] * zp[c];
-            a -= s2;
+            double zv;
+            if (!poll_values(z, p0, np, 0, spinLimit, timeout, zv)) return;
+            // Lp is zero beyond np and lanes >= np hold 0: a fixed-trip
+            // unrolled product with the values broadcast from registers
+            double s2[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < 64; ++c) s2[c & 3] += Lp[i][c] * lane_bcast(zv, c);
+            a -= (s2[0] + s2[1]) + (s2[2] + s2[3]);
         }
         double lrow[64];
 #pragma unroll
@@ -794,7 +853,6 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     __shared__ double Ld[64][65];   // Ld[c][i] = L(r0+c, r0+i), i < c
     __shared__ double Ln[64][65];   // Ln[j][i] = L(q0+j, r0+i), rows of the following tile
     __shared__ double part[4][64];
-    __shared__ double xn[64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
@@ -808,15 +866,17 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     for (int id = tid; id < 64 * 64; id += 256) {
         const int c = id >> 6, i = id & 63;
         double v = 0.0, u = 0.0;
-        if (i < nr) {
-            if (c < nr && i < c) {
-                const int gc = r0 + c;
-                if (r0 + i >= sfirst[gc]) v = lt[rowoff[gc] + r0 + i];
-            }
-            if (c < nq) {
-                const int gj = q0 + c;
-                if (r0 + i >= sfirst[gj]) u = lt[rowoff[gj] + r0 + i];
-            }
+        // c is uniform across the wave; loads at indices clamped into the
+        // stored range of row r0+c (q0+c), then selects
+        if (c < nr) {
+            const int gc = r0 + c, sc = sfirst[gc];
+            const double vv = lt[rowoff[gc] + min(max(r0 + i, sc), gc)];
+            v = (i < nr && i < c && r0 + i >= sc) ? vv : 0.0;
+        }
+        if (c < nq) {
+            const int gj = q0 + c, sj = sfirst[gj];
+            const double uu = lt[rowoff[gj] + min(max(r0 + i, sj), gj)];
+            u = (i < nr && r0 + i >= sj) ? uu : 0.0;
         }
         Ld[c][i] = v;
         Ln[c][i] = u;
@@ -824,6 +884,8 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     double p = 0.0;
     const int ci = r0 + lane;
     const bool lv = lane < nr;
+    // an in-range column for idle lanes' loads; their partial sums are unused
+    const int cl = lv ? ci : r0;
     const int lastFar = hasNext ? T.depLast : t;
     // Far rows: from the tile after the following one up to the last tile
     // holding a row that reaches this tile.  Published tiles form a top-down
@@ -833,32 +895,65 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
         const int jLow = Nx.r1;
         int j1 = tiles[T.depFirst].r1;  // rows [jStop, j1) still to do, top-down
         int J = T.depFirst;
-        while (j1 > jLow) {
-            const int pend = wave_poll(flags, tiles, T.chain, J, lastFar, epoch, spinLimit, timeout);
-            if (pend < 0) return;
-            const int jStop = pend >= lastFar ? jLow : tiles[pend].r1;
-            // wave w takes 16-row groups jb = jStop + 16 w + 64 q; lanes 0..15
-            // fetch the group's first columns, offsets and x, broadcast below
-            for (int jb = jStop + 16 * w; jb < j1; jb += 64) {
-                int sjl = 0x7fffffff;
-                long long rol = 0;
-                double xl = 0.0;
-                if (lane < 16 && jb + lane < j1) {
-                    sjl = sfirst[jb + lane];
-                    rol = rowoff[jb + lane];
-                    xl = coherent_load(&x[jb + lane]);
-                }
+        bool front = false;
+        while (j1 > jLow && J < lastFar) {
+            if (!front) {
+                // bulk: every row of the published prefix of far tiles; wave
+                // w takes 16-row groups jb = jStop + 16 w + 64 q, lanes 0..15
+                // fetch the group's first columns, offsets and x, broadcast
+                const int pend = first_pending(flags, tiles, T.chain, J, lastFar, epoch);
+                const int jStop = pend >= lastFar ? jLow : tiles[pend].r1;
+                for (int jb = jStop + 16 * w; jb < j1; jb += 64) {
+                    int sjl = 0x7fffffff;
+                    long long rol = 0;
+                    double xl = 0.0;
+                    if (lane < 16 && jb + lane < j1) {
+                        sjl = sfirst[jb + lane];
+                        rol = rowoff[jb + lane];
+                        xl = coherent_load(&x[jb + lane]);
+                    }
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int sj = __builtin_amdgcn_readlane(sjl, r);
-                    const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
-                                         (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
-                    const double xj = lane_bcast(xl, r);
-                    if (lv && ci >= sj) p += lt[ro + ci] * xj;
+                    for (int r = 0; r < 16; ++r) {
+                        const int sj = __builtin_amdgcn_readlane(sjl, r);
+                        const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
+                                             (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
+                        const double xj = lane_bcast(xl, r);
+                        const double v = lt[ro + cl];
+                        p += (ci >= sj ? v : 0.0) * xj;
+                    }
                 }
+                j1 = jStop;
+                J = pend;
+                if (pend >= lastFar) break;
             }
-            j1 = jStop;
-            J = pend;
+            // tile J is not known to be done: this wave's 16 of its rows,
+            // coupling entries loaded first, then wait for their values
+            const SweepTile Q = tiles[J];
+            const int jb = Q.r0 + 16 * w;
+            const int cnt = max(0, min(16, Q.r1 - jb));
+            int sjl = 0x7fffffff;
+            long long rol = 0;
+            if (lane < cnt) {
+                sjl = sfirst[jb + lane];
+                rol = rowoff[jb + lane];
+            }
+            double lc[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int sj = __builtin_amdgcn_readlane(sjl, r);
+                const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
+                                     (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
+                const double v = lt[ro + cl];
+                lc[r] = ci >= sj ? v : 0.0;
+            }
+            const int dist = (Q.r0 - r0) >> 6;
+            double xl;
+            if (!poll_values(x, jb, cnt, backoff_for(dist), spinLimit, timeout, xl)) return;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) p += lc[r] * lane_bcast(xl, r);
+            j1 = Q.r0;
+            J = next_in_chain(tiles, T.chain, J, lastFar);
+            front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
     part[w][lane] = p;
@@ -871,16 +966,12 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
             a = z[gi] / lt[rowoff[gi] + gi] - (part[0][i] + part[1][i] + part[2][i] + part[3][i]);
         }
         if (hasNext) {
-            bool ok = true;
-            if (i == 0) ok = wait_flag(&flags[T.depLast], epoch, spinLimit, timeout);
-            ok = __shfl(ok ? 1 : 0, 0) != 0;
-            if (!ok) return;
-            xn[i] = i < nq ? coherent_load(&x[q0 + i]) : 0.0;
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            double s2 = 0.0;
-            for (int j = 0; j < nq; ++j) s2 += Ln[j][i] * xn[j];
-            a -= s2;
+            double xv;
+            if (!poll_values(x, q0, nq, 0, spinLimit, timeout, xv)) return;
+            double s2[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < 64; ++j) s2[j & 3] += Ln[j][i] * lane_bcast(xv, j);
+            a -= (s2[0] + s2[1]) + (s2[2] + s2[3]);
         }
         double lcol[64];
 #pragma unroll
