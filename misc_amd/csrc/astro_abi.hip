@@ -91,6 +91,7 @@ struct DeviceProfile {
     int nBlocks = 0, panelsPerBlock = 0, splitK = 1, scratchRows = 64, maxTiles = 1;
     DBuf<PanelInfo> panels;
     DBuf<double> scratch;
+    DBuf<double> diag;  // D of the factor, written by step FT, read by step U
     DBuf<int> zeroPivot;
     // sweeps
     int numTiles = 0;
@@ -193,6 +194,7 @@ struct DeviceProfile {
         ASTRO_TRY(panels.upload(hp, s));
         ASTRO_TRY(scratch.alloc((size_t)splitK * nBlocks * scratchRows * 64));
         ASTRO_TRY(zeroPivot.alloc(1));
+        ASTRO_TRY(diag.alloc((size_t)std::max(1, n)));
         // sweep tiles, chains interleaved in ticket order
         std::vector<std::vector<int>> chainTiles(nBlocks);  // tile start rows per chain
         int maxCount = 0;
@@ -277,9 +279,9 @@ struct DeviceProfile {
                                    rowoff.p, sfirst.p, scratch.p, scratchRows);
             else
                 hipLaunchKernelGGL(k_ldl_update_mfma, gu, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK,
-                                   lt.p, rowoff.p, sfirst.p, scratch.p, scratchRows);
+                                   lt.p, rowoff.p, sfirst.p, diag.p, scratch.p, scratchRows);
             hipLaunchKernelGGL(k_ldl_factor_panel, gf, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK,
-                               lt.p, rowoff.p, sfirst.p, scratch.p, scratchRows, zeroPivot.p);
+                               lt.p, rowoff.p, sfirst.p, scratch.p, scratchRows, diag.p, zeroPivot.p);
         }
         HIP_TRY(hipGetLastError());
         int zp = 0;

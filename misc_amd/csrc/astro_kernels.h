@@ -39,14 +39,14 @@ __global__ void k_csr_to_profile(const double* val, const int* cols, const int* 
 __global__ void k_penalty_to_profile(const double* cmat, const int* ccols, const int* crange, int cdim,
                                      const int* sfirst, const int64_t* rowoff, double* lt);
 __global__ void k_ldl_update_mfma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
-                                  double* lt, const int64_t* rowoff, const int* sfirst, double* scratch,
-                                  int scratchRows);
+                                  double* lt, const int64_t* rowoff, const int* sfirst, const double* diag,
+                                  double* scratch, int scratchRows);
 __global__ void k_ldl_update_fma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
                                  double* lt, const int64_t* rowoff, const int* sfirst, double* scratch,
                                  int scratchRows);
 __global__ void k_ldl_factor_panel(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
                                    double* lt, const int64_t* rowoff, const int* sfirst, const double* scratch,
-                                   int scratchRows, int* zeroPivot);
+                                   int scratchRows, double* diag, int* zeroPivot);
 __global__ void k_sweep_forward(const SweepTile* tiles, int numTiles, const double* lt, const int64_t* rowoff,
                                 const int* sfirst, const double* b, double* z, int* flags, int* ticket, int epoch,
                                 long spinLimit, int* timeout);

@@ -321,6 +321,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __restrict__ panels, int panelIdx, int numPanelsPerBlock,
                                                     int splitK, double* __restrict__ lt,
                                                     const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
+                                                    const double* __restrict__ diag,
                                                     double* __restrict__ scratch, int scratchRows) {
     const int blk = blockIdx.y;
     const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
@@ -351,26 +352,37 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
     const int srow = tid >> 2, sk = (tid & 3) * 8;  // 64 rows x 4 groups of 8 k
     const bool aLive = srow < nr, bLive = srow < np;
     const int aRow = r0 + srow, bRow = P.p + srow;
-    const int aS = aLive ? sfirst[aRow] : 0x7fffffff, bS = bLive ? sfirst[bRow] : 0x7fffffff;
-    const int64_t aOff = aLive ? rowoff[aRow] : 0, bOff = bLive ? rowoff[bRow] : 0;
+    // dead staging rows read a valid row (r0, P.p) and are masked to zero
+    const int aR = aLive ? aRow : r0, bR = bLive ? bRow : P.p;
+    const int aS0 = sfirst[aR], bS0 = sfirst[bR];
+    const int64_t aOff = rowoff[aR], bOff = rowoff[bR];
+    const int aS = aLive ? aS0 : 0x7fffffff, bS = bLive ? bS0 : 0x7fffffff;
     double4_t acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
     // register double buffer: the next chunk's operands are fetched while the
-    // current one runs on the matrix cores
+    // current one runs on the matrix cores.  Loads go to indices clamped into
+    // the row's stored range and are masked afterwards, so all of them are in
+    // flight together (no per-entry branch and memory wait).
     double ra[8], rb[8], rd = 0.0;
     auto fetch = [&](int k0) {
+        double va[8], vb[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + sk + q;
+            va[q] = lt[aOff + min(max(k, aS0), aR)];
+            vb[q] = lt[bOff + min(max(k, bS0), bR)];
+        }
+        const int kd = k0 + (tid & (KC - 1));
+        const double dv = diag[min(kd, ke - 1)];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int k = k0 + sk + q;
             const bool kv = k < ke;
-            ra[q] = (kv && k >= aS) ? lt[aOff + k] : 0.0;
-            rb[q] = (kv && k >= bS) ? lt[bOff + k] : 0.0;
+            ra[q] = (kv && k >= aS) ? va[q] : 0.0;
+            rb[q] = (kv && k >= bS) ? vb[q] : 0.0;
         }
-        if (tid < KC) {
-            const int k = k0 + tid;
-            rd = k < ke ? lt[rowoff[k] + k] : 0.0;
-        }
+        rd = kd < ke ? dv : 0.0;
     };
     if (kb < ke) fetch(kb);
     for (int k0 = kb; k0 < ke; k0 += KC) {
@@ -494,12 +506,43 @@ __global__ __launch_bounds__(256) void k_ldl_update_fma(const PanelInfo* __restr
     }
 }
 
+// Broadcast lane S of each group of four lanes (DPP quad permute, no LDS).
+template <int S>
+__device__ __forceinline__ double quad_bcast(double v) {
+    constexpr int ctrl = S | (S << 2) | (S << 4) | (S << 6);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double quad_bcast_rt(double v, int s) {
+    switch (s & 3) {
+        case 0: return quad_bcast<0>(v);
+        case 1: return quad_bcast<1>(v);
+        case 2: return quad_bcast<2>(v);
+        default: return quad_bcast<3>(v);
+    }
+}
+
+// Step FT.  Thread (r = tid / 4, q = tid % 4) holds 16 entries of one row in
+// registers: columns q, q+4, ..., q+60 of the panel.
+//  1. gather: A(row, P) minus the split-K partial sums of step U; loads are
+//     issued at indices clamped into the row's stored range and masked after
+//     (no per-entry branch and memory wait);
+//  2. every workgroup factors the 64x64 diagonal block (right-looking, one
+//     barrier per column, the column double-buffered in LDS); tile 0 writes
+//     it back and records D in `diag`;
+//  3. tiles >= 1 solve their rows against the unit lower factor: y_j is
+//     final at step j and reaches the other three lanes of its row by a DPP
+//     quad broadcast; L(i, j) = y_j / D_j.
+// Both loops are fully unrolled, so register indices and the lane that owns
+// column j are compile-time constants.
 __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __restrict__ panels, int panelIdx,
                                                           int numPanelsPerBlock, int splitK, double* __restrict__ lt,
                                                           const int64_t* __restrict__ rowoff,
                                                           const int* __restrict__ sfirst,
                                                           const double* __restrict__ scratch, int scratchRows,
-                                                          int* __restrict__ zeroPivot) {
+                                                          double* __restrict__ diag, int* __restrict__ zeroPivot) {
     const int blk = blockIdx.y;
     const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
     if (P.p < 0) return;
@@ -508,92 +551,107 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
     if (r0 >= P.iend) return;
     const int nr = min(64, P.iend - r0);
     const int np = P.pe - P.p;
-    __shared__ double Dg[64][65];
-    __shared__ double Y[64][65];
-    const int tid = threadIdx.x;
-    const size_t nb = gridDim.y;
-    auto wsum = [&](int rowInPanel, int j) {
-        double s = 0.0;
-        for (int q = 0; q < splitK; ++q) s += scratch[(((size_t)q * nb + blk) * scratchRows + rowInPanel) * 64 + j];
-        return s;
-    };
-    // updated diagonal block (lower triangle; zeros outside the profile)
-    for (int id = tid; id < 64 * 64; id += 256) {
-        const int i = id >> 6, j = id & 63;
-        double v = 0.0;
-        if (i < np && j <= i) {
-            const int gi = P.p + i, gj = P.p + j;
-            if (gj >= sfirst[gi]) v = lt[rowoff[gi] + gj] - wsum(i, j);
-        }
-        Dg[i][j] = v;
-    }
-    __syncthreads();
-    // right-looking factor of the 64x64 block: column j is scaled into colL,
-    // then thread (i = tid / 4, q = tid % 4) updates row i at columns
-    // k = j+1+q, j+5+q, ... <= i
-    {
-        __shared__ double colL[64];
-        const int fi = tid >> 2, fq = tid & 3;
-        for (int j = 0; j < np; ++j) {
-            const double d = Dg[j][j];
-            if (tid == 0 && d == 0.0 && tile == 0) atomicMin(zeroPivot, P.p + j + 1);
-            const double rinv = 1.0 / d;
-            if (tid < 64) colL[tid] = (tid > j && tid < np) ? Dg[tid][j] * rinv : 0.0;
-            __syncthreads();
-            if (fi > j && fi < np) {
-                const double l = colL[fi];
-                const double f = l * d;
-                // columns k = j+1+fq+4m <= fi, unrolled with predication so
-                // the LDS loads of all 16 candidates are in flight together
+    __shared__ double Lc[64][65];  // factored diagonal block: L below, D on the diagonal
+    __shared__ double colb[2][64];
+    const int tid = threadIdx.x, fr = tid >> 2, fq = tid & 3;
+    const size_t qstride = (size_t)gridDim.y * scratchRows * 64;
+
+    // gather row `rowInPanel` of the updated panel block; entry m is column
+    // k = fq + 4 m of the panel, kept where live && k < kmax && inside the profile
+    auto gather = [&](int gi, int rowInPanel, bool live, int kmax, double (&a)[16], int& si, int64_t& off) {
+        const int gr = live ? gi : P.p;
+        si = sfirst[gr];
+        off = rowoff[gr];
+        const int rp = live ? rowInPanel : 0;
+        double raw[16], ws[16];
 #pragma unroll
-                for (int m4 = 0; m4 < 16; ++m4) {
-                    const int k = j + 1 + fq + 4 * m4;
-                    if (k <= fi) Dg[fi][k] -= f * colL[k];
-                }
-                if (fq == 0) Dg[fi][j] = l;
-            }
+        for (int m = 0; m < 16; ++m) {
+            const int c = P.p + fq + 4 * m;
+            raw[m] = lt[off + min(max(c, si), gr)];
+            ws[m] = 0.0;
+        }
+        const double* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
+        for (int q = 0; q < splitK; ++q) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) ws[m] += sp[(size_t)q * qstride + 4 * m];
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int k = fq + 4 * m;
+            a[m] = (live && k < kmax && P.p + k >= si) ? raw[m] - ws[m] : 0.0;
+        }
+    };
+
+    // --- 1-2: the diagonal block, row fr
+    double a[16];
+    int dsi;
+    int64_t doff;
+    gather(P.p + fr, fr, fr < np, fr + 1, a, dsi, doff);
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        if (j < np) {
+            double* cb = colb[j & 1];
+            if (fq == (j & 3)) cb[fr] = a[j >> 2];  // A(fr, j) after steps < j
             __syncthreads();
+            const double d = cb[j];
+            if (d == 0.0 && tid == 0 && tile == 0) atomicMin(zeroPivot, P.p + j + 1);
+            const double li = cb[fr] / d;  // L(fr, j) for fr > j
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                if (4 * m + 3 > j) {
+                    const int k = 4 * m + fq;
+                    const double ck = cb[k];  // A(k, j) = L(k, j) D_j
+                    const double upd = a[m] - li * ck;
+                    a[m] = (k > j && k <= fr) ? upd : a[m];
+                }
+            }
+            if (fq == (j & 3) && fr > j) a[j >> 2] = li;
         }
     }
     if (tile == 0) {
-        for (int id = tid; id < 64 * 64; id += 256) {
-            const int i = id >> 6, j = id & 63;
-            if (i < np && j <= i) {
-                const int gi = P.p + i, gj = P.p + j;
-                if (gj >= sfirst[gi]) lt[rowoff[gi] + gj] = Dg[i][j];
+        const int gi = P.p + fr;
+        if (fr < np) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int k = fq + 4 * m;
+                if (k <= fr && P.p + k >= dsi) lt[doff + P.p + k] = a[m];
+                if (k == fr) diag[gi] = a[m];
             }
         }
         return;
     }
-    // rows below the panel: y = L_PP^-1 a, L(i, j) = y_j / D_j; 4 lanes per row.
-    // The updated a(i, P) is gathered first (all loads in flight together).
-    const int row = tid >> 2, q = tid & 3;
-    const int gi = r0 + row;
-    const bool live = row < nr;
-    const int si = live ? sfirst[gi] : 0x7fffffff;
-    for (int j = q; j < 64; j += 4) {
-        const int gj = P.p + j;
-        double aij = 0.0;
-        if (live && j < np && gj >= si) aij = lt[rowoff[gi] + gj] - wsum(r0 - P.p + row, j);
-        Y[row][j] = aij;
-    }
-    for (int j = 0; j < np; ++j) {
-        double pa = 0.0, pb = 0.0;
 #pragma unroll
-        for (int m4 = 0; m4 < 16; m4 += 2) {
-            const int k0 = q + 4 * m4, k1 = k0 + 4;
-            if (k0 < j) pa += Dg[j][k0] * Y[row][k0];
-            if (k1 < j) pb += Dg[j][k1] * Y[row][k1];
+    for (int m = 0; m < 16; ++m) Lc[fr][fq + 4 * m] = a[m];
+    __syncthreads();
+
+    // --- 3: rows r0 + fr of this tile
+    const int gi = r0 + fr;
+    const bool live = fr < nr;
+    double y[16];
+    int si;
+    int64_t off;
+    gather(gi, r0 - P.p + fr, live, np, y, si, off);
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        if (j < np) {
+            const double yj = quad_bcast_rt(y[j >> 2], j);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                if (4 * m + 3 > j) {
+                    const int k = 4 * m + fq;
+                    const double lkj = Lc[k][j];
+                    const double upd = y[m] - lkj * yj;
+                    y[m] = k > j ? upd : y[m];
+                }
+            }
         }
-        double part = pa + pb;
-        part += __shfl_xor(part, 1);
-        part += __shfl_xor(part, 2);
-        if ((j & 3) == q) Y[row][j] -= part;
     }
-    // (each lane only reads the Y entries it wrote itself: k == q mod 4)
-    for (int j = q; j < np; j += 4) {
-        const int gj = P.p + j;
-        if (live && gj >= si) lt[rowoff[gi] + gj] = Y[row][j] / Dg[j][j];
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int k = fq + 4 * m;
+            if (k < np && P.p + k >= si) lt[off + P.p + k] = y[m] / Lc[k][k];
+        }
     }
 }
 
