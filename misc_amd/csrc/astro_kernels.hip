@@ -861,11 +861,29 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
             front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
+    // speculative load of the preceding tile's values: usually already
+    // produced, and its latency overlaps the reduction and barrier below
+    double zv = 0.0;
+    if (w == 0 && hasPrev) zv = lane < np ? coherent_load(&z[p0 + lane]) : 0.0;
+    // transpose-reduce the 16 row partials across the wave: each xor stage
+    // halves the rows a lane carries, so lane l ends with row (l >> 2) & 15
+    // summed over all 64 lanes (17 exchanges instead of 16 x 6)
+    {
+        const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+        double q1[8], q2[4], q3[2];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        double v = p[r];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) part[w][r] = v;
+        for (int r = 0; r < 8; ++r)
+            q1[r] = (b5 ? p[r + 8] : p[r]) + __shfl_xor(b5 ? p[r] : p[r + 8], 32);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            q2[r] = (b4 ? q1[r + 4] : q1[r]) + __shfl_xor(b4 ? q1[r] : q1[r + 4], 16);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+            q3[r] = (b3 ? q2[r + 2] : q2[r]) + __shfl_xor(b3 ? q2[r] : q2[r + 2], 8);
+        double q4 = (b2 ? q3[1] : q3[0]) + __shfl_xor(b2 ? q3[0] : q3[1], 4);
+        q4 += __shfl_xor(q4, 2);
+        q4 += __shfl_xor(q4, 1);
+        if ((lane & 3) == 0) part[w][(lane >> 2) & 15] = q4;
     }
     __syncthreads();
     if (w == 0) {
@@ -873,8 +891,9 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
         double a = 0.0;
         if (i < nr) a = b[r0 + i] - part[i >> 4][i & 15];
         if (hasPrev) {
-            double zv;
-            if (!poll_values(z, p0, np, 0, spinLimit, timeout, zv)) return;
+            if (__ballot(i < np && __double_as_longlong(zv) == ASTRO_SENTINEL) &&
+                !poll_values(z, p0, np, 0, spinLimit, timeout, zv))
+                return;
             // Lp is zero beyond np and lanes >= np hold 0: a fixed-trip
             // unrolled product with the values broadcast from registers
             double s2[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1014,6 +1033,8 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
             front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
+    double xv = 0.0;
+    if (w == 0 && hasNext) xv = lane < nq ? coherent_load(&x[q0 + lane]) : 0.0;
     part[w][lane] = p;
     __syncthreads();
     if (w == 0) {
@@ -1024,8 +1045,9 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
             a = z[gi] / lt[rowoff[gi] + gi] - (part[0][i] + part[1][i] + part[2][i] + part[3][i]);
         }
         if (hasNext) {
-            double xv;
-            if (!poll_values(x, q0, nq, 0, spinLimit, timeout, xv)) return;
+            if (__ballot(i < nq && __double_as_longlong(xv) == ASTRO_SENTINEL) &&
+                !poll_values(x, q0, nq, 0, spinLimit, timeout, xv))
+                return;
             double s2[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < 64; ++j) s2[j & 3] += Ln[j][i] * lane_bcast(xv, j);
