@@ -34,6 +34,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def factor_flops(lt_range):
+    """Exact LDL^T flop count of the profile: sum over rows i and stored
+    columns j < i of 2 (j - max(s_i, s_j)).  When the first stored column s
+    never decreases down the rows (true for the level-sorted meshes here) the
+    inner sum is L (L - 1) / 2 with L = i - s_i; returns None otherwise."""
+    import numpy as np
+    R = np.asarray(lt_range, dtype=np.int64)
+    rowlen = np.diff(R)
+    s = np.arange(len(rowlen)) - rowlen + 1
+    if len(s) > 1 and np.any(np.diff(s) < 0):
+        return None
+    L = (rowlen - 1).astype(np.float64)
+    return float(np.sum(L * (L - 1.0)))
 
 
 def cpu_baseline(sample=(16, 16, 8)):
@@ -64,12 +80,16 @@ def cpu_baseline(sample=(16, 16, 8)):
 
 
 def load_traffic():
-    """HBM bytes per sweep launch measured by rocprofv3 PMC runs (FETCH_SIZE,
-    WRITE_SIZE, calibrated against a copy kernel), if recorded."""
-    f = os.path.join(ROOT, "profiles", "sweep_traffic.json")
-    if os.path.exists(f):
-        with open(f) as fh:
-            return json.load(fh).get("bytes_per_launch")
+    """HBM bytes per sweep launch measured by rocprofv3 PMC runs (FETCH_SIZE and
+    WRITE_SIZE, each calibrated against a copy kernel of known size), as
+    recorded in profiles/k_sweep_traffic_brick8_64x64x32.md; None if absent."""
+    f = os.path.join(ROOT, "profiles", "k_sweep_traffic_brick8_64x64x32.md")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
     return None
 
 
@@ -123,6 +143,7 @@ def main():
     fwd_n, bwd_n = ctx.kernel_stat("sweep_fwd_count"), ctx.kernel_stat("sweep_bwd_count")
     ctx.set_timing(False)
     rel_res = ctx.true_residual()
+    flops = factor_flops(ctx.geti("elMatLTRange"))
     value = n * args.steps / elapsed
     # one sweep reads the strict lower profile (or, backward, all of it) and
     # reads b / writes x: 8 bytes per profile entry + 16 per row
@@ -151,10 +172,19 @@ def main():
         "phases_ms_per_step": {k: v / args.steps for k, v in phases.items()},
         "roofline": {"kernel": "k_sweep_forward / k_sweep_backward (preconditioner triangular sweeps)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(), "avg_launch_ms": avg_sweep_ms,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic() if (args.nx, args.ny, args.nz, args.maxbw) == (64, 64, 32, "int64")
+                     else None,
+                     "avg_launch_ms": avg_sweep_ms,
                      "launches": int(fwd_n + bwd_n), "algorithmic_bytes_per_launch": sweep_bytes},
         "cpu_baseline": None,
     }
+    if flops is not None:
+        fac_s = phases["factor"] / args.steps * 1e-3
+        out["roofline_factor"] = {"kernel": "k_ldl_update_mfma + k_ldl_factor_panel (whole factor phase)",
+                                  "bound": "mfma", "achieved": flops / fac_s / 1e12, "peak": FP64_MFMA_PEAK_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": flops / fac_s / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+                                  "flops_per_factor": flops}
     ctx.close()
     if not args.no_cpu_baseline:
         try:
