@@ -179,6 +179,21 @@ int astro_ctx_reset(astro_ctx* ctx);
 /* Whole solve() sequence: assemble, scale, profile, factor, solve. */
 int astro_ctx_static_solve(astro_ctx* ctx);
 
+/* Newmark dynamics, the dynamic branch of solve()
+ * (Working/AStrO_commandFunctions.f90:762-810; element residual
+ * r_getElRu/r_getElRuk/r_getElRum, Working/AStrO_r_elementEqns.f90:2659-2696,
+ * 2424-2458, 2628-2653), with each element gathering its own nodes' solution
+ * (the reference's r_getElementSolution indexes by local node number).
+ * dynamic_init: tangent K + M / (beta dt^2) at rest, penalty scaling, profile,
+ * factor; state (u, v, a and their previous values) zeroed.
+ * dynamic_step: one step with the nodal loads times loadScale; GMRES from the
+ * previous increment to the context's rtol.
+ * dynamic_run: nSteps steps with loadScale[k]; scalar "dynMs" = its time.
+ * Arrays "nodeVel", "nodeAcc", "prevDisp" are readable with astro_ctx_get_f64. */
+int astro_ctx_dynamic_init(astro_ctx* ctx, double beta, double gamma, double dt);
+int astro_ctx_dynamic_step(astro_ctx* ctx, double loadScale);
+int astro_ctx_dynamic_run(astro_ctx* ctx, int nSteps, const double* loadScale);
+
 /* Thermal twins (Working/AStrO_commandFunctions.f90:16-74, 229-258, 689-703,
  * 763-769). */
 int astro_ctx_assemble_thermal(astro_ctx* ctx);

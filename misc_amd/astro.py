@@ -33,7 +33,8 @@ EXPORTS = [
     "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_precond_apply",
     "astro_ctx_operator_apply", "astro_ctx_sync", "astro_ctx_phase_ms", "astro_prep_create", "astro_prep_destroy",
     "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64", "astro_ctx_set_timing",
-    "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual",
+    "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual", "astro_ctx_dynamic_init",
+    "astro_ctx_dynamic_step", "astro_ctx_dynamic_run",
 ]
 
 
@@ -94,6 +95,9 @@ def load_library(path: str = LIB_PATH):
               "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_sync",
               "astro_ctx_reset"):
         getattr(L, f).argtypes = [C.c_void_p]
+    L.astro_ctx_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+    L.astro_ctx_dynamic_step.argtypes = [C.c_void_p, C.c_double]
+    L.astro_ctx_dynamic_run.argtypes = [C.c_void_p, C.c_int, _dp]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_ctx_operator_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_prep_create.argtypes = [C.POINTER(_Model), C.POINTER(_Options), C.POINTER(C.c_void_p)]
@@ -302,7 +306,8 @@ class Context:
             "currentRank": self.model.num_nodes, "nDofIndex": 6 * self.model.num_nodes,
             "elMatRange": n("elMatDim") + 1, "elMatLTRange": n("elMatDim") + 1, "elMatCols": n("elMatSize"),
             "elasticMat": n("elMatSize"), "elMatLT": n("elMatLTSize"), "elasticLoad": n("elMatDim"),
-            "nodeDisp": n("elMatDim"), "delDisp": n("elMatDim"), "thermMatRange": self.model.num_nodes + 1,
+            "nodeDisp": n("elMatDim"), "delDisp": n("elMatDim"), "nodeVel": n("elMatDim"), "nodeAcc": n("elMatDim"),
+            "prevDisp": n("elMatDim"), "thermMatRange": self.model.num_nodes + 1,
             "thermMatLTRange": self.model.num_nodes + 1, "thermMatCols": n("thermMatSize"),
             "thermMat": n("thermMatSize"), "thermMatLT": n("thermMatLTSize"), "thermalLoad": self.model.num_nodes,
             "nodeTemp": self.model.num_nodes, "elMPCMatRange": n("elMPCDim") + 1,
@@ -351,6 +356,18 @@ class Context:
 
     def static_solve(self):
         _check(load_library().astro_ctx_static_solve(self.h))
+
+    def dynamic_init(self, beta: float = 0.25, gamma: float = 0.5, dt: float = 1e-5):
+        """Newmark set-up: tangent K + M/(beta dt^2), penalty scaling, profile,
+        factor; the state starts at rest."""
+        _check(load_library().astro_ctx_dynamic_init(self.h, float(beta), float(gamma), float(dt)))
+
+    def dynamic_step(self, load_scale: float):
+        _check(load_library().astro_ctx_dynamic_step(self.h, float(load_scale)))
+
+    def dynamic_run(self, load_scales):
+        f = np.ascontiguousarray(load_scales, dtype=np.float64)
+        _check(load_library().astro_ctx_dynamic_run(self.h, int(f.size), f))
 
     def assemble_thermal(self):
         _check(load_library().astro_ctx_assemble_thermal(self.h))

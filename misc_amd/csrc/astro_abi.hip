@@ -546,6 +546,12 @@ struct astro_ctx {
     DeviceOperator top;
     Gmres tgm;
     bool thermalReady = false;
+    // Newmark state
+    bool dynReady = false;
+    double nmBeta = 0.25, nmGamma = 0.5, delT = 0.0;
+    DBuf<double> prevDisp, prevVel, prevAcc, nodeVel, nodeAcc;
+    int64_t dynSteps = 0, dynGmresSteps = 0;
+    double dynMs = 0.0;
     // timing
     hipEvent_t ev[8] = {};
     double phaseMs[6] = {0, 0, 0, 0, 0, 0};
@@ -803,6 +809,8 @@ int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
     if (s == "numColors") return (int64_t)c->colorStart.size() - 1;
     if (s == "profileBlocks") return c->prof.nBlocks;
     if (s == "numVecs") return c->numVecs;
+    if (s == "dynSteps") return c->dynSteps;
+    if (s == "dynGmresSteps") return c->dynGmresSteps;
     return -1;
 }
 
@@ -812,6 +820,7 @@ double astro_ctx_scalar_f(const astro_ctx* c, const char* name) {
     if (s == "gmresRes") return c->gm.res;
     if (s == "gmresRes0") return c->gm.res0;
     if (s == "thermGmresRes") return c->tgm.res;
+    if (s == "dynMs") return c->dynMs;
     return NAN;
 }
 
@@ -856,6 +865,9 @@ static double* dev_double_array(astro_ctx* c, const std::string& s, int64_t& len
     if (s == "thermMatLT") return pick(c->tprof.lt, c->tprof.size);
     if (s == "thermalLoad") return pick(c->tLoad, p.numNodes);
     if (s == "nodeTemp") return pick(c->nodeTemp, p.numNodes);
+    if (s == "nodeVel") return pick(c->nodeVel, p.elMatDim);
+    if (s == "nodeAcc") return pick(c->nodeAcc, p.elMatDim);
+    if (s == "prevDisp") return pick(c->prevDisp, p.elMatDim);
     len = -1;
     return nullptr;
 }
@@ -938,7 +950,7 @@ static int ctx_rhs(astro_ctx* c) {
     const int nl = (int)c->loadDofs.n;
     if (nl > 0)
         hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
-                           nl, c->load.p);
+                           nl, 1.0, c->load.p);
     const int cd = c->prep.elMPCDim;
     if (cd > 0)
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
@@ -987,6 +999,83 @@ int astro_ctx_static_solve(astro_ctx* c) {
         HIP_TRY(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
         c->phaseMs[k] = ms;
     }
+    return ASTRO_OK;
+}
+
+// Newmark dynamics: the dynamic branch of solve (Working/AStrO_commandFunctions.f90:
+// 762-810).  Set-up assembles the tangent K + M / (beta dt^2) at u = 0, scales
+// the penalty rows by it, builds the profile and factors it once; the state
+// starts at rest.
+int astro_ctx_dynamic_init(astro_ctx* c, double beta, double gamma, double dt) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (!(beta > 0.0) || !(dt > 0.0) || !(gamma >= 0.0))
+        return fail(ASTRO_ERR_ARG, "Newmark parameters need beta > 0, gamma >= 0, dt > 0");
+    const int n = c->prep.elMatDim;
+    c->nmBeta = beta;
+    c->nmGamma = gamma;
+    c->delT = dt;
+    for (DBuf<double>* b : {&c->prevDisp, &c->prevVel, &c->prevAcc, &c->nodeVel, &c->nodeAcc})
+        if (b->n != (size_t)n) ASTRO_TRY(b->alloc((size_t)n));
+    for (DBuf<double>* b : {&c->prevDisp, &c->prevVel, &c->prevAcc, &c->nodeVel, &c->nodeAcc, &c->nodeDisp, &c->delDisp})
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, b->p, (int64_t)n, 0.0);
+    ASTRO_TRY(astro_ctx_assemble_elastic(c, 1.0 / (beta * dt * dt)));
+    ASTRO_TRY(astro_ctx_scale_mpc(c));
+    ASTRO_TRY(astro_ctx_build_profile(c));
+    ASTRO_TRY(astro_ctx_factor(c));
+    c->dynSteps = c->dynGmresSteps = 0;
+    c->dynMs = 0.0;
+    c->dynReady = true;
+    return ASTRO_OK;
+}
+
+// One Newmark step (stepSolve :704-745 with maxNLIt = 1, then :792-806):
+// load = loadScale * F - sum_e R_e(u) + C^T (rhs - C u); GMRES from the
+// previous increment; u += du; acceleration, velocity and the previous state.
+int astro_ctx_dynamic_step(astro_ctx* c, double loadScale) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (!c->dynReady) return fail(ASTRO_ERR_ARG, "astro_ctx_dynamic_init has not been called");
+    const int n = c->prep.elMatDim, nS = c->prep.numSolidNds;
+    const double beta = c->nmBeta, dt = c->delT, c1 = 0.5 / beta, c2 = 2.0 / (dt * dt);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->load.p, (int64_t)n, 0.0);
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
+                           c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p, nS,
+                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, c->load.p);
+    }
+    const int nl = (int)c->loadDofs.n;
+    if (nl > 0)
+        hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
+                           nl, loadScale, c->load.p);
+    const int cd = c->prep.elMPCDim;
+    if (cd > 0)
+        hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
+                           c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p);
+    HIP_TRY(hipGetLastError());
+    ASTRO_TRY(c->gm.solve(c->op, c->prof, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s));
+    hipLaunchKernelGGL(k_newmark_update, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, n, c->nodeDisp.p,
+                       c->delDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c->nodeVel.p, c->nodeAcc.p, c1, c2, dt,
+                       beta, c->nmGamma);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->s));
+    c->dynSteps += 1;
+    c->dynGmresSteps += c->gm.steps;
+    return ASTRO_OK;
+}
+
+// nSteps Newmark steps with the load factors loadScale[0..nSteps-1]; the
+// elapsed time of the run is kept as scalar "dynMs".
+int astro_ctx_dynamic_run(astro_ctx* c, int nSteps, const double* loadScale) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (nSteps < 0 || (nSteps > 0 && !loadScale)) return fail(ASTRO_ERR_ARG, "bad step count or load factors");
+    HIP_TRY(hipEventRecord(c->ev[6], c->s));
+    for (int k = 0; k < nSteps; ++k) ASTRO_TRY(astro_ctx_dynamic_step(c, loadScale[k]));
+    HIP_TRY(hipEventRecord(c->ev[7], c->s));
+    HIP_TRY(hipEventSynchronize(c->ev[7]));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+    c->dynMs = ms;
     return ASTRO_OK;
 }
 

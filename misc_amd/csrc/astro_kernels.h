@@ -27,6 +27,12 @@ struct SweepTile {
 __global__ void k_brick8_tangent(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                  const double* elScale, const double* matC, const double* elRho, double massFactor,
                                  const int* rank, int nS, const int* csrRange, const uint8_t* elPos, double* val);
+__global__ void k_brick8_residual(const int* perm, int count, const int* conn, const double* X, const int* elMat,
+                                  const double* elScale, const double* matC, const double* elRho, const int* rank,
+                                  int nS, const double* u, const double* pu, const double* pv, const double* pa,
+                                  double c1, double c2, double dt, double beta, double* load);
+__global__ void k_newmark_update(int n, double* u, const double* du, double* pu, double* pv, double* pa, double* v,
+                                 double* a, double c1, double c2, double dt, double beta, double gamma);
 __global__ void k_brick8_conduct(const int* perm, int count, const int* conn, const double* X, const double* elCond,
                                  const int* rank, const int* csrRange, const uint8_t* thPos, double* val);
 __global__ void k_max_abs(const double* v, int64_t n, unsigned long long* out);
@@ -57,7 +63,7 @@ __global__ void k_reset_ticket(int* ticket);
 __global__ void k_spmv(const double* val, const int* cols, const int* range, int n, const double* x, double* y);
 __global__ void k_mpc_apply(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,
                             double* y);
-__global__ void k_nodal_loads(const int* dofs, const double* vals, int count, double* load);
+__global__ void k_nodal_loads(const int* dofs, const double* vals, int count, double scale, double* load);
 __global__ void k_constraint_load(const double* cmat, const int* ccols, const int* crange, const double* crhs,
                                   int cdim, const double* u, double* load);
 __global__ void k_multidot(const double* V, int64_t ldv, int nqTotal, const double* w, int n, double* partial,

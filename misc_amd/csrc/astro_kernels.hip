@@ -15,6 +15,53 @@ namespace astro {
 // pattern; arithmetic never yields it.
 #define ASTRO_SENTINEL 0x7ff4d1c3a5b6c7e9LL
 
+// Gauss point gp (0..7, signs of the natural coordinates from its bits) of a
+// brick8 with nodal coordinates X: shape functions N, their x-derivatives Nx
+// and detJ (the 2x2x2 weights are 1).  Working/AStrO_r_elementEqns.f90:27-36,
+// 110-135 (N, Ns), 212-238 (adjugate inverse), 580-620 (Nx = Ns J^-1).
+__device__ __forceinline__ void brick8_gauss_point(int gp, const double (&sX)[8][3], double (&N)[8],
+                                                   double (&Nx)[8][3], double& w) {
+    const double g = 0.57735026918962576451;  // 1/sqrt(3)
+    const double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
+    double Ns[8][3];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double sg0 = ((n + 1) & 2) ? 1.0 : -1.0;  // nodes 2,3,6,7 at +s1
+        const double sg1 = (n & 2) ? 1.0 : -1.0;        // nodes 3,4,7,8 at +s2
+        const double sg2 = (n & 4) ? 1.0 : -1.0;        // nodes 5-8 at +s3
+        const double f0 = 1.0 + sg0 * s0, f1 = 1.0 + sg1 * s1, f2 = 1.0 + sg2 * s2;
+        N[n] = 0.125 * f0 * f1 * f2;
+        Ns[n][0] = 0.125 * sg0 * f1 * f2;
+        Ns[n][1] = 0.125 * sg1 * f0 * f2;
+        Ns[n][2] = 0.125 * sg2 * f0 * f1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) J[i][j] += sX[n][i] * Ns[n][j];
+    }
+    const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
+    const double di = 1.0 / det;
+    double I[3][3];
+    I[0][0] = di * a11;
+    I[1][0] = di * a12;
+    I[2][0] = di * a13;
+    I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+    I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+    I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+    I[0][2] = di * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+    I[1][2] = di * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
+    I[2][2] = di * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Nx[n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j] + Ns[n][2] * I[2][j];
+    w = det;  // Gauss weight 1
+}
+
 // ---------------------------------------------------------------------------
 // Element tangents: brick8, 2x2x2 Gauss, B^T C B (+ massFactor * rho N^T N),
 // one 64-lane wavefront per element, lane (a, b) owning the 3x3 block of node
@@ -48,48 +95,7 @@ __global__ __launch_bounds__(64) void k_brick8_tangent(
         sX[lane][2] = X[3 * (size_t)nd + 2];
     }
     __syncthreads();
-    if (lane < 8) {
-        const double g = 0.57735026918962576451;  // 1/sqrt(3)
-        const int gp = lane;
-        const double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
-        double Ns[8][3];
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            const double sg0 = ((n + 1) & 2) ? 1.0 : -1.0;  // nodes 2,3,6,7 at +s1
-            const double sg1 = (n & 2) ? 1.0 : -1.0;        // nodes 3,4,7,8 at +s2
-            const double sg2 = (n & 4) ? 1.0 : -1.0;        // nodes 5-8 at +s3
-            const double f0 = 1.0 + sg0 * s0, f1 = 1.0 + sg1 * s1, f2 = 1.0 + sg2 * s2;
-            sN[gp][n] = 0.125 * f0 * f1 * f2;
-            Ns[n][0] = 0.125 * sg0 * f1 * f2;
-            Ns[n][1] = 0.125 * sg1 * f0 * f2;
-            Ns[n][2] = 0.125 * sg2 * f0 * f1;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) J[i][j] += sX[n][i] * Ns[n][j];
-        }
-        const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
-        const double di = 1.0 / det;
-        double I[3][3];
-        I[0][0] = di * a11;
-        I[1][0] = di * a12;
-        I[2][0] = di * a13;
-        I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
-        I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
-        I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
-        I[0][2] = di * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
-        I[1][2] = di * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
-        I[2][2] = di * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-#pragma unroll
-        for (int n = 0; n < 8; ++n)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) sNx[gp][n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j] + Ns[n][2] * I[2][j];
-        sW[gp] = det;  // Gauss weight 1
-    }
+    if (lane < 8) brick8_gauss_point(lane, sX, sN[lane], sNx[lane], sW[lane]);
     __syncthreads();
     const int a = lane >> 3, b = lane & 7;
     const double sc = elScale[e];
@@ -136,6 +142,132 @@ __global__ __launch_bounds__(64) void k_brick8_tangent(
         v[p0] += K[c][0];
         v[p1] += K[c][1];
         v[p2] += K[c][2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element residuals of the Newmark step, brick8: R_e = Ruk(u_e) + M_e a_e with
+// the Green-Lagrange internal force (Working/AStrO_r_elementEqns.f90:2424-2458,
+// strain and derivative r_getNLStrain :622-662, orient = identity, T = 0) and
+// the consistent-mass inertia of a_e = c1 (c2 (u - pu - dt pv) - (1 - 2 beta) pa)
+// (r_getElRu :2684-2690, r_getElRum :2628-2653).  Every element gathers its own
+// nodes' values (the reference's r_getElementSolution indexes by local node
+// number, D4).  load[dof] -= R_e; elements of one launch share no node.
+// One wave per element: lanes 0..7 own a Gauss point, lanes 0..23 a DOF
+// (component c = d / 8, node d % 8).
+__global__ __launch_bounds__(64) void k_brick8_residual(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const int* __restrict__ elMat, const double* __restrict__ elScale, const double* __restrict__ matC,
+    const double* __restrict__ elRho, const int* __restrict__ rank, int nS, const double* __restrict__ u,
+    const double* __restrict__ pu, const double* __restrict__ pv, const double* __restrict__ pa, double c1, double c2,
+    double dt, double beta, double* __restrict__ load) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[8][3];
+    __shared__ double sNx[8][8][3];
+    __shared__ double sN[8][8];
+    __shared__ double sW[8];
+    __shared__ int sNode[8];
+    __shared__ double sU[3][8], sA[3][8];
+    __shared__ double sUx[8][3][3], sS[8][6], sAb[8][3];
+    if (lane < 8) {
+        int nd = conn[8 * (size_t)e + lane] - 1;
+        sNode[lane] = nd;
+        sX[lane][0] = X[3 * (size_t)nd];
+        sX[lane][1] = X[3 * (size_t)nd + 1];
+        sX[lane][2] = X[3 * (size_t)nd + 2];
+    }
+    __syncthreads();
+    const int comp = lane >> 3, nd = lane & 7;
+    int dof = 0;
+    if (lane < 24) {
+        dof = rank[sNode[nd]] + comp * nS;  // 1-based
+        const size_t k = dof - 1;
+        const double uk = u[k];
+        sU[comp][nd] = uk;
+        sA[comp][nd] = c1 * (c2 * (uk - pu[k] - dt * pv[k]) - (1.0 - 2.0 * beta) * pa[k]);
+    }
+    if (lane < 8) brick8_gauss_point(lane, sX, sN[lane], sNx[lane], sW[lane]);
+    __syncthreads();
+    if (lane < 8) {
+        const int g = lane;
+        double ux[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int n = 0; n < 8; ++n) acc += sU[i][n] * sNx[g][n][j];
+                ux[i][j] = acc;
+                sUx[g][i][j] = acc;
+            }
+        double E[6];
+        E[0] = ux[0][0] + 0.5 * (ux[0][0] * ux[0][0] + ux[1][0] * ux[1][0] + ux[2][0] * ux[2][0]);
+        E[1] = ux[1][1] + 0.5 * (ux[0][1] * ux[0][1] + ux[1][1] * ux[1][1] + ux[2][1] * ux[2][1]);
+        E[2] = ux[2][2] + 0.5 * (ux[0][2] * ux[0][2] + ux[1][2] * ux[1][2] + ux[2][2] * ux[2][2]);
+        E[3] = ux[0][1] + ux[1][0] + ux[0][0] * ux[0][1] + ux[1][0] * ux[1][1] + ux[2][0] * ux[2][1];
+        E[4] = ux[0][2] + ux[2][0] + ux[0][0] * ux[0][2] + ux[1][0] * ux[1][2] + ux[2][0] * ux[2][2];
+        E[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
+        const double sc = elScale[e];
+        const double* Cm = matC + 36 * (size_t)elMat[e];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc += Cm[r + 6 * k] * E[k];
+            sS[g][r] = sc * acc;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+#pragma unroll
+            for (int n = 0; n < 8; ++n) acc += sN[g][n] * sA[c][n];
+            sAb[g][c] = acc;
+        }
+    }
+    __syncthreads();
+    if (lane >= 24) return;
+    const double rho = elRho[e];
+    const double o0 = comp == 0 ? 1.0 : 0.0, o1 = comp == 1 ? 1.0 : 0.0, o2 = comp == 2 ? 1.0 : 0.0;
+    double R = 0.0;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        const double x0 = sNx[g][nd][0], x1 = sNx[g][nd][1], x2 = sNx[g][nd][2];
+        const double u0 = sUx[g][comp][0], u1 = sUx[g][comp][1], u2 = sUx[g][comp][2];
+        const double dE0 = o0 * x0 + x0 * u0;
+        const double dE1 = o1 * x1 + x1 * u1;
+        const double dE2 = o2 * x2 + x2 * u2;
+        const double dE3 = o0 * x1 + o1 * x0 + x0 * u1 + x1 * u0;
+        const double dE4 = o0 * x2 + o2 * x0 + x0 * u2 + x2 * u0;
+        const double dE5 = o1 * x2 + o2 * x1 + x1 * u2 + x2 * u1;
+        const double* S = sS[g];
+        const double w = sW[g];
+        R += w * (S[0] * dE0 + S[1] * dE1 + S[2] * dE2 + S[3] * dE3 + S[4] * dE4 + S[5] * dE5);
+        R += rho * w * sN[g][nd] * sAb[g][comp];
+    }
+    load[dof - 1] -= R;
+}
+
+// Newmark update after the step's increment du: u += du; the new acceleration
+// and velocity from the previous state; then the previous state takes the new
+// one.  Working/AStrO_commandFunctions.f90:792-806.
+__global__ void k_newmark_update(int n, double* __restrict__ u, const double* __restrict__ du,
+                                 double* __restrict__ pu, double* __restrict__ pv, double* __restrict__ pa,
+                                 double* __restrict__ v, double* __restrict__ a, double c1, double c2, double dt,
+                                 double beta, double gamma) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double un = u[i] + du[i];
+        const double an = c1 * (c2 * (un - pu[i] - dt * pv[i]) - (1.0 - 2.0 * beta) * pa[i]);
+        const double vn = pv[i] + dt * ((1.0 - gamma) * pa[i] + gamma * an);
+        u[i] = un;
+        a[i] = an;
+        v[i] = vn;
+        pu[i] = un;
+        pv[i] = vn;
+        pa[i] = an;
     }
 }
 
@@ -1116,10 +1248,10 @@ __global__ void k_mpc_apply(const double* __restrict__ cmat, const int* __restri
 // RHS pieces: nodal loads and the constraint load C^T (rhs - C u).
 // Working/AStrO_commandFunctions.f90:368-392, 506-535.
 __global__ void k_nodal_loads(const int* __restrict__ dofs, const double* __restrict__ vals, int count,
-                              double* __restrict__ load) {
+                              double scale, double* __restrict__ load) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
-    atomicAdd(&load[dofs[k] - 1], vals[k]);
+    atomicAdd(&load[dofs[k] - 1], scale * vals[k]);
 }
 
 __global__ void k_constraint_load(const double* __restrict__ cmat, const int* __restrict__ ccols,

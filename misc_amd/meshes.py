@@ -168,3 +168,68 @@ def uniform_e_field(num_els, lo, hi, seed):
     """Young's modulus per element drawn i.i.d. uniform in [lo, hi]."""
     rng = np.random.default_rng(seed)
     return rng.uniform(lo, hi, size=num_els)
+
+
+def lognormal_e_field(nx, ny, nz, median, log_sigma, corr_len, seed):
+    """Young's modulus per element, log E = log(median) + log_sigma * g.
+
+    g is white noise on the element grid, filtered by a Gaussian kernel of
+    standard deviation corr_len / 2 elements (periodic edges), then shifted
+    and scaled to zero mean and unit variance.  With that kernel the field's
+    correlation function is exp(-r^2 / corr_len^2), so corr_len is the
+    correlation length in elements.  Element order is x fastest, then y, z,
+    as brick_block numbers them."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((nz, ny, nx))
+    g = gaussian_filter(w, sigma=corr_len / 2.0, mode="wrap")
+    g = (g - g.mean()) / g.std()
+    return np.exp(np.log(median) + log_sigma * g).ravel()
+
+
+def top_patch_loads(nx, ny, nz, lx, ly, lz, px, py, total):
+    """Consistent nodal forces of a uniform traction over a px x py-element
+    patch centred on the top face z = lz, summing to `total` in -z: each
+    patch element adds a quarter of its share to each of its four top nodes."""
+    NX, NY = nx + 1, ny + 1
+    i0, j0 = (nx - px) // 2, (ny - py) // 2
+    share = total / (px * py)
+    f = np.zeros((NY, NX))
+    for j in range(j0, j0 + py):
+        for i in range(i0, i0 + px):
+            f[j:j + 2, i:i + 2] += 0.25 * share
+    jj, ii = np.nonzero(f)
+    ids = (ii + NX * (jj + NY * nz) + 1).astype(np.int32)
+    vals = np.zeros((ids.size, 7))
+    vals[:, 2] = -f[jj, ii]
+    return ids, vals
+
+
+def impulse_block(nx=48, ny=48, nz=24, lx=1.0, ly=1.0, lz=0.5, median_E=70e9, log_sigma=0.5, corr_len=8.0,
+                  nu=0.3, rho=2700.0, patch=8, total=1e3, seed=0, name=None):
+    """The Newmark workload: brick8 block with a log-normal Young's modulus
+    field, density rho, clamped on z = 0, and a load of `total` newtons
+    (at load factor 1) on a patch x patch-element area centred on the top
+    face.  The time history is applied through per-step load factors
+    (see half_sine)."""
+    nodes, conn, colors = brick_block(nx, ny, nz, lx, ly, lz)
+    E_el = lognormal_e_field(nx, ny, nz, median_E, log_sigma, corr_len, seed)
+    E = conn.shape[0]
+    NX, NY = nx + 1, ny + 1
+    bottom = np.arange(1, NX * NY + 1, dtype=np.int32)
+    m = Model(name=name or f"impulse{nx}x{ny}x{nz}", nodes=nodes, elements=conn,
+              etype=np.full(E, BRICK8, np.int32), eprops=np.stack([iso_props(e, nu) for e in E_el]),
+              density=np.full(E, float(rho)), cond=np.zeros((E, 6)), colors=colors)
+    m.mpc = clamp_set_mpc(bottom, [1, 2, 3], "bottom")
+    m.mpc_rhs = [0.0, 0.0, 0.0]
+    m.node_sets["bottom"] = bottom
+    m.load_nodes, m.load_vals = top_patch_loads(nx, ny, nz, lx, ly, lz, min(patch, nx), min(patch, ny), total)
+    return m
+
+
+def half_sine(n_steps, dt, duration):
+    """Load factor at the end of each step t_k = k dt (k = 1..n_steps): a
+    half-sine pulse sin(pi t / duration) for t <= duration, zero after."""
+    t = dt * np.arange(1, n_steps + 1)
+    return np.where(t <= duration, np.sin(np.pi * t / duration), 0.0)

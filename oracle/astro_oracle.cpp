@@ -262,6 +262,54 @@ void orc_brick8_mass(const double* xyz, double rho, double* Me) {
     }
 }
 
+// a10 + a8 (solid branch at any u): internal force with the Green-Lagrange
+// strain, R_i = sum_gp detJ*w * S . dE/du_i with S = C E(u) (T = 0).
+// Working/AStrO_r_elementEqns.f90:2424-2458 (ux :2426-2432, stress :2437-2447,
+// Ruk :2448-2458); strain and its first derivative r_getNLStrain :622-662 with
+// orient = identity.  ue: 24 DOFs component-major; R: 24.
+void orc_brick8_internal_force(const double* xyz, const double* C, const double* ue, double* R) {
+    ElProfile p;
+    element_profile(8, p);
+    for (int d = 0; d < 24; ++d) R[d] = 0.0;
+    for (int g = 0; g < 8; ++g) {
+        double N[8], Nx[8][3];
+        double detJ = brick_ipdata(xyz, p.intPts[g], N, Nx);
+        double ux[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int d = 0; d < 24; ++d) {
+            int var = p.dofTable[0][d] - 1, nd = p.dofTable[1][d] - 1;
+            for (int j = 0; j < 3; ++j) ux[var][j] += ue[d] * Nx[nd][j];
+        }
+        double E[6];
+        E[0] = ux[0][0] + 0.5 * (ux[0][0] * ux[0][0] + ux[1][0] * ux[1][0] + ux[2][0] * ux[2][0]);
+        E[1] = ux[1][1] + 0.5 * (ux[0][1] * ux[0][1] + ux[1][1] * ux[1][1] + ux[2][1] * ux[2][1]);
+        E[2] = ux[2][2] + 0.5 * (ux[0][2] * ux[0][2] + ux[1][2] * ux[1][2] + ux[2][2] * ux[2][2]);
+        E[3] = ux[0][1] + ux[1][0] + ux[0][0] * ux[0][1] + ux[1][0] * ux[1][1] + ux[2][0] * ux[2][1];
+        E[4] = ux[0][2] + ux[2][0] + ux[0][0] * ux[0][2] + ux[1][0] * ux[1][2] + ux[2][0] * ux[2][2];
+        E[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
+        double S[6];
+        for (int r = 0; r < 6; ++r) {
+            S[r] = 0.0;
+            for (int k = 0; k < 6; ++k) S[r] += C[r + 6 * k] * E[k];
+        }
+        double wd = detJ * p.ipWt[g];
+        for (int d = 0; d < 24; ++d) {
+            int v = p.dofTable[0][d] - 1, nd = p.dofTable[1][d] - 1;
+            double o[3] = {v == 0 ? 1.0 : 0.0, v == 1 ? 1.0 : 0.0, v == 2 ? 1.0 : 0.0};
+            const double* x = Nx[nd];
+            double dE[6];
+            dE[0] = o[0] * x[0] + x[0] * ux[v][0];
+            dE[1] = o[1] * x[1] + x[1] * ux[v][1];
+            dE[2] = o[2] * x[2] + x[2] * ux[v][2];
+            dE[3] = o[0] * x[1] + o[1] * x[0] + x[0] * ux[v][1] + x[1] * ux[v][0];
+            dE[4] = o[0] * x[2] + o[2] * x[0] + x[0] * ux[v][2] + x[2] * ux[v][0];
+            dE[5] = o[1] * x[2] + o[2] * x[1] + x[1] * ux[v][2] + x[2] * ux[v][1];
+            double acc = 0.0;
+            for (int k = 0; k < 6; ++k) acc += S[k] * dE[k];
+            R[d] += wd * acc;
+        }
+    }
+}
+
 // a13: brick8 conductivity sum_gp detJ*w * Nx k Nx^T.
 // Working/AStrO_r_elementEqns.f90:2062-2124.  k: 3x3 column-major; Kt: 8x8.
 void orc_brick8_conductivity(const double* xyz, const double* k, double* Kt) {
@@ -666,6 +714,9 @@ struct OrcModel {
     std::vector<double> thermMPCMat, thermMPCRHS;
     int gmresCycles = 0, gmresSteps = 0;
     double gmresRes = 0;
+    // Newmark state (solve's dynamic branch)
+    double nmBeta = 0.25, nmGamma = 0.5, delT = 0.0;
+    std::vector<double> prevDisp, prevVel, prevAcc, nodeVel, nodeAcc;
     std::vector<double> timings;  // seconds per phase of the last static solve
 };
 
@@ -1226,6 +1277,109 @@ void orc_solve_thermal(OrcModel* mp, int numVecs, int mode, double rtol) {
                               &m.gmresRes, &m.gmresSteps);
 }
 
+// ---------------------------------------------------------------------------
+// Newmark dynamics, the dynamic branch of solve (Working/AStrO_commandFunctions.f90:
+// 762-810) with the intended element gather (D4 corrected: every element sees
+// its own nodes' solution, velocity and acceleration not swapped).
+// c1 = 0.5/beta, c2 = 2/dt^2 (:787-788, Working/AStrO_r_elementEqns.f90:2685-2686).
+
+// Assemble the tangent K + c1*c2*M at u = 0 (getElasticSolnLoad(1) with
+// dynamic = 1), scale the penalty rows, build the profile; zero the state.
+void orc_dyn_init(OrcModel* mp, double beta, double gamma, double dt) {
+    OrcModel& m = *mp;
+    m.nmBeta = beta;
+    m.nmGamma = gamma;
+    m.delT = dt;
+    const double c1 = 0.5 / beta, c2 = 2.0 / (dt * dt);
+    orc_assemble_elastic(mp, c1 * c2);
+    orc_scale_elastic_mpc(mp);
+    orc_build_elastic_profile(mp);
+    const size_t n = m.elMatDim;
+    m.nodeDisp.assign(n, 0.0);
+    m.delDisp.assign(n, 0.0);
+    m.prevDisp.assign(n, 0.0);
+    m.prevVel.assign(n, 0.0);
+    m.prevAcc.assign(n, 0.0);
+    m.nodeVel.assign(n, 0.0);
+    m.nodeAcc.assign(n, 0.0);
+}
+
+// Right-hand side of one step (stepSolve :718-723): minus the element
+// residuals Ru = Ruk(u) + M a_next(u) (r_getElRu :2659-2696), plus the nodal
+// loads times loadScale (the caller's time function; the reference switches
+// whole loads on and off by their active-time windows, :366), plus the
+// constraint load C^T (rhs - C u) (:506-535).
+void orc_dyn_residual_load(OrcModel* mp, double loadScale) {
+    OrcModel& m = *mp;
+    const double c1 = 0.5 / m.nmBeta, c2 = 2.0 / (m.delT * m.delT), dt = m.delT, beta = m.nmBeta;
+    std::fill(m.elasticLoad.begin(), m.elasticLoad.end(), 0.0);
+    ElProfile p;
+    element_profile(8, p);
+    double xyz[24], ue[24], ae[24], Ri[24], Me[576];
+    int dof[24];
+    for (int e = 0; e < m.numEls; ++e) {
+        orc_element_coords(&m, e, xyz);
+        for (int d = 0; d < 24; ++d) {
+            int node = m.elementList[8 * (size_t)e + p.dofTable[1][d] - 1];
+            dof[d] = m.nDofIndex[6 * (size_t)(node - 1) + p.dofTable[0][d] - 1];
+            const size_t k = dof[d] - 1;
+            ue[d] = m.nodeDisp[k];
+            ae[d] = c1 * (c2 * (m.nodeDisp[k] - m.prevDisp[k] - dt * m.prevVel[k]) - (1.0 - 2.0 * beta) * m.prevAcc[k]);
+        }
+        orc_brick8_internal_force(xyz, &m.elC[36 * (size_t)e], ue, Ri);
+        orc_brick8_mass(xyz, m.elDen[e], Me);
+        for (int i = 0; i < 24; ++i) {
+            double rm = 0.0;
+            for (int j = 0; j < 24; ++j) rm += Me[i + 24 * j] * ae[j];
+            m.elasticLoad[dof[i] - 1] -= Ri[i] + rm;
+        }
+    }
+    for (size_t l = 0; l < m.loadNode.size(); ++l)
+        for (int c = 0; c < 6; ++c) {
+            int d = m.nDofIndex[6 * (size_t)(m.loadNode[l] - 1) + c];
+            if (d != 0) m.elasticLoad[d - 1] += loadScale * m.loadVal[7 * l + c];
+        }
+    std::vector<double> t(m.elMPCDim);
+    for (int i = 1; i <= m.elMPCDim; ++i) {
+        double v = m.elMPCRHS[i - 1];
+        for (int s = m.elMPCMatRange[i - 1] + 1; s <= m.elMPCMatRange[i]; ++s) {
+            int c = m.elMPCMatCols[s - 1];
+            if (c != 0) v -= m.elMPCMat[s - 1] * m.nodeDisp[c - 1];
+        }
+        t[i - 1] = v;
+    }
+    for (int i = 1; i <= m.elMPCDim; ++i)
+        for (int s = m.elMPCMatRange[i - 1] + 1; s <= m.elMPCMatRange[i]; ++s) {
+            int c = m.elMPCMatCols[s - 1];
+            if (c != 0) m.elasticLoad[c - 1] += m.elMPCMat[s - 1] * t[i - 1];
+        }
+}
+
+// One time step: RHS, GMRES from the previous increment (stepSolve passes
+// delDisp as the initial guess), u += du, then the Newmark updates of
+// acceleration and velocity and the shift to the previous state (:792-806).
+// mode/rtol as orc_solve_elastic.
+void orc_dyn_step(OrcModel* mp, double loadScale, int numVecs, int mode, double rtol) {
+    OrcModel& m = *mp;
+    orc_dyn_residual_load(mp, loadScale);
+    if (numVecs <= 0) numVecs = orc_num_vecs(m.elMatDim);
+    int maxIt = mode == 0 ? m.elMatDim : 1000 * numVecs;
+    m.gmresCycles = orc_gmres_mode(m.delDisp.data(), m.elasticMat.data(), m.elMatCols.data(), m.elMatRange.data(),
+                                   m.elMatDim, m.elMPCMat.data(), m.elMPCMatCols.data(), m.elMPCMatRange.data(),
+                                   m.elMPCDim, m.elMatLT.data(), m.elMatLTRange.data(), m.elasticLoad.data(),
+                                   numVecs, maxIt, mode, rtol, &m.gmresRes, &m.gmresSteps);
+    const double c1 = 0.5 / m.nmBeta, c2 = 2.0 / (m.delT * m.delT), dt = m.delT;
+    const double beta = m.nmBeta, gam = m.nmGamma;
+    for (int i = 0; i < m.elMatDim; ++i) {
+        m.nodeDisp[i] += m.delDisp[i];
+        m.nodeAcc[i] = c1 * (c2 * (m.nodeDisp[i] - m.prevDisp[i] - dt * m.prevVel[i]) - (1.0 - 2.0 * beta) * m.prevAcc[i]);
+        m.nodeVel[i] = m.prevVel[i] + dt * ((1.0 - gam) * m.prevAcc[i] + gam * m.nodeAcc[i]);
+        m.prevDisp[i] = m.nodeDisp[i];
+        m.prevVel[i] = m.nodeVel[i];
+        m.prevAcc[i] = m.nodeAcc[i];
+    }
+}
+
 // Array getters for the Python side of the tests.
 const int* orc_geti(OrcModel* m, const char* name, long long* len) {
     std::string s(name);
@@ -1267,6 +1421,9 @@ double* orc_getd(OrcModel* m, const char* name, long long* len) {
     else if (s == "thermMPCMat") v = &m->thermMPCMat;
     else if (s == "thermMPCRHS") v = &m->thermMPCRHS;
     else if (s == "elC") v = &m->elC;
+    else if (s == "nodeVel") v = &m->nodeVel;
+    else if (s == "nodeAcc") v = &m->nodeAcc;
+    else if (s == "prevDisp") v = &m->prevDisp;
     if (!v) { *len = -1; return nullptr; }
     *len = (long long)v->size();
     return v->data();

@@ -69,6 +69,10 @@ def lib():
         L.orc_material_stiffness.argtypes = [_dp, _dp]
         L.orc_brick8_stiffness.argtypes = [_dp, _dp, _dp]
         L.orc_brick8_mass.argtypes = [_dp, C.c_double, _dp]
+        L.orc_brick8_internal_force.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_dyn_init.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.orc_dyn_residual_load.argtypes = [C.c_void_p, C.c_double]
+        L.orc_dyn_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double]
         L.orc_brick8_conductivity.argtypes = [_dp, _dp, _dp]
         L.orc_brick8_volume.restype = C.c_double
         L.orc_brick8_volume.argtypes = [_dp]
@@ -111,6 +115,15 @@ def brick8_mass(xyz38, rho):
     Me = np.zeros(576)
     lib().orc_brick8_mass(_f64(np.asarray(xyz38).reshape(3, 8).ravel(order="F")), float(rho), Me)
     return Me.reshape(24, 24, order="F")
+
+
+def brick8_internal_force(xyz38, Cmat, ue24):
+    """Green-Lagrange internal force of one brick8 at element DOFs ue
+    (component-major)."""
+    R = np.zeros(24)
+    lib().orc_brick8_internal_force(_f64(np.asarray(xyz38).reshape(3, 8).ravel(order="F")),
+                                    _f64(np.asarray(Cmat).ravel(order="F")), _f64(ue24), R)
+    return R
 
 
 def brick8_conductivity(xyz38, k33):
@@ -242,6 +255,20 @@ class OracleModel:
         if stop_after == "factor":
             return self
         L.orc_solve_elastic(self.h, int(num_vecs), int(mode), float(rtol))
+        return self
+
+    def dynamic_init(self, beta: float = 0.25, gamma: float = 0.5, dt: float = 1e-5):
+        """solve()'s dynamic set-up: tangent K + M/(beta dt^2), penalty scaling,
+        profile and factor; the state starts at rest."""
+        L = lib()
+        L.orc_dyn_init(self.h, float(beta), float(gamma), float(dt))
+        bad = L.orc_factor_elastic(self.h)
+        if bad:
+            raise RuntimeError(f"zero pivot at row {bad}")
+        return self
+
+    def dynamic_step(self, load_scale: float, num_vecs: int = 0, mode: int = 1, rtol: float = 1e-12):
+        lib().orc_dyn_step(self.h, float(load_scale), int(num_vecs), int(mode), float(rtol))
         return self
 
     def static_thermal(self, literal_d8: bool = True, num_vecs: int = 0, mode: int = 0, rtol: float = 1e-12):
