@@ -111,6 +111,15 @@ typedef struct {
     int32_t numLoads;
     const int32_t* loadNode;
     const double* loadVal; /* 7 per load */
+    /* Shell sections (shell4 elements; may be null for solid models), the
+     * section/layup data of Working/AStrO_input.f90's sections block resolved
+     * per element: numLayers layers of (E1, E2, nu12, G12, thickness, angle in
+     * radians) per element, the section z offset per element, and the two
+     * orientation vectors (x axis, a vector in the x-y plane) per element. */
+    int32_t numLayers;
+    const double* layers;     /* numEls * numLayers * 6 */
+    const double* zOffset;    /* numEls */
+    const double* orient;     /* numEls * 6 */
 } astro_model;
 
 /* Options of the analysis (solver parameters of the *solve command). */

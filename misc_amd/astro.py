@@ -58,7 +58,8 @@ class _Model(C.Structure):
                 ("mpcDof", C.c_void_p), ("mpcCoef", C.c_void_p), ("mpcIsSet", C.c_void_p),
                 ("mpcNodeStart", C.c_void_p), ("mpcNodes", C.c_void_p), ("numMPC", C.c_int32),
                 ("mpcRHS", C.c_void_p), ("numLoads", C.c_int32), ("loadNode", C.c_void_p),
-                ("loadVal", C.c_void_p)]
+                ("loadVal", C.c_void_p), ("numLayers", C.c_int32), ("layers", C.c_void_p), ("zOffset", C.c_void_p),
+                ("orient", C.c_void_p)]
 
 
 class _Options(C.Structure):
@@ -227,6 +228,11 @@ def _marshal(model, o: "Options"):
     if M.numLoads:
         M.loadNode = ptr(model.load_nodes, np.int32)
         M.loadVal = ptr(model.load_vals.reshape(-1), np.float64)
+    if getattr(model, "layers", None) is not None:
+        M.numLayers = int(model.layers.shape[1])
+        M.layers = ptr(model.layers.reshape(-1), np.float64)
+        M.zOffset = ptr(model.z_offset, np.float64)
+        M.orient = ptr(model.orient.reshape(-1), np.float64)
     O = _Options(int(o.maxBW), 1 if o.literalMaxBW else 0, 1 if o.thermLiteralD8 else 0, int(o.numVecs),
                  float(o.rtol), int(o.maxCycles))
     return M, O, keep

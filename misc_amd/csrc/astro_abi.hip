@@ -529,6 +529,7 @@ struct astro_ctx {
     int numVecs = 100;
     // model on device
     DBuf<double> X, matC, elScale, elRho, elCond;
+    DBuf<double> elABD, elAlpha;  // shell4 models: layup stiffness 81, frame 9 per element
     DBuf<int> conn, elMat, rank, perm;
     DBuf<uint8_t> elPos, thPos;
     std::vector<int> colorStart;
@@ -611,6 +612,24 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
         if (m->density) rho[e] = m->density[e];
         if (m->cond)
             for (int i = 0; i < 6; ++i) cond[6 * (size_t)e + i] = m->cond[6 * (size_t)e + i];
+    }
+    if (p.shellMesh) {
+        if (m->numLayers < 1 || !m->layers || !m->orient)
+            return fail(ASTRO_ERR_ARG, "shell4 elements need section layers and orientation");
+        std::vector<double> abd(81 * (size_t)E), al(9 * (size_t)E);
+        for (int e = 0; e < E; ++e) {
+            double xyz[12];
+            for (int a = 0; a < 4; ++a) {
+                const int nd = m->elementList[8 * (size_t)e + a];
+                if (nd < 1 || nd > N) return fail(ASTRO_ERR_ARG, "shell4 node out of range");
+                for (int q = 0; q < 3; ++q) xyz[q + 3 * a] = m->nodeList[3 * (size_t)(nd - 1) + q];
+            }
+            shell_frame(m->orient + 6 * (size_t)e, xyz, &al[9 * (size_t)e]);
+            shell_section_abd(m->numLayers, m->layers + 6 * (size_t)m->numLayers * e, m->zOffset ? m->zOffset[e] : 0.0,
+                              &abd[81 * (size_t)e]);
+        }
+        ASTRO_TRY(c->elABD.upload(abd, s));
+        ASTRO_TRY(c->elAlpha.upload(al, s));
     }
     ASTRO_TRY(c->matC.upload(mats, s));
     ASTRO_TRY(c->elMat.upload(elMat, s));
@@ -698,7 +717,8 @@ int astro_ctx_create(const astro_model* m, const astro_options* o, astro_ctx** o
     std::string err = analysis_prep(d, c->opt.maxBW, c->opt.literalMaxBW != 0, c->opt.thermLiteralD8 != 0, c->prep);
     if (!err.empty()) return fail(ASTRO_ERR_ARG, err);
     for (int e = 0; e < m->numEls; ++e)
-        if (m->elementType[e] != 8) return fail(ASTRO_ERR_UNSUPPORTED, "device path supports brick8 elements only");
+        if (m->elementType[e] != (c->prep.shellMesh ? 41 : 8))
+            return fail(ASTRO_ERR_UNSUPPORTED, "device path supports all-brick8 or all-shell4 meshes");
     const int n = c->prep.elMatDim;
     c->numVecs = c->opt.numVecs > 0 ? c->opt.numVecs : (n < 200 ? n / 2 : 100);
     if (c->numVecs < 1 || c->numVecs > 255) return fail(ASTRO_ERR_ARG, "numVecs must be in [1, 255]");
@@ -900,6 +920,18 @@ int astro_ctx_assemble_elastic(astro_ctx* c, double massFactor) {
     const int64_t nnz = (int64_t)c->prep.elMatCols.size();
     hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, c->s, c->csrVal.p, nnz, 0.0);
     const int nS = c->prep.numSolidNds;
+    if (c->prep.shellMesh) {
+        if (massFactor != 0.0) return fail(ASTRO_ERR_UNSUPPORTED, "shell4 inertia is not part of the device path");
+        for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+            const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+            if (cnt <= 0) continue;
+            hipLaunchKernelGGL(k_shell4_tangent, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
+                               c->conn.p, c->X.p, c->elABD.p, c->elAlpha.p, c->rank.p, nS, c->prep.numSectionNds,
+                               c->csrRange.p, c->elPos.p, c->csrVal.p);
+        }
+        HIP_TRY(hipGetLastError());
+        return ASTRO_OK;
+    }
     for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
         const int cnt = c->colorStart[k + 1] - c->colorStart[k];
         if (cnt <= 0) continue;
@@ -1010,6 +1042,7 @@ int astro_ctx_dynamic_init(astro_ctx* c, double beta, double gamma, double dt) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     if (!(beta > 0.0) || !(dt > 0.0) || !(gamma >= 0.0))
         return fail(ASTRO_ERR_ARG, "Newmark parameters need beta > 0, gamma >= 0, dt > 0");
+    if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "Newmark dynamics of shell4 meshes is not part of the device path");
     const int n = c->prep.elMatDim;
     c->nmBeta = beta;
     c->nmGamma = gamma;
@@ -1122,6 +1155,7 @@ static int ctx_thermal_setup(astro_ctx* c) {
 
 int astro_ctx_assemble_thermal(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "thermal analysis of shell4 meshes is not part of the device path");
     ASTRO_TRY(ctx_thermal_setup(c));
     const int64_t nnz = (int64_t)c->prep.thermMatCols.size();
     hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, c->s, c->tVal.p, nnz, 0.0);

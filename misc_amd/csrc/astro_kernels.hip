@@ -146,6 +146,199 @@ __global__ __launch_bounds__(64) void k_brick8_tangent(
 }
 
 // ---------------------------------------------------------------------------
+// Shell4 tangent at rest with static condensation, one 64-lane workgroup per
+// element (colour classes: elements of one launch share no node).
+// Working/AStrO_r_elementEqns.f90:2245-2423 and 2478-2494 (sum_gp detJ w
+// dedU^T ABD dedU), generalised strains r_getShellDef :800-814, instantaneous
+// DOFs r_getInstDof :703-891 linearised at rest with exact derivatives
+// (u_L(a) = alpha (u_a - rbar x X_a), r_L(a) = alpha (r_a - rbar), rbar the
+// mean nodal rotation), shape functions :44-55 / 143-162 (4 bilinear + 6
+// bubbles carrying the 8 internal DOFs), condensation r_condenseMat
+// :2722-2743.  DOF order of the element: (var, node) var-major, var 1..6 on
+// nodes 1..4, then internal (1,5) (1,6) (2,5) (2,6) (3,7) (3,8) (3,9) (3,10).
+// Rows of node a start at csrRange[dof(a, v) - 1], dof(a, v) = rank(a) +
+// v * nSec + 2 * nSolid; the slot of (node b, var w) is elPos[e][(a*4+b)*6+w].
+__global__ __launch_bounds__(64) void k_shell4_tangent(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ elABD, const double* __restrict__ elAlpha, const int* __restrict__ rank, int nSolid,
+    int nSec, const int* __restrict__ csrRange, const uint8_t* __restrict__ elPos, double* __restrict__ val) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[4][3], sAl[3][3], sD[9][9];
+    __shared__ double sN[4][10], sNx[4][10][2], sW[4];
+    __shared__ double sB[4][9][32], sDB[4][9][32];
+    __shared__ double sK[32][33];
+    __shared__ double sXs[24][8];
+    __shared__ int sNode[4];
+    if (lane < 4) {
+        const int nd = conn[8 * (size_t)e + lane] - 1;
+        sNode[lane] = nd;
+        for (int c = 0; c < 3; ++c) sX[lane][c] = X[3 * (size_t)nd + c];
+    }
+    for (int k = lane; k < 81; k += 64) sD[k % 9][k / 9] = elABD[81 * (size_t)e + k];
+    if (lane < 9) sAl[lane % 3][lane / 3] = elAlpha[9 * (size_t)e + lane];
+    __syncthreads();
+    if (lane < 4) {
+        // Gauss point g: natural coordinates (+-1/sqrt3, +-1/sqrt3), bit 0 -> s1
+        const int g = lane;
+        const double q = 0.57735026918962576451;
+        const double s1 = (g & 1) ? q : -q, s2 = (g & 2) ? q : -q;
+        double N[10], Ns[10][2];
+        N[0] = 0.25 * (1 - s1) * (1 - s2);
+        N[1] = 0.25 * (1 + s1) * (1 - s2);
+        N[2] = 0.25 * (1 + s1) * (1 + s2);
+        N[3] = 0.25 * (1 - s1) * (1 + s2);
+        N[4] = 1 - s1 * s1;
+        N[5] = 1 - s2 * s2;
+        N[6] = (1 - s1 * s1) * (1 - s2);
+        N[7] = (1 - s2 * s2) * (1 + s1);
+        N[8] = (1 - s1 * s1) * (1 + s2);
+        N[9] = (1 - s2 * s2) * (1 - s1);
+        Ns[0][0] = -0.25 * (1 - s2); Ns[0][1] = -0.25 * (1 - s1);
+        Ns[1][0] = 0.25 * (1 - s2);  Ns[1][1] = -0.25 * (1 + s1);
+        Ns[2][0] = 0.25 * (1 + s2);  Ns[2][1] = 0.25 * (1 + s1);
+        Ns[3][0] = -0.25 * (1 + s2); Ns[3][1] = 0.25 * (1 - s1);
+        Ns[4][0] = -2 * s1;          Ns[4][1] = 0.0;
+        Ns[5][0] = 0.0;              Ns[5][1] = -2 * s2;
+        Ns[6][0] = -2 * s1 * (1 - s2); Ns[6][1] = -(1 - s1 * s1);
+        Ns[7][0] = 1 - s2 * s2;        Ns[7][1] = -2 * s2 * (1 + s1);
+        Ns[8][0] = -2 * s1 * (1 + s2); Ns[8][1] = 1 - s1 * s1;
+        Ns[9][0] = -(1 - s2 * s2);     Ns[9][1] = -2 * s2 * (1 - s1);
+        // dx/ds of the local coordinates alpha X, third column e3 (flat element)
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 1}};
+        for (int a = 0; a < 4; ++a) {
+            double loc[3];
+            for (int i = 0; i < 3; ++i) loc[i] = sAl[i][0] * sX[a][0] + sAl[i][1] * sX[a][1] + sAl[i][2] * sX[a][2];
+            for (int i = 0; i < 3; ++i) {
+                J[i][0] += loc[i] * Ns[a][0];
+                J[i][1] += loc[i] * Ns[a][1];
+            }
+        }
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double di = 1.0 / det;
+        const double I00 = di * c00, I10 = di * c01;
+        const double I01 = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+        const double I11 = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+        for (int n = 0; n < 10; ++n) {
+            sN[g][n] = N[n];
+            sNx[g][n][0] = Ns[n][0] * I00 + Ns[n][1] * I10;
+            sNx[g][n][1] = Ns[n][0] * I01 + Ns[n][1] * I11;
+        }
+        sW[g] = det;
+    }
+    __syncthreads();
+    // B: lane = DOF d (0..31) for Gauss points 2 (lane >> 5) and 2 (lane >> 5) + 1
+    {
+        const int d = lane & 31;
+        for (int g = 2 * (lane >> 5); g < 2 * (lane >> 5) + 2; ++g) {
+            double ux[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rx[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rot[3] = {0, 0, 0};
+            if (d >= 24) {
+                const int k = d - 24;
+                const int var = k < 2 ? 0 : (k < 4 ? 1 : 2);
+                const int fn = k < 4 ? 4 + (k & 1) : 6 + (k - 4);
+                ux[var][0] = sNx[g][fn][0];
+                ux[var][1] = sNx[g][fn][1];
+            } else if (d < 12) {
+                const int var = d >> 2, nd = d & 3;
+                for (int i = 0; i < 3; ++i) {
+                    ux[i][0] = sAl[i][var] * sNx[g][nd][0];
+                    ux[i][1] = sAl[i][var] * sNx[g][nd][1];
+                }
+            } else {
+                const int q = (d >> 2) - 3, nd = d & 3;
+                for (int a = 0; a < 4; ++a) {
+                    // -(e_q x X_a) / 4 in local components
+                    double ex[3] = {0, 0, 0};
+                    if (q == 0) { ex[1] = -sX[a][2]; ex[2] = sX[a][1]; }
+                    if (q == 1) { ex[0] = sX[a][2]; ex[2] = -sX[a][0]; }
+                    if (q == 2) { ex[0] = -sX[a][1]; ex[1] = sX[a][0]; }
+                    const double f = (a == nd ? 1.0 : 0.0) - 0.25;
+                    for (int i = 0; i < 3; ++i) {
+                        const double v = -0.25 * (sAl[i][0] * ex[0] + sAl[i][1] * ex[1] + sAl[i][2] * ex[2]);
+                        ux[i][0] += v * sNx[g][a][0];
+                        ux[i][1] += v * sNx[g][a][1];
+                        rx[i][0] += sAl[i][q] * f * sNx[g][a][0];
+                        rx[i][1] += sAl[i][q] * f * sNx[g][a][1];
+                        rot[i] += sAl[i][q] * f * sN[g][a];
+                    }
+                }
+            }
+            sB[g][0][d] = ux[0][0];
+            sB[g][1][d] = ux[1][1];
+            sB[g][2][d] = ux[0][1] + ux[1][0];
+            sB[g][3][d] = rx[1][0];
+            sB[g][4][d] = -rx[0][1];
+            sB[g][5][d] = rx[1][1] - rx[0][0];
+            sB[g][6][d] = ux[2][0] + rot[1];
+            sB[g][7][d] = ux[2][1] - rot[0];
+            sB[g][8][d] = 2.0 * rot[2] - ux[1][0] + ux[0][1];
+        }
+    }
+    __syncthreads();
+    for (int k = lane; k < 4 * 9 * 32; k += 64) {
+        const int g = k / 288, r = (k / 32) % 9, d = k % 32;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) acc += sD[r][m] * sB[g][m][d];
+        sDB[g][r][d] = sW[g] * acc;
+    }
+    __syncthreads();
+    for (int k = lane; k < 32 * 32; k += 64) {
+        const int i = k >> 5, j = k & 31;
+        double acc = 0.0;
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) acc += sB[g][m][i] * sDB[g][m][j];
+        sK[i][j] = acc;
+    }
+    __syncthreads();
+    // condensation: LU (no pivoting) of the internal block, rows/cols 24..31
+    if (lane == 0) {
+        for (int i = 24; i < 32; ++i) {
+            for (int j = i; j < 32; ++j)
+                for (int k = 24; k < i; ++k) sK[i][j] -= sK[i][k] * sK[k][j];
+            const double f = 1.0 / sK[i][i];
+            for (int j = i + 1; j < 32; ++j) {
+                for (int k = 24; k < i; ++k) sK[j][i] -= sK[j][k] * sK[k][i];
+                sK[j][i] *= f;
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < 24) {
+        // x = K_ii^-1 K_i,c for nodal column c = lane
+        double z[8], x[8];
+        for (int i = 0; i < 8; ++i) {
+            double v = sK[24 + i][lane];
+            for (int k = 0; k < i; ++k) v -= sK[24 + i][24 + k] * z[k];
+            z[i] = v;
+        }
+        for (int i = 7; i >= 0; --i) {
+            double v = z[i];
+            for (int k = i + 1; k < 8; ++k) v -= sK[24 + i][24 + k] * x[k];
+            x[i] = v / sK[24 + i][24 + i];
+        }
+        for (int i = 0; i < 8; ++i) sXs[lane][i] = x[i];
+    }
+    __syncthreads();
+    for (int k = lane; k < 24 * 24; k += 64) {
+        const int i = k / 24, j = k % 24;
+        double v = sK[i][j];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v -= sK[i][24 + m] * sXs[j][m];
+        const int vi = i >> 2, a = i & 3, vj = j >> 2, b = j & 3;
+        const int row = rank[sNode[a]] + vi * nSec + 2 * nSolid;
+        const int slot = elPos[192 * (size_t)e + (a * 4 + b) * 6 + vj];
+        val[csrRange[row - 1] + slot] += v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Element residuals of the Newmark step, brick8: R_e = Ruk(u_e) + M_e a_e with
 // the Green-Lagrange internal force (Working/AStrO_r_elementEqns.f90:2424-2458,
 // strain and derivative r_getNLStrain :622-662, orient = identity, T = 0) and

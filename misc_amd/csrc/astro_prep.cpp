@@ -285,11 +285,17 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
     // (Working/AStrO_commandFunctions.f90:323-347).  All rows of one node see the
     // same sequence, so one list per node serves its rows.  Alongside, record the
     // slot of (node b, component d) in node a's list for every element.
-    bool allBrick = true;
-    for (int e = 0; e < E; ++e) allBrick &= (m.elementType[e] == 8);
+    bool allBrick = true, allShell = true;
+    for (int e = 0; e < E; ++e) {
+        allBrick &= (m.elementType[e] == 8);
+        allShell &= (m.elementType[e] == 41);
+    }
+    // element node count and DOFs per node of the all-brick / all-shell4 mesh
+    const int enn = allShell ? 4 : 8, edof = allShell ? 6 : 3;
+    p.shellMesh = allShell && E > 0;
     p.elMatCols.assign(p.elMatRange[n], 0);
     p.thermMatCols.assign(p.thermMatRange[N], 0);
-    if (allBrick) {
+    if (allBrick || allShell) {
         p.elPos.assign(192 * (size_t)E, 0);
         p.thPos.assign(64 * (size_t)E, 0);
         // per node: slot of each (adjacent-or-self node, component); looked up by
@@ -303,29 +309,33 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
                 if (p.nodalConn[s] == nb) return q;
             return -1;
         };
+        if (allShell) slot.assign((size_t)N * 27 * 6, -1);
         for (int e = 0; e < E; ++e) {
             const int* el = m.elementList + 8 * (size_t)e;
-            for (int a = 0; a < 8; ++a) {
+            for (int a = 0; a < enn; ++a) {
                 int na = el[a];
                 int row0 = dofOf(na, 0);
                 int rbeg = p.elMatRange[row0 - 1], cap = p.elMatRange[row0] - rbeg;
                 int trow = p.currentRank[na - 1];
                 int tbeg = p.thermMatRange[trow - 1], tcap = p.thermMatRange[trow] - tbeg;
                 int q[8];
-                for (int b = 0; b < 8; ++b) {
+                for (int b = 0; b < enn; ++b) {
                     q[b] = nbIndex(na, el[b]);
                     if (q[b] < 0 || q[b] >= 27) return "adjacency larger than a brick mesh allows";
                 }
-                for (int d = 0; d < 3; ++d)
-                    for (int b = 0; b < 8; ++b) {
-                        int16_t& s = slot[((size_t)(na - 1) * 27 + q[b]) * 3 + d];
+                // columns in dofTable order: component-major over the element's nodes
+                for (int d = 0; d < edof; ++d)
+                    for (int b = 0; b < enn; ++b) {
+                        int16_t& s = slot[((size_t)(na - 1) * 27 + q[b]) * edof + d];
                         if (s < 0) {
-                            if (used[na] >= cap) return "CSR row capacity exceeded";
+                            if (used[na] >= cap || used[na] > 255) return "CSR row capacity exceeded";
                             s = (int16_t)used[na]++;
-                            for (int c = 0; c < 3; ++c) p.elMatCols[p.elMatRange[dofOf(na, c) - 1] + s] = dofOf(el[b], d);
+                            for (int c = 0; c < edof; ++c)
+                                p.elMatCols[p.elMatRange[dofOf(na, c) - 1] + s] = dofOf(el[b], d);
                         }
-                        p.elPos[192 * (size_t)e + (a * 8 + b) * 3 + d] = (uint8_t)s;
+                        p.elPos[192 * (size_t)e + (a * enn + b) * edof + d] = (uint8_t)s;
                     }
+                if (allShell) continue;
                 // thermal row: columns are node ranks, one per node; with the
                 // reference's `.lt.` probe bound (:57) the last slot never fills
                 int tlimit = thermLiteralD8 ? tcap - 1 : tcap;
@@ -350,13 +360,13 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
         for (int e = 0; e < E; ++e) {
             const int* el = m.elementList + 8 * (size_t)e;
             uint32_t busy = 0;
-            for (int a = 0; a < 8; ++a) busy |= nodeMask[el[a]];
+            for (int a = 0; a < enn; ++a) busy |= nodeMask[el[a]];
             int c = 0;
             while (c < 32 && (busy >> c) & 1u) ++c;
             if (c == 32) return "element colouring needs more than 32 colours";
             color[e] = c;
             ncol = std::max(ncol, c + 1);
-            for (int a = 0; a < 8; ++a) nodeMask[el[a]] |= 1u << c;
+            for (int a = 0; a < enn; ++a) nodeMask[el[a]] |= 1u << c;
         }
         p.colorStart.assign(ncol + 1, 0);
         for (int e = 0; e < E; ++e) p.colorStart[color[e] + 1]++;

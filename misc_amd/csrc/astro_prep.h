@@ -45,8 +45,10 @@ struct Prep {
     int elMPCDim = 0, thermMPCDim = 0;
     std::vector<int> elMPCMatRange, elMPCMatCols, thermMPCMatRange, thermMPCMatCols;
     std::vector<double> elMPCMat, elMPCRHS, thermMPCMat, thermMPCRHS;
-    // device helpers (brick8 models)
-    std::vector<uint8_t> elPos;   // 192 per element: slot of column (node b, comp d) in node a's row list
+    // device helpers (all-brick8 or all-shell4 models)
+    bool shellMesh = false;       // every element is a shell4 (4 nodes, 6 DOFs per node)
+    std::vector<uint8_t> elPos;   // 192 per element: slot of column (node b, comp d) in node a's row list,
+                                  // at (a*8+b)*3+d for bricks and (a*4+b)*6+d for shells
     std::vector<uint8_t> thPos;   // 64 per element: slot of node b in node a's thermal row
     std::vector<int> colorPerm;   // elements sorted by colour
     std::vector<int> colorStart;  // numColors + 1
@@ -60,5 +62,12 @@ int solver_max_bw(int numNodes, bool literal);
 // Full analysis.  maxBW <= 0 selects solver_max_bw(numNodes, literalMaxBW).
 // Returns "" on success or an error message.
 std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool thermLiteralD8, Prep& out);
+
+// Shell section data (astro_shell.cpp): frame rows = local axes from the two
+// orientation vectors aligned to the element normal; alpha[i + 3 j].  Layup
+// stiffness ABD (9 x 9 column-major) from nL layers of (E1, E2, nu12, G12,
+// thickness, angle in radians) and the z offset.
+void shell_frame(const double* orient6, const double* xyz34, double* alpha);
+void shell_section_abd(int nL, const double* layers, double zOff, double* ABD);
 
 }  // namespace astro

@@ -53,6 +53,12 @@ class Model:
     load_vals: np.ndarray = field(default_factory=lambda: np.zeros((0, 7)))
     node_sets: dict = field(default_factory=dict)
     colors: Optional[np.ndarray] = None  # (E,) element colour, conflict-free scatter
+    # shell sections (shell elements only): per element nL layers of
+    # (E1, E2, nu12, G12, thickness, angle in radians), z offset, and the two
+    # section orientation vectors (x axis, a vector in the x-y plane)
+    layers: Optional[np.ndarray] = None   # (E, nL, 6)
+    z_offset: Optional[np.ndarray] = None  # (E,)
+    orient: Optional[np.ndarray] = None   # (E, 6)
 
     @property
     def num_nodes(self) -> int:
@@ -233,3 +239,48 @@ def half_sine(n_steps, dt, duration):
     half-sine pulse sin(pi t / duration) for t <= duration, zero after."""
     t = dt * np.arange(1, n_steps + 1)
     return np.where(t <= duration, np.sin(np.pi * t / duration), 0.0)
+
+
+def shell_plate(nx, ny, lx=1.0, ly=0.1, E=70e9, nu=0.33, thickness=0.002, t_field=None, tip_load=1.0, name=None):
+    """Cantilever plate of nx x ny shell4 elements in the x-y plane, clamped on
+    x = 0 (DOFs 1-6 of every root node through node-set constraint equations),
+    tip_load newtons in +z on every node of x = lx.  One isotropic layer per
+    element, G = E / (2 (1 + nu)); t_field optionally gives the thickness per
+    element.  Section orientation [1,0,0], [0,1,0], zero offset.  Node id =
+    i + NX j + 1, elements x fastest with connectivity (i,j), (i+1,j),
+    (i+1,j+1), (i,j+1)."""
+    NX, NY = nx + 1, ny + 1
+    J, I = np.meshgrid(np.arange(NY), np.arange(NX), indexing="ij")
+    nodes = np.stack([I.ravel() * (lx / nx), J.ravel() * (ly / ny), np.zeros(NX * NY)], axis=1)
+    ej, ei = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    ei, ej = ei.ravel(), ej.ravel()
+
+    def nid(a, b):
+        return a + NX * b + 1
+
+    conn = np.zeros((ei.size, 8), np.int32)
+    conn[:, 0] = nid(ei, ej)
+    conn[:, 1] = nid(ei + 1, ej)
+    conn[:, 2] = nid(ei + 1, ej + 1)
+    conn[:, 3] = nid(ei, ej + 1)
+    nE = conn.shape[0]
+    colors = ((ei & 1) | ((ej & 1) << 1)).astype(np.int32)
+    t_el = np.full(nE, float(thickness)) if t_field is None else np.asarray(t_field, np.float64)
+    G = E / (2.0 * (1.0 + nu))
+    layers = np.zeros((nE, 1, 6))
+    layers[:, 0, :] = [E, E, nu, G, 0.0, 0.0]
+    layers[:, 0, 4] = t_el
+    m = Model(name=name or f"plate{nx}x{ny}", nodes=np.ascontiguousarray(nodes), elements=conn,
+              etype=np.full(nE, SHELL4, np.int32), eprops=np.tile(iso_props(E, nu), (nE, 1)),
+              density=np.zeros(nE), cond=np.zeros((nE, 6)), colors=colors, layers=layers,
+              z_offset=np.zeros(nE), orient=np.tile([1.0, 0.0, 0.0, 0.0, 1.0, 0.0], (nE, 1)))
+    root = (np.arange(NY) * NX + 1).astype(np.int32)
+    tip = root + nx
+    m.mpc = clamp_set_mpc(root, [1, 2, 3, 4, 5, 6], "root")
+    m.mpc_rhs = [0.0] * 6
+    m.node_sets["root"] = root
+    m.node_sets["tip"] = tip
+    vals = np.zeros((tip.size, 7))
+    vals[:, 2] = tip_load
+    m.load_nodes, m.load_vals = tip, vals
+    return m

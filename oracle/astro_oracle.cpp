@@ -61,6 +61,22 @@ void element_profile(int eType, ElProfile& p) {
                     p.ipWt[k] = 1.0;
                     ++k;
                 }
+    } else if (eType == 41) {
+        // :399-416: 4 Gauss points in the element plane, 8 internal DOFs on
+        // the bubble functions (u1, u2 on "nodes" 5-6; u3 on 7-10)
+        p.numNds = 4; p.dofPerNd = 6; p.numIntDof = 8; p.numIntPts = 4;
+        const double pts[4][2] = {{-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+        for (int k = 0; k < 4; ++k) {
+            p.intPts[k][0] = kOneRt3 * pts[k][0];
+            p.intPts[k][1] = kOneRt3 * pts[k][1];
+            p.intPts[k][2] = 0.0;
+            p.ipWt[k] = 1.0;
+        }
+        const int tab[8][2] = {{1, 5}, {1, 6}, {2, 5}, {2, 6}, {3, 7}, {3, 8}, {3, 9}, {3, 10}};
+        for (int k = 0; k < 8; ++k) {
+            p.dofTable[0][24 + k] = tab[k][0];
+            p.dofTable[1][24 + k] = tab[k][1];
+        }
     } else {
         std::fprintf(stderr, "oracle: element type %d not supported\n", eType);
         std::abort();
@@ -171,9 +187,296 @@ void lu_solve(double* x, const double* LU, const double* b, int n, int st, int e
     }
 }
 
+// Shell4 shape functions: 4 bilinear and 6 bubbles (incompatible modes) with
+// their natural derivatives.  Working/AStrO_r_elementEqns.f90:44-55, 143-162.
+void shell4_shape(double s1, double s2, double N[10], double Ns[10][2]) {
+    N[0] = 0.25 * (1 - s1) * (1 - s2);
+    N[1] = 0.25 * (1 + s1) * (1 - s2);
+    N[2] = 0.25 * (1 + s1) * (1 + s2);
+    N[3] = 0.25 * (1 - s1) * (1 + s2);
+    N[4] = 1 - s1 * s1;
+    N[5] = 1 - s2 * s2;
+    N[6] = (1 - s1 * s1) * (1 - s2);
+    N[7] = (1 - s2 * s2) * (1 + s1);
+    N[8] = (1 - s1 * s1) * (1 + s2);
+    N[9] = (1 - s2 * s2) * (1 - s1);
+    Ns[0][0] = -0.25 * (1 - s2); Ns[0][1] = -0.25 * (1 - s1);
+    Ns[1][0] = 0.25 * (1 - s2);  Ns[1][1] = -0.25 * (1 + s1);
+    Ns[2][0] = 0.25 * (1 + s2);  Ns[2][1] = 0.25 * (1 + s1);
+    Ns[3][0] = -0.25 * (1 + s2); Ns[3][1] = 0.25 * (1 - s1);
+    Ns[4][0] = -2 * s1;          Ns[4][1] = 0.0;
+    Ns[5][0] = 0.0;              Ns[5][1] = -2 * s2;
+    Ns[6][0] = -2 * s1 * (1 - s2); Ns[6][1] = -(1 - s1 * s1);
+    Ns[7][0] = (1 - s2 * s2);      Ns[7][1] = -2 * s2 * (1 + s1);
+    Ns[8][0] = -2 * s1 * (1 + s2); Ns[8][1] = (1 - s1 * s1);
+    Ns[9][0] = -(1 - s2 * s2);     Ns[9][1] = -2 * s2 * (1 - s1);
+}
+
+// Rotation matrix of the rotation vector v (Rodrigues).
+void rotation_matrix(const double v[3], double R[3][3]) {
+    const double th = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double k[3] = {0, 0, 0};
+    if (th > 0) for (int i = 0; i < 3; ++i) k[i] = v[i] / th;
+    const double c = std::cos(th), sn = std::sin(th), t = 1 - c;
+    R[0][0] = c + k[0] * k[0] * t;        R[0][1] = k[0] * k[1] * t - k[2] * sn; R[0][2] = k[0] * k[2] * t + k[1] * sn;
+    R[1][0] = k[1] * k[0] * t + k[2] * sn; R[1][1] = c + k[1] * k[1] * t;        R[1][2] = k[1] * k[2] * t - k[0] * sn;
+    R[2][0] = k[2] * k[0] * t - k[1] * sn; R[2][1] = k[2] * k[1] * t + k[0] * sn; R[2][2] = c + k[2] * k[2] * t;
+}
+
 }  // namespace
 
 extern "C" {
+
+// ---------------------------------------------------------------------------
+// a6: shell frame.  Rows of alpha are the local axes: x along the section's
+// first orientation vector, z = x cross (second vector), y = z cross x, all
+// normalised (Working/AStrO_input.f90:1071-1095); then correctAlpha
+// (Working/AStrO_r_elementEqns.f90:240-286) turns the frame about
+// z_sec cross n by asin(|z_sec cross n|) so that its z axis is the element
+// normal n (diagonal cross product, oriented with z_sec).  The reference's
+// r_asin / r_rotateAlpha run on r_sin/r_cos, which read an uninitialised angle
+// (D1); here the rotation is exact.  xyz: 3x4 column-major; alpha[i + 3 j] =
+// component j of axis i.
+void orc_shell_frame(const double* or6, const double* xyz, double* alpha) {
+    double x[3] = {or6[0], or6[1], or6[2]}, v[3] = {or6[3], or6[4], or6[5]};
+    auto norm = [](double* a) {
+        double m = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        for (int i = 0; i < 3; ++i) a[i] /= m;
+    };
+    auto cross = [](const double* a, const double* b, double* c) {
+        c[0] = a[1] * b[2] - a[2] * b[1];
+        c[1] = a[2] * b[0] - a[0] * b[2];
+        c[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double z[3], y[3];
+    norm(x);
+    cross(x, v, z);
+    norm(z);
+    cross(z, x, y);
+    double ax[3][3] = {{x[0], x[1], x[2]}, {y[0], y[1], y[2]}, {z[0], z[1], z[2]}};
+    double d1[3], d2[3], n[3];
+    for (int i = 0; i < 3; ++i) {
+        d1[i] = xyz[i + 3 * 2] - xyz[i + 3 * 0];
+        d2[i] = xyz[i + 3 * 3] - xyz[i + 3 * 1];
+    }
+    cross(d1, d2, n);
+    if (n[0] * z[0] + n[1] * z[1] + n[2] * z[2] < 0) for (int i = 0; i < 3; ++i) n[i] = -n[i];
+    norm(n);
+    double w[3];
+    cross(z, n, w);
+    const double mag = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (mag > 1e-5) {
+        const double th = std::asin(std::min(1.0, mag));
+        for (int i = 0; i < 3; ++i) w[i] *= th / mag;
+        double R[3][3];
+        rotation_matrix(w, R);
+        double out[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int i = 0; i < 3; ++i) {
+                out[r][i] = 0;
+                for (int k = 0; k < 3; ++k) out[r][i] += R[i][k] * ax[r][k];
+            }
+        std::memcpy(ax, out, sizeof ax);
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) alpha[i + 3 * j] = ax[i][j];
+}
+
+// a5: shell ABD by layer integration, Working/AStrO_r_designPropertyFunctions.f90:
+// 259-363.  layers: nL x 6 (E1, E2, nu12, G12, thickness, angle in radians);
+// the stack starts at z = -t (zOff + 1) / 2.  Q = S^-1 of the in-plane
+// compliance, rotated by the layer angle (Ts Q Te^-1), A += dz Q,
+// B += dz2 / 2 Q, D += dz3 / 3 Q; transverse shear and drilling take A33
+// (:359-361, no shear correction).  The reference's r_cos/r_sin read an
+// uninitialised angle (D1) and carry single-precision coefficients (D2): here
+// cos/sin are exact.  ABD: 9x9 column-major.
+void orc_shell_abd(int nL, const double* layers, double zOff, double* ABD) {
+    for (int k = 0; k < 81; ++k) ABD[k] = 0.0;
+    double t = 0;
+    for (int l = 0; l < nL; ++l) t += layers[6 * l + 4];
+    double z = -0.5 * t * (zOff + 1.0);
+    auto A = [&](int r, int c) -> double& { return ABD[(r - 1) + 9 * (c - 1)]; };
+    for (int l = 0; l < nL; ++l) {
+        const double* L = layers + 6 * l;
+        const double zn = z + L[4];
+        double S[9] = {1.0 / L[0], -L[2] / L[0], 0, -L[2] / L[0], 1.0 / L[1], 0, 0, 0, 1.0 / L[3]};
+        double Q[9];
+        lu_factor(S, 3, 0, 2);
+        for (int c = 0; c < 3; ++c) {
+            double f[3] = {0, 0, 0};
+            f[c] = 1.0;
+            lu_solve(Q + 3 * c, S, f, 3, 0, 2);
+        }
+        const double a11 = std::cos(L[5]), a21 = std::sin(L[5]), a12 = -a21, a22 = a11;
+        double Ts[3][3] = {{a11 * a11, a12 * a12, 2 * a11 * a12},
+                           {a21 * a21, a22 * a22, 2 * a22 * a21},
+                           {a11 * a21, a12 * a22, a11 * a22 + a12 * a21}};
+        double Te[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double v = Ts[r][c];
+                if (c == 2) v *= 0.5;
+                if (r == 2) v *= 2.0;
+                Te[r + 3 * c] = v;
+            }
+        double TeInv[9];
+        lu_factor(Te, 3, 0, 2);
+        for (int c = 0; c < 3; ++c) {
+            double f[3] = {0, 0, 0};
+            f[c] = 1.0;
+            lu_solve(TeInv + 3 * c, Te, f, 3, 0, 2);
+        }
+        double QT[3][3], Qb[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                QT[r][c] = 0;
+                for (int k = 0; k < 3; ++k) QT[r][c] += Q[r + 3 * k] * TeInv[k + 3 * c];
+            }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                Qb[r][c] = 0;
+                for (int k = 0; k < 3; ++k) Qb[r][c] += Ts[r][k] * QT[k][c];
+            }
+        for (int r = 1; r <= 3; ++r)
+            for (int c = 1; c <= 3; ++c) {
+                A(r, c) += (zn - z) * Qb[r - 1][c - 1];
+                A(r, c + 3) += 0.5 * (zn * zn - z * z) * Qb[r - 1][c - 1];
+                A(r + 3, c + 3) += (1.0 / 3.0) * (zn * zn * zn - z * z * z) * Qb[r - 1][c - 1];
+            }
+        z = zn;
+    }
+    for (int r = 2; r <= 6; ++r)
+        for (int c = 1; c < r; ++c) A(r, c) = A(c, r);
+    A(7, 7) = A(3, 3);
+    A(8, 8) = A(3, 3);
+    A(9, 9) = A(3, 3);
+}
+
+// a10 (shell branch at u = 0) + a9 + a14: shell4 tangent
+// sum_gp detJ*w * dedU^T ABD dedU (Working/AStrO_r_elementEqns.f90:2245-2423,
+// 2478-2494) with the 9 generalised strains of r_getShellDef (:800-814), then
+// static condensation of the 8 internal DOFs (r_condenseMat :2722-2743).
+// dedU follows r_getInstDof (:703-891) linearised at rest with exact
+// derivatives: local displacement u_L(a) = alpha (u_a - rbar x X_a) and local
+// rotation r_L(a) = alpha (r_a - rbar), rbar = mean nodal rotation (the
+// reference's derivative branch drops d rbar / d r_b from the rotations and
+// reads past instDofMat for the interpolated rotation, D3).  Interpolation:
+// u_L over the 4 nodes plus the bubbles of the internal DOFs, r_L over the 4
+// nodes (:2253-2262).  xyz: 3x4 global coordinates; alpha as orc_shell_frame;
+// ABD 9x9.  K32 (optional): 32x32 before condensation; Kc: 24x24 condensed,
+// both column-major in the element DOF order of dofTable.
+void orc_shell4_stiffness(const double* xyz, const double* alpha, const double* ABD, double* K32, double* Kc) {
+    ElProfile p;
+    element_profile(41, p);
+    double al[3][3], loc[3][4];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) al[i][j] = alpha[i + 3 * j];
+    for (int i = 0; i < 3; ++i)
+        for (int a = 0; a < 4; ++a) {
+            loc[i][a] = 0;
+            for (int k = 0; k < 3; ++k) loc[i][a] += al[i][k] * xyz[k + 3 * a];
+        }
+    std::vector<double> K(33 * 33, 0.0);  // r_condenseMat works on the 33x33 array
+    auto Kf = [&](int r, int c) -> double& { return K[r + 33 * (size_t)c]; };
+    for (int g = 0; g < 4; ++g) {
+        double N[10], Ns[10][2];
+        shell4_shape(p.intPts[g][0], p.intPts[g][1], N, Ns);
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int a = 0; a < 4; ++a) J[i][j] += loc[i][a] * Ns[a][j];
+        J[2][2] = 1.0;
+        const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
+        const double di = 1.0 / det;
+        double I[3][3];
+        I[0][0] = di * a11;
+        I[1][0] = di * a12;
+        I[2][0] = di * a13;
+        I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+        I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+        I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+        double Nx[10][2];
+        for (int n = 0; n < 10; ++n)
+            for (int j = 0; j < 2; ++j) Nx[n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j];
+        double B[9][32];
+        for (int d = 0; d < 32; ++d) {
+            const int var = p.dofTable[0][d], nd = p.dofTable[1][d] - 1;
+            double ux[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rx[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rot[3] = {0, 0, 0};
+            if (d >= 24) {  // internal DOF: local component var on bubble nd
+                for (int j = 0; j < 2; ++j) ux[var - 1][j] = Nx[nd][j];
+            } else if (var <= 3) {  // nodal translation: u_L(nd) = alpha e_var
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 2; ++j) ux[i][j] = al[i][var - 1] * Nx[nd][j];
+            } else {  // nodal rotation e_q at node nd
+                const int q = var - 4;
+                for (int a = 0; a < 4; ++a) {
+                    // d u_L(a) = -alpha (e_q x X_a) / 4
+                    const double* X = xyz + 3 * a;
+                    double ex[3] = {0, 0, 0};
+                    if (q == 0) { ex[1] = -X[2]; ex[2] = X[1]; }
+                    if (q == 1) { ex[0] = X[2]; ex[2] = -X[0]; }
+                    if (q == 2) { ex[0] = -X[1]; ex[1] = X[0]; }
+                    for (int i = 0; i < 3; ++i) {
+                        double v = 0;
+                        for (int k = 0; k < 3; ++k) v += al[i][k] * ex[k];
+                        v *= -0.25;
+                        for (int j = 0; j < 2; ++j) ux[i][j] += v * Nx[a][j];
+                    }
+                    // d r_L(a) = alpha e_q (delta_a,nd - 1/4)
+                    const double f = (a == nd ? 1.0 : 0.0) - 0.25;
+                    for (int i = 0; i < 3; ++i) {
+                        for (int j = 0; j < 2; ++j) rx[i][j] += al[i][q] * f * Nx[a][j];
+                        rot[i] += al[i][q] * f * N[a];
+                    }
+                }
+            }
+            B[0][d] = ux[0][0];
+            B[1][d] = ux[1][1];
+            B[2][d] = ux[0][1] + ux[1][0];
+            B[3][d] = rx[1][0];
+            B[4][d] = -rx[0][1];
+            B[5][d] = rx[1][1] - rx[0][0];
+            B[6][d] = ux[2][0] + rot[1];
+            B[7][d] = ux[2][1] - rot[0];
+            B[8][d] = 2.0 * rot[2] - ux[1][0] + ux[0][1];
+        }
+        const double wd = det * p.ipWt[g];
+        double DB[9][32];
+        for (int r = 0; r < 9; ++r)
+            for (int d = 0; d < 32; ++d) {
+                double acc = 0;
+                for (int k = 0; k < 9; ++k) acc += ABD[r + 9 * k] * B[k][d];
+                DB[r][d] = wd * acc;
+            }
+        for (int j = 0; j < 32; ++j)
+            for (int i = 0; i < 32; ++i) {
+                double acc = 0;
+                for (int k = 0; k < 9; ++k) acc += B[k][i] * DB[k][j];
+                Kf(i, j) += acc;
+            }
+    }
+    if (K32)
+        for (int j = 0; j < 32; ++j)
+            for (int i = 0; i < 32; ++i) K32[i + 32 * j] = Kf(i, j);
+    // condensation: LU of rows/cols 24..31, then each nodal column
+    lu_factor(K.data(), 33, 24, 31);
+    std::vector<double> b(33), x(33);
+    for (int c = 0; c < 24; ++c) {
+        for (int r = 0; r < 33; ++r) b[r] = Kf(r, c);
+        std::fill(x.begin(), x.end(), 0.0);
+        lu_solve(x.data(), K.data(), b.data(), 33, 24, 31);
+        for (int r = 0; r < 24; ++r) {
+            double acc = 0;
+            for (int k = 24; k < 32; ++k) acc += Kf(r, k) * x[k];
+            Kf(r, c) -= acc;
+        }
+    }
+    for (int j = 0; j < 24; ++j)
+        for (int i = 0; i < 24; ++i) Kc[i + 24 * j] = Kf(i, j);
+}
 
 // ---------------------------------------------------------------------------
 // a4: material stiffness C = S^-1 from 9 engineering constants
@@ -689,6 +992,8 @@ struct OrcModel {
     std::vector<double> elC;        // 36*E
     std::vector<double> elDen;      // E
     std::vector<double> elCond;     // 9*E
+    std::vector<double> elABD;      // 81*E (shell elements)
+    std::vector<double> elOrient;   // 6*E section orientation vectors (shell elements)
     // constraints (Working/AStrO_input.f90 constraints block, after set expansion
     // of the node names): term t has equation mpcEqn, dof, coef and either a
     // plain node (isSet = 0, one node) or a node set (isSet = 1, node list)
@@ -894,6 +1199,17 @@ OrcModel* orc_create(int numNodes, const double* nodeList, int numEls, const int
 }
 
 void orc_free(OrcModel* m) { delete m; }
+
+// Shell sections per element: nL layers of (E1, E2, nu12, G12, thickness,
+// angle) each, the z offset and the two orientation vectors of the section.
+void orc_set_shell_sections(OrcModel* m, int nL, const double* layers, const double* zOff, const double* orient6) {
+    m->elABD.assign(81 * (size_t)m->numEls, 0.0);
+    m->elOrient.assign(6 * (size_t)m->numEls, 0.0);
+    for (int e = 0; e < m->numEls; ++e) {
+        orc_shell_abd(nL, layers + 6 * (size_t)nL * e, zOff ? zOff[e] : 0.0, &m->elABD[81 * (size_t)e]);
+        for (int k = 0; k < 6; ++k) m->elOrient[6 * (size_t)e + k] = orient6[6 * (size_t)e + k];
+    }
+}
 
 // Per-element properties: eprops (9*E engineering constants), den (E),
 // cond6 (6*E: k11,k22,k33,k12,k13,k23).  Any pointer may be null.
@@ -1104,8 +1420,30 @@ void orc_assemble_elastic(OrcModel* mp, double massFactor) {
     std::vector<double> Ke(576), Me(576);
     double xyz[24];
     for (int e = 0; e < m.numEls; ++e) {
+        if (m.elementType[e] == 41) {
+            if (massFactor != 0.0 || m.elABD.empty()) {
+                std::fprintf(stderr, "oracle: shell4 needs sections and supports statics only\n");
+                std::abort();
+            }
+            double alpha[9];
+            orc_element_coords(&m, e, xyz);
+            orc_shell_frame(&m.elOrient[6 * (size_t)e], xyz, alpha);
+            orc_shell4_stiffness(xyz, alpha, &m.elABD[81 * (size_t)e], nullptr, Ke.data());
+            ElProfile p;
+            element_profile(41, p);
+            for (int i = 0; i < 24; ++i) {
+                int kGi = m.nDofIndex[6 * (size_t)(m.elementList[8 * (size_t)e + p.dofTable[1][i] - 1] - 1) +
+                                      p.dofTable[0][i] - 1];
+                for (int j = 0; j < 24; ++j) {
+                    int kGj = m.nDofIndex[6 * (size_t)(m.elementList[8 * (size_t)e + p.dofTable[1][j] - 1] - 1) +
+                                          p.dofTable[0][j] - 1];
+                    probe_add(m.elasticMat, m.elMatCols, m.elMatRange, kGi, kGj, Ke[i + 24 * j], false);
+                }
+            }
+            continue;
+        }
         if (m.elementType[e] != 8) {
-            std::fprintf(stderr, "oracle: assemble supports brick8 only\n");
+            std::fprintf(stderr, "oracle: assemble supports brick8 and shell4 only\n");
             std::abort();
         }
         orc_element_coords(&m, e, xyz);

@@ -70,6 +70,10 @@ def lib():
         L.orc_brick8_stiffness.argtypes = [_dp, _dp, _dp]
         L.orc_brick8_mass.argtypes = [_dp, C.c_double, _dp]
         L.orc_brick8_internal_force.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_shell_frame.argtypes = [_dp, _dp, _dp]
+        L.orc_shell_abd.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        L.orc_shell4_stiffness.argtypes = [_dp, _dp, _dp, C.c_void_p, _dp]
+        L.orc_set_shell_sections.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.orc_dyn_init.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.orc_dyn_residual_load.argtypes = [C.c_void_p, C.c_double]
         L.orc_dyn_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double]
@@ -124,6 +128,32 @@ def brick8_internal_force(xyz38, Cmat, ue24):
     lib().orc_brick8_internal_force(_f64(np.asarray(xyz38).reshape(3, 8).ravel(order="F")),
                                     _f64(np.asarray(Cmat).ravel(order="F")), _f64(ue24), R)
     return R
+
+
+def shell_frame(orient6, xyz34):
+    """Rows = local axes of a shell4 element (section orientation aligned to the
+    element normal)."""
+    a = np.zeros(9)
+    lib().orc_shell_frame(_f64(orient6), _f64(np.asarray(xyz34).reshape(3, 4).ravel(order="F")), a)
+    return a.reshape(3, 3, order="F")
+
+
+def shell_abd(layers, z_off=0.0):
+    """9x9 ABD of a layup; layers: (nL, 6) E1, E2, nu12, G12, thickness, angle."""
+    L = np.asarray(layers, np.float64).reshape(-1, 6)
+    A = np.zeros(81)
+    lib().orc_shell_abd(L.shape[0], _f64(L.ravel()), float(z_off), A)
+    return A.reshape(9, 9, order="F")
+
+
+def shell4_stiffness(xyz34, alpha33, abd99):
+    """(K32 before condensation, Kc 24x24 condensed) of one shell4."""
+    K32 = np.zeros(32 * 32)
+    Kc = np.zeros(24 * 24)
+    lib().orc_shell4_stiffness(_f64(np.asarray(xyz34).reshape(3, 4).ravel(order="F")),
+                               _f64(np.asarray(alpha33).ravel(order="F")), _f64(np.asarray(abd99).ravel(order="F")),
+                               K32.ctypes.data, Kc)
+    return K32.reshape(32, 32, order="F"), Kc.reshape(24, 24, order="F")
 
 
 def brick8_conductivity(xyz38, k33):
@@ -208,6 +238,10 @@ class OracleModel:
         if model.load_nodes.size:
             L.orc_set_nodal_loads(self.h, int(model.load_nodes.size), _i32(model.load_nodes),
                                   _f64(model.load_vals.reshape(-1)))
+        if model.layers is not None:
+            lay = _f64(model.layers.reshape(-1))
+            L.orc_set_shell_sections(self.h, int(model.layers.shape[1]), lay, _f64(model.z_offset),
+                                     _f64(model.orient.reshape(-1)))
 
     def __del__(self):
         try:
