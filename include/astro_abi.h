@@ -130,6 +130,9 @@ typedef struct {
     int32_t numVecs;          /* GMRES restart length; <= 0: reference rule (n/2 below 200, else 100) */
     double rtol;              /* GMRES relative tolerance on |P^-1 r| (default 1e-12 when <= 0) */
     int32_t maxCycles;        /* GMRES restart cycles cap; <= 0: 1000 */
+    int32_t coupledProfile;   /* 0: profile rows of one component, as the reference (the factor is a
+                                 component-block preconditioner); 1: rows reach every component of the
+                                 node and its neighbours (the factor solves the system directly) */
 } astro_options;
 
 /* Host-only analysisPrep (Working/AStrO_bookKeeping.f90:794-1039): level
@@ -202,6 +205,19 @@ int astro_ctx_static_solve(astro_ctx* ctx);
 int astro_ctx_dynamic_init(astro_ctx* ctx, double beta, double gamma, double dt);
 int astro_ctx_dynamic_step(astro_ctx* ctx, double loadScale);
 int astro_ctx_dynamic_run(astro_ctx* ctx, int nSteps, const double* loadScale);
+
+/* Design sensitivities by complex step (shell4 layer thickness), replacing
+ * getdRelasticdD's per-variable loop (Working/AStrO_commandFunctions.f90:
+ * 1172-1375, element-id and DOF indexing corrected) and, for compliance,
+ * getTotaldLdD (:1377-1466).  set_design: element e's layer dvLayer (0-based)
+ * thickness changes by elCoef[e] * d[elDV[e]] (elDV < 0: none; elCoef null:
+ * 1); h = complex step.  dRdD: elMatDim x nDV column-major at the current
+ * displacement.  compliance_gradient: dC/dd (nDV) of C = F . u by the adjoint
+ * lambda = u.  All design variables are evaluated in one launch per element
+ * colour. */
+int astro_ctx_set_design(astro_ctx* ctx, int nDV, const int32_t* elDV, const double* elCoef, int dvLayer, double h);
+int astro_ctx_dRdD(astro_ctx* ctx, double* dRdD);
+int astro_ctx_compliance_gradient(astro_ctx* ctx, double* grad);
 
 /* Thermal twins (Working/AStrO_commandFunctions.f90:16-74, 229-258, 689-703,
  * 763-769). */

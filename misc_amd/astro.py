@@ -34,7 +34,8 @@ EXPORTS = [
     "astro_ctx_operator_apply", "astro_ctx_sync", "astro_ctx_phase_ms", "astro_prep_create", "astro_prep_destroy",
     "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64", "astro_ctx_set_timing",
     "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual", "astro_ctx_dynamic_init",
-    "astro_ctx_dynamic_step", "astro_ctx_dynamic_run",
+    "astro_ctx_dynamic_step", "astro_ctx_dynamic_run", "astro_ctx_set_design", "astro_ctx_dRdD",
+    "astro_ctx_compliance_gradient",
 ]
 
 
@@ -64,7 +65,8 @@ class _Model(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("maxBW", C.c_int32), ("literalMaxBW", C.c_int32), ("thermLiteralD8", C.c_int32),
-                ("numVecs", C.c_int32), ("rtol", C.c_double), ("maxCycles", C.c_int32)]
+                ("numVecs", C.c_int32), ("rtol", C.c_double), ("maxCycles", C.c_int32),
+                ("coupledProfile", C.c_int32)]
 
 
 def load_library(path: str = LIB_PATH):
@@ -99,6 +101,9 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.astro_ctx_dynamic_step.argtypes = [C.c_void_p, C.c_double]
     L.astro_ctx_dynamic_run.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.astro_ctx_set_design.argtypes = [C.c_void_p, C.c_int, _ip, C.c_void_p, C.c_int, C.c_double]
+    L.astro_ctx_dRdD.argtypes = [C.c_void_p, _dp]
+    L.astro_ctx_compliance_gradient.argtypes = [C.c_void_p, _dp]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_ctx_operator_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_prep_create.argtypes = [C.POINTER(_Model), C.POINTER(_Options), C.POINTER(C.c_void_p)]
@@ -193,6 +198,7 @@ class Options:
     numVecs: int = 0
     rtol: float = 1e-12
     maxCycles: int = 1000
+    coupledProfile: bool = False
 
 
 def _marshal(model, o: "Options"):
@@ -234,7 +240,7 @@ def _marshal(model, o: "Options"):
         M.zOffset = ptr(model.z_offset, np.float64)
         M.orient = ptr(model.orient.reshape(-1), np.float64)
     O = _Options(int(o.maxBW), 1 if o.literalMaxBW else 0, 1 if o.thermLiteralD8 else 0, int(o.numVecs),
-                 float(o.rtol), int(o.maxCycles))
+                 float(o.rtol), int(o.maxCycles), 1 if o.coupledProfile else 0)
     return M, O, keep
 
 
@@ -374,6 +380,26 @@ class Context:
     def dynamic_run(self, load_scales):
         f = np.ascontiguousarray(load_scales, dtype=np.float64)
         _check(load_library().astro_ctx_dynamic_run(self.h, int(f.size), f))
+
+    def set_design(self, el_dv, el_coef=None, dv_layer: int = 0, h: float = 1e-30):
+        """Design variables for complex-step sensitivities (shell4 layer thickness)."""
+        dv = np.ascontiguousarray(el_dv, dtype=np.int32)
+        self._coef = None if el_coef is None else np.ascontiguousarray(el_coef, dtype=np.float64)
+        self.n_dv = int(dv.max()) + 1
+        _check(load_library().astro_ctx_set_design(self.h, self.n_dv, dv,
+                                                   None if self._coef is None else self._coef.ctypes.data,
+                                                   int(dv_layer), float(h)))
+
+    def dRdD(self):
+        n = self.scalar("elMatDim")
+        out = np.zeros(n * self.n_dv)
+        _check(load_library().astro_ctx_dRdD(self.h, out))
+        return out.reshape(self.n_dv, n).T
+
+    def compliance_gradient(self):
+        g = np.zeros(self.n_dv)
+        _check(load_library().astro_ctx_compliance_gradient(self.h, g))
+        return g
 
     def assemble_thermal(self):
         _check(load_library().astro_ctx_assemble_thermal(self.h))

@@ -530,6 +530,14 @@ struct astro_ctx {
     // model on device
     DBuf<double> X, matC, elScale, elRho, elCond;
     DBuf<double> elABD, elAlpha;  // shell4 models: layup stiffness 81, frame 9 per element
+    int nLayers = 0;
+    std::vector<double> hLayers, hZOff;  // host copy of the shell sections
+    // design variables (complex-step sensitivities)
+    int nDV = 0;
+    std::vector<int> hElDV;
+    DBuf<int> elDV;
+    DBuf<double> elABDim, dRdD, gPart;
+    double csH = 1e-30;
     DBuf<int> conn, elMat, rank, perm;
     DBuf<uint8_t> elPos, thPos;
     std::vector<int> colorStart;
@@ -630,6 +638,10 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
         }
         ASTRO_TRY(c->elABD.upload(abd, s));
         ASTRO_TRY(c->elAlpha.upload(al, s));
+        c->nLayers = m->numLayers;
+        c->hLayers.assign(m->layers, m->layers + 6 * (size_t)m->numLayers * E);
+        c->hZOff.assign(E, 0.0);
+        if (m->zOffset) c->hZOff.assign(m->zOffset, m->zOffset + E);
     }
     ASTRO_TRY(c->matC.upload(mats, s));
     ASTRO_TRY(c->elMat.upload(elMat, s));
@@ -714,7 +726,8 @@ int astro_ctx_create(const astro_model* m, const astro_options* o, astro_ctx** o
     HIP_TRY(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     ModelDesc d = to_desc(m);
-    std::string err = analysis_prep(d, c->opt.maxBW, c->opt.literalMaxBW != 0, c->opt.thermLiteralD8 != 0, c->prep);
+    std::string err = analysis_prep(d, c->opt.maxBW, c->opt.literalMaxBW != 0, c->opt.thermLiteralD8 != 0, c->prep,
+                                    c->opt.coupledProfile != 0);
     if (!err.empty()) return fail(ASTRO_ERR_ARG, err);
     for (int e = 0; e < m->numEls; ++e)
         if (m->elementType[e] != (c->prep.shellMesh ? 41 : 8))
@@ -734,7 +747,8 @@ int astro_prep_create(const astro_model* m, const astro_options* o, astro_prep**
     astro_options opt{};
     if (o) opt = *o;
     std::unique_ptr<astro_prep> p(new astro_prep);
-    std::string err = analysis_prep(to_desc(m), opt.maxBW, opt.literalMaxBW != 0, opt.thermLiteralD8 != 0, p->p);
+    std::string err = analysis_prep(to_desc(m), opt.maxBW, opt.literalMaxBW != 0, opt.thermLiteralD8 != 0, p->p,
+                                    opt.coupledProfile != 0);
     if (!err.empty()) return fail(ASTRO_ERR_ARG, err);
     *out = p.release();
     return ASTRO_OK;
@@ -1109,6 +1123,79 @@ int astro_ctx_dynamic_run(astro_ctx* c, int nSteps, const double* loadScale) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
     c->dynMs = ms;
+    return ASTRO_OK;
+}
+
+// Design variables for complex-step sensitivities of a shell4 model: element
+// e's layer dvLayer thickness changes by elCoef[e] * d[elDV[e]] (elDV < 0: no
+// variable; elCoef null: 1).  h is the complex step.  The imaginary parts of
+// the perturbed layups are formed here, once.
+int astro_ctx_set_design(astro_ctx* c, int nDV, const int32_t* elDV, const double* elCoef, int dvLayer, double h) {
+    if (!c || !elDV) return fail(ASTRO_ERR_ARG, "null argument");
+    if (!c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "design sensitivities cover shell4 layer thickness");
+    if (nDV < 1 || dvLayer < 0 || dvLayer >= c->nLayers || !(h > 0.0))
+        return fail(ASTRO_ERR_ARG, "bad design-variable count, layer or step");
+    const int E = c->prep.numEls;
+    std::vector<double> im(81 * (size_t)E, 0.0), re(81);
+    c->hElDV.assign(elDV, elDV + E);
+    for (int e = 0; e < E; ++e) {
+        const int k = elDV[e];
+        if (k >= nDV) return fail(ASTRO_ERR_ARG, "design variable index out of range");
+        if (k < 0) continue;
+        const double coef = elCoef ? elCoef[e] : 1.0;
+        shell_section_abd_cs(c->nLayers, &c->hLayers[6 * (size_t)c->nLayers * e], c->hZOff[e], dvLayer, h * coef,
+                             re.data(), &im[81 * (size_t)e]);
+    }
+    c->nDV = nDV;
+    c->csH = h;
+    ASTRO_TRY(c->elDV.upload(c->hElDV, c->s));
+    ASTRO_TRY(c->elABDim.upload(im, c->s));
+    ASTRO_TRY(c->dRdD.alloc((size_t)c->prep.elMatDim * nDV));
+    ASTRO_TRY(c->gPart.alloc((size_t)E));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+static int ctx_sensitivities(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (c->nDV < 1) return fail(ASTRO_ERR_ARG, "astro_ctx_set_design has not been called");
+    const int64_t n = c->prep.elMatDim;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n * c->nDV, 256, 8192)), dim3(256), 0, c->s, c->dRdD.p, n * c->nDV,
+                       0.0);
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_shell4_dtangent_cs, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
+                           c->conn.p, c->X.p, c->elABD.p, c->elABDim.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
+                           c->prep.numSectionNds, c->nodeDisp.p, c->elDV.p, 1.0 / c->csH, n, c->dRdD.p, c->gPart.p);
+    }
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+// dR/dd at the current displacement, all design variables in one launch per
+// element colour: column k = sum over the elements of variable k of
+// Im(Kc(t + i h coef) u_e) / h.  Working/AStrO_commandFunctions.f90:1172-1375.
+int astro_ctx_dRdD(astro_ctx* c, double* dRdD) {
+    if (!dRdD) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(ctx_sensitivities(c));
+    HIP_TRY(hipMemcpyAsync(dRdD, c->dRdD.p, sizeof(double) * c->dRdD.n, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+// Gradient of the compliance C = F . u (loads independent of d) by the
+// adjoint lambda = u: dC/dd_k = -u . dR/dd_k, summed per variable over its
+// elements in element order.  Working/AStrO_commandFunctions.f90:1377-1466.
+int astro_ctx_compliance_gradient(astro_ctx* c, double* grad) {
+    if (!grad) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(ctx_sensitivities(c));
+    std::vector<double> g(c->prep.numEls);
+    HIP_TRY(hipMemcpyAsync(g.data(), c->gPart.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    for (int k = 0; k < c->nDV; ++k) grad[k] = 0.0;
+    for (size_t e = 0; e < g.size(); ++e)
+        if (c->hElDV[e] >= 0) grad[c->hElDV[e]] += g[e];
     return ASTRO_OK;
 }
 

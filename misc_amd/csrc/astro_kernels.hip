@@ -158,26 +158,20 @@ __global__ __launch_bounds__(64) void k_brick8_tangent(
 // nodes 1..4, then internal (1,5) (1,6) (2,5) (2,6) (3,7) (3,8) (3,9) (3,10).
 // Rows of node a start at csrRange[dof(a, v) - 1], dof(a, v) = rank(a) +
 // v * nSec + 2 * nSolid; the slot of (node b, var w) is elPos[e][(a*4+b)*6+w].
-__global__ __launch_bounds__(64) void k_shell4_tangent(
-    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
-    const double* __restrict__ elABD, const double* __restrict__ elAlpha, const int* __restrict__ rank, int nSolid,
-    int nSec, const int* __restrict__ csrRange, const uint8_t* __restrict__ elPos, double* __restrict__ val) {
-    const int idx = blockIdx.x;
-    if (idx >= count) return;
-    const int e = perm[idx];
-    const int lane = threadIdx.x;
-    __shared__ double sX[4][3], sAl[3][3], sD[9][9];
-    __shared__ double sN[4][10], sNx[4][10][2], sW[4];
-    __shared__ double sB[4][9][32], sDB[4][9][32];
-    __shared__ double sK[32][33];
-    __shared__ double sXs[24][8];
-    __shared__ int sNode[4];
+// Shared stage of the shell4 kernels: node ids and coordinates, frame, shape
+// functions at the 4 Gauss points, and the 9 x 32 strain-displacement matrix B
+// per Gauss point (lane = DOF, two Gauss points per lane half).  Leaves sB,
+// sN, sNx, sW, sX, sAl, sNode in LDS; ends with a barrier.
+__device__ __forceinline__ void shell4_stage_B(int e, int lane, const int* __restrict__ conn,
+                                               const double* __restrict__ X, const double* __restrict__ elAlpha,
+                                               double (&sX)[4][3], double (&sAl)[3][3], double (&sN)[4][10],
+                                               double (&sNx)[4][10][2], double (&sW)[4], double (&sB)[4][9][32],
+                                               int (&sNode)[4]) {
     if (lane < 4) {
         const int nd = conn[8 * (size_t)e + lane] - 1;
         sNode[lane] = nd;
         for (int c = 0; c < 3; ++c) sX[lane][c] = X[3 * (size_t)nd + c];
     }
-    for (int k = lane; k < 81; k += 64) sD[k % 9][k / 9] = elABD[81 * (size_t)e + k];
     if (lane < 9) sAl[lane % 3][lane / 3] = elAlpha[9 * (size_t)e + lane];
     __syncthreads();
     if (lane < 4) {
@@ -280,6 +274,25 @@ __global__ __launch_bounds__(64) void k_shell4_tangent(
         }
     }
     __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_shell4_tangent(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ elABD, const double* __restrict__ elAlpha, const int* __restrict__ rank, int nSolid,
+    int nSec, const int* __restrict__ csrRange, const uint8_t* __restrict__ elPos, double* __restrict__ val) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[4][3], sAl[3][3], sD[9][9];
+    __shared__ double sN[4][10], sNx[4][10][2], sW[4];
+    __shared__ double sB[4][9][32], sDB[4][9][32];
+    __shared__ double sK[32][33];
+    __shared__ double sXs[24][8];
+    __shared__ int sNode[4];
+    for (int k = lane; k < 81; k += 64) sD[k % 9][k / 9] = elABD[81 * (size_t)e + k];
+    shell4_stage_B(e, lane, conn, X, elAlpha, sX, sAl, sN, sNx, sW, sB, sNode);
+    __syncthreads();
     for (int k = lane; k < 4 * 9 * 32; k += 64) {
         const int g = k / 288, r = (k / 32) % 9, d = k % 32;
         double acc = 0.0;
@@ -335,6 +348,128 @@ __global__ __launch_bounds__(64) void k_shell4_tangent(
         const int row = rank[sNode[a]] + vi * nSec + 2 * nSolid;
         const int slot = elPos[192 * (size_t)e + (a * 4 + b) * 6 + vj];
         val[csrRange[row - 1] + slot] += v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Design sensitivities by complex step, shell4 layer thickness: every element
+// evaluates its condensed tangent with the thickness of its design
+// variable's layer perturbed by i h coef (ABD real and imaginary parts from
+// the host), in complex arithmetic through the condensation, and contributes
+// Im(Kc u_e) / h = coef dKc/dt u_e to column elDV[e] of dRdD (n x nDV,
+// column-major) and -u_e . that to gPart[e] (the adjoint compliance term).
+// Elements of one launch share no node; elements of different design
+// variables write different columns.  Working/AStrO_commandFunctions.f90:
+// 1172-1375 (getdRelasticdD) and the AStrO_c_elementEqns twin of the shell
+// tangent.
+struct cdbl {
+    double r, i;
+};
+__device__ __forceinline__ cdbl cmul(cdbl a, cdbl b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cdbl csub(cdbl a, cdbl b) { return {a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cdbl cdiv(cdbl a, cdbl b) {
+    const double d = 1.0 / (b.r * b.r + b.i * b.i);
+    return {(a.r * b.r + a.i * b.i) * d, (a.i * b.r - a.r * b.i) * d};
+}
+
+__global__ __launch_bounds__(64) void k_shell4_dtangent_cs(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ elABD, const double* __restrict__ elABDim, const double* __restrict__ elAlpha,
+    const int* __restrict__ rank, int nSolid, int nSec, const double* __restrict__ u, const int* __restrict__ elDV,
+    double invH, int64_t n, double* __restrict__ dRdD, double* __restrict__ gPart) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[4][3], sAl[3][3], sDr[9][9], sDi[9][9];
+    __shared__ double sN[4][10], sNx[4][10][2], sW[4];
+    __shared__ double sB[4][9][32], sDBr[4][9][32], sDBi[4][9][32];
+    __shared__ cdbl sK[32][33];
+    __shared__ cdbl sXs[24][8];
+    __shared__ double sU[24], sV[24];
+    __shared__ int sNode[4], sDof[24];
+    for (int k = lane; k < 81; k += 64) {
+        sDr[k % 9][k / 9] = elABD[81 * (size_t)e + k];
+        sDi[k % 9][k / 9] = elABDim[81 * (size_t)e + k];
+    }
+    shell4_stage_B(e, lane, conn, X, elAlpha, sX, sAl, sN, sNx, sW, sB, sNode);
+    for (int k = lane; k < 4 * 9 * 32; k += 64) {
+        const int g = k / 288, r = (k / 32) % 9, d = k % 32;
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) {
+            ar += sDr[r][m] * sB[g][m][d];
+            ai += sDi[r][m] * sB[g][m][d];
+        }
+        sDBr[g][r][d] = sW[g] * ar;
+        sDBi[g][r][d] = sW[g] * ai;
+    }
+    if (lane < 24) {
+        const int var = lane >> 2, a = lane & 3;
+        const int row = rank[sNode[a]] + var * nSec + 2 * nSolid;
+        sDof[lane] = row;
+        sU[lane] = u[row - 1];
+    }
+    __syncthreads();
+    for (int k = lane; k < 32 * 32; k += 64) {
+        const int i = k >> 5, j = k & 31;
+        double ar = 0.0, ai = 0.0;
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                ar += sB[g][m][i] * sDBr[g][m][j];
+                ai += sB[g][m][i] * sDBi[g][m][j];
+            }
+        sK[i][j] = {ar, ai};
+    }
+    __syncthreads();
+    if (lane == 0) {  // complex LU (no pivoting) of the internal block
+        for (int i = 24; i < 32; ++i) {
+            for (int j = i; j < 32; ++j)
+                for (int k = 24; k < i; ++k) sK[i][j] = csub(sK[i][j], cmul(sK[i][k], sK[k][j]));
+            const cdbl piv = sK[i][i];
+            for (int j = i + 1; j < 32; ++j) {
+                for (int k = 24; k < i; ++k) sK[j][i] = csub(sK[j][i], cmul(sK[j][k], sK[k][i]));
+                sK[j][i] = cdiv(sK[j][i], piv);
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < 24) {
+        cdbl z[8], x[8];
+        for (int i = 0; i < 8; ++i) {
+            cdbl v = sK[24 + i][lane];
+            for (int k = 0; k < i; ++k) v = csub(v, cmul(sK[24 + i][24 + k], z[k]));
+            z[i] = v;
+        }
+        for (int i = 7; i >= 0; --i) {
+            cdbl v = z[i];
+            for (int k = i + 1; k < 8; ++k) v = csub(v, cmul(sK[24 + i][24 + k], x[k]));
+            x[i] = cdiv(v, sK[24 + i][24 + i]);
+        }
+        for (int i = 0; i < 8; ++i) sXs[lane][i] = x[i];
+    }
+    __syncthreads();
+    const int dv = elDV[e];
+    if (lane < 24) {
+        // Im(Kc) u with Kc = Kcc - Kci X
+        const int i = lane;
+        double v = 0.0;
+        for (int j = 0; j < 24; ++j) {
+            double im = sK[i][j].i;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) im -= cmul(sK[i][24 + m], sXs[j][m]).i;
+            v += im * sU[j];
+        }
+        v *= invH;
+        sV[i] = v;
+        if (dv >= 0) dRdD[(size_t)dv * n + sDof[i] - 1] += v;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double g = 0.0;
+        for (int i = 0; i < 24; ++i) g -= sU[i] * sV[i];
+        gPart[e] = dv >= 0 ? g : 0.0;
     }
 }
 

@@ -147,7 +147,8 @@ int solver_max_bw(int numNodes, bool literal) {
     return (int)b;
 }
 
-std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool thermLiteralD8, Prep& p) {
+std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool thermLiteralD8, Prep& p,
+                          bool coupledProfile) {
     const int N = m.numNodes, E = m.numEls;
     if (N <= 0 || E <= 0) return "empty model";
     for (int e = 0; e < E; ++e) {
@@ -241,20 +242,30 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
             if (dofOf(i, c)) p.elMatRange[dofOf(i, c)] = cap;
     }
     for (int r = 1; r <= n; ++r) p.elMatRange[r] += p.elMatRange[r - 1];
-    // profile lengths: same-component lower-ranked neighbours within solverMaxBW (:380-409)
+    // profile lengths: same-component lower-ranked neighbours within solverMaxBW
+    // (:380-409); with coupledProfile every component of the node and of those
+    // neighbours
     p.elMatLTRange.assign(n + 1, 0);
     for (int i = 1; i <= N; ++i)
         for (int c = 0; c < 6; ++c) {
             int r = dofOf(i, c);
             if (!r) continue;
             int len = 1;
+            if (coupledProfile)  // the node's own lower components
+                for (int c2 = 0; c2 < 6; ++c2) {
+                    int q = dofOf(i, c2);
+                    if (q != 0 && q <= r) len = std::max(len, r - q + 1);
+                }
             for (int s = p.nCRange[i - 1]; s < p.nCRange[i]; ++s) {
                 int nb = p.nodalConn[s];
                 if (!nb) continue;
                 int dr = p.currentRank[i - 1] - p.currentRank[nb - 1];
                 if (dr < 0 || dr > p.solverMaxBW) continue;
-                int q = dofOf(nb, c);
-                if (q != 0 && q <= r) len = std::max(len, r - q + 1);
+                for (int c2 = 0; c2 < 6; ++c2) {
+                    if (!coupledProfile && c2 != c) continue;
+                    int q = dofOf(nb, c2);
+                    if (q != 0 && q <= r) len = std::max(len, r - q + 1);
+                }
             }
             p.elMatLTRange[r] = len;
         }

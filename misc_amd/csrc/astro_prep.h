@@ -60,8 +60,11 @@ struct Prep {
 int solver_max_bw(int numNodes, bool literal);
 
 // Full analysis.  maxBW <= 0 selects solver_max_bw(numNodes, literalMaxBW).
+// coupledProfile widens every profile row to all components of the node and
+// its neighbours (the reference keeps one component per row).
 // Returns "" on success or an error message.
-std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool thermLiteralD8, Prep& out);
+std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool thermLiteralD8, Prep& out,
+                          bool coupledProfile = false);
 
 // Shell section data (astro_shell.cpp): frame rows = local axes from the two
 // orientation vectors aligned to the element normal; alpha[i + 3 j].  Layup
@@ -69,5 +72,9 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
 // thickness, angle in radians) and the z offset.
 void shell_frame(const double* orient6, const double* xyz34, double* alpha);
 void shell_section_abd(int nL, const double* layers, double zOff, double* ABD);
+// The same with the thickness of `layer` carrying the imaginary part hc
+// (complex step); real and imaginary parts of ABD separately.
+void shell_section_abd_cs(int nL, const double* layers, double zOff, int layer, double hc, double* ABDre,
+                          double* ABDim);
 
 }  // namespace astro

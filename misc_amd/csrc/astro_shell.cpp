@@ -2,6 +2,7 @@
 // layup stiffness ABD of every shell4 element, computed once per model.
 
 #include <cmath>
+#include <complex>
 
 #include "astro_prep.h"
 
@@ -130,6 +131,47 @@ void shell_section_abd(int nL, const double* layers, double zOff, double* ABD) {
         z0 = z1;
     }
     ABD[6 + 9 * 6] = ABD[7 + 9 * 7] = ABD[8 + 9 * 8] = ABD[2 + 9 * 2];
+}
+
+}  // namespace astro
+
+namespace astro {
+
+// Complex-step layup stiffness: the thickness of layer `layer` carries the
+// imaginary part hc; the integration of shell_section_abd runs in complex
+// arithmetic (the reference's complex twin c_getShellStiffness).  The real
+// and imaginary parts of ABD go to ABDre / ABDim (9 x 9 column-major).
+void shell_section_abd_cs(int nL, const double* layers, double zOff, int layer, double hc, double* ABDre,
+                          double* ABDim) {
+    using cd = std::complex<double>;
+    cd A[81];
+    for (cd& v : A) v = 0.0;
+    cd t = 0.0;
+    for (int l = 0; l < nL; ++l) t += cd(layers[6 * l + 4], l == layer ? hc : 0.0);
+    cd z0 = -0.5 * t * (zOff + 1.0);
+    for (int l = 0; l < nL; ++l) {
+        const double* L = layers + 6 * l;
+        const cd z1 = z0 + cd(L[4], l == layer ? hc : 0.0);
+        // the real layer matrix Qbar, from the real routine with a unit layer
+        double lay[6] = {L[0], L[1], L[2], L[3], 1.0, L[5]};
+        double Q1[81];
+        shell_section_abd(1, lay, -1.0, Q1);  // zOff = -1: z in [0, 1], A block = Qbar
+        const cd fA = z1 - z0, fB = 0.5 * (z1 * z1 - z0 * z0), fD = (z1 * z1 * z1 - z0 * z0 * z0) / 3.0;
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) {
+                const double Qb = Q1[r + 9 * q];
+                A[r + 9 * q] += fA * Qb;
+                A[r + 9 * (q + 3)] += fB * Qb;
+                A[(r + 3) + 9 * q] += fB * Q1[q + 9 * r];
+                A[(r + 3) + 9 * (q + 3)] += fD * Qb;
+            }
+        z0 = z1;
+    }
+    A[6 + 9 * 6] = A[7 + 9 * 7] = A[8 + 9 * 8] = A[2 + 9 * 2];
+    for (int k = 0; k < 81; ++k) {
+        ABDre[k] = A[k].real();
+        ABDim[k] = A[k].imag();
+    }
 }
 
 }  // namespace astro

@@ -284,3 +284,18 @@ def shell_plate(nx, ny, lx=1.0, ly=0.1, E=70e9, nu=0.33, thickness=0.002, t_fiel
     vals[:, 2] = tip_load
     m.load_nodes, m.load_vals = tip, vals
     return m
+
+
+def strip_plate(nx=128, ny=16, lx=1.0, ly=0.1, E=70e9, nu=0.33, t0=0.002, t_spread=0.1, strip=2, seed=0,
+                tip_load=1.0, name=None):
+    """The design-sensitivity workload: cantilever shell plate (see
+    shell_plate) whose element thickness is t0 (1 + U(-t_spread, t_spread))
+    from seed, with one design variable per `strip` element columns (the
+    thickness of the strip's single layer).  Returns (model, elDV) with elDV
+    the 0-based design variable of each element."""
+    rng = np.random.default_rng(seed)
+    t = t0 * (1.0 + rng.uniform(-t_spread, t_spread, size=nx * ny))
+    m = shell_plate(nx, ny, lx=lx, ly=ly, E=E, nu=nu, t_field=t, tip_load=tip_load,
+                    name=name or f"strips{nx}x{ny}")
+    ei = np.tile(np.arange(nx), ny)
+    return m, (ei // strip).astype(np.int32)

@@ -155,12 +155,13 @@ double brick_ipdata(const double* xyz, const double s[3], double N[8], double Nx
 
 // Doolittle LU without pivoting on rows/cols [st, en] (0-based, inclusive) of
 // a column-major n x n matrix.  Working/AStrO_r_designPropertyFunctions.f90:1015-1039.
-void lu_factor(double* A, int n, int st, int en) {
-    auto a = [&](int r, int c) -> double& { return A[r + (size_t)n * c]; };
+template <typename T>
+void lu_factor(T* A, int n, int st, int en) {
+    auto a = [&](int r, int c) -> T& { return A[r + (size_t)n * c]; };
     for (int i = st; i <= en; ++i) {
         for (int j = i; j <= en; ++j)
             for (int k = st; k < i; ++k) a(i, j) -= a(i, k) * a(k, j);
-        double f = 1.0 / a(i, i);
+        T f = T(1.0) / a(i, i);
         for (int j = i + 1; j <= en; ++j) {
             for (int k = st; k < i; ++k) a(j, i) -= a(j, k) * a(k, i);
             a(j, i) = f * a(j, i);
@@ -170,8 +171,9 @@ void lu_factor(double* A, int n, int st, int en) {
 
 // Forward/back substitution with the factor of lu_factor.
 // Working/AStrO_r_designPropertyFunctions.f90:1042-1073.
-void lu_solve(double* x, const double* LU, const double* b, int n, int st, int en) {
-    std::vector<double> z(n, 0.0);
+template <typename T>
+void lu_solve(T* x, const T* LU, const T* b, int n, int st, int en) {
+    std::vector<T> z(n, T(0.0));
     auto lu = [&](int r, int c) { return LU[r + (size_t)n * c]; };
     z[st] = b[st];
     for (int i = st + 1; i <= en; ++i) {
@@ -180,8 +182,8 @@ void lu_solve(double* x, const double* LU, const double* b, int n, int st, int e
     }
     x[en] = z[en] / lu(en, en);
     for (int i = en - 1; i >= st; --i) {
-        double f = 1.0 / lu(i, i);
-        double dp = 0;
+        T f = T(1.0) / lu(i, i);
+        T dp = T(0);
         for (int k = i + 1; k <= en; ++k) dp += lu(i, k) * x[k];
         x[i] = f * (z[i] - dp);
     }
@@ -221,6 +223,186 @@ void rotation_matrix(const double v[3], double R[3][3]) {
     R[0][0] = c + k[0] * k[0] * t;        R[0][1] = k[0] * k[1] * t - k[2] * sn; R[0][2] = k[0] * k[2] * t + k[1] * sn;
     R[1][0] = k[1] * k[0] * t + k[2] * sn; R[1][1] = c + k[1] * k[1] * t;        R[1][2] = k[1] * k[2] * t - k[0] * sn;
     R[2][0] = k[2] * k[0] * t - k[1] * sn; R[2][1] = k[2] * k[1] * t + k[0] * sn; R[2][2] = c + k[2] * k[2] * t;
+}
+
+// Bodies of orc_shell_abd and orc_shell4_stiffness (documented there),
+// generic in the scalar type so that the complex-step derivative runs the
+// same arithmetic on std::complex<double> (the reference's AStrO_c_* twins).
+template <typename T>
+void shell_abd_t(int nL, const double* layers, const T* thick, double zOff, T* ABD) {
+    for (int k = 0; k < 81; ++k) ABD[k] = T(0.0);
+    T t = T(0);
+    for (int l = 0; l < nL; ++l) t += thick[l];
+    T z = -0.5 * t * (zOff + 1.0);
+    auto A = [&](int r, int c) -> T& { return ABD[(r - 1) + 9 * (c - 1)]; };
+    for (int l = 0; l < nL; ++l) {
+        const double* L = layers + 6 * l;
+        const T zn = z + thick[l];
+        double S[9] = {1.0 / L[0], -L[2] / L[0], 0, -L[2] / L[0], 1.0 / L[1], 0, 0, 0, 1.0 / L[3]};
+        double Q[9];
+        lu_factor(S, 3, 0, 2);
+        for (int c = 0; c < 3; ++c) {
+            double f[3] = {0, 0, 0};
+            f[c] = 1.0;
+            lu_solve(Q + 3 * c, S, f, 3, 0, 2);
+        }
+        const double a11 = std::cos(L[5]), a21 = std::sin(L[5]), a12 = -a21, a22 = a11;
+        double Ts[3][3] = {{a11 * a11, a12 * a12, 2 * a11 * a12},
+                           {a21 * a21, a22 * a22, 2 * a22 * a21},
+                           {a11 * a21, a12 * a22, a11 * a22 + a12 * a21}};
+        double Te[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double v = Ts[r][c];
+                if (c == 2) v *= 0.5;
+                if (r == 2) v *= 2.0;
+                Te[r + 3 * c] = v;
+            }
+        double TeInv[9];
+        lu_factor(Te, 3, 0, 2);
+        for (int c = 0; c < 3; ++c) {
+            double f[3] = {0, 0, 0};
+            f[c] = 1.0;
+            lu_solve(TeInv + 3 * c, Te, f, 3, 0, 2);
+        }
+        double QT[3][3], Qb[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                QT[r][c] = 0;
+                for (int k = 0; k < 3; ++k) QT[r][c] += Q[r + 3 * k] * TeInv[k + 3 * c];
+            }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                Qb[r][c] = 0;
+                for (int k = 0; k < 3; ++k) Qb[r][c] += Ts[r][k] * QT[k][c];
+            }
+        for (int r = 1; r <= 3; ++r)
+            for (int c = 1; c <= 3; ++c) {
+                A(r, c) += (zn - z) * Qb[r - 1][c - 1];
+                A(r, c + 3) += 0.5 * (zn * zn - z * z) * Qb[r - 1][c - 1];
+                A(r + 3, c + 3) += (1.0 / 3.0) * (zn * zn * zn - z * z * z) * Qb[r - 1][c - 1];
+            }
+        z = zn;
+    }
+    for (int r = 2; r <= 6; ++r)
+        for (int c = 1; c < r; ++c) A(r, c) = A(c, r);
+    A(7, 7) = A(3, 3);
+    A(8, 8) = A(3, 3);
+    A(9, 9) = A(3, 3);
+}
+
+template <typename T>
+void shell4_stiffness_t(const double* xyz, const double* alpha, const T* ABD, T* K32, T* Kc) {
+    ElProfile p;
+    element_profile(41, p);
+    double al[3][3], loc[3][4];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) al[i][j] = alpha[i + 3 * j];
+    for (int i = 0; i < 3; ++i)
+        for (int a = 0; a < 4; ++a) {
+            loc[i][a] = 0;
+            for (int k = 0; k < 3; ++k) loc[i][a] += al[i][k] * xyz[k + 3 * a];
+        }
+    std::vector<T> K(33 * 33, T(0.0));  // r_condenseMat works on the 33x33 array
+    auto Kf = [&](int r, int c) -> T& { return K[r + 33 * (size_t)c]; };
+    for (int g = 0; g < 4; ++g) {
+        double N[10], Ns[10][2];
+        shell4_shape(p.intPts[g][0], p.intPts[g][1], N, Ns);
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int a = 0; a < 4; ++a) J[i][j] += loc[i][a] * Ns[a][j];
+        J[2][2] = 1.0;
+        const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
+        const double di = 1.0 / det;
+        double I[3][3];
+        I[0][0] = di * a11;
+        I[1][0] = di * a12;
+        I[2][0] = di * a13;
+        I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+        I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+        I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+        double Nx[10][2];
+        for (int n = 0; n < 10; ++n)
+            for (int j = 0; j < 2; ++j) Nx[n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j];
+        double B[9][32];
+        for (int d = 0; d < 32; ++d) {
+            const int var = p.dofTable[0][d], nd = p.dofTable[1][d] - 1;
+            double ux[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rx[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rot[3] = {0, 0, 0};
+            if (d >= 24) {  // internal DOF: local component var on bubble nd
+                for (int j = 0; j < 2; ++j) ux[var - 1][j] = Nx[nd][j];
+            } else if (var <= 3) {  // nodal translation: u_L(nd) = alpha e_var
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 2; ++j) ux[i][j] = al[i][var - 1] * Nx[nd][j];
+            } else {  // nodal rotation e_q at node nd
+                const int q = var - 4;
+                for (int a = 0; a < 4; ++a) {
+                    // d u_L(a) = -alpha (e_q x X_a) / 4
+                    const double* X = xyz + 3 * a;
+                    double ex[3] = {0, 0, 0};
+                    if (q == 0) { ex[1] = -X[2]; ex[2] = X[1]; }
+                    if (q == 1) { ex[0] = X[2]; ex[2] = -X[0]; }
+                    if (q == 2) { ex[0] = -X[1]; ex[1] = X[0]; }
+                    for (int i = 0; i < 3; ++i) {
+                        double v = 0;
+                        for (int k = 0; k < 3; ++k) v += al[i][k] * ex[k];
+                        v *= -0.25;
+                        for (int j = 0; j < 2; ++j) ux[i][j] += v * Nx[a][j];
+                    }
+                    // d r_L(a) = alpha e_q (delta_a,nd - 1/4)
+                    const double f = (a == nd ? 1.0 : 0.0) - 0.25;
+                    for (int i = 0; i < 3; ++i) {
+                        for (int j = 0; j < 2; ++j) rx[i][j] += al[i][q] * f * Nx[a][j];
+                        rot[i] += al[i][q] * f * N[a];
+                    }
+                }
+            }
+            B[0][d] = ux[0][0];
+            B[1][d] = ux[1][1];
+            B[2][d] = ux[0][1] + ux[1][0];
+            B[3][d] = rx[1][0];
+            B[4][d] = -rx[0][1];
+            B[5][d] = rx[1][1] - rx[0][0];
+            B[6][d] = ux[2][0] + rot[1];
+            B[7][d] = ux[2][1] - rot[0];
+            B[8][d] = 2.0 * rot[2] - ux[1][0] + ux[0][1];
+        }
+        const double wd = det * p.ipWt[g];
+        T DB[9][32];
+        for (int r = 0; r < 9; ++r)
+            for (int d = 0; d < 32; ++d) {
+                T acc = T(0);
+                for (int k = 0; k < 9; ++k) acc += ABD[r + 9 * k] * B[k][d];
+                DB[r][d] = wd * acc;
+            }
+        for (int j = 0; j < 32; ++j)
+            for (int i = 0; i < 32; ++i) {
+                T acc = T(0);
+                for (int k = 0; k < 9; ++k) acc += B[k][i] * DB[k][j];
+                Kf(i, j) += acc;
+            }
+    }
+    if (K32)
+        for (int j = 0; j < 32; ++j)
+            for (int i = 0; i < 32; ++i) K32[i + 32 * j] = Kf(i, j);
+    // condensation: LU of rows/cols 24..31, then each nodal column
+    lu_factor(K.data(), 33, 24, 31);
+    std::vector<T> b(33), x(33);
+    for (int c = 0; c < 24; ++c) {
+        for (int r = 0; r < 33; ++r) b[r] = Kf(r, c);
+        std::fill(x.begin(), x.end(), T(0.0));
+        lu_solve(x.data(), K.data(), b.data(), 33, 24, 31);
+        for (int r = 0; r < 24; ++r) {
+            T acc = T(0);
+            for (int k = 24; k < 32; ++k) acc += Kf(r, k) * x[k];
+            Kf(r, c) -= acc;
+        }
+    }
+    for (int j = 0; j < 24; ++j)
+        for (int i = 0; i < 24; ++i) Kc[i + 24 * j] = Kf(i, j);
 }
 
 }  // namespace
@@ -291,65 +473,9 @@ void orc_shell_frame(const double* or6, const double* xyz, double* alpha) {
 // uninitialised angle (D1) and carry single-precision coefficients (D2): here
 // cos/sin are exact.  ABD: 9x9 column-major.
 void orc_shell_abd(int nL, const double* layers, double zOff, double* ABD) {
-    for (int k = 0; k < 81; ++k) ABD[k] = 0.0;
-    double t = 0;
-    for (int l = 0; l < nL; ++l) t += layers[6 * l + 4];
-    double z = -0.5 * t * (zOff + 1.0);
-    auto A = [&](int r, int c) -> double& { return ABD[(r - 1) + 9 * (c - 1)]; };
-    for (int l = 0; l < nL; ++l) {
-        const double* L = layers + 6 * l;
-        const double zn = z + L[4];
-        double S[9] = {1.0 / L[0], -L[2] / L[0], 0, -L[2] / L[0], 1.0 / L[1], 0, 0, 0, 1.0 / L[3]};
-        double Q[9];
-        lu_factor(S, 3, 0, 2);
-        for (int c = 0; c < 3; ++c) {
-            double f[3] = {0, 0, 0};
-            f[c] = 1.0;
-            lu_solve(Q + 3 * c, S, f, 3, 0, 2);
-        }
-        const double a11 = std::cos(L[5]), a21 = std::sin(L[5]), a12 = -a21, a22 = a11;
-        double Ts[3][3] = {{a11 * a11, a12 * a12, 2 * a11 * a12},
-                           {a21 * a21, a22 * a22, 2 * a22 * a21},
-                           {a11 * a21, a12 * a22, a11 * a22 + a12 * a21}};
-        double Te[9];
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) {
-                double v = Ts[r][c];
-                if (c == 2) v *= 0.5;
-                if (r == 2) v *= 2.0;
-                Te[r + 3 * c] = v;
-            }
-        double TeInv[9];
-        lu_factor(Te, 3, 0, 2);
-        for (int c = 0; c < 3; ++c) {
-            double f[3] = {0, 0, 0};
-            f[c] = 1.0;
-            lu_solve(TeInv + 3 * c, Te, f, 3, 0, 2);
-        }
-        double QT[3][3], Qb[3][3];
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) {
-                QT[r][c] = 0;
-                for (int k = 0; k < 3; ++k) QT[r][c] += Q[r + 3 * k] * TeInv[k + 3 * c];
-            }
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) {
-                Qb[r][c] = 0;
-                for (int k = 0; k < 3; ++k) Qb[r][c] += Ts[r][k] * QT[k][c];
-            }
-        for (int r = 1; r <= 3; ++r)
-            for (int c = 1; c <= 3; ++c) {
-                A(r, c) += (zn - z) * Qb[r - 1][c - 1];
-                A(r, c + 3) += 0.5 * (zn * zn - z * z) * Qb[r - 1][c - 1];
-                A(r + 3, c + 3) += (1.0 / 3.0) * (zn * zn * zn - z * z * z) * Qb[r - 1][c - 1];
-            }
-        z = zn;
-    }
-    for (int r = 2; r <= 6; ++r)
-        for (int c = 1; c < r; ++c) A(r, c) = A(c, r);
-    A(7, 7) = A(3, 3);
-    A(8, 8) = A(3, 3);
-    A(9, 9) = A(3, 3);
+    std::vector<double> th(nL);
+    for (int l = 0; l < nL; ++l) th[l] = layers[6 * l + 4];
+    shell_abd_t<double>(nL, layers, th.data(), zOff, ABD);
 }
 
 // a10 (shell branch at u = 0) + a9 + a14: shell4 tangent
@@ -366,116 +492,7 @@ void orc_shell_abd(int nL, const double* layers, double zOff, double* ABD) {
 // ABD 9x9.  K32 (optional): 32x32 before condensation; Kc: 24x24 condensed,
 // both column-major in the element DOF order of dofTable.
 void orc_shell4_stiffness(const double* xyz, const double* alpha, const double* ABD, double* K32, double* Kc) {
-    ElProfile p;
-    element_profile(41, p);
-    double al[3][3], loc[3][4];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) al[i][j] = alpha[i + 3 * j];
-    for (int i = 0; i < 3; ++i)
-        for (int a = 0; a < 4; ++a) {
-            loc[i][a] = 0;
-            for (int k = 0; k < 3; ++k) loc[i][a] += al[i][k] * xyz[k + 3 * a];
-        }
-    std::vector<double> K(33 * 33, 0.0);  // r_condenseMat works on the 33x33 array
-    auto Kf = [&](int r, int c) -> double& { return K[r + 33 * (size_t)c]; };
-    for (int g = 0; g < 4; ++g) {
-        double N[10], Ns[10][2];
-        shell4_shape(p.intPts[g][0], p.intPts[g][1], N, Ns);
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 2; ++j)
-                for (int a = 0; a < 4; ++a) J[i][j] += loc[i][a] * Ns[a][j];
-        J[2][2] = 1.0;
-        const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
-        const double di = 1.0 / det;
-        double I[3][3];
-        I[0][0] = di * a11;
-        I[1][0] = di * a12;
-        I[2][0] = di * a13;
-        I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
-        I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
-        I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
-        double Nx[10][2];
-        for (int n = 0; n < 10; ++n)
-            for (int j = 0; j < 2; ++j) Nx[n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j];
-        double B[9][32];
-        for (int d = 0; d < 32; ++d) {
-            const int var = p.dofTable[0][d], nd = p.dofTable[1][d] - 1;
-            double ux[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rx[3][2] = {{0, 0}, {0, 0}, {0, 0}}, rot[3] = {0, 0, 0};
-            if (d >= 24) {  // internal DOF: local component var on bubble nd
-                for (int j = 0; j < 2; ++j) ux[var - 1][j] = Nx[nd][j];
-            } else if (var <= 3) {  // nodal translation: u_L(nd) = alpha e_var
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 2; ++j) ux[i][j] = al[i][var - 1] * Nx[nd][j];
-            } else {  // nodal rotation e_q at node nd
-                const int q = var - 4;
-                for (int a = 0; a < 4; ++a) {
-                    // d u_L(a) = -alpha (e_q x X_a) / 4
-                    const double* X = xyz + 3 * a;
-                    double ex[3] = {0, 0, 0};
-                    if (q == 0) { ex[1] = -X[2]; ex[2] = X[1]; }
-                    if (q == 1) { ex[0] = X[2]; ex[2] = -X[0]; }
-                    if (q == 2) { ex[0] = -X[1]; ex[1] = X[0]; }
-                    for (int i = 0; i < 3; ++i) {
-                        double v = 0;
-                        for (int k = 0; k < 3; ++k) v += al[i][k] * ex[k];
-                        v *= -0.25;
-                        for (int j = 0; j < 2; ++j) ux[i][j] += v * Nx[a][j];
-                    }
-                    // d r_L(a) = alpha e_q (delta_a,nd - 1/4)
-                    const double f = (a == nd ? 1.0 : 0.0) - 0.25;
-                    for (int i = 0; i < 3; ++i) {
-                        for (int j = 0; j < 2; ++j) rx[i][j] += al[i][q] * f * Nx[a][j];
-                        rot[i] += al[i][q] * f * N[a];
-                    }
-                }
-            }
-            B[0][d] = ux[0][0];
-            B[1][d] = ux[1][1];
-            B[2][d] = ux[0][1] + ux[1][0];
-            B[3][d] = rx[1][0];
-            B[4][d] = -rx[0][1];
-            B[5][d] = rx[1][1] - rx[0][0];
-            B[6][d] = ux[2][0] + rot[1];
-            B[7][d] = ux[2][1] - rot[0];
-            B[8][d] = 2.0 * rot[2] - ux[1][0] + ux[0][1];
-        }
-        const double wd = det * p.ipWt[g];
-        double DB[9][32];
-        for (int r = 0; r < 9; ++r)
-            for (int d = 0; d < 32; ++d) {
-                double acc = 0;
-                for (int k = 0; k < 9; ++k) acc += ABD[r + 9 * k] * B[k][d];
-                DB[r][d] = wd * acc;
-            }
-        for (int j = 0; j < 32; ++j)
-            for (int i = 0; i < 32; ++i) {
-                double acc = 0;
-                for (int k = 0; k < 9; ++k) acc += B[k][i] * DB[k][j];
-                Kf(i, j) += acc;
-            }
-    }
-    if (K32)
-        for (int j = 0; j < 32; ++j)
-            for (int i = 0; i < 32; ++i) K32[i + 32 * j] = Kf(i, j);
-    // condensation: LU of rows/cols 24..31, then each nodal column
-    lu_factor(K.data(), 33, 24, 31);
-    std::vector<double> b(33), x(33);
-    for (int c = 0; c < 24; ++c) {
-        for (int r = 0; r < 33; ++r) b[r] = Kf(r, c);
-        std::fill(x.begin(), x.end(), 0.0);
-        lu_solve(x.data(), K.data(), b.data(), 33, 24, 31);
-        for (int r = 0; r < 24; ++r) {
-            double acc = 0;
-            for (int k = 24; k < 32; ++k) acc += Kf(r, k) * x[k];
-            Kf(r, c) -= acc;
-        }
-    }
-    for (int j = 0; j < 24; ++j)
-        for (int i = 0; i < 24; ++i) Kc[i + 24 * j] = Kf(i, j);
+    shell4_stiffness_t<double>(xyz, alpha, ABD, K32, Kc);
 }
 
 // ---------------------------------------------------------------------------
@@ -994,6 +1011,13 @@ struct OrcModel {
     std::vector<double> elCond;     // 9*E
     std::vector<double> elABD;      // 81*E (shell elements)
     std::vector<double> elOrient;   // 6*E section orientation vectors (shell elements)
+    std::vector<double> elLayers;   // nLayers*6*E (shell elements)
+    std::vector<double> elZOff;     // E
+    int nLayers = 0;
+    // design variables: layer dvLayer's thickness of element e += elCoef[e] * d[elDV[e]]
+    int nDV = 0, dvLayer = 0;
+    std::vector<int> elDV;
+    std::vector<double> elCoef;
     // constraints (Working/AStrO_input.f90 constraints block, after set expansion
     // of the node names): term t has equation mpcEqn, dof, coef and either a
     // plain node (isSet = 0, one node) or a node set (isSet = 1, node list)
@@ -1005,6 +1029,7 @@ struct OrcModel {
     std::vector<double> loadVal;
     // bookkeeping
     int solverMaxBW = 0, numSolidNds = 0, numSectionNds = 0;
+    int coupledProfile = 0;  // 1: profile rows reach every component of the neighbours
     std::vector<int> nCRange, nodalConn, currentRank, originalRank, nDofIndex;
     int elMatDim = 0;
     std::vector<int> elMatRange, elMatCols, elMatLTRange;
@@ -1200,11 +1225,21 @@ OrcModel* orc_create(int numNodes, const double* nodeList, int numEls, const int
 
 void orc_free(OrcModel* m) { delete m; }
 
+// Profile option set before orc_analysis_prep: 0 = the reference's rows (same
+// component only, Working/AStrO_bookKeeping.f90:380-402), 1 = rows reaching
+// every component of the node and its neighbours (the factor is then a direct
+// solve instead of a component-block preconditioner).
+void orc_set_coupled_profile(OrcModel* m, int on) { m->coupledProfile = on; }
+
 // Shell sections per element: nL layers of (E1, E2, nu12, G12, thickness,
 // angle) each, the z offset and the two orientation vectors of the section.
 void orc_set_shell_sections(OrcModel* m, int nL, const double* layers, const double* zOff, const double* orient6) {
     m->elABD.assign(81 * (size_t)m->numEls, 0.0);
     m->elOrient.assign(6 * (size_t)m->numEls, 0.0);
+    m->nLayers = nL;
+    m->elLayers.assign(layers, layers + 6 * (size_t)nL * m->numEls);
+    m->elZOff.assign(m->numEls, 0.0);
+    if (zOff) m->elZOff.assign(zOff, zOff + m->numEls);
     for (int e = 0; e < m->numEls; ++e) {
         orc_shell_abd(nL, layers + 6 * (size_t)nL * e, zOff ? zOff[e] : 0.0, &m->elABD[81 * (size_t)e]);
         for (int k = 0; k < 6; ++k) m->elOrient[6 * (size_t)e + k] = orient6[6 * (size_t)e + k];
@@ -1351,15 +1386,23 @@ void orc_analysis_prep(OrcModel* mp, int maxBW) {
             int i3 = ndi(i2, i1);
             if (i3 == 0) continue;
             m.elMatLTRange[i3] = 1;
+            if (m.coupledProfile)  // the node's own lower components
+                for (int c = 1; c <= 6; ++c) {
+                    int i7 = ndi(c, i1);
+                    if (i7 != 0 && i7 <= i3) m.elMatLTRange[i3] = std::max(m.elMatLTRange[i3], i3 - i7 + 1);
+                }
             for (int s = m.nCRange[i1 - 1] + 1; s <= m.nCRange[i1]; ++s) {
                 int i5 = m.nodalConn[s - 1];
                 if (i5 == 0) continue;
                 int i6 = m.currentRank[i1 - 1] - m.currentRank[i5 - 1];
                 if (i6 >= 0 && i6 <= m.solverMaxBW) {
-                    int i7 = ndi(i2, i5);
-                    if (i7 != 0 && i7 <= i3) {
-                        int i8 = i3 - i7 + 1;
-                        if (i8 > m.elMatLTRange[i3]) m.elMatLTRange[i3] = i8;
+                    for (int c = 1; c <= 6; ++c) {
+                        if (!m.coupledProfile && c != i2) continue;
+                        int i7 = ndi(c, i5);
+                        if (i7 != 0 && i7 <= i3) {
+                            int i8 = i3 - i7 + 1;
+                            if (i8 > m.elMatLTRange[i3]) m.elMatLTRange[i3] = i8;
+                        }
                     }
                 }
             }
@@ -1715,6 +1758,87 @@ void orc_dyn_step(OrcModel* mp, double loadScale, int numVecs, int mode, double 
         m.prevDisp[i] = m.nodeDisp[i];
         m.prevVel[i] = m.nodeVel[i];
         m.prevAcc[i] = m.nodeAcc[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// a15 + a18: design sensitivities of shell4 layer thickness by complex step.
+// Design variable k adds coef_e * d_k to the thickness of layer dvLayer
+// (0-based) of every element e with elDV[e] = k (the reference's 'layup'
+// 'thickness' category, Working/AStrO_r_designPropertyFunctions.f90:283-302).
+void orc_set_design(OrcModel* m, int nDV, const int* elDV, const double* elCoef, int dvLayer) {
+    m->nDV = nDV;
+    m->dvLayer = dvLayer;
+    m->elDV.assign(elDV, elDV + m->numEls);
+    m->elCoef.assign(elCoef, elCoef + m->numEls);
+}
+
+// Complex-step derivative of one shell4's condensed tangent with respect to
+// the thickness of layer `layer`: Im(Kc(t + i h)) / h, the whole element
+// evaluation (ABD, tangent, condensation) run in complex arithmetic as the
+// reference's AStrO_c_* twins do.  K32 (optional) gets the derivative before
+// condensation.  dKc: 24x24 column-major.
+void orc_shell4_dstiffness_cs(const double* xyz, const double* alpha, int nL, const double* layers, double zOff,
+                              int layer, double h, double* K32, double* dKc) {
+    using cd = std::complex<double>;
+    std::vector<cd> th(nL), abd(81), k32(32 * 32), kc(24 * 24);
+    for (int l = 0; l < nL; ++l) th[l] = cd(layers[6 * l + 4], l == layer ? h : 0.0);
+    shell_abd_t<cd>(nL, layers, th.data(), zOff, abd.data());
+    shell4_stiffness_t<cd>(xyz, alpha, abd.data(), k32.data(), kc.data());
+    if (K32)
+        for (int k = 0; k < 32 * 32; ++k) K32[k] = k32[k].imag() / h;
+    for (int k = 0; k < 24 * 24; ++k) dKc[k] = kc[k].imag() / h;
+}
+
+// dR/dd at the current nodeDisp, the per-variable loop of getdRelasticdD
+// (Working/AStrO_commandFunctions.f90:1172-1375, with its element-id and DOF
+// indexing corrected, D9): for each design variable k, the elements on k
+// evaluate R_e = Kc_e(t + i h coef_e) u_e in complex arithmetic and column k
+// of dRdD takes Im(R_e) / h at the element's DOFs.  The analysis is linear
+// (R = K(d) u - F with F independent of d), so dR/dd_k = (dK/dd_k) u.
+// dRdD: elMatDim x nDV column-major.
+void orc_dRdD(OrcModel* mp, double h, double* dRdD) {
+    OrcModel& m = *mp;
+    const size_t n = m.elMatDim;
+    std::fill(dRdD, dRdD + n * m.nDV, 0.0);
+    ElProfile p;
+    element_profile(41, p);
+    double xyz[24], alpha[9], dK[576];
+    for (int k = 0; k < m.nDV; ++k)
+        for (int e = 0; e < m.numEls; ++e) {
+            if (m.elDV[e] != k || m.elementType[e] != 41) continue;
+            orc_element_coords(&m, e, xyz);
+            orc_shell_frame(&m.elOrient[6 * (size_t)e], xyz, alpha);
+            orc_shell4_dstiffness_cs(xyz, alpha, m.nLayers, &m.elLayers[6 * (size_t)m.nLayers * e], m.elZOff[e],
+                                     m.dvLayer, h * m.elCoef[e], nullptr, dK);
+            int dof[24];
+            double ue[24];
+            for (int d = 0; d < 24; ++d) {
+                int node = m.elementList[8 * (size_t)e + p.dofTable[1][d] - 1];
+                dof[d] = m.nDofIndex[6 * (size_t)(node - 1) + p.dofTable[0][d] - 1];
+                ue[d] = m.nodeDisp[dof[d] - 1];
+            }
+            for (int i = 0; i < 24; ++i) {
+                double acc = 0;
+                for (int j = 0; j < 24; ++j) acc += dK[i + 24 * j] * ue[j];
+                dRdD[(size_t)k * n + dof[i] - 1] += m.elCoef[e] * acc;
+            }
+        }
+}
+
+// Tip-compliance (or any work-conjugate compliance C = F . u) gradient by the
+// adjoint: K symmetric and F independent of d give lambda = u, so
+// dC/dd_k = -u . dR/dd_k (getTotaldLdD, Working/AStrO_commandFunctions.f90:
+// 1377-1466, for this objective).
+void orc_compliance_gradient(OrcModel* mp, double h, double* grad) {
+    OrcModel& m = *mp;
+    const size_t n = m.elMatDim;
+    std::vector<double> R(n * m.nDV);
+    orc_dRdD(mp, h, R.data());
+    for (int k = 0; k < m.nDV; ++k) {
+        double acc = 0;
+        for (size_t i = 0; i < n; ++i) acc += m.nodeDisp[i] * R[(size_t)k * n + i];
+        grad[k] = -acc;
     }
 }
 

@@ -74,6 +74,11 @@ def lib():
         L.orc_shell_abd.argtypes = [C.c_int, _dp, C.c_double, _dp]
         L.orc_shell4_stiffness.argtypes = [_dp, _dp, _dp, C.c_void_p, _dp]
         L.orc_set_shell_sections.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.orc_set_coupled_profile.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_design.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int]
+        L.orc_shell4_dstiffness_cs.argtypes = [_dp, _dp, C.c_int, _dp, C.c_double, C.c_int, C.c_double, C.c_void_p, _dp]
+        L.orc_dRdD.argtypes = [C.c_void_p, C.c_double, _dp]
+        L.orc_compliance_gradient.argtypes = [C.c_void_p, C.c_double, _dp]
         L.orc_dyn_init.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.orc_dyn_residual_load.argtypes = [C.c_void_p, C.c_double]
         L.orc_dyn_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double]
@@ -153,6 +158,17 @@ def shell4_stiffness(xyz34, alpha33, abd99):
     lib().orc_shell4_stiffness(_f64(np.asarray(xyz34).reshape(3, 4).ravel(order="F")),
                                _f64(np.asarray(alpha33).ravel(order="F")), _f64(np.asarray(abd99).ravel(order="F")),
                                K32.ctypes.data, Kc)
+    return K32.reshape(32, 32, order="F"), Kc.reshape(24, 24, order="F")
+
+
+def shell4_dstiffness_cs(xyz34, alpha33, layers, layer, h=1e-30, z_off=0.0):
+    """Complex-step d/dt (thickness of `layer`) of a shell4: (dK32, dKc)."""
+    L = np.asarray(layers, np.float64).reshape(-1, 6)
+    K32 = np.zeros(32 * 32)
+    Kc = np.zeros(24 * 24)
+    lib().orc_shell4_dstiffness_cs(_f64(np.asarray(xyz34).reshape(3, 4).ravel(order="F")),
+                                   _f64(np.asarray(alpha33).ravel(order="F")), L.shape[0], _f64(L.ravel()),
+                                   float(z_off), int(layer), float(h), K32.ctypes.data, Kc)
     return K32.reshape(32, 32, order="F"), Kc.reshape(24, 24, order="F")
 
 
@@ -250,7 +266,8 @@ class OracleModel:
         except Exception:
             pass
 
-    def prep(self, max_bw: int = 0):
+    def prep(self, max_bw: int = 0, coupled_profile: bool = False):
+        lib().orc_set_coupled_profile(self.h, 1 if coupled_profile else 0)
         lib().orc_analysis_prep(self.h, int(max_bw))
         return self
 
@@ -267,6 +284,16 @@ class OracleModel:
         if n.value < 0:
             raise KeyError(name)
         return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0)
+
+    def setd(self, name, values):
+        """Overwrite one of the model's double arrays (same length)."""
+        n = C.c_longlong(0)
+        p = lib().orc_getd(self.h, name.encode(), C.byref(n))
+        v = np.asarray(values, np.float64)
+        if n.value != v.size:
+            raise ValueError(f"{name}: length {n.value}, got {v.size}")
+        np.ctypeslib.as_array(p, shape=(n.value,))[:] = v
+        return self
 
     def scalar(self, name):
         if name == "gmresRes":
@@ -290,6 +317,24 @@ class OracleModel:
             return self
         L.orc_solve_elastic(self.h, int(num_vecs), int(mode), float(rtol))
         return self
+
+    def set_design(self, el_dv, el_coef=None, dv_layer=0):
+        el_dv = _i32(el_dv)
+        coef = _f64(np.ones(len(el_dv)) if el_coef is None else el_coef)
+        self.n_dv = int(el_dv.max()) + 1
+        lib().orc_set_design(self.h, self.n_dv, el_dv, coef, int(dv_layer))
+        return self
+
+    def dRdD(self, h=1e-30):
+        n = self.scalar("elMatDim")
+        out = np.zeros(n * self.n_dv)
+        lib().orc_dRdD(self.h, float(h), out)
+        return out.reshape(self.n_dv, n).T
+
+    def compliance_gradient(self, h=1e-30):
+        g = np.zeros(self.n_dv)
+        lib().orc_compliance_gradient(self.h, float(h), g)
+        return g
 
     def dynamic_init(self, beta: float = 0.25, gamma: float = 0.5, dt: float = 1e-5):
         """solve()'s dynamic set-up: tangent K + M/(beta dt^2), penalty scaling,
