@@ -192,7 +192,8 @@ struct DeviceProfile {
         maxTiles = scratchRows / 64;
         splitK = std::max(1, std::min(16, (1536 + nBlocks * maxTiles - 1) / std::max(1, nBlocks * maxTiles)));
         ASTRO_TRY(panels.upload(hp, s));
-        ASTRO_TRY(scratch.alloc((size_t)splitK * nBlocks * scratchRows * 64));
+        // two buffers (panel parity), each with splitK "old" partials and one "new"
+        ASTRO_TRY(scratch.alloc(2 * (size_t)(splitK + 1) * nBlocks * scratchRows * 64));
         ASTRO_TRY(zeroPivot.alloc(1));
         ASTRO_TRY(diag.alloc((size_t)std::max(1, n)));
         // sweep tiles, chains interleaved in ticket order
@@ -270,18 +271,23 @@ struct DeviceProfile {
     // Left-looking blocked factor, two launches per panel.  Synchronous.
     int factor(hipStream_t s) {
         HIP_TRY(hipMemsetAsync(zeroPivot.p, 0x7f, sizeof(int), s));
-        dim3 gu(maxTiles * splitK, nBlocks), gf(maxTiles, nBlocks);
-        const char* ev = std::getenv("ASTRO_LDL_UPDATE");
-        const bool useFma = ev && std::strcmp(ev, "fma") == 0;
+        // Panel q needs W = sum_{k < p} L(i,k) D_k L(P,k): the "old" part
+        // (k before panel q-1) runs in the same launch as FT of panel q-1, the
+        // "new" part (panel q-1's columns) right after it.  Partial sums live
+        // in scratch buffer q & 1: slots 0..splitK-1 old, slot splitK new.
+        const size_t bufSize = (size_t)(splitK + 1) * nBlocks * scratchRows * 64;
+        auto buf = [&](int q) { return scratch.p + (size_t)(q & 1) * bufSize; };
+        const dim3 gOld(maxTiles * splitK, nBlocks), gNew(maxTiles, nBlocks), gStep(maxTiles * (splitK + 1) * nBlocks);
+        hipLaunchKernelGGL(k_ldl_update_mfma, gOld, dim3(256), 0, s, panels.p, 0, panelsPerBlock, splitK, LDL_K_OLD, 0,
+                           lt.p, rowoff.p, sfirst.p, diag.p, buf(0), scratchRows);
+        hipLaunchKernelGGL(k_ldl_update_mfma, gNew, dim3(256), 0, s, panels.p, 0, panelsPerBlock, 1, LDL_K_NEW,
+                           splitK, lt.p, rowoff.p, sfirst.p, diag.p, buf(0), scratchRows);
         for (int q = 0; q < panelsPerBlock; ++q) {
-            if (useFma)
-                hipLaunchKernelGGL(k_ldl_update_fma, gu, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK, lt.p,
-                                   rowoff.p, sfirst.p, scratch.p, scratchRows);
-            else
-                hipLaunchKernelGGL(k_ldl_update_mfma, gu, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK,
-                                   lt.p, rowoff.p, sfirst.p, diag.p, scratch.p, scratchRows);
-            hipLaunchKernelGGL(k_ldl_factor_panel, gf, dim3(256), 0, s, panels.p, q, panelsPerBlock, splitK,
-                               lt.p, rowoff.p, sfirst.p, scratch.p, scratchRows, diag.p, zeroPivot.p);
+            hipLaunchKernelGGL(k_ldl_step, gStep, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles, splitK, lt.p,
+                               rowoff.p, sfirst.p, diag.p, buf(q), buf(q + 1), scratchRows, zeroPivot.p);
+            if (q + 1 < panelsPerBlock)
+                hipLaunchKernelGGL(k_ldl_update_mfma, gNew, dim3(256), 0, s, panels.p, q + 1, panelsPerBlock, 1,
+                                   LDL_K_NEW, splitK, lt.p, rowoff.p, sfirst.p, diag.p, buf(q + 1), scratchRows);
         }
         HIP_TRY(hipGetLastError());
         int zp = 0;

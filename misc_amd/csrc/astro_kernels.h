@@ -42,6 +42,20 @@ __global__ void k_newmark_update(int n, double* u, const double* du, double* pu,
                                  double* a, double c1, double c2, double dt, double beta, double gamma);
 __global__ void k_brick8_conduct(const int* perm, int count, const int* conn, const double* X, const double* elCond,
                                  const int* rank, const int* csrRange, const uint8_t* thPos, double* val);
+// k ranges of step U: all columns before the panel, only those before the
+// previous panel ("old", which does not wait for the previous panel's
+// factor), or only the previous panel's 64 ("new").
+enum { LDL_K_ALL = 0, LDL_K_OLD = 1, LDL_K_NEW = 2 };
+
+// Blocked left-looking LDL^T: step U alone (kMode 0 all columns before the
+// panel, 1 all but the previous panel, 2 the previous panel only) and the fused
+// step (FT of panelIdx, "old" U of panelIdx + 1).
+__global__ void k_ldl_update_mfma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int nSplit, int kMode,
+                                  int slot0, const double* lt, const int64_t* rowoff, const int* sfirst,
+                                  const double* diag, double* scratch, int scratchRows);
+__global__ void k_ldl_step(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
+                           double* lt, const int64_t* rowoff, const int* sfirst, double* diag, const double* scratchFT,
+                           double* scratchU, int scratchRows, int* zeroPivot);
 __global__ void k_max_abs(const double* v, int64_t n, unsigned long long* out);
 __global__ void k_scale_mpc(double* cmat, double* rhs, const int* crange, int cdim, const unsigned long long* maxBits);
 __global__ void k_fill(double* p, int64_t n, double v);
@@ -51,15 +65,6 @@ __global__ void k_csr_to_profile(const double* val, const int* cols, const int* 
                                  const int64_t* rowoff, double* lt);
 __global__ void k_penalty_to_profile(const double* cmat, const int* ccols, const int* crange, int cdim,
                                      const int* sfirst, const int64_t* rowoff, double* lt);
-__global__ void k_ldl_update_mfma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
-                                  double* lt, const int64_t* rowoff, const int* sfirst, const double* diag,
-                                  double* scratch, int scratchRows);
-__global__ void k_ldl_update_fma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
-                                 double* lt, const int64_t* rowoff, const int* sfirst, double* scratch,
-                                 int scratchRows);
-__global__ void k_ldl_factor_panel(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int splitK,
-                                   double* lt, const int64_t* rowoff, const int* sfirst, const double* scratch,
-                                   int scratchRows, double* diag, int* zeroPivot);
 __global__ void k_sweep_forward(const SweepTile* tiles, int numTiles, const double* lt, const int64_t* rowoff,
                                 const int* sfirst, const double* b, double* z, int* flags, int* ticket, int epoch,
                                 long spinLimit, int* timeout);

@@ -778,24 +778,22 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // k is staged through LDS in chunks of 32.  Operand lane layouts (measured on
 // gfx950): A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k, D(i, j) in
 // lane j + 16 (i % 4), register i / 4.
-__global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __restrict__ panels, int panelIdx, int numPanelsPerBlock,
-                                                    int splitK, double* __restrict__ lt,
-                                                    const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
-                                                    const double* __restrict__ diag,
-                                                    double* __restrict__ scratch, int scratchRows) {
-    const int blk = blockIdx.y;
-    const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
-    if (P.p < 0) return;
-    const int tile = blockIdx.x / splitK, sp = blockIdx.x % splitK;
+constexpr int LDL_SMEM_BYTES = 64 * 65 * 8 + 2 * 64 * 8;  // the larger of the two step bodies
+
+__device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
+                                                int kMode, int slot, const double* __restrict__ lt,
+                                                const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
+                                                const double* __restrict__ diag, double* __restrict__ scratch,
+                                                int scratchRows, char* smem) {
     const int r0 = P.p + 64 * tile;
     if (r0 >= P.iend) return;
     const int nr = min(64, P.iend - r0);
     const int np = P.pe - P.p;
     constexpr int KC = 32;
-    __shared__ double As[KC][64 + 1];
-    __shared__ double Bs[KC][64 + 1];
-    __shared__ double Dk[KC];
-    __shared__ int sMin;
+    double(*As)[65] = reinterpret_cast<double(*)[65]>(smem);
+    double(*Bs)[65] = reinterpret_cast<double(*)[65]>(smem + KC * 65 * 8);
+    double* Dk = reinterpret_cast<double*>(smem + 2 * KC * 65 * 8);
+    int& sMin = *reinterpret_cast<int*>(smem + 2 * KC * 65 * 8 + KC * 8);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid < 64) {
         int v = tid < nr ? sfirst[r0 + tid] : 0x7fffffff;
@@ -803,9 +801,11 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
         if (tid == 0) sMin = v;
     }
     __syncthreads();
-    const int kLo = max(sMin, P.smin), kHi = P.p;
+    int kLo = max(sMin, P.smin), kHi = P.p;
+    if (kMode == LDL_K_OLD) kHi = P.p - 64;
+    if (kMode == LDL_K_NEW) kLo = max(kLo, P.p - 64);
     const int total = max(0, kHi - kLo);
-    int chunk = (total + splitK - 1) / splitK;
+    int chunk = (total + nSplit - 1) / nSplit;
     chunk = (chunk + KC - 1) / KC * KC;
     const int kb = kLo + sp * chunk, ke = min(kHi, kb + chunk);
     // per-thread staging rows (fixed across chunks)
@@ -867,7 +867,7 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
             }
         }
     }
-    double* out = scratch + (((size_t)sp * gridDim.y + blk) * scratchRows + (r0 - P.p)) * 64;
+    double* out = scratch + (((size_t)slot * nb + blk) * scratchRows + (r0 - P.p)) * 64;
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
@@ -878,92 +878,23 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
         }
 }
 
-// Step U with plain FP64 FMAs (4x4 register tiles per thread); kept beside
-// the matrix-core form as its cross-check.
-__global__ __launch_bounds__(256) void k_ldl_update_fma(const PanelInfo* __restrict__ panels, int panelIdx, int numPanelsPerBlock,
-                                                    int splitK, double* __restrict__ lt,
-                                                    const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
-                                                    double* __restrict__ scratch, int scratchRows) {
+// Step U alone (the first panel's zero partials, and the "new" part): grid
+// (tiles x nSplit, blocks); partial sp of the k range goes to slot slot0 + sp.
+__global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __restrict__ panels, int panelIdx,
+                                                         int numPanelsPerBlock, int nSplit, int kMode, int slot0,
+                                                         const double* __restrict__ lt,
+                                                         const int64_t* __restrict__ rowoff,
+                                                         const int* __restrict__ sfirst,
+                                                         const double* __restrict__ diag,
+                                                         double* __restrict__ scratch, int scratchRows) {
+    __shared__ __align__(16) char smem[LDL_SMEM_BYTES];
     const int blk = blockIdx.y;
     const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
     if (P.p < 0) return;
-    const int tile = blockIdx.x / splitK, sp = blockIdx.x % splitK;
-    const int r0 = P.p + 64 * tile;
-    if (r0 >= P.iend) return;
-    const int nr = min(64, P.iend - r0);
-    const int np = P.pe - P.p;
-    __shared__ double As[16][64 + 1];
-    __shared__ double Bs[16][64 + 1];
-    __shared__ double Dk[16];
-    __shared__ int sMin[2];
-    const int tid = threadIdx.x;
-    // k range: the tile's and the panel's first columns bound it from below
-    if (tid < 64) {
-        int v = tid < nr ? sfirst[r0 + tid] : 0x7fffffff;
-        for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-        if (tid == 0) sMin[0] = v;
-    }
-    __syncthreads();
-    const int kLo = max(sMin[0], P.smin), kHi = P.p;
-    const int total = max(0, kHi - kLo);
-    int chunk = (total + splitK - 1) / splitK;
-    chunk = (chunk + 15) & ~15;
-    const int kb = kLo + sp * chunk, ke = min(kHi, kb + chunk);
-    const int tr = tid >> 4, tc = tid & 15;
-    double acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        __syncthreads();
-        if (tid < 16) {
-            const int k = k0 + tid;
-            Dk[tid] = k < ke ? lt[rowoff[k] + k] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int id = q * 256 + tid;
-            const int rr = id >> 4, kk = id & 15;
-            const int k = k0 + kk;
-            double va = 0.0, vb = 0.0;
-            if (k < ke) {
-                if (rr < nr) {
-                    const int i = r0 + rr;
-                    if (k >= sfirst[i]) va = lt[rowoff[i] + k];
-                }
-                if (rr < np) {
-                    const int j = P.p + rr;
-                    if (k >= sfirst[j]) vb = lt[rowoff[j] + k];
-                }
-            }
-            As[kk][rr] = va;
-            Bs[kk][rr] = vb;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 16; ++kk) {
-            const double d = Dk[kk];
-            double av[4], bv[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[kk][tr * 4 + i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = d * Bs[kk][tc * 4 + j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
-        }
-    }
-    double* out = scratch + (((size_t)sp * gridDim.y + blk) * scratchRows + (r0 - P.p)) * 64;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int rr = tr * 4 + i;
-        if (rr < nr) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) out[(size_t)rr * 64 + tc * 4 + j] = acc[i][j];
-        }
-    }
+    const int tile = blockIdx.x / nSplit, sp = blockIdx.x % nSplit;
+    if (P.p + 64 * tile >= P.iend) return;
+    ldl_update_body(P, blk, gridDim.y, tile, sp, nSplit, kMode, slot0 + sp, lt, rowoff, sfirst, diag, scratch,
+                    scratchRows, smem);
 }
 
 // Broadcast lane S of each group of four lanes (DPP quad permute, no LDS).
@@ -997,24 +928,19 @@ __device__ __forceinline__ double quad_bcast_rt(double v, int s) {
 //     quad broadcast; L(i, j) = y_j / D_j.
 // Both loops are fully unrolled, so register indices and the lane that owns
 // column j are compile-time constants.
-__global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __restrict__ panels, int panelIdx,
-                                                          int numPanelsPerBlock, int splitK, double* __restrict__ lt,
-                                                          const int64_t* __restrict__ rowoff,
-                                                          const int* __restrict__ sfirst,
-                                                          const double* __restrict__ scratch, int scratchRows,
-                                                          double* __restrict__ diag, int* __restrict__ zeroPivot) {
-    const int blk = blockIdx.y;
-    const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
-    if (P.p < 0) return;
-    const int tile = blockIdx.x;
+__device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int nb, int tile, int nSlots,
+                                               double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                               const int* __restrict__ sfirst, const double* __restrict__ scratch,
+                                               int scratchRows, double* __restrict__ diag,
+                                               int* __restrict__ zeroPivot, char* smem) {
     const int r0 = P.p + 64 * tile;
     if (r0 >= P.iend) return;
     const int nr = min(64, P.iend - r0);
     const int np = P.pe - P.p;
-    __shared__ double Lc[64][65];  // factored diagonal block: L below, D on the diagonal
-    __shared__ double colb[2][64];
+    double(*Lc)[65] = reinterpret_cast<double(*)[65]>(smem);  // factored diagonal block: L below, D on the diagonal
+    double(*colb)[64] = reinterpret_cast<double(*)[64]>(smem + 64 * 65 * 8);
     const int tid = threadIdx.x, fr = tid >> 2, fq = tid & 3;
-    const size_t qstride = (size_t)gridDim.y * scratchRows * 64;
+    const size_t qstride = (size_t)nb * scratchRows * 64;
 
     // gather row `rowInPanel` of the updated panel block; entry m is column
     // k = fq + 4 m of the panel, kept where live && k < kmax && inside the profile
@@ -1031,7 +957,7 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
             ws[m] = 0.0;
         }
         const double* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
-        for (int q = 0; q < splitK; ++q) {
+        for (int q = 0; q < nSlots; ++q) {
 #pragma unroll
             for (int m = 0; m < 16; ++m) ws[m] += sp[(size_t)q * qstride + 4 * m];
         }
@@ -1112,6 +1038,38 @@ __global__ __launch_bounds__(256) void k_ldl_factor_panel(const PanelInfo* __res
             const int k = fq + 4 * m;
             if (k < np && P.p + k >= si) lt[off + P.p + k] = y[m] / Lc[k][k];
         }
+    }
+}
+
+// One factor step: blocks x < numTiles run FT for panel panelIdx (reading
+// the partial sums in scratchFT, nSlots of them), the others run the "old" U
+// of panel panelIdx + 1 into scratchU (nSplit partials): that part reads only
+// columns finished before this launch, so the two overlap.
+__global__ __launch_bounds__(256) void k_ldl_step(const PanelInfo* __restrict__ panels, int panelIdx,
+                                                  int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
+                                                  double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                                  const int* __restrict__ sfirst, double* __restrict__ diag,
+                                                  const double* __restrict__ scratchFT, double* __restrict__ scratchU,
+                                                  int scratchRows, int* __restrict__ zeroPivot) {
+    __shared__ __align__(16) char smem[LDL_SMEM_BYTES];
+    // 1-D grid, FT workgroups of every block first: they carry the critical
+    // path and are dispatched before any U work
+    const int x = blockIdx.x, nFT = numBlocks * numTiles;
+    if (x < nFT) {
+        const int blk = x / numTiles, tile = x % numTiles;
+        const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
+        if (P.p < 0 || P.p + 64 * tile >= P.iend) return;
+        ldl_panel_body(P, blk, numBlocks, tile, nSplit + 1, lt, rowoff, sfirst, scratchFT, scratchRows, diag,
+                       zeroPivot, smem);
+    } else {
+        if (panelIdx + 1 >= numPanelsPerBlock) return;
+        const int id = x - nFT;
+        const int per = numTiles * nSplit;
+        const int blk = id / per, tile = (id % per) / nSplit, sp = id % nSplit;
+        const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx + 1];
+        if (P.p < 0 || P.p + 64 * tile >= P.iend) return;
+        ldl_update_body(P, blk, numBlocks, tile, sp, nSplit, LDL_K_OLD, sp, lt, rowoff, sfirst, diag, scratchU,
+                        scratchRows, smem);
     }
 }
 
