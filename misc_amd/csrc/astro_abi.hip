@@ -96,6 +96,10 @@ struct DeviceProfile {
     // sweeps
     int numTiles = 0;
     DBuf<SweepTile> fwdTiles, bwdTiles;
+    DBuf<SweepBlock> blockInfo;  // per tile, indexed by SweepTile::blk
+    DBuf<double> blocks;         // SWEEP_KINDS blocks of 64 x 64 per tile
+    bool blocksReady = false;    // blocks match the current factor in lt
+    DBuf<double> gnear;          // per sweep: each tile's L_tt^-1 (or ^-T) part, for its neighbour
     DBuf<int> fwdFlags, bwdFlags, ticket, timeout;
     int fwdEpoch = 0, bwdEpoch = 0;
     DBuf<double> z;
@@ -206,55 +210,72 @@ struct DeviceProfile {
         numTiles = 0;
         for (auto& c : chainTiles) numTiles += (int)c.size();
         std::vector<SweepTile> f, bw;
-        std::vector<std::vector<int>> fIdx(nBlocks), bIdx(nBlocks);
+        // block index of a tile: chains in order, tiles in row order
+        std::vector<int> blkBase(nBlocks + 1, 0);
+        std::vector<SweepBlock> hb;
         for (int k = 0; k < nBlocks; ++k) {
-            fIdx[k].assign(chainTiles[k].size(), -1);
-            bIdx[k].assign(chainTiles[k].size(), -1);
+            blkBase[k + 1] = blkBase[k] + (int)chainTiles[k].size();
+            const int cnt = (int)chainTiles[k].size();
+            auto rowEnd = [&](int q) { return std::min(chainTiles[k][q] + 64, bounds[k + 1]); };
+            for (int q = 0; q < cnt; ++q) {
+                SweepBlock B;
+                B.r0 = chainTiles[k][q];
+                B.r1 = rowEnd(q);
+                B.pm1 = q >= 1 ? chainTiles[k][q - 1] : B.r0;
+                B.pm2 = q >= 2 ? chainTiles[k][q - 2] : B.pm1;
+                B.q1 = q + 1 < cnt ? rowEnd(q + 1) : B.r1;
+                B.q2 = q + 2 < cnt ? rowEnd(q + 2) : B.q1;
+                B.prevBlk = q >= 1 ? blkBase[k] + q - 1 : -1;
+                B.nextBlk = q + 1 < cnt ? blkBase[k] + q + 1 : -1;
+                hb.push_back(B);
+            }
         }
-        for (int q = 0; q < maxCount; ++q)
-            for (int k = 0; k < nBlocks; ++k)
-                if (q < (int)chainTiles[k].size()) {
-                    fIdx[k][q] = (int)f.size();
-                    int r0 = chainTiles[k][q], r1 = std::min(r0 + 64, bounds[k + 1]);
-                    f.push_back(SweepTile{r0, r1, k, 0, -1});
-                }
+        // tickets: chains interleaved by position, forward upwards, backward
+        // downwards; each tile records its chain position's neighbours
         for (int q = 0; q < maxCount; ++q)
             for (int k = 0; k < nBlocks; ++k) {
-                int cnt = (int)chainTiles[k].size();
-                int t = cnt - 1 - q;
-                if (t >= 0) {
-                    bIdx[k][t] = (int)bw.size();
-                    int r0 = chainTiles[k][t], r1 = std::min(r0 + 64, bounds[k + 1]);
-                    bw.push_back(SweepTile{r0, r1, k, 0, -1});
-                }
+                const int cnt = (int)chainTiles[k].size();
+                if (q >= cnt) continue;
+                const int b0 = bounds[k], b1 = bounds[k + 1];
+                const int r0 = chainTiles[k][q], r1 = std::min(r0 + 64, b1);
+                int sm = r0;
+                for (int r = r0; r < r1; ++r) sm = std::min(sm, hS[r]);
+                const int td = (sm - b0) / 64;
+                f.push_back(SweepTile{r0, r1, blkBase[k] + q, std::min(3, q), td <= q - 4 ? blkBase[k] + td : -1, b1});
             }
+        std::vector<std::vector<int>> reachBack(nBlocks);
         for (int k = 0; k < nBlocks; ++k) {
             const int b0 = bounds[k], b1 = bounds[k + 1];
             const int cnt = (int)chainTiles[k].size();
-            // forward: first tile holding the smallest first column of the tile's rows
-            for (int t = 0; t < cnt; ++t) {
-                int r0 = b0 + 64 * t, r1 = std::min(r0 + 64, b1), sm = r0;
-                for (int r = r0; r < r1; ++r) sm = std::min(sm, hS[r]);
-                int td = (sm - b0) / 64;
-                f[fIdx[k][t]].depFirst = (td < t) ? fIdx[k][td] : fIdx[k][t];
-                f[fIdx[k][t]].depLast = (td < t) ? fIdx[k][t - 1] : -1;
-            }
-            // backward: last tile holding a row whose profile reaches the tile
+            // last tile holding a row whose profile reaches tile q
             std::vector<int> maxRowByTile(cnt, -1);
             for (int r = b0; r < b1; ++r) {
                 int tc = (hS[r] - b0) / 64;
                 maxRowByTile[tc] = std::max(maxRowByTile[tc], r);
             }
             int run = -1;
-            for (int t = 0; t < cnt; ++t) {
-                run = std::max(run, maxRowByTile[t]);
-                int td = (run - b0) / 64;
-                bw[bIdx[k][t]].depFirst = (td > t) ? bIdx[k][td] : bIdx[k][t];
-                bw[bIdx[k][t]].depLast = (td > t) ? bIdx[k][t + 1] : -1;
+            reachBack[k].resize(cnt);
+            for (int q = 0; q < cnt; ++q) {
+                run = std::max(run, maxRowByTile[q]);
+                reachBack[k][q] = std::max(q, (run - b0) / 64);
             }
         }
+        for (int q = 0; q < maxCount; ++q)
+            for (int k = 0; k < nBlocks; ++k) {
+                const int cnt = (int)chainTiles[k].size();
+                const int t = cnt - 1 - q;
+                if (t < 0) continue;
+                const int r0 = chainTiles[k][t], r1 = std::min(r0 + 64, bounds[k + 1]);
+                const int td = reachBack[k][t];
+                bw.push_back(SweepTile{r0, r1, blkBase[k] + t, std::min(3, cnt - 1 - t),
+                                       td >= t + 4 ? blkBase[k] + td : -1, bounds[k + 1]});
+            }
         ASTRO_TRY(fwdTiles.upload(f, s));
         ASTRO_TRY(bwdTiles.upload(bw, s));
+        ASTRO_TRY(blockInfo.upload(hb, s));
+        ASTRO_TRY(blocks.alloc((size_t)numTiles * SWEEP_KINDS * SWEEP_BLOCK));
+        ASTRO_TRY(gnear.alloc(n));
+        blocksReady = false;
         ASTRO_TRY(fwdFlags.alloc(numTiles));
         ASTRO_TRY(bwdFlags.alloc(numTiles));
         HIP_TRY(hipMemsetAsync(fwdFlags.p, 0, sizeof(int) * numTiles, s));
@@ -290,6 +311,7 @@ struct DeviceProfile {
                                    LDL_K_NEW, splitK, lt.p, rowoff.p, sfirst.p, diag.p, buf(q + 1), scratchRows);
         }
         HIP_TRY(hipGetLastError());
+        ASTRO_TRY(prepare_blocks(s));
         int zp = 0;
         HIP_TRY(hipMemcpyAsync(&zp, zeroPivot.p, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
@@ -297,28 +319,45 @@ struct DeviceProfile {
         return ASTRO_OK;
     }
 
+    // The sweeps' per-tile blocks, derived from the factor now in lt.  Every
+    // writer of lt other than factor() clears blocksReady.
+    int prepare_blocks(hipStream_t s) {
+        if (numTiles > 0) {
+            hipLaunchKernelGGL(k_sweep_blocks, dim3(numTiles), dim3(256), 0, s, blockInfo.p, lt.p, rowoff.p,
+                               sfirst.p, blocks.p);
+            hipLaunchKernelGGL(k_sweep_blocks_pair, dim3(numTiles), dim3(256), 0, s, blockInfo.p, blocks.p);
+        }
+        HIP_TRY(hipGetLastError());
+        blocksReady = true;
+        return ASTRO_OK;
+    }
+
     // x = P^-1 b (forward then backward sweep); z keeps L^-1 b.  Async.
     int apply(const double* b, double* x, hipStream_t s) {
+        if (!blocksReady) ASTRO_TRY(prepare_blocks(s));
         ++fwdEpoch;
         // the sweeps poll the neighbouring tile's values: mark them unproduced
         hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, z.p, (int64_t)n);
         hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, x, (int64_t)n);
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, gnear.p, (int64_t)n);
         hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
         if (timing) {
             evKind.push_back(0);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
         hipLaunchKernelGGL(k_sweep_forward, dim3(numTiles), dim3(256), 0, s, fwdTiles.p, numTiles, lt.p, rowoff.p,
-                           sfirst.p, b, z.p, fwdFlags.p, ticket.p, fwdEpoch, spinLimit, timeout.p);
+                           sfirst.p, blocks.p, b, z.p, gnear.p, fwdFlags.p, ticket.p, fwdEpoch, spinLimit, timeout.p);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         ++bwdEpoch;
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, gnear.p, (int64_t)n);
         hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
         if (timing) {
             evKind.push_back(1);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
         hipLaunchKernelGGL(k_sweep_backward, dim3(numTiles), dim3(256), 0, s, bwdTiles.p, numTiles, lt.p,
-                           rowoff.p, sfirst.p, z.p, x, bwdFlags.p, ticket.p, bwdEpoch, spinLimit, timeout.p);
+                           rowoff.p, sfirst.p, blocks.p, z.p, x, gnear.p, bwdFlags.p, ticket.p, bwdEpoch, spinLimit,
+                           timeout.p);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
@@ -930,6 +969,8 @@ int astro_ctx_set_f64(astro_ctx* c, const char* name, const double* host, int64_
     double* d = dev_double_array(c, name, len);
     if (len < 0 || !d) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
     if (len != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
+    if (d == c->prof.lt.p) c->prof.blocksReady = false;
+    if (d == c->tprof.lt.p) c->tprof.blocksReady = false;
     HIP_TRY(hipMemcpyAsync(d, host, sizeof(double) * count, hipMemcpyHostToDevice, c->s));
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
@@ -980,6 +1021,7 @@ int astro_ctx_build_profile(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     DeviceProfile& P = c->prof;
     const int n = c->prep.elMatDim;
+    P.blocksReady = false;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, c->s, P.lt.p, P.size, 0.0);
     hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, c->s,
                        c->csrVal.p, c->csrCols.p, c->csrRange.p, n, P.sfirst.p, P.rowoff.p, P.lt.p);
@@ -1277,6 +1319,7 @@ int astro_ctx_thermal_solve(astro_ctx* c) {
         hipLaunchKernelGGL(k_scale_mpc, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p, c->tMpcRhs.p,
                            c->tMpcRange.p, cd, c->maxBits.p);
     DeviceProfile& P = c->tprof;
+    P.blocksReady = false;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, s, P.lt.p, P.size, 0.0);
     hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)N * 32, 256, 1 << 30)), dim3(256), 0, s, c->tVal.p,
                        c->tCols.p, c->tRange.p, N, P.sfirst.p, P.rowoff.p, P.lt.p);

@@ -15,13 +15,38 @@ struct PanelInfo {
 };
 
 // One 64-row tile of a triangular sweep in processing (ticket) order: rows
-// [r0, r1) of chain `chain` (an independent diagonal block); depFirst is the
-// ticket index of the first tile it depends on (== own index if none).
-// depLast: ticket index of the immediately neighbouring tile of the chain
-// (preceding for the forward sweep, following for the backward sweep) when
-// the tile depends on it, else -1; it is handled last, on the critical path.
+// [r0, r1).  Tiles are numbered by blk: the chains (independent diagonal
+// blocks) one after another, each in row order, so the tiles of a chain are
+// consecutive 64-row blocks with consecutive blk.  In sweep direction (up for
+// the forward sweep, down for the backward sweep) the 1st..3rd neighbours are
+// the "near" tiles, nNear of them exist (0..3); the tiles beyond them that
+// the tile couples to are "far": blk from farBlk (the farthest one; -1 for
+// none) up to the 4th neighbour.  lim: the chain's last row + 1, which bounds
+// a ragged final tile.  Completion flags are indexed by blk.
 struct SweepTile {
-    int r0, r1, chain, depFirst, depLast;
+    int r0, r1, blk, nNear, farBlk, lim;
+};
+
+// Per-tile 64x64 blocks derived from the factor.  With L_tt the unit-lower
+// diagonal block, F_t = L_tt^-1 L(t, t-1) and H_t = L_tt^-1 L(t, t-2), the
+// forward sweep z = L^-1 b obeys, with g_t = L_tt^-1 (b_t - sum over tiles
+// s <= t-3 of L(t, s) z_s),
+//   z_t = g_t - F_t g_{t-1} + (F_t H_{t-1}) z_{t-3} + (F_t F_{t-1} - H_t) z_{t-2},
+// so a tile waits on the tile two back, not on its neighbour, and only the
+// last product sits on the critical path.  The backward sweep is the same
+// with G_t = L_tt^-T L(t+1, t)^T, HB_t = L_tt^-T L(t+2, t)^T and the tiles
+// after t.  Missing neighbours count as zero blocks.  Tile rows [r0, r1);
+// rows [pm1, r0) and [pm2, pm1) are tiles t-1 and t-2 (empty: none), rows
+// [r1, q1) and [q1, q2) tiles t+1 and t+2; prevBlk / nextBlk index the
+// blocks of t-1 / t+1 (-1: none).  Each block is column-major (entry (i, c)
+// at c * 64 + i) and zero outside the rows that exist.
+struct SweepBlock {
+    int r0, r1, pm1, pm2, q1, q2, prevBlk, nextBlk;
+};
+enum {
+    SWEEP_DINV = 0, SWEEP_F = 1, SWEEP_H = 2, SWEEP_FH = 3, SWEEP_K = 4,
+    SWEEP_DINVT = 5, SWEEP_G = 6, SWEEP_HB = 7, SWEEP_GHB = 8, SWEEP_KB = 9,
+    SWEEP_KINDS = 10, SWEEP_BLOCK = 64 * 64
 };
 
 __global__ void k_brick8_tangent(const int* perm, int count, const int* conn, const double* X, const int* elMat,
@@ -65,12 +90,15 @@ __global__ void k_csr_to_profile(const double* val, const int* cols, const int* 
                                  const int64_t* rowoff, double* lt);
 __global__ void k_penalty_to_profile(const double* cmat, const int* ccols, const int* crange, int cdim,
                                      const int* sfirst, const int64_t* rowoff, double* lt);
+__global__ void k_sweep_blocks(const SweepBlock* blocks, const double* lt, const int64_t* rowoff, const int* sfirst,
+                               double* out);
+__global__ void k_sweep_blocks_pair(const SweepBlock* blocks, double* out);
 __global__ void k_sweep_forward(const SweepTile* tiles, int numTiles, const double* lt, const int64_t* rowoff,
-                                const int* sfirst, const double* b, double* z, int* flags, int* ticket, int epoch,
-                                long spinLimit, int* timeout);
+                                const int* sfirst, const double* blocks, const double* b, double* z, double* g,
+                                int* flags, int* ticket, int epoch, long spinLimit, int* timeout);
 __global__ void k_sweep_backward(const SweepTile* tiles, int numTiles, const double* lt, const int64_t* rowoff,
-                                 const int* sfirst, const double* z, double* x, int* flags, int* ticket, int epoch,
-                                 long spinLimit, int* timeout);
+                                 const int* sfirst, const double* blocks, const double* z, double* x, double* g,
+                                 int* flags, int* ticket, int epoch, long spinLimit, int* timeout);
 __global__ void k_reset_ticket(int* ticket);
 __global__ void k_spmv(const double* val, const int* cols, const int* range, int n, const double* x, double* y);
 __global__ void k_mpc_apply(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,

@@ -1092,20 +1092,35 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// sum_c row[c] * v(lane c): broadcasts taken in groups of eight before their
-// products, so the read-lane -> scalar-operand waits overlap (about 20 instead
-// of 33 cycles per column on gfx950).
-__device__ __forceinline__ double coupling_product(const double (&row)[64], double v) {
-    double s2[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int c0 = 0; c0 < 64; c0 += 8) {
-        double vb[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) vb[k] = lane_bcast(v, c0 + k);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) s2[k & 3] += row[c0 + k] * vb[k];
-    }
-    return (s2[0] + s2[1]) + (s2[2] + s2[3]);
+// gfx950 cross-lane moves on doubles.  swap_halves32(a, c): lanes 32..63 of
+// a take lanes 0..31 of c and lanes 0..31 of c take lanes 32..63 of a, so
+// a + c is, in lanes 0..31, a summed with its l ^ 32 partner and, in lanes
+// 32..63, c summed with its partner.  swap_rows16 does the same between the
+// odd 16-lane rows of a and the even rows of c (partner l ^ 16).
+__device__ __forceinline__ void swap_halves32(double& a, double& c) {
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(c);
+    const auto lo = __builtin_amdgcn_permlane32_swap((int)x, (int)y, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((int)(x >> 32), (int)(y >> 32), false, false);
+    a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+    c = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+__device__ __forceinline__ void swap_rows16(double& a, double& c) {
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(c);
+    const auto lo = __builtin_amdgcn_permlane16_swap((int)x, (int)y, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((int)(x >> 32), (int)(y >> 32), false, false);
+    a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+    c = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+// DPP controls: row_shl:n reads lane l + n, row_shr:n lane l - n (within a
+// 16-lane row); quad_perm 2,3,0,1 and 1,0,3,2 read lanes l ^ 2 and l ^ 1.
+enum { DPP_ROW_SHL8 = 0x108, DPP_ROW_SHR8 = 0x118, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR4 = 0x114,
+       DPP_QUAD_XOR2 = 0x4E, DPP_QUAD_XOR1 = 0xB1 };
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Cross-workgroup hand-off without cache-maintenance fences: the values a
@@ -1153,29 +1168,33 @@ __device__ __forceinline__ bool poll_values(const double* v, int base, int cnt, 
     }
 }
 
-// Non-spinning scan: the first ticket in [from, last) of `chain` whose flag
-// does not hold `epoch` (last if none).  Tickets of other chains count as
-// done.
-__device__ __forceinline__ int first_pending(const int* flags, const SweepTile* tiles, int chain, int from, int last,
-                                             int epoch) {
+// Non-spinning scans of the completion flags of a chain's tiles (indexed by
+// blk).  first_pending_up: the first blk in [from, last) whose flag does not
+// hold `epoch` (last if none); first_pending_down: the first in (last, from]
+// going down (last if none).
+__device__ __forceinline__ int first_pending_up(const int* flags, int from, int last, int epoch) {
     const int lane = threadIdx.x & 63;
     while (from < last) {
         const int J = from + lane;
-        bool done = true;
-        if (J < last && tiles[J].chain == chain)
-            done = __hip_atomic_load(&flags[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+        const bool done =
+            J >= last || __hip_atomic_load(&flags[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
         const unsigned long long pend = __ballot(!done);
         if (pend) return from + (int)__builtin_ctzll(pend);
         from += 64;
     }
     return last;
 }
-
-// Next ticket after J of the same chain, or `last`.
-__device__ __forceinline__ int next_in_chain(const SweepTile* tiles, int chain, int J, int last) {
-    int K = J + 1;
-    while (K < last && tiles[K].chain != chain) ++K;
-    return K;
+__device__ __forceinline__ int first_pending_down(const int* flags, int from, int last, int epoch) {
+    const int lane = threadIdx.x & 63;
+    while (from > last) {
+        const int J = from - lane;
+        const bool done =
+            J <= last || __hip_atomic_load(&flags[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+        const unsigned long long pend = __ballot(!done);
+        if (pend) return from - (int)__builtin_ctzll(pend);
+        from -= 64;
+    }
+    return last;
 }
 
 // Waiters closer than this many tiles (of their chain) to the tile they
@@ -1191,75 +1210,317 @@ __device__ __forceinline__ void publish_flag(int* flag, int epoch) {
     __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Forward sweep tile: phase 1 accumulates the contributions of all earlier
-// tiles except the immediately preceding one (wave w owns rows w*16..+15,
-// lane = column, partials reduced across lanes once); phase 2 waits for the
-// preceding tile, applies its 64x64 coupling block (prefetched into LDS) and
-// solves the unit-lower diagonal block, so only LDS work and one coherent
-// load/store pair sit on the critical path between consecutive tiles.
-__global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
-                                                       const double* __restrict__ lt,
-                                                       const int64_t* __restrict__ rowoff,
-                                                       const int* __restrict__ sfirst, const double* __restrict__ b,
-                                                       double* __restrict__ z, int* __restrict__ flags,
-                                                       int* __restrict__ ticket, int epoch, long spinLimit,
-                                                       int* __restrict__ timeout) {
+// C[i][c] = L(rowStart + i, colStart + c) inside the profile (every column
+// used here lies left of its row), zero elsewhere and beyond nRows x nCols.
+__device__ __forceinline__ void load_coupling(double (*C)[65], int rowStart, int nRows, int colStart, int nCols,
+                                              const double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                              const int* __restrict__ sfirst) {
+    for (int id = threadIdx.x; id < 64 * 64; id += 256) {
+        const int i = id >> 6, c = id & 63;
+        double u = 0.0;
+        if (i < nRows && c < nCols) {
+            const int gi = rowStart + i;
+            if (colStart + c >= sfirst[gi]) u = lt[rowoff[gi] + colStart + c];
+        }
+        C[i][c] = u;
+    }
+}
+
+// dst(i, j) = sum_{c <= i} X(i, c) C(c, j)  (X lower triangular), column-major.
+__device__ __forceinline__ void lower_times(const double (*X)[65], const double (*C)[65], double* __restrict__ dst) {
+    const int i = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * 16;
+    for (int j = j0; j < j0 + 16; ++j) {
+        double a = 0.0;
+        for (int c = 0; c <= i; ++c) a += X[i][c] * C[c][j];
+        dst[j * 64 + i] = a;
+    }
+}
+
+// dst(i, j) = sum_{c >= i} X(c, i) C(j, c)  (X^T C^T), column-major.
+__device__ __forceinline__ void lower_t_times_t(const double (*X)[65], const double (*C)[65],
+                                                double* __restrict__ dst) {
+    const int i = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * 16;
+    for (int j = j0; j < j0 + 16; ++j) {
+        double a = 0.0;
+        for (int c = i; c < 64; ++c) a += X[c][i] * C[j][c];
+        dst[j * 64 + i] = a;
+    }
+}
+
+// First pass over the tiles (one workgroup each): L_tt^-1 by forward
+// substitution of the identity columns, then its products with the coupling
+// blocks to the two preceding and two following tiles.  Rows beyond the
+// tile's end are the identity in L_tt.
+__global__ __launch_bounds__(256) void k_sweep_blocks(const SweepBlock* __restrict__ blocks,
+                                                      const double* __restrict__ lt,
+                                                      const int64_t* __restrict__ rowoff,
+                                                      const int* __restrict__ sfirst, double* __restrict__ out) {
+    __shared__ double L[64][65];  // unit lower L_tt
+    __shared__ double X[64][65];  // L_tt^-1
+    __shared__ double C[64][65];  // one coupling block at a time
+    const SweepBlock S = blocks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int nr = S.r1 - S.r0;
+    double* o = out + (size_t)blockIdx.x * SWEEP_KINDS * SWEEP_BLOCK;
+    for (int id = tid; id < 64 * 64; id += 256) {
+        const int i = id >> 6, c = id & 63;
+        double l = c == i ? 1.0 : 0.0;
+        if (i < nr && c < i) {
+            const int gi = S.r0 + i;
+            if (S.r0 + c >= sfirst[gi]) l = lt[rowoff[gi] + S.r0 + c];
+        }
+        L[i][c] = l;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        // column c of L_tt^-1: entries above the diagonal are zero
+        const int c = tid;
+        for (int r = 0; r < 64; ++r) {
+            double a0 = r == c ? 1.0 : 0.0, a1 = 0.0;
+            int k = c;
+            for (; k + 1 < r; k += 2) {
+                a0 -= L[r][k] * X[k][c];
+                a1 -= L[r][k + 1] * X[k + 1][c];
+            }
+            if (k < r) a0 -= L[r][k] * X[k][c];
+            X[r][c] = r < c ? 0.0 : a0 + a1;
+        }
+    }
+    __syncthreads();
+    for (int id = tid; id < 64 * 64; id += 256) {
+        const int cc = id >> 6, ii = id & 63;
+        o[SWEEP_DINV * SWEEP_BLOCK + cc * 64 + ii] = X[ii][cc];
+        o[SWEEP_DINVT * SWEEP_BLOCK + cc * 64 + ii] = X[cc][ii];
+    }
+    load_coupling(C, S.r0, nr, S.pm1, S.r0 - S.pm1, lt, rowoff, sfirst);
+    __syncthreads();
+    lower_times(X, C, o + SWEEP_F * SWEEP_BLOCK);
+    __syncthreads();
+    load_coupling(C, S.r0, nr, S.pm2, S.pm1 - S.pm2, lt, rowoff, sfirst);
+    __syncthreads();
+    lower_times(X, C, o + SWEEP_H * SWEEP_BLOCK);
+    __syncthreads();
+    load_coupling(C, S.r1, S.q1 - S.r1, S.r0, nr, lt, rowoff, sfirst);
+    __syncthreads();
+    lower_t_times_t(X, C, o + SWEEP_G * SWEEP_BLOCK);
+    __syncthreads();
+    load_coupling(C, S.q1, S.q2 - S.q1, S.r0, nr, lt, rowoff, sfirst);
+    __syncthreads();
+    lower_t_times_t(X, C, o + SWEEP_HB * SWEEP_BLOCK);
+}
+
+// dst = A B (- E): column-major 64 x 64 blocks from global memory; a null B
+// counts as zero.
+__device__ __forceinline__ void block_gemm(const double* __restrict__ A, const double* __restrict__ B,
+                                           const double* __restrict__ E, double* __restrict__ dst,
+                                           double (*As)[65], double (*Bs)[65]) {
+    for (int id = threadIdx.x; id < 64 * 64; id += 256) {
+        const int c = id >> 6, i = id & 63;
+        As[i][c] = A[id];
+        Bs[i][c] = B ? B[id] : 0.0;
+    }
+    __syncthreads();
+    const int i = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * 16;
+    for (int j = j0; j < j0 + 16; ++j) {
+        double a = 0.0;
+        for (int c = 0; c < 64; ++c) a += As[i][c] * Bs[c][j];
+        dst[j * 64 + i] = E ? a - E[j * 64 + i] : a;
+    }
+    __syncthreads();
+}
+
+// Second pass: the products of a tile's blocks with its neighbour's,
+// FH = F_t H_{t-1}, K = F_t F_{t-1} - H_t, GHB = G_t HB_{t+1},
+// KB = G_t G_{t+1} - HB_t.
+__global__ __launch_bounds__(256) void k_sweep_blocks_pair(const SweepBlock* __restrict__ blocks,
+                                                           double* __restrict__ out) {
+    __shared__ double As[64][65];
+    __shared__ double Bs[64][65];
+    const SweepBlock S = blocks[blockIdx.x];
+    double* o = out + (size_t)blockIdx.x * SWEEP_KINDS * SWEEP_BLOCK;
+    const double* p = S.prevBlk >= 0 ? out + (size_t)S.prevBlk * SWEEP_KINDS * SWEEP_BLOCK : nullptr;
+    const double* n = S.nextBlk >= 0 ? out + (size_t)S.nextBlk * SWEEP_KINDS * SWEEP_BLOCK : nullptr;
+    block_gemm(o + SWEEP_F * SWEEP_BLOCK, p ? p + SWEEP_H * SWEEP_BLOCK : nullptr, nullptr,
+               o + SWEEP_FH * SWEEP_BLOCK, As, Bs);
+    block_gemm(o + SWEEP_F * SWEEP_BLOCK, p ? p + SWEEP_F * SWEEP_BLOCK : nullptr, o + SWEEP_H * SWEEP_BLOCK,
+               o + SWEEP_K * SWEEP_BLOCK, As, Bs);
+    block_gemm(o + SWEEP_G * SWEEP_BLOCK, n ? n + SWEEP_HB * SWEEP_BLOCK : nullptr, nullptr,
+               o + SWEEP_GHB * SWEEP_BLOCK, As, Bs);
+    block_gemm(o + SWEEP_G * SWEEP_BLOCK, n ? n + SWEEP_G * SWEEP_BLOCK : nullptr, o + SWEEP_HB * SWEEP_BLOCK,
+               o + SWEEP_KB * SWEEP_BLOCK, As, Bs);
+}
+
+// This wave's share of a block product: sum over 16 columns k of
+// m[k] v(lane lane0 + k), m[k] = M(lane, c(k)).  Broadcasts are taken in
+// groups of eight before their products, so the read-lane -> scalar-operand
+// waits overlap.
+__device__ __forceinline__ double block_product16(const double (&m)[16], double v, int lane0) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 8) {
+        double vb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) vb[k] = lane_bcast(v, lane0 + k0 + k);
+#pragma unroll
+        for (int k = 0; k < 8; k += 2) {
+            s0 += m[k0 + k] * vb[k];
+            s1 += m[k0 + k + 1] * vb[k + 1];
+        }
+    }
+    return s0 + s1;
+}
+
+// Values of rows [base, base + cnt) in lanes 0..cnt-1 (others 0), loaded
+// speculatively; near_wait completes them.
+struct NearVals {
+    int base, cnt;
+    double v;
+};
+__device__ __forceinline__ NearVals near_load(const double* v, int base, int cnt) {
+    const int lane = threadIdx.x & 63;
+    NearVals n{base, cnt, 0.0};
+    n.v = lane < cnt ? coherent_load(&v[base + lane]) : 0.0;
+    return n;
+}
+__device__ __forceinline__ bool near_wait(const double* v, NearVals& n, long spinLimit, int* timeout) {
+    const int lane = threadIdx.x & 63;
+    if (!__ballot(lane < n.cnt && __double_as_longlong(n.v) == ASTRO_SENTINEL)) return true;
+    return poll_values(v, n.base, n.cnt, 0, spinLimit, timeout, n.v);
+}
+
+// The end of a sweep tile, after the far part (including the third near
+// tile) has been reduced to h (lane = row, zero beyond the tile): g_t = D h is
+// published to gout; then
+//   out_t = g_t - B1 g_{near1} + B3 v_{near3} + B2 v_{near2}
+// (v = the sweep's own output, gin = gout of the same launch).  Every product
+// is split over the four waves (16 columns each: dv and kv in registers, B1
+// and B3 from LDS; v3 is broadcast from lanes v3lane0 + k); only the B2
+// product follows the wait on v_{near2}.
+__device__ __forceinline__ bool lookahead_tail(int r0, int nr, int nNear, int flagIdx, double h,
+                                               const double (&dv)[16], const double (&kv)[16],
+                                               const double (*B1s)[64], const double (*B3s)[64], double v3,
+                                               int v3lane0, double* gout, double* out, int* flags, int epoch,
+                                               long spinLimit, int* timeout, NearVals& n1, NearVals& n2,
+                                               double (*acc)[64], double (*acc2)[64]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    acc[w][lane] = block_product16(dv, h, 16 * w);
+    __syncthreads();
+    const double gt = (acc[0][lane] + acc[1][lane]) + (acc[2][lane] + acc[3][lane]);
+    if (w == 0 && lane < nr) coherent_store(&gout[r0 + lane], gt);
+    double pw = 0.0;
+    if (nNear >= 3) {
+        double m[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m[k] = B3s[k][lane];
+        pw += block_product16(m, v3, v3lane0);
+    }
+    if (nNear >= 1) {
+        if (!near_wait(gout, n1, spinLimit, timeout)) return false;
+        double m[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m[k] = B1s[k][lane];
+        pw -= block_product16(m, n1.v, 0);
+    }
+    if (nNear >= 2) {
+        if (!near_wait(out, n2, spinLimit, timeout)) return false;
+        pw += block_product16(kv, n2.v, 0);
+    }
+    acc2[w][lane] = pw;
+    __syncthreads();
+    if (w == 0) {
+        const double a = gt + ((acc2[0][lane] + acc2[1][lane]) + (acc2[2][lane] + acc2[3][lane]));
+        if (lane < nr) coherent_store(&out[r0 + lane], a);
+        stores_complete();
+        if (lane == 0) publish_flag(&flags[flagIdx], epoch);
+    }
+    return true;
+}
+
+// Forward sweep tile.  Phase 1 accumulates the contributions of the far
+// tiles (four or more tiles back; wave w owns rows w*16..+15, lane =
+// column) and of the third near tile (its coupling entries prefetched at
+// the start), partials reduced across lanes once; the tail combines the near
+// tiles through the precomputed blocks.
+__global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
+                                                          const double* __restrict__ lt,
+                                                          const int64_t* __restrict__ rowoff,
+                                                          const int* __restrict__ sfirst,
+                                                          const double* __restrict__ blocks,
+                                                          const double* __restrict__ b, double* __restrict__ z,
+                                                          double* __restrict__ g, int* __restrict__ flags,
+                                                          int* __restrict__ ticket, int epoch, long spinLimit,
+                                                          int* __restrict__ timeout) {
     __shared__ int sT;
-    __shared__ double Ld[64][65];   // Ld[i][c] = L(r0+i, r0+c), c < i
-    __shared__ double Lp[64][65];   // Lp[i][c] = L(r0+i, p0+c), preceding tile columns
     __shared__ double part[4][16];
+    __shared__ double acc[4][64];
+    __shared__ double acc2[4][64];
+    __shared__ double B1s[4][16][64];  // wave w's 16 columns of F_t
+    __shared__ double B3s[4][16][64];  // and of F_t H_{t-1}
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
     const int t = sT;
     if (t >= numTiles) return;
     const SweepTile T = tiles[t];
-    const int r0 = T.r0, nr = T.r1 - T.r0;
-    const bool hasPrev = T.depLast >= 0;
-    const SweepTile Pv = hasPrev ? tiles[T.depLast] : T;
-    const int p0 = Pv.r0, np = hasPrev ? Pv.r1 - Pv.r0 : 0;
-    for (int id = tid; id < 64 * 64; id += 256) {
-        const int i = id >> 6, c = id & 63;
-        double v = 0.0, u = 0.0;
-        if (i < nr) {
-            // loads at indices clamped into row gi's stored range, then
-            // selects: the loads of all columns are in flight together
-            const int gi = r0 + i, si = sfirst[gi];
-            const int64_t off = rowoff[gi];
-            const double vv = lt[off + min(max(r0 + c, si), gi)];
-            const double uu = lt[off + min(max(p0 + c, si), gi)];
-            v = (c < i && r0 + c >= si) ? vv : 0.0;
-            u = (c < np && p0 + c >= si) ? uu : 0.0;
-        }
-        Ld[i][c] = v;
-        Lp[i][c] = u;
+    const int r0 = T.r0, nr = T.r1 - T.r0, nNear = T.nNear;
+    // this wave's 16 columns of L_tt^-1 and of K_t, coalesced; F_t and
+    // F_t H_{t-1} go to LDS now, far from the critical path (each wave reads
+    // back only what it wrote)
+    const double* Bt = blocks + (size_t)T.blk * SWEEP_KINDS * SWEEP_BLOCK;
+    double dv[16], kv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        dv[k] = Bt[SWEEP_DINV * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
+        kv[k] = nNear >= 2 ? Bt[SWEEP_K * SWEEP_BLOCK + (16 * w + k) * 64 + lane] : 0.0;
     }
-    int rs[16];
-    int64_t ro[16];
+    if (nNear >= 1) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) B1s[w][k][lane] = Bt[SWEEP_F * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
+    }
+    if (nNear >= 3) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) B3s[w][k][lane] = Bt[SWEEP_FH * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
+    }
+    const double bl = lane < nr ? b[r0 + lane] : 0.0;
+    // the wave's rows as 32-bit offsets from its first row's (rows beyond
+    // the tile read that row's entries: column c < r0 is inside every row's
+    // addressable range, so the unselected loads stay in the profile)
+    const int rb = min(r0 + 16 * w, T.r1 - 1);
+    const double* ltw = lt + rowoff[rb];
+    int rs[16], rd[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int i = w * 16 + r;
         rs[r] = i < nr ? sfirst[r0 + i] : 0x7fffffff;
-        ro[r] = i < nr ? rowoff[r0 + i] : 0;
+        rd[r] = i < nr ? (int)(rowoff[r0 + i] - rowoff[rb]) : 0;
+    }
+    // coupling entries to the third near tile's columns
+    double lv3[16];
+    {
+        const int c = r0 - 192 + lane;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) lv3[r] = nNear >= 3 ? ltw[(unsigned int)(rd[r] + c)] : 0.0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) lv3[r] = c >= rs[r] ? lv3[r] : 0.0;
     }
     double p[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) p[r] = 0.0;
-    const int lastFar = hasPrev ? T.depLast : t;  // phase 1 covers tickets [depFirst, lastFar)
-    // The far columns are the rows of earlier tiles of this chain, contiguous
-    // from the first far tile's r0 up to the preceding tile's r0.  Each wave
-    // streams the columns of the tiles already published and re-polls at the
-    // first pending one; no block-wide barrier.
-    if (T.depFirst < lastFar) {
-        const int cEnd = p0;
-        int c0 = tiles[T.depFirst].r0;
-        int J = T.depFirst;
+    // Far tiles: blk farBlk .. blk-4, whose rows are the contiguous columns
+    // [r0 - 64 (blk - farBlk), r0 - 192).  Each wave streams the columns of
+    // the tiles already published and re-polls at the first pending one; no
+    // block-wide barrier.
+    if (T.farBlk >= 0) {
+        const int bEnd = T.blk - 3;
+        int J = T.farBlk;
+        int c0 = r0 - 64 * (T.blk - J);
         bool front = false;
-        while (c0 < cEnd && J < lastFar) {
+        while (J < bEnd) {
             if (!front) {
                 // bulk: every column of the published prefix of far tiles
-                const int pend = first_pending(flags, tiles, T.chain, J, lastFar, epoch);
-                const int cStop = pend >= lastFar ? cEnd : tiles[pend].r0;
+                const int pend = first_pending_up(flags, J, bEnd, epoch);
+                const int cStop = r0 - 64 * (T.blk - pend);
+#pragma unroll 1
                 for (int c = c0 + lane; c < cStop; c += 64) {
                     const double zc = coherent_load(&z[c]);
                     // unconditional loads (a column left of a row's first
@@ -1268,152 +1529,173 @@ __global__ __launch_bounds__(256) void k_sweep_forward(const SweepTile* __restri
                     // and wait
                     double lv[16];
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) lv[r] = lt[ro[r] + c];
+                    for (int r = 0; r < 16; ++r) lv[r] = ltw[(unsigned int)(rd[r] + c)];
 #pragma unroll
                     for (int r = 0; r < 16; ++r) p[r] += (c >= rs[r] ? lv[r] : 0.0) * zc;
                 }
                 c0 = cStop;
                 J = pend;
-                if (pend >= lastFar) break;
+                if (J >= bEnd) break;
             }
             // tile J is not known to be done: load this wave's coupling
             // entries to its columns first, then wait for its values
-            const SweepTile Q = tiles[J];
-            const int c = Q.r0 + lane;
+            const int c = c0 + lane;
             double lv[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) lv[r] = lt[ro[r] + c];
+            for (int r = 0; r < 16; ++r) lv[r] = ltw[(unsigned int)(rd[r] + c)];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) lv[r] = (c < Q.r1 && c >= rs[r]) ? lv[r] : 0.0;
-            const int dist = (r0 - Q.r0) >> 6;
+            for (int r = 0; r < 16; ++r) lv[r] = c >= rs[r] ? lv[r] : 0.0;
+            const int dist = T.blk - J;
             double zc;
-            if (!poll_values(z, Q.r0, Q.r1 - Q.r0, backoff_for(dist), spinLimit, timeout, zc)) return;
+            if (!poll_values(z, c0, 64, backoff_for(dist), spinLimit, timeout, zc)) return;
 #pragma unroll
             for (int r = 0; r < 16; ++r) p[r] += lv[r] * zc;
-            c0 = Q.r1;
-            J = next_in_chain(tiles, T.chain, J, lastFar);
+            c0 += 64;
+            ++J;
             front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
-    // speculative load of the preceding tile's values: usually already
-    // produced, and its latency overlaps the reduction and barrier below
-    double zv = 0.0;
-    if (w == 0 && hasPrev) zv = lane < np ? coherent_load(&z[p0 + lane]) : 0.0;
-    // transpose-reduce the 16 row partials across the wave: each xor stage
-    // halves the rows a lane carries, so lane l ends with row (l >> 2) & 15
-    // summed over all 64 lanes (17 exchanges instead of 16 x 6)
+    // the near tiles' values, loaded speculatively (usually produced by now);
+    // the third near tile's are needed at once
+    NearVals n1{0, 0, 0.0}, n2{0, 0, 0.0}, n3{0, 0, 0.0};
+    if (nNear >= 1) n1 = near_load(g, r0 - 64 + 16 * w, 16);
+    if (nNear >= 2) n2 = near_load(z, r0 - 128 + 16 * w, 16);
+    if (nNear >= 3) {
+        n3 = near_load(z, r0 - 192, 64);
+        if (!near_wait(z, n3, spinLimit, timeout)) return;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) p[r] += lv3[r] * n3.v;
+    }
+    // transpose-reduce the 16 row partials across the wave: each stage
+    // exchanges half of the rows a lane carries with its partner lane
+    // (l ^ 32, l ^ 16, l ^ 8, l ^ 4) and adds, so lane l ends with row
+    // (l >> 2) & 15 summed over all 64 lanes.  The exchanges are VALU
+    // cross-lane moves (permlane swaps, DPP), not LDS permutes.
     {
-        const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+        const bool b3 = lane & 8, b2 = lane & 4;
         double q1[8], q2[4], q3[2];
 #pragma unroll
-        for (int r = 0; r < 8; ++r)
-            q1[r] = (b5 ? p[r + 8] : p[r]) + __shfl_xor(b5 ? p[r] : p[r + 8], 32);
+        for (int r = 0; r < 8; ++r) {
+            double a = p[r], c = p[r + 8];
+            swap_halves32(a, c);
+            q1[r] = a + c;  // row r + 8 * bit5
+        }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            q2[r] = (b4 ? q1[r + 4] : q1[r]) + __shfl_xor(b4 ? q1[r] : q1[r + 4], 16);
+        for (int r = 0; r < 4; ++r) {
+            double a = q1[r], c = q1[r + 4];
+            swap_rows16(a, c);
+            q2[r] = a + c;  // row r + 8 * bit5 + 4 * bit4
+        }
+        // both shifts are taken in every lane before the select: a DPP move
+        // inside a divergent branch would read disabled source lanes as 0
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
-            q3[r] = (b3 ? q2[r + 2] : q2[r]) + __shfl_xor(b3 ? q2[r] : q2[r + 2], 8);
-        double q4 = (b2 ? q3[1] : q3[0]) + __shfl_xor(b2 ? q3[0] : q3[1], 4);
-        q4 += __shfl_xor(q4, 2);
-        q4 += __shfl_xor(q4, 1);
+        for (int r = 0; r < 2; ++r) {
+            const double send = b3 ? q2[r] : q2[r + 2];
+            const double up = dpp_f64<DPP_ROW_SHL8>(send), down = dpp_f64<DPP_ROW_SHR8>(send);
+            q3[r] = (b3 ? q2[r + 2] : q2[r]) + (b3 ? down : up);
+        }
+        const double send = b2 ? q3[0] : q3[1];
+        const double up = dpp_f64<DPP_ROW_SHL4>(send), down = dpp_f64<DPP_ROW_SHR4>(send);
+        double q4 = (b2 ? q3[1] : q3[0]) + (b2 ? down : up);
+        q4 += dpp_f64<DPP_QUAD_XOR2>(q4);
+        q4 += dpp_f64<DPP_QUAD_XOR1>(q4);
         if ((lane & 3) == 0) part[w][(lane >> 2) & 15] = q4;
     }
     __syncthreads();
-    if (w == 0) {
-        const int i = lane;
-        double a = 0.0;
-        if (i < nr) a = b[r0 + i] - part[i >> 4][i & 15];
-        if (hasPrev) {
-            if (__ballot(i < np && __double_as_longlong(zv) == ASTRO_SENTINEL) &&
-                !poll_values(z, p0, np, 0, spinLimit, timeout, zv))
-                return;
-            // Lp is zero beyond np and lanes >= np hold 0: a fixed-trip
-            // unrolled product with the values broadcast from registers; the
-            // row is read from LDS in one burst first
-            double lp[64];
-#pragma unroll
-            for (int c = 0; c < 64; ++c) lp[c] = Lp[i][c];
-            a -= coupling_product(lp, zv);
-        }
-        // Ld[i][c] is zero for c >= i, so every lane applies every column
-        // (no predicate on the dependent chain)
-        double lrow[64];
-#pragma unroll
-        for (int c = 0; c < 64; ++c) lrow[c] = Ld[i][c];
-#pragma unroll
-        for (int c = 0; c < 64; ++c) a -= lrow[c] * lane_bcast(a, c);
-        if (i < nr) coherent_store(&z[r0 + i], a);
-        stores_complete();
-        if (i == 0) publish_flag(&flags[t], epoch);
-    }
+    const double h = lane < nr ? bl - part[lane >> 4][lane & 15] : 0.0;
+    lookahead_tail(r0, nr, nNear, T.blk, h, dv, kv, B1s[w], B3s[w], n3.v, 16 * w, g, z, flags, epoch, spinLimit,
+                   timeout, n1, n2, acc, acc2);
 }
 
-// Backward sweep tile (columns [r0, r1)): phase 1 takes the rows of all later
-// tiles except the immediately following one; phase 2 waits for that tile and
-// applies its rows' block (prefetched into LDS), then solves L^T on the
-// diagonal block.
+// Backward sweep tile (columns [r0, r1)): phase 1 takes the rows of the far
+// tiles (four or more tiles on) and of the third near tile (coupling entries
+// prefetched), the tail combines the near tiles.
 __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restrict__ tiles, int numTiles,
                                                         const double* __restrict__ lt,
                                                         const int64_t* __restrict__ rowoff,
-                                                        const int* __restrict__ sfirst, const double* __restrict__ z,
-                                                        double* __restrict__ x, int* __restrict__ flags,
+                                                        const int* __restrict__ sfirst,
+                                                        const double* __restrict__ blocks,
+                                                        const double* __restrict__ z, double* __restrict__ x,
+                                                        double* __restrict__ g, int* __restrict__ flags,
                                                         int* __restrict__ ticket, int epoch, long spinLimit,
                                                         int* __restrict__ timeout) {
     __shared__ int sT;
-    __shared__ double Ld[64][65];   // Ld[c][i] = L(r0+c, r0+i), i < c
-    __shared__ double Ln[64][65];   // Ln[j][i] = L(q0+j, r0+i), rows of the following tile
     __shared__ double part[4][64];
+    __shared__ double acc[4][64];
+    __shared__ double acc2[4][64];
+    __shared__ double B1s[4][16][64];  // wave w's 16 columns of G_t
+    __shared__ double B3s[4][16][64];  // and of G_t HB_{t+1}
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
     const int t = sT;
     if (t >= numTiles) return;
     const SweepTile T = tiles[t];
-    const int r0 = T.r0, nr = T.r1 - T.r0;
-    const bool hasNext = T.depLast >= 0;
-    const SweepTile Nx = hasNext ? tiles[T.depLast] : T;
-    const int q0 = Nx.r0, nq = hasNext ? Nx.r1 - Nx.r0 : 0;
-    for (int id = tid; id < 64 * 64; id += 256) {
-        const int c = id >> 6, i = id & 63;
-        double v = 0.0, u = 0.0;
-        // c is uniform across the wave; loads at indices clamped into the
-        // stored range of row r0+c (q0+c), then selects
-        if (c < nr) {
-            const int gc = r0 + c, sc = sfirst[gc];
-            const double vv = lt[rowoff[gc] + min(max(r0 + i, sc), gc)];
-            v = (i < nr && i < c && r0 + i >= sc) ? vv : 0.0;
+    const int r0 = T.r0, r1 = T.r1, nr = T.r1 - T.r0, nNear = T.nNear, lim = T.lim;
+    const double* Bt = blocks + (size_t)T.blk * SWEEP_KINDS * SWEEP_BLOCK;
+    double dv[16], kv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        dv[k] = Bt[SWEEP_DINVT * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
+        kv[k] = nNear >= 2 ? Bt[SWEEP_KB * SWEEP_BLOCK + (16 * w + k) * 64 + lane] : 0.0;
+    }
+    if (nNear >= 1) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) B1s[w][k][lane] = Bt[SWEEP_G * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
+    }
+    if (nNear >= 3) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) B3s[w][k][lane] = Bt[SWEEP_GHB * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
+    }
+    // D^-1 z for this tile's rows (z is complete: the forward sweep has ended)
+    double zd = 0.0;
+    if (lane < nr) {
+        const int gi = r0 + lane;
+        zd = z[gi] / lt[rowoff[gi] + gi];
+    }
+    const int ci = r0 + lane;
+    // an in-range column for idle lanes' loads; their partial sums are unused
+    const int cl = lane < nr ? ci : r0;
+    // coupling entries of this wave's 16 rows of the third near tile
+    const int jb3 = r1 + 128 + 16 * w;
+    const int cnt3 = nNear >= 3 ? max(0, min(16, lim - jb3)) : 0;
+    double lc3[16];
+    {
+        int sjl = 0x7fffffff;
+        long long rol = 0;
+        if (lane < cnt3) {
+            sjl = sfirst[jb3 + lane];
+            rol = rowoff[jb3 + lane];
         }
-        if (c < nq) {
-            const int gj = q0 + c, sj = sfirst[gj];
-            const double uu = lt[rowoff[gj] + min(max(r0 + i, sj), gj)];
-            u = (i < nr && r0 + i >= sj) ? uu : 0.0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int sj = __builtin_amdgcn_readlane(sjl, r);
+            const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
+                                 (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
+            // rows beyond cnt3 have ro = 0: lt[cl] is inside the profile
+            const double v = lt[ro + cl];
+            lc3[r] = ci >= sj ? v : 0.0;
         }
-        Ld[c][i] = v;
-        Ln[c][i] = u;
     }
     double p = 0.0;
-    const int ci = r0 + lane;
-    const bool lv = lane < nr;
-    // an in-range column for idle lanes' loads; their partial sums are unused
-    const int cl = lv ? ci : r0;
-    const int lastFar = hasNext ? T.depLast : t;
-    // Far rows: from the tile after the following one up to the last tile
-    // holding a row that reaches this tile.  Published tiles form a top-down
-    // prefix of that range (tiles finish in descending order); each wave
-    // streams the published rows and re-polls, without block barriers.
-    if (T.depFirst < lastFar) {
-        const int jLow = Nx.r1;
-        int j1 = tiles[T.depFirst].r1;  // rows [jStop, j1) still to do, top-down
-        int J = T.depFirst;
+    // Far tiles: blk blk+4 .. farBlk, rows [r1 + 192, end of farBlk).
+    // Published tiles form a top-down prefix of that range (tiles finish in
+    // descending order); each wave streams the published rows and re-polls,
+    // without block barriers.
+    if (T.farBlk >= 0) {
+        const int bLow = T.blk + 3;
+        const int jLow = r1 + 192;
+        int J = T.farBlk;
+        int j1 = min(r0 + 64 * (J - T.blk) + 64, lim);  // rows [jStop, j1) still to do, top-down
         bool front = false;
-        while (j1 > jLow && J < lastFar) {
+        while (J > bLow) {
             if (!front) {
                 // bulk: every row of the published prefix of far tiles; wave
                 // w takes 16-row groups jb = jStop + 16 w + 64 q, lanes 0..15
                 // fetch the group's first columns, offsets and x, broadcast
-                const int pend = first_pending(flags, tiles, T.chain, J, lastFar, epoch);
-                const int jStop = pend >= lastFar ? jLow : tiles[pend].r1;
+                const int pend = first_pending_down(flags, J, bLow, epoch);
+                const int jStop = pend <= bLow ? jLow : min(r0 + 64 * (pend - T.blk) + 64, lim);
+#pragma unroll 1
                 for (int jb = jStop + 16 * w; jb < j1; jb += 64) {
                     int sjl = 0x7fffffff;
                     long long rol = 0;
@@ -1435,13 +1717,13 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                 }
                 j1 = jStop;
                 J = pend;
-                if (pend >= lastFar) break;
+                if (J <= bLow) break;
             }
             // tile J is not known to be done: this wave's 16 of its rows,
             // coupling entries loaded first, then wait for their values
-            const SweepTile Q = tiles[J];
-            const int jb = Q.r0 + 16 * w;
-            const int cnt = max(0, min(16, Q.r1 - jb));
+            const int q0 = r0 + 64 * (J - T.blk);
+            const int jb = q0 + 16 * w;
+            const int cnt = max(0, min(16, min(q0 + 64, lim) - jb));
             int sjl = 0x7fffffff;
             long long rol = 0;
             if (lane < cnt) {
@@ -1457,46 +1739,30 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                 const double v = lt[ro + cl];
                 lc[r] = ci >= sj ? v : 0.0;
             }
-            const int dist = (Q.r0 - r0) >> 6;
+            const int dist = J - T.blk;
             double xl;
             if (!poll_values(x, jb, cnt, backoff_for(dist), spinLimit, timeout, xl)) return;
 #pragma unroll
             for (int r = 0; r < 16; ++r) p += lc[r] * lane_bcast(xl, r);
-            j1 = Q.r0;
-            J = next_in_chain(tiles, T.chain, J, lastFar);
+            j1 = q0;
+            --J;
             front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
-    double xv = 0.0;
-    if (w == 0 && hasNext) xv = lane < nq ? coherent_load(&x[q0 + lane]) : 0.0;
+    NearVals n1{0, 0, 0.0}, n2{0, 0, 0.0}, n3{0, 0, 0.0};
+    if (nNear >= 1) n1 = near_load(g, r1 + 16 * w, max(0, min(16, lim - (r1 + 16 * w))));
+    if (nNear >= 2) n2 = near_load(x, r1 + 64 + 16 * w, max(0, min(16, lim - (r1 + 64 + 16 * w))));
+    if (nNear >= 3) {
+        n3 = near_load(x, jb3, cnt3);
+        if (!near_wait(x, n3, spinLimit, timeout)) return;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) p += lc3[r] * lane_bcast(n3.v, r);
+    }
     part[w][lane] = p;
     __syncthreads();
-    if (w == 0) {
-        const int i = lane;
-        double a = 0.0;
-        if (i < nr) {
-            const int gi = r0 + i;
-            a = z[gi] / lt[rowoff[gi] + gi] - (part[0][i] + part[1][i] + part[2][i] + part[3][i]);
-        }
-        if (hasNext) {
-            if (__ballot(i < nq && __double_as_longlong(xv) == ASTRO_SENTINEL) &&
-                !poll_values(x, q0, nq, 0, spinLimit, timeout, xv))
-                return;
-            double ln[64];
-#pragma unroll
-            for (int j = 0; j < 64; ++j) ln[j] = Ln[j][i];
-            a -= coupling_product(ln, xv);
-        }
-        // Ld[c][i] is zero for i >= c: no predicate on the dependent chain
-        double lcol[64];
-#pragma unroll
-        for (int c = 0; c < 64; ++c) lcol[c] = Ld[c][i];
-#pragma unroll
-        for (int c = 63; c >= 0; --c) a -= lcol[c] * lane_bcast(a, c);
-        if (i < nr) coherent_store(&x[r0 + i], a);
-        stores_complete();
-        if (i == 0) publish_flag(&flags[t], epoch);
-    }
+    const double h = lane < nr ? zd - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : 0.0;
+    lookahead_tail(r0, nr, nNear, T.blk, h, dv, kv, B1s[w], B3s[w], n3.v, 0, g, x, flags, epoch, spinLimit, timeout,
+                   n1, n2, acc, acc2);
 }
 
 __global__ void k_reset_ticket(int* ticket) { *ticket = 0; }
