@@ -1351,22 +1351,20 @@ __global__ __launch_bounds__(256) void k_sweep_blocks_pair(const SweepBlock* __r
 }
 
 // This wave's share of a block product: sum over 16 columns k of
-// m[k] v(lane lane0 + k), m[k] = M(lane, c(k)).  Broadcasts are taken in
-// groups of eight before their products, so the read-lane -> scalar-operand
-// waits overlap.
-__device__ __forceinline__ double block_product16(const double (&m)[16], double v, int lane0) {
+// m[k] v(lane lane0 + k), m[k] = M(lane, c(k)).  The 16 values go through
+// this wave's LDS slot vs and come back as broadcast reads (the wave's own
+// LDS accesses stay in order, so no barrier is needed).
+__device__ __forceinline__ double block_product16(const double (&m)[16], double v, int lane0, double* vs) {
+    const int lane = threadIdx.x & 63;
+    if (lane >= lane0 && lane < lane0 + 16) vs[lane - lane0] = v;
+    __builtin_amdgcn_wave_barrier();
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int k0 = 0; k0 < 16; k0 += 8) {
-        double vb[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) vb[k] = lane_bcast(v, lane0 + k0 + k);
-#pragma unroll
-        for (int k = 0; k < 8; k += 2) {
-            s0 += m[k0 + k] * vb[k];
-            s1 += m[k0 + k + 1] * vb[k + 1];
-        }
+    for (int k = 0; k < 16; k += 2) {
+        s0 += m[k] * vs[k];
+        s1 += m[k + 1] * vs[k + 1];
     }
+    __builtin_amdgcn_wave_barrier();
     return s0 + s1;
 }
 
@@ -1387,6 +1385,25 @@ __device__ __forceinline__ bool near_wait(const double* v, NearVals& n, long spi
     if (!__ballot(lane < n.cnt && __double_as_longlong(n.v) == ASTRO_SENTINEL)) return true;
     return poll_values(v, n.base, n.cnt, 0, spinLimit, timeout, n.v);
 }
+// Waits for a, re-polling b alongside it whenever b is still pending, so b's
+// arrival is seen while waiting for a.
+__device__ __forceinline__ bool near_wait_with(const double* va, NearVals& a, const double* vb, NearVals& b,
+                                               long spinLimit, int* timeout) {
+    const int lane = threadIdx.x & 63;
+    long it = 0;
+    while (true) {
+        const bool pa = __ballot(lane < a.cnt && __double_as_longlong(a.v) == ASTRO_SENTINEL) != 0ull;
+        if (!pa) return true;
+        const bool pb = __ballot(lane < b.cnt && __double_as_longlong(b.v) == ASTRO_SENTINEL) != 0ull;
+        if (++it > spinLimit) {
+            if (lane == 0) atomicExch(timeout, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        a.v = lane < a.cnt ? coherent_load(&va[a.base + lane]) : 0.0;
+        if (pb) b.v = lane < b.cnt ? coherent_load(&vb[b.base + lane]) : 0.0;
+    }
+}
 
 // The end of a sweep tile, after the far part (including the third near
 // tile) has been reduced to h (lane = row, zero beyond the tile): g_t = D h is
@@ -1401,9 +1418,9 @@ __device__ __forceinline__ bool lookahead_tail(int r0, int nr, int nNear, int fl
                                                const double (*B1s)[64], const double (*B3s)[64], double v3,
                                                int v3lane0, double* gout, double* out, int* flags, int epoch,
                                                long spinLimit, int* timeout, NearVals& n1, NearVals& n2,
-                                               double (*acc)[64], double (*acc2)[64]) {
+                                               double (*acc)[64], double (*acc2)[64], double* vs) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    acc[w][lane] = block_product16(dv, h, 16 * w);
+    acc[w][lane] = block_product16(dv, h, 16 * w, vs);
     __syncthreads();
     const double gt = (acc[0][lane] + acc[1][lane]) + (acc[2][lane] + acc[3][lane]);
     if (w == 0 && lane < nr) coherent_store(&gout[r0 + lane], gt);
@@ -1412,18 +1429,18 @@ __device__ __forceinline__ bool lookahead_tail(int r0, int nr, int nNear, int fl
         double m[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) m[k] = B3s[k][lane];
-        pw += block_product16(m, v3, v3lane0);
+        pw += block_product16(m, v3, v3lane0, vs);
     }
     if (nNear >= 1) {
-        if (!near_wait(gout, n1, spinLimit, timeout)) return false;
+        if (!near_wait_with(gout, n1, out, n2, spinLimit, timeout)) return false;
         double m[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) m[k] = B1s[k][lane];
-        pw -= block_product16(m, n1.v, 0);
+        pw -= block_product16(m, n1.v, 0, vs);
     }
     if (nNear >= 2) {
         if (!near_wait(out, n2, spinLimit, timeout)) return false;
-        pw += block_product16(kv, n2.v, 0);
+        pw += block_product16(kv, n2.v, 0, vs);
     }
     acc2[w][lane] = pw;
     __syncthreads();
@@ -1456,6 +1473,7 @@ __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __res
     __shared__ double acc2[4][64];
     __shared__ double B1s[4][16][64];  // wave w's 16 columns of F_t
     __shared__ double B3s[4][16][64];  // and of F_t H_{t-1}
+    __shared__ double vsh[4][16];      // per-wave broadcast slot
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
@@ -1604,7 +1622,7 @@ __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __res
     __syncthreads();
     const double h = lane < nr ? bl - part[lane >> 4][lane & 15] : 0.0;
     lookahead_tail(r0, nr, nNear, T.blk, h, dv, kv, B1s[w], B3s[w], n3.v, 16 * w, g, z, flags, epoch, spinLimit,
-                   timeout, n1, n2, acc, acc2);
+                   timeout, n1, n2, acc, acc2, vsh[w]);
 }
 
 // Backward sweep tile (columns [r0, r1)): phase 1 takes the rows of the far
@@ -1625,6 +1643,7 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     __shared__ double acc2[4][64];
     __shared__ double B1s[4][16][64];  // wave w's 16 columns of G_t
     __shared__ double B3s[4][16][64];  // and of G_t HB_{t+1}
+    __shared__ double vsh[4][16];      // per-wave broadcast slot
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) sT = atomicAdd(ticket, 1);
     __syncthreads();
@@ -1762,7 +1781,7 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     __syncthreads();
     const double h = lane < nr ? zd - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : 0.0;
     lookahead_tail(r0, nr, nNear, T.blk, h, dv, kv, B1s[w], B3s[w], n3.v, 0, g, x, flags, epoch, spinLimit, timeout,
-                   n1, n2, acc, acc2);
+                   n1, n2, acc, acc2, vsh[w]);
 }
 
 __global__ void k_reset_ticket(int* ticket) { *ticket = 0; }
