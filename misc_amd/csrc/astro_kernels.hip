@@ -820,41 +820,34 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     double4_t acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
-    // register double buffer: the next chunk's operands are fetched while the
-    // current one runs on the matrix cores.  Loads go to indices clamped into
-    // the row's stored range and are masked afterwards, so all of them are in
-    // flight together (no per-entry branch and memory wait).
-    double ra[8], rb[8], rd = 0.0;
-    auto fetch = [&](int k0) {
-        double va[8], vb[8];
+    // register double buffer: the next chunk's operands are loaded while the
+    // current one runs on the matrix cores; they are masked only when staged
+    // into LDS, so nothing waits on them before the MFMAs.  Loads go to
+    // indices clamped into each row's stored range (all in flight together,
+    // no per-entry branch and memory wait).
+    double va[8], vb[8], vd = 0.0;
+    auto issue = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int k = k0 + sk + q;
             va[q] = lt[aOff + min(max(k, aS0), aR)];
             vb[q] = lt[bOff + min(max(k, bS0), bR)];
         }
-        const int kd = k0 + (tid & (KC - 1));
-        const double dv = diag[min(kd, ke - 1)];
+        vd = diag[min(k0 + (tid & (KC - 1)), ke - 1)];
+    };
+    if (kb < ke) issue(kb);
+    for (int k0 = kb; k0 < ke; k0 += KC) {
+        __syncthreads();
+        if (tid < KC) Dk[tid] = k0 + tid < ke ? vd : 0.0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int k = k0 + sk + q;
             const bool kv = k < ke;
-            ra[q] = (kv && k >= aS) ? va[q] : 0.0;
-            rb[q] = (kv && k >= bS) ? vb[q] : 0.0;
-        }
-        rd = kd < ke ? dv : 0.0;
-    };
-    if (kb < ke) fetch(kb);
-    for (int k0 = kb; k0 < ke; k0 += KC) {
-        __syncthreads();
-        if (tid < KC) Dk[tid] = rd;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            As[sk + q][srow] = ra[q];
-            Bs[sk + q][srow] = rb[q];
+            As[sk + q][srow] = (kv && k >= aS) ? va[q] : 0.0;
+            Bs[sk + q][srow] = (kv && k >= bS) ? vb[q] : 0.0;
         }
         __syncthreads();
-        if (k0 + KC < ke) fetch(k0 + KC);
+        if (k0 + KC < ke) issue(k0 + KC);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
