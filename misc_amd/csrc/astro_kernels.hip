@@ -758,6 +758,37 @@ __global__ void k_penalty_to_profile(const double* __restrict__ cmat, const int*
     }
 }
 
+// gfx950 cross-lane moves on doubles.  swap_halves32(a, c): lanes 32..63 of
+// a take lanes 0..31 of c and lanes 0..31 of c take lanes 32..63 of a, so
+// a + c is, in lanes 0..31, a summed with its l ^ 32 partner and, in lanes
+// 32..63, c summed with its partner.  swap_rows16 does the same between the
+// odd 16-lane rows of a and the even rows of c (partner l ^ 16).
+__device__ __forceinline__ void swap_halves32(double& a, double& c) {
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(c);
+    const auto lo = __builtin_amdgcn_permlane32_swap((int)x, (int)y, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((int)(x >> 32), (int)(y >> 32), false, false);
+    a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+    c = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+__device__ __forceinline__ void swap_rows16(double& a, double& c) {
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(c);
+    const auto lo = __builtin_amdgcn_permlane16_swap((int)x, (int)y, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((int)(x >> 32), (int)(y >> 32), false, false);
+    a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+    c = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+// DPP controls: row_shl:n reads lane l + n, row_shr:n lane l - n (within a
+// 16-lane row); quad_perm 2,3,0,1 and 1,0,3,2 read lanes l ^ 2 and l ^ 1.
+enum { DPP_ROW_SHL8 = 0x108, DPP_ROW_SHR8 = 0x118, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR4 = 0x114,
+       DPP_QUAD_XOR2 = 0x4E, DPP_QUAD_XOR1 = 0xB1 };
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // ---------------------------------------------------------------------------
 // Blocked left-looking LDL^T on the row-skyline profile.
 //
@@ -778,7 +809,12 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // k is staged through LDS in chunks of 32.  Operand lane layouts (measured on
 // gfx950): A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k, D(i, j) in
 // lane j + 16 (i % 4), register i / 4.
-constexpr int LDL_SMEM_BYTES = 64 * 65 * 8 + 2 * 64 * 8;  // the larger of the two step bodies
+constexpr int LDL_U_SMEM_BYTES = 64 * 65 * 8 + 2 * 64 * 8;  // step U alone
+// step FT: two 64 x 65 blocks (factored diagonal block, then the tile rows;
+// its inverse), the column double buffer, 1/D and the tile rows' offsets
+constexpr int LDL_FT_A = 0, LDL_FT_X = 64 * 65 * 8, LDL_FT_COL = 2 * 64 * 65 * 8,
+              LDL_FT_RD = LDL_FT_COL + 2 * 64 * 8, LDL_FT_OFF = LDL_FT_RD + 64 * 8,
+              LDL_FT_SI = LDL_FT_OFF + 64 * 8, LDL_STEP_SMEM_BYTES = LDL_FT_SI + 64 * 4;
 
 __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
                                                 int kMode, int slot, const double* __restrict__ lt,
@@ -880,7 +916,7 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
                                                          const int* __restrict__ sfirst,
                                                          const double* __restrict__ diag,
                                                          double* __restrict__ scratch, int scratchRows) {
-    __shared__ __align__(16) char smem[LDL_SMEM_BYTES];
+    __shared__ __align__(16) char smem[LDL_U_SMEM_BYTES];
     const int blk = blockIdx.y;
     const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
     if (P.p < 0) return;
@@ -888,24 +924,6 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
     if (P.p + 64 * tile >= P.iend) return;
     ldl_update_body(P, blk, gridDim.y, tile, sp, nSplit, kMode, slot0 + sp, lt, rowoff, sfirst, diag, scratch,
                     scratchRows, smem);
-}
-
-// Broadcast lane S of each group of four lanes (DPP quad permute, no LDS).
-template <int S>
-__device__ __forceinline__ double quad_bcast(double v) {
-    constexpr int ctrl = S | (S << 2) | (S << 4) | (S << 6);
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double quad_bcast_rt(double v, int s) {
-    switch (s & 3) {
-        case 0: return quad_bcast<0>(v);
-        case 1: return quad_bcast<1>(v);
-        case 2: return quad_bcast<2>(v);
-        default: return quad_bcast<3>(v);
-    }
 }
 
 // Step FT.  Thread (r = tid / 4, q = tid % 4) holds 16 entries of one row in
@@ -930,8 +948,12 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
     if (r0 >= P.iend) return;
     const int nr = min(64, P.iend - r0);
     const int np = P.pe - P.p;
-    double(*Lc)[65] = reinterpret_cast<double(*)[65]>(smem);  // factored diagonal block: L below, D on the diagonal
-    double(*colb)[64] = reinterpret_cast<double(*)[64]>(smem + 64 * 65 * 8);
+    double(*Lc)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_A);  // factored diagonal block: L below, D on the diagonal
+    double(*Xi)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_X);  // L_PP^-1 (unit lower)
+    double(*colb)[64] = reinterpret_cast<double(*)[64]>(smem + LDL_FT_COL);
+    double* rD = reinterpret_cast<double*>(smem + LDL_FT_RD);
+    int64_t* rowOff = reinterpret_cast<int64_t*>(smem + LDL_FT_OFF);
+    int* rowS = reinterpret_cast<int*>(smem + LDL_FT_SI);
     const int tid = threadIdx.x, fr = tid >> 2, fq = tid & 3;
     const size_t qstride = (size_t)nb * scratchRows * 64;
 
@@ -1001,35 +1023,70 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) Lc[fr][fq + 4 * m] = a[m];
-    __syncthreads();
 
-    // --- 3: rows r0 + fr of this tile
+    // --- 3: rows r0 + fr of this tile, gathered now so that their loads
+    // overlap the inverse below
     const int gi = r0 + fr;
     const bool live = fr < nr;
     double y[16];
     int si;
     int64_t off;
     gather(gi, r0 - P.p + fr, live, np, y, si, off);
+    if (fq == 0) {
+        rowOff[fr] = off;
+        rowS[fr] = live ? si : 0x7fffffff;
+    }
+    __syncthreads();
+    if (tid < 64) rD[tid] = tid < np ? 1.0 / Lc[tid][tid] : 0.0;
+
+    // --- 4: X = L_PP^-1 (unit lower; rows past np are the identity), column
+    // c by the quad of threads 4c..4c+3, which split the sum over k and
+    // combine it with DPP; X(r, c) = -sum_{c <= k < r} L(r, k) X(k, c)
+    {
+        const int c = tid >> 2, part = tid & 3;
+        if (part == 0) Xi[c][c] = 1.0;
+        for (int r = c + 1; r < 64; ++r) {
+            double sum = 0.0;
+            for (int k = c + part; k < r; k += 4) sum += Lc[r][k] * Xi[k][c];
+            sum += dpp_f64<DPP_QUAD_XOR1>(sum);
+            sum += dpp_f64<DPP_QUAD_XOR2>(sum);
+            if (part == 0) Xi[r][c] = -sum;
+            __builtin_amdgcn_wave_barrier();
+        }
+        for (int r = part; r < c; r += 4) Xi[r][c] = 0.0;
+    }
+    __syncthreads();
+
+    // --- 5: the tile rows into LDS (replacing the diagonal block), then
+    // L(i, P) = Y(i, :) D^-1 with Y = rows X^T on the matrix cores
 #pragma unroll
-    for (int j = 0; j < 64; ++j) {
-        if (j < np) {
-            const double yj = quad_bcast_rt(y[j >> 2], j);
+    for (int m = 0; m < 16; ++m) Lc[fr][fq + 4 * m] = y[m];
+    __syncthreads();
+    const int lane = tid & 63, w = tid >> 6;
+    double4_t acc[4];
 #pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                if (4 * m + 3 > j) {
-                    const int k = 4 * m + fq;
-                    const double lkj = Lc[k][j];
-                    const double upd = y[m] - lkj * yj;
-                    y[m] = k > j ? upd : y[m];
-                }
-            }
+    for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 64; ks += 4) {
+        const int kk = ks + (lane >> 4);
+        const double av = Lc[w * 16 + (lane & 15)][kk];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const double bv = Xi[cb * 16 + (lane & 15)][kk];
+            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
         }
     }
-    if (live) {
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const int k = fq + 4 * m;
-            if (k < np && P.p + k >= si) lt[off + P.p + k] = y[m] / Lc[k][k];
+    for (int v = 0; v < 4; ++v) {
+        const int i = w * 16 + 4 * v + (lane >> 4);
+        if (i < nr) {
+            const int64_t ro = rowOff[i];
+            const int s0 = rowS[i];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const int j = cb * 16 + (lane & 15);
+                if (j < np && P.p + j >= s0) lt[ro + P.p + j] = acc[cb][v] * rD[j];
+            }
         }
     }
 }
@@ -1044,7 +1101,7 @@ __global__ __launch_bounds__(256) void k_ldl_step(const PanelInfo* __restrict__ 
                                                   const int* __restrict__ sfirst, double* __restrict__ diag,
                                                   const double* __restrict__ scratchFT, double* __restrict__ scratchU,
                                                   int scratchRows, int* __restrict__ zeroPivot) {
-    __shared__ __align__(16) char smem[LDL_SMEM_BYTES];
+    __shared__ __align__(16) char smem[LDL_STEP_SMEM_BYTES];
     // 1-D grid, FT workgroups of every block first: they carry the critical
     // path and are dispatched before any U work
     const int x = blockIdx.x, nFT = numBlocks * numTiles;
@@ -1082,37 +1139,6 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// gfx950 cross-lane moves on doubles.  swap_halves32(a, c): lanes 32..63 of
-// a take lanes 0..31 of c and lanes 0..31 of c take lanes 32..63 of a, so
-// a + c is, in lanes 0..31, a summed with its l ^ 32 partner and, in lanes
-// 32..63, c summed with its partner.  swap_rows16 does the same between the
-// odd 16-lane rows of a and the even rows of c (partner l ^ 16).
-__device__ __forceinline__ void swap_halves32(double& a, double& c) {
-    const long long x = __double_as_longlong(a), y = __double_as_longlong(c);
-    const auto lo = __builtin_amdgcn_permlane32_swap((int)x, (int)y, false, false);
-    const auto hi = __builtin_amdgcn_permlane32_swap((int)(x >> 32), (int)(y >> 32), false, false);
-    a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
-    c = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
-}
-__device__ __forceinline__ void swap_rows16(double& a, double& c) {
-    const long long x = __double_as_longlong(a), y = __double_as_longlong(c);
-    const auto lo = __builtin_amdgcn_permlane16_swap((int)x, (int)y, false, false);
-    const auto hi = __builtin_amdgcn_permlane16_swap((int)(x >> 32), (int)(y >> 32), false, false);
-    a = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
-    c = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
-}
-// DPP controls: row_shl:n reads lane l + n, row_shr:n lane l - n (within a
-// 16-lane row); quad_perm 2,3,0,1 and 1,0,3,2 read lanes l ^ 2 and l ^ 1.
-enum { DPP_ROW_SHL8 = 0x108, DPP_ROW_SHR8 = 0x118, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR4 = 0x114,
-       DPP_QUAD_XOR2 = 0x4E, DPP_QUAD_XOR1 = 0xB1 };
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
