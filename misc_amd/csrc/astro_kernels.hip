@@ -809,7 +809,8 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // k is staged through LDS in chunks of 32.  Operand lane layouts (measured on
 // gfx950): A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k, D(i, j) in
 // lane j + 16 (i % 4), register i / 4.
-constexpr int LDL_U_SMEM_BYTES = 64 * 65 * 8 + 2 * 64 * 8;  // step U alone
+constexpr int LDL_KC = 32;  // k per staged chunk of step U
+constexpr int LDL_U_SMEM_BYTES = 2 * LDL_KC * 65 * 8 + LDL_KC * 8 + 16;  // step U alone
 // step FT: two 64 x 65 blocks (factored diagonal block, then the tile rows;
 // its inverse), the column double buffer, 1/D and the tile rows' offsets
 constexpr int LDL_FT_A = 0, LDL_FT_X = 64 * 65 * 8, LDL_FT_COL = 2 * 64 * 65 * 8,
@@ -825,7 +826,7 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     if (r0 >= P.iend) return;
     const int nr = min(64, P.iend - r0);
     const int np = P.pe - P.p;
-    constexpr int KC = 32;
+    constexpr int KC = LDL_KC;
     double(*As)[65] = reinterpret_cast<double(*)[65]>(smem);
     double(*Bs)[65] = reinterpret_cast<double(*)[65]>(smem + KC * 65 * 8);
     double* Dk = reinterpret_cast<double*>(smem + 2 * KC * 65 * 8);
@@ -844,15 +845,27 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     int chunk = (total + nSplit - 1) / nSplit;
     chunk = (chunk + KC - 1) / KC * KC;
     const int kb = kLo + sp * chunk, ke = min(kHi, kb + chunk);
-    // per-thread staging rows (fixed across chunks)
-    const int srow = tid >> 2, sk = (tid & 3) * 8;  // 64 rows x 4 groups of 8 k
-    const bool aLive = srow < nr, bLive = srow < np;
-    const int aRow = r0 + srow, bRow = P.p + srow;
-    // dead staging rows read a valid row (r0, P.p) and are masked to zero
-    const int aR = aLive ? aRow : r0, bR = bLive ? bRow : P.p;
-    const int aS0 = sfirst[aR], bS0 = sfirst[bR];
-    const int64_t aOff = rowoff[aR], bOff = rowoff[bR];
-    const int aS = aLive ? aS0 : 0x7fffffff, bS = bLive ? bS0 : 0x7fffffff;
+    // staging: wave w loads rows 16 w .. 16 w + 15 of both operands, two
+    // rows per instruction with lane l on k = k0 + (l & 31) of row
+    // 16 w + 2 i + (l >> 5), so every load covers 2 x 256 contiguous bytes
+    static_assert(KC == 32, "the staging layout assumes 32 k per chunk");
+    const int kl = lane & 31;
+    int aR[8], aS0[8], aS[8], bR[8], bS0[8], bS[8];
+    int64_t aOff[8], bOff[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int sr = 16 * w + 2 * i + (lane >> 5);
+        const bool aLive = sr < nr, bLive = sr < np;
+        // dead staging rows read a valid row (r0, P.p) and are masked to zero
+        aR[i] = aLive ? r0 + sr : r0;
+        bR[i] = bLive ? P.p + sr : P.p;
+        aS0[i] = sfirst[aR[i]];
+        bS0[i] = sfirst[bR[i]];
+        aOff[i] = rowoff[aR[i]];
+        bOff[i] = rowoff[bR[i]];
+        aS[i] = aLive ? aS0[i] : 0x7fffffff;
+        bS[i] = bLive ? bS0[i] : 0x7fffffff;
+    }
     double4_t acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
@@ -863,11 +876,11 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     // no per-entry branch and memory wait).
     double va[8], vb[8], vd = 0.0;
     auto issue = [&](int k0) {
+        const int k = k0 + kl;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int k = k0 + sk + q;
-            va[q] = lt[aOff + min(max(k, aS0), aR)];
-            vb[q] = lt[bOff + min(max(k, bS0), bR)];
+        for (int i = 0; i < 8; ++i) {
+            va[i] = lt[aOff[i] + min(max(k, aS0[i]), aR[i])];
+            vb[i] = lt[bOff[i] + min(max(k, bS0[i]), bR[i])];
         }
         vd = diag[min(k0 + (tid & (KC - 1)), ke - 1)];
     };
@@ -875,12 +888,15 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     for (int k0 = kb; k0 < ke; k0 += KC) {
         __syncthreads();
         if (tid < KC) Dk[tid] = k0 + tid < ke ? vd : 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int k = k0 + sk + q;
+        {
+            const int k = k0 + kl;
             const bool kv = k < ke;
-            As[sk + q][srow] = (kv && k >= aS) ? va[q] : 0.0;
-            Bs[sk + q][srow] = (kv && k >= bS) ? vb[q] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int sr = 16 * w + 2 * i + (lane >> 5);
+                As[kl][sr] = (kv && k >= aS[i]) ? va[i] : 0.0;
+                Bs[kl][sr] = (kv && k >= bS[i]) ? vb[i] : 0.0;
+            }
         }
         __syncthreads();
         if (k0 + KC < ke) issue(k0 + KC);
