@@ -812,9 +812,11 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr int LDL_KC = 32;  // k per staged chunk of step U
 constexpr int LDL_U_SMEM_BYTES = 2 * LDL_KC * 65 * 8 + LDL_KC * 8 + 16;  // step U alone
 // step FT: two 64 x 65 blocks (factored diagonal block, then the tile rows;
-// its inverse), the column double buffer, 1/D and the tile rows' offsets
+// its inverse), the column double buffer, per-wave 16 x 16 products, 1/D and
+// the tile rows' offsets
 constexpr int LDL_FT_A = 0, LDL_FT_X = 64 * 65 * 8, LDL_FT_COL = 2 * 64 * 65 * 8,
-              LDL_FT_RD = LDL_FT_COL + 2 * 64 * 8, LDL_FT_OFF = LDL_FT_RD + 64 * 8,
+              LDL_FT_T = LDL_FT_COL + 2 * 64 * 8, LDL_FT_RD = LDL_FT_T + 4 * 16 * 17 * 8,
+              LDL_FT_OFF = LDL_FT_RD + 64 * 8,
               LDL_FT_SI = LDL_FT_OFF + 64 * 8, LDL_STEP_SMEM_BYTES = LDL_FT_SI + 64 * 4;
 
 __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
@@ -1039,9 +1041,69 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) Lc[fr][fq + 4 * m] = a[m];
+    __syncthreads();
+    const int lane = tid & 63, w = tid >> 6;
 
-    // --- 3: rows r0 + fr of this tile, gathered now so that their loads
-    // overlap the inverse below
+    // --- 3: X = L_PP^-1 (unit lower; rows past np are the identity) by 16 x 16
+    // blocks: the four diagonal blocks by forward substitution (thread =
+    // column), then block rows 1..3 as X_ij = -X_ii sum_{j <= k < i} L_ik X_kj
+    // on the matrix cores.  The strictly upper blocks are zero.
+    if (tid < 64) {
+        const int bb = tid >> 4, c = tid & 15, o = 16 * bb;
+        double xv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            double acc2 = r == c ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < r; ++k) acc2 -= Lc[o + r][o + k] * xv[k];
+            xv[r] = r < c ? 0.0 : acc2;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Xi[o + r][o + c] = xv[r];
+        rD[tid] = tid < np ? 1.0 / Lc[tid][tid] : 0.0;
+    }
+    for (int id = tid; id < 6 * 256; id += 256) {
+        // upper blocks (bi, bj), bi < bj: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
+        const int q = id >> 8, e = id & 255;
+        const int bi = q < 3 ? 0 : (q < 5 ? 1 : 2), bj = q < 3 ? q + 1 : (q < 5 ? q - 1 : 3);
+        Xi[16 * bi + (e >> 4)][16 * bj + (e & 15)] = 0.0;
+    }
+    __syncthreads();
+    {
+        double(*Tw)[17] = reinterpret_cast<double(*)[17]>(smem + LDL_FT_T + w * 16 * 17 * 8);
+        for (int bi = 1; bi < 4; ++bi) {
+            const int bj = w;  // wave w computes block (bi, w) for w < bi
+            if (bj < bi) {
+                double4_t t = double4_t{0.0, 0.0, 0.0, 0.0};
+                for (int bk = bj; bk < bi; ++bk) {
+#pragma unroll
+                    for (int ks = 0; ks < 16; ks += 4) {
+                        const int kk = ks + (lane >> 4);
+                        const double av = Lc[16 * bi + (lane & 15)][16 * bk + kk];
+                        const double bv = Xi[16 * bk + kk][16 * bj + (lane & 15)];
+                        t = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, t, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Tw[4 * v + (lane >> 4)][lane & 15] = t[v];
+                __builtin_amdgcn_wave_barrier();
+                double4_t xb = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < 16; ks += 4) {
+                    const int kk = ks + (lane >> 4);
+                    const double av = Xi[16 * bi + (lane & 15)][16 * bi + kk];
+                    const double bv = Tw[kk][lane & 15];
+                    xb = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, xb, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Xi[16 * bi + 4 * v + (lane >> 4)][16 * bj + (lane & 15)] = -xb[v];
+            }
+            __syncthreads();
+        }
+    }
+
+    // --- 4: rows r0 + fr of this tile into LDS (replacing the diagonal
+    // block), then L(i, P) = Y(i, :) D^-1 with Y = rows X^T on the matrix cores
     const int gi = r0 + fr;
     const bool live = fr < nr;
     double y[16];
@@ -1052,33 +1114,9 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
         rowOff[fr] = off;
         rowS[fr] = live ? si : 0x7fffffff;
     }
-    __syncthreads();
-    if (tid < 64) rD[tid] = tid < np ? 1.0 / Lc[tid][tid] : 0.0;
-
-    // --- 4: X = L_PP^-1 (unit lower; rows past np are the identity), column
-    // c by the quad of threads 4c..4c+3, which split the sum over k and
-    // combine it with DPP; X(r, c) = -sum_{c <= k < r} L(r, k) X(k, c)
-    {
-        const int c = tid >> 2, part = tid & 3;
-        if (part == 0) Xi[c][c] = 1.0;
-        for (int r = c + 1; r < 64; ++r) {
-            double sum = 0.0;
-            for (int k = c + part; k < r; k += 4) sum += Lc[r][k] * Xi[k][c];
-            sum += dpp_f64<DPP_QUAD_XOR1>(sum);
-            sum += dpp_f64<DPP_QUAD_XOR2>(sum);
-            if (part == 0) Xi[r][c] = -sum;
-            __builtin_amdgcn_wave_barrier();
-        }
-        for (int r = part; r < c; r += 4) Xi[r][c] = 0.0;
-    }
-    __syncthreads();
-
-    // --- 5: the tile rows into LDS (replacing the diagonal block), then
-    // L(i, P) = Y(i, :) D^-1 with Y = rows X^T on the matrix cores
 #pragma unroll
     for (int m = 0; m < 16; ++m) Lc[fr][fq + 4 * m] = y[m];
     __syncthreads();
-    const int lane = tid & 63, w = tid >> 6;
     double4_t acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
