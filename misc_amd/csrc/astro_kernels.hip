@@ -876,8 +876,9 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     // into LDS, so nothing waits on them before the MFMAs.  Loads go to
     // indices clamped into each row's stored range (all in flight together,
     // no per-entry branch and memory wait).
-    double va[8], vb[8], vd = 0.0;
-    auto issue = [&](int k0) {
+    // two register sets, so the loads of the next two chunks are in flight
+    double va0[8], vb0[8], vd0 = 0.0, va1[8], vb1[8], vd1 = 0.0;
+    auto issue = [&](int k0, double (&va)[8], double (&vb)[8], double& vd) {
         const int k = k0 + kl;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -886,8 +887,9 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
         }
         vd = diag[min(k0 + (tid & (KC - 1)), ke - 1)];
     };
-    if (kb < ke) issue(kb);
-    for (int k0 = kb; k0 < ke; k0 += KC) {
+    // stage chunk k0 from its register set into LDS, then, once every wave
+    // has staged, refill that set with chunk k0 + 2 KC and multiply
+    auto step = [&](int k0, double (&va)[8], double (&vb)[8], double& vd) {
         __syncthreads();
         if (tid < KC) Dk[tid] = k0 + tid < ke ? vd : 0.0;
         {
@@ -901,18 +903,24 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
             }
         }
         __syncthreads();
-        if (k0 + KC < ke) issue(k0 + KC);
+        if (k0 + 2 * KC < ke) issue(k0 + 2 * KC, va, vb, vd);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
-            const double a = As[kk][w * 16 + (lane & 15)];
+            const double av = As[kk][w * 16 + (lane & 15)];
             const double d = Dk[kk];
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
                 const double bv = d * Bs[kk][cb * 16 + (lane & 15)];
-                acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[cb], 0, 0, 0);
+                acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
             }
         }
+    };
+    if (kb < ke) issue(kb, va0, vb0, vd0);
+    if (kb + KC < ke) issue(kb + KC, va1, vb1, vd1);
+    for (int k0 = kb; k0 < ke; k0 += 2 * KC) {
+        step(k0, va0, vb0, vd0);
+        if (k0 + KC < ke) step(k0 + KC, va1, vb1, vd1);
     }
     double* out = scratch + (((size_t)slot * nb + blk) * scratchRows + (r0 - P.p)) * 64;
 #pragma unroll
