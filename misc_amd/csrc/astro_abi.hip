@@ -87,6 +87,7 @@ struct DeviceProfile {
     DBuf<double> lt;
     DBuf<int64_t> rowoff;
     DBuf<int> sfirst;
+    DBuf<int> winMin;  // min first column over rows [r, r + 64) of r's diagonal block
     // factor
     int nBlocks = 0, panelsPerBlock = 0, splitK = 1, scratchRows = 64, maxTiles = 1;
     DBuf<PanelInfo> panels;
@@ -165,6 +166,19 @@ struct DeviceProfile {
                 bounds.push_back(b);
         bounds.push_back(n);
         nBlocks = (int)bounds.size() - 1;
+        {
+            // window minimum of the first columns over the next 64 rows of the block
+            std::vector<int> wm(n);
+            for (int k = 0; k < nBlocks; ++k) {
+                const int b0 = bounds[k], b1 = bounds[k + 1];
+                for (int r = b0; r < b1; ++r) {
+                    int m = INT32_MAX;
+                    for (int q = r; q < std::min(r + 64, b1); ++q) m = std::min(m, hS[q]);
+                    wm[r] = m;
+                }
+            }
+            ASTRO_TRY(winMin.upload(wm, s));
+        }
         // panels
         panelsPerBlock = 0;
         for (int k = 0; k < nBlocks; ++k)
@@ -300,15 +314,15 @@ struct DeviceProfile {
         auto buf = [&](int q) { return scratch.p + (size_t)(q & 1) * bufSize; };
         const dim3 gOld(maxTiles * splitK, nBlocks), gNew(maxTiles, nBlocks), gStep(maxTiles * (splitK + 1) * nBlocks);
         hipLaunchKernelGGL(k_ldl_update_mfma, gOld, dim3(256), 0, s, panels.p, 0, panelsPerBlock, splitK, LDL_K_OLD, 0,
-                           lt.p, rowoff.p, sfirst.p, diag.p, buf(0), scratchRows);
+                           lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(0), scratchRows);
         hipLaunchKernelGGL(k_ldl_update_mfma, gNew, dim3(256), 0, s, panels.p, 0, panelsPerBlock, 1, LDL_K_NEW,
-                           splitK, lt.p, rowoff.p, sfirst.p, diag.p, buf(0), scratchRows);
+                           splitK, lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(0), scratchRows);
         for (int q = 0; q < panelsPerBlock; ++q) {
             hipLaunchKernelGGL(k_ldl_step, gStep, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles, splitK, lt.p,
-                               rowoff.p, sfirst.p, diag.p, buf(q), buf(q + 1), scratchRows, zeroPivot.p);
+                               rowoff.p, sfirst.p, winMin.p, diag.p, buf(q), buf(q + 1), scratchRows, zeroPivot.p);
             if (q + 1 < panelsPerBlock)
                 hipLaunchKernelGGL(k_ldl_update_mfma, gNew, dim3(256), 0, s, panels.p, q + 1, panelsPerBlock, 1,
-                                   LDL_K_NEW, splitK, lt.p, rowoff.p, sfirst.p, diag.p, buf(q + 1), scratchRows);
+                                   LDL_K_NEW, splitK, lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(q + 1), scratchRows);
         }
         HIP_TRY(hipGetLastError());
         ASTRO_TRY(prepare_blocks(s));

@@ -77,10 +77,10 @@ enum { LDL_K_ALL = 0, LDL_K_OLD = 1, LDL_K_NEW = 2 };
 // step (FT of panelIdx, "old" U of panelIdx + 1).
 __global__ void k_ldl_update_mfma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int nSplit, int kMode,
                                   int slot0, const double* lt, const int64_t* rowoff, const int* sfirst,
-                                  const double* diag, double* scratch, int scratchRows);
+                                  const int* winMin, const double* diag, double* scratch, int scratchRows);
 __global__ void k_ldl_step(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
-                           double* lt, const int64_t* rowoff, const int* sfirst, double* diag, const double* scratchFT,
-                           double* scratchU, int scratchRows, int* zeroPivot);
+                           double* lt, const int64_t* rowoff, const int* sfirst, const int* winMin, double* diag,
+                           const double* scratchFT, double* scratchU, int scratchRows, int* zeroPivot);
 __global__ void k_max_abs(const double* v, int64_t n, unsigned long long* out);
 __global__ void k_scale_mpc(double* cmat, double* rhs, const int* crange, int cdim, const unsigned long long* maxBits);
 __global__ void k_fill(double* p, int64_t n, double v);

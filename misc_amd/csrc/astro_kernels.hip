@@ -810,7 +810,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // gfx950): A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k, D(i, j) in
 // lane j + 16 (i % 4), register i / 4.
 constexpr int LDL_KC = 32;  // k per staged chunk of step U
-constexpr int LDL_U_SMEM_BYTES = 2 * LDL_KC * 65 * 8 + LDL_KC * 8 + 16;  // step U alone
+constexpr int LDL_U_SMEM_BYTES = 2 * LDL_KC * 65 * 8;  // step U alone
 // step FT: two 64 x 65 blocks (factored diagonal block, then the tile rows;
 // its inverse), the column double buffer, per-wave 16 x 16 products, 1/D and
 // the tile rows' offsets
@@ -822,6 +822,7 @@ constexpr int LDL_FT_A = 0, LDL_FT_X = 64 * 65 * 8, LDL_FT_COL = 2 * 64 * 65 * 8
 __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
                                                 int kMode, int slot, const double* __restrict__ lt,
                                                 const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
+                                                const int* __restrict__ winMin,
                                                 const double* __restrict__ diag, double* __restrict__ scratch,
                                                 int scratchRows, char* smem) {
     const int r0 = P.p + 64 * tile;
@@ -831,16 +832,8 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     constexpr int KC = LDL_KC;
     double(*As)[65] = reinterpret_cast<double(*)[65]>(smem);
     double(*Bs)[65] = reinterpret_cast<double(*)[65]>(smem + KC * 65 * 8);
-    double* Dk = reinterpret_cast<double*>(smem + 2 * KC * 65 * 8);
-    int& sMin = *reinterpret_cast<int*>(smem + 2 * KC * 65 * 8 + KC * 8);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid < 64) {
-        int v = tid < nr ? sfirst[r0 + tid] : 0x7fffffff;
-        for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-        if (tid == 0) sMin = v;
-    }
-    __syncthreads();
-    int kLo = max(sMin, P.smin), kHi = P.p;
+    int kLo = max(winMin[r0], P.smin), kHi = P.p;
     if (kMode == LDL_K_OLD) kHi = P.p - 64;
     if (kMode == LDL_K_NEW) kLo = max(kLo, P.p - 64);
     const int total = max(0, kHi - kLo);
@@ -885,21 +878,21 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
             va[i] = lt[aOff[i] + min(max(k, aS0[i]), aR[i])];
             vb[i] = lt[bOff[i] + min(max(k, bS0[i]), bR[i])];
         }
-        vd = diag[min(k0 + (tid & (KC - 1)), ke - 1)];
+        vd = diag[min(k0 + kl, ke - 1)];
     };
     // stage chunk k0 from its register set into LDS, then, once every wave
     // has staged, refill that set with chunk k0 + 2 KC and multiply
     auto step = [&](int k0, double (&va)[8], double (&vb)[8], double& vd) {
         __syncthreads();
-        if (tid < KC) Dk[tid] = k0 + tid < ke ? vd : 0.0;
         {
+            // the panel rows are staged as L(P, k) D_k (vd = D of this lane's k)
             const int k = k0 + kl;
             const bool kv = k < ke;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int sr = 16 * w + 2 * i + (lane >> 5);
                 As[kl][sr] = (kv && k >= aS[i]) ? va[i] : 0.0;
-                Bs[kl][sr] = (kv && k >= bS[i]) ? vb[i] : 0.0;
+                Bs[kl][sr] = (kv && k >= bS[i]) ? vb[i] * vd : 0.0;
             }
         }
         __syncthreads();
@@ -908,10 +901,9 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
             const double av = As[kk][w * 16 + (lane & 15)];
-            const double d = Dk[kk];
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
-                const double bv = d * Bs[kk][cb * 16 + (lane & 15)];
+                const double bv = Bs[kk][cb * 16 + (lane & 15)];
                 acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
             }
         }
@@ -940,6 +932,7 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
                                                          const double* __restrict__ lt,
                                                          const int64_t* __restrict__ rowoff,
                                                          const int* __restrict__ sfirst,
+                                                         const int* __restrict__ winMin,
                                                          const double* __restrict__ diag,
                                                          double* __restrict__ scratch, int scratchRows) {
     __shared__ __align__(16) char smem[LDL_U_SMEM_BYTES];
@@ -948,8 +941,8 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
     if (P.p < 0) return;
     const int tile = blockIdx.x / nSplit, sp = blockIdx.x % nSplit;
     if (P.p + 64 * tile >= P.iend) return;
-    ldl_update_body(P, blk, gridDim.y, tile, sp, nSplit, kMode, slot0 + sp, lt, rowoff, sfirst, diag, scratch,
-                    scratchRows, smem);
+    ldl_update_body(P, blk, gridDim.y, tile, sp, nSplit, kMode, slot0 + sp, lt, rowoff, sfirst, winMin, diag,
+                    scratch, scratchRows, smem);
 }
 
 // Step FT.  Thread (r = tid / 4, q = tid % 4) holds 16 entries of one row in
@@ -1160,9 +1153,10 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
 __global__ __launch_bounds__(256) void k_ldl_step(const PanelInfo* __restrict__ panels, int panelIdx,
                                                   int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
                                                   double* __restrict__ lt, const int64_t* __restrict__ rowoff,
-                                                  const int* __restrict__ sfirst, double* __restrict__ diag,
-                                                  const double* __restrict__ scratchFT, double* __restrict__ scratchU,
-                                                  int scratchRows, int* __restrict__ zeroPivot) {
+                                                  const int* __restrict__ sfirst, const int* __restrict__ winMin,
+                                                  double* __restrict__ diag, const double* __restrict__ scratchFT,
+                                                  double* __restrict__ scratchU, int scratchRows,
+                                                  int* __restrict__ zeroPivot) {
     __shared__ __align__(16) char smem[LDL_STEP_SMEM_BYTES];
     // 1-D grid, FT workgroups of every block first: they carry the critical
     // path and are dispatched before any U work
@@ -1180,8 +1174,8 @@ __global__ __launch_bounds__(256) void k_ldl_step(const PanelInfo* __restrict__ 
         const int blk = id / per, tile = (id % per) / nSplit, sp = id % nSplit;
         const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx + 1];
         if (P.p < 0 || P.p + 64 * tile >= P.iend) return;
-        ldl_update_body(P, blk, numBlocks, tile, sp, nSplit, LDL_K_OLD, sp, lt, rowoff, sfirst, diag, scratchU,
-                        scratchRows, smem);
+        ldl_update_body(P, blk, numBlocks, tile, sp, nSplit, LDL_K_OLD, sp, lt, rowoff, sfirst, winMin, diag,
+                        scratchU, scratchRows, smem);
     }
 }
 
