@@ -815,7 +815,7 @@ constexpr int LDL_U_SMEM_BYTES = 2 * LDL_KC * 65 * 8;  // step U alone
 // its inverse), the column double buffer, per-wave 16 x 16 products, 1/D and
 // the tile rows' offsets
 constexpr int LDL_FT_A = 0, LDL_FT_X = 64 * 65 * 8, LDL_FT_COL = 2 * 64 * 65 * 8,
-              LDL_FT_T = LDL_FT_COL + 2 * 64 * 8, LDL_FT_RD = LDL_FT_T + 4 * 16 * 17 * 8,
+              LDL_FT_T = LDL_FT_COL + 2 * 4 * 64 * 8, LDL_FT_RD = LDL_FT_T + 4 * 16 * 17 * 8,
               LDL_FT_OFF = LDL_FT_RD + 64 * 8,
               LDL_FT_SI = LDL_FT_OFF + 64 * 8, LDL_STEP_SMEM_BYTES = LDL_FT_SI + 64 * 4;
 
@@ -969,7 +969,7 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
     const int np = P.pe - P.p;
     double(*Lc)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_A);  // factored diagonal block: L below, D on the diagonal
     double(*Xi)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_X);  // L_PP^-1 (unit lower)
-    double(*colb)[64] = reinterpret_cast<double(*)[64]>(smem + LDL_FT_COL);
+    double(*colb)[4 * 64] = reinterpret_cast<double(*)[4 * 64]>(smem + LDL_FT_COL);
     double* rD = reinterpret_cast<double*>(smem + LDL_FT_RD);
     int64_t* rowOff = reinterpret_cast<int64_t*>(smem + LDL_FT_OFF);
     int* rowS = reinterpret_cast<int*>(smem + LDL_FT_SI);
@@ -1007,25 +1007,64 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
     int dsi;
     int64_t doff;
     gather(P.p + fr, fr, fr < np, fr + 1, a, dsi, doff);
+    // right-looking in blocks of four columns, one barrier per block: the 4x4
+    // diagonal block is factored by every thread, then each row forms its
+    // four L entries and applies the rank-4 update to its later columns
 #pragma unroll
-    for (int j = 0; j < 64; ++j) {
-        if (j < np) {
-            double* cb = colb[j & 1];
-            if (fq == (j & 3)) cb[fr] = a[j >> 2];  // A(fr, j) after steps < j
+    for (int jb = 0; jb < 16; ++jb) {
+        const int j0 = 4 * jb;
+        if (j0 < np) {
+            double(*cb)[64] = reinterpret_cast<double(*)[64]>(&colb[jb & 1][0]);  // cb[c][row] = A(row, j0 + c)
+            cb[fq][fr] = a[jb];  // thread (fr, fq) holds column j0 + fq
             __syncthreads();
-            const double d = cb[j];
-            if (d == 0.0 && tid == 0 && tile == 0) atomicMin(zeroPivot, P.p + j + 1);
-            const double li = cb[fr] / d;  // L(fr, j) for fr > j
+            // 4x4 block: d[c], l[r][c] = L(j0 + r, j0 + c); columns past np are inert
+            double d[4], rd[4], l[4][4];
 #pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                if (4 * m + 3 > j) {
-                    const int k = 4 * m + fq;
-                    const double ck = cb[k];  // A(k, j) = L(k, j) D_j
-                    const double upd = a[m] - li * ck;
-                    a[m] = (k > j && k <= fr) ? upd : a[m];
+            for (int c = 0; c < 4; ++c) {
+                const bool valid = j0 + c < np;
+                double dc = cb[c][j0 + c];
+#pragma unroll
+                for (int c2 = 0; c2 < c; ++c2) dc -= l[c][c2] * l[c][c2] * d[c2];
+                d[c] = valid ? dc : 1.0;
+                rd[c] = 1.0 / d[c];
+                if (valid && dc == 0.0 && tid == 0 && tile == 0) atomicMin(zeroPivot, P.p + j0 + c + 1);
+#pragma unroll
+                for (int r = c + 1; r < 4; ++r) {
+                    double v = cb[c][j0 + r];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; ++c2) v -= l[r][c2] * l[c][c2] * d[c2];
+                    l[r][c] = valid ? v * rd[c] : 0.0;
                 }
             }
-            if (fq == (j & 3) && fr > j) a[j >> 2] = li;
+            // this row's L(fr, j0 + c)
+            double lf[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                double v = cb[c][fr];
+#pragma unroll
+                for (int c2 = 0; c2 < c; ++c2) v -= lf[c2] * d[c2] * l[c][c2];
+                lf[c] = j0 + c < np ? v * rd[c] : 0.0;
+            }
+            // rank-4 update of the later columns k <= fr: u_k[c] = D_c L(k, j0 + c)
+#pragma unroll
+            for (int m = jb + 1; m < 16; ++m) {
+                const int k = 4 * m + fq;
+                double u[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    double v = cb[c][k];
+#pragma unroll
+                    for (int c2 = 0; c2 < c; ++c2) v -= u[c2] * l[c][c2];
+                    u[c] = j0 + c < np ? v : 0.0;
+                }
+                const double upd = a[m] - (lf[0] * u[0] + lf[1] * u[1] + lf[2] * u[2] + lf[3] * u[3]);
+                a[m] = k <= fr ? upd : a[m];
+            }
+            // this thread's own column of the block: L below the diagonal, D on it
+            const int jc = j0 + fq;
+            const double lmine = fq == 0 ? lf[0] : (fq == 1 ? lf[1] : (fq == 2 ? lf[2] : lf[3]));
+            const double dmine = fq == 0 ? d[0] : (fq == 1 ? d[1] : (fq == 2 ? d[2] : d[3]));
+            a[jb] = jc < fr ? lmine : (jc == fr && jc < np ? dmine : a[jb]);
         }
     }
     if (tile == 0) {
