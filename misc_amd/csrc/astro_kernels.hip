@@ -819,6 +819,95 @@ constexpr int LDL_FT_A = 0, LDL_FT_X = 64 * 65 * 8, LDL_FT_COL = 2 * 64 * 65 * 8
               LDL_FT_OFF = LDL_FT_RD + 64 * 8,
               LDL_FT_SI = LDL_FT_OFF + 64 * 8, LDL_STEP_SMEM_BYTES = LDL_FT_SI + 64 * 4;
 
+// X = L^-1 for the unit-lower 64 x 64 block whose strictly lower part is in
+// L (entries on and above the diagonal are not read), by 16 x 16 blocks: the
+// four diagonal blocks by forward substitution (thread = column), then block
+// rows 1..3 as X_ij = -X_ii sum_{j <= k < i} L_ik X_kj on the matrix cores;
+// the strictly upper blocks are set to zero.  256 threads; Tw is this wave's
+// 16 x 17 scratch.  Ends with a barrier.
+__device__ __forceinline__ void unit_lower_inverse(const double (*L)[65], double (*X)[65], double (*Tw)[17]) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid < 64) {
+        const int bb = tid >> 4, c = tid & 15, o = 16 * bb;
+        double xv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            double acc2 = r == c ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < r; ++k) acc2 -= L[o + r][o + k] * xv[k];
+            xv[r] = r < c ? 0.0 : acc2;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) X[o + r][o + c] = xv[r];
+    }
+    for (int id = tid; id < 6 * 256; id += 256) {
+        // upper blocks (bi, bj), bi < bj: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
+        const int q = id >> 8, e = id & 255;
+        const int bi = q < 3 ? 0 : (q < 5 ? 1 : 2), bj = q < 3 ? q + 1 : (q < 5 ? q - 1 : 3);
+        X[16 * bi + (e >> 4)][16 * bj + (e & 15)] = 0.0;
+    }
+    __syncthreads();
+    for (int bi = 1; bi < 4; ++bi) {
+        const int bj = w;  // wave w computes block (bi, w) for w < bi
+        if (bj < bi) {
+            double4_t t = double4_t{0.0, 0.0, 0.0, 0.0};
+            for (int bk = bj; bk < bi; ++bk) {
+#pragma unroll
+                for (int ks = 0; ks < 16; ks += 4) {
+                    const int kk = ks + (lane >> 4);
+                    const double av = L[16 * bi + (lane & 15)][16 * bk + kk];
+                    const double bv = X[16 * bk + kk][16 * bj + (lane & 15)];
+                    t = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, t, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Tw[4 * v + (lane >> 4)][lane & 15] = t[v];
+            __builtin_amdgcn_wave_barrier();
+            double4_t xb = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < 16; ks += 4) {
+                const int kk = ks + (lane >> 4);
+                const double av = X[16 * bi + (lane & 15)][16 * bi + kk];
+                const double bv = Tw[kk][lane & 15];
+                xb = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, xb, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) X[16 * bi + 4 * v + (lane >> 4)][16 * bj + (lane & 15)] = -xb[v];
+        }
+        __syncthreads();
+    }
+}
+
+// C = op(A) op(B) for 64 x 64 blocks in LDS on the matrix cores, written
+// column-major to dst (minus E, if given, also column-major): wave w owns
+// rows 16 w .. 16 w + 15.  TA / TB: use A^T / B^T.
+template <bool TA, bool TB>
+__device__ __forceinline__ void mfma_block64(const double (*A)[65], const double (*B)[65], const double* E,
+                                             double* __restrict__ dst) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 64; ks += 4) {
+        const int kk = ks + (lane >> 4), i = w * 16 + (lane & 15);
+        const double av = TA ? A[kk][i] : A[i][kk];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int j = cb * 16 + (lane & 15);
+            const double bv = TB ? B[j][kk] : B[kk][j];
+            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = w * 16 + 4 * v + (lane >> 4), j = cb * 16 + (lane & 15);
+            dst[j * 64 + i] = E ? acc[cb][v] - E[j * 64 + i] : acc[cb][v];
+        }
+}
+
 __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
                                                 int kMode, int slot, const double* __restrict__ lt,
                                                 const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
@@ -1084,63 +1173,9 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
     __syncthreads();
     const int lane = tid & 63, w = tid >> 6;
 
-    // --- 3: X = L_PP^-1 (unit lower; rows past np are the identity) by 16 x 16
-    // blocks: the four diagonal blocks by forward substitution (thread =
-    // column), then block rows 1..3 as X_ij = -X_ii sum_{j <= k < i} L_ik X_kj
-    // on the matrix cores.  The strictly upper blocks are zero.
-    if (tid < 64) {
-        const int bb = tid >> 4, c = tid & 15, o = 16 * bb;
-        double xv[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            double acc2 = r == c ? 1.0 : 0.0;
-#pragma unroll
-            for (int k = 0; k < r; ++k) acc2 -= Lc[o + r][o + k] * xv[k];
-            xv[r] = r < c ? 0.0 : acc2;
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) Xi[o + r][o + c] = xv[r];
-        rD[tid] = tid < np ? 1.0 / Lc[tid][tid] : 0.0;
-    }
-    for (int id = tid; id < 6 * 256; id += 256) {
-        // upper blocks (bi, bj), bi < bj: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
-        const int q = id >> 8, e = id & 255;
-        const int bi = q < 3 ? 0 : (q < 5 ? 1 : 2), bj = q < 3 ? q + 1 : (q < 5 ? q - 1 : 3);
-        Xi[16 * bi + (e >> 4)][16 * bj + (e & 15)] = 0.0;
-    }
-    __syncthreads();
-    {
-        double(*Tw)[17] = reinterpret_cast<double(*)[17]>(smem + LDL_FT_T + w * 16 * 17 * 8);
-        for (int bi = 1; bi < 4; ++bi) {
-            const int bj = w;  // wave w computes block (bi, w) for w < bi
-            if (bj < bi) {
-                double4_t t = double4_t{0.0, 0.0, 0.0, 0.0};
-                for (int bk = bj; bk < bi; ++bk) {
-#pragma unroll
-                    for (int ks = 0; ks < 16; ks += 4) {
-                        const int kk = ks + (lane >> 4);
-                        const double av = Lc[16 * bi + (lane & 15)][16 * bk + kk];
-                        const double bv = Xi[16 * bk + kk][16 * bj + (lane & 15)];
-                        t = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, t, 0, 0, 0);
-                    }
-                }
-#pragma unroll
-                for (int v = 0; v < 4; ++v) Tw[4 * v + (lane >> 4)][lane & 15] = t[v];
-                __builtin_amdgcn_wave_barrier();
-                double4_t xb = double4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int ks = 0; ks < 16; ks += 4) {
-                    const int kk = ks + (lane >> 4);
-                    const double av = Xi[16 * bi + (lane & 15)][16 * bi + kk];
-                    const double bv = Tw[kk][lane & 15];
-                    xb = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, xb, 0, 0, 0);
-                }
-#pragma unroll
-                for (int v = 0; v < 4; ++v) Xi[16 * bi + 4 * v + (lane >> 4)][16 * bj + (lane & 15)] = -xb[v];
-            }
-            __syncthreads();
-        }
-    }
+    // --- 3: X = L_PP^-1 (rows past np are the identity: L is zero there)
+    if (tid < 64) rD[tid] = tid < np ? 1.0 / Lc[tid][tid] : 0.0;
+    unit_lower_inverse(Lc, Xi, reinterpret_cast<double(*)[17]>(smem + LDL_FT_T + w * 16 * 17 * 8));
 
     // --- 4: rows r0 + fr of this tile into LDS (replacing the diagonal
     // block), then L(i, P) = Y(i, :) D^-1 with Y = rows X^T on the matrix cores
@@ -1340,45 +1375,25 @@ __device__ __forceinline__ void load_coupling(double (*C)[65], int rowStart, int
     }
 }
 
-// dst(i, j) = sum_{c <= i} X(i, c) C(c, j)  (X lower triangular), column-major.
-__device__ __forceinline__ void lower_times(const double (*X)[65], const double (*C)[65], double* __restrict__ dst) {
-    const int i = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * 16;
-    for (int j = j0; j < j0 + 16; ++j) {
-        double a = 0.0;
-        for (int c = 0; c <= i; ++c) a += X[i][c] * C[c][j];
-        dst[j * 64 + i] = a;
-    }
-}
-
-// dst(i, j) = sum_{c >= i} X(c, i) C(j, c)  (X^T C^T), column-major.
-__device__ __forceinline__ void lower_t_times_t(const double (*X)[65], const double (*C)[65],
-                                                double* __restrict__ dst) {
-    const int i = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * 16;
-    for (int j = j0; j < j0 + 16; ++j) {
-        double a = 0.0;
-        for (int c = i; c < 64; ++c) a += X[c][i] * C[j][c];
-        dst[j * 64 + i] = a;
-    }
-}
-
-// First pass over the tiles (one workgroup each): L_tt^-1 by forward
-// substitution of the identity columns, then its products with the coupling
-// blocks to the two preceding and two following tiles.  Rows beyond the
-// tile's end are the identity in L_tt.
+// First pass over the tiles (one workgroup each): L_tt^-1 (blocked inverse
+// on the matrix cores), then its products with the coupling blocks to the
+// two preceding and two following tiles.  Rows beyond the tile's end are the
+// identity in L_tt.
 __global__ __launch_bounds__(256) void k_sweep_blocks(const SweepBlock* __restrict__ blocks,
                                                       const double* __restrict__ lt,
                                                       const int64_t* __restrict__ rowoff,
                                                       const int* __restrict__ sfirst, double* __restrict__ out) {
-    __shared__ double L[64][65];  // unit lower L_tt
+    __shared__ double L[64][65];  // strictly lower L_tt, then one coupling block at a time
     __shared__ double X[64][65];  // L_tt^-1
-    __shared__ double C[64][65];  // one coupling block at a time
+    __shared__ double Tw[4][16][17];
     const SweepBlock S = blocks[blockIdx.x];
     const int tid = threadIdx.x;
     const int nr = S.r1 - S.r0;
     double* o = out + (size_t)blockIdx.x * SWEEP_KINDS * SWEEP_BLOCK;
+    // L_tt strictly below the diagonal (zero on and above it)
     for (int id = tid; id < 64 * 64; id += 256) {
         const int i = id >> 6, c = id & 63;
-        double l = c == i ? 1.0 : 0.0;
+        double l = 0.0;
         if (i < nr && c < i) {
             const int gi = S.r0 + i;
             if (S.r0 + c >= sfirst[gi]) l = lt[rowoff[gi] + S.r0 + c];
@@ -1386,61 +1401,36 @@ __global__ __launch_bounds__(256) void k_sweep_blocks(const SweepBlock* __restri
         L[i][c] = l;
     }
     __syncthreads();
-    if (tid < 64) {
-        // column c of L_tt^-1: entries above the diagonal are zero
-        const int c = tid;
-        for (int r = 0; r < 64; ++r) {
-            double a0 = r == c ? 1.0 : 0.0, a1 = 0.0;
-            int k = c;
-            for (; k + 1 < r; k += 2) {
-                a0 -= L[r][k] * X[k][c];
-                a1 -= L[r][k + 1] * X[k + 1][c];
-            }
-            if (k < r) a0 -= L[r][k] * X[k][c];
-            X[r][c] = r < c ? 0.0 : a0 + a1;
-        }
-    }
-    __syncthreads();
+    unit_lower_inverse(L, X, Tw[tid >> 6]);
     for (int id = tid; id < 64 * 64; id += 256) {
         const int cc = id >> 6, ii = id & 63;
         o[SWEEP_DINV * SWEEP_BLOCK + cc * 64 + ii] = X[ii][cc];
         o[SWEEP_DINVT * SWEEP_BLOCK + cc * 64 + ii] = X[cc][ii];
     }
-    load_coupling(C, S.r0, nr, S.pm1, S.r0 - S.pm1, lt, rowoff, sfirst);
+    // F = X L(t, t-1), H = X L(t, t-2), G = X^T L(t+1, t)^T, HB = X^T L(t+2, t)^T
+    load_coupling(L, S.r0, nr, S.pm1, S.r0 - S.pm1, lt, rowoff, sfirst);
     __syncthreads();
-    lower_times(X, C, o + SWEEP_F * SWEEP_BLOCK);
+    mfma_block64<false, false>(X, L, nullptr, o + SWEEP_F * SWEEP_BLOCK);
     __syncthreads();
-    load_coupling(C, S.r0, nr, S.pm2, S.pm1 - S.pm2, lt, rowoff, sfirst);
+    load_coupling(L, S.r0, nr, S.pm2, S.pm1 - S.pm2, lt, rowoff, sfirst);
     __syncthreads();
-    lower_times(X, C, o + SWEEP_H * SWEEP_BLOCK);
+    mfma_block64<false, false>(X, L, nullptr, o + SWEEP_H * SWEEP_BLOCK);
     __syncthreads();
-    load_coupling(C, S.r1, S.q1 - S.r1, S.r0, nr, lt, rowoff, sfirst);
+    load_coupling(L, S.r1, S.q1 - S.r1, S.r0, nr, lt, rowoff, sfirst);
     __syncthreads();
-    lower_t_times_t(X, C, o + SWEEP_G * SWEEP_BLOCK);
+    mfma_block64<true, true>(X, L, nullptr, o + SWEEP_G * SWEEP_BLOCK);
     __syncthreads();
-    load_coupling(C, S.q1, S.q2 - S.q1, S.r0, nr, lt, rowoff, sfirst);
+    load_coupling(L, S.q1, S.q2 - S.q1, S.r0, nr, lt, rowoff, sfirst);
     __syncthreads();
-    lower_t_times_t(X, C, o + SWEEP_HB * SWEEP_BLOCK);
+    mfma_block64<true, true>(X, L, nullptr, o + SWEEP_HB * SWEEP_BLOCK);
 }
 
-// dst = A B (- E): column-major 64 x 64 blocks from global memory; a null B
-// counts as zero.
-__device__ __forceinline__ void block_gemm(const double* __restrict__ A, const double* __restrict__ B,
-                                           const double* __restrict__ E, double* __restrict__ dst,
-                                           double (*As)[65], double (*Bs)[65]) {
+// A column-major 64 x 64 block from global memory into As[i][c] (zero if null).
+__device__ __forceinline__ void load_block64(const double* __restrict__ src, double (*As)[65]) {
     for (int id = threadIdx.x; id < 64 * 64; id += 256) {
         const int c = id >> 6, i = id & 63;
-        As[i][c] = A[id];
-        Bs[i][c] = B ? B[id] : 0.0;
+        As[i][c] = src ? src[id] : 0.0;
     }
-    __syncthreads();
-    const int i = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * 16;
-    for (int j = j0; j < j0 + 16; ++j) {
-        double a = 0.0;
-        for (int c = 0; c < 64; ++c) a += As[i][c] * Bs[c][j];
-        dst[j * 64 + i] = E ? a - E[j * 64 + i] : a;
-    }
-    __syncthreads();
 }
 
 // Second pass: the products of a tile's blocks with its neighbour's,
@@ -1454,14 +1444,23 @@ __global__ __launch_bounds__(256) void k_sweep_blocks_pair(const SweepBlock* __r
     double* o = out + (size_t)blockIdx.x * SWEEP_KINDS * SWEEP_BLOCK;
     const double* p = S.prevBlk >= 0 ? out + (size_t)S.prevBlk * SWEEP_KINDS * SWEEP_BLOCK : nullptr;
     const double* n = S.nextBlk >= 0 ? out + (size_t)S.nextBlk * SWEEP_KINDS * SWEEP_BLOCK : nullptr;
-    block_gemm(o + SWEEP_F * SWEEP_BLOCK, p ? p + SWEEP_H * SWEEP_BLOCK : nullptr, nullptr,
-               o + SWEEP_FH * SWEEP_BLOCK, As, Bs);
-    block_gemm(o + SWEEP_F * SWEEP_BLOCK, p ? p + SWEEP_F * SWEEP_BLOCK : nullptr, o + SWEEP_H * SWEEP_BLOCK,
-               o + SWEEP_K * SWEEP_BLOCK, As, Bs);
-    block_gemm(o + SWEEP_G * SWEEP_BLOCK, n ? n + SWEEP_HB * SWEEP_BLOCK : nullptr, nullptr,
-               o + SWEEP_GHB * SWEEP_BLOCK, As, Bs);
-    block_gemm(o + SWEEP_G * SWEEP_BLOCK, n ? n + SWEEP_G * SWEEP_BLOCK : nullptr, o + SWEEP_HB * SWEEP_BLOCK,
-               o + SWEEP_KB * SWEEP_BLOCK, As, Bs);
+    load_block64(o + SWEEP_F * SWEEP_BLOCK, As);
+    load_block64(p ? p + SWEEP_H * SWEEP_BLOCK : nullptr, Bs);
+    __syncthreads();
+    mfma_block64<false, false>(As, Bs, nullptr, o + SWEEP_FH * SWEEP_BLOCK);
+    __syncthreads();
+    load_block64(p ? p + SWEEP_F * SWEEP_BLOCK : nullptr, Bs);
+    __syncthreads();
+    mfma_block64<false, false>(As, Bs, o + SWEEP_H * SWEEP_BLOCK, o + SWEEP_K * SWEEP_BLOCK);
+    __syncthreads();
+    load_block64(o + SWEEP_G * SWEEP_BLOCK, As);
+    load_block64(n ? n + SWEEP_HB * SWEEP_BLOCK : nullptr, Bs);
+    __syncthreads();
+    mfma_block64<false, false>(As, Bs, nullptr, o + SWEEP_GHB * SWEEP_BLOCK);
+    __syncthreads();
+    load_block64(n ? n + SWEEP_G * SWEEP_BLOCK : nullptr, Bs);
+    __syncthreads();
+    mfma_block64<false, false>(As, Bs, o + SWEEP_HB * SWEEP_BLOCK, o + SWEEP_KB * SWEEP_BLOCK);
 }
 
 // This wave's share of a block product: sum over 16 columns k of
