@@ -255,7 +255,7 @@ struct DeviceProfile {
                 int sm = r0;
                 for (int r = r0; r < r1; ++r) sm = std::min(sm, hS[r]);
                 const int td = (sm - b0) / 64;
-                f.push_back(SweepTile{r0, r1, blkBase[k] + q, std::min(3, q), td <= q - 4 ? blkBase[k] + td : -1, b1});
+                f.push_back(SweepTile{r0, r1, blkBase[k] + q, std::min(4, q), td <= q - 5 ? blkBase[k] + td : -1, b1});
             }
         std::vector<std::vector<int>> reachBack(nBlocks);
         for (int k = 0; k < nBlocks; ++k) {
@@ -281,8 +281,8 @@ struct DeviceProfile {
                 if (t < 0) continue;
                 const int r0 = chainTiles[k][t], r1 = std::min(r0 + 64, bounds[k + 1]);
                 const int td = reachBack[k][t];
-                bw.push_back(SweepTile{r0, r1, blkBase[k] + t, std::min(3, cnt - 1 - t),
-                                       td >= t + 4 ? blkBase[k] + td : -1, bounds[k + 1]});
+                bw.push_back(SweepTile{r0, r1, blkBase[k] + t, std::min(4, cnt - 1 - t),
+                                       td >= t + 5 ? blkBase[k] + td : -1, bounds[k + 1]});
             }
         ASTRO_TRY(fwdTiles.upload(f, s));
         ASTRO_TRY(bwdTiles.upload(bw, s));

@@ -19,9 +19,10 @@ struct PanelInfo {
 // blocks) one after another, each in row order, so the tiles of a chain are
 // consecutive 64-row blocks with consecutive blk.  In sweep direction (up for
 // the forward sweep, down for the backward sweep) the 1st..3rd neighbours are
-// the "near" tiles, nNear of them exist (0..3); the tiles beyond them that
-// the tile couples to are "far": blk from farBlk (the farthest one; -1 for
-// none) up to the 4th neighbour.  lim: the chain's last row + 1, which bounds
+// the "near" tiles; nNear says how many neighbours exist, counted up to 4
+// (the 3rd and 4th neighbours' coupling entries are read into registers at
+// the start).  The tiles beyond them that the tile couples to are "far": blk
+// from farBlk (the farthest one; -1 for none) up to the 5th neighbour.  lim: the chain's last row + 1, which bounds
 // a ragged final tile.  Completion flags are indexed by blk.
 struct SweepTile {
     int r0, r1, blk, nNear, farBlk, lim;

@@ -1567,10 +1567,10 @@ __device__ __forceinline__ bool lookahead_tail(int r0, int nr, int nNear, int fl
 }
 
 // Forward sweep tile.  Phase 1 accumulates the contributions of the far
-// tiles (four or more tiles back; wave w owns rows w*16..+15, lane =
-// column) and of the third near tile (its coupling entries prefetched at
-// the start), partials reduced across lanes once; the tail combines the near
-// tiles through the precomputed blocks.
+// tiles (five or more tiles back; wave w owns rows w*16..+15, lane =
+// column) and of the tiles four and three back (their coupling entries
+// prefetched at the start), partials reduced across lanes once; the tail
+// combines the near tiles through the precomputed blocks.
 __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
                                                           const double* __restrict__ lt,
                                                           const int64_t* __restrict__ rowoff,
@@ -1594,16 +1594,13 @@ __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __res
     if (t >= numTiles) return;
     const SweepTile T = tiles[t];
     const int r0 = T.r0, nr = T.r1 - T.r0, nNear = T.nNear;
-    // this wave's 16 columns of L_tt^-1 and of K_t, coalesced; F_t and
-    // F_t H_{t-1} go to LDS now, far from the critical path (each wave reads
-    // back only what it wrote)
+    // this wave's 16 columns of K_t, coalesced; F_t and F_t H_{t-1} go to
+    // LDS now, far from the critical path (each wave reads back only what it
+    // wrote)
     const double* Bt = blocks + (size_t)T.blk * SWEEP_KINDS * SWEEP_BLOCK;
-    double dv[16], kv[16];
+    double kv[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        dv[k] = Bt[SWEEP_DINV * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
-        kv[k] = nNear >= 2 ? Bt[SWEEP_K * SWEEP_BLOCK + (16 * w + k) * 64 + lane] : 0.0;
-    }
+    for (int k = 0; k < 16; ++k) kv[k] = nNear >= 2 ? Bt[SWEEP_K * SWEEP_BLOCK + (16 * w + k) * 64 + lane] : 0.0;
     if (nNear >= 1) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) B1s[w][k][lane] = Bt[SWEEP_F * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
@@ -1625,24 +1622,30 @@ __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __res
         rs[r] = i < nr ? sfirst[r0 + i] : 0x7fffffff;
         rd[r] = i < nr ? (int)(rowoff[r0 + i] - rowoff[rb]) : 0;
     }
-    // coupling entries to the third near tile's columns
-    double lv3[16];
+    // coupling entries to the columns of the third and fourth tiles back
+    double lv3[16], lv4[16];
     {
-        const int c = r0 - 192 + lane;
+        const int c = r0 - 192 + lane, c4 = r0 - 256 + lane;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) lv3[r] = nNear >= 3 ? ltw[(unsigned int)(rd[r] + c)] : 0.0;
+        for (int r = 0; r < 16; ++r) {
+            lv3[r] = nNear >= 3 ? ltw[(unsigned int)(rd[r] + c)] : 0.0;
+            lv4[r] = nNear >= 4 ? ltw[(unsigned int)(rd[r] + c4)] : 0.0;
+        }
 #pragma unroll
-        for (int r = 0; r < 16; ++r) lv3[r] = c >= rs[r] ? lv3[r] : 0.0;
+        for (int r = 0; r < 16; ++r) {
+            lv3[r] = c >= rs[r] ? lv3[r] : 0.0;
+            lv4[r] = c4 >= rs[r] ? lv4[r] : 0.0;
+        }
     }
     double p[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) p[r] = 0.0;
-    // Far tiles: blk farBlk .. blk-4, whose rows are the contiguous columns
-    // [r0 - 64 (blk - farBlk), r0 - 192).  Each wave streams the columns of
+    // Far tiles: blk farBlk .. blk-5, whose rows are the contiguous columns
+    // [r0 - 64 (blk - farBlk), r0 - 256).  Each wave streams the columns of
     // the tiles already published and re-polls at the first pending one; no
     // block-wide barrier.
     if (T.farBlk >= 0) {
-        const int bEnd = T.blk - 3;
+        const int bEnd = T.blk - 4;
         int J = T.farBlk;
         int c0 = r0 - 64 * (T.blk - J);
         bool front = false;
@@ -1686,13 +1689,23 @@ __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __res
             front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
-    // the near tiles' values, loaded speculatively (usually produced by now);
-    // the third near tile's are needed at once
-    NearVals n1{0, 0, 0.0}, n2{0, 0, 0.0}, n3{0, 0, 0.0};
+    // the values of the tiles four and three back (needed at once) and of
+    // the two nearest, loaded speculatively (usually produced by now), then
+    // this wave's 16 columns of L_tt^-1
+    NearVals n1{0, 0, 0.0}, n2{0, 0, 0.0}, n3{0, 0, 0.0}, n4{0, 0, 0.0};
+    if (nNear >= 4) n4 = near_load(z, r0 - 256, 64);
+    if (nNear >= 3) n3 = near_load(z, r0 - 192, 64);
     if (nNear >= 1) n1 = near_load(g, r0 - 64 + 16 * w, 16);
     if (nNear >= 2) n2 = near_load(z, r0 - 128 + 16 * w, 16);
+    double dv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) dv[k] = Bt[SWEEP_DINV * SWEEP_BLOCK + (16 * w + k) * 64 + lane];
+    if (nNear >= 4) {
+        if (!near_wait(z, n4, spinLimit, timeout)) return;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) p[r] += lv4[r] * n4.v;
+    }
     if (nNear >= 3) {
-        n3 = near_load(z, r0 - 192, 64);
         if (!near_wait(z, n3, spinLimit, timeout)) return;
 #pragma unroll
         for (int r = 0; r < 16; ++r) p[r] += lv3[r] * n3.v;
@@ -1739,8 +1752,8 @@ __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __res
 }
 
 // Backward sweep tile (columns [r0, r1)): phase 1 takes the rows of the far
-// tiles (four or more tiles on) and of the third near tile (coupling entries
-// prefetched), the tail combines the near tiles.
+// tiles (five or more tiles on) and of the tiles three and four on (coupling
+// entries prefetched), the tail combines the near tiles.
 __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restrict__ tiles, int numTiles,
                                                         const double* __restrict__ lt,
                                                         const int64_t* __restrict__ rowoff,
@@ -1788,35 +1801,48 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     const int ci = r0 + lane;
     // an in-range column for idle lanes' loads; their partial sums are unused
     const int cl = lane < nr ? ci : r0;
-    // coupling entries of this wave's 16 rows of the third near tile
-    const int jb3 = r1 + 128 + 16 * w;
-    const int cnt3 = nNear >= 3 ? max(0, min(16, lim - jb3)) : 0;
-    double lc3[16];
-    {
-        int sjl = 0x7fffffff;
-        long long rol = 0;
-        if (lane < cnt3) {
-            sjl = sfirst[jb3 + lane];
-            rol = rowoff[jb3 + lane];
+    // coupling entries lc[r] = L(jb + r, ci) of 16 rows (lanes 0..cnt-1
+    // carry their first columns and offsets; rows beyond cnt have offset 0,
+    // so their loads read lt[cl], inside the profile, and are masked)
+    auto rows_meta = [&](int jb, int cnt, int& sjl, long long& rol) {
+        sjl = 0x7fffffff;
+        rol = 0;
+        if (lane < cnt) {
+            sjl = sfirst[jb + lane];
+            rol = rowoff[jb + lane];
         }
+    };
+    auto row_entries = [&](int sjl, long long rol, double (&lc)[16]) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int sj = __builtin_amdgcn_readlane(sjl, r);
             const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
                                  (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
-            // rows beyond cnt3 have ro = 0: lt[cl] is inside the profile
             const double v = lt[ro + cl];
-            lc3[r] = ci >= sj ? v : 0.0;
+            lc[r] = ci >= sj ? v : 0.0;
         }
+    };
+    // this wave's 16 rows of the tiles three and four on
+    const int jb3 = r1 + 128 + 16 * w, jb4 = r1 + 192 + 16 * w;
+    const int cnt3 = nNear >= 3 ? max(0, min(16, lim - jb3)) : 0;
+    const int cnt4 = nNear >= 4 ? max(0, min(16, lim - jb4)) : 0;
+    double lc3[16], lc4[16];
+    {
+        int s3, s4;
+        long long o3, o4;
+        rows_meta(jb3, cnt3, s3, o3);
+        rows_meta(jb4, cnt4, s4, o4);
+        row_entries(s3, o3, lc3);
+        row_entries(s4, o4, lc4);
     }
     double p = 0.0;
-    // Far tiles: blk blk+4 .. farBlk, rows [r1 + 192, end of farBlk).
+    // Far tiles: blk blk+5 .. farBlk, rows [r1 + 256, end of farBlk).
     // Published tiles form a top-down prefix of that range (tiles finish in
     // descending order); each wave streams the published rows and re-polls,
     // without block barriers.
     if (T.farBlk >= 0) {
-        const int bLow = T.blk + 3;
-        const int jLow = r1 + 192;
+        const int bLow = T.blk + 4;
+        const int jLow = r1 + 256;
         int J = T.farBlk;
         int j1 = min(r0 + 64 * (J - T.blk) + 64, lim);  // rows [jStop, j1) still to do, top-down
         bool front = false;
@@ -1827,7 +1853,7 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                 // fetch the group's first columns, offsets and x, broadcast
                 const int pend = first_pending_down(flags, J, bLow, epoch);
                 const int jStop = pend <= bLow ? jLow : min(r0 + 64 * (pend - T.blk) + 64, lim);
-#pragma unroll 1
+#pragma unroll 2
                 for (int jb = jStop + 16 * w; jb < j1; jb += 64) {
                     int sjl = 0x7fffffff;
                     long long rol = 0;
@@ -1856,21 +1882,11 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
             const int q0 = r0 + 64 * (J - T.blk);
             const int jb = q0 + 16 * w;
             const int cnt = max(0, min(16, min(q0 + 64, lim) - jb));
-            int sjl = 0x7fffffff;
-            long long rol = 0;
-            if (lane < cnt) {
-                sjl = sfirst[jb + lane];
-                rol = rowoff[jb + lane];
-            }
+            int sjl;
+            long long rol;
+            rows_meta(jb, cnt, sjl, rol);
             double lc[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int sj = __builtin_amdgcn_readlane(sjl, r);
-                const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
-                                     (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
-                const double v = lt[ro + cl];
-                lc[r] = ci >= sj ? v : 0.0;
-            }
+            row_entries(sjl, rol, lc);
             const int dist = J - T.blk;
             double xl;
             if (!poll_values(x, jb, cnt, backoff_for(dist), spinLimit, timeout, xl)) return;
@@ -1881,11 +1897,17 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
             front = dist - 1 <= ASTRO_NEAR_TILES;
         }
     }
-    NearVals n1{0, 0, 0.0}, n2{0, 0, 0.0}, n3{0, 0, 0.0};
+    NearVals n1{0, 0, 0.0}, n2{0, 0, 0.0}, n3{0, 0, 0.0}, n4{0, 0, 0.0};
+    if (nNear >= 4) n4 = near_load(x, jb4, cnt4);
+    if (nNear >= 3) n3 = near_load(x, jb3, cnt3);
     if (nNear >= 1) n1 = near_load(g, r1 + 16 * w, max(0, min(16, lim - (r1 + 16 * w))));
     if (nNear >= 2) n2 = near_load(x, r1 + 64 + 16 * w, max(0, min(16, lim - (r1 + 64 + 16 * w))));
+    if (nNear >= 4) {
+        if (!near_wait(x, n4, spinLimit, timeout)) return;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) p += lc4[r] * lane_bcast(n4.v, r);
+    }
     if (nNear >= 3) {
-        n3 = near_load(x, jb3, cnt3);
         if (!near_wait(x, n3, spinLimit, timeout)) return;
 #pragma unroll
         for (int r = 0; r < 16; ++r) p += lc3[r] * lane_bcast(n3.v, r);
