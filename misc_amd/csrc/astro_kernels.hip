@@ -1853,16 +1853,26 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                 // fetch the group's first columns, offsets and x, broadcast
                 const int pend = first_pending_down(flags, J, bLow, epoch);
                 const int jStop = pend <= bLow ? jLow : min(r0 + 64 * (pend - T.blk) + 64, lim);
-#pragma unroll 2
-                for (int jb = jStop + 16 * w; jb < j1; jb += 64) {
-                    int sjl = 0x7fffffff;
-                    long long rol = 0;
-                    double xl = 0.0;
+                // software-pipelined: the next group's first columns, offsets
+                // and x are fetched while the current group's entries load
+                auto group_meta = [&](int jb, int& sjl, long long& rol, double& xl) {
+                    sjl = 0x7fffffff;
+                    rol = 0;
+                    xl = 0.0;
                     if (lane < 16 && jb + lane < j1) {
                         sjl = sfirst[jb + lane];
                         rol = rowoff[jb + lane];
                         xl = coherent_load(&x[jb + lane]);
                     }
+                };
+                int sjl, nsjl;
+                long long rol, nrol;
+                double xl, nxl;
+                int jb = jStop + 16 * w;
+                group_meta(jb, sjl, rol, xl);
+#pragma unroll 1
+                for (; jb < j1; jb += 64) {
+                    group_meta(jb + 64, nsjl, nrol, nxl);
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const int sj = __builtin_amdgcn_readlane(sjl, r);
@@ -1872,6 +1882,9 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                         const double v = lt[ro + cl];
                         p += (ci >= sj ? v : 0.0) * xj;
                     }
+                    sjl = nsjl;
+                    rol = nrol;
+                    xl = nxl;
                 }
                 j1 = jStop;
                 J = pend;
