@@ -1865,26 +1865,39 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                         xl = coherent_load(&x[jb + lane]);
                     }
                 };
-                int sjl, nsjl;
-                long long rol, nrol;
-                double xl, nxl;
+                // two groups (jb, jb + 64) per step, so 32 row loads are in flight
+                int sjl0, sjl1, nsjl0, nsjl1;
+                long long rol0, rol1, nrol0, nrol1;
+                double xl0, xl1, nxl0, nxl1;
                 int jb = jStop + 16 * w;
-                group_meta(jb, sjl, rol, xl);
+                group_meta(jb, sjl0, rol0, xl0);
+                group_meta(jb + 64, sjl1, rol1, xl1);
 #pragma unroll 1
-                for (; jb < j1; jb += 64) {
-                    group_meta(jb + 64, nsjl, nrol, nxl);
+                for (; jb < j1; jb += 128) {
+                    group_meta(jb + 128, nsjl0, nrol0, nxl0);
+                    group_meta(jb + 192, nsjl1, nrol1, nxl1);
+                    double v0[16], v1[16];
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const int sj = __builtin_amdgcn_readlane(sjl, r);
-                        const long long ro = ((long long)__builtin_amdgcn_readlane((int)(rol >> 32), r) << 32) |
-                                             (unsigned int)__builtin_amdgcn_readlane((int)(rol & 0xffffffffLL), r);
-                        const double xj = lane_bcast(xl, r);
-                        const double v = lt[ro + cl];
-                        p += (ci >= sj ? v : 0.0) * xj;
+                        const long long ro0 = ((long long)__builtin_amdgcn_readlane((int)(rol0 >> 32), r) << 32) |
+                                              (unsigned int)__builtin_amdgcn_readlane((int)(rol0 & 0xffffffffLL), r);
+                        const long long ro1 = ((long long)__builtin_amdgcn_readlane((int)(rol1 >> 32), r) << 32) |
+                                              (unsigned int)__builtin_amdgcn_readlane((int)(rol1 & 0xffffffffLL), r);
+                        v0[r] = lt[ro0 + cl];
+                        v1[r] = lt[ro1 + cl];
                     }
-                    sjl = nsjl;
-                    rol = nrol;
-                    xl = nxl;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int sj0 = __builtin_amdgcn_readlane(sjl0, r), sj1 = __builtin_amdgcn_readlane(sjl1, r);
+                        p += (ci >= sj0 ? v0[r] : 0.0) * lane_bcast(xl0, r);
+                        p += (ci >= sj1 ? v1[r] : 0.0) * lane_bcast(xl1, r);
+                    }
+                    sjl0 = nsjl0;
+                    rol0 = nrol0;
+                    xl0 = nxl0;
+                    sjl1 = nsjl1;
+                    rol1 = nrol1;
+                    xl1 = nxl1;
                 }
                 j1 = jStop;
                 J = pend;
