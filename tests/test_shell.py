@@ -248,5 +248,8 @@ def test_gpu_coupled_profile_factor_and_sweeps_match_oracle():
     assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
     ctx.reset()
     ctx.static_solve()
-    assert ctx.scalar("gmresSteps") <= 3
+    # the factor is exact up to rounding at cond ~1e16, so GMRES needs a few
+    # steps to reach 1e-8 (the count moves with summation order); a broken
+    # preconditioner needs thousands
+    assert ctx.scalar("gmresSteps") <= 6
     ctx.close()
