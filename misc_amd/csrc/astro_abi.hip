@@ -440,7 +440,7 @@ struct Gmres {
     void multidot(const double* Vb, int nq, const double* w, double* out, hipStream_t s) {
         hipLaunchKernelGGL(k_multidot, dim3(nblk, (nq + 7) / 8), dim3(256), 0, s, Vb, (int64_t)n, nq, w, n,
                            partial.p, m + 8);
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial.p, nblk, m + 8, nq, out);
+        hipLaunchKernelGGL(k_sum_partials, dim3(nq), dim3(64), 0, s, partial.p, nblk, m + 8, nq, out);
     }
     double* Vcol(int q) { return V.p + (size_t)q * n; }
 

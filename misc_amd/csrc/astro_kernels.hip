@@ -2029,13 +2029,17 @@ __global__ __launch_bounds__(256) void k_multidot(const double* __restrict__ V, 
     if (tid < nq) partial[(size_t)blockIdx.x * pstride + q0 + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partial, int nblk, int pstride, int nq,
-                               double* __restrict__ out) {
-    const int q = threadIdx.x;
+// out[q] = sum over blocks of partial[b][q]: one wave per q (grid nq x 64
+// threads), lanes take blocks b = lane, lane + 64, ..., then a fixed
+// butterfly, so the order of the additions never changes.
+__global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ partial, int nblk, int pstride,
+                                                     int nq, double* __restrict__ out) {
+    const int q = blockIdx.x, lane = threadIdx.x;
     if (q >= nq) return;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * pstride + q];
-    out[q] = s;
+    for (int b = lane; b < nblk; b += 64) s += partial[(size_t)b * pstride + q];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) out[q] = s;
 }
 
 // w -= sum_q V[q] h[q]   (h on the device)
