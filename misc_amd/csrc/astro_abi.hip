@@ -94,6 +94,7 @@ struct DeviceProfile {
     DBuf<double> scratch;
     DBuf<double> diag;  // D of the factor, written by step FT, read by step U
     DBuf<int> zeroPivot;
+    DBuf<double> ftX;  // per block: L_PP^-1 (64 x 64) and 1 / D of the current panel's diagonal block
     // sweeps
     int numTiles = 0;
     DBuf<SweepTile> fwdTiles, bwdTiles;
@@ -208,11 +209,16 @@ struct DeviceProfile {
             }
         }
         maxTiles = scratchRows / 64;
-        splitK = std::max(1, std::min(16, (1536 + nBlocks * maxTiles - 1) / std::max(1, nBlocks * maxTiles)));
+        // about 768 old-update workgroups per launch (4 splits per tile at the
+        // 64x64x32 block): fewer and longer ones lose the balance, more add
+        // partial sums to every gather (measured 3: 201 ms, 4: 186 ms,
+        // 6: 187 ms, 8: 196 ms for the whole factor)
+        splitK = std::max(1, std::min(16, (768 + nBlocks * maxTiles - 1) / std::max(1, nBlocks * maxTiles)));
         ASTRO_TRY(panels.upload(hp, s));
         // two buffers (panel parity), each with splitK "old" partials and one "new"
         ASTRO_TRY(scratch.alloc(2 * (size_t)(splitK + 1) * nBlocks * scratchRows * 64));
         ASTRO_TRY(zeroPivot.alloc(1));
+        ASTRO_TRY(ftX.alloc((size_t)std::max(1, nBlocks) * (64 * 64 + 64)));
         ASTRO_TRY(diag.alloc((size_t)std::max(1, n)));
         // sweep tiles, chains interleaved in ticket order
         std::vector<std::vector<int>> chainTiles(nBlocks);  // tile start rows per chain
@@ -307,22 +313,25 @@ struct DeviceProfile {
     int factor(hipStream_t s) {
         HIP_TRY(hipMemsetAsync(zeroPivot.p, 0x7f, sizeof(int), s));
         // Panel q needs W = sum_{k < p} L(i,k) D_k L(P,k): the "old" part
-        // (k before panel q-1) runs in the same launch as FT of panel q-1, the
-        // "new" part (panel q-1's columns) right after it.  Partial sums live
-        // in scratch buffer q & 1: slots 0..splitK-1 old, slot splitK new.
+        // (columns before panel q-1, nSplit partials) is computed by launch
+        // L_{q-1}, the "new" part (panel q-1's columns) by L_q, which also
+        // factors panel q's diagonal block; R_q then forms the panel's other
+        // rows.  Partial sums live in scratch buffer q & 1: slots
+        // 0..splitK-1 old, slot splitK new.
         const size_t bufSize = (size_t)(splitK + 1) * nBlocks * scratchRows * 64;
         auto buf = [&](int q) { return scratch.p + (size_t)(q & 1) * bufSize; };
-        const dim3 gOld(maxTiles * splitK, nBlocks), gNew(maxTiles, nBlocks), gStep(maxTiles * (splitK + 1) * nBlocks);
+        const dim3 gOld(maxTiles * splitK, nBlocks), gLead(nBlocks * maxTiles * (splitK + 1));
+        const dim3 gRows(nBlocks * std::max(1, maxTiles - 1));
+        // panel 0 has no old part: its slots are written as zeros
         hipLaunchKernelGGL(k_ldl_update_mfma, gOld, dim3(256), 0, s, panels.p, 0, panelsPerBlock, splitK, LDL_K_OLD, 0,
                            lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(0), scratchRows);
-        hipLaunchKernelGGL(k_ldl_update_mfma, gNew, dim3(256), 0, s, panels.p, 0, panelsPerBlock, 1, LDL_K_NEW,
-                           splitK, lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(0), scratchRows);
         for (int q = 0; q < panelsPerBlock; ++q) {
-            hipLaunchKernelGGL(k_ldl_step, gStep, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles, splitK, lt.p,
-                               rowoff.p, sfirst.p, winMin.p, diag.p, buf(q), buf(q + 1), scratchRows, zeroPivot.p);
-            if (q + 1 < panelsPerBlock)
-                hipLaunchKernelGGL(k_ldl_update_mfma, gNew, dim3(256), 0, s, panels.p, q + 1, panelsPerBlock, 1,
-                                   LDL_K_NEW, splitK, lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(q + 1), scratchRows);
+            hipLaunchKernelGGL(k_ldl_lead, gLead, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles,
+                               splitK, lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(q), buf(q + 1), scratchRows,
+                               zeroPivot.p, ftX.p);
+            if (maxTiles > 1)
+                hipLaunchKernelGGL(k_ldl_rows, gRows, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles,
+                                   splitK, lt.p, rowoff.p, sfirst.p, buf(q), scratchRows, ftX.p);
         }
         HIP_TRY(hipGetLastError());
         ASTRO_TRY(prepare_blocks(s));

@@ -79,9 +79,12 @@ enum { LDL_K_ALL = 0, LDL_K_OLD = 1, LDL_K_NEW = 2 };
 __global__ void k_ldl_update_mfma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int nSplit, int kMode,
                                   int slot0, const double* lt, const int64_t* rowoff, const int* sfirst,
                                   const int* winMin, const double* diag, double* scratch, int scratchRows);
-__global__ void k_ldl_step(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
+__global__ void k_ldl_lead(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
                            double* lt, const int64_t* rowoff, const int* sfirst, const int* winMin, double* diag,
-                           const double* scratchFT, double* scratchU, int scratchRows, int* zeroPivot);
+                           double* bufQ, double* bufQ1, int scratchRows, int* zeroPivot, double* ftX);
+__global__ void k_ldl_rows(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
+                           double* lt, const int64_t* rowoff, const int* sfirst, const double* bufQ, int scratchRows,
+                           const double* ftX);
 __global__ void k_max_abs(const double* v, int64_t n, unsigned long long* out);
 __global__ void k_scale_mpc(double* cmat, double* rhs, const int* crange, int cdim, const unsigned long long* maxBits);
 __global__ void k_fill(double* p, int64_t n, double v);

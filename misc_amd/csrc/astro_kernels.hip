@@ -1034,24 +1034,22 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
                     scratch, scratchRows, smem);
 }
 
-// Step FT.  Thread (r = tid / 4, q = tid % 4) holds 16 entries of one row in
-// registers: columns q, q+4, ..., q+60 of the panel.
-//  1. gather: A(row, P) minus the split-K partial sums of step U; loads are
-//     issued at indices clamped into the row's stored range and masked after
-//     (no per-entry branch and memory wait);
-//  2. every workgroup factors the 64x64 diagonal block (right-looking, one
-//     barrier per column, the column double-buffered in LDS); tile 0 writes
-//     it back and records D in `diag`;
-//  3. tiles >= 1 solve their rows against the unit lower factor: y_j is
-//     final at step j and reaches the other three lanes of its row by a DPP
-//     quad broadcast; L(i, j) = y_j / D_j.
-// Both loops are fully unrolled, so register indices and the lane that owns
-// column j are compile-time constants.
-__device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int nb, int tile, int nSlots,
+// Step FT of a panel P, in two parts.  Thread (r = tid / 4, q = tid % 4)
+// holds 16 entries of one row in registers: columns q, q+4, ..., q+60 of the
+// panel.  gather: A(row, P) minus the split-K partial sums of step U; loads
+// are issued at indices clamped into the row's stored range and masked after
+// (no per-entry branch and memory wait).
+//  - ldl_panel_diag (one workgroup per block): factors the 64x64 diagonal
+//    block right-looking in blocks of four columns, writes it back with D in
+//    `diag`, and stores X = L_PP^-1 and 1 / D in Xg for the row tiles;
+//  - ldl_panel_rows (one workgroup per further 64-row tile, in the next
+//    launch): L(i, P) = rows X^T D^-1 on the matrix cores.
+__device__ __forceinline__ void ldl_panel_diag(const PanelInfo P, int blk, int nb, int nSlots,
                                                double* __restrict__ lt, const int64_t* __restrict__ rowoff,
                                                const int* __restrict__ sfirst, const double* __restrict__ scratch,
                                                int scratchRows, double* __restrict__ diag,
-                                               int* __restrict__ zeroPivot, char* smem) {
+                                               int* __restrict__ zeroPivot, double* __restrict__ Xg, char* smem) {
+    const int tile = 0;
     const int r0 = P.p + 64 * tile;
     if (r0 >= P.iend) return;
     const int nr = min(64, P.iend - r0);
@@ -1156,7 +1154,7 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
             a[jb] = jc < fr ? lmine : (jc == fr && jc < np ? dmine : a[jb]);
         }
     }
-    if (tile == 0) {
+    {
         const int gi = P.p + fr;
         if (fr < np) {
 #pragma unroll
@@ -1166,19 +1164,68 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
                 if (k == fr) diag[gi] = a[m];
             }
         }
-        return;
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) Lc[fr][fq + 4 * m] = a[m];
     __syncthreads();
-    const int lane = tid & 63, w = tid >> 6;
-
-    // --- 3: X = L_PP^-1 (rows past np are the identity: L is zero there)
+    const int w = tid >> 6;
+    // X = L_PP^-1 (rows past np are the identity: L is zero there) and 1 / D,
+    // for the row tiles of the next launch
     if (tid < 64) rD[tid] = tid < np ? 1.0 / Lc[tid][tid] : 0.0;
     unit_lower_inverse(Lc, Xi, reinterpret_cast<double(*)[17]>(smem + LDL_FT_T + w * 16 * 17 * 8));
+    __syncthreads();
+    for (int id = tid; id < 64 * 64; id += 256) Xg[id] = Xi[id >> 6][id & 63];
+    if (tid < 64) Xg[64 * 64 + tid] = rD[tid];
+}
 
-    // --- 4: rows r0 + fr of this tile into LDS (replacing the diagonal
-    // block), then L(i, P) = Y(i, :) D^-1 with Y = rows X^T on the matrix cores
+__device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int nb, int tile, int nSlots,
+                                               double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                               const int* __restrict__ sfirst, const double* __restrict__ scratch,
+                                               int scratchRows, const double* __restrict__ Xg, char* smem) {
+    const int r0 = P.p + 64 * tile;
+    if (r0 >= P.iend) return;
+    const int nr = min(64, P.iend - r0);
+    const int np = P.pe - P.p;
+    double(*Lc)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_A);  // this tile's rows
+    double(*Xi)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_X);  // L_PP^-1 (unit lower)
+    double* rD = reinterpret_cast<double*>(smem + LDL_FT_RD);
+    int64_t* rowOff = reinterpret_cast<int64_t*>(smem + LDL_FT_OFF);
+    int* rowS = reinterpret_cast<int*>(smem + LDL_FT_SI);
+    const int tid = threadIdx.x, fr = tid >> 2, fq = tid & 3;
+    const size_t qstride = (size_t)nb * scratchRows * 64;
+    const int lane = tid & 63, w = tid >> 6;
+    // X and 1 / D of the diagonal block (written by the previous launch)
+    for (int id = tid; id < 64 * 64; id += 256) Xi[id >> 6][id & 63] = Xg[id];
+    if (tid < 64) rD[tid] = Xg[64 * 64 + tid];
+
+    // gather row `rowInPanel` of the updated panel block; entry m is column
+    // k = fq + 4 m of the panel, kept where live && k < kmax && inside the profile
+    auto gather = [&](int gi, int rowInPanel, bool live, int kmax, double (&a)[16], int& si, int64_t& off) {
+        const int gr = live ? gi : P.p;
+        si = sfirst[gr];
+        off = rowoff[gr];
+        const int rp = live ? rowInPanel : 0;
+        double raw[16], ws[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int c = P.p + fq + 4 * m;
+            raw[m] = lt[off + min(max(c, si), gr)];
+            ws[m] = 0.0;
+        }
+        const double* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
+        for (int q = 0; q < nSlots; ++q) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) ws[m] += sp[(size_t)q * qstride + 4 * m];
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int k = fq + 4 * m;
+            a[m] = (live && k < kmax && P.p + k >= si) ? raw[m] - ws[m] : 0.0;
+        }
+    };
+
+    // rows r0 + fr of this tile into LDS, then L(i, P) = Y(i, :) D^-1 with
+    // Y = rows X^T on the matrix cores
     const int gi = r0 + fr;
     const bool live = fr < nr;
     double y[16];
@@ -1220,37 +1267,68 @@ __device__ __forceinline__ void ldl_panel_body(const PanelInfo P, int blk, int n
     }
 }
 
-// One factor step: blocks x < numTiles run FT for panel panelIdx (reading
-// the partial sums in scratchFT, nSlots of them), the others run the "old" U
-// of panel panelIdx + 1 into scratchU (nSplit partials): that part reads only
-// columns finished before this launch, so the two overlap.
-__global__ __launch_bounds__(256) void k_ldl_step(const PanelInfo* __restrict__ panels, int panelIdx,
+// Factor launch L_q (one per panel index q, every block's panel q):
+//  - x < numBlocks: the "new" update of panel q's diagonal tile (the
+//    previous panel's columns), then ldl_panel_diag, which factors the
+//    diagonal block and stores L_PP^-1 and 1 / D in ftX;
+//  - the next numBlocks * (numTiles - 1): the "new" update of panel q's
+//    other tiles into scratch slot nSplit of bufQ;
+//  - the rest: the "old" update of panel q + 1 (every column before panel
+//    q, nSplit partials into bufQ1), which needs only panels finished by
+//    earlier launches, so it overlaps the diagonal factor.  Tile-major
+//    order: the tiles nearest the diagonal have the longest column ranges
+//    and are dispatched first.
+// The row tiles of panel q follow in k_ldl_rows.
+__global__ __launch_bounds__(256) void k_ldl_lead(const PanelInfo* __restrict__ panels, int panelIdx,
                                                   int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
                                                   double* __restrict__ lt, const int64_t* __restrict__ rowoff,
                                                   const int* __restrict__ sfirst, const int* __restrict__ winMin,
-                                                  double* __restrict__ diag, const double* __restrict__ scratchFT,
-                                                  double* __restrict__ scratchU, int scratchRows,
-                                                  int* __restrict__ zeroPivot) {
+                                                  double* __restrict__ diag, double* __restrict__ bufQ,
+                                                  double* __restrict__ bufQ1, int scratchRows,
+                                                  int* __restrict__ zeroPivot, double* __restrict__ ftX) {
     __shared__ __align__(16) char smem[LDL_STEP_SMEM_BYTES];
-    // 1-D grid, FT workgroups of every block first: they carry the critical
-    // path and are dispatched before any U work
-    const int x = blockIdx.x, nFT = numBlocks * numTiles;
-    if (x < nFT) {
-        const int blk = x / numTiles, tile = x % numTiles;
+    const int x = blockIdx.x;
+    const int nNew = numBlocks * (numTiles - 1);
+    if (x < numBlocks) {
+        const int blk = x;
+        const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
+        if (P.p < 0 || P.p >= P.iend) return;
+        ldl_update_body(P, blk, numBlocks, 0, 0, 1, LDL_K_NEW, nSplit, lt, rowoff, sfirst, winMin, diag, bufQ,
+                        scratchRows, smem);
+        __syncthreads();  // its partial sums (global) are read back by the gather
+        ldl_panel_diag(P, blk, numBlocks, nSplit + 1, lt, rowoff, sfirst, bufQ, scratchRows, diag, zeroPivot,
+                       ftX + (size_t)blk * (64 * 64 + 64), smem);
+    } else if (x < numBlocks + nNew) {
+        const int id = x - numBlocks;
+        const int blk = id / (numTiles - 1), tile = 1 + id % (numTiles - 1);
         const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
         if (P.p < 0 || P.p + 64 * tile >= P.iend) return;
-        ldl_panel_body(P, blk, numBlocks, tile, nSplit + 1, lt, rowoff, sfirst, scratchFT, scratchRows, diag,
-                       zeroPivot, smem);
+        ldl_update_body(P, blk, numBlocks, tile, 0, 1, LDL_K_NEW, nSplit, lt, rowoff, sfirst, winMin, diag, bufQ,
+                        scratchRows, smem);
     } else {
         if (panelIdx + 1 >= numPanelsPerBlock) return;
-        const int id = x - nFT;
-        const int per = numTiles * nSplit;
-        const int blk = id / per, tile = (id % per) / nSplit, sp = id % nSplit;
+        const int id = x - numBlocks - nNew;
+        const int tile = id / (numBlocks * nSplit), blk = (id / nSplit) % numBlocks, sp = id % nSplit;
         const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx + 1];
         if (P.p < 0 || P.p + 64 * tile >= P.iend) return;
-        ldl_update_body(P, blk, numBlocks, tile, sp, nSplit, LDL_K_OLD, sp, lt, rowoff, sfirst, winMin, diag,
-                        scratchU, scratchRows, smem);
+        ldl_update_body(P, blk, numBlocks, tile, sp, nSplit, LDL_K_OLD, sp, lt, rowoff, sfirst, winMin, diag, bufQ1,
+                        scratchRows, smem);
     }
+}
+
+// Factor launch R_q: the row tiles >= 1 of every block's panel q (grid
+// numBlocks * (numTiles - 1)), against the L_PP^-1 that L_q stored.
+__global__ __launch_bounds__(256) void k_ldl_rows(const PanelInfo* __restrict__ panels, int panelIdx,
+                                                  int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
+                                                  double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                                  const int* __restrict__ sfirst, const double* __restrict__ bufQ,
+                                                  int scratchRows, const double* __restrict__ ftX) {
+    __shared__ __align__(16) char smem[LDL_STEP_SMEM_BYTES];
+    const int blk = blockIdx.x / (numTiles - 1), tile = 1 + blockIdx.x % (numTiles - 1);
+    const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
+    if (P.p < 0 || P.p + 64 * tile >= P.iend) return;
+    ldl_panel_rows(P, blk, numBlocks, tile, nSplit + 1, lt, rowoff, sfirst, bufQ, scratchRows,
+                   ftX + (size_t)blk * (64 * 64 + 64), smem);
 }
 
 // ---------------------------------------------------------------------------
