@@ -313,7 +313,7 @@ def main():
     }
     if flops is not None:
         fac_s = phases["factor"] / args.steps * 1e-3
-        out["roofline_factor"] = {"kernel": "k_ldl_step + k_ldl_update_mfma + k_sweep_blocks (whole factor phase)",
+        out["roofline_factor"] = {"kernel": "k_ldl_lead + k_ldl_rows + k_sweep_blocks (whole factor phase)",
                                   "bound": "mfma", "achieved": flops / fac_s / 1e12, "peak": FP64_MFMA_PEAK_TFLOPS,
                                   "unit": "TFLOP/s", "frac": flops / fac_s / 1e12 / FP64_MFMA_PEAK_TFLOPS,
                                   "flops_per_factor": flops}
