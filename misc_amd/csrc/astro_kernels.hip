@@ -1201,10 +1201,6 @@ __device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int n
     const int tid = threadIdx.x, fr = tid >> 2, fq = tid & 3;
     const size_t qstride = (size_t)nb * scratchRows * 64;
     const int lane = tid & 63, w = tid >> 6;
-    // X and 1 / D of the diagonal block (written by the previous launch)
-    for (int id = tid; id < 64 * 64; id += 256) Xi[id >> 6][id & 63] = Xg[id];
-    if (tid < 64) rD[tid] = Xg[64 * 64 + tid];
-
     // gather row `rowInPanel` of the updated panel block; entry m is column
     // k = fq + 4 m of the panel, kept where live && k < kmax && inside the profile
     auto gather = [&](int gi, int rowInPanel, bool live, int kmax, double (&a)[16], int& si, int64_t& off) {
@@ -1239,6 +1235,10 @@ __device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int n
     int si;
     int64_t off;
     gather(gi, r0 - P.p + fr, live, np, y, si, off);
+    // X and 1 / D of the diagonal block (written by the lead launch), loaded
+    // while the row gather is in flight
+    for (int id = tid; id < 64 * 64; id += 256) Xi[id >> 6][id & 63] = Xg[id];
+    if (tid < 64) rD[tid] = Xg[64 * 64 + tid];
     if (fq == 0) {
         rowOff[fr] = off;
         rowS[fr] = live ? si : 0x7fffffff;
