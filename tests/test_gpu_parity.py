@@ -130,6 +130,46 @@ def test_getSparseLDLFact_and_solve(n, maxlen, seed):
     assert rel_norm(x, xr) < 1e-12
 
 
+def block_profile(sizes, maxlen, seed):
+    """Independent diagonal blocks (no row reaches below its block's first row)
+    of the given sizes, random lower entries within maxlen of the diagonal,
+    diagonal dominant; returns the profile and the symmetric entries."""
+    rng = np.random.default_rng(seed)
+    n = int(sum(sizes))
+    first = np.repeat(np.cumsum([0] + list(sizes[:-1])), sizes)
+    i = np.arange(n)
+    lens = np.minimum(rng.integers(1, maxlen + 1, size=n), i - first + 1)
+    R = np.zeros(n + 1, np.int32)
+    R[1:] = np.cumsum(lens)
+    rows = np.repeat(i, lens)
+    cols = rows - lens[rows] + 1 + (np.arange(R[-1]) - np.repeat(R[:-1], lens))
+    vals = rng.standard_normal(R[-1])
+    off = cols < rows
+    absrow = np.bincount(rows[off], np.abs(vals[off]), n) + np.bincount(cols[off], np.abs(vals[off]), n)
+    vals[~off] = absrow + 1.0  # the diagonal entry, last in each row
+    return vals, R, rows, cols, n
+
+
+@pytest.mark.parametrize("sizes", [(16484, 20000, 16421)])
+def test_getSparseLDLFact_unequal_blocks(sizes):
+    """Three independent blocks of unequal length (ragged last tiles): the
+    factor's later panel indices find no panel in the shorter blocks, and
+    the sweeps' chains end at different positions."""
+    lt, R, rows, cols, n = block_profile(sizes, 90, 7)
+    print("blocks", sizes, "n", n, "profile", R[-1], flush=True)
+    F = A.getSparseLDLFact(lt, R, n)
+    Fr, bad = O.ldl_factor(lt, R, n)
+    assert bad == 0
+    assert entry_rel(F, Fr, floor=1e-8) < 1e-10
+    b = np.random.default_rng(8).standard_normal(n)
+    x, z = A.solveSparseLDLFact(F, R, b)
+    xr = O.ldl_solve(Fr, R, b)
+    assert rel_norm(x, xr) < 1e-12
+    off = cols < rows
+    Ax = np.bincount(rows, lt * x[cols], n) + np.bincount(cols[off], lt[off] * x[rows[off]], n)
+    assert np.linalg.norm(Ax - b) / np.linalg.norm(b) < 1e-12
+
+
 def test_getSparseLDLFact_zero_pivot_status():
     R = np.array([0, 1, 3], np.int32)
     with pytest.raises(A.AstroError) as e:
