@@ -925,6 +925,10 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
     int kLo = max(winMin[r0], P.smin), kHi = P.p;
     if (kMode == LDL_K_OLD) kHi = P.p - 64;
     if (kMode == LDL_K_NEW) kLo = max(kLo, P.p - 64);
+    // start on a column = P.p (mod 16): with rows laid out aligned per block
+    // every 32-column chunk is two whole 128-byte lines (the columns added
+    // lie left of every row's first stored column and are masked)
+    kLo = P.p - ((P.p - kLo + 15) / 16) * 16;
     const int total = max(0, kHi - kLo);
     int chunk = (total + nSplit - 1) / nSplit;
     chunk = (chunk + KC - 1) / KC * KC;

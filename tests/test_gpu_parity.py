@@ -170,6 +170,32 @@ def test_getSparseLDLFact_unequal_blocks(sizes):
     assert np.linalg.norm(Ax - b) / np.linalg.norm(b) < 1e-12
 
 
+def test_aligned_profile_layout_roundtrip():
+    """The context keeps its profile with rows padded to 128-byte lines per
+    diagonal block (three blocks of 19,683 rows here, starting at rows that
+    are not multiples of 16).  Reading it back gives the packed profile, and
+    writing a packed profile back and factoring it gives, bit for bit, the
+    factor the packed layer-1 routine computes from the same array."""
+    m = cantilever_block(26, 26, 26)
+    ctx = A.Context(m)
+    ctx.assemble()
+    ctx.scale_mpc()
+    ctx.build_profile()
+    ctx.sync()
+    R = ctx.geti("elMatLTRange")
+    n = len(R) - 1
+    print("aligned layout n", n, "profile", R[-1], flush=True)
+    P = ctx.getd("elMatLT")
+    ctx.setd("elMatLT", P)
+    np.testing.assert_array_equal(ctx.getd("elMatLT"), P)
+    ctx.factor()
+    ctx.sync()
+    F_ctx = ctx.getd("elMatLT")
+    ctx.close()
+    F_l1 = A.getSparseLDLFact(P, R, n)
+    np.testing.assert_array_equal(F_ctx, F_l1)
+
+
 def test_getSparseLDLFact_zero_pivot_status():
     R = np.array([0, 1, 3], np.int32)
     with pytest.raises(A.AstroError) as e:
