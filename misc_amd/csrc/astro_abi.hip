@@ -433,7 +433,7 @@ struct DeviceProfile {
             evKind.push_back(0);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
-        hipLaunchKernelGGL(k_sweep_forward, dim3(numTiles), dim3(256), 0, s, fwdTiles.p, numTiles, lt.p, rowoff.p,
+        hipLaunchKernelGGL(k_sweep_forward, dim3(numTiles), dim3(256), SWEEP_LDS_RESERVE, s, fwdTiles.p, numTiles, lt.p, rowoff.p,
                            sfirst.p, blocks.p, b, z.p, gnear.p, fwdFlags.p, ticket.p, fwdEpoch, spinLimit, timeout.p);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         ++bwdEpoch;
@@ -443,7 +443,7 @@ struct DeviceProfile {
             evKind.push_back(1);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
-        hipLaunchKernelGGL(k_sweep_backward, dim3(numTiles), dim3(256), 0, s, bwdTiles.p, numTiles, lt.p,
+        hipLaunchKernelGGL(k_sweep_backward, dim3(numTiles), dim3(256), SWEEP_LDS_RESERVE, s, bwdTiles.p, numTiles, lt.p,
                            rowoff.p, sfirst.p, blocks.p, z.p, x, gnear.p, bwdFlags.p, ticket.p, bwdEpoch, spinLimit,
                            timeout.p);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
