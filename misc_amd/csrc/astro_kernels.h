@@ -50,13 +50,6 @@ enum {
     SWEEP_KINDS = 10, SWEEP_BLOCK = 64 * 64
 };
 
-// Dynamic LDS (bytes, unused by the code) requested with every sweep launch
-// so that a sweep workgroup (about 71 KB of static LDS) holds its CU alone.
-// With two per CU, a tile waiting on its neighbour's values shares the CU's
-// memory pipeline with another tile streaming its far rows, and every poll
-// waits behind that tile's loads: one per CU took the sweeps on the 64x64x32
-// brick from 3.64 / 4.15 ms to 3.37 / 3.58 ms (forward / backward).
-constexpr unsigned SWEEP_LDS_RESERVE = 16384;
 
 __global__ void k_brick8_tangent(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                  const double* elScale, const double* matC, const double* elRho, double massFactor,
