@@ -1660,7 +1660,7 @@ __device__ __forceinline__ bool lookahead_tail(int r0, int nr, int nNear, int fl
 // column) and of the tiles four and three back (their coupling entries
 // prefetched at the start), partials reduced across lanes once; the tail
 // combines the near tiles through the precomputed blocks.
-__global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
+__global__ __launch_bounds__(256, 1) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
                                                           const double* __restrict__ lt,
                                                           const int64_t* __restrict__ rowoff,
                                                           const int* __restrict__ sfirst,
@@ -1743,18 +1743,32 @@ __global__ __launch_bounds__(256, 2) void k_sweep_forward(const SweepTile* __res
                 // bulk: every column of the published prefix of far tiles
                 const int pend = first_pending_up(flags, J, bEnd, epoch);
                 const int cStop = r0 - 64 * (T.blk - pend);
+                // two 64-column chunks per step, so 32 row loads are in
+                // flight (one workgroup per CU leaves four waves to cover the
+                // memory latency); a lone last chunk reloads the first one
+                // and adds it times zero (profile entries are finite)
 #pragma unroll 1
-                for (int c = c0 + lane; c < cStop; c += 64) {
+                for (int cb = c0; cb < cStop; cb += 128) {
+                    const int c = cb + lane;
+                    const bool two = cb + 128 <= cStop;
+                    const int c2 = two ? c + 64 : c;
                     const double zc = coherent_load(&z[c]);
+                    const double zc2 = two ? coherent_load(&z[c2]) : 0.0;
                     // unconditional loads (a column left of a row's first
                     // stored column addresses earlier rows' entries, still
                     // inside the profile), then a select: no per-row branch
                     // and wait
-                    double lv[16];
+                    double lv[16], lv2[16];
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) lv[r] = ltw[(unsigned int)(rd[r] + c)];
+                    for (int r = 0; r < 16; ++r) {
+                        lv[r] = ltw[(unsigned int)(rd[r] + c)];
+                        lv2[r] = ltw[(unsigned int)(rd[r] + c2)];
+                    }
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) p[r] += (c >= rs[r] ? lv[r] : 0.0) * zc;
+                    for (int r = 0; r < 16; ++r) {
+                        p[r] += (c >= rs[r] ? lv[r] : 0.0) * zc;
+                        p[r] += (c2 >= rs[r] ? lv2[r] : 0.0) * zc2;
+                    }
                 }
                 c0 = cStop;
                 J = pend;
