@@ -162,10 +162,15 @@ double astro_ctx_scalar_f(const astro_ctx* ctx, const char* name);
  * "thermMatRange", "thermMatCols", "thermMatLTRange"; double arrays
  * "elasticMat", "elMatLT", "elMPCMat", "elMPCRHS", "elasticLoad", "nodeDisp",
  * "thermMat", "thermMatLT", "thermalLoad", "nodeTemp".  count is the element
- * count of the host buffer; it must equal the array length. */
+ * count of the host buffer; it must equal the array length.  "elMatLT" and
+ * "thermMatLT" (profile, or factor after astro_ctx_factor) cross in the
+ * reference's packed row-skyline layout of elMatLTRange / thermMatLTRange;
+ * on the device their rows are padded to 128-byte lines, and the copy
+ * repacks them on the host. */
 int astro_ctx_get_i32(astro_ctx* ctx, const char* name, int32_t* host, int64_t count);
 int astro_ctx_get_f64(astro_ctx* ctx, const char* name, double* host, int64_t count);
-/* Overwrite a double array of the context from host memory. */
+/* Overwrite a double array of the context from host memory (profiles in the
+ * packed layout, as above). */
 int astro_ctx_set_f64(astro_ctx* ctx, const char* name, const double* host, int64_t count);
 
 /* Elastic hot path, all on the device, asynchronous on the context stream
