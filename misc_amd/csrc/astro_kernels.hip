@@ -2059,8 +2059,15 @@ __global__ __launch_bounds__(256) void k_spmv(const double* __restrict__ val, co
     const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     const int l = threadIdx.x & 31;
     if (row >= n) return;
-    double s = 0.0;
-    for (int q = range[row] + l; q < range[row + 1]; q += 32) {
+    // a brick row holds at most 81 slots: each lane's first three slots are
+    // loaded together (columns, values, then x), any further ones in a loop
+    const int e = range[row + 1];
+    const int q0 = range[row] + l, q1 = q0 + 32, q2 = q0 + 64;
+    const int c0 = q0 < e ? cols[q0] : 0, c1 = q1 < e ? cols[q1] : 0, c2 = q2 < e ? cols[q2] : 0;
+    const double v0 = q0 < e ? val[q0] : 0.0, v1 = q1 < e ? val[q1] : 0.0, v2 = q2 < e ? val[q2] : 0.0;
+    const double x0 = c0 ? x[c0 - 1] : 0.0, x1 = c1 ? x[c1 - 1] : 0.0, x2 = c2 ? x[c2 - 1] : 0.0;
+    double s = (c0 ? v0 * x0 : 0.0) + (c1 ? v1 * x1 : 0.0) + (c2 ? v2 * x2 : 0.0);
+    for (int q = q0 + 96; q < e; q += 32) {
         const int c = cols[q];
         if (c) s += val[q] * x[c - 1];
     }
