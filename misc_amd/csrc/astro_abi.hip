@@ -408,7 +408,7 @@ struct DeviceProfile {
         const size_t bufSize = (size_t)(splitK + 1) * nBlocks * scratchRows * 64;
         auto buf = [&](int q) { return scratch.p + (size_t)(q & 1) * bufSize; };
         const dim3 gOld(maxTiles * splitK, nBlocks), gLead(nBlocks * maxTiles * (splitK + 1));
-        const dim3 gRows(nBlocks * std::max(1, maxTiles - 1));
+        const dim3 gRows(2 * nBlocks * std::max(1, maxTiles - 1));
         // panel 0 has no old part: its slots are written as zeros
         hipLaunchKernelGGL(k_ldl_update_mfma, gOld, dim3(256), 0, s, panels.p, 0, panelsPerBlock, splitK, LDL_K_OLD, 0,
                            lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(0), scratchRows);

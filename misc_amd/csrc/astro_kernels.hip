@@ -1038,7 +1038,7 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
                     scratch, scratchRows, smem);
 }
 
-// Step FT of a panel P, in two parts.  Thread (r = tid / 4, q = tid % 4)
+// Step FT of a panel P, in two parts.  In the diagonal part, thread (r = tid / 4, q = tid % 4)
 // holds 16 entries of one row in registers: columns q, q+4, ..., q+60 of the
 // panel.  gather: A(row, P) minus the split-K partial sums of step U; loads
 // are issued at indices clamped into the row's stored range and masked after
@@ -1046,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
 //  - ldl_panel_diag (one workgroup per block): factors the 64x64 diagonal
 //    block right-looking in blocks of four columns, writes it back with D in
 //    `diag`, and stores X = L_PP^-1 and 1 / D in Xg for the row tiles;
-//  - ldl_panel_rows (one workgroup per further 64-row tile, in the next
+//  - ldl_panel_rows (two workgroups per further 64-row tile, in the next
 //    launch): L(i, P) = rows X^T D^-1 on the matrix cores.
 __device__ __forceinline__ void ldl_panel_diag(const PanelInfo P, int blk, int nb, int nSlots,
                                                double* __restrict__ lt, const int64_t* __restrict__ rowoff,
@@ -1189,58 +1189,45 @@ __device__ __forceinline__ void ldl_panel_diag(const PanelInfo P, int blk, int n
     if (tid < 64) Xg[64 * 64 + tid] = rD[tid];
 }
 
-__device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int nb, int tile, int nSlots,
-                                               double* __restrict__ lt, const int64_t* __restrict__ rowoff,
-                                               const int* __restrict__ sfirst, const double* __restrict__ scratch,
-                                               int scratchRows, const double* __restrict__ Xg, char* smem) {
-    const int r0 = P.p + 64 * tile;
+// Rows [32 half, 32 half + 32) of a further row tile (half 0 or 1, one
+// workgroup each: twice the workgroups of one per tile, each with half the
+// gather and product; factor 179.6 -> 177.1 ms on the 64x64x32 brick).
+// Thread (r = tid / 8, q = tid % 8) gathers 8 entries of one row (columns q,
+// q+8, ..., q+56); wave w forms the 16 x 32 product block at rows 16 (w & 1),
+// columns 32 (w >> 1).
+__device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int nb, int tile, int half, int nSlots,
+                                                    double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                                    const int* __restrict__ sfirst, const double* __restrict__ scratch,
+                                                    int scratchRows, const double* __restrict__ Xg, char* smem) {
+    const int r0 = P.p + 64 * tile + 32 * half;
     if (r0 >= P.iend) return;
-    const int nr = min(64, P.iend - r0);
+    const int nr = min(32, P.iend - r0);
     const int np = P.pe - P.p;
-    double(*Lc)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_A);  // this tile's rows
-    double(*Xi)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_X);  // L_PP^-1 (unit lower)
+    double(*Lc)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_A);
+    double(*Xi)[65] = reinterpret_cast<double(*)[65]>(smem + LDL_FT_X);
     double* rD = reinterpret_cast<double*>(smem + LDL_FT_RD);
     int64_t* rowOff = reinterpret_cast<int64_t*>(smem + LDL_FT_OFF);
     int* rowS = reinterpret_cast<int*>(smem + LDL_FT_SI);
-    const int tid = threadIdx.x, fr = tid >> 2, fq = tid & 3;
+    const int tid = threadIdx.x, fr = tid >> 3, fq = tid & 7;
     const size_t qstride = (size_t)nb * scratchRows * 64;
     const int lane = tid & 63, w = tid >> 6;
-    // gather row `rowInPanel` of the updated panel block; entry m is column
-    // k = fq + 4 m of the panel, kept where live && k < kmax && inside the profile
-    auto gather = [&](int gi, int rowInPanel, bool live, int kmax, double (&a)[16], int& si, int64_t& off) {
-        const int gr = live ? gi : P.p;
-        si = sfirst[gr];
-        off = rowoff[gr];
-        const int rp = live ? rowInPanel : 0;
-        double raw[16], ws[16];
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const int c = P.p + fq + 4 * m;
-            raw[m] = lt[off + min(max(c, si), gr)];
-            ws[m] = 0.0;
-        }
-        const double* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
-        for (int q = 0; q < nSlots; ++q) {
-#pragma unroll
-            for (int m = 0; m < 16; ++m) ws[m] += sp[(size_t)q * qstride + 4 * m];
-        }
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const int k = fq + 4 * m;
-            a[m] = (live && k < kmax && P.p + k >= si) ? raw[m] - ws[m] : 0.0;
-        }
-    };
-
-    // rows r0 + fr of this tile into LDS, then L(i, P) = Y(i, :) D^-1 with
-    // Y = rows X^T on the matrix cores
-    const int gi = r0 + fr;
     const bool live = fr < nr;
-    double y[16];
-    int si;
-    int64_t off;
-    gather(gi, r0 - P.p + fr, live, np, y, si, off);
-    // X and 1 / D of the diagonal block (written by the lead launch), loaded
-    // while the row gather is in flight
+    const int gr = live ? r0 + fr : P.p;
+    const int si = sfirst[gr];
+    const int64_t off = rowoff[gr];
+    const int rp = live ? (r0 - P.p) + fr : 0;
+    double raw[8], ws[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int c = P.p + fq + 8 * m;
+        raw[m] = lt[off + min(max(c, si), gr)];
+        ws[m] = 0.0;
+    }
+    const double* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
+    for (int q = 0; q < nSlots; ++q) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) ws[m] += sp[(size_t)q * qstride + 8 * m];
+    }
     for (int id = tid; id < 64 * 64; id += 256) Xi[id >> 6][id & 63] = Xg[id];
     if (tid < 64) rD[tid] = Xg[64 * 64 + tid];
     if (fq == 0) {
@@ -1248,30 +1235,34 @@ __device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int n
         rowS[fr] = live ? si : 0x7fffffff;
     }
 #pragma unroll
-    for (int m = 0; m < 16; ++m) Lc[fr][fq + 4 * m] = y[m];
+    for (int m = 0; m < 8; ++m) {
+        const int k = fq + 8 * m;
+        Lc[fr][k] = (live && k < np && P.p + k >= si) ? raw[m] - ws[m] : 0.0;
+    }
     __syncthreads();
-    double4_t acc[4];
+    const int rb = w & 1, cb0 = 2 * (w >> 1);
+    double4_t acc[2];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < 2; ++q) acc[q] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int ks = 0; ks < 64; ks += 4) {
         const int kk = ks + (lane >> 4);
-        const double av = Lc[w * 16 + (lane & 15)][kk];
+        const double av = Lc[rb * 16 + (lane & 15)][kk];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            const double bv = Xi[cb * 16 + (lane & 15)][kk];
+        for (int cb = 0; cb < 2; ++cb) {
+            const double bv = Xi[(cb0 + cb) * 16 + (lane & 15)][kk];
             acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
         }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int i = w * 16 + 4 * v + (lane >> 4);
+        const int i = rb * 16 + 4 * v + (lane >> 4);
         if (i < nr) {
             const int64_t ro = rowOff[i];
             const int s0 = rowS[i];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                const int j = cb * 16 + (lane & 15);
+            for (int cb = 0; cb < 2; ++cb) {
+                const int j = (cb0 + cb) * 16 + (lane & 15);
                 if (j < np && P.p + j >= s0) lt[ro + P.p + j] = acc[cb][v] * rD[j];
             }
         }
@@ -1335,10 +1326,11 @@ __global__ __launch_bounds__(256) void k_ldl_rows(const PanelInfo* __restrict__ 
                                                   const int* __restrict__ sfirst, const double* __restrict__ bufQ,
                                                   int scratchRows, const double* __restrict__ ftX) {
     __shared__ __align__(16) char smem[LDL_STEP_SMEM_BYTES];
-    const int blk = blockIdx.x / (numTiles - 1), tile = 1 + blockIdx.x % (numTiles - 1);
+    const int half = blockIdx.x & 1, idx = blockIdx.x >> 1;
+    const int blk = idx / (numTiles - 1), tile = 1 + idx % (numTiles - 1);
     const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
-    if (P.p < 0 || P.p + 64 * tile >= P.iend) return;
-    ldl_panel_rows(P, blk, numBlocks, tile, nSplit + 1, lt, rowoff, sfirst, bufQ, scratchRows,
+    if (P.p < 0 || P.p + 64 * tile + 32 * half >= P.iend) return;
+    ldl_panel_rows(P, blk, numBlocks, tile, half, nSplit + 1, lt, rowoff, sfirst, bufQ, scratchRows,
                    ftX + (size_t)blk * (64 * 64 + 64), smem);
 }
 
