@@ -193,8 +193,17 @@ int astro_ctx_solve(astro_ctx* ctx);
 /* Zero delDisp and nodeDisp (the state a fresh job starts from,
  * Working/AStrO_bookKeeping.f90:960, Working/AStrO_commandFunctions.f90:653-679). */
 int astro_ctx_reset(astro_ctx* ctx);
-/* Whole solve() sequence: assemble, scale, profile, factor, solve. */
+/* Whole solve() sequence: assemble, scale, profile, factor, solve.  The step
+ * right-hand side holds -R(u) of the current nodeDisp (see below), so a second
+ * call from a converged state returns the same displacements. */
 int astro_ctx_static_solve(astro_ctx* ctx);
+/* getElasticSolnLoad(0) on a zeroed load (Working/AStrO_commandFunctions.f90:
+ * 260-351, as the reference's reactionForce output calls it,
+ * Working/AStrO_runJob.f90:228-233): "elasticLoad" = -R(u) for the current
+ * nodeDisp, with R(u) = K u, K the assembled tangent at rest (linear static
+ * model).  Reactions are -elasticLoad at the constrained nodes
+ * (Working/AStrO_output.f90:122-141).  Synchronous. */
+int astro_ctx_elastic_soln_load(astro_ctx* ctx);
 
 /* Newmark dynamics, the dynamic branch of solve()
  * (Working/AStrO_commandFunctions.f90:762-810; element residual
@@ -228,9 +237,14 @@ int astro_ctx_compliance_gradient(astro_ctx* ctx, double* grad);
  * 763-769). */
 int astro_ctx_assemble_thermal(astro_ctx* ctx);
 int astro_ctx_thermal_solve(astro_ctx* ctx);
+/* getThermalSolnLoad(0) on a zeroed load (Working/AStrO_commandFunctions.f90:16-74):
+ * "thermalLoad" = -K T for the current nodeTemp (reactionHeatGen is
+ * -thermalLoad).  Synchronous; needs astro_ctx_assemble_thermal first. */
+int astro_ctx_thermal_soln_load(astro_ctx* ctx);
 
 /* Preconditioner apply on device vectors of length elMatDim:
- * x = P^-1 b via the two triangular sweeps. */
+ * x = P^-1 b via the two triangular sweeps.  b and x must be distinct buffers
+ * (ASTRO_ERR_ARG otherwise). */
 int astro_ctx_precond_apply(astro_ctx* ctx, const double* d_b, double* d_x);
 /* Operator apply y = (A + C^T C) x on device vectors. */
 int astro_ctx_operator_apply(astro_ctx* ctx, const double* d_x, double* d_y);
@@ -242,8 +256,9 @@ int astro_ctx_operator_apply(astro_ctx* ctx, const double* d_x, double* d_y);
 int astro_ctx_set_timing(astro_ctx* ctx, int on);
 double astro_ctx_kernel_stat(const astro_ctx* ctx, const char* name);
 
-/* |b - (A + C^T C) u| / |b| for the current elastic load b and nodeDisp u,
- * computed on the device (synchronous). */
+/* |b - (A + C^T C) u| / |b| of the whole static system, b = nodal loads +
+ * C^T rhs, at the current nodeDisp u; the residual is formed in double-double
+ * on the device (synchronous). */
 int astro_ctx_true_residual(astro_ctx* ctx, double* relRes);
 
 /* Wait for all work queued on the context stream. */

@@ -35,7 +35,7 @@ EXPORTS = [
     "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64", "astro_ctx_set_timing",
     "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual", "astro_ctx_dynamic_init",
     "astro_ctx_dynamic_step", "astro_ctx_dynamic_run", "astro_ctx_set_design", "astro_ctx_dRdD",
-    "astro_ctx_compliance_gradient",
+    "astro_ctx_compliance_gradient", "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load",
 ]
 
 
@@ -96,7 +96,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_assemble_elastic.argtypes = [C.c_void_p, C.c_double]
     for f in ("astro_ctx_scale_mpc", "astro_ctx_build_profile", "astro_ctx_factor", "astro_ctx_solve",
               "astro_ctx_static_solve", "astro_ctx_assemble_thermal", "astro_ctx_thermal_solve", "astro_ctx_sync",
-              "astro_ctx_reset"):
+              "astro_ctx_reset", "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load"):
         getattr(L, f).argtypes = [C.c_void_p]
     L.astro_ctx_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.astro_ctx_dynamic_step.argtypes = [C.c_void_p, C.c_double]
@@ -158,10 +158,12 @@ def solveSparseLDLFact(Lmat, LRange, rhs):
 
 
 def gMRes(xVec, Amat, Acols, ARange, Cmat, Ccols, CRange, Pmat, PRange, bVec, numVecs, maxIt, rtol=1e-12,
-          atol=0.0):
+          atol=0.0, allow_no_converge=False):
     """Preconditioned GMRES(numVecs) (Working/AStrO_solvers.f90:128-297).  xVec is
     the initial guess; returns the solution.  rtol <= 0 and atol <= 0 select the
-    reference's absolute 1e-12 stop."""
+    reference's absolute 1e-12 stop.  allow_no_converge: return the last
+    iterate when the maxIt/numVecs + 1 restart cycles run out (the reference
+    returns it without a word) instead of raising."""
     x = _f64(xVec).copy()
     A, Ac, Ar = _f64(Amat), _i(Acols), _i(ARange)
     cdim = len(CRange) - 1 if CRange is not None and len(CRange) > 1 else 0
@@ -169,8 +171,10 @@ def gMRes(xVec, Amat, Acols, ARange, Cmat, Ccols, CRange, Pmat, PRange, bVec, nu
     Cc = _i(Ccols) if cdim else np.zeros(1, np.int32)
     Cr = _i(CRange) if cdim else np.zeros(1, np.int32)
     P = _f64(Pmat)
-    _check(load_library().astro_gMRes(x, A, Ac, Ar, A.size, x.size, Cm, Cc, Cr, Cm.size, cdim, P, _i(PRange), P.size,
-                                      _f64(bVec), int(numVecs), int(maxIt), float(rtol), float(atol)))
+    rc = load_library().astro_gMRes(x, A, Ac, Ar, A.size, x.size, Cm, Cc, Cr, Cm.size, cdim, P, _i(PRange), P.size,
+                                    _f64(bVec), int(numVecs), int(maxIt), float(rtol), float(atol))
+    if not (allow_no_converge and rc == 4):
+        _check(rc)
     return x
 
 
@@ -368,6 +372,43 @@ class Context:
 
     def static_solve(self):
         _check(load_library().astro_ctx_static_solve(self.h))
+
+    def elastic_soln_load(self):
+        """getElasticSolnLoad(0) on a zeroed load: "elasticLoad" = -R(u) of the
+        current nodeDisp (Working/AStrO_commandFunctions.f90:260-351)."""
+        _check(load_library().astro_ctx_elastic_soln_load(self.h))
+
+    def thermal_soln_load(self):
+        """getThermalSolnLoad(0) on a zeroed load: "thermalLoad" = -K T."""
+        _check(load_library().astro_ctx_thermal_soln_load(self.h))
+
+    def node_values(self, name, nodes):
+        """Values of a DOF vector ("nodeDisp", "elasticLoad", ...) at the given
+        1-based nodes as (len(nodes), 6), 0 where a node has no such DOF, the
+        layout of writeNodeResults (Working/AStrO_output.f90:60-141)."""
+        v = self.getd(name)
+        nd = self.geti("nDofIndex").reshape(-1, 6)
+        out = np.zeros((len(nodes), 6))
+        for r, node in enumerate(nodes):
+            for c in range(6):
+                d = nd[node - 1, c]
+                if d:
+                    out[r, c] = v[d - 1]
+        return out
+
+    def reaction_force(self, nodes):
+        """writeNodeResults' reactionForce field at `nodes`: -elasticLoad after
+        getElasticSolnLoad(0) (Working/AStrO_runJob.f90:228-233,
+        Working/AStrO_output.f90:122-141)."""
+        self.elastic_soln_load()
+        return -self.node_values("elasticLoad", nodes)
+
+    def displacement_objective(self, nodes, component, coef=1.0, exponent=1.0, target=0.0):
+        """The 'displacement' objective term: coef * sum over the node set of
+        (u_component - target)^exponent (Working/AStrO_objective.f90:147-158);
+        component is 1-based."""
+        u = self.node_values("nodeDisp", nodes)[:, component - 1]
+        return float(coef * np.sum((u - target) ** exponent))
 
     def dynamic_init(self, beta: float = 0.25, gamma: float = 0.5, dt: float = 1e-5):
         """Newmark set-up: tangent K + M/(beta dt^2), penalty scaling, profile,

@@ -473,12 +473,57 @@ struct DeviceProfile {
         return ASTRO_OK;
     }
 
+    // Reports (and clears, so that later calls on the context start clean) a
+    // bounded sweep wait that expired since the last check.
     int check_timeout(hipStream_t s) {
         int t = 0;
         HIP_TRY(hipMemcpyAsync(&t, timeout.p, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (timing) collect();
-        if (t) return fail(ASTRO_ERR_TIMEOUT, "triangular sweep wait exceeded its spin limit");
+        if (t) {
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, timeout.p, (int64_t)1, 0);
+            HIP_TRY(hipStreamSynchronize(s));
+            return fail(ASTRO_ERR_TIMEOUT, "triangular sweep wait exceeded its spin limit");
+        }
+        return ASTRO_OK;
+    }
+};
+
+// The constraint matrix by column: for DOF i (0-based), entries k in
+// [range[i], range[i + 1]) name the position pos[k] (0-based) of one of its
+// coefficients in the constraint arrays and that coefficient's row row[k].
+struct MpcTranspose {
+    std::vector<int> range, pos, row;
+    void build(const int* crange, const int* ccols, int cdim, int n) {
+        range.assign(n + 1, 0);
+        for (int r = 0; r < cdim; ++r)
+            for (int q = crange[r]; q < crange[r + 1]; ++q)
+                if (ccols[q] >= 1 && ccols[q] <= n) ++range[ccols[q]];
+        for (int i = 0; i < n; ++i) range[i + 1] += range[i];
+        pos.assign(range[n], 0);
+        row.assign(range[n], 0);
+        std::vector<int> fill(range.begin(), range.end() - 1);
+        for (int r = 0; r < cdim; ++r)
+            for (int q = crange[r]; q < crange[r + 1]; ++q)
+                if (ccols[q] >= 1 && ccols[q] <= n) {
+                    const int k = fill[ccols[q] - 1]++;
+                    pos[k] = q;
+                    row[k] = r;
+                }
+    }
+};
+
+// Device copy of an MpcTranspose plus the double-double C x buffer.
+struct DeviceMpcT {
+    DBuf<int> range, pos, row;
+    DBuf<double> t;
+    int upload(const int* crange, const int* ccols, int cdim, int n, hipStream_t s) {
+        MpcTranspose T;
+        T.build(crange, ccols, cdim, n);
+        ASTRO_TRY(range.upload(T.range, s));
+        ASTRO_TRY(pos.upload(T.pos, s));
+        ASTRO_TRY(row.upload(T.row, s));
+        ASTRO_TRY(t.alloc(2 * (size_t)std::max(cdim, 1)));
         return ASTRO_OK;
     }
 };
@@ -492,12 +537,35 @@ struct DeviceOperator {
     const double* cmat = nullptr;
     const int* ccols = nullptr;
     const int* crange = nullptr;
+    // constraint transpose and C x scratch for true residuals (set when cdim > 0)
+    const int* ctRange = nullptr;
+    const int* ctPos = nullptr;
+    const int* ctRow = nullptr;
+    double* ctT = nullptr;
+    void set_transpose(DeviceMpcT& T) {
+        ctRange = T.range.p;
+        ctPos = T.pos.p;
+        ctRow = T.row.p;
+        ctT = T.t.p;
+    }
     int apply(const double* x, double* y, hipStream_t s) const {
         hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val, cols,
                            range, n, x, y);
         if (cdim > 0)
             hipLaunchKernelGGL(k_mpc_apply, dim3(grid_for(cdim, 256)), dim3(256), 0, s, cmat, ccols, crange, cdim, x,
                                y);
+        HIP_TRY(hipGetLastError());
+        return ASTRO_OK;
+    }
+    // r = b - (A x + C^T C x), carried in double-double and rounded once.
+    int residual(const double* b, const double* x, double* r, hipStream_t s) const {
+        const bool pen = cdim > 0;
+        if (pen && !ctRange) return fail(ASTRO_ERR_ARG, "operator has constraints but no transpose");
+        if (pen)
+            hipLaunchKernelGGL(k_mpc_dd, dim3(grid_for(cdim, 256)), dim3(256), 0, s, cmat, ccols, crange, cdim, x,
+                               ctT);
+        hipLaunchKernelGGL(k_residual_dd, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val, cols,
+                           range, n, pen ? ctRange : nullptr, ctPos, ctRow, cmat, ctT, b, x, r);
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
     }
@@ -508,7 +576,9 @@ struct DeviceOperator {
 // Gram-Schmidt: the same Arnoldi relation, two multi-vector passes per step
 // instead of j dependent dot/axpy pairs.  Standard Givens rotations and a
 // beta*e1 right-hand side; a cycle ends when the rotated residual estimate
-// meets the target.  Working/AStrO_solvers.f90:128-297 is the reference form.
+// meets the target.  The residual b - A x that starts each cycle is formed
+// in double-double (DeviceOperator::residual).  Working/AStrO_solvers.f90:128-297
+// is the reference form.
 struct Gmres {
     int n = 0, m = 0;
     DBuf<double> V, t, t2, r, partial, hdev;
@@ -545,9 +615,10 @@ struct Gmres {
         steps = cycles = 0;
         const char* ev = std::getenv("ASTRO_VERBOSE");
         const bool verbose = ev && ev[0] == '1';
+        // the true residual at every restart is exact to double-double, so the
+        // restarts refine the iterate past the FP64 residual floor
         auto residual = [&](double& out) -> int {
-            ASTRO_TRY(A.apply(x, t.p, s));
-            hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b, t.p, t2.p, n, 1.0, -1.0);
+            ASTRO_TRY(A.residual(b, x, t2.p, s));
             ASTRO_TRY(P.apply(t2.p, r.p, s));
             multidot(r.p, 1, r.p, hdev.p, s);
             HIP_TRY(hipMemcpyAsync(hHost, hdev.p, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -702,12 +773,14 @@ struct astro_ctx {
     DBuf<unsigned long long> maxBits;
     DeviceProfile prof;
     DeviceOperator op;
+    DeviceMpcT mpcT;
     Gmres gm;
     // thermal system
     DBuf<int> tRange, tCols, tMpcRange, tMpcCols;
     DBuf<double> tVal, tMpcMat, tMpcRhs, tLoad, tDel, nodeTemp;
     DeviceProfile tprof;
     DeviceOperator top;
+    DeviceMpcT tMpcT;
     Gmres tgm;
     bool thermalReady = false;
     // Newmark state
@@ -848,6 +921,8 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
     c->op.cmat = c->mpcMat.p;
     c->op.ccols = c->mpcCols.p;
     c->op.crange = c->mpcRange.p;
+    ASTRO_TRY(c->mpcT.upload(p.elMPCMatRange.data(), p.elMPCMatCols.data(), p.elMPCDim, n, s));
+    c->op.set_transpose(c->mpcT);
     ASTRO_TRY(c->gm.setup(n, c->numVecs));
     return ASTRO_OK;
 }
@@ -1150,9 +1225,23 @@ int astro_ctx_factor(astro_ctx* c) {
     return c->prof.factor(c->s);
 }
 
+// load = -R(u) for the linear static model, R(u) = K u with K the tangent at
+// rest in the CSR: getElasticSolnLoad(0) (Working/AStrO_commandFunctions.f90:
+// 260-351) on a zeroed load.  The reference evaluates Green-Lagrange strains
+// there even without nLGeom (D20); the linear model's internal force is K u.
+static void ctx_minus_internal_force(hipStream_t s, const DBuf<double>& val, const DBuf<int>& cols,
+                                     const DBuf<int>& range, int n, const double* u, double* load) {
+    hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val.p, cols.p,
+                       range.p, n, u, load);
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, load, load, load, n, -1.0, 0.0);
+}
+
+// Right-hand side of a static step (stepSolve, Working/AStrO_commandFunctions.f90:
+// 716-723): -R(u) + nodal loads + C^T (rhs - C u).  With the internal force in
+// it, a second solve from a converged state leaves the state unchanged.
 static int ctx_rhs(astro_ctx* c) {
     const int n = c->prep.elMatDim;
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->load.p, (int64_t)n, 0.0);
+    ctx_minus_internal_force(c->s, c->csrVal, c->csrCols, c->csrRange, n, c->nodeDisp.p, c->load.p);
     const int nl = (int)c->loadDofs.n;
     if (nl > 0)
         hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
@@ -1183,6 +1272,14 @@ int astro_ctx_reset(astro_ctx* c) {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->delDisp.p, (int64_t)n, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->nodeDisp.p, (int64_t)n, 0.0);
     HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+int astro_ctx_elastic_soln_load(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    ctx_minus_internal_force(c->s, c->csrVal, c->csrCols, c->csrRange, c->prep.elMatDim, c->nodeDisp.p, c->load.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
 }
 
@@ -1394,6 +1491,8 @@ static int ctx_thermal_setup(astro_ctx* c) {
     c->top.cmat = c->tMpcMat.p;
     c->top.ccols = c->tMpcCols.p;
     c->top.crange = c->tMpcRange.p;
+    ASTRO_TRY(c->tMpcT.upload(p.thermMPCMatRange.data(), p.thermMPCMatCols.data(), p.thermMPCDim, N, s));
+    c->top.set_transpose(c->tMpcT);
     int nv = c->opt.numVecs > 0 ? c->opt.numVecs : (N < 200 ? N / 2 : 100);
     ASTRO_TRY(c->tgm.setup(N, nv));
     c->thermalReady = true;
@@ -1440,7 +1539,7 @@ int astro_ctx_thermal_solve(astro_ctx* c) {
                            c->tMpcCols.p, c->tMpcRange.p, cd, P.sfirst.p, P.rowoff.p, P.lt.p);
     HIP_TRY(hipGetLastError());
     ASTRO_TRY(P.factor(s));
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->tLoad.p, (int64_t)N, 0.0);
+    ctx_minus_internal_force(s, c->tVal, c->tCols, c->tRange, N, c->nodeTemp.p, c->tLoad.p);
     if (cd > 0)
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p, c->tMpcCols.p,
                            c->tMpcRange.p, c->tMpcRhs.p, cd, c->nodeTemp.p, c->tLoad.p);
@@ -1452,8 +1551,19 @@ int astro_ctx_thermal_solve(astro_ctx* c) {
     return ASTRO_OK;
 }
 
+int astro_ctx_thermal_soln_load(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (!c->thermalReady) return fail(ASTRO_ERR_ARG, "the thermal system has not been assembled");
+    ctx_minus_internal_force(c->s, c->tVal, c->tCols, c->tRange, c->prep.numNodes, c->nodeTemp.p, c->tLoad.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
 int astro_ctx_precond_apply(astro_ctx* c, const double* d_b, double* d_x) {
     if (!c || !d_b || !d_x) return fail(ASTRO_ERR_ARG, "null argument");
+    // the sweeps mark x unproduced before the forward sweep reads b
+    if (d_b == d_x) return fail(ASTRO_ERR_ARG, "precond_apply needs distinct b and x buffers");
     ASTRO_TRY(c->prof.apply(d_b, d_x, c->s));
     return c->prof.check_timeout(c->s);
 }
@@ -1493,11 +1603,18 @@ int astro_ctx_true_residual(astro_ctx* c, double* relRes) {
     if (!c || !relRes) return fail(ASTRO_ERR_ARG, "null argument");
     const int n = c->prep.elMatDim;
     Gmres& g = c->gm;
-    ASTRO_TRY(c->op.apply(c->nodeDisp.p, g.t.p, c->s));
-    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->load.p, g.t.p, g.t2.p, n, 1.0,
-                       -1.0);
+    // b = nodal loads + C^T rhs, the right-hand side of the whole system
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, g.t.p, (int64_t)n, 0.0);
+    const int nl = (int)c->loadDofs.n, cd = c->prep.elMPCDim;
+    if (nl > 0)
+        hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
+                           nl, 1.0, g.t.p);
+    if (cd > 0)
+        hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
+                           c->mpcRange.p, c->mpcRhs.p, cd, (const double*)nullptr, g.t.p);
+    ASTRO_TRY(c->op.residual(g.t.p, c->nodeDisp.p, g.t2.p, c->s));
     g.multidot(g.t2.p, 1, g.t2.p, g.hdev.p, c->s);
-    g.multidot(c->load.p, 1, c->load.p, g.hdev.p + 1, c->s);
+    g.multidot(g.t.p, 1, g.t.p, g.hdev.p + 1, c->s);
     HIP_TRY(hipMemcpyAsync(g.hHost, g.hdev.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIP_TRY(hipStreamSynchronize(c->s));
     *relRes = std::sqrt(g.hHost[0]) / std::sqrt(g.hHost[1]);
@@ -1598,6 +1715,7 @@ int astro_gMRes(double* xVec, const double* Amat, const int32_t* Acols, const in
         DeviceProfile P;
         DBuf<double> av, cv, b, x;
         DBuf<int> ac, ar, cc, cr;
+        DeviceMpcT ct;
         Gmres gm;
         DeviceOperator op;
         rc = P.setup(PRange, ADim, s);
@@ -1609,6 +1727,7 @@ int astro_gMRes(double* xVec, const double* Amat, const int32_t* Acols, const in
         if (rc == ASTRO_OK && CDim > 0) rc = cv.upload(Cmat, CRange[CDim], s);
         if (rc == ASTRO_OK && CDim > 0) rc = cc.upload(Ccols, CRange[CDim], s);
         if (rc == ASTRO_OK && CDim > 0) rc = cr.upload(CRange, CDim + 1, s);
+        if (rc == ASTRO_OK && CDim > 0) rc = ct.upload(CRange, Ccols, CDim, ADim, s);
         if (rc == ASTRO_OK) rc = b.upload(bVec, ADim, s);
         if (rc == ASTRO_OK) rc = x.upload(xVec, ADim, s);
         if (rc == ASTRO_OK) rc = gm.setup(ADim, numVecs);
@@ -1621,6 +1740,7 @@ int astro_gMRes(double* xVec, const double* Amat, const int32_t* Acols, const in
             op.cmat = cv.p;
             op.ccols = cc.p;
             op.crange = cr.p;
+            if (CDim > 0) op.set_transpose(ct);
             const int maxCycles = maxIt / numVecs + 1;
             if (rtol <= 0 && atol <= 0) atol = (double)1e-12f;
             rc = gm.solve(op, P, b.p, x.p, rtol > 0 ? rtol : 0.0, atol, maxCycles, s);

@@ -106,6 +106,11 @@ __global__ void k_sweep_backward(const SweepTile* tiles, int numTiles, const dou
                                  int* flags, int* ticket, int epoch, long spinLimit, int* timeout);
 __global__ void k_reset_ticket(int* ticket);
 __global__ void k_spmv(const double* val, const int* cols, const int* range, int n, const double* x, double* y);
+__global__ void k_mpc_dd(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,
+                         double* t);
+__global__ void k_residual_dd(const double* val, const int* cols, const int* range, int n, const int* ctRange,
+                              const int* ctPos, const int* ctRow, const double* cmat, const double* t,
+                              const double* b, const double* x, double* r);
 __global__ void k_mpc_apply(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,
                             double* y);
 __global__ void k_nodal_loads(const int* dofs, const double* vals, int count, double scale, double* load);

@@ -1381,6 +1381,14 @@ __device__ __forceinline__ void stores_complete() { asm volatile("s_waitcnt vmcn
 // ~1000-cycle sleeps between polls: waiters whose own result is many tiles
 // away poll rarely, which keeps the few freshly written lines every waiter
 // wants from being hammered.  Returns false if the bounded wait expired.
+// The timeout flag is shared by the whole launch: once any wave's bounded
+// wait has expired, every other waiter sees it within 256 polls and gives up
+// too, and tiles claimed afterwards exit at once, so one stuck tile ends the
+// launch in about one spin limit instead of one per waiting tile.
+__device__ __forceinline__ bool timed_out(const int* timeout) {
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0;
+}
+
 __device__ __forceinline__ bool poll_values(const double* v, int base, int cnt, int backoff, long spinLimit,
                                             int* timeout, double& out) {
     const int lane = threadIdx.x & 63;
@@ -1393,6 +1401,7 @@ __device__ __forceinline__ bool poll_values(const double* v, int base, int cnt, 
             if (lane == 0) atomicExch(timeout, 1);
             return false;
         }
+        if ((it & 255) == 0 && timed_out(timeout)) return false;
         for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(16);
         __builtin_amdgcn_s_sleep(1);
     }
@@ -1597,6 +1606,7 @@ __device__ __forceinline__ bool near_wait_with(const double* va, NearVals& a, co
             if (lane == 0) atomicExch(timeout, 1);
             return false;
         }
+        if ((it & 255) == 0 && timed_out(timeout)) return false;
         __builtin_amdgcn_s_sleep(1);
         a.v = lane < a.cnt ? coherent_load(&va[a.base + lane]) : 0.0;
         if (pb) b.v = lane < b.cnt ? coherent_load(&vb[b.base + lane]) : 0.0;
@@ -1673,7 +1683,7 @@ __global__ __launch_bounds__(256, 1) void k_sweep_forward(const SweepTile* __res
     __shared__ double B3s[4][16][64];  // and of F_t H_{t-1}
     __shared__ double vsh[4][16];      // per-wave broadcast slot
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid == 0) sT = atomicAdd(ticket, 1);
+    if (tid == 0) sT = timed_out(timeout) ? numTiles : atomicAdd(ticket, 1);
     __syncthreads();
     const int t = sT;
     if (t >= numTiles) return;
@@ -1870,7 +1880,7 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     __shared__ double B3s[4][16][64];  // and of G_t HB_{t+1}
     __shared__ double vsh[4][16];      // per-wave broadcast slot
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid == 0) sT = atomicAdd(ticket, 1);
+    if (tid == 0) sT = timed_out(timeout) ? numTiles : atomicAdd(ticket, 1);
     __syncthreads();
     const int t = sT;
     if (t >= numTiles) return;
@@ -2071,6 +2081,85 @@ __global__ __launch_bounds__(256) void k_spmv(const double* __restrict__ val, co
     if (l == 0) y[row] = s;
 }
 
+// Double-double arithmetic (value = hi + lo, |lo| <= ulp(hi) / 2) for true
+// residuals: TwoSum (Knuth) and TwoProduct through fma give the rounding
+// error of each sum and product exactly.  Both need every product and sum
+// rounded on its own: contraction (which would fuse a*b into the sum that
+// follows, s = fma(a, b, hi)) is switched off in these helpers.
+__device__ __forceinline__ void dd_two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double bh, double bl) {
+#pragma clang fp contract(off)
+    double s, e;
+    dd_two_sum(hi, bh, s, e);
+    e += lo + bl;
+    hi = s + e;
+    lo = e - (hi - s);
+}
+__device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double a, double b) {
+#pragma clang fp contract(off)
+    const double p = a * b;
+    dd_add(hi, lo, p, __builtin_fma(a, b, -p));
+}
+
+// t[r] = (C x)_r as a double-double pair (t[2r], t[2r + 1]).
+__global__ void k_mpc_dd(const double* __restrict__ cmat, const int* __restrict__ ccols,
+                         const int* __restrict__ crange, int cdim, const double* __restrict__ x,
+                         double* __restrict__ t) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cdim) return;
+    double hi = 0.0, lo = 0.0;
+    for (int s = crange[r]; s < crange[r + 1]; ++s)
+        if (ccols[s]) dd_add_prod(hi, lo, cmat[s], x[ccols[s] - 1]);
+    t[2 * r] = hi;
+    t[2 * r + 1] = lo;
+}
+
+// True residual r = b - (A x + C^T (C x)) with every product and sum carried
+// in double-double and rounded once at the end, so r is accurate even when it
+// is 1e-16 of |A| |x| (the restarts of GMRES then refine the solution like
+// iterative refinement with extended-precision residuals).  32 lanes per row;
+// the penalty term reads C transposed: for row i, entries ctPos[k] of cmat
+// in constraint rows ctRow[k], k in [ctRange[i], ctRange[i + 1]), with
+// t = C x from k_mpc_dd.
+__global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ val, const int* __restrict__ cols,
+                                                     const int* __restrict__ range, int n,
+                                                     const int* __restrict__ ctRange, const int* __restrict__ ctPos,
+                                                     const int* __restrict__ ctRow, const double* __restrict__ cmat,
+                                                     const double* __restrict__ t, const double* __restrict__ b,
+                                                     const double* __restrict__ x, double* __restrict__ r) {
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const int l = threadIdx.x & 31;
+    if (row >= n) return;
+    double hi = 0.0, lo = 0.0;
+    for (int q = range[row] + l; q < range[row + 1]; q += 32) {
+        const int c = cols[q];
+        if (c) dd_add_prod(hi, lo, val[q], x[c - 1]);
+    }
+    if (ctRange) {
+        for (int k = ctRange[row] + l; k < ctRange[row + 1]; k += 32) {
+            const double a = cmat[ctPos[k]];
+            const int cr = ctRow[k];
+            dd_add_prod(hi, lo, a, t[2 * cr]);
+            dd_add_prod(hi, lo, a, t[2 * cr + 1]);
+        }
+    }
+    for (int off = 16; off > 0; off >>= 1) {
+        const double oh = __shfl_xor(hi, off, 32), ol = __shfl_xor(lo, off, 32);
+        dd_add(hi, lo, oh, ol);
+    }
+    if (l == 0) {
+#pragma clang fp contract(off)
+        double s, e;
+        dd_two_sum(b[row], -hi, s, e);
+        r[row] = s + (e - lo);
+    }
+}
+
 __global__ void k_mpc_apply(const double* __restrict__ cmat, const int* __restrict__ ccols,
                             const int* __restrict__ crange, int cdim, const double* __restrict__ x,
                             double* __restrict__ y) {
@@ -2083,8 +2172,8 @@ __global__ void k_mpc_apply(const double* __restrict__ cmat, const int* __restri
         if (ccols[s]) atomicAdd(&y[ccols[s] - 1], cmat[s] * t);
 }
 
-// RHS pieces: nodal loads and the constraint load C^T (rhs - C u).
-// Working/AStrO_commandFunctions.f90:368-392, 506-535.
+// RHS pieces: nodal loads and the constraint load C^T (rhs - C u) (u null:
+// C^T rhs).  Working/AStrO_commandFunctions.f90:368-392, 506-535.
 __global__ void k_nodal_loads(const int* __restrict__ dofs, const double* __restrict__ vals, int count,
                               double scale, double* __restrict__ load) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2098,8 +2187,9 @@ __global__ void k_constraint_load(const double* __restrict__ cmat, const int* __
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= cdim) return;
     double t = crhs[r];
-    for (int s = crange[r]; s < crange[r + 1]; ++s)
-        if (ccols[s]) t -= cmat[s] * u[ccols[s] - 1];
+    if (u)
+        for (int s = crange[r]; s < crange[r + 1]; ++s)
+            if (ccols[s]) t -= cmat[s] * u[ccols[s] - 1];
     for (int s = crange[r]; s < crange[r + 1]; ++s)
         if (ccols[s]) atomicAdd(&load[ccols[s] - 1], cmat[s] * t);
 }

@@ -2,6 +2,7 @@
 #include "astro_prep.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 
@@ -241,7 +242,14 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
         for (int c = 0; c < 6; ++c)
             if (dofOf(i, c)) p.elMatRange[dofOf(i, c)] = cap;
     }
-    for (int r = 1; r <= n; ++r) p.elMatRange[r] += p.elMatRange[r - 1];
+    {
+        int64_t acc = 0;
+        for (int r = 1; r <= n; ++r) {
+            acc += p.elMatRange[r];
+            if (acc > INT32_MAX) return "elastic CSR capacity exceeds 2^31 - 1 entries";
+            p.elMatRange[r] = (int)acc;
+        }
+    }
     // profile lengths: same-component lower-ranked neighbours within solverMaxBW
     // (:380-409); with coupledProfile every component of the node and of those
     // neighbours
@@ -269,7 +277,16 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
             }
             p.elMatLTRange[r] = len;
         }
-    for (int r = 1; r <= n; ++r) p.elMatLTRange[r] += p.elMatLTRange[r - 1];
+    // prefix sums in 64 bits: the reference keeps the ranges in default
+    // INTEGER, so a profile past 2^31 - 1 entries cannot cross the boundary
+    {
+        int64_t acc = 0;
+        for (int r = 1; r <= n; ++r) {
+            acc += p.elMatLTRange[r];
+            if (acc > INT32_MAX) return "elastic profile exceeds 2^31 - 1 entries";
+            p.elMatLTRange[r] = (int)acc;
+        }
+    }
     // thermal capacity and profile, indexed by node rank (:413-472)
     p.thermMatRange.assign(N + 1, 0);
     p.thermMatLTRange.assign(N + 1, 0);
@@ -286,9 +303,15 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
         p.thermMatRange[r] = cap;
         p.thermMatLTRange[r] = len;
     }
-    for (int r = 1; r <= N; ++r) {
-        p.thermMatRange[r] += p.thermMatRange[r - 1];
-        p.thermMatLTRange[r] += p.thermMatLTRange[r - 1];
+    {
+        int64_t accR = 0, accL = 0;
+        for (int r = 1; r <= N; ++r) {
+            accR += p.thermMatRange[r];
+            accL += p.thermMatLTRange[r];
+            if (accR > INT32_MAX || accL > INT32_MAX) return "thermal matrix or profile exceeds 2^31 - 1 entries";
+            p.thermMatRange[r] = (int)accR;
+            p.thermMatLTRange[r] = (int)accL;
+        }
     }
     // First-touch column order.  The assembly loop visits elements in order and,
     // for every row of an element, the element's DOFs in dofTable order

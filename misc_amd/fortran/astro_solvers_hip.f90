@@ -7,10 +7,18 @@
 ! Working/AStrO_solvers.f90 and links libastro_hip.so runs its factor, the
 ! preconditioner sweeps and GMRES on the GPU; nothing else changes.
 !
-! Differences the caller can see: a nonzero status from the library stops the
-! run with its message (the reference has no status); gMRes ends on the
-! stable rule |P^-1 r| <= 1e-12 |P^-1 r0| instead of the absolute 1e-12 of
-! Working/AStrO_solvers.f90:184 (see the library documentation).
+! Stopping rule of gMRes: by default the reference's own, |P^-1 r| <= 1e-12
+! absolute (the single-precision literal of Working/AStrO_solvers.f90:184)
+! within maxIt/numVecs + 1 restart cycles.  Setting the environment variable
+! ASTRO_GMRES_RTOL (e.g. 1e-12) selects the relative rule |P^-1 r| <= rtol
+! |P^-1 r0| instead.  Either way a cycle ends as soon as the rotated residual
+! estimate meets the target (the literal form keeps iterating a converged
+! cycle and returns rounding noise, Working/AStrO_solvers.f90:187-245).
+!
+! Differences the caller can see: a device or argument error stops the run
+! with its message (the reference has no status); running out of restart
+! cycles prints a warning and returns the last iterate, as the reference's
+! gMRes returns without a word.
 module AStrO_solvers
     use, intrinsic :: iso_c_binding
     use AStrO_constantVals
@@ -58,14 +66,30 @@ contains
         integer :: i
         if (status .eq. 0) return
         call c_f_pointer(astro_last_error(), msg, [512])
-        write(*,'(A,A,I0,A)', advance='no') what, ': status ', status, ': '
+        if (status .eq. 4) then
+            write(*,'(A,A)', advance='no') what, ': warning: '
+        else
+            write(*,'(A,A,I0,A)', advance='no') what, ': status ', status, ': '
+        endif
         do i = 1, 512
             if (msg(i) .eq. c_null_char) exit
             write(*,'(A)', advance='no') msg(i)
         enddo
         write(*,*)
+        ! ASTRO_ERR_NO_CONVERGE: the iterate is returned, as the reference does
+        if (status .eq. 4) return
         stop 1
     end subroutine astro_stop_on
+
+    ! Relative tolerance from ASTRO_GMRES_RTOL, or 0 (the reference's rule).
+    real(c_double) function gmres_rtol()
+        character(len=64) :: v
+        integer :: ln, st
+        gmres_rtol = 0.0d0
+        call get_environment_variable('ASTRO_GMRES_RTOL', v, ln, st)
+        if (st .eq. 0 .and. ln .gt. 0) read(v, *, iostat=st) gmres_rtol
+        if (st .ne. 0) gmres_rtol = 0.0d0
+    end function gmres_rtol
 
     ! Working/AStrO_solvers.f90:10-15
     subroutine getSparseLDLFact(ALTri, ASize, ARange, n)
@@ -92,9 +116,13 @@ contains
         integer, intent(in) :: ACols(ASize), ARange(0:ADim), Ccols(CSize), CRange(0:CDim), PRange(0:ADim)
         real*8, intent(in) :: Amat(ASize), Cmat(CSize), Pmat(PSize), bVec(ADim)
         real*8, intent(out) :: xVec(ADim)
+        real(c_double) :: rtol, atol
+        rtol = gmres_rtol()
+        atol = 0.0d0
+        if (rtol .le. 0.0d0) atol = dble(1e-12)
         call astro_stop_on(astro_gMRes(xVec, Amat, Acols, ARange, int(ASize, c_int64_t), ADim, Cmat, Ccols, CRange, &
                 int(CSize, c_int64_t), CDim, Pmat, PRange, int(PSize, c_int64_t), bVec, numVecs, maxIt, &
-                1.0d-12, 0.0d0), 'gMRes')
+                rtol, atol), 'gMRes')
     end subroutine gMRes
 
 end module AStrO_solvers

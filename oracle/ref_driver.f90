@@ -7,7 +7,12 @@
 ! oracle restatement and the HIP library against it.
 !
 ! usage: astro_ref_driver <model.yaml> <out-prefix> <elastic|thermal|prep>
-! (prep stops after analysisPrep and its dumps)
+!                         [numVecs maxIt solverMaxBW]
+! (prep stops after analysisPrep and its dumps).  The optional integers, when
+! positive, replace the GMRES restart length and iteration cap of stepSolve
+! (Working/AStrO_commandFunctions.f90:711-715, 732-733) and the solverMaxBW of
+! the *solve block, so that a job can stop GMRES after a chosen number of full
+! restart cycles.
 !
 ! Solver parameters follow the *solve block: solverBlockDim = numNodes and
 ! solverMaxBW from 27*numNodes*numNodes in default INTEGER arithmetic
@@ -26,13 +31,25 @@ program astro_ref_driver
 
     character(len=128) :: modelFile
     character(len=256) :: prefix
-    character(len=16) :: mode
+    character(len=16) :: mode, argv
     integer :: nnd2, bw3, ticks0, ticks1, rate, tu
+    integer :: nvArg, itArg, bwArg
     real*8 :: tPrep, tAsm, tScale, tProf, tFact, tSolve
 
     call get_command_argument(1, modelFile)
     call get_command_argument(2, prefix)
     call get_command_argument(3, mode)
+    nvArg = 0
+    itArg = 0
+    bwArg = 0
+    if (command_argument_count() .ge. 6) then
+        call get_command_argument(4, argv)
+        read(argv, *) nvArg
+        call get_command_argument(5, argv)
+        read(argv, *) itArg
+        call get_command_argument(6, argv)
+        read(argv, *) bwArg
+    endif
 
     lfUnit = 30
     open(unit=lfUnit, file=trim(prefix)//'log.txt', action='write', status='replace')
@@ -47,6 +64,7 @@ program astro_ref_driver
         solverMaxBW = solverMaxBW + 1
         bw3 = solverMaxBW*solverMaxBW*solverMaxBW
     enddo
+    if (bwArg .gt. 0) solverMaxBW = bwArg
     solveThermal = 0
     solveElastic = 1
     nLGeom = 0
@@ -204,19 +222,22 @@ contains
     ! an uninitialised index (r_getNodeDLoad, Working/AStrO_r_designPropertyFunctions.f90:980),
     ! which faults under flang; with no design variables that loop adds nothing.
     subroutine elasticStep()
-        integer :: numVecs
+        integer :: numVecs, maxIt
         if (elMatDim .lt. 200) then
             numVecs = elMatDim/2
         else
             numVecs = 100
         endif
+        maxIt = elMatDim
+        if (nvArg .gt. 0) numVecs = nvArg
+        if (itArg .gt. 0) maxIt = itArg
         elasticLoad(:) = r_0
         intElasticLoad(:) = r_0
         call getElasticSolnLoad(nLGeom)
         call nodalLoads()
         call getElasticConstraintLoad()
         call gMRes(delDisp,elasticMat,elMatCols,elMatRange,elMatSize,elMatDim,elMPCMat,elMPCMatCols, &
-            elMPCMatRange,elMPCSize,elMPCDim,elMatLT,elMatLTRange,elMatLTSize,elasticLoad,numVecs,elMatDim)
+            elMPCMatRange,elMPCSize,elMPCDim,elMatLT,elMatLTRange,elMatLTSize,elasticLoad,numVecs,maxIt)
         nodeDisp(:) = nodeDisp(:) + delDisp(:)
     end subroutine elasticStep
 

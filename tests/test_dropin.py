@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import entry_rel, rel_norm
+from helpers import entry_rel, golden, rel_norm
 from misc_amd.meshes import cantilever_block
 from oracle import oracle as O
 
@@ -21,7 +21,8 @@ pytestmark = [pytest.mark.gpu,
 def test_reference_host_with_gpu_solvers(dims):
     m = cantilever_block(*dims)
     print("drop-in", dims, flush=True)
-    r = O.run_reference(m, "elastic", driver=O.DROPIN_DRIVER)
+    # the relative stop, so that the result is the converged solution
+    r = O.run_reference(m, "elastic", driver=O.DROPIN_DRIVER, env={"ASTRO_GMRES_RTOL": "1e-12"})
     om = O.OracleModel(m).prep().static_elastic(stop_after="profile")
     np.testing.assert_array_equal(r["elMatCols"], om.geti("elMatCols"))
     Kr = om.getd("elasticMat")
@@ -34,3 +35,18 @@ def test_reference_host_with_gpu_solvers(dims):
     assert e_fact < 1e-8
     om2 = O.OracleModel(m).prep().static_elastic(mode=1, rtol=1e-13)
     assert rel_norm(r["nodeDisp"], om2.getd("nodeDisp")) < 1e-10
+
+
+GMRES_PIN = golden("gmres")
+
+
+@pytest.mark.parametrize("dims,ref", GMRES_PIN, ids=[str(d) for d, _ in GMRES_PIN])
+def test_reference_host_reproduces_reference_displacements(dims, ref):
+    """The same reference host and inputs, once with its own Fortran solvers
+    (the committed fixture) and once with the GPU solver module, both under
+    the reference's own GMRES rule (absolute 1e-12, maxIt / numVecs + 1
+    restart cycles) stopped before convergence: the same nodeDisp."""
+    nv, it, bw = (int(v) for v in ref["gmres"])
+    print("drop-in gmres pin", dims, flush=True)
+    r = O.run_reference(cantilever_block(*dims), "elastic", driver=O.DROPIN_DRIVER, gmres=(nv, it, bw))
+    assert rel_norm(r["nodeDisp"], ref["nodeDisp"]) <= 1e-10

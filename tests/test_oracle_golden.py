@@ -2,10 +2,12 @@
 tests/golden/make_golden.py from the reference's own Fortran, compiled from
 source).  Integers bit-exact; FP64 arrays to rounding."""
 
+import os
+
 import numpy as np
 import pytest
 
-from helpers import entry_rel, golden
+from helpers import GOLDEN, entry_rel, golden
 from misc_amd.meshes import cantilever_block, thermal_block
 from oracle import oracle as O
 
@@ -82,3 +84,96 @@ def test_live_reference_on_a_skewed_mesh():
         np.testing.assert_array_equal(om.geti(k), r[k])
     assert entry_rel(om.getd("elasticMat"), r["elasticMat"], floor=1e-3 * np.abs(r["elasticMat"]).max()) < 1e-12
     assert entry_rel(om.getd("elMatLT"), r["elMatLTfact"], floor=1e-6) < 1e-9
+
+
+GMRES_PIN = golden("gmres")
+
+
+@pytest.mark.parametrize("dims,ref", GMRES_PIN, ids=[str(d) for d, _ in GMRES_PIN])
+def test_literal_gmres_reproduces_reference_before_convergence(dims, ref):
+    """The reference's GMRES, stopped by its own rule after several full
+    restart cycles (numVecs, maxIt and solverMaxBW chosen so that it has not
+    converged): the oracle's literal GMRES (mode 0: every cycle runs all m
+    steps, V^T r projection, absolute 1e-12 stop, maxIt/m + 1 cycles) returns
+    the reference's nodeDisp to rounding, and so does the stable form
+    (mode 1), whose only difference, beta e1 for V^T r, is rounding before
+    convergence.  Working/AStrO_solvers.f90:128-297."""
+    nv, it, bw = (int(v) for v in ref["gmres"])
+    om = O.OracleModel(cantilever_block(*dims)).prep(max_bw=bw).static_elastic(stop_after="factor")
+    assert om.scalar("solverMaxBW") == bw
+    O.lib().orc_elastic_rhs(om.h)
+    b = om.getd("elasticLoad")
+    np.testing.assert_allclose(b, ref["elasticLoad"], rtol=1e-14, atol=0)
+    args = (om.getd("elasticMat"), om.geti("elMatCols"), om.geti("elMatRange"), om.getd("elMPCMat"),
+            om.geti("elMPCMatCols"), om.geti("elMPCMatRange"), om.scalar("elMPCDim"), om.getd("elMatLT"),
+            om.geti("elMatLTRange"), b, nv, it)
+    x0, cycles, res, _ = O.gmres(np.zeros(b.size), *args, mode=0)
+    assert cycles == it // nv + 1 and res > 1e-12  # stopped by the cycle cap, not converged
+    u = ref["nodeDisp"]
+    assert np.linalg.norm(x0 - u) <= 1e-12 * np.linalg.norm(u)
+    x1, _, _, _ = O.gmres(np.zeros(b.size), *args, mode=1, rtol=1e-300)
+    assert np.linalg.norm(x1 - u) <= 1e-12 * np.linalg.norm(u)
+
+
+def test_reference_displacements_are_noise_after_early_convergence():
+    """D25 on the reference's own output: with its default restart length the
+    reference's GMRES converges inside the first cycle, keeps iterating, and
+    projects the residual on a basis that has lost orthogonality; its nodeDisp
+    then differs from the solution of the system it assembled by far more than
+    any solver tolerance (fixtures written by the flang-compiled reference).
+    This is why displacement parity is taken against the direct solution and
+    against the pre-convergence runs above, not these outputs."""
+    import scipy.sparse.linalg as spl
+
+    from helpers import csr_sparse
+
+    errs = {}
+    for dims, ref in ELASTIC:
+        n = int(ref["scalars"][1])
+        K = csr_sparse(ref["elasticMat"], ref["elMatCols"], ref["elMatRange"], n)
+        C = csr_sparse(ref["elMPCMat"], ref["elMPCMatCols"], ref["elMPCMatRange0"], n)
+        x = spl.spsolve((K + C.T @ C).tocsc(), ref["elasticLoad"])
+        errs[dims] = np.linalg.norm(ref["nodeDisp"] - x) / np.linalg.norm(x)
+    assert errs[(4, 4, 2)] > 1e-3 and errs[(5, 3, 4)] > 1e-2
+    assert min(errs.values()) > 1e-7
+
+
+def test_cpu_fixture_brick_int64_rule_solves_its_system():
+    """The 24 x 24 x 16 block with the 64-bit solverMaxBW rule (1450): the
+    committed direct solution (sparse LU refined with extended-precision
+    residuals) satisfies the oracle's penalised system to FP64 residual
+    accuracy, and its profile is the oracle's."""
+    d = dict(np.load(os.path.join(GOLDEN, "cpu_brick_24x24x16_bw1450.npz"), allow_pickle=False))
+    om = O.OracleModel(cantilever_block(24, 24, 16)).prep(max_bw=1450).static_elastic(stop_after="assemble")
+    np.testing.assert_array_equal(om.geti("elMatLTRange"), d["elMatLTRange"])
+    O.lib().orc_scale_elastic_mpc(om.h)
+    O.lib().orc_elastic_rhs(om.h)
+    b = om.getd("elasticLoad")
+    Au = O.apply_operator(d["nodeDisp"], om.getd("elasticMat"), om.geti("elMatCols"), om.geti("elMatRange"),
+                          om.getd("elMPCMat"), om.geti("elMPCMatCols"), om.geti("elMPCMatRange"),
+                          om.scalar("elMPCDim"))
+    assert np.linalg.norm(Au - b) <= 1e-11 * np.linalg.norm(b)
+
+
+def test_cpu_fixture_shell_plate_equilibrium():
+    """Config 1 fixture (64 x 8 shell4, 2 mm, 1 N +z on each of the 9 tip
+    nodes): reactions balance the loads (sum Rz = -9 N, sum My = 9 N m about
+    the root line at x = 0), up to the rounding of K's rigid-body null space;
+    the tip compliance is the sum of tip w; Euler-Bernoulli with the plate
+    modulus E / (1 - nu^2) bounds the deflection from below within 2 %."""
+    from misc_amd.meshes import shell_plate
+
+    d = dict(np.load(os.path.join(GOLDEN, "cpu_shell_plate_64x8.npz"), allow_pickle=False))
+    R = d["reactionForce"]
+    assert abs(R[:, 2].sum() + 9.0) < 1e-6
+    assert abs(R[:, 4].sum() - 9.0) < 1e-6
+    assert np.abs(R[:, [0, 1, 3, 5]].sum(0)).max() < 1e-6
+    m = shell_plate(64, 8)
+    om = O.OracleModel(m).prep()
+    nd = om.geti("nDofIndex").reshape(-1, 6)
+    u = d["nodeDisp"]
+    w = np.array([u[nd[i - 1, 2] - 1] for i in m.node_sets["tip"]])
+    assert abs(w.sum() - d["tipCompliance"]) < 1e-14 * abs(w.sum())
+    I = 0.1 * 0.002 ** 3 / 12
+    w_beam = 9.0 / (3 * 70e9 * I)
+    assert w_beam * (1 - 0.33 ** 2) < w.mean() < w_beam
