@@ -261,6 +261,22 @@ double astro_ctx_kernel_stat(const astro_ctx* ctx, const char* name);
  * on the device (synchronous). */
 int astro_ctx_true_residual(astro_ctx* ctx, double* relRes);
 
+/* Allocation-free variant of astro_ctx_true_residual.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * For a caller-provided device displacement d_u (elMatDim doubles) it writes
+ * the double-double-accurate residual r = b - (A + C^T C) u to the caller's
+ * device buffer d_r (distinct from d_u) and |r| / |b| to *relRes, with
+ * b = nodal loads + C^T rhs, using the same kernels and reduction order as
+ * astro_ctx_true_residual (so at u = nodeDisp the two return the same value).
+ * It calls no hipMalloc/hipFree: its working vectors live in a static device
+ * array of 4.5 MiB that each GPU holds until the process ends.  Limits:
+ * elMatDim <= 524288 and elMPCDim <= 32768 (ASTRO_ERR_ARG beyond).  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (from
+ * several threads or contexts) race; the array keeps the last call's
+ * contents.  Synchronous on the context stream. */
+int astro_ctx_true_residual_noalloc(astro_ctx* ctx, const double* d_u, double* d_r, double* relRes);
+
 /* Wait for all work queued on the context stream. */
 int astro_ctx_sync(astro_ctx* ctx);
 /* Per-phase device times (ms) of the last astro_ctx_static_solve, measured

@@ -36,6 +36,7 @@ EXPORTS = [
     "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual", "astro_ctx_dynamic_init",
     "astro_ctx_dynamic_step", "astro_ctx_dynamic_run", "astro_ctx_set_design", "astro_ctx_dRdD",
     "astro_ctx_compliance_gradient", "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load",
+    "astro_ctx_true_residual_noalloc",
 ]
 
 
@@ -113,6 +114,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_prep_get_i32.argtypes = [C.c_void_p, C.c_char_p, _ip, _i64]
     L.astro_prep_get_f64.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
     L.astro_ctx_true_residual.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.astro_ctx_true_residual_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
     L.astro_ctx_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.astro_ctx_kernel_stat.restype = C.c_double
     L.astro_ctx_kernel_stat.argtypes = [C.c_void_p, C.c_char_p]
@@ -454,6 +456,14 @@ class Context:
     def true_residual(self):
         r = C.c_double(0)
         _check(load_library().astro_ctx_true_residual(self.h, C.byref(r)))
+        return r.value
+
+    def true_residual_noalloc(self, d_u: int, d_r: int):
+        """|b - (A + C^T C) u| / |b| for a device displacement at address d_u,
+        writing the residual to the device buffer at d_r; allocates nothing
+        on the device (see include/astro_abi.h)."""
+        r = C.c_double(0)
+        _check(load_library().astro_ctx_true_residual_noalloc(self.h, C.c_void_p(d_u), C.c_void_p(d_r), C.byref(r)))
         return r.value
 
     def set_timing(self, on: bool):

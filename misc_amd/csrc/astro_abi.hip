@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -750,6 +751,7 @@ struct astro_prep {
 // Context
 struct astro_ctx {
     hipStream_t s = nullptr;
+    int dev = 0;  // the HIP device the context lives on
     Prep prep;
     astro_options opt{};
     int numVecs = 100;
@@ -953,6 +955,7 @@ int astro_ctx_create(const astro_model* m, const astro_options* o, astro_ctx** o
     if (o) c->opt = *o;
     if (c->opt.rtol <= 0) c->opt.rtol = 1e-12;
     if (c->opt.maxCycles <= 0) c->opt.maxCycles = 1000;
+    HIP_TRY(hipGetDevice(&c->dev));
     HIP_TRY(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     ModelDesc d = to_desc(m);
@@ -1619,6 +1622,110 @@ int astro_ctx_true_residual(astro_ctx* c, double* relRes) {
     HIP_TRY(hipStreamSynchronize(c->s));
     *relRes = std::sqrt(g.hHost[0]) / std::sqrt(g.hHost[1]);
     return ASTRO_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free variant of astro_ctx_true_residual ---------------------
+// Its working vectors live in one file-scope device array instead of buffers
+// the library allocates: the right-hand side b (n doubles), C u as
+// double-double pairs (2 cdim), and the block partials and results of the two
+// norms.  Each GPU holds its own copy of the array for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device.
+namespace {
+constexpr int64_t kNoAllocMaxN = int64_t(1) << 19;     // rows (DOFs)
+constexpr int64_t kNoAllocMaxCdim = int64_t(1) << 15;  // constraint rows
+constexpr int64_t kNoAllocMaxBlk = 1024;               // reduction blocks (Gmres::nblk <= 1024)
+constexpr int64_t kNoAllocCx = 0;
+constexpr int64_t kNoAllocPartR = kNoAllocCx + 2 * kNoAllocMaxCdim;
+constexpr int64_t kNoAllocPartB = kNoAllocPartR + kNoAllocMaxBlk;
+constexpr int64_t kNoAllocNorms = kNoAllocPartB + kNoAllocMaxBlk;
+constexpr int64_t kNoAllocB = kNoAllocNorms + 8;
+constexpr int64_t kNoAllocDoubles = kNoAllocB + kNoAllocMaxN;  // 4.5 MiB
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noAllocMutex;
+double* g_noAllocPtr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+// Device address of `scratch` on device dev (which must be current), looked
+// up once per device.
+static int noalloc_scratch(int dev, double** out) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(ASTRO_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_noAllocMutex);
+    if (!g_noAllocPtr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ASTRO_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noAllocPtr[dev] = static_cast<double*>(p);
+    }
+    *out = g_noAllocPtr[dev];
+    return ASTRO_OK;
+}
+
+static int true_residual_noalloc(astro_ctx* c, const double* d_u, double* d_r, double* relRes) {
+    const int n = c->prep.elMatDim, cd = c->prep.elMPCDim, nl = (int)c->loadDofs.n;
+    double* base = nullptr;
+    ASTRO_TRY(noalloc_scratch(c->dev, &base));
+    double* cx = base + kNoAllocCx;
+    double* partR = base + kNoAllocPartR;
+    double* partB = base + kNoAllocPartB;
+    double* norms = base + kNoAllocNorms;
+    double* b = base + kNoAllocB;
+    hipStream_t s = c->s;
+    // b = nodal loads + C^T rhs; every scratch element read below is written
+    // first, so nothing is carried over from the previous call
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b, (int64_t)n, 0.0);
+    if (nl > 0)
+        hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, s, c->loadDofs.p, c->loadVals.p, nl,
+                           1.0, b);
+    if (cd > 0) {
+        hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->mpcMat.p, c->mpcCols.p,
+                           c->mpcRange.p, c->mpcRhs.p, cd, (const double*)nullptr, b);
+        hipLaunchKernelGGL(k_mpc_dd, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->mpcMat.p, c->mpcCols.p,
+                           c->mpcRange.p, cd, d_u, cx);
+    }
+    // the kernels and reduction order of astro_ctx_true_residual
+    hipLaunchKernelGGL(k_residual_dd, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, c->csrVal.p,
+                       c->csrCols.p, c->csrRange.p, n, cd > 0 ? c->mpcT.range.p : (const int*)nullptr, c->mpcT.pos.p,
+                       c->mpcT.row.p, c->mpcMat.p, (const double*)cx, (const double*)b, d_u, d_r);
+    const int nblk = c->gm.nblk;
+    hipLaunchKernelGGL(k_multidot, dim3(nblk, 1), dim3(256), 0, s, (const double*)d_r, (int64_t)n, 1,
+                       (const double*)d_r, n, partR, 1);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double*)partR, nblk, 1, 1, norms);
+    hipLaunchKernelGGL(k_multidot, dim3(nblk, 1), dim3(256), 0, s, (const double*)b, (int64_t)n, 1, (const double*)b,
+                       n, partB, 1);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double*)partB, nblk, 1, 1, norms + 1);
+    HIP_TRY(hipGetLastError());
+    // staged through the context's pinned buffer, allocated at create
+    HIP_TRY(hipMemcpyAsync(c->gm.hHost, norms, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *relRes = std::sqrt(c->gm.hHost[0]) / std::sqrt(c->gm.hHost[1]);
+    return ASTRO_OK;
+}
+
+extern "C" {
+
+int astro_ctx_true_residual_noalloc(astro_ctx* c, const double* d_u, double* d_r, double* relRes) {
+    if (!c || !d_u || !d_r || !relRes) return fail(ASTRO_ERR_ARG, "null argument");
+    if (d_u == d_r) return fail(ASTRO_ERR_ARG, "true_residual_noalloc needs distinct u and r buffers");
+    const int64_t n = c->prep.elMatDim, cd = c->prep.elMPCDim;
+    if (n > kNoAllocMaxN)
+        return fail(ASTRO_ERR_ARG, "true_residual_noalloc: " + std::to_string(n) + " DOFs exceed its scratch limit of " +
+                                       std::to_string(kNoAllocMaxN));
+    if (cd > kNoAllocMaxCdim)
+        return fail(ASTRO_ERR_ARG, "true_residual_noalloc: " + std::to_string(cd) +
+                                       " constraint rows exceed its scratch limit of " + std::to_string(kNoAllocMaxCdim));
+    if (c->gm.nblk > kNoAllocMaxBlk) return fail(ASTRO_ERR_ARG, "true_residual_noalloc: too many reduction blocks");
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != c->dev) HIP_TRY(hipSetDevice(c->dev));
+    const int rc = true_residual_noalloc(c, d_u, d_r, relRes);
+    if (cur != c->dev) HIP_TRY(hipSetDevice(cur));
+    return rc;
 }
 
 int astro_ctx_sync(astro_ctx* c) {

@@ -76,3 +76,48 @@ def refined_solve(A, b, steps=6):
         np.add.at(Au, rows, vals * u.astype(np.longdouble)[cols])
         u = u + lu.solve(np.asarray(bl - Au, np.float64))
     return u
+
+
+class DeviceBuffer:
+    """Device memory for tests of entry points that take device pointers,
+    allocated through the HIP runtime the library itself links
+    (libamdhip64.so.7), so that one runtime serves the whole process."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        import ctypes as C
+
+        if cls._hip is None:
+            from misc_amd import astro
+
+            astro.load_library()  # loads libamdhip64.so.7 as the library's dependency
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host):
+        import ctypes as C
+
+        self.host = np.ascontiguousarray(host, dtype=np.float64)
+        self.nbytes = self.host.nbytes
+        p = C.c_void_p()
+        assert self.hip().hipMalloc(C.byref(p), max(self.nbytes, 8)) == 0
+        self.ptr = p.value
+        assert self.hip().hipMemcpy(self.ptr, self.host.ctypes.data, self.nbytes, 1) == 0  # host to device
+
+    def get(self):
+        out = np.empty_like(self.host)
+        assert self.hip().hipDeviceSynchronize() == 0
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
