@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
 def factor_flops(lt_range):
@@ -209,6 +210,122 @@ def bench_sensitivity(args):
     print(json.dumps(out))
 
 
+def timed_static(ctx, steps, warmup):
+    """warmup + steps static solves from rest on ctx; returns (seconds per
+    solve, phase ms per solve, GMRES steps per solve, sweep ms, sweep count)."""
+    for _ in range(warmup):
+        ctx.reset()
+        ctx.static_solve()
+    ctx.sync()
+    ctx.set_timing(True)
+    phases = {k: 0.0 for k in ("assemble", "scale", "profile", "factor", "solve")}
+    gsteps = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.reset()
+        ctx.static_solve()
+        for ph in phases:
+            phases[ph] += ctx.phase_ms(ph)
+        gsteps += ctx.scalar("gmresSteps")
+        print(f"bench: step {k} done", file=sys.stderr, flush=True)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    sw_ms = ctx.kernel_stat("sweep_fwd_ms") + ctx.kernel_stat("sweep_bwd_ms")
+    sw_n = ctx.kernel_stat("sweep_fwd_count") + ctx.kernel_stat("sweep_bwd_count")
+    ctx.set_timing(False)
+    return el / steps, {k: v / steps for k, v in phases.items()}, gsteps / steps, sw_ms, sw_n
+
+
+def bench_precision(args):
+    """Config 3, the precision form on the 64 x 64 x 32 brick block with E per
+    element drawn uniform in [150, 250] GPa (seed 3): one step = one static
+    solve from rest, timed for the plain FP64 path and for the mixed path
+    (FP32-stored profile and factor, FP32 matrix-core trailing updates, FP64
+    pivots / diagonal blocks / sweeps / operator, GMRES restarts forming the
+    true residual in double-double).  The value is the mixed path's DOF/s;
+    the line reports the speed-up over the plain path, the error of every
+    mixed result against the plain FP64 result, and the steady thermal case
+    (k = 50 W/m K, 100 K on top, 0 K at the bottom) on the same block in both
+    precisions."""
+    from misc_amd import astro
+    from misc_amd.meshes import cantilever_block, thermal_block, uniform_e_field
+
+    nx, ny, nz = args.nx, args.ny, args.nz
+    model = cantilever_block(nx, ny, nz, e_field=uniform_e_field(nx * ny * nz, 150e9, 250e9, 3))
+    lit = args.maxbw == "literal"
+    print(f"bench: precision form, brick8 {nx}x{ny}x{nz}, E ~ U[150, 250] GPa", file=sys.stderr, flush=True)
+    res = {}
+    for name, prec in (("plain", 0), ("mixed", 1)):
+        ctx = astro.Context(model, astro.Options(literalMaxBW=lit, precision=prec))
+        n, prof = ctx.scalar("elMatDim"), ctx.scalar("elMatLTSize")
+        sec, ph, gs, sw_ms, sw_n = timed_static(ctx, args.steps, args.warmup)
+        u = ctx.getd("nodeDisp")
+        bottom = model.node_sets["bottom"]
+        react = ctx.reaction_force(bottom)
+        res[name] = dict(sec=sec, phases=ph, gmres=gs, sweep_ms=sw_ms / max(1.0, sw_n), u=u, react=react,
+                         rel_res=ctx.true_residual(), flops=factor_flops(ctx.geti("elMatLTRange")))
+        ctx.close()
+        print(f"bench: {name} {1e3 * sec:.1f} ms per solve, GMRES steps {gs}", file=sys.stderr, flush=True)
+    th = {}
+    tm = thermal_block(nx, ny, nz)
+    for name, prec in (("plain", 0), ("mixed", 1)):
+        ctx = astro.Context(tm, astro.Options(literalMaxBW=lit, thermLiteralD8=False, precision=prec))
+        ctx.assemble_thermal()
+        ctx.thermal_solve()  # warm-up
+        ctx.reset()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.assemble_thermal()
+        ctx.thermal_solve()
+        ctx.sync()
+        th[name] = dict(sec=time.perf_counter() - t0, T=ctx.getd("nodeTemp"), rank=ctx.geti("currentRank"),
+                        steps=ctx.scalar("thermGmresSteps"))
+        ctx.close()
+    z = tm.nodes[np.argsort(th["plain"]["rank"]), 2]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    p, mx = res["plain"], res["mixed"]
+    sweep_bytes = 4.0 * prof + 16.0 * n
+    fac_s = mx["phases"]["factor"] * 1e-3
+    out = {
+        "metric": "assembled+solved DOF/s at <=1e-10 rel. err vs FP64; mixed vs plain speedup",
+        "value": n / mx["sec"], "unit": "DOF/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * mx["sec"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 factor storage and trailing updates, f64 pivots / sweeps / operator / residuals",
+        "data": "synthetic",
+        "config": {"workload": f"brick8 block {nx}x{ny}x{nz} linear static, E per element U[150, 250] GPa (seed 3), "
+                               f"nu 0.3, bottom clamped, 1 MPa top; skyline LDL^T-preconditioned GMRES(100) to "
+                               f"1e-12 (preconditioned, relative), solverMaxBW rule {args.maxbw}; plain FP64 vs "
+                               f"mixed (precision = 1)",
+                   "dof": n, "profile_entries": prof,
+                   "plain_ms_per_solve": 1e3 * p["sec"], "mixed_ms_per_solve": 1e3 * mx["sec"],
+                   "speedup_mixed_vs_plain": p["sec"] / mx["sec"],
+                   "gmres_steps_plain": p["gmres"], "gmres_steps_mixed": mx["gmres"],
+                   "phases_ms_plain": p["phases"], "phases_ms_mixed": mx["phases"],
+                   "err_vs_fp64": {"nodeDisp": rel(mx["u"], p["u"]),
+                                   "reactionForce_bottom": rel(mx["react"], p["react"]),
+                                   "true_rel_residual_plain": p["rel_res"], "true_rel_residual_mixed": mx["rel_res"]},
+                   "thermal": {"dof": tm.num_nodes, "plain_ms": 1e3 * th["plain"]["sec"],
+                               "mixed_ms": 1e3 * th["mixed"]["sec"],
+                               "speedup_mixed_vs_plain": th["plain"]["sec"] / th["mixed"]["sec"],
+                               "gmres_steps_plain": th["plain"]["steps"], "gmres_steps_mixed": th["mixed"]["steps"],
+                               "err_vs_fp64_nodeTemp": rel(th["mixed"]["T"], th["plain"]["T"]),
+                               "max_abs_T_minus_200z_K": float(np.abs(th["mixed"]["T"] - 200.0 * z).max())}},
+        "roofline": {"kernel": "k_sweep_forward<float> / k_sweep_backward<float> (FP32-stored preconditioner "
+                               "sweeps)", "bound": "hbm", "achieved": sweep_bytes / (mx["sweep_ms"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": sweep_bytes / (mx["sweep_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": mx["sweep_ms"], "algorithmic_bytes_per_launch": sweep_bytes},
+        "cpu_baseline": None,
+    }
+    if mx["flops"] is not None:
+        out["roofline_factor"] = {"kernel": "k_ldl_lead<float> + k_ldl_rows<float> + k_sweep_blocks<float> "
+                                            "(whole factor phase, FP32 matrix cores)", "bound": "mfma",
+                                  "achieved": mx["flops"] / fac_s / 1e12, "peak": FP32_MFMA_PEAK_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": mx["flops"] / fac_s / 1e12 / FP32_MFMA_PEAK_TFLOPS,
+                                  "flops_per_factor": mx["flops"]}
+    print(json.dumps(out))
+
+
 def ctx_profile_entries(model, args):
     from misc_amd import astro
     p = astro.Prep(model, astro.Options(literalMaxBW=(args.maxbw == "literal")))
@@ -225,11 +342,12 @@ def main():
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--maxbw", choices=["literal", "int64"], default="int64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["static", "newmark", "sensitivity"], default="static",
+    ap.add_argument("--workload", choices=["static", "precision", "newmark", "sensitivity"], default="static",
                     help="static: the 64x64x32 linear-static block (default); newmark: the 48x48x24 "
                          "log-normal-E block, 200 Newmark steps of a 1 kN half-sine top-patch pulse; "
                          "sensitivity: 128x16 shell plate, compliance gradient of 64 thickness strips by "
-                         "complex step")
+                         "complex step; precision: config 3, the 64x64x32 block with E ~ U[150, 250] GPa in "
+                         "FP64 and in the mixed form, plus the thermal case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported. "
@@ -241,6 +359,8 @@ def main():
         return bench_newmark(args)
     if args.workload == "sensitivity":
         return bench_sensitivity(args)
+    if args.workload == "precision":
+        return bench_precision(args)
 
     from misc_amd import astro
     from misc_amd.meshes import cantilever_block

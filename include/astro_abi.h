@@ -133,6 +133,12 @@ typedef struct {
     int32_t coupledProfile;   /* 0: profile rows of one component, as the reference (the factor is a
                                  component-block preconditioner); 1: rows reach every component of the
                                  node and its neighbours (the factor solves the system directly) */
+    int32_t precision;        /* 0: FP64 profile and factor, the reference's arithmetic; 1: mixed: the
+                                 profile and L stored in FP32 and the factor's trailing updates on FP32
+                                 matrix cores with FP32 accumulation, while the pivots D, the diagonal
+                                 blocks' factor, the sweeps' arithmetic, the operator and the GMRES
+                                 vectors stay FP64 and every restart forms the true residual in
+                                 double-double (the restarts are iterative refinement) */
 } astro_options;
 
 /* Host-only analysisPrep (Working/AStrO_bookKeeping.f90:794-1039): level
@@ -190,8 +196,9 @@ int astro_ctx_factor(astro_ctx* ctx);
  * Working/AStrO_commandFunctions.f90:353-535), GMRES from delDisp, nodeDisp +=
  * delDisp.  Synchronous. */
 int astro_ctx_solve(astro_ctx* ctx);
-/* Zero delDisp and nodeDisp (the state a fresh job starts from,
- * Working/AStrO_bookKeeping.f90:960, Working/AStrO_commandFunctions.f90:653-679). */
+/* Zero delDisp and nodeDisp, and once the thermal system exists its twins
+ * (the state a fresh job starts from, Working/AStrO_bookKeeping.f90:960,
+ * Working/AStrO_commandFunctions.f90:653-679). */
 int astro_ctx_reset(astro_ctx* ctx);
 /* Whole solve() sequence: assemble, scale, profile, factor, solve.  The step
  * right-hand side holds -R(u) of the current nodeDisp (see below), so a second

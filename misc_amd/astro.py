@@ -67,7 +67,7 @@ class _Model(C.Structure):
 class _Options(C.Structure):
     _fields_ = [("maxBW", C.c_int32), ("literalMaxBW", C.c_int32), ("thermLiteralD8", C.c_int32),
                 ("numVecs", C.c_int32), ("rtol", C.c_double), ("maxCycles", C.c_int32),
-                ("coupledProfile", C.c_int32)]
+                ("coupledProfile", C.c_int32), ("precision", C.c_int32)]
 
 
 def load_library(path: str = LIB_PATH):
@@ -205,6 +205,7 @@ class Options:
     rtol: float = 1e-12
     maxCycles: int = 1000
     coupledProfile: bool = False
+    precision: int = 0  # 0: FP64 factor; 1: FP32-stored factor, FP32 matrix-core updates, FP64 refinement
 
 
 def _marshal(model, o: "Options"):
@@ -246,7 +247,7 @@ def _marshal(model, o: "Options"):
         M.zOffset = ptr(model.z_offset, np.float64)
         M.orient = ptr(model.orient.reshape(-1), np.float64)
     O = _Options(int(o.maxBW), 1 if o.literalMaxBW else 0, 1 if o.thermLiteralD8 else 0, int(o.numVecs),
-                 float(o.rtol), int(o.maxCycles), 1 if o.coupledProfile else 0)
+                 float(o.rtol), int(o.maxCycles), 1 if o.coupledProfile else 0, int(o.precision))
     return M, O, keep
 
 
@@ -457,6 +458,10 @@ class Context:
         r = C.c_double(0)
         _check(load_library().astro_ctx_true_residual(self.h, C.byref(r)))
         return r.value
+
+    def precond_apply(self, d_b: int, d_x: int):
+        """x = P^-1 b on device vectors at addresses d_b, d_x (distinct)."""
+        _check(load_library().astro_ctx_precond_apply(self.h, C.c_void_p(d_b), C.c_void_p(d_x)))
 
     def true_residual_noalloc(self, d_u: int, d_r: int):
         """|b - (A + C^T C) u| / |b| for a device displacement at address d_u,

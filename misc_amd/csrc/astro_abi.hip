@@ -80,22 +80,28 @@ static inline int grid_for(int64_t n, int block, int cap = 65536) {
 }
 
 // ---------------------------------------------------------------------------
-// Factor + sweeps of one row-skyline profile.
-struct DeviceProfile {
+// Factor + sweeps of one row-skyline profile whose entries are stored as LT:
+// double (the FP64 path) or float (the mixed-precision factor: FP32 L and
+// step-U partials on FP32 matrix cores; pivots, the diagonal-block factor,
+// the sweep blocks and every vector stay FP64).
+template <typename LT>
+struct DeviceProfileT {
+    // rows start on 128-byte lines: 16 columns of FP64, 32 of FP32
+    static constexpr int kAlign = 128 / (int)sizeof(LT);
     int n = 0;
     int64_t size = 0;        // entries of the device array lt (gaps included)
     int64_t packedSize = 0;  // entries of the reference's packed profile
     bool aligned = false;    // rows padded: see setup
     std::vector<int> hS;     // first column per row
     std::vector<int64_t> hOff;  // device row offsets: entry (r, c) at lt[hOff[r] + c]
-    DBuf<double> lt;
+    DBuf<LT> lt;
     DBuf<int64_t> rowoff;
     DBuf<int> sfirst;
     DBuf<int> winMin;  // min first column over rows [r, r + 64) of r's diagonal block
     // factor
     int nBlocks = 0, panelsPerBlock = 0, splitK = 1, scratchRows = 64, maxTiles = 1;
     DBuf<PanelInfo> panels;
-    DBuf<double> scratch;
+    DBuf<LT> scratch;
     DBuf<double> diag;  // D of the factor, written by step FT, read by step U
     DBuf<int> zeroPivot;
     DBuf<double> ftX;  // per block: L_PP^-1 (64 x 64) and 1 / D of the current panel's diagonal block
@@ -105,6 +111,7 @@ struct DeviceProfile {
     DBuf<SweepBlock> blockInfo;  // per tile, indexed by SweepTile::blk
     DBuf<double> blocks;         // SWEEP_KINDS blocks of 64 x 64 per tile
     bool blocksReady = false;    // blocks match the current factor in lt
+    bool factored = false;       // lt holds a factor computed here (diag is valid)
     DBuf<double> gnear;          // per sweep: each tile's L_tt^-1 (or ^-T) part, for its neighbour
     DBuf<int> fwdFlags, bwdFlags, ticket, timeout;
     int fwdEpoch = 0, bwdEpoch = 0;
@@ -118,7 +125,7 @@ struct DeviceProfile {
     long fwdCount = 0, bwdCount = 0;
     std::vector<int> evKind;  // 0 = forward, 1 = backward, per event pair
 
-    ~DeviceProfile() {
+    ~DeviceProfileT() {
         for (auto e : evPool) (void)hipEventDestroy(e);
     }
     hipEvent_t ev_next() {
@@ -148,7 +155,7 @@ struct DeviceProfile {
     }
 
     // alignRows: lay the rows out with gaps so that, inside each independent
-    // diagonal block starting at row b0, every column c = b0 (mod 16) of
+    // diagonal block starting at row b0, every column c = b0 (mod kAlign) of
     // every row starts a 128-byte line.  The factor's 32-column chunks and
     // the sweeps' 64-column segments then read whole lines (about 1 % faster
     // end to end on the 64x64x32 brick).  Without it lt is the reference's
@@ -166,8 +173,8 @@ struct DeviceProfile {
         HIP_TRY(hipGetDevice(&dev));
         HIP_TRY(hipDeviceGetAttribute(&ldsPerCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
         hipFuncAttributes fa{}, ba{};
-        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_sweep_forward)));
-        HIP_TRY(hipFuncGetAttributes(&ba, reinterpret_cast<const void*>(k_sweep_backward)));
+        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_sweep_forward<LT>)));
+        HIP_TRY(hipFuncGetAttributes(&ba, reinterpret_cast<const void*>(k_sweep_backward<LT>)));
         const long half = ldsPerCU / 2 + 1;
         sweepLdsFwd = (unsigned)std::max(0L, half - (long)fa.sharedSizeBytes);
         sweepLdsBwd = (unsigned)std::max(0L, half - (long)ba.sharedSizeBytes);
@@ -202,14 +209,14 @@ struct DeviceProfile {
         nBlocks = (int)bounds.size() - 1;
         if (aligned) {
             // row r's first stored entry goes at the first position after
-            // row r - 1 at which hOff[r] + b0 = 0 (mod 16); gaps stay zero
+            // row r - 1 at which hOff[r] + b0 = 0 (mod kAlign); gaps stay zero
             int64_t end = 0;
             for (int k = 0; k < nBlocks; ++k) {
                 const int b0 = bounds[k];
                 for (int r = b0; r < bounds[k + 1]; ++r) {
                     int64_t v = end - hS[r];
-                    const int64_t rem = ((v + b0) % 16 + 16) % 16;
-                    if (rem) v += 16 - rem;
+                    const int64_t rem = ((v + b0) % kAlign + kAlign) % kAlign;
+                    if (rem) v += kAlign - rem;
                     off[r] = v;
                     end = v + r + 1;
                 }
@@ -360,39 +367,54 @@ struct DeviceProfile {
     }
 
     // The profile (or factor) in the reference's packed layout, packedSize
-    // entries; through a host staging copy when the rows are aligned.
+    // doubles; through a host staging copy when the rows are aligned or the
+    // storage is FP32.  An FP32 factor reads back its FP64 pivots on the
+    // diagonal.
     int get_packed(double* host, hipStream_t s) {
-        if (!aligned) {
+        if (!aligned && sizeof(LT) == sizeof(double)) {
             HIP_TRY(hipMemcpyAsync(host, lt.p, sizeof(double) * size, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             return ASTRO_OK;
         }
-        std::vector<double> tmp(size);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), lt.p, sizeof(double) * size, hipMemcpyDeviceToHost, s));
+        std::vector<LT> tmp(size);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), lt.p, sizeof(LT) * size, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         int64_t pos = 0;
         for (int r = 0; r < n; ++r) {
             const int64_t len = r - hS[r] + 1;
-            std::memcpy(host + pos, tmp.data() + hOff[r] + hS[r], sizeof(double) * len);
+            const LT* src = tmp.data() + hOff[r] + hS[r];
+            for (int64_t q = 0; q < len; ++q) host[pos + q] = (double)src[q];
             pos += len;
+        }
+        if (sizeof(LT) != sizeof(double) && factored) {
+            std::vector<double> d(n);
+            HIP_TRY(hipMemcpyAsync(d.data(), diag.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            pos = 0;
+            for (int r = 0; r < n; ++r) {
+                pos += r - hS[r] + 1;
+                host[pos - 1] = d[r];
+            }
         }
         return ASTRO_OK;
     }
     int set_packed(const double* host, hipStream_t s) {
         blocksReady = false;
-        if (!aligned) {
+        factored = false;
+        if (!aligned && sizeof(LT) == sizeof(double)) {
             HIP_TRY(hipMemcpyAsync(lt.p, host, sizeof(double) * size, hipMemcpyHostToDevice, s));
             HIP_TRY(hipStreamSynchronize(s));
             return ASTRO_OK;
         }
-        std::vector<double> tmp(size, 0.0);
+        std::vector<LT> tmp(size, (LT)0);
         int64_t pos = 0;
         for (int r = 0; r < n; ++r) {
             const int64_t len = r - hS[r] + 1;
-            std::memcpy(tmp.data() + hOff[r] + hS[r], host + pos, sizeof(double) * len);
+            LT* dst = tmp.data() + hOff[r] + hS[r];
+            for (int64_t q = 0; q < len; ++q) dst[q] = (LT)host[pos + q];
             pos += len;
         }
-        HIP_TRY(hipMemcpyAsync(lt.p, tmp.data(), sizeof(double) * size, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(lt.p, tmp.data(), sizeof(LT) * size, hipMemcpyHostToDevice, s));
         HIP_TRY(hipStreamSynchronize(s));
         return ASTRO_OK;
     }
@@ -411,17 +433,18 @@ struct DeviceProfile {
         const dim3 gOld(maxTiles * splitK, nBlocks), gLead(nBlocks * maxTiles * (splitK + 1));
         const dim3 gRows(2 * nBlocks * std::max(1, maxTiles - 1));
         // panel 0 has no old part: its slots are written as zeros
-        hipLaunchKernelGGL(k_ldl_update_mfma, gOld, dim3(256), 0, s, panels.p, 0, panelsPerBlock, splitK, LDL_K_OLD, 0,
-                           lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(0), scratchRows);
+        hipLaunchKernelGGL(k_ldl_update_mfma<LT>, gOld, dim3(256), 0, s, panels.p, 0, panelsPerBlock, splitK,
+                           LDL_K_OLD, 0, (const LT*)lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(0), scratchRows);
         for (int q = 0; q < panelsPerBlock; ++q) {
-            hipLaunchKernelGGL(k_ldl_lead, gLead, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles,
+            hipLaunchKernelGGL(k_ldl_lead<LT>, gLead, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles,
                                splitK, lt.p, rowoff.p, sfirst.p, winMin.p, diag.p, buf(q), buf(q + 1), scratchRows,
                                zeroPivot.p, ftX.p);
             if (maxTiles > 1)
-                hipLaunchKernelGGL(k_ldl_rows, gRows, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks, maxTiles,
-                                   splitK, lt.p, rowoff.p, sfirst.p, buf(q), scratchRows, ftX.p);
+                hipLaunchKernelGGL(k_ldl_rows<LT>, gRows, dim3(256), 0, s, panels.p, q, panelsPerBlock, nBlocks,
+                                   maxTiles, splitK, lt.p, rowoff.p, sfirst.p, (const LT*)buf(q), scratchRows, ftX.p);
         }
         HIP_TRY(hipGetLastError());
+        factored = true;
         ASTRO_TRY(prepare_blocks(s));
         int zp = 0;
         HIP_TRY(hipMemcpyAsync(&zp, zeroPivot.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -434,8 +457,8 @@ struct DeviceProfile {
     // writer of lt other than factor() clears blocksReady.
     int prepare_blocks(hipStream_t s) {
         if (numTiles > 0) {
-            hipLaunchKernelGGL(k_sweep_blocks, dim3(numTiles), dim3(256), 0, s, blockInfo.p, lt.p, rowoff.p,
-                               sfirst.p, blocks.p);
+            hipLaunchKernelGGL(k_sweep_blocks<LT>, dim3(numTiles), dim3(256), 0, s, blockInfo.p, (const LT*)lt.p,
+                               rowoff.p, sfirst.p, blocks.p);
             hipLaunchKernelGGL(k_sweep_blocks_pair, dim3(numTiles), dim3(256), 0, s, blockInfo.p, blocks.p);
         }
         HIP_TRY(hipGetLastError());
@@ -456,8 +479,9 @@ struct DeviceProfile {
             evKind.push_back(0);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
-        hipLaunchKernelGGL(k_sweep_forward, dim3(numTiles), dim3(256), sweepLdsFwd, s, fwdTiles.p, numTiles, lt.p, rowoff.p,
-                           sfirst.p, blocks.p, b, z.p, gnear.p, fwdFlags.p, ticket.p, fwdEpoch, spinLimit, timeout.p);
+        hipLaunchKernelGGL(k_sweep_forward<LT>, dim3(numTiles), dim3(256), sweepLdsFwd, s, fwdTiles.p, numTiles,
+                           (const LT*)lt.p, rowoff.p, sfirst.p, (const double*)blocks.p, b, z.p, gnear.p, fwdFlags.p,
+                           ticket.p, fwdEpoch, spinLimit, timeout.p);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         ++bwdEpoch;
         hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, gnear.p, (int64_t)n);
@@ -466,9 +490,9 @@ struct DeviceProfile {
             evKind.push_back(1);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
-        hipLaunchKernelGGL(k_sweep_backward, dim3(numTiles), dim3(256), sweepLdsBwd, s, bwdTiles.p, numTiles, lt.p,
-                           rowoff.p, sfirst.p, blocks.p, z.p, x, gnear.p, bwdFlags.p, ticket.p, bwdEpoch, spinLimit,
-                           timeout.p);
+        hipLaunchKernelGGL(k_sweep_backward<LT>, dim3(numTiles), dim3(256), sweepLdsBwd, s, bwdTiles.p, numTiles,
+                           (const LT*)lt.p, rowoff.p, sfirst.p, (const double*)blocks.p, (const double*)z.p, x,
+                           gnear.p, bwdFlags.p, ticket.p, bwdEpoch, spinLimit, timeout.p, (const double*)diag.p);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
@@ -528,6 +552,9 @@ struct DeviceMpcT {
         return ASTRO_OK;
     }
 };
+
+using DeviceProfile = DeviceProfileT<double>;
+using DeviceProfile32 = DeviceProfileT<float>;
 
 // CSR operator with optional penalty rows (device arrays, 1-based columns).
 struct DeviceOperator {
@@ -611,7 +638,8 @@ struct Gmres {
     }
     double* Vcol(int q) { return V.p + (size_t)q * n; }
 
-    int solve(const DeviceOperator& A, DeviceProfile& P, const double* b, double* x, double rtol, double atol,
+    template <class Prof>
+    int solve(const DeviceOperator& A, Prof& P, const double* b, double* x, double rtol, double atol,
               int maxCycles, hipStream_t s) {
         steps = cycles = 0;
         const char* ev = std::getenv("ASTRO_VERBOSE");
@@ -774,6 +802,8 @@ struct astro_ctx {
     DBuf<double> csrVal, mpcMat, mpcRhs, loadVals, load, delDisp, nodeDisp;
     DBuf<unsigned long long> maxBits;
     DeviceProfile prof;
+    DeviceProfile32 prof32;  // the elastic profile in FP32 storage (precision = 1)
+    bool mixed = false;      // precision = 1: the FP32 profiles are the ones in use
     DeviceOperator op;
     DeviceMpcT mpcT;
     Gmres gm;
@@ -781,6 +811,7 @@ struct astro_ctx {
     DBuf<int> tRange, tCols, tMpcRange, tMpcCols;
     DBuf<double> tVal, tMpcMat, tMpcRhs, tLoad, tDel, nodeTemp;
     DeviceProfile tprof;
+    DeviceProfile32 tprof32;
     DeviceOperator top;
     DeviceMpcT tMpcT;
     Gmres tgm;
@@ -800,6 +831,17 @@ struct astro_ctx {
         if (s) (void)hipStreamDestroy(s);
     }
 };
+
+// Call f with the elastic (thermal) profile in use: FP64, or FP32 storage
+// with precision = 1.
+template <class F>
+static int with_prof(astro_ctx* c, F f) {
+    return c->mixed ? f(c->prof32) : f(c->prof);
+}
+template <class F>
+static int with_tprof(astro_ctx* c, F f) {
+    return c->mixed ? f(c->tprof32) : f(c->tprof);
+}
 
 extern "C" {
 
@@ -914,7 +956,7 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
     HIP_TRY(hipMemsetAsync(c->delDisp.p, 0, sizeof(double) * n, s));
     HIP_TRY(hipMemsetAsync(c->nodeDisp.p, 0, sizeof(double) * n, s));
     ASTRO_TRY(c->maxBits.alloc(1));
-    ASTRO_TRY(c->prof.setup(p.elMatLTRange.data(), n, s, true));
+    ASTRO_TRY(with_prof(c, [&](auto& P) { return P.setup(p.elMatLTRange.data(), n, s, true); }));
     c->op.n = n;
     c->op.cdim = p.elMPCDim;
     c->op.val = c->csrVal.p;
@@ -955,6 +997,8 @@ int astro_ctx_create(const astro_model* m, const astro_options* o, astro_ctx** o
     if (o) c->opt = *o;
     if (c->opt.rtol <= 0) c->opt.rtol = 1e-12;
     if (c->opt.maxCycles <= 0) c->opt.maxCycles = 1000;
+    if (c->opt.precision < 0 || c->opt.precision > 1) return fail(ASTRO_ERR_ARG, "precision must be 0 or 1");
+    c->mixed = c->opt.precision == 1;
     HIP_TRY(hipGetDevice(&c->dev));
     HIP_TRY(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
@@ -1074,7 +1118,8 @@ int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
     if (s == "gmresCycles") return c->gm.cycles;
     if (s == "thermGmresSteps") return c->tgm.steps;
     if (s == "numColors") return (int64_t)c->colorStart.size() - 1;
-    if (s == "profileBlocks") return c->prof.nBlocks;
+    if (s == "profileBlocks") return c->mixed ? c->prof32.nBlocks : c->prof.nBlocks;
+    if (s == "precision") return c->mixed ? 1 : 0;
     if (s == "numVecs") return c->numVecs;
     if (s == "dynSteps") return c->dynSteps;
     if (s == "dynGmresSteps") return c->dynGmresSteps;
@@ -1122,14 +1167,20 @@ static double* dev_double_array(astro_ctx* c, const std::string& s, int64_t& len
     };
     const Prep& p = c->prep;
     if (s == "elasticMat") return pick(c->csrVal, (int64_t)p.elMatCols.size());
-    if (s == "elMatLT") return pick(c->prof.lt, c->prof.packedSize);
+    if (s == "elMatLT") {
+        len = c->mixed ? c->prof32.packedSize : c->prof.packedSize;
+        return nullptr;  // profiles cross through get_packed / set_packed
+    }
     if (s == "elMPCMat") return pick(c->mpcMat, (int64_t)p.elMPCMat.size());
     if (s == "elMPCRHS") return pick(c->mpcRhs, (int64_t)p.elMPCRHS.size());
     if (s == "elasticLoad") return pick(c->load, p.elMatDim);
     if (s == "nodeDisp") return pick(c->nodeDisp, p.elMatDim);
     if (s == "delDisp") return pick(c->delDisp, p.elMatDim);
     if (s == "thermMat") return pick(c->tVal, (int64_t)p.thermMatCols.size());
-    if (s == "thermMatLT") return pick(c->tprof.lt, c->tprof.packedSize);
+    if (s == "thermMatLT") {
+        len = c->mixed ? c->tprof32.packedSize : c->tprof.packedSize;
+        return nullptr;
+    }
     if (s == "thermalLoad") return pick(c->tLoad, p.numNodes);
     if (s == "nodeTemp") return pick(c->nodeTemp, p.numNodes);
     if (s == "nodeVel") return pick(c->nodeVel, p.elMatDim);
@@ -1145,9 +1196,13 @@ int astro_ctx_get_f64(astro_ctx* c, const char* name, double* host, int64_t coun
     double* d = dev_double_array(c, name, len);
     if (len < 0) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
     if (len != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
+    const std::string nm(name);
+    if (nm == "elMatLT") return with_prof(c, [&](auto& P) { return P.get_packed(host, c->s); });
+    if (nm == "thermMatLT") {
+        if (!c->thermalReady) return fail(ASTRO_ERR_ARG, "array not built yet: thermMatLT");
+        return with_tprof(c, [&](auto& P) { return P.get_packed(host, c->s); });
+    }
     if (!d) return fail(ASTRO_ERR_ARG, std::string("array not built yet: ") + name);
-    if (d == c->prof.lt.p) return c->prof.get_packed(host, c->s);
-    if (d == c->tprof.lt.p) return c->tprof.get_packed(host, c->s);
     HIP_TRY(hipMemcpyAsync(host, d, sizeof(double) * count, hipMemcpyDeviceToHost, c->s));
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
@@ -1157,10 +1212,15 @@ int astro_ctx_set_f64(astro_ctx* c, const char* name, const double* host, int64_
     if (!c || !name || !host) return fail(ASTRO_ERR_ARG, "null argument");
     int64_t len = 0;
     double* d = dev_double_array(c, name, len);
-    if (len < 0 || !d) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
+    if (len < 0) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
     if (len != count) return fail(ASTRO_ERR_ARG, std::string("size mismatch for ") + name);
-    if (d == c->prof.lt.p) return c->prof.set_packed(host, c->s);
-    if (d == c->tprof.lt.p) return c->tprof.set_packed(host, c->s);
+    const std::string nm(name);
+    if (nm == "elMatLT") return with_prof(c, [&](auto& P) { return P.set_packed(host, c->s); });
+    if (nm == "thermMatLT") {
+        if (!c->thermalReady) return fail(ASTRO_ERR_ARG, "array not built yet: thermMatLT");
+        return with_tprof(c, [&](auto& P) { return P.set_packed(host, c->s); });
+    }
+    if (!d) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
     HIP_TRY(hipMemcpyAsync(d, host, sizeof(double) * count, hipMemcpyHostToDevice, c->s));
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
@@ -1207,25 +1267,37 @@ int astro_ctx_scale_mpc(astro_ctx* c) {
     return ASTRO_OK;
 }
 
-int astro_ctx_build_profile(astro_ctx* c) {
-    if (!c) return fail(ASTRO_ERR_ARG, "null context");
-    DeviceProfile& P = c->prof;
-    const int n = c->prep.elMatDim;
+// convertToLTri on the device: CSR values plus the penalty C^T C into the
+// profile of storage type LT (Working/AStrO_bookKeeping.f90:754-792).
+extern "C++" {
+template <typename LT>
+static int profile_from_csr(DeviceProfileT<LT>& P, hipStream_t s, const double* val, const int* cols,
+                            const int* range, int n, const double* cmat, const int* ccols, const int* crange,
+                            int cd) {
     P.blocksReady = false;
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, c->s, P.lt.p, P.size, 0.0);
-    hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, c->s,
-                       c->csrVal.p, c->csrCols.p, c->csrRange.p, n, P.sfirst.p, P.rowoff.p, P.lt.p);
-    const int cd = c->prep.elMPCDim;
+    P.factored = false;
+    hipLaunchKernelGGL(k_fill_t<LT>, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, s, P.lt.p, P.size, (LT)0);
+    hipLaunchKernelGGL(k_csr_to_profile<LT>, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val,
+                       cols, range, n, (const int*)P.sfirst.p, (const int64_t*)P.rowoff.p, P.lt.p);
     if (cd > 0)
-        hipLaunchKernelGGL(k_penalty_to_profile, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p,
-                           c->mpcCols.p, c->mpcRange.p, cd, P.sfirst.p, P.rowoff.p, P.lt.p);
+        hipLaunchKernelGGL(k_penalty_to_profile<LT>, dim3(grid_for(cd, 256)), dim3(256), 0, s, cmat, ccols, crange,
+                           cd, (const int*)P.sfirst.p, (const int64_t*)P.rowoff.p, P.lt.p);
     HIP_TRY(hipGetLastError());
     return ASTRO_OK;
+}
+}  // extern "C++"
+
+int astro_ctx_build_profile(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    return with_prof(c, [&](auto& P) {
+        return profile_from_csr(P, c->s, c->csrVal.p, c->csrCols.p, c->csrRange.p, c->prep.elMatDim, c->mpcMat.p,
+                                c->mpcCols.p, c->mpcRange.p, c->prep.elMPCDim);
+    });
 }
 
 int astro_ctx_factor(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
-    return c->prof.factor(c->s);
+    return with_prof(c, [&](auto& P) { return P.factor(c->s); });
 }
 
 // load = -R(u) for the linear static model, R(u) = K u with K the tangent at
@@ -1261,7 +1333,9 @@ int astro_ctx_solve(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     ASTRO_TRY(ctx_rhs(c));
     const int n = c->prep.elMatDim;
-    ASTRO_TRY(c->gm.solve(c->op, c->prof, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s));
+    ASTRO_TRY(with_prof(c, [&](auto& P) {
+        return c->gm.solve(c->op, P, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s);
+    }));
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->nodeDisp.p, c->delDisp.p,
                        c->nodeDisp.p, n, 1.0, 1.0);
     HIP_TRY(hipGetLastError());
@@ -1274,6 +1348,11 @@ int astro_ctx_reset(astro_ctx* c) {
     const int n = c->prep.elMatDim;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->delDisp.p, (int64_t)n, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->nodeDisp.p, (int64_t)n, 0.0);
+    if (c->thermalReady) {
+        const int N = c->prep.numNodes;
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, c->s, c->tDel.p, (int64_t)N, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, c->s, c->nodeTemp.p, (int64_t)N, 0.0);
+    }
     HIP_TRY(hipGetLastError());
     return ASTRO_OK;
 }
@@ -1360,7 +1439,9 @@ int astro_ctx_dynamic_step(astro_ctx* c, double loadScale) {
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
                            c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p);
     HIP_TRY(hipGetLastError());
-    ASTRO_TRY(c->gm.solve(c->op, c->prof, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s));
+    ASTRO_TRY(with_prof(c, [&](auto& P) {
+        return c->gm.solve(c->op, P, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s);
+    }));
     hipLaunchKernelGGL(k_newmark_update, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, n, c->nodeDisp.p,
                        c->delDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c->nodeVel.p, c->nodeAcc.p, c1, c2, dt,
                        beta, c->nmGamma);
@@ -1485,7 +1566,7 @@ static int ctx_thermal_setup(astro_ctx* c) {
     ASTRO_TRY(c->nodeTemp.alloc(N));
     HIP_TRY(hipMemsetAsync(c->tDel.p, 0, sizeof(double) * N, s));
     HIP_TRY(hipMemsetAsync(c->nodeTemp.p, 0, sizeof(double) * N, s));
-    ASTRO_TRY(c->tprof.setup(p.thermMatLTRange.data(), N, s, true));
+    ASTRO_TRY(with_tprof(c, [&](auto& P) { return P.setup(p.thermMatLTRange.data(), N, s, true); }));
     c->top.n = N;
     c->top.cdim = p.thermMPCDim;
     c->top.val = c->tVal.p;
@@ -1532,26 +1613,21 @@ int astro_ctx_thermal_solve(astro_ctx* c) {
     if (cd > 0)
         hipLaunchKernelGGL(k_scale_mpc, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p, c->tMpcRhs.p,
                            c->tMpcRange.p, cd, c->maxBits.p);
-    DeviceProfile& P = c->tprof;
-    P.blocksReady = false;
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, s, P.lt.p, P.size, 0.0);
-    hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)N * 32, 256, 1 << 30)), dim3(256), 0, s, c->tVal.p,
-                       c->tCols.p, c->tRange.p, N, P.sfirst.p, P.rowoff.p, P.lt.p);
-    if (cd > 0)
-        hipLaunchKernelGGL(k_penalty_to_profile, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p,
-                           c->tMpcCols.p, c->tMpcRange.p, cd, P.sfirst.p, P.rowoff.p, P.lt.p);
-    HIP_TRY(hipGetLastError());
-    ASTRO_TRY(P.factor(s));
-    ctx_minus_internal_force(s, c->tVal, c->tCols, c->tRange, N, c->nodeTemp.p, c->tLoad.p);
-    if (cd > 0)
-        hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p, c->tMpcCols.p,
-                           c->tMpcRange.p, c->tMpcRhs.p, cd, c->nodeTemp.p, c->tLoad.p);
-    ASTRO_TRY(c->tgm.solve(c->top, P, c->tLoad.p, c->tDel.p, c->opt.rtol, 0.0, c->opt.maxCycles, s));
-    hipLaunchKernelGGL(k_axpby, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->nodeTemp.p, c->tDel.p,
-                       c->nodeTemp.p, N, 1.0, 1.0);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s));
-    return ASTRO_OK;
+    return with_tprof(c, [&](auto& P) -> int {
+        ASTRO_TRY(profile_from_csr(P, s, c->tVal.p, c->tCols.p, c->tRange.p, N, c->tMpcMat.p, c->tMpcCols.p,
+                                   c->tMpcRange.p, cd));
+        ASTRO_TRY(P.factor(s));
+        ctx_minus_internal_force(s, c->tVal, c->tCols, c->tRange, N, c->nodeTemp.p, c->tLoad.p);
+        if (cd > 0)
+            hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p,
+                               c->tMpcCols.p, c->tMpcRange.p, c->tMpcRhs.p, cd, c->nodeTemp.p, c->tLoad.p);
+        ASTRO_TRY(c->tgm.solve(c->top, P, c->tLoad.p, c->tDel.p, c->opt.rtol, 0.0, c->opt.maxCycles, s));
+        hipLaunchKernelGGL(k_axpby, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->nodeTemp.p, c->tDel.p,
+                           c->nodeTemp.p, N, 1.0, 1.0);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));
+        return ASTRO_OK;
+    });
 }
 
 int astro_ctx_thermal_soln_load(astro_ctx* c) {
@@ -1567,8 +1643,10 @@ int astro_ctx_precond_apply(astro_ctx* c, const double* d_b, double* d_x) {
     if (!c || !d_b || !d_x) return fail(ASTRO_ERR_ARG, "null argument");
     // the sweeps mark x unproduced before the forward sweep reads b
     if (d_b == d_x) return fail(ASTRO_ERR_ARG, "precond_apply needs distinct b and x buffers");
-    ASTRO_TRY(c->prof.apply(d_b, d_x, c->s));
-    return c->prof.check_timeout(c->s);
+    return with_prof(c, [&](auto& P) -> int {
+        ASTRO_TRY(P.apply(d_b, d_x, c->s));
+        return P.check_timeout(c->s);
+    });
 }
 
 int astro_ctx_operator_apply(astro_ctx* c, const double* d_x, double* d_y) {
@@ -1581,17 +1659,19 @@ int astro_ctx_operator_apply(astro_ctx* c, const double* d_x, double* d_y) {
 int astro_ctx_set_timing(astro_ctx* c, int on) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     HIP_TRY(hipStreamSynchronize(c->s));
-    c->prof.collect();
-    c->prof.timing = on != 0;
-    c->prof.fwdMs = c->prof.bwdMs = 0;
-    c->prof.fwdCount = c->prof.bwdCount = 0;
+    with_prof(c, [&](auto& P) {
+        P.collect();
+        P.timing = on != 0;
+        P.fwdMs = P.bwdMs = 0;
+        P.fwdCount = P.bwdCount = 0;
+        return ASTRO_OK;
+    });
     return ASTRO_OK;
 }
 
-double astro_ctx_kernel_stat(const astro_ctx* c, const char* name) {
-    if (!c || !name) return NAN;
-    std::string s(name);
-    const DeviceProfile& P = c->prof;
+extern "C++" {
+template <class Prof>
+static double profile_stat(const Prof& P, const std::string& s) {
     if (s == "sweep_fwd_ms") return P.fwdMs;
     if (s == "sweep_bwd_ms") return P.bwdMs;
     if (s == "sweep_fwd_count") return (double)P.fwdCount;
@@ -1600,6 +1680,13 @@ double astro_ctx_kernel_stat(const astro_ctx* c, const char* name) {
     if (s == "panels_per_block") return (double)P.panelsPerBlock;
     if (s == "split_k") return (double)P.splitK;
     return NAN;
+}
+}  // extern "C++"
+
+double astro_ctx_kernel_stat(const astro_ctx* c, const char* name) {
+    if (!c || !name) return NAN;
+    const std::string s(name);
+    return c->mixed ? profile_stat(c->prof32, s) : profile_stat(c->prof, s);
 }
 
 int astro_ctx_true_residual(astro_ctx* c, double* relRes) {
@@ -1903,10 +1990,10 @@ int astro_convertToLTri(double* lTri, const int32_t* lTRange, int64_t lTSize, co
         if (rc == ASTRO_OK && cDim > 0) rc = cr.upload(cRange, cDim + 1, s);
         if (rc == ASTRO_OK) {
             hipLaunchKernelGGL(k_fill, dim3(grid_for(sz, 256, 16384)), dim3(256), 0, s, lt.p, sz, 0.0);
-            hipLaunchKernelGGL(k_csr_to_profile, dim3(grid_for((int64_t)mainDim * 32, 256, 1 << 30)), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_csr_to_profile<double>, dim3(grid_for((int64_t)mainDim * 32, 256, 1 << 30)), dim3(256), 0, s,
                                mv.p, mc.p, mr.p, mainDim, sfd.p, offd.p, lt.p);
             if (cDim > 0)
-                hipLaunchKernelGGL(k_penalty_to_profile, dim3(grid_for(cDim, 256)), dim3(256), 0, s, cv.p, cc.p, cr.p,
+                hipLaunchKernelGGL(k_penalty_to_profile<double>, dim3(grid_for(cDim, 256)), dim3(256), 0, s, cv.p, cc.p, cr.p,
                                    cDim, sfd.p, offd.p, lt.p);
             if (hipGetLastError() != hipSuccess ||
                 hipMemcpyAsync(lTri, lt.p, sizeof(double) * sz, hipMemcpyDeviceToHost, s) != hipSuccess ||

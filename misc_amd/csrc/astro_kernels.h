@@ -77,33 +77,50 @@ enum { LDL_K_ALL = 0, LDL_K_OLD = 1, LDL_K_NEW = 2 };
 // Blocked left-looking LDL^T: step U alone (kMode 0 all columns before the
 // panel, 1 all but the previous panel, 2 the previous panel only) and the fused
 // step (FT of panelIdx, "old" U of panelIdx + 1).
+// The profile kernels are templates on the storage type LT of the profile
+// (and of step U's partial sums): double for the FP64 path, float for the
+// mixed-precision factor (FP32 L and step U on FP32 matrix cores, FP64
+// pivots in diag, FP64 diagonal-block factor and vectors).  Instantiated for
+// both in astro_kernels.hip.
+template <typename LT>
 __global__ void k_ldl_update_mfma(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int nSplit, int kMode,
-                                  int slot0, const double* lt, const int64_t* rowoff, const int* sfirst,
-                                  const int* winMin, const double* diag, double* scratch, int scratchRows);
+                                  int slot0, const LT* lt, const int64_t* rowoff, const int* sfirst,
+                                  const int* winMin, const double* diag, LT* scratch, int scratchRows);
+template <typename LT>
 __global__ void k_ldl_lead(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
-                           double* lt, const int64_t* rowoff, const int* sfirst, const int* winMin, double* diag,
-                           double* bufQ, double* bufQ1, int scratchRows, int* zeroPivot, double* ftX);
+                           LT* lt, const int64_t* rowoff, const int* sfirst, const int* winMin, double* diag,
+                           LT* bufQ, LT* bufQ1, int scratchRows, int* zeroPivot, double* ftX);
+template <typename LT>
 __global__ void k_ldl_rows(const PanelInfo* panels, int panelIdx, int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
-                           double* lt, const int64_t* rowoff, const int* sfirst, const double* bufQ, int scratchRows,
+                           LT* lt, const int64_t* rowoff, const int* sfirst, const LT* bufQ, int scratchRows,
                            const double* ftX);
 __global__ void k_max_abs(const double* v, int64_t n, unsigned long long* out);
 __global__ void k_scale_mpc(double* cmat, double* rhs, const int* crange, int cdim, const unsigned long long* maxBits);
 __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_fill_i32(int* p, int64_t n, int v);
 __global__ void k_fill_sentinel(double* p, int64_t n);
+template <typename T>
+__global__ void k_fill_t(T* p, int64_t n, T v);
+template <typename LT>
 __global__ void k_csr_to_profile(const double* val, const int* cols, const int* range, int n, const int* sfirst,
-                                 const int64_t* rowoff, double* lt);
+                                 const int64_t* rowoff, LT* lt);
+template <typename LT>
 __global__ void k_penalty_to_profile(const double* cmat, const int* ccols, const int* crange, int cdim,
-                                     const int* sfirst, const int64_t* rowoff, double* lt);
-__global__ void k_sweep_blocks(const SweepBlock* blocks, const double* lt, const int64_t* rowoff, const int* sfirst,
+                                     const int* sfirst, const int64_t* rowoff, LT* lt);
+template <typename LT>
+__global__ void k_sweep_blocks(const SweepBlock* blocks, const LT* lt, const int64_t* rowoff, const int* sfirst,
                                double* out);
 __global__ void k_sweep_blocks_pair(const SweepBlock* blocks, double* out);
-__global__ void k_sweep_forward(const SweepTile* tiles, int numTiles, const double* lt, const int64_t* rowoff,
+template <typename LT>
+__global__ void k_sweep_forward(const SweepTile* tiles, int numTiles, const LT* lt, const int64_t* rowoff,
                                 const int* sfirst, const double* blocks, const double* b, double* z, double* g,
                                 int* flags, int* ticket, int epoch, long spinLimit, int* timeout);
-__global__ void k_sweep_backward(const SweepTile* tiles, int numTiles, const double* lt, const int64_t* rowoff,
+// diag: FP64 pivots, read by the FP32 instantiation (the FP64 one reads them
+// from the profile's diagonal)
+template <typename LT>
+__global__ void k_sweep_backward(const SweepTile* tiles, int numTiles, const LT* lt, const int64_t* rowoff,
                                  const int* sfirst, const double* blocks, const double* z, double* x, double* g,
-                                 int* flags, int* ticket, int epoch, long spinLimit, int* timeout);
+                                 int* flags, int* ticket, int epoch, long spinLimit, int* timeout, const double* diag);
 __global__ void k_reset_ticket(int* ticket);
 __global__ void k_spmv(const double* val, const int* cols, const int* range, int n, const double* x, double* y);
 __global__ void k_mpc_dd(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,

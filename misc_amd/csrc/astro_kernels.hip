@@ -726,10 +726,11 @@ __global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
 
 // In-profile CSR entries into the (zeroed) profile: row r keeps columns
 // s[r] <= c <= r.  Working/AStrO_bookKeeping.f90:764-774.  32 lanes per row.
+template <typename LT>
 __global__ __launch_bounds__(256) void k_csr_to_profile(const double* __restrict__ val, const int* __restrict__ cols,
                                                         const int* __restrict__ range, int n,
                                                         const int* __restrict__ sfirst,
-                                                        const int64_t* __restrict__ rowoff, double* __restrict__ lt) {
+                                                        const int64_t* __restrict__ rowoff, LT* __restrict__ lt) {
     const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     const int l = threadIdx.x & 31;
     if (row >= n) return;
@@ -737,15 +738,20 @@ __global__ __launch_bounds__(256) void k_csr_to_profile(const double* __restrict
     const int64_t off = rowoff[row];
     for (int q = range[row] + l; q < range[row + 1]; q += 32) {
         const int c = cols[q] - 1;
-        if (c >= s && c <= row) lt[off + c] += val[q];
+        if (c >= s && c <= row) lt[off + c] += (LT)val[q];
     }
 }
+template __global__ void k_csr_to_profile<double>(const double*, const int*, const int*, int, const int*,
+                                                  const int64_t*, double*);
+template __global__ void k_csr_to_profile<float>(const double*, const int*, const int*, int, const int*,
+                                                 const int64_t*, float*);
 
 // Penalty C^T C into the profile (Working/AStrO_bookKeeping.f90:776-790); one
 // thread per constraint row, atomics because rows may share a DOF.
+template <typename LT>
 __global__ void k_penalty_to_profile(const double* __restrict__ cmat, const int* __restrict__ ccols,
                                      const int* __restrict__ crange, int cdim, const int* __restrict__ sfirst,
-                                     const int64_t* __restrict__ rowoff, double* __restrict__ lt) {
+                                     const int64_t* __restrict__ rowoff, LT* __restrict__ lt) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= cdim) return;
     for (int s2 = crange[r]; s2 < crange[r + 1]; ++s2) {
@@ -753,10 +759,23 @@ __global__ void k_penalty_to_profile(const double* __restrict__ cmat, const int*
         if (i4 < 0) continue;
         for (int s3 = crange[r]; s3 < crange[r + 1]; ++s3) {
             const int i5 = ccols[s3] - 1;
-            if (i5 >= sfirst[i4] && i5 <= i4) atomicAdd(&lt[rowoff[i4] + i5], cmat[s2] * cmat[s3]);
+            if (i5 >= sfirst[i4] && i5 <= i4) atomicAdd(&lt[rowoff[i4] + i5], (LT)(cmat[s2] * cmat[s3]));
         }
     }
 }
+template __global__ void k_penalty_to_profile<double>(const double*, const int*, const int*, int, const int*,
+                                                      const int64_t*, double*);
+template __global__ void k_penalty_to_profile<float>(const double*, const int*, const int*, int, const int*,
+                                                     const int64_t*, float*);
+
+// Fill of a profile array (either storage precision).
+template <typename T>
+__global__ void k_fill_t(T* __restrict__ p, int64_t n, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+template __global__ void k_fill_t<double>(double*, int64_t, double);
+template __global__ void k_fill_t<float>(float*, int64_t, float);
 
 // gfx950 cross-lane moves on doubles.  swap_halves32(a, c): lanes 32..63 of
 // a take lanes 0..31 of c and lanes 0..31 of c take lanes 32..63 of a, so
@@ -908,6 +927,118 @@ __device__ __forceinline__ void mfma_block64(const double (*A)[65], const double
         }
 }
 
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// Step U for a profile stored in FP32 (the mixed-precision factor): the same
+// tiling, k range and staging as ldl_update_body below, with FP32 operands
+// staged in LDS, v_mfma_f32_16x16x4f32 with FP32 accumulation, and FP32
+// partial sums.  B = L(P, k) D_k is formed in FP64 (D is FP64) and rounded
+// once.  Result layout (measured on gfx950): D(i, j) in lane j + 16 (i / 4),
+// register i % 4.
+__device__ __forceinline__ void ldl_update_body_f32(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
+                                                    int kMode, int slot, const float* __restrict__ lt,
+                                                    const int64_t* __restrict__ rowoff,
+                                                    const int* __restrict__ sfirst, const int* __restrict__ winMin,
+                                                    const double* __restrict__ diag, float* __restrict__ scratch,
+                                                    int scratchRows, char* smem) {
+    const int r0 = P.p + 64 * tile;
+    if (r0 >= P.iend) return;
+    const int nr = min(64, P.iend - r0);
+    const int np = P.pe - P.p;
+    constexpr int KC = LDL_KC;
+    float(*As)[65] = reinterpret_cast<float(*)[65]>(smem);
+    float(*Bs)[65] = reinterpret_cast<float(*)[65]>(smem + KC * 65 * 4);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int kLo = max(winMin[r0], P.smin), kHi = P.p;
+    if (kMode == LDL_K_OLD) kHi = P.p - 64;
+    if (kMode == LDL_K_NEW) kLo = max(kLo, P.p - 64);
+    kLo = P.p - ((P.p - kLo + 31) / 32) * 32;
+    const int total = max(0, kHi - kLo);
+    int chunk = (total + nSplit - 1) / nSplit;
+    chunk = (chunk + KC - 1) / KC * KC;
+    const int kb = kLo + sp * chunk, ke = min(kHi, kb + chunk);
+    const int kl = lane & 31;
+    int aR[8], aS0[8], aS[8], bR[8], bS0[8], bS[8];
+    int64_t aOff[8], bOff[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int sr = 16 * w + 2 * i + (lane >> 5);
+        const bool aLive = sr < nr, bLive = sr < np;
+        aR[i] = aLive ? r0 + sr : r0;
+        bR[i] = bLive ? P.p + sr : P.p;
+        aS0[i] = sfirst[aR[i]];
+        bS0[i] = sfirst[bR[i]];
+        aOff[i] = rowoff[aR[i]];
+        bOff[i] = rowoff[bR[i]];
+        aS[i] = aLive ? aS0[i] : 0x7fffffff;
+        bS[i] = bLive ? bS0[i] : 0x7fffffff;
+    }
+    float4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    float va0[8], vb0[8], va1[8], vb1[8];
+    double vd0 = 0.0, vd1 = 0.0;
+    auto issue = [&](int k0, float (&va)[8], float (&vb)[8], double& vd) {
+        const int k = k0 + kl;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            va[i] = lt[aOff[i] + min(max(k, aS0[i]), aR[i])];
+            vb[i] = lt[bOff[i] + min(max(k, bS0[i]), bR[i])];
+        }
+        vd = diag[min(k0 + kl, ke - 1)];
+    };
+    auto step = [&](int k0, float (&va)[8], float (&vb)[8], double& vd) {
+        __syncthreads();
+        {
+            const int k = k0 + kl;
+            const bool kv = k < ke;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int sr = 16 * w + 2 * i + (lane >> 5);
+                As[kl][sr] = (kv && k >= aS[i]) ? va[i] : 0.0f;
+                Bs[kl][sr] = (kv && k >= bS[i]) ? (float)((double)vb[i] * vd) : 0.0f;
+            }
+        }
+        __syncthreads();
+        if (k0 + 2 * KC < ke) issue(k0 + 2 * KC, va, vb, vd);
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            const float av = As[kk][w * 16 + (lane & 15)];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const float bv = Bs[kk][cb * 16 + (lane & 15)];
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[cb], 0, 0, 0);
+            }
+        }
+    };
+    if (kb < ke) issue(kb, va0, vb0, vd0);
+    if (kb + KC < ke) issue(kb + KC, va1, vb1, vd1);
+    for (int k0 = kb; k0 < ke; k0 += 2 * KC) {
+        step(k0, va0, vb0, vd0);
+        if (k0 + KC < ke) step(k0 + KC, va1, vb1, vd1);
+    }
+    float* out = scratch + (((size_t)slot * nb + blk) * scratchRows + (r0 - P.p)) * 64;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = w * 16 + 4 * (lane >> 4) + v;
+            const int j = cb * 16 + (lane & 15);
+            if (i < nr) out[(size_t)i * 64 + j] = acc[cb][v];
+        }
+}
+
+__device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
+                                                int kMode, int slot, const float* __restrict__ lt,
+                                                const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
+                                                const int* __restrict__ winMin,
+                                                const double* __restrict__ diag, float* __restrict__ scratch,
+                                                int scratchRows, char* smem) {
+    ldl_update_body_f32(P, blk, nb, tile, sp, nSplit, kMode, slot, lt, rowoff, sfirst, winMin, diag, scratch,
+                        scratchRows, smem);
+}
+
 __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int nb, int tile, int sp, int nSplit,
                                                 int kMode, int slot, const double* __restrict__ lt,
                                                 const int64_t* __restrict__ rowoff, const int* __restrict__ sfirst,
@@ -1020,14 +1151,15 @@ __device__ __forceinline__ void ldl_update_body(const PanelInfo P, int blk, int 
 
 // Step U alone (the first panel's zero partials, and the "new" part): grid
 // (tiles x nSplit, blocks); partial sp of the k range goes to slot slot0 + sp.
+template <typename LT>
 __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __restrict__ panels, int panelIdx,
                                                          int numPanelsPerBlock, int nSplit, int kMode, int slot0,
-                                                         const double* __restrict__ lt,
+                                                         const LT* __restrict__ lt,
                                                          const int64_t* __restrict__ rowoff,
                                                          const int* __restrict__ sfirst,
                                                          const int* __restrict__ winMin,
                                                          const double* __restrict__ diag,
-                                                         double* __restrict__ scratch, int scratchRows) {
+                                                         LT* __restrict__ scratch, int scratchRows) {
     __shared__ __align__(16) char smem[LDL_U_SMEM_BYTES];
     const int blk = blockIdx.y;
     const PanelInfo P = panels[blk * numPanelsPerBlock + panelIdx];
@@ -1037,6 +1169,11 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
     ldl_update_body(P, blk, gridDim.y, tile, sp, nSplit, kMode, slot0 + sp, lt, rowoff, sfirst, winMin, diag,
                     scratch, scratchRows, smem);
 }
+template __global__ void k_ldl_update_mfma<double>(const PanelInfo*, int, int, int, int, int, const double*,
+                                                   const int64_t*, const int*, const int*, const double*, double*,
+                                                   int);
+template __global__ void k_ldl_update_mfma<float>(const PanelInfo*, int, int, int, int, int, const float*,
+                                                  const int64_t*, const int*, const int*, const double*, float*, int);
 
 // Step FT of a panel P, in two parts.  In the diagonal part, thread (r = tid / 4, q = tid % 4)
 // holds 16 entries of one row in registers: columns q, q+4, ..., q+60 of the
@@ -1048,9 +1185,10 @@ __global__ __launch_bounds__(256) void k_ldl_update_mfma(const PanelInfo* __rest
 //    `diag`, and stores X = L_PP^-1 and 1 / D in Xg for the row tiles;
 //  - ldl_panel_rows (two workgroups per further 64-row tile, in the next
 //    launch): L(i, P) = rows X^T D^-1 on the matrix cores.
+template <typename LT>
 __device__ __forceinline__ void ldl_panel_diag(const PanelInfo P, int blk, int nb, int nSlots,
-                                               double* __restrict__ lt, const int64_t* __restrict__ rowoff,
-                                               const int* __restrict__ sfirst, const double* __restrict__ scratch,
+                                               LT* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                               const int* __restrict__ sfirst, const LT* __restrict__ scratch,
                                                int scratchRows, double* __restrict__ diag,
                                                int* __restrict__ zeroPivot, double* __restrict__ Xg, char* smem) {
     const int tile = 0;
@@ -1081,10 +1219,10 @@ __device__ __forceinline__ void ldl_panel_diag(const PanelInfo P, int blk, int n
             raw[m] = lt[off + min(max(c, si), gr)];
             ws[m] = 0.0;
         }
-        const double* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
+        const LT* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
         for (int q = 0; q < nSlots; ++q) {
 #pragma unroll
-            for (int m = 0; m < 16; ++m) ws[m] += sp[(size_t)q * qstride + 4 * m];
+            for (int m = 0; m < 16; ++m) ws[m] += (double)sp[(size_t)q * qstride + 4 * m];
         }
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -1171,7 +1309,7 @@ __device__ __forceinline__ void ldl_panel_diag(const PanelInfo P, int blk, int n
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int k = fq + 4 * m;
-                if (k <= fr && P.p + k >= dsi) lt[doff + P.p + k] = a[m];
+                if (k <= fr && P.p + k >= dsi) lt[doff + P.p + k] = (LT)a[m];
                 if (k == fr) diag[gi] = a[m];
             }
         }
@@ -1195,9 +1333,10 @@ __device__ __forceinline__ void ldl_panel_diag(const PanelInfo P, int blk, int n
 // Thread (r = tid / 8, q = tid % 8) gathers 8 entries of one row (columns q,
 // q+8, ..., q+56); wave w forms the 16 x 32 product block at rows 16 (w & 1),
 // columns 32 (w >> 1).
+template <typename LT>
 __device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int nb, int tile, int half, int nSlots,
-                                                    double* __restrict__ lt, const int64_t* __restrict__ rowoff,
-                                                    const int* __restrict__ sfirst, const double* __restrict__ scratch,
+                                                    LT* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                                    const int* __restrict__ sfirst, const LT* __restrict__ scratch,
                                                     int scratchRows, const double* __restrict__ Xg, char* smem) {
     const int r0 = P.p + 64 * tile + 32 * half;
     if (r0 >= P.iend) return;
@@ -1223,10 +1362,10 @@ __device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int n
         raw[m] = lt[off + min(max(c, si), gr)];
         ws[m] = 0.0;
     }
-    const double* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
+    const LT* sp = scratch + ((size_t)blk * scratchRows + rp) * 64 + fq;
     for (int q = 0; q < nSlots; ++q) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) ws[m] += sp[(size_t)q * qstride + 8 * m];
+        for (int m = 0; m < 8; ++m) ws[m] += (double)sp[(size_t)q * qstride + 8 * m];
     }
     for (int id = tid; id < 64 * 64; id += 256) Xi[id >> 6][id & 63] = Xg[id];
     if (tid < 64) rD[tid] = Xg[64 * 64 + tid];
@@ -1263,7 +1402,7 @@ __device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int n
 #pragma unroll
             for (int cb = 0; cb < 2; ++cb) {
                 const int j = (cb0 + cb) * 16 + (lane & 15);
-                if (j < np && P.p + j >= s0) lt[ro + P.p + j] = acc[cb][v] * rD[j];
+                if (j < np && P.p + j >= s0) lt[ro + P.p + j] = (LT)(acc[cb][v] * rD[j]);
             }
         }
     }
@@ -1281,12 +1420,13 @@ __device__ __forceinline__ void ldl_panel_rows(const PanelInfo P, int blk, int n
 //    order: the tiles nearest the diagonal have the longest column ranges
 //    and are dispatched first.
 // The row tiles of panel q follow in k_ldl_rows.
+template <typename LT>
 __global__ __launch_bounds__(256) void k_ldl_lead(const PanelInfo* __restrict__ panels, int panelIdx,
                                                   int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
-                                                  double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                                  LT* __restrict__ lt, const int64_t* __restrict__ rowoff,
                                                   const int* __restrict__ sfirst, const int* __restrict__ winMin,
-                                                  double* __restrict__ diag, double* __restrict__ bufQ,
-                                                  double* __restrict__ bufQ1, int scratchRows,
+                                                  double* __restrict__ diag, LT* __restrict__ bufQ,
+                                                  LT* __restrict__ bufQ1, int scratchRows,
                                                   int* __restrict__ zeroPivot, double* __restrict__ ftX) {
     __shared__ __align__(16) char smem[LDL_STEP_SMEM_BYTES];
     const int x = blockIdx.x;
@@ -1317,13 +1457,18 @@ __global__ __launch_bounds__(256) void k_ldl_lead(const PanelInfo* __restrict__ 
                         scratchRows, smem);
     }
 }
+template __global__ void k_ldl_lead<double>(const PanelInfo*, int, int, int, int, int, double*, const int64_t*,
+                                            const int*, const int*, double*, double*, double*, int, int*, double*);
+template __global__ void k_ldl_lead<float>(const PanelInfo*, int, int, int, int, int, float*, const int64_t*,
+                                           const int*, const int*, double*, float*, float*, int, int*, double*);
 
 // Factor launch R_q: the row tiles >= 1 of every block's panel q (grid
 // numBlocks * (numTiles - 1)), against the L_PP^-1 that L_q stored.
+template <typename LT>
 __global__ __launch_bounds__(256) void k_ldl_rows(const PanelInfo* __restrict__ panels, int panelIdx,
                                                   int numPanelsPerBlock, int numBlocks, int numTiles, int nSplit,
-                                                  double* __restrict__ lt, const int64_t* __restrict__ rowoff,
-                                                  const int* __restrict__ sfirst, const double* __restrict__ bufQ,
+                                                  LT* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                                  const int* __restrict__ sfirst, const LT* __restrict__ bufQ,
                                                   int scratchRows, const double* __restrict__ ftX) {
     __shared__ __align__(16) char smem[LDL_STEP_SMEM_BYTES];
     const int half = blockIdx.x & 1, idx = blockIdx.x >> 1;
@@ -1333,6 +1478,10 @@ __global__ __launch_bounds__(256) void k_ldl_rows(const PanelInfo* __restrict__ 
     ldl_panel_rows(P, blk, numBlocks, tile, half, nSplit + 1, lt, rowoff, sfirst, bufQ, scratchRows,
                    ftX + (size_t)blk * (64 * 64 + 64), smem);
 }
+template __global__ void k_ldl_rows<double>(const PanelInfo*, int, int, int, int, int, double*, const int64_t*,
+                                            const int*, const double*, int, const double*);
+template __global__ void k_ldl_rows<float>(const PanelInfo*, int, int, int, int, int, float*, const int64_t*,
+                                           const int*, const float*, int, const double*);
 
 // ---------------------------------------------------------------------------
 // Triangular sweeps on the factored profile, one 64-row tile per workgroup in
@@ -1455,8 +1604,9 @@ __device__ __forceinline__ void publish_flag(int* flag, int epoch) {
 
 // C[i][c] = L(rowStart + i, colStart + c) inside the profile (every column
 // used here lies left of its row), zero elsewhere and beyond nRows x nCols.
+template <typename LT>
 __device__ __forceinline__ void load_coupling(double (*C)[65], int rowStart, int nRows, int colStart, int nCols,
-                                              const double* __restrict__ lt, const int64_t* __restrict__ rowoff,
+                                              const LT* __restrict__ lt, const int64_t* __restrict__ rowoff,
                                               const int* __restrict__ sfirst) {
     for (int id = threadIdx.x; id < 64 * 64; id += 256) {
         const int i = id >> 6, c = id & 63;
@@ -1473,8 +1623,9 @@ __device__ __forceinline__ void load_coupling(double (*C)[65], int rowStart, int
 // on the matrix cores), then its products with the coupling blocks to the
 // two preceding and two following tiles.  Rows beyond the tile's end are the
 // identity in L_tt.
+template <typename LT>
 __global__ __launch_bounds__(256) void k_sweep_blocks(const SweepBlock* __restrict__ blocks,
-                                                      const double* __restrict__ lt,
+                                                      const LT* __restrict__ lt,
                                                       const int64_t* __restrict__ rowoff,
                                                       const int* __restrict__ sfirst, double* __restrict__ out) {
     __shared__ double L[64][65];  // strictly lower L_tt, then one coupling block at a time
@@ -1518,6 +1669,9 @@ __global__ __launch_bounds__(256) void k_sweep_blocks(const SweepBlock* __restri
     __syncthreads();
     mfma_block64<true, true>(X, L, nullptr, o + SWEEP_HB * SWEEP_BLOCK);
 }
+template __global__ void k_sweep_blocks<double>(const SweepBlock*, const double*, const int64_t*, const int*,
+                                                double*);
+template __global__ void k_sweep_blocks<float>(const SweepBlock*, const float*, const int64_t*, const int*, double*);
 
 // A column-major 64 x 64 block from global memory into As[i][c] (zero if null).
 __device__ __forceinline__ void load_block64(const double* __restrict__ src, double (*As)[65]) {
@@ -1666,8 +1820,9 @@ __device__ __forceinline__ bool lookahead_tail(int r0, int nr, int nNear, int fl
 // column) and of the tiles four and three back (their coupling entries
 // prefetched at the start), partials reduced across lanes once; the tail
 // combines the near tiles through the precomputed blocks.
+template <typename LT>
 __global__ __launch_bounds__(256, 1) void k_sweep_forward(const SweepTile* __restrict__ tiles, int numTiles,
-                                                          const double* __restrict__ lt,
+                                                          const LT* __restrict__ lt,
                                                           const int64_t* __restrict__ rowoff,
                                                           const int* __restrict__ sfirst,
                                                           const double* __restrict__ blocks,
@@ -1709,7 +1864,7 @@ __global__ __launch_bounds__(256, 1) void k_sweep_forward(const SweepTile* __res
     // the tile read that row's entries: column c < r0 is inside every row's
     // addressable range, so the unselected loads stay in the profile)
     const int rb = min(r0 + 16 * w, T.r1 - 1);
-    const double* ltw = lt + rowoff[rb];
+    const LT* ltw = lt + rowoff[rb];
     int rs[16], rd[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -1859,19 +2014,26 @@ __global__ __launch_bounds__(256, 1) void k_sweep_forward(const SweepTile* __res
     lookahead_tail(r0, nr, nNear, T.blk, h, dv, kv, B1s[w], B3s[w], n3.v, 16 * w, g, z, flags, epoch, spinLimit,
                    timeout, n1, n2, acc, acc2, vsh[w]);
 }
+template __global__ void k_sweep_forward<double>(const SweepTile*, int, const double*, const int64_t*, const int*,
+                                                 const double*, const double*, double*, double*, int*, int*, int,
+                                                 long, int*);
+template __global__ void k_sweep_forward<float>(const SweepTile*, int, const float*, const int64_t*, const int*,
+                                                const double*, const double*, double*, double*, int*, int*, int,
+                                                long, int*);
 
 // Backward sweep tile (columns [r0, r1)): phase 1 takes the rows of the far
 // tiles (five or more tiles on) and of the tiles three and four on (coupling
 // entries prefetched), the tail combines the near tiles.
+template <typename LT>
 __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restrict__ tiles, int numTiles,
-                                                        const double* __restrict__ lt,
+                                                        const LT* __restrict__ lt,
                                                         const int64_t* __restrict__ rowoff,
                                                         const int* __restrict__ sfirst,
                                                         const double* __restrict__ blocks,
                                                         const double* __restrict__ z, double* __restrict__ x,
                                                         double* __restrict__ g, int* __restrict__ flags,
                                                         int* __restrict__ ticket, int epoch, long spinLimit,
-                                                        int* __restrict__ timeout) {
+                                                        int* __restrict__ timeout, const double* __restrict__ diag) {
     __shared__ int sT;
     __shared__ double part[4][64];
     __shared__ double acc[4][64];
@@ -1905,7 +2067,13 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     double zd = 0.0;
     if (lane < nr) {
         const int gi = r0 + lane;
-        zd = z[gi] / lt[rowoff[gi] + gi];
+        // the pivot: from the profile's diagonal in FP64 storage (which may
+        // hold a factor handed in from the host); an FP32 profile keeps its
+        // pivots in FP64 in diag
+        if constexpr (sizeof(LT) == sizeof(double))
+            zd = z[gi] / lt[rowoff[gi] + gi];
+        else
+            zd = z[gi] / diag[gi];
     }
     const int ci = r0 + lane;
     // an in-range column for idle lanes' loads; their partial sums are unused
@@ -2053,6 +2221,12 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
     lookahead_tail(r0, nr, nNear, T.blk, h, dv, kv, B1s[w], B3s[w], n3.v, 0, g, x, flags, epoch, spinLimit, timeout,
                    n1, n2, acc, acc2, vsh[w]);
 }
+template __global__ void k_sweep_backward<double>(const SweepTile*, int, const double*, const int64_t*, const int*,
+                                                  const double*, const double*, double*, double*, int*, int*, int,
+                                                  long, int*, const double*);
+template __global__ void k_sweep_backward<float>(const SweepTile*, int, const float*, const int64_t*, const int*,
+                                                 const double*, const double*, double*, double*, int*, int*, int,
+                                                 long, int*, const double*);
 
 __global__ void k_reset_ticket(int* ticket) { *ticket = 0; }
 
