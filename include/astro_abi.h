@@ -268,6 +268,53 @@ double astro_ctx_kernel_stat(const astro_ctx* ctx, const char* name);
  * on the device (synchronous). */
 int astro_ctx_true_residual(astro_ctx* ctx, double* relRes);
 
+/* ---- Learned block preconditioner --------------------------------------
+ * No counterpart in the reference (its Newmark loop, Working/AStrO_commandFunctions.f90:
+ * 781-815, preconditions GMRES with the skyline LDL^T alone).  A graph network
+ * over the mesh's node graph (encoder, `layers` message-passing layers of
+ * `hidden` FP32 units with mean aggregation, linear head) maps per-node
+ * features of the assembled operator A (its normalised 3 x 3 diagonal block
+ * D_i, log stiffness, constraint flag, neighbour coupling) to symmetric
+ * blocks B_i = L_i^-T (I + Y_i) L_i^-1, D_i = L_i L_i^T.  With the blocks
+ * attached, GMRES uses the two-stage preconditioner z = P^-1 r, z += B (r - A z);
+ * the plain LDL^T then judges the result (a plain-preconditioned GMRES with
+ * absolute target rtol |P^-1 r0| returns at once when the target is met) and
+ * continues as fallback when it is not.  brick8 meshes.
+ *
+ * Training data are generated by the solver: astro_ctx_gnn_collect, on a
+ * context whose operator is factored (after astro_ctx_dynamic_init), solves
+ * A x = b for nRhs seeded random b by LDL^T-preconditioned GMRES to rtol and
+ * keeps r1 = b - A P^-1 b, the residual the LDL^T stage leaves, with
+ * sum e1^T r1, e1 = x - P^-1 b = A^-1 r1 its error, and a copy of A; the loss
+ * of a field is the energy norm of the error the block stage leaves,
+ * sum (e1 - B r1)^T A (e1 - B r1) / sum e1^T A e1 over its pairs.
+ * astro_gnn_train runs Adam (lr, epochs over the collected fields in seeded
+ * order) with the library's forward and backward kernels; lossPerEpoch
+ * (epochs, may be null) receives the mean field loss of each epoch.
+ * astro_gnn_loss_grad evaluates one field's loss and gradient (grad: the
+ * parameter count, may be null); astro_gnn_get_field / _field_size /
+ * _field_blocks expose a field's inputs, statistics and blocks.
+ * astro_ctx_gnn_get reads "features" (numNodes x 9), "G" and "blocks"
+ * (numNodes x 9) of a context. */
+typedef struct astro_gnn astro_gnn;
+int astro_gnn_create(int hidden, int layers, uint64_t seed, astro_gnn** out);
+int astro_gnn_destroy(astro_gnn* gnn);
+int64_t astro_gnn_num_params(const astro_gnn* gnn);
+int astro_gnn_num_fields(const astro_gnn* gnn);
+int astro_gnn_get_params(astro_gnn* gnn, float* host, int64_t count);
+int astro_gnn_set_params(astro_gnn* gnn, const float* host, int64_t count);
+int astro_ctx_gnn_collect(astro_ctx* ctx, astro_gnn* gnn, int nRhs, uint64_t seed, double rtol);
+int astro_gnn_train(astro_gnn* gnn, int epochs, double lr, uint64_t seed, double* lossPerEpoch);
+int astro_gnn_loss_grad(astro_gnn* gnn, int field, double* loss, float* grad);
+int astro_gnn_field_size(const astro_gnn* gnn, int field, int32_t* numNodes, int32_t* numAdj, int32_t* numDofs,
+                         int32_t* samples);
+int astro_gnn_get_field(astro_gnn* gnn, int field, int32_t* range, int32_t* adj, float* X, double* G, double* R1,
+                        double* eTr);
+int astro_gnn_field_blocks(astro_gnn* gnn, int field, double* B);
+int astro_ctx_gnn_attach(astro_ctx* ctx, astro_gnn* gnn);
+int astro_ctx_gnn_detach(astro_ctx* ctx);
+int astro_ctx_gnn_get(astro_ctx* ctx, const char* name, double* host, int64_t count);
+
 /* Allocation-free variant of astro_ctx_true_residual.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -36,7 +36,10 @@ EXPORTS = [
     "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual", "astro_ctx_dynamic_init",
     "astro_ctx_dynamic_step", "astro_ctx_dynamic_run", "astro_ctx_set_design", "astro_ctx_dRdD",
     "astro_ctx_compliance_gradient", "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load",
-    "astro_ctx_true_residual_noalloc",
+    "astro_ctx_true_residual_noalloc", "astro_gnn_create", "astro_gnn_destroy", "astro_gnn_num_params",
+    "astro_gnn_num_fields", "astro_gnn_get_params", "astro_gnn_set_params", "astro_ctx_gnn_collect",
+    "astro_gnn_train", "astro_gnn_loss_grad", "astro_gnn_field_size", "astro_gnn_get_field",
+    "astro_gnn_field_blocks", "astro_ctx_gnn_attach", "astro_ctx_gnn_detach", "astro_ctx_gnn_get",
 ]
 
 
@@ -120,6 +123,24 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_kernel_stat.argtypes = [C.c_void_p, C.c_char_p]
     L.astro_ctx_phase_ms.restype = C.c_double
     L.astro_ctx_phase_ms.argtypes = [C.c_void_p, C.c_char_p]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.astro_gnn_create.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]
+    L.astro_gnn_destroy.argtypes = [C.c_void_p]
+    L.astro_gnn_num_params.restype = C.c_int64
+    L.astro_gnn_num_params.argtypes = [C.c_void_p]
+    L.astro_gnn_num_fields.argtypes = [C.c_void_p]
+    L.astro_gnn_get_params.argtypes = [C.c_void_p, _fp, _i64]
+    L.astro_gnn_set_params.argtypes = [C.c_void_p, _fp, _i64]
+    L.astro_ctx_gnn_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_double]
+    L.astro_gnn_train.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint64, C.c_void_p]
+    L.astro_gnn_loss_grad.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_void_p]
+    L.astro_gnn_field_size.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.astro_gnn_get_field.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _fp, _dp, _dp, C.POINTER(C.c_double)]
+    L.astro_gnn_field_blocks.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.astro_ctx_gnn_attach.argtypes = [C.c_void_p, C.c_void_p]
+    L.astro_ctx_gnn_detach.argtypes = [C.c_void_p]
+    L.astro_ctx_gnn_get.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
     _lib = L
     return L
 
@@ -471,6 +492,26 @@ class Context:
         _check(load_library().astro_ctx_true_residual_noalloc(self.h, C.c_void_p(d_u), C.c_void_p(d_r), C.byref(r)))
         return r.value
 
+    # ---- learned block preconditioner
+    def gnn_collect(self, gnn: "Gnn", n_rhs: int, seed: int = 0, rtol: float = 1e-10):
+        """Add one training field: n_rhs solver-generated pairs on the current
+        (factored) operator."""
+        _check(load_library().astro_ctx_gnn_collect(self.h, gnn.h, int(n_rhs), int(seed), float(rtol)))
+
+    def gnn_attach(self, gnn: "Gnn"):
+        """Blocks of the network for this operator; GMRES then runs with the
+        two-stage preconditioner, the plain LDL^T as judge and fallback."""
+        _check(load_library().astro_ctx_gnn_attach(self.h, gnn.h))
+
+    def gnn_detach(self):
+        _check(load_library().astro_ctx_gnn_detach(self.h))
+
+    def gnn_get(self, name):
+        N = self.model.num_nodes
+        out = np.zeros(N * 9)
+        _check(load_library().astro_ctx_gnn_get(self.h, name.encode(), out, out.size))
+        return out.reshape(N, 9)
+
     def set_timing(self, on: bool):
         _check(load_library().astro_ctx_set_timing(self.h, 1 if on else 0))
 
@@ -479,3 +520,79 @@ class Context:
 
     def phase_ms(self, phase):
         return float(load_library().astro_ctx_phase_ms(self.h, phase.encode()))
+
+
+class Gnn:
+    """The learned block preconditioner's graph network (FP32 weights on the
+    device): solver-generated training fields, Adam training with the
+    library's HIP kernels, parameters as a flat float32 vector."""
+
+    def __init__(self, hidden: int = 64, layers: int = 3, seed: int = 0):
+        h = C.c_void_p()
+        _check(load_library().astro_gnn_create(int(hidden), int(layers), int(seed), C.byref(h)))
+        self.h = h
+        self.hidden, self.layers = hidden, layers
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().astro_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self):
+        return int(load_library().astro_gnn_num_params(self.h))
+
+    @property
+    def num_fields(self):
+        return int(load_library().astro_gnn_num_fields(self.h))
+
+    def get_params(self):
+        p = np.zeros(self.num_params, np.float32)
+        _check(load_library().astro_gnn_get_params(self.h, p, p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(load_library().astro_gnn_set_params(self.h, p, p.size))
+
+    def train(self, epochs: int, lr: float = 1e-3, seed: int = 0):
+        """Adam over the collected fields; returns the mean field loss per epoch."""
+        out = np.zeros(max(1, epochs))
+        _check(load_library().astro_gnn_train(self.h, int(epochs), float(lr), int(seed), out.ctypes.data))
+        return out[:epochs]
+
+    def loss_grad(self, field: int = 0):
+        loss = C.c_double(0)
+        g = np.zeros(self.num_params, np.float32)
+        _check(load_library().astro_gnn_loss_grad(self.h, int(field), C.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def _size(self, field):
+        N, na, nd, ns = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        _check(load_library().astro_gnn_field_size(self.h, int(field), C.byref(N), C.byref(na), C.byref(nd),
+                                                   C.byref(ns)))
+        return N.value, na.value, nd.value, ns.value
+
+    def field(self, field: int = 0):
+        """range, adj, X, G, R1 (samples x DOFs) and eTr of a collected field."""
+        N, na, nd, ns = self._size(field)
+        rng = np.zeros(N + 1, np.int32)
+        adj = np.zeros(max(1, na), np.int32)
+        X = np.zeros(N * 9, np.float32)
+        G, R1 = np.zeros(N * 9), np.zeros(max(1, nd * ns))
+        eTr = C.c_double(0)
+        _check(load_library().astro_gnn_get_field(self.h, int(field), rng, adj, X, G, R1, C.byref(eTr)))
+        return dict(range=rng, adj=adj[:na], X=X.reshape(N, 9), G=G.reshape(N, 3, 3),
+                    R1=R1[:nd * ns].reshape(ns, nd), eTr=eTr.value)
+
+    def field_blocks(self, field: int = 0):
+        N, _, _, _ = self._size(field)
+        B = np.zeros(N * 9)
+        _check(load_library().astro_gnn_field_blocks(self.h, int(field), B))
+        return B.reshape(N, 3, 3)

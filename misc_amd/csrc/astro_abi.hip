@@ -13,6 +13,8 @@
 #include <vector>
 
 #include "../../include/astro_abi.h"
+#include "astro_gnn.h"
+#include "astro_internal.h"
 #include "astro_kernels.h"
 #include "astro_prep.h"
 
@@ -20,63 +22,9 @@ namespace astro {
 
 static thread_local std::string g_lastError;
 
-static int fail(int code, const std::string& msg) {
+int fail(int code, const std::string& msg) {
     g_lastError = msg;
     return code;
-}
-
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return fail(ASTRO_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
-    } while (0)
-
-#define ASTRO_TRY(expr)            \
-    do {                           \
-        int s_ = (expr);           \
-        if (s_ != ASTRO_OK) return s_; \
-    } while (0)
-
-// Device buffer owned by RAII.
-template <class T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    int alloc(size_t count) {
-        release();
-        n = count;
-        if (count == 0) return ASTRO_OK;
-        hipError_t e = hipMalloc(&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(ASTRO_ERR_DEVICE, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes: " +
-                                              hipGetErrorString(e));
-        }
-        return ASTRO_OK;
-    }
-    int upload(const T* host, size_t count, hipStream_t s) {
-        ASTRO_TRY(alloc(count));
-        if (count) HIP_TRY(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
-        return ASTRO_OK;
-    }
-    int upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
-};
-
-static inline int grid_for(int64_t n, int block, int cap = 65536) {
-    int64_t g = (n + block - 1) / block;
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (int)g;
 }
 
 // ---------------------------------------------------------------------------
@@ -638,6 +586,20 @@ struct Gmres {
     }
     double* Vcol(int q) { return V.p + (size_t)q * n; }
 
+    // out = |P^-1 (b - A x)|, the residual formed in double-double.  Synchronous.
+    template <class Prof>
+    int residual_norm(const DeviceOperator& A, Prof& P, const double* b, const double* x, double& out,
+                      hipStream_t s) {
+        ASTRO_TRY(A.residual(b, x, t2.p, s));
+        ASTRO_TRY(P.apply(t2.p, r.p, s));
+        multidot(r.p, 1, r.p, hdev.p, s);
+        HIP_TRY(hipMemcpyAsync(hHost, hdev.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        ASTRO_TRY(P.check_timeout(s));
+        out = std::sqrt(hHost[0]);
+        return ASTRO_OK;
+    }
+
     template <class Prof>
     int solve(const DeviceOperator& A, Prof& P, const double* b, double* x, double rtol, double atol,
               int maxCycles, hipStream_t s) {
@@ -725,6 +687,31 @@ struct Gmres {
                                                                  std::to_string(cycles) + " cycles");
         return ASTRO_OK;
     }
+};
+
+// The learned two-stage preconditioner: x = P^-1 b, then x += B (b - A x)
+// with B the graph network's 3 x 3 node blocks.  A fixed linear operator, so
+// plain (left-preconditioned) GMRES applies.
+template <class Prof>
+struct TwoStagePrec {
+    Prof& P;
+    const DeviceOperator& A;
+    const double* B;
+    const int* nodeDofs;
+    int N, n;
+    double* t1;
+    double* t2;
+    int apply(const double* b, double* x, hipStream_t s) {
+        ASTRO_TRY(P.apply(b, x, s));
+        ASTRO_TRY(A.apply(x, t1, s));
+        hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b, (const double*)t1, t2, n, 1.0,
+                           -1.0);
+        hipLaunchKernelGGL(k_node_block_apply, dim3(grid_for(N, 256)), dim3(256), 0, s, N, nodeDofs, B,
+                           (const double*)t2, x);
+        HIP_TRY(hipGetLastError());
+        return ASTRO_OK;
+    }
+    int check_timeout(hipStream_t s) { return P.check_timeout(s); }
 };
 
 // Host LU inverse without pivoting of a 6x6 compliance matrix (column-major),
@@ -822,6 +809,15 @@ struct astro_ctx {
     DBuf<double> prevDisp, prevVel, prevAcc, nodeVel, nodeAcc;
     int64_t dynSteps = 0, dynGmresSteps = 0;
     double dynMs = 0.0;
+    // learned block preconditioner (graph network over the node graph)
+    bool gnnReady = false, gnnOn = false;
+    std::vector<int> gnnRange, gnnAdj;  // node graph, 0-based
+    GnnGraph gnnGraph;
+    DBuf<int> nodeDofs, dofNode;
+    DBuf<float> gnnX;
+    DBuf<double> gnnG, gnnB, gnnT1, gnnT2;
+    int64_t gnnFallbackSteps = 0;
+    int lastSolveSteps = 0;
     // timing
     hipEvent_t ev[8] = {};
     double phaseMs[6] = {0, 0, 0, 0, 0, 0};
@@ -841,6 +837,34 @@ static int with_prof(astro_ctx* c, F f) {
 template <class F>
 static int with_tprof(astro_ctx* c, F f) {
     return c->mixed ? f(c->tprof32) : f(c->tprof);
+}
+
+// GMRES on the elastic system for the step load b from x.  Plain: the
+// skyline LDL^T preconditioner.  With the learned blocks attached: GMRES with
+// the two-stage preconditioner, then the plain LDL^T as judge and fallback: a
+// plain-preconditioned GMRES whose absolute target is rtol |P^-1 r0| of the
+// step's starting residual returns at once when the learned solve meets it
+// and otherwise iterates on.  lastSolveSteps counts the Arnoldi steps of both.
+static int ctx_gmres(astro_ctx* c, const double* b, double* x) {
+    return with_prof(c, [&](auto& P) -> int {
+        const double rtol = c->opt.rtol;
+        if (!c->gnnOn) {
+            const int rc = c->gm.solve(c->op, P, b, x, rtol, 0.0, c->opt.maxCycles, c->s);
+            c->lastSolveSteps = c->gm.steps;
+            return rc;
+        }
+        double res0 = 0.0;
+        ASTRO_TRY(c->gm.residual_norm(c->op, P, b, x, res0, c->s));
+        TwoStagePrec<std::decay_t<decltype(P)>> T{P,        c->op, c->gnnB.p,  c->nodeDofs.p, c->prep.numNodes,
+                                                  c->prep.elMatDim, c->gnnT1.p, c->gnnT2.p};
+        const int rc1 = c->gm.solve(c->op, T, b, x, rtol, 0.0, c->opt.maxCycles, c->s);
+        if (rc1 != ASTRO_OK && rc1 != ASTRO_ERR_NO_CONVERGE) return rc1;
+        const int s1 = c->gm.steps;
+        const int rc2 = c->gm.solve(c->op, P, b, x, 0.0, rtol * res0, c->opt.maxCycles, c->s);
+        c->gnnFallbackSteps += c->gm.steps;
+        c->lastSolveSteps = s1 + c->gm.steps;
+        return rc2;
+    });
 }
 
 extern "C" {
@@ -1114,7 +1138,9 @@ int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
     if (s == "elMPCSize") return (int64_t)p.elMPCMat.size();
     if (s == "thermMatSize") return (int64_t)p.thermMatCols.size();
     if (s == "thermMatLTSize") return p.thermMatLTRange.empty() ? 0 : p.thermMatLTRange.back();
-    if (s == "gmresSteps") return c->gm.steps;
+    if (s == "gmresSteps") return c->lastSolveSteps;
+    if (s == "gnnFallbackSteps") return c->gnnFallbackSteps;
+    if (s == "gnnAttached") return c->gnnOn ? 1 : 0;
     if (s == "gmresCycles") return c->gm.cycles;
     if (s == "thermGmresSteps") return c->tgm.steps;
     if (s == "numColors") return (int64_t)c->colorStart.size() - 1;
@@ -1333,9 +1359,7 @@ int astro_ctx_solve(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     ASTRO_TRY(ctx_rhs(c));
     const int n = c->prep.elMatDim;
-    ASTRO_TRY(with_prof(c, [&](auto& P) {
-        return c->gm.solve(c->op, P, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s);
-    }));
+    ASTRO_TRY(ctx_gmres(c, c->load.p, c->delDisp.p));
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->nodeDisp.p, c->delDisp.p,
                        c->nodeDisp.p, n, 1.0, 1.0);
     HIP_TRY(hipGetLastError());
@@ -1439,16 +1463,14 @@ int astro_ctx_dynamic_step(astro_ctx* c, double loadScale) {
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
                            c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p);
     HIP_TRY(hipGetLastError());
-    ASTRO_TRY(with_prof(c, [&](auto& P) {
-        return c->gm.solve(c->op, P, c->load.p, c->delDisp.p, c->opt.rtol, 0.0, c->opt.maxCycles, c->s);
-    }));
+    ASTRO_TRY(ctx_gmres(c, c->load.p, c->delDisp.p));
     hipLaunchKernelGGL(k_newmark_update, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, n, c->nodeDisp.p,
                        c->delDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c->nodeVel.p, c->nodeAcc.p, c1, c2, dt,
                        beta, c->nmGamma);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->s));
     c->dynSteps += 1;
-    c->dynGmresSteps += c->gm.steps;
+    c->dynGmresSteps += c->lastSolveSteps;
     return ASTRO_OK;
 }
 
@@ -1635,6 +1657,171 @@ int astro_ctx_thermal_soln_load(astro_ctx* c) {
     if (!c->thermalReady) return fail(ASTRO_ERR_ARG, "the thermal system has not been assembled");
     ctx_minus_internal_force(c->s, c->tVal, c->tCols, c->tRange, c->prep.numNodes, c->nodeTemp.p, c->tLoad.p);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+// ---- learned block preconditioner --------------------------------------------
+// Node graph, DOF maps and buffers, once per context.
+static int ctx_gnn_setup(astro_ctx* c) {
+    if (c->gnnReady) return ASTRO_OK;
+    if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "the learned preconditioner covers brick8 meshes");
+    const Prep& p = c->prep;
+    const int N = p.numNodes, n = p.elMatDim;
+    std::vector<int> nd(3 * (size_t)N, -1), dn(n, -1);
+    for (int i = 0; i < N; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const int d = p.nDofIndex[6 * (size_t)i + a];
+            if (d > 0) {
+                nd[3 * (size_t)i + a] = d - 1;
+                dn[d - 1] = i;
+            }
+        }
+    c->gnnRange.assign(N + 1, 0);
+    c->gnnAdj.clear();
+    for (int i = 0; i < N; ++i) {
+        for (int q = p.nCRange[i]; q < p.nCRange[i + 1]; ++q) {
+            const int nb = p.nodalConn[q];
+            if (nb > 0 && nb - 1 != i) c->gnnAdj.push_back(nb - 1);
+        }
+        c->gnnRange[i + 1] = (int)c->gnnAdj.size();
+    }
+    ASTRO_TRY(c->nodeDofs.upload(nd, c->s));
+    ASTRO_TRY(c->dofNode.upload(dn, c->s));
+    ASTRO_TRY(c->gnnGraph.build(c->gnnRange, c->gnnAdj, c->s));
+    ASTRO_TRY(c->gnnX.alloc((size_t)N * GNN_F));
+    ASTRO_TRY(c->gnnG.alloc((size_t)N * 9));
+    ASTRO_TRY(c->gnnB.alloc((size_t)N * 9));
+    ASTRO_TRY(c->gnnT1.alloc(n));
+    ASTRO_TRY(c->gnnT2.alloc(n));
+    c->gnnReady = true;
+    return ASTRO_OK;
+}
+
+// Network inputs of the current operator (the assembled CSR and the scaled
+// penalty rows): features, centred log stiffness, and G = L^-T per node.
+static int ctx_gnn_features(astro_ctx* c) {
+    const int N = c->prep.numNodes, cd = c->prep.elMPCDim;
+    hipLaunchKernelGGL(k_gnn_features, dim3(grid_for(N, 256)), dim3(256), 0, c->s, N, (const int*)c->nodeDofs.p,
+                       (const int*)c->dofNode.p, (const double*)c->csrVal.p, (const int*)c->csrCols.p,
+                       (const int*)c->csrRange.p, cd > 0 ? (const int*)c->mpcT.range.p : (const int*)nullptr,
+                       (const int*)c->mpcT.pos.p, (const int*)c->mpcT.row.p, (const double*)c->mpcMat.p, c->gnnX.p,
+                       c->gnnG.p);
+    HIP_TRY(hipGetLastError());
+    std::vector<float> X((size_t)N * GNN_F);
+    HIP_TRY(hipMemcpyAsync(X.data(), c->gnnX.p, sizeof(float) * X.size(), hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    double sum = 0.0;
+    int cnt = 0;
+    for (int i = 0; i < N; ++i)
+        if (X[(size_t)i * GNN_F + 7] < 0.5f) {
+            sum += X[(size_t)i * GNN_F + 6];
+            ++cnt;
+        }
+    const float mu = cnt ? (float)(sum / cnt) : 0.0f;
+    hipLaunchKernelGGL(k_gnn_center_log, dim3(grid_for(N, 256)), dim3(256), 0, c->s, N, c->gnnX.p, mu);
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+int astro_ctx_gnn_collect(astro_ctx* c, astro_gnn* g, int nRhs, uint64_t seed, double rtol) {
+    if (!c || !g) return fail(ASTRO_ERR_ARG, "null argument");
+    if (nRhs < 1 || !(rtol > 0.0)) return fail(ASTRO_ERR_ARG, "need nRhs >= 1 and rtol > 0");
+    if (!with_prof(c, [&](auto& P) { return P.factored ? 1 : 0; }))
+        return fail(ASTRO_ERR_ARG, "collect needs a factored operator (astro_ctx_dynamic_init or a static factor)");
+    ASTRO_TRY(ctx_gnn_setup(c));
+    ASTRO_TRY(ctx_gnn_features(c));
+    hipStream_t s = c->s;
+    const int N = c->prep.numNodes, n = c->prep.elMatDim;
+    std::unique_ptr<GnnField> f(new GnnField);
+    ASTRO_TRY(f->graph.build(c->gnnRange, c->gnnAdj, s));
+    ASTRO_TRY(f->X.alloc((size_t)N * GNN_F));
+    ASTRO_TRY(f->G.alloc((size_t)N * 9));
+    HIP_TRY(hipMemcpyAsync(f->X.p, c->gnnX.p, sizeof(float) * N * GNN_F, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(f->G.p, c->gnnG.p, sizeof(double) * N * 9, hipMemcpyDeviceToDevice, s));
+    // the field's operator A = K + C^T C and DOF map, for the energy-norm loss
+    const Prep& p = c->prep;
+    const int cd = p.elMPCDim;
+    f->n = n;
+    f->cdim = cd;
+    ASTRO_TRY(f->nodeDofs.alloc((size_t)N * 3));
+    ASTRO_TRY(f->val.alloc(p.elMatCols.size()));
+    HIP_TRY(hipMemcpyAsync(f->nodeDofs.p, c->nodeDofs.p, sizeof(int) * N * 3, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(f->val.p, c->csrVal.p, sizeof(double) * p.elMatCols.size(), hipMemcpyDeviceToDevice, s));
+    ASTRO_TRY(f->cols.upload(p.elMatCols, s));
+    ASTRO_TRY(f->range.upload(p.elMatRange, s));
+    ASTRO_TRY(f->cmat.alloc(p.elMPCMat.size()));
+    if (!p.elMPCMat.empty())
+        HIP_TRY(hipMemcpyAsync(f->cmat.p, c->mpcMat.p, sizeof(double) * p.elMPCMat.size(), hipMemcpyDeviceToDevice, s));
+    ASTRO_TRY(f->ccols.upload(p.elMPCMatCols, s));
+    ASTRO_TRY(f->crange.upload(p.elMPCMatRange, s));
+    ASTRO_TRY(f->R1.alloc((size_t)n * nRhs));
+    DBuf<double> b, x, z0, t, e1;
+    for (DBuf<double>* v : {&b, &x, &z0, &t, &e1}) ASTRO_TRY(v->alloc(n));
+    f->eTr = 0.0;
+    for (int k = 0; k < nRhs; ++k) {
+        // a solver-generated pair: b random, x = A^-1 b by LDL^T-preconditioned GMRES;
+        // after the LDL^T stage z0 = P^-1 b the residual is r1 = b - A z0 and the error e1 = x - z0
+        hipLaunchKernelGGL(k_normal_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b.p, (int64_t)n,
+                           seed * 1000003ull + (uint64_t)k);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x.p, (int64_t)n, 0.0);
+        ASTRO_TRY(with_prof(c, [&](auto& P) -> int {
+            ASTRO_TRY(c->gm.solve(c->op, P, b.p, x.p, rtol, 0.0, c->opt.maxCycles, s));
+            ASTRO_TRY(P.apply(b.p, z0.p, s));
+            return P.check_timeout(s);
+        }));
+        double* r1 = f->R1.p + (size_t)k * n;
+        ASTRO_TRY(c->op.apply(z0.p, t.p, s));
+        hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)b.p,
+                           (const double*)t.p, r1, n, 1.0, -1.0);
+        hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)x.p,
+                           (const double*)z0.p, e1.p, n, 1.0, -1.0);
+        c->gm.multidot(e1.p, 1, r1, c->gm.hdev.p, s);
+        HIP_TRY(hipMemcpyAsync(c->gm.hHost, c->gm.hdev.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        f->eTr += c->gm.hHost[0];
+    }
+    if (!(f->eTr > 0.0)) return fail(ASTRO_ERR_ARG, "collected pairs carry no error after the LDL^T stage");
+    f->samples = nRhs;
+    g->fields.push_back(std::move(f));
+    return ASTRO_OK;
+}
+
+int astro_ctx_gnn_attach(astro_ctx* c, astro_gnn* g) {
+    if (!c || !g) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(ctx_gnn_setup(c));
+    ASTRO_TRY(ctx_gnn_features(c));
+    HIP_TRY(hipStreamSynchronize(g->s));
+    ASTRO_TRY(g->net.forward(c->gnnGraph, c->gnnX.p, c->gnnG.p, c->gnnB.p, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    c->gnnOn = true;
+    return ASTRO_OK;
+}
+
+int astro_ctx_gnn_detach(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    c->gnnOn = false;
+    return ASTRO_OK;
+}
+
+int astro_ctx_gnn_get(astro_ctx* c, const char* name, double* host, int64_t count) {
+    if (!c || !name || !host) return fail(ASTRO_ERR_ARG, "null argument");
+    ASTRO_TRY(ctx_gnn_setup(c));
+    const std::string nm(name);
+    const int64_t N = c->prep.numNodes;
+    if (nm == "features") {
+        if (count != N * GNN_F) return fail(ASTRO_ERR_ARG, "size mismatch for features");
+        ASTRO_TRY(ctx_gnn_features(c));
+        std::vector<float> X(count);
+        HIP_TRY(hipMemcpyAsync(X.data(), c->gnnX.p, sizeof(float) * count, hipMemcpyDeviceToHost, c->s));
+        HIP_TRY(hipStreamSynchronize(c->s));
+        for (int64_t i = 0; i < count; ++i) host[i] = X[i];
+        return ASTRO_OK;
+    }
+    DBuf<double>* src = nm == "G" ? &c->gnnG : (nm == "blocks" ? &c->gnnB : nullptr);
+    if (!src) return fail(ASTRO_ERR_ARG, "unknown learned-preconditioner array " + nm);
+    if (count != N * 9) return fail(ASTRO_ERR_ARG, "size mismatch for " + nm);
+    HIP_TRY(hipMemcpyAsync(host, src->p, sizeof(double) * count, hipMemcpyDeviceToHost, c->s));
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
 }
