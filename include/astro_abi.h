@@ -139,6 +139,9 @@ typedef struct {
                                  blocks' factor, the sweeps' arithmetic, the operator and the GMRES
                                  vectors stay FP64 and every restart forms the true residual in
                                  double-double (the restarts are iterative refinement) */
+    int32_t stepGraphs;       /* 1: dynamic_run records the Newmark step (residual, loads, one GMRES
+                                 cycle with its Givens work on the device, judge, update) into HIP
+                                 graphs and replays one per step; 0: eager launches per step */
 } astro_options;
 
 /* Host-only analysisPrep (Working/AStrO_bookKeeping.f90:794-1039): level

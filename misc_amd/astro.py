@@ -70,7 +70,8 @@ class _Model(C.Structure):
 class _Options(C.Structure):
     _fields_ = [("maxBW", C.c_int32), ("literalMaxBW", C.c_int32), ("thermLiteralD8", C.c_int32),
                 ("numVecs", C.c_int32), ("rtol", C.c_double), ("maxCycles", C.c_int32),
-                ("coupledProfile", C.c_int32), ("precision", C.c_int32)]
+                ("coupledProfile", C.c_int32), ("precision", C.c_int32),
+                ("stepGraphs", C.c_int32)]
 
 
 def load_library(path: str = LIB_PATH):
@@ -227,6 +228,7 @@ class Options:
     maxCycles: int = 1000
     coupledProfile: bool = False
     precision: int = 0  # 0: FP64 factor; 1: FP32-stored factor, FP32 matrix-core updates, FP64 refinement
+    stepGraphs: bool = False  # dynamic_run replays one HIP graph per Newmark step
 
 
 def _marshal(model, o: "Options"):
@@ -268,7 +270,8 @@ def _marshal(model, o: "Options"):
         M.zOffset = ptr(model.z_offset, np.float64)
         M.orient = ptr(model.orient.reshape(-1), np.float64)
     O = _Options(int(o.maxBW), 1 if o.literalMaxBW else 0, 1 if o.thermLiteralD8 else 0, int(o.numVecs),
-                 float(o.rtol), int(o.maxCycles), 1 if o.coupledProfile else 0, int(o.precision))
+                 float(o.rtol), int(o.maxCycles), 1 if o.coupledProfile else 0, int(o.precision),
+                 1 if o.stepGraphs else 0)
     return M, O, keep
 
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <climits>
 #include <cstdlib>
 #include <cmath>
@@ -415,32 +416,34 @@ struct DeviceProfileT {
     }
 
     // x = P^-1 b (forward then backward sweep); z keeps L^-1 b.  Async.
-    int apply(const double* b, double* x, hipStream_t s) {
+    // gate: as DeviceOperator::apply.  Every call takes fresh epochs, so the
+    // calls recorded into a graph each poll their own flag value.
+    int apply(const double* b, double* x, hipStream_t s, const int* gate = nullptr) {
         if (!blocksReady) ASTRO_TRY(prepare_blocks(s));
         ++fwdEpoch;
         // the sweeps poll the neighbouring tile's values: mark them unproduced
-        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, z.p, (int64_t)n);
-        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, x, (int64_t)n);
-        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, gnear.p, (int64_t)n);
-        hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, z.p, (int64_t)n, gate);
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, x, (int64_t)n, gate);
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, gnear.p, (int64_t)n, gate);
+        hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p, gate);
         if (timing) {
             evKind.push_back(0);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
         hipLaunchKernelGGL(k_sweep_forward<LT>, dim3(numTiles), dim3(256), sweepLdsFwd, s, fwdTiles.p, numTiles,
                            (const LT*)lt.p, rowoff.p, sfirst.p, (const double*)blocks.p, b, z.p, gnear.p, fwdFlags.p,
-                           ticket.p, fwdEpoch, spinLimit, timeout.p);
+                           ticket.p, fwdEpoch, spinLimit, timeout.p, gate);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         ++bwdEpoch;
-        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, gnear.p, (int64_t)n);
-        hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p);
+        hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, gnear.p, (int64_t)n, gate);
+        hipLaunchKernelGGL(k_reset_ticket, dim3(1), dim3(1), 0, s, ticket.p, gate);
         if (timing) {
             evKind.push_back(1);
             HIP_TRY(hipEventRecord(ev_next(), s));
         }
         hipLaunchKernelGGL(k_sweep_backward<LT>, dim3(numTiles), dim3(256), sweepLdsBwd, s, bwdTiles.p, numTiles,
                            (const LT*)lt.p, rowoff.p, sfirst.p, (const double*)blocks.p, (const double*)z.p, x,
-                           gnear.p, bwdFlags.p, ticket.p, bwdEpoch, spinLimit, timeout.p, (const double*)diag.p);
+                           gnear.p, bwdFlags.p, ticket.p, bwdEpoch, spinLimit, timeout.p, (const double*)diag.p, gate);
         if (timing) HIP_TRY(hipEventRecord(ev_next(), s));
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
@@ -524,24 +527,25 @@ struct DeviceOperator {
         ctRow = T.row.p;
         ctT = T.t.p;
     }
-    int apply(const double* x, double* y, hipStream_t s) const {
+    // gate (device int, may be null): every kernel returns at once while *gate == 0
+    int apply(const double* x, double* y, hipStream_t s, const int* gate = nullptr) const {
         hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val, cols,
-                           range, n, x, y);
+                           range, n, x, y, gate);
         if (cdim > 0)
             hipLaunchKernelGGL(k_mpc_apply, dim3(grid_for(cdim, 256)), dim3(256), 0, s, cmat, ccols, crange, cdim, x,
-                               y);
+                               y, gate);
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
     }
     // r = b - (A x + C^T C x), carried in double-double and rounded once.
-    int residual(const double* b, const double* x, double* r, hipStream_t s) const {
+    int residual(const double* b, const double* x, double* r, hipStream_t s, const int* gate = nullptr) const {
         const bool pen = cdim > 0;
         if (pen && !ctRange) return fail(ASTRO_ERR_ARG, "operator has constraints but no transpose");
         if (pen)
             hipLaunchKernelGGL(k_mpc_dd, dim3(grid_for(cdim, 256)), dim3(256), 0, s, cmat, ccols, crange, cdim, x,
-                               ctT);
+                               ctT, gate);
         hipLaunchKernelGGL(k_residual_dd, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val, cols,
-                           range, n, pen ? ctRange : nullptr, ctPos, ctRow, cmat, ctT, b, x, r);
+                           range, n, pen ? ctRange : nullptr, ctPos, ctRow, cmat, ctT, b, x, r, gate);
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
     }
@@ -579,10 +583,10 @@ struct Gmres {
         return ASTRO_OK;
     }
     // out[0..nq) = V[0..nq)^T w  (device)
-    void multidot(const double* Vb, int nq, const double* w, double* out, hipStream_t s) {
+    void multidot(const double* Vb, int nq, const double* w, double* out, hipStream_t s, const int* gate = nullptr) {
         hipLaunchKernelGGL(k_multidot, dim3(nblk, (nq + 7) / 8), dim3(256), 0, s, Vb, (int64_t)n, nq, w, n,
-                           partial.p, m + 8);
-        hipLaunchKernelGGL(k_sum_partials, dim3(nq), dim3(64), 0, s, partial.p, nblk, m + 8, nq, out);
+                           partial.p, m + 8, gate);
+        hipLaunchKernelGGL(k_sum_partials, dim3(nq), dim3(64), 0, s, partial.p, nblk, m + 8, nq, out, gate);
     }
     double* Vcol(int q) { return V.p + (size_t)q * n; }
 
@@ -625,7 +629,7 @@ struct Gmres {
         auto h = [&](int i, int j) -> double& { return H[(size_t)i + (size_t)(m + 1) * j]; };
         while (res > target && cycles < maxCycles) {
             hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, r.p, r.p, Vcol(0), n,
-                               1.0 / res, 0.0);
+                               1.0 / res, 0.0, (const int*)nullptr);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = res;
             int k = 0;
@@ -639,10 +643,10 @@ struct Gmres {
                 double* hn = hdev.p + 2 * (m + 8);
                 multidot(V.p, j + 1, w, h1, s);
                 hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, V.p, (int64_t)n,
-                                   j + 1, h1, w, n, -1.0);
+                                   j + 1, h1, w, n, -1.0, (const int*)nullptr);
                 multidot(V.p, j + 1, w, h2, s);
                 hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, V.p, (int64_t)n,
-                                   j + 1, h2, w, n, -1.0);
+                                   j + 1, h2, w, n, -1.0, (const int*)nullptr);
                 multidot(w, 1, w, hn, s);
                 HIP_TRY(hipMemcpyAsync(hHost, hdev.p, sizeof(double) * 3 * (m + 8), hipMemcpyDeviceToHost, s));
                 HIP_TRY(hipStreamSynchronize(s));
@@ -668,7 +672,7 @@ struct Gmres {
                 g[j] = cs[j] * g[j];
                 if (hnorm == 0.0 || std::fabs(g[j + 1]) <= target) break;
                 hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, w, w, w, n, 1.0 / hnorm,
-                                   0.0);
+                                   0.0, (const int*)nullptr);
             }
             for (int i = k - 1; i >= 0; --i) {
                 double v = g[i];
@@ -677,7 +681,7 @@ struct Gmres {
             }
             HIP_TRY(hipMemcpyAsync(hdev.p, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, V.p, (int64_t)n, k,
-                               hdev.p, x, n, 1.0);
+                               hdev.p, x, n, 1.0, (const int*)nullptr);
             ++cycles;
             ASTRO_TRY(residual(res));
             if (verbose)
@@ -701,13 +705,13 @@ struct TwoStagePrec {
     int N, n;
     double* t1;
     double* t2;
-    int apply(const double* b, double* x, hipStream_t s) {
-        ASTRO_TRY(P.apply(b, x, s));
-        ASTRO_TRY(A.apply(x, t1, s));
+    int apply(const double* b, double* x, hipStream_t s, const int* gate = nullptr) {
+        ASTRO_TRY(P.apply(b, x, s, gate));
+        ASTRO_TRY(A.apply(x, t1, s, gate));
         hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b, (const double*)t1, t2, n, 1.0,
-                           -1.0);
+                           -1.0, gate);
         hipLaunchKernelGGL(k_node_block_apply, dim3(grid_for(N, 256)), dim3(256), 0, s, N, nodeDofs, B,
-                           (const double*)t2, x);
+                           (const double*)t2, x, gate);
         HIP_TRY(hipGetLastError());
         return ASTRO_OK;
     }
@@ -809,6 +813,15 @@ struct astro_ctx {
     DBuf<double> prevDisp, prevVel, prevAcc, nodeVel, nodeAcc;
     int64_t dynSteps = 0, dynGmresSteps = 0;
     double dynMs = 0.0;
+    // Newmark steps replayed from HIP graphs (stepGraphs = 1): the device
+    // GMRES scalars (gmd, gmi), a step's three residual norms, the load
+    // factors, and pinned snapshots of gmi (two, for one step of lookahead)
+    DBuf<double> gmd, gmNrm, dynScales;
+    DBuf<int> gmi;
+    int* gmiHost = nullptr;
+    hipEvent_t gmEv[2] = {};
+    int64_t dynGraphSteps = 0, dynEagerSteps = 0, dynGraphs = 0;
+    int dynGraphCap = 0;
     // learned block preconditioner (graph network over the node graph)
     bool gnnReady = false, gnnOn = false;
     std::vector<int> gnnRange, gnnAdj;  // node graph, 0-based
@@ -824,6 +837,9 @@ struct astro_ctx {
     ~astro_ctx() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : gmEv)
+            if (e) (void)hipEventDestroy(e);
+        if (gmiHost) (void)hipHostFree(gmiHost);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -1149,6 +1165,10 @@ int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
     if (s == "numVecs") return c->numVecs;
     if (s == "dynSteps") return c->dynSteps;
     if (s == "dynGmresSteps") return c->dynGmresSteps;
+    if (s == "dynGraphSteps") return c->dynGraphSteps;
+    if (s == "dynEagerSteps") return c->dynEagerSteps;
+    if (s == "dynGraphs") return c->dynGraphs;
+    if (s == "dynGraphCap") return c->dynGraphCap;
     return -1;
 }
 
@@ -1333,8 +1353,8 @@ int astro_ctx_factor(astro_ctx* c) {
 static void ctx_minus_internal_force(hipStream_t s, const DBuf<double>& val, const DBuf<int>& cols,
                                      const DBuf<int>& range, int n, const double* u, double* load) {
     hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val.p, cols.p,
-                       range.p, n, u, load);
-    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, load, load, load, n, -1.0, 0.0);
+                       range.p, n, u, load, (const int*)nullptr);
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, load, load, load, n, -1.0, 0.0, (const int*)nullptr);
 }
 
 // Right-hand side of a static step (stepSolve, Working/AStrO_commandFunctions.f90:
@@ -1346,11 +1366,11 @@ static int ctx_rhs(astro_ctx* c) {
     const int nl = (int)c->loadDofs.n;
     if (nl > 0)
         hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
-                           nl, 1.0, c->load.p);
+                           nl, 1.0, c->load.p, (const int*)nullptr);
     const int cd = c->prep.elMPCDim;
     if (cd > 0)
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
-                           c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p);
+                           c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p, (const int*)nullptr);
     HIP_TRY(hipGetLastError());
     return ASTRO_OK;
 }
@@ -1361,7 +1381,7 @@ int astro_ctx_solve(astro_ctx* c) {
     const int n = c->prep.elMatDim;
     ASTRO_TRY(ctx_gmres(c, c->load.p, c->delDisp.p));
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, c->nodeDisp.p, c->delDisp.p,
-                       c->nodeDisp.p, n, 1.0, 1.0);
+                       c->nodeDisp.p, n, 1.0, 1.0, (const int*)nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
@@ -1433,6 +1453,7 @@ int astro_ctx_dynamic_init(astro_ctx* c, double beta, double gamma, double dt) {
     ASTRO_TRY(astro_ctx_build_profile(c));
     ASTRO_TRY(astro_ctx_factor(c));
     c->dynSteps = c->dynGmresSteps = 0;
+    c->dynGraphSteps = c->dynEagerSteps = c->dynGraphs = 0;
     c->dynMs = 0.0;
     c->dynReady = true;
     return ASTRO_OK;
@@ -1452,21 +1473,21 @@ int astro_ctx_dynamic_step(astro_ctx* c, double loadScale) {
         if (cnt <= 0) continue;
         hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
                            c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p, nS,
-                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, c->load.p);
+                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, c->load.p, (const int*)nullptr);
     }
     const int nl = (int)c->loadDofs.n;
     if (nl > 0)
         hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
-                           nl, loadScale, c->load.p);
+                           nl, loadScale, c->load.p, (const int*)nullptr);
     const int cd = c->prep.elMPCDim;
     if (cd > 0)
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
-                           c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p);
+                           c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p, (const int*)nullptr);
     HIP_TRY(hipGetLastError());
     ASTRO_TRY(ctx_gmres(c, c->load.p, c->delDisp.p));
     hipLaunchKernelGGL(k_newmark_update, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, n, c->nodeDisp.p,
                        c->delDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c->nodeVel.p, c->nodeAcc.p, c1, c2, dt,
-                       beta, c->nmGamma);
+                       beta, c->nmGamma, (const int*)nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->s));
     c->dynSteps += 1;
@@ -1474,13 +1495,262 @@ int astro_ctx_dynamic_step(astro_ctx* c, double loadScale) {
     return ASTRO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Newmark steps as HIP graphs.  One graph holds a whole step: the load
+// (the eager step's kernels, the load factor read from a device table at the
+// device step index), the starting residual, one GMRES cycle of at most K
+// Arnoldi steps whose Givens rotations and least squares run on the device
+// (k_gm_*), the judge (the true residual after the cycle against the
+// cycle's target; with learned blocks against rtol |P^-1 r0| of the plain
+// LDL^T, the eager path's judge), the Newmark update and the step index.
+// Arnoldi steps past convergence are skipped through the device flag
+// active; a step that misses its target clears run, so that the steps
+// queued behind it do nothing, and the host finishes that step eagerly (a
+// plain LDL^T GMRES to the same absolute target, from the cycle's iterate)
+// before it replays again.  The arithmetic of a replayed step is the eager
+// step's, kernel for kernel; only the rotations' rounding (device fma and
+// hypot) differs.
+
+// Records one step into the stream s, which is capturing.  P: the LDL^T
+// profile; Q: the cycle's preconditioner (P itself, or the two-stage form
+// with judge = true).
+extern "C++" {
+template <class Prof, class Pc>
+static int record_newmark_step(astro_ctx* c, Prof& P, Pc& Q, bool judge, int K, hipStream_t s) {
+    Gmres& g = c->gm;
+    const int n = c->prep.elMatDim, nS = c->prep.numSolidNds, m = g.m;
+    const double beta = c->nmBeta, dt = c->delT, c1 = 0.5 / beta, c2 = 2.0 / (dt * dt), rtol = c->opt.rtol;
+    int* gi = c->gmi.p;
+    double* gd = c->gmd.p;
+    double* nrm = c->gmNrm.p;  // starting (cycle), starting (judge), after the cycle
+    const int* run = gi + GMI_RUN;
+    const int* act = gi + GMI_ACTIVE;
+    hipLaunchKernelGGL(k_fill_gated, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, c->load.p, (int64_t)n, 0.0, run);
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[k], cnt,
+                           c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p, nS,
+                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, c->load.p, run);
+    }
+    const int nl = (int)c->loadDofs.n;
+    if (nl > 0)
+        hipLaunchKernelGGL(k_nodal_loads_dev, dim3(grid_for(nl, 256)), dim3(256), 0, s, c->loadDofs.p,
+                           c->loadVals.p, nl, (const double*)c->dynScales.p, (const int*)(gi + GMI_STEP), c->load.p,
+                           run);
+    const int cd = c->prep.elMPCDim;
+    if (cd > 0)
+        hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->mpcMat.p, c->mpcCols.p,
+                           c->mpcRange.p, c->mpcRhs.p, cd, c->nodeDisp.p, c->load.p, run);
+    // starting residual (double-double), preconditioned
+    ASTRO_TRY(c->op.residual(c->load.p, c->delDisp.p, g.t2.p, s, run));
+    if (judge) {
+        ASTRO_TRY(P.apply(g.t2.p, g.r.p, s, run));
+        g.multidot(g.r.p, 1, g.r.p, nrm + 1, s, run);
+        hipLaunchKernelGGL(k_gm_judge_target, dim3(1), dim3(64), 0, s, (const double*)(nrm + 1), rtol, gd,
+                           (const int*)gi);
+    }
+    ASTRO_TRY(Q.apply(g.t2.p, g.r.p, s, run));
+    g.multidot(g.r.p, 1, g.r.p, nrm, s, run);
+    hipLaunchKernelGGL(k_gm_begin, dim3(1), dim3(64), 0, s, (const double*)nrm, rtol, 0.0, gd, gi, m);
+    hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)g.r.p, g.Vcol(0), n,
+                       (const double*)(gd + GMD_INVRES), act);
+    double* h1 = g.hdev.p;
+    double* h2 = g.hdev.p + (m + 8);
+    double* hn = g.hdev.p + 2 * (m + 8);
+    for (int j = 0; j < K; ++j) {
+        double* w = g.Vcol(j + 1);
+        ASTRO_TRY(c->op.apply(g.Vcol(j), g.t.p, s, act));
+        ASTRO_TRY(Q.apply(g.t.p, w, s, act));
+        g.multidot(g.V.p, j + 1, w, h1, s, act);
+        hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, (const double*)g.V.p,
+                           (int64_t)n, j + 1, (const double*)h1, w, n, -1.0, act);
+        g.multidot(g.V.p, j + 1, w, h2, s, act);
+        hipLaunchKernelGGL(k_sub_combination, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, (const double*)g.V.p,
+                           (int64_t)n, j + 1, (const double*)h2, w, n, -1.0, act);
+        g.multidot(w, 1, w, hn, s, act);
+        hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(64), 0, s, j, (const double*)h1, (const double*)h2,
+                           (const double*)hn, gd, gi, m);
+        hipLaunchKernelGGL(k_scale_by, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)w, w, n,
+                           (const double*)(gd + GMD_INVH), act);
+    }
+    hipLaunchKernelGGL(k_gm_lsq, dim3(1), dim3(64), 0, s, gd, (const int*)gi, m);
+    hipLaunchKernelGGL(k_sub_combination_dn, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, (const double*)g.V.p,
+                       (int64_t)n, (const int*)(gi + GMI_K), (const double*)(gd + gmd_y(m)), c->delDisp.p, n, 1.0,
+                       run);
+    // the judge
+    ASTRO_TRY(c->op.residual(c->load.p, c->delDisp.p, g.t2.p, s, run));
+    if (judge)
+        ASTRO_TRY(P.apply(g.t2.p, g.r.p, s, run));
+    else
+        ASTRO_TRY(Q.apply(g.t2.p, g.r.p, s, run));
+    g.multidot(g.r.p, 1, g.r.p, nrm + 2, s, run);
+    hipLaunchKernelGGL(k_gm_check, dim3(1), dim3(64), 0, s, (const double*)(nrm + 2), gd, gi,
+                       judge ? (int)GMD_JUDGE : (int)GMD_TARGET);
+    hipLaunchKernelGGL(k_newmark_update, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, c->nodeDisp.p,
+                       c->delDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c->nodeVel.p, c->nodeAcc.p, c1, c2, dt,
+                       beta, c->nmGamma, run);
+    hipLaunchKernelGGL(k_gm_advance, dim3(1), dim3(64), 0, s, gi);
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+}  // extern "C++"
+
+// The Arnoldi cap of a step graph for a recent need of s steps.
+static int graph_cap(int s, int m) { return std::min(m, std::max(4, s + std::max(3, s / 4))); }
+
+// Finishes step f eagerly after its replay missed the target: GMRES with the
+// plain LDL^T from the cycle's iterate to the replay's absolute target, then
+// the Newmark update.  cycleSteps: the replay's Arnoldi steps.
+static int finish_step_eagerly(astro_ctx* c, int cycleSteps) {
+    double gdh[8];
+    HIP_TRY(hipMemcpyAsync(gdh, c->gmd.p, sizeof(gdh), hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    const double target = c->gnnOn ? gdh[GMD_JUDGE] : gdh[GMD_TARGET];
+    ASTRO_TRY(with_prof(c, [&](auto& P) -> int {
+        return c->gm.solve(c->op, P, c->load.p, c->delDisp.p, 0.0, target, c->opt.maxCycles, c->s);
+    }));
+    if (c->gnnOn) c->gnnFallbackSteps += c->gm.steps;
+    const int n = c->prep.elMatDim;
+    const double beta = c->nmBeta, dt = c->delT, c1 = 0.5 / beta, c2 = 2.0 / (dt * dt);
+    hipLaunchKernelGGL(k_newmark_update, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, n, c->nodeDisp.p,
+                       c->delDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c->nodeVel.p, c->nodeAcc.p, c1, c2, dt,
+                       beta, c->nmGamma, (const int*)nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->s));
+    c->dynSteps += 1;
+    c->dynEagerSteps += 1;
+    c->lastSolveSteps = cycleSteps + c->gm.steps;
+    c->dynGmresSteps += c->lastSolveSteps;
+    return ASTRO_OK;
+}
+
+// Steps 1..nSteps-1 of a run from graphs (step 0 runs eagerly: it readies
+// the sweeps and gives the first Arnoldi count).  The host keeps one step
+// of lookahead: it launches step k + 1 before it reads step k's snapshot of
+// gmi.  Graphs live for one run (they hold the context's buffers and the
+// preconditioner's state as recorded) and are cached by their cap K.
+static int dynamic_run_graphs(astro_ctx* c, int nSteps, const double* loadScale) {
+    ASTRO_TRY(astro_ctx_dynamic_step(c, loadScale[0]));
+    const int m = c->gm.m;
+    if (c->gmd.n < gmd_size(m)) ASTRO_TRY(c->gmd.alloc(gmd_size(m)));
+    if (c->gmi.n < (size_t)GMI_COUNT) ASTRO_TRY(c->gmi.alloc(GMI_COUNT));
+    if (c->gmNrm.n < 3) ASTRO_TRY(c->gmNrm.alloc(3));
+    ASTRO_TRY(c->dynScales.upload(loadScale, (size_t)nSteps, c->s));
+    if (!c->gmiHost) HIP_TRY(hipHostMalloc((void**)&c->gmiHost, sizeof(int) * 2 * GMI_COUNT));
+    for (auto& e : c->gmEv)
+        if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    struct Execs {
+        std::map<int, hipGraphExec_t> x;
+        ~Execs() {
+            for (auto& kv : x) (void)hipGraphExecDestroy(kv.second);
+        }
+    } execs;
+    auto graph_for = [&](int K, hipGraphExec_t& out) -> int {
+        auto it = execs.x.find(K);
+        if (it != execs.x.end()) {
+            out = it->second;
+            return ASTRO_OK;
+        }
+        HIP_TRY(hipStreamSynchronize(c->s));
+        HIP_TRY(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        const int rc = with_prof(c, [&](auto& P) -> int {
+            if (!c->gnnOn) return record_newmark_step(c, P, P, false, K, c->s);
+            TwoStagePrec<std::decay_t<decltype(P)>> T{P,        c->op, c->gnnB.p,  c->nodeDofs.p, c->prep.numNodes,
+                                                      c->prep.elMatDim, c->gnnT1.p, c->gnnT2.p};
+            return record_newmark_step(c, P, T, true, K, c->s);
+        });
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->s, &g);
+        if (rc != ASTRO_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIP_TRY(e);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(ei);
+        execs.x[K] = x;
+        c->dynGraphs += 1;
+        out = x;
+        return ASTRO_OK;
+    };
+    auto set_state = [&](int step) -> int {
+        int* h = c->gmiHost;  // both snapshot slots are idle here
+        for (int q = 0; q < GMI_COUNT; ++q) h[q] = 0;
+        h[GMI_RUN] = 1;
+        h[GMI_STEP] = step;
+        h[GMI_FAILED] = -1;
+        HIP_TRY(hipMemcpyAsync(c->gmi.p, h, sizeof(int) * GMI_COUNT, hipMemcpyHostToDevice, c->s));
+        HIP_TRY(hipStreamSynchronize(c->s));
+        return ASTRO_OK;
+    };
+    int K = graph_cap(c->lastSolveSteps, m);
+    std::vector<int> recent;  // Arnoldi counts of the last replayed steps
+    int k = 1, pending = -1;
+    ASTRO_TRY(set_state(k));
+    while (k < nSteps || pending >= 0) {
+        int launched = -1;
+        if (k < nSteps) {
+            hipGraphExec_t x = nullptr;
+            ASTRO_TRY(graph_for(K, x));
+            HIP_TRY(hipGraphLaunch(x, c->s));
+            HIP_TRY(hipMemcpyAsync(c->gmiHost + GMI_COUNT * (k & 1), c->gmi.p, sizeof(int) * GMI_COUNT,
+                                   hipMemcpyDeviceToHost, c->s));
+            HIP_TRY(hipEventRecord(c->gmEv[k & 1], c->s));
+            launched = k++;
+        }
+        if (pending >= 0) {
+            HIP_TRY(hipEventSynchronize(c->gmEv[pending & 1]));
+            const int* h = c->gmiHost + GMI_COUNT * (pending & 1);
+            if (!h[GMI_RUN]) {
+                if (h[GMI_FAILED] != pending) return fail(ASTRO_ERR_DEVICE, "step graph state out of order");
+                // the launch queued behind the missed step did nothing
+                HIP_TRY(hipStreamSynchronize(c->s));
+                const int cycleSteps = h[GMI_K];
+                ASTRO_TRY(finish_step_eagerly(c, cycleSteps));
+                K = graph_cap(cycleSteps + c->gm.steps, m);
+                recent.clear();
+                k = pending + 1;
+                pending = -1;
+                if (k < nSteps) ASTRO_TRY(set_state(k));
+                continue;
+            }
+            c->dynSteps += 1;
+            c->dynGraphSteps += 1;
+            c->lastSolveSteps = h[GMI_K];
+            c->dynGmresSteps += h[GMI_K];
+            recent.push_back(h[GMI_K]);
+            if (recent.size() > 16) recent.erase(recent.begin());
+            // a cap far above the recent need costs skipped launches: narrow it
+            const int need = *std::max_element(recent.begin(), recent.end());
+            if (recent.size() == 16 && graph_cap(need, m) * 2 < K) K = graph_cap(need, m);
+        }
+        pending = launched;
+    }
+    c->dynGraphCap = K;
+    return with_prof(c, [&](auto& P) -> int { return P.check_timeout(c->s); });
+}
+
 // nSteps Newmark steps with the load factors loadScale[0..nSteps-1]; the
-// elapsed time of the run is kept as scalar "dynMs".
+// elapsed time of the run is kept as scalar "dynMs".  With stepGraphs = 1
+// the steps after the first replay HIP graphs (dynamic_run_graphs).
 int astro_ctx_dynamic_run(astro_ctx* c, int nSteps, const double* loadScale) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     if (nSteps < 0 || (nSteps > 0 && !loadScale)) return fail(ASTRO_ERR_ARG, "bad step count or load factors");
+    if (!c->dynReady) return fail(ASTRO_ERR_ARG, "astro_ctx_dynamic_init has not been called");
+    bool timing = false;
+    with_prof(c, [&](auto& P) -> int {
+        timing = P.timing;
+        return ASTRO_OK;
+    });
     HIP_TRY(hipEventRecord(c->ev[6], c->s));
-    for (int k = 0; k < nSteps; ++k) ASTRO_TRY(astro_ctx_dynamic_step(c, loadScale[k]));
+    if (c->opt.stepGraphs && !timing && nSteps > 1) {
+        ASTRO_TRY(dynamic_run_graphs(c, nSteps, loadScale));
+    } else {
+        for (int k = 0; k < nSteps; ++k) ASTRO_TRY(astro_ctx_dynamic_step(c, loadScale[k]));
+    }
     HIP_TRY(hipEventRecord(c->ev[7], c->s));
     HIP_TRY(hipEventSynchronize(c->ev[7]));
     float ms = 0;
@@ -1642,10 +1912,10 @@ int astro_ctx_thermal_solve(astro_ctx* c) {
         ctx_minus_internal_force(s, c->tVal, c->tCols, c->tRange, N, c->nodeTemp.p, c->tLoad.p);
         if (cd > 0)
             hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p,
-                               c->tMpcCols.p, c->tMpcRange.p, c->tMpcRhs.p, cd, c->nodeTemp.p, c->tLoad.p);
+                               c->tMpcCols.p, c->tMpcRange.p, c->tMpcRhs.p, cd, c->nodeTemp.p, c->tLoad.p, (const int*)nullptr);
         ASTRO_TRY(c->tgm.solve(c->top, P, c->tLoad.p, c->tDel.p, c->opt.rtol, 0.0, c->opt.maxCycles, s));
         hipLaunchKernelGGL(k_axpby, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->nodeTemp.p, c->tDel.p,
-                           c->nodeTemp.p, N, 1.0, 1.0);
+                           c->nodeTemp.p, N, 1.0, 1.0, (const int*)nullptr);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(s));
         return ASTRO_OK;
@@ -1773,9 +2043,9 @@ int astro_ctx_gnn_collect(astro_ctx* c, astro_gnn* g, int nRhs, uint64_t seed, d
         double* r1 = f->R1.p + (size_t)k * n;
         ASTRO_TRY(c->op.apply(z0.p, t.p, s));
         hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)b.p,
-                           (const double*)t.p, r1, n, 1.0, -1.0);
+                           (const double*)t.p, r1, n, 1.0, -1.0, (const int*)nullptr);
         hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)x.p,
-                           (const double*)z0.p, e1.p, n, 1.0, -1.0);
+                           (const double*)z0.p, e1.p, n, 1.0, -1.0, (const int*)nullptr);
         c->gm.multidot(e1.p, 1, r1, c->gm.hdev.p, s);
         HIP_TRY(hipMemcpyAsync(c->gm.hHost, c->gm.hdev.p, sizeof(double), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
@@ -1885,10 +2155,10 @@ int astro_ctx_true_residual(astro_ctx* c, double* relRes) {
     const int nl = (int)c->loadDofs.n, cd = c->prep.elMPCDim;
     if (nl > 0)
         hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
-                           nl, 1.0, g.t.p);
+                           nl, 1.0, g.t.p, (const int*)nullptr);
     if (cd > 0)
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, c->s, c->mpcMat.p, c->mpcCols.p,
-                           c->mpcRange.p, c->mpcRhs.p, cd, (const double*)nullptr, g.t.p);
+                           c->mpcRange.p, c->mpcRhs.p, cd, (const double*)nullptr, g.t.p, (const int*)nullptr);
     ASTRO_TRY(c->op.residual(g.t.p, c->nodeDisp.p, g.t2.p, c->s));
     g.multidot(g.t2.p, 1, g.t2.p, g.hdev.p, c->s);
     g.multidot(g.t.p, 1, g.t.p, g.hdev.p + 1, c->s);
@@ -1955,24 +2225,24 @@ static int true_residual_noalloc(astro_ctx* c, const double* d_u, double* d_r, d
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b, (int64_t)n, 0.0);
     if (nl > 0)
         hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, s, c->loadDofs.p, c->loadVals.p, nl,
-                           1.0, b);
+                           1.0, b, (const int*)nullptr);
     if (cd > 0) {
         hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->mpcMat.p, c->mpcCols.p,
-                           c->mpcRange.p, c->mpcRhs.p, cd, (const double*)nullptr, b);
+                           c->mpcRange.p, c->mpcRhs.p, cd, (const double*)nullptr, b, (const int*)nullptr);
         hipLaunchKernelGGL(k_mpc_dd, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->mpcMat.p, c->mpcCols.p,
-                           c->mpcRange.p, cd, d_u, cx);
+                           c->mpcRange.p, cd, d_u, cx, (const int*)nullptr);
     }
     // the kernels and reduction order of astro_ctx_true_residual
     hipLaunchKernelGGL(k_residual_dd, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, c->csrVal.p,
                        c->csrCols.p, c->csrRange.p, n, cd > 0 ? c->mpcT.range.p : (const int*)nullptr, c->mpcT.pos.p,
-                       c->mpcT.row.p, c->mpcMat.p, (const double*)cx, (const double*)b, d_u, d_r);
+                       c->mpcT.row.p, c->mpcMat.p, (const double*)cx, (const double*)b, d_u, d_r, (const int*)nullptr);
     const int nblk = c->gm.nblk;
     hipLaunchKernelGGL(k_multidot, dim3(nblk, 1), dim3(256), 0, s, (const double*)d_r, (int64_t)n, 1,
-                       (const double*)d_r, n, partR, 1);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double*)partR, nblk, 1, 1, norms);
+                       (const double*)d_r, n, partR, 1, (const int*)nullptr);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double*)partR, nblk, 1, 1, norms, (const int*)nullptr);
     hipLaunchKernelGGL(k_multidot, dim3(nblk, 1), dim3(256), 0, s, (const double*)b, (int64_t)n, 1, (const double*)b,
-                       n, partB, 1);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double*)partB, nblk, 1, 1, norms + 1);
+                       n, partB, 1, (const int*)nullptr);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, s, (const double*)partB, nblk, 1, 1, norms + 1, (const int*)nullptr);
     HIP_TRY(hipGetLastError());
     // staged through the context's pinned buffer, allocated at create
     HIP_TRY(hipMemcpyAsync(c->gm.hHost, norms, 2 * sizeof(double), hipMemcpyDeviceToHost, s));

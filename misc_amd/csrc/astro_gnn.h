@@ -85,7 +85,7 @@ __global__ void k_gnn_features(int N, const int* nodeDofs, const int* dofNode, c
 __global__ void k_gnn_center_log(int N, float* X, float mu);
 
 // z[dofs of i] += B_i t[dofs of i] (B: N x 9 row-major).
-__global__ void k_node_block_apply(int N, const int* nodeDofs, const double* B, const double* t, double* z);
+__global__ void k_node_block_apply(int N, const int* nodeDofs, const double* B, const double* t, double* z, const int* gate);
 // Deterministic standard normal vector from (seed, index).
 __global__ void k_normal_fill(double* x, int64_t n, uint64_t seed);
 

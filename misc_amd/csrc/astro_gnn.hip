@@ -375,7 +375,8 @@ __global__ void k_gnn_center_log(int N, float* __restrict__ X, float mu) {
 }
 
 __global__ void k_node_block_apply(int N, const int* __restrict__ nodeDofs, const double* __restrict__ B,
-                                   const double* __restrict__ t, double* __restrict__ z) {
+                                   const double* __restrict__ t, double* __restrict__ z, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int d[3];
@@ -570,10 +571,10 @@ int GnnNet::loss_grad(const GnnField& f, double* lossDev, hipStream_t s) {
                            (const double*)B.p, r, q.p);
         hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s,
                            (const double*)f.val.p, (const int*)f.cols.p, (const int*)f.range.p, n, (const double*)q.p,
-                           w.p);
+                           w.p, (const int*)nullptr);
         if (f.cdim > 0)
             hipLaunchKernelGGL(k_mpc_apply, dim3(grid_for(f.cdim, 256)), dim3(256), 0, s, (const double*)f.cmat.p,
-                               (const int*)f.ccols.p, (const int*)f.crange.p, f.cdim, (const double*)q.p, w.p);
+                               (const int*)f.ccols.p, (const int*)f.crange.p, f.cdim, (const double*)q.p, w.p, (const int*)nullptr);
         hipLaunchKernelGGL(k_gnn_energy_acc, dim3(grid_for(N, 256)), dim3(256), 0, s, N, (const int*)f.nodeDofs.p, r,
                            (const double*)q.p, (const double*)w.p, gB.p, lossNode.p);
     }

@@ -64,9 +64,9 @@ __global__ void k_shell4_dtangent_cs(const int* perm, int count, const int* conn
 __global__ void k_brick8_residual(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                   const double* elScale, const double* matC, const double* elRho, const int* rank,
                                   int nS, const double* u, const double* pu, const double* pv, const double* pa,
-                                  double c1, double c2, double dt, double beta, double* load);
+                                  double c1, double c2, double dt, double beta, double* load, const int* gate);
 __global__ void k_newmark_update(int n, double* u, const double* du, double* pu, double* pv, double* pa, double* v,
-                                 double* a, double c1, double c2, double dt, double beta, double gamma);
+                                 double* a, double c1, double c2, double dt, double beta, double gamma, const int* gate);
 __global__ void k_brick8_conduct(const int* perm, int count, const int* conn, const double* X, const double* elCond,
                                  const int* rank, const int* csrRange, const uint8_t* thPos, double* val);
 // k ranges of step U: all columns before the panel, only those before the
@@ -98,7 +98,7 @@ __global__ void k_max_abs(const double* v, int64_t n, unsigned long long* out);
 __global__ void k_scale_mpc(double* cmat, double* rhs, const int* crange, int cdim, const unsigned long long* maxBits);
 __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_fill_i32(int* p, int64_t n, int v);
-__global__ void k_fill_sentinel(double* p, int64_t n);
+__global__ void k_fill_sentinel(double* p, int64_t n, const int* gate);
 template <typename T>
 __global__ void k_fill_t(T* p, int64_t n, T v);
 template <typename LT>
@@ -114,32 +114,50 @@ __global__ void k_sweep_blocks_pair(const SweepBlock* blocks, double* out);
 template <typename LT>
 __global__ void k_sweep_forward(const SweepTile* tiles, int numTiles, const LT* lt, const int64_t* rowoff,
                                 const int* sfirst, const double* blocks, const double* b, double* z, double* g,
-                                int* flags, int* ticket, int epoch, long spinLimit, int* timeout);
+                                int* flags, int* ticket, int epoch, long spinLimit, int* timeout, const int* gate);
 // diag: FP64 pivots, read by the FP32 instantiation (the FP64 one reads them
 // from the profile's diagonal)
 template <typename LT>
 __global__ void k_sweep_backward(const SweepTile* tiles, int numTiles, const LT* lt, const int64_t* rowoff,
                                  const int* sfirst, const double* blocks, const double* z, double* x, double* g,
-                                 int* flags, int* ticket, int epoch, long spinLimit, int* timeout, const double* diag);
-__global__ void k_reset_ticket(int* ticket);
-__global__ void k_spmv(const double* val, const int* cols, const int* range, int n, const double* x, double* y);
+                                 int* flags, int* ticket, int epoch, long spinLimit, int* timeout, const double* diag, const int* gate);
+__global__ void k_reset_ticket(int* ticket, const int* gate);
+__global__ void k_spmv(const double* val, const int* cols, const int* range, int n, const double* x, double* y, const int* gate);
 __global__ void k_mpc_dd(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,
-                         double* t);
+                         double* t, const int* gate);
 __global__ void k_residual_dd(const double* val, const int* cols, const int* range, int n, const int* ctRange,
                               const int* ctPos, const int* ctRow, const double* cmat, const double* t,
-                              const double* b, const double* x, double* r);
+                              const double* b, const double* x, double* r, const int* gate);
 __global__ void k_mpc_apply(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,
-                            double* y);
-__global__ void k_nodal_loads(const int* dofs, const double* vals, int count, double scale, double* load);
+                            double* y, const int* gate);
+__global__ void k_nodal_loads(const int* dofs, const double* vals, int count, double scale, double* load, const int* gate);
 __global__ void k_constraint_load(const double* cmat, const int* ccols, const int* crange, const double* crhs,
-                                  int cdim, const double* u, double* load);
+                                  int cdim, const double* u, double* load, const int* gate);
 __global__ void k_multidot(const double* V, int64_t ldv, int nqTotal, const double* w, int n, double* partial,
-                           int pstride);
-__global__ void k_sum_partials(const double* partial, int nblk, int pstride, int nq, double* out);
+                           int pstride, const int* gate);
+__global__ void k_sum_partials(const double* partial, int nblk, int pstride, int nq, double* out, const int* gate);
 __global__ void k_sub_combination(const double* V, int64_t ldv, int nq, const double* h, double* w, int n,
-                                  double sign);
+                                  double sign, const int* gate);
+// Device-side GMRES cycle of a captured time step (layout in astro_kernels.hip).
+enum { GMD_RES = 0, GMD_TARGET = 1, GMD_RES0 = 2, GMD_HNORM = 3, GMD_INVH = 4, GMD_INVRES = 5, GMD_JUDGE = 6, GMD_R = 8 };
+enum { GMI_ACTIVE = 0, GMI_K = 1, GMI_RUN = 2, GMI_STEP = 3, GMI_FAILED = 4, GMI_COUNT = 8 };
+// doubles of gd for restart length m
+inline size_t gmd_size(int m) { return GMD_R + (size_t)(m + 1) * m + 2 * (size_t)m + (m + 1) + m; }
+inline size_t gmd_y(int m) { return GMD_R + (size_t)(m + 1) * m + 2 * (size_t)m + (m + 1); }
+__global__ void k_gm_begin(const double* nrm2, double rtol, double atol, double* gd, int* gi, int m);
+__global__ void k_gm_givens(int j, const double* h1, const double* h2, const double* hn, double* gd, int* gi, int m);
+__global__ void k_gm_lsq(double* gd, const int* gi, int m);
+__global__ void k_sub_combination_dn(const double* V, int64_t ldv, const int* nq, const double* h, double* w, int n,
+                                     double sign, const int* gate);
+__global__ void k_gm_judge_target(const double* nrm2, double rtol, double* gd, const int* gi);
+__global__ void k_gm_check(const double* nrm2, double* gd, int* gi, int slot);
+__global__ void k_gm_advance(int* gi);
+__global__ void k_scale_by(const double* x, double* y, int n, const double* s, const int* gate);
+__global__ void k_fill_gated(double* p, int64_t n, double v, const int* gate);
+__global__ void k_nodal_loads_dev(const int* dofs, const double* vals, int count, const double* scales,
+                                  const int* stepIdx, double* load, const int* gate);
 __global__ void k_scale_copy(const double* x, double* y, int n, const double* den);
-__global__ void k_axpby(const double* x, const double* y, double* out, int n, double a, double b);
+__global__ void k_axpby(const double* x, const double* y, double* out, int n, double a, double b, const int* gate);
 __global__ void k_count_zero_cols(const int* cols, int64_t nnz, int* out);
 
 }  // namespace astro

@@ -488,7 +488,8 @@ __global__ __launch_bounds__(64) void k_brick8_residual(
     const int* __restrict__ elMat, const double* __restrict__ elScale, const double* __restrict__ matC,
     const double* __restrict__ elRho, const int* __restrict__ rank, int nS, const double* __restrict__ u,
     const double* __restrict__ pu, const double* __restrict__ pv, const double* __restrict__ pa, double c1, double c2,
-    double dt, double beta, double* __restrict__ load) {
+    double dt, double beta, double* __restrict__ load, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int idx = blockIdx.x;
     if (idx >= count) return;
     const int e = perm[idx];
@@ -585,7 +586,8 @@ __global__ __launch_bounds__(64) void k_brick8_residual(
 __global__ void k_newmark_update(int n, double* __restrict__ u, const double* __restrict__ du,
                                  double* __restrict__ pu, double* __restrict__ pv, double* __restrict__ pa,
                                  double* __restrict__ v, double* __restrict__ a, double c1, double c2, double dt,
-                                 double beta, double gamma) {
+                                 double beta, double gamma, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double un = u[i] + du[i];
         const double an = c1 * (c2 * (un - pu[i] - dt * pv[i]) - (1.0 - 2.0 * beta) * pa[i]);
@@ -713,7 +715,8 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
         p[i] = v;
 }
 
-__global__ void k_fill_sentinel(double* __restrict__ p, int64_t n) {
+__global__ void k_fill_sentinel(double* __restrict__ p, int64_t n, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const double v = __longlong_as_double(ASTRO_SENTINEL);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -1829,7 +1832,8 @@ __global__ __launch_bounds__(256, 1) void k_sweep_forward(const SweepTile* __res
                                                           const double* __restrict__ b, double* __restrict__ z,
                                                           double* __restrict__ g, int* __restrict__ flags,
                                                           int* __restrict__ ticket, int epoch, long spinLimit,
-                                                          int* __restrict__ timeout) {
+                                                          int* __restrict__ timeout, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ int sT;
     __shared__ double part[4][16];
     __shared__ double acc[4][64];
@@ -2016,10 +2020,10 @@ __global__ __launch_bounds__(256, 1) void k_sweep_forward(const SweepTile* __res
 }
 template __global__ void k_sweep_forward<double>(const SweepTile*, int, const double*, const int64_t*, const int*,
                                                  const double*, const double*, double*, double*, int*, int*, int,
-                                                 long, int*);
+                                                 long, int*, const int*);
 template __global__ void k_sweep_forward<float>(const SweepTile*, int, const float*, const int64_t*, const int*,
                                                 const double*, const double*, double*, double*, int*, int*, int,
-                                                long, int*);
+                                                long, int*, const int*);
 
 // Backward sweep tile (columns [r0, r1)): phase 1 takes the rows of the far
 // tiles (five or more tiles on) and of the tiles three and four on (coupling
@@ -2033,7 +2037,8 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
                                                         const double* __restrict__ z, double* __restrict__ x,
                                                         double* __restrict__ g, int* __restrict__ flags,
                                                         int* __restrict__ ticket, int epoch, long spinLimit,
-                                                        int* __restrict__ timeout, const double* __restrict__ diag) {
+                                                        int* __restrict__ timeout, const double* __restrict__ diag, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ int sT;
     __shared__ double part[4][64];
     __shared__ double acc[4][64];
@@ -2223,19 +2228,21 @@ __global__ __launch_bounds__(256) void k_sweep_backward(const SweepTile* __restr
 }
 template __global__ void k_sweep_backward<double>(const SweepTile*, int, const double*, const int64_t*, const int*,
                                                   const double*, const double*, double*, double*, int*, int*, int,
-                                                  long, int*, const double*);
+                                                  long, int*, const double*, const int*);
 template __global__ void k_sweep_backward<float>(const SweepTile*, int, const float*, const int64_t*, const int*,
                                                  const double*, const double*, double*, double*, int*, int*, int,
-                                                 long, int*, const double*);
+                                                 long, int*, const double*, const int*);
 
-__global__ void k_reset_ticket(int* ticket) { *ticket = 0; }
+__global__ void k_reset_ticket(int* ticket, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return; *ticket = 0; }
 
 // ---------------------------------------------------------------------------
 // Operator y = A x + C^T (C x) with the reference's CSR (1-based columns,
 // 0 = unused slot).  Working/AStrO_solvers.f90:143-171.  32 lanes per row.
 __global__ __launch_bounds__(256) void k_spmv(const double* __restrict__ val, const int* __restrict__ cols,
                                               const int* __restrict__ range, int n, const double* __restrict__ x,
-                                              double* __restrict__ y) {
+                                              double* __restrict__ y, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     const int l = threadIdx.x & 31;
     if (row >= n) return;
@@ -2283,7 +2290,8 @@ __device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double a, do
 // t[r] = (C x)_r as a double-double pair (t[2r], t[2r + 1]).
 __global__ void k_mpc_dd(const double* __restrict__ cmat, const int* __restrict__ ccols,
                          const int* __restrict__ crange, int cdim, const double* __restrict__ x,
-                         double* __restrict__ t) {
+                         double* __restrict__ t, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= cdim) return;
     double hi = 0.0, lo = 0.0;
@@ -2305,7 +2313,8 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
                                                      const int* __restrict__ ctRange, const int* __restrict__ ctPos,
                                                      const int* __restrict__ ctRow, const double* __restrict__ cmat,
                                                      const double* __restrict__ t, const double* __restrict__ b,
-                                                     const double* __restrict__ x, double* __restrict__ r) {
+                                                     const double* __restrict__ x, double* __restrict__ r, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     const int l = threadIdx.x & 31;
     if (row >= n) return;
@@ -2336,7 +2345,8 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
 
 __global__ void k_mpc_apply(const double* __restrict__ cmat, const int* __restrict__ ccols,
                             const int* __restrict__ crange, int cdim, const double* __restrict__ x,
-                            double* __restrict__ y) {
+                            double* __restrict__ y, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= cdim) return;
     double t = 0.0;
@@ -2349,7 +2359,8 @@ __global__ void k_mpc_apply(const double* __restrict__ cmat, const int* __restri
 // RHS pieces: nodal loads and the constraint load C^T (rhs - C u) (u null:
 // C^T rhs).  Working/AStrO_commandFunctions.f90:368-392, 506-535.
 __global__ void k_nodal_loads(const int* __restrict__ dofs, const double* __restrict__ vals, int count,
-                              double scale, double* __restrict__ load) {
+                              double scale, double* __restrict__ load, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     atomicAdd(&load[dofs[k] - 1], scale * vals[k]);
@@ -2357,7 +2368,8 @@ __global__ void k_nodal_loads(const int* __restrict__ dofs, const double* __rest
 
 __global__ void k_constraint_load(const double* __restrict__ cmat, const int* __restrict__ ccols,
                                   const int* __restrict__ crange, const double* __restrict__ crhs, int cdim,
-                                  const double* __restrict__ u, double* __restrict__ load) {
+                                  const double* __restrict__ u, double* __restrict__ load, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= cdim) return;
     double t = crhs[r];
@@ -2374,7 +2386,8 @@ __global__ void k_constraint_load(const double* __restrict__ cmat, const int* __
 // the partials in a second fixed-order pass (deterministic).
 __global__ __launch_bounds__(256) void k_multidot(const double* __restrict__ V, int64_t ldv, int nqTotal,
                                                   const double* __restrict__ w, int n, double* __restrict__ partial,
-                                                  int pstride) {
+                                                  int pstride, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int tid = threadIdx.x;
     const int q0 = blockIdx.y * 8, nq = min(8, nqTotal - q0);
     double acc[8];
@@ -2403,7 +2416,8 @@ __global__ __launch_bounds__(256) void k_multidot(const double* __restrict__ V, 
 // threads), lanes take blocks b = lane, lane + 64, ..., then a fixed
 // butterfly, so the order of the additions never changes.
 __global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ partial, int nblk, int pstride,
-                                                     int nq, double* __restrict__ out) {
+                                                     int nq, double* __restrict__ out, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int q = blockIdx.x, lane = threadIdx.x;
     if (q >= nq) return;
     double s = 0.0;
@@ -2412,9 +2426,154 @@ __global__ __launch_bounds__(64) void k_sum_partials(const double* __restrict__ 
     if (lane == 0) out[q] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Device-side GMRES cycle for captured (hipGraph) time steps.  The cycle's
+// scalars live on the device: gd = [res, target, res0, hnorm, 1/hnorm,
+// 1/res, judge target, -, R (m+1 x m, column-major), cs (m), sn (m),
+// g (m+1), y (m)], gi = [active, k, run, stepIdx, failedStep].  run = 0 after a step
+// whose solve missed its target (every later gated kernel then skips until
+// the host has finished that step); active = 1 while the cycle's Arnoldi
+// steps are still needed.  The arithmetic is the host cycle's
+// (Gmres::solve): CGS2 coefficients, Givens rotations on the new column,
+// g[j + 1] = -sn g[j], the stop |g[j + 1]| <= target, back substitution.
+__device__ __forceinline__ size_t gmd_cs(int m) { return GMD_R + (size_t)(m + 1) * m; }
+
+// Cycle start from nrm2 = |P^-1 r|^2: target = max(rtol res, atol) (the
+// solve's first cycle), g = res e1, active = run && res > target.
+__global__ void k_gm_begin(const double* __restrict__ nrm2, double rtol, double atol, double* __restrict__ gd,
+                           int* __restrict__ gi, int m) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!gi[GMI_RUN]) {
+        gi[GMI_ACTIVE] = 0;
+        return;
+    }
+    const double res = sqrt(*nrm2);
+    gd[GMD_RES] = res;
+    gd[GMD_RES0] = res;
+    gd[GMD_TARGET] = fmax(rtol * res, atol);
+    gd[GMD_INVRES] = res > 0.0 ? 1.0 / res : 0.0;
+    double* g = gd + gmd_cs(m) + 2 * (size_t)m;
+    for (int i = 0; i <= m; ++i) g[i] = 0.0;
+    g[0] = res;
+    gi[GMI_K] = 0;
+    gi[GMI_ACTIVE] = res > gd[GMD_TARGET] ? 1 : 0;
+}
+
+// Arnoldi step j: the new column h1 + h2 (CGS2 passes) and |w| = sqrt(hn),
+// rotated by the earlier rotations, a new rotation, g, and the stop test.
+__global__ void k_gm_givens(int j, const double* __restrict__ h1, const double* __restrict__ h2,
+                            const double* __restrict__ hn, double* __restrict__ gd, int* __restrict__ gi, int m) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!gi[GMI_ACTIVE]) return;
+    double* R = gd + GMD_R + (size_t)(m + 1) * j;
+    double* cs = gd + gmd_cs(m);
+    double* sn = cs + m;
+    double* g = sn + m;
+    const double hnorm = sqrt(*hn);
+    for (int i = 0; i <= j; ++i) R[i] = h1[i] + h2[i];
+    for (int i = 0; i < j; ++i) {
+        const double a = cs[i] * R[i] + sn[i] * R[i + 1];
+        const double b = -sn[i] * R[i] + cs[i] * R[i + 1];
+        R[i] = a;
+        R[i + 1] = b;
+    }
+    const double rr = hypot(R[j], hnorm);
+    cs[j] = rr == 0.0 ? 1.0 : R[j] / rr;
+    sn[j] = rr == 0.0 ? 0.0 : hnorm / rr;
+    R[j] = rr;
+    R[j + 1] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    gi[GMI_K] = j + 1;
+    gd[GMD_HNORM] = hnorm;
+    gd[GMD_INVH] = hnorm > 0.0 ? 1.0 / hnorm : 0.0;
+    if (hnorm == 0.0 || fabs(g[j + 1]) <= gd[GMD_TARGET]) gi[GMI_ACTIVE] = 0;
+}
+
+// Judge target rtol sqrt(nrm2) (the plain-preconditioned starting residual).
+__global__ void k_gm_judge_target(const double* __restrict__ nrm2, double rtol, double* __restrict__ gd,
+                                  const int* __restrict__ gi) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (gi[GMI_RUN]) gd[GMD_JUDGE] = rtol * sqrt(*nrm2);
+}
+
+// y = R^-1 g over the cycle's k columns (run only).
+__global__ void k_gm_lsq(double* __restrict__ gd, const int* __restrict__ gi, int m) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!gi[GMI_RUN]) return;
+    const int k = gi[GMI_K];
+    const double* cs = gd + gmd_cs(m);
+    const double* g = cs + 2 * (size_t)m;
+    double* y = (double*)g + (m + 1);
+    for (int i = k - 1; i >= 0; --i) {
+        double v = g[i];
+        for (int q = i + 1; q < k; ++q) v -= gd[GMD_R + (size_t)(m + 1) * q + i] * y[q];
+        y[i] = v / gd[GMD_R + (size_t)(m + 1) * i + i];
+    }
+}
+
+// w += sign sum_{q < *nq} V[q] h[q] (nq on the device).
+__global__ void k_sub_combination_dn(const double* __restrict__ V, int64_t ldv, const int* __restrict__ nq,
+                                     const double* __restrict__ h, double* __restrict__ w, int n, double sign,
+                                     const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int k = *nq;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < k; ++q) s += V[(size_t)q * ldv + i] * h[q];
+        w[i] += sign * s;
+    }
+}
+
+// The step's judge: the true preconditioned residual after the cycle,
+// sqrt(nrm2), against gd[slot] (the cycle's target or the judge target); a
+// miss stops the pipeline.
+__global__ void k_gm_check(const double* __restrict__ nrm2, double* __restrict__ gd, int* __restrict__ gi,
+                           int slot) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!gi[GMI_RUN]) return;
+    const double res = sqrt(*nrm2);
+    gd[GMD_RES] = res;
+    if (!(res <= gd[slot])) {
+        gi[GMI_RUN] = 0;
+        gi[GMI_FAILED] = gi[GMI_STEP];
+    }
+}
+
+__global__ void k_gm_advance(int* __restrict__ gi) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (gi[GMI_RUN]) gi[GMI_STEP] += 1;
+}
+
+// y = x * (*s), gated.
+__global__ void k_scale_by(const double* __restrict__ x, double* __restrict__ y, int n, const double* __restrict__ sp,
+                           const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const double a = *sp;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[i];
+}
+
+__global__ void k_fill_gated(double* __restrict__ p, int64_t n, double v, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Nodal loads times scales[*stepIdx] (the captured step reads its load
+// factor from the device table).
+__global__ void k_nodal_loads_dev(const int* __restrict__ dofs, const double* __restrict__ vals, int count,
+                                  const double* __restrict__ scales, const int* __restrict__ stepIdx,
+                                  double* __restrict__ load, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    atomicAdd(&load[dofs[k] - 1], scales[*stepIdx] * vals[k]);
+}
+
 // w -= sum_q V[q] h[q]   (h on the device)
 __global__ void k_sub_combination(const double* __restrict__ V, int64_t ldv, int nq, const double* __restrict__ h,
-                                  double* __restrict__ w, int n, double sign) {
+                                  double* __restrict__ w, int n, double sign, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ double sh[256];
     for (int q = threadIdx.x; q < nq; q += blockDim.x) sh[q] = h[q];
     __syncthreads();
@@ -2433,7 +2592,8 @@ __global__ void k_scale_copy(const double* __restrict__ x, double* __restrict__ 
 }
 
 __global__ void k_axpby(const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ out,
-                        int n, double a, double b) {
+                        int n, double a, double b, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         out[i] = a * x[i] + b * y[i];
 }
