@@ -285,12 +285,16 @@ int astro_ctx_true_residual(astro_ctx* ctx, double* relRes);
  * continues as fallback when it is not.  brick8 meshes.
  *
  * Training data are generated by the solver: astro_ctx_gnn_collect, on a
- * context whose operator is factored (after astro_ctx_dynamic_init), solves
- * A x = b for nRhs seeded random b by LDL^T-preconditioned GMRES to rtol and
- * keeps r1 = b - A P^-1 b, the residual the LDL^T stage leaves, with
- * sum e1^T r1, e1 = x - P^-1 b = A^-1 r1 its error, and a copy of A; the loss
- * of a field is the energy norm of the error the block stage leaves,
- * sum (e1 - B r1)^T A (e1 - B r1) / sum e1^T A e1 over its pairs.
+ * context whose operator is factored (after astro_ctx_dynamic_init), forms
+ * nRhs seeded pairs (b, x) of A x = b and keeps r1 = b - A P^-1 b, the
+ * residual the LDL^T stage leaves, with sum e1^T r1, e1 = x - P^-1 b =
+ * A^-1 r1 its error, and a copy of A; the loss of a field is the energy norm
+ * of the error the block stage leaves, sum (e1 - B r1)^T A (e1 - B r1) /
+ * sum e1^T A e1 over its pairs.  pairs = ASTRO_GNN_PAIRS_SOLUTION: x standard
+ * normal, b = A x (the errors cover the whole spectrum, so the loss sees the
+ * modes a block stage can amplify); ASTRO_GNN_PAIRS_SOLVE: b standard normal,
+ * x = A^-1 b by LDL^T-preconditioned GMRES to rtol (the errors are A^-1-
+ * smoothed and the loss is blind to amplified stiff modes).
  * astro_gnn_train runs Adam (lr, epochs over the collected fields in seeded
  * order) with the library's forward and backward kernels; lossPerEpoch
  * (epochs, may be null) receives the mean field loss of each epoch.
@@ -306,7 +310,8 @@ int64_t astro_gnn_num_params(const astro_gnn* gnn);
 int astro_gnn_num_fields(const astro_gnn* gnn);
 int astro_gnn_get_params(astro_gnn* gnn, float* host, int64_t count);
 int astro_gnn_set_params(astro_gnn* gnn, const float* host, int64_t count);
-int astro_ctx_gnn_collect(astro_ctx* ctx, astro_gnn* gnn, int nRhs, uint64_t seed, double rtol);
+enum { ASTRO_GNN_PAIRS_SOLUTION = 0, ASTRO_GNN_PAIRS_SOLVE = 1 };
+int astro_ctx_gnn_collect(astro_ctx* ctx, astro_gnn* gnn, int nRhs, uint64_t seed, double rtol, int32_t pairs);
 int astro_gnn_train(astro_gnn* gnn, int epochs, double lr, uint64_t seed, double* lossPerEpoch);
 int astro_gnn_loss_grad(astro_gnn* gnn, int field, double* loss, float* grad);
 int astro_gnn_field_size(const astro_gnn* gnn, int field, int32_t* numNodes, int32_t* numAdj, int32_t* numDofs,

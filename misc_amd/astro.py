@@ -132,7 +132,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_gnn_num_fields.argtypes = [C.c_void_p]
     L.astro_gnn_get_params.argtypes = [C.c_void_p, _fp, _i64]
     L.astro_gnn_set_params.argtypes = [C.c_void_p, _fp, _i64]
-    L.astro_ctx_gnn_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_double]
+    L.astro_ctx_gnn_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_double, C.c_int32]
     L.astro_gnn_train.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint64, C.c_void_p]
     L.astro_gnn_loss_grad.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_void_p]
     L.astro_gnn_field_size.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -496,10 +496,15 @@ class Context:
         return r.value
 
     # ---- learned block preconditioner
-    def gnn_collect(self, gnn: "Gnn", n_rhs: int, seed: int = 0, rtol: float = 1e-10):
-        """Add one training field: n_rhs solver-generated pairs on the current
-        (factored) operator."""
-        _check(load_library().astro_ctx_gnn_collect(self.h, gnn.h, int(n_rhs), int(seed), float(rtol)))
+    def gnn_collect(self, gnn: "Gnn", n_rhs: int, seed: int = 0, rtol: float = 1e-10, pairs: str = "solution"):
+        """Add one training field: n_rhs pairs of the current (factored)
+        operator, "solution" (x random, b = A x) or "solve" (b random, x by
+        GMRES to rtol), with the residual its LDL^T stage leaves."""
+        kinds = {"solution": 0, "solve": 1}
+        if pairs not in kinds:
+            raise ValueError("pairs must be 'solution' or 'solve'")
+        _check(load_library().astro_ctx_gnn_collect(self.h, gnn.h, int(n_rhs), int(seed), float(rtol),
+                                                    kinds[pairs]))
 
     def gnn_attach(self, gnn: "Gnn"):
         """Blocks of the network for this operator; GMRES then runs with the

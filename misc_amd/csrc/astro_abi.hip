@@ -1994,9 +1994,12 @@ static int ctx_gnn_features(astro_ctx* c) {
     return ASTRO_OK;
 }
 
-int astro_ctx_gnn_collect(astro_ctx* c, astro_gnn* g, int nRhs, uint64_t seed, double rtol) {
+int astro_ctx_gnn_collect(astro_ctx* c, astro_gnn* g, int nRhs, uint64_t seed, double rtol, int32_t pairs) {
     if (!c || !g) return fail(ASTRO_ERR_ARG, "null argument");
-    if (nRhs < 1 || !(rtol > 0.0)) return fail(ASTRO_ERR_ARG, "need nRhs >= 1 and rtol > 0");
+    if (pairs != ASTRO_GNN_PAIRS_SOLUTION && pairs != ASTRO_GNN_PAIRS_SOLVE)
+        return fail(ASTRO_ERR_ARG, "pairs must be ASTRO_GNN_PAIRS_SOLUTION or ASTRO_GNN_PAIRS_SOLVE");
+    if (nRhs < 1 || (pairs == ASTRO_GNN_PAIRS_SOLVE && !(rtol > 0.0)))
+        return fail(ASTRO_ERR_ARG, "need nRhs >= 1 (and rtol > 0 for solved pairs)");
     if (!with_prof(c, [&](auto& P) { return P.factored ? 1 : 0; }))
         return fail(ASTRO_ERR_ARG, "collect needs a factored operator (astro_ctx_dynamic_init or a static factor)");
     ASTRO_TRY(ctx_gnn_setup(c));
@@ -2030,13 +2033,19 @@ int astro_ctx_gnn_collect(astro_ctx* c, astro_gnn* g, int nRhs, uint64_t seed, d
     for (DBuf<double>* v : {&b, &x, &z0, &t, &e1}) ASTRO_TRY(v->alloc(n));
     f->eTr = 0.0;
     for (int k = 0; k < nRhs; ++k) {
-        // a solver-generated pair: b random, x = A^-1 b by LDL^T-preconditioned GMRES;
-        // after the LDL^T stage z0 = P^-1 b the residual is r1 = b - A z0 and the error e1 = x - z0
-        hipLaunchKernelGGL(k_normal_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b.p, (int64_t)n,
-                           seed * 1000003ull + (uint64_t)k);
-        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x.p, (int64_t)n, 0.0);
+        // a pair of A x = b: x random and b = A x, or b random and x = A^-1 b by
+        // LDL^T-preconditioned GMRES; after the LDL^T stage z0 = P^-1 b the
+        // residual is r1 = b - A z0 and the error e1 = x - z0
+        const uint64_t sk = seed * 1000003ull + (uint64_t)k;
+        if (pairs == ASTRO_GNN_PAIRS_SOLUTION) {
+            hipLaunchKernelGGL(k_normal_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x.p, (int64_t)n, sk);
+            ASTRO_TRY(c->op.apply(x.p, b.p, s));
+        } else {
+            hipLaunchKernelGGL(k_normal_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b.p, (int64_t)n, sk);
+            hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x.p, (int64_t)n, 0.0);
+        }
         ASTRO_TRY(with_prof(c, [&](auto& P) -> int {
-            ASTRO_TRY(c->gm.solve(c->op, P, b.p, x.p, rtol, 0.0, c->opt.maxCycles, s));
+            if (pairs == ASTRO_GNN_PAIRS_SOLVE) ASTRO_TRY(c->gm.solve(c->op, P, b.p, x.p, rtol, 0.0, c->opt.maxCycles, s));
             ASTRO_TRY(P.apply(b.p, z0.p, s));
             return P.check_timeout(s);
         }));

@@ -462,7 +462,9 @@ int GnnNet::init(int hidden, int layers, uint64_t seed, hipStream_t s) {
     o += GNN_OUT;
     nParams = o;
     // Glorot-uniform weights, zero biases; the head starts at a tenth of that
-    // scale, so the initial blocks are close to the diagonal-block inverses
+    // scale with Y = -I/2 from its bias, so the initial blocks are close to
+    // half the diagonal-block inverses: damped block Jacobi, which does not
+    // amplify the stiff modes that the undamped form does for 3-D elasticity
     std::vector<float> p(nParams, 0.0f);
     uint64_t st = seed;
     auto fillW = [&](size_t off, int fanOut, int fanIn, double gain) {
@@ -478,6 +480,7 @@ int GnnNet::init(int hidden, int layers, uint64_t seed, hipStream_t s) {
         fillW(offWn[l], H, H, 1.0);
     }
     fillW(offWout, GNN_OUT, H, 0.1);
+    for (int d = 0; d < 3; ++d) p[offBout + d] = -0.5f;
     ASTRO_TRY(params.upload(p, s));
     ASTRO_TRY(grad.alloc(nParams));
     ASTRO_TRY(m.alloc(nParams));

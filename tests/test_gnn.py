@@ -8,7 +8,7 @@ Newmark steps on a field it was not trained on, with the same displacements
 import numpy as np
 import pytest
 
-from helpers import csr_sparse, rel_norm
+from helpers import csr_sparse, refined_solve, rel_norm
 from misc_amd import astro as A
 from misc_amd.meshes import half_sine, impulse_block
 from oracle import gnn_oracle as GO
@@ -60,13 +60,18 @@ def test_features_match_oracle():
     ctx.close()
 
 
-def test_loss_gradient_and_blocks_match_oracle():
+@pytest.mark.parametrize("pairs", ["solution", "solve"])
+def test_loss_gradient_and_blocks_match_oracle(pairs):
     gnn = A.Gnn(64, 3, seed=0)
     m, ctx = dyn_ctx(2, (6, 5, 3))
-    print("gnn collect 6x5x3", flush=True)
-    ctx.gnn_collect(gnn, 6, seed=1, rtol=1e-10)
+    print("gnn collect 6x5x3", pairs, flush=True)
+    ctx.gnn_collect(gnn, 6, seed=1, rtol=1e-12, pairs=pairs)
     fo = oracle_field(gnn.field(0), ctx)
     ctx.close()
+    # the pairs' error statistic: sum e1^T r1 with e1 = A^-1 r1 (exact for
+    # solution pairs, to the GMRES tolerance for solved ones)
+    eTr = sum(r @ refined_solve(fo["A"], r) for r in fo["R1"])
+    assert abs(fo["eTr"] - eTr) <= 1e-8 * eTr
     rng = np.random.default_rng(3)
     p0 = gnn.get_params()
     for p in (p0, (p0 + 0.05 * rng.standard_normal(p0.size)).astype(np.float32)):
