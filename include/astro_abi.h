@@ -242,6 +242,25 @@ int astro_ctx_dynamic_run(astro_ctx* ctx, int nSteps, const double* loadScale);
 int astro_ctx_set_design(astro_ctx* ctx, int nDV, const int32_t* elDV, const double* elCoef, int dvLayer, double h);
 int astro_ctx_dRdD(astro_ctx* ctx, double* dRdD);
 int astro_ctx_compliance_gradient(astro_ctx* ctx, double* grad);
+/* The same gradient by direct complex-step solves, all design variables in
+ * one persistent launch (replaces the per-variable loop of getdRelasticdD +
+ * getTotaldLdD, Working/AStrO_commandFunctions.f90:1172-1466): variable k's
+ * system (K + C^T C)(d + i h e_k) u_k = F is assembled, factored as a
+ * complex-symmetric LDL^T and solved in complex128, grad[k] =
+ * Im(F . u_k) / h and compliance[k] (nDV, may be null) = Re(F . u_k).  The
+ * instances use a node-major DOF order of their own and start their complex
+ * arithmetic at the first row their variable touches.  Needs
+ * astro_ctx_set_design; assembles the current design itself.  Memory:
+ * 16 bytes x profile entries x nDV.  Scalar "csBatchMs": the launch's time. */
+int astro_ctx_cs_batch_gradient(astro_ctx* ctx, double* grad, double* compliance);
+/* The batched kernel's autotuner: every (row block, column chunk) tile that
+ * fits in LDS is run twice on the context's batch and timed on the second
+ * run; the fastest is kept for later astro_ctx_cs_batch_gradient calls.
+ * tiles[t] = RB * 100 + CB and ms[t] (ms < 0: does not fit) for t < count. */
+int astro_ctx_cs_batch_autotune(astro_ctx* ctx, int32_t* tiles, float* ms, int32_t count);
+/* Selects the tile RB * 100 + CB for later calls (ASTRO_ERR_ARG if it is not
+ * one of the kernel's tiles). */
+int astro_ctx_cs_batch_tile(astro_ctx* ctx, int32_t tile);
 
 /* Thermal twins (Working/AStrO_commandFunctions.f90:16-74, 229-258, 689-703,
  * 763-769). */

@@ -35,7 +35,9 @@ EXPORTS = [
     "astro_prep_scalar", "astro_prep_get_i32", "astro_prep_get_f64", "astro_ctx_set_timing",
     "astro_ctx_kernel_stat", "astro_ctx_reset", "astro_ctx_true_residual", "astro_ctx_dynamic_init",
     "astro_ctx_dynamic_step", "astro_ctx_dynamic_run", "astro_ctx_set_design", "astro_ctx_dRdD",
-    "astro_ctx_compliance_gradient", "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load",
+    "astro_ctx_compliance_gradient", "astro_ctx_cs_batch_gradient", "astro_ctx_cs_batch_autotune",
+    "astro_ctx_cs_batch_tile",
+    "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load",
     "astro_ctx_true_residual_noalloc", "astro_gnn_create", "astro_gnn_destroy", "astro_gnn_num_params",
     "astro_gnn_num_fields", "astro_gnn_get_params", "astro_gnn_set_params", "astro_ctx_gnn_collect",
     "astro_gnn_train", "astro_gnn_loss_grad", "astro_gnn_field_size", "astro_gnn_get_field",
@@ -109,6 +111,9 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_set_design.argtypes = [C.c_void_p, C.c_int, _ip, C.c_void_p, C.c_int, C.c_double]
     L.astro_ctx_dRdD.argtypes = [C.c_void_p, _dp]
     L.astro_ctx_compliance_gradient.argtypes = [C.c_void_p, _dp]
+    L.astro_ctx_cs_batch_gradient.argtypes = [C.c_void_p, _dp, C.c_void_p]
+    L.astro_ctx_cs_batch_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+    L.astro_ctx_cs_batch_tile.argtypes = [C.c_void_p, C.c_int32]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_ctx_operator_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_prep_create.argtypes = [C.POINTER(_Model), C.POINTER(_Options), C.POINTER(C.c_void_p)]
@@ -468,6 +473,26 @@ class Context:
         g = np.zeros(self.n_dv)
         _check(load_library().astro_ctx_compliance_gradient(self.h, g))
         return g
+
+    def cs_batch_gradient(self):
+        """The compliance gradient by direct complex-step solves of all design
+        variables in one launch; returns (gradient, per-instance compliance)."""
+        g = np.zeros(self.n_dv)
+        cm = np.zeros(self.n_dv)
+        _check(load_library().astro_ctx_cs_batch_gradient(self.h, g, cm.ctypes.data))
+        return g, cm
+
+    def cs_batch_autotune(self):
+        """Times every tile shape of the batched kernel; keeps the fastest.
+        Returns {RB * 100 + CB: ms} (ms < 0: does not fit)."""
+        t = np.zeros(8, np.int32)
+        ms = np.zeros(8, np.float32)
+        _check(load_library().astro_ctx_cs_batch_autotune(self.h, t.ctypes.data, ms.ctypes.data, 8))
+        return {int(a): float(b) for a, b in zip(t, ms) if a > 0}
+
+    def cs_batch_tile(self, tile: int):
+        """Selects the batched kernel's tile RB * 100 + CB."""
+        _check(load_library().astro_ctx_cs_batch_tile(self.h, int(tile)))
 
     def assemble_thermal(self):
         _check(load_library().astro_ctx_assemble_thermal(self.h))

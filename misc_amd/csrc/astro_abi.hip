@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/astro_abi.h"
+#include "astro_batch.h"
 #include "astro_gnn.h"
 #include "astro_internal.h"
 #include "astro_kernels.h"
@@ -785,6 +786,22 @@ struct astro_ctx {
     DBuf<int> elDV;
     DBuf<double> elABDim, dRdD, gPart;
     double csH = 1e-30;
+    // batched complex-step solves (astro_ctx_cs_batch_gradient): node-major
+    // profile, instances, and their device arrays
+    struct CsBatch {
+        bool ready = false;
+        int nInst = 0, maxRow = 0, tile = 0;
+        int64_t total = 0;
+        std::vector<int> inv, first, instStart;
+        std::vector<int64_t> rowoff;
+        DBuf<int> dInv, dDofOrder, dFirst, dOrder, dInstStart, dElStart, dElList, status;
+        DBuf<int64_t> dRowoff;
+        DBuf<double> profRe, F, work, vec, sol, ed, grad, comp;
+        float lastMs = 0.0f, tileMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double flops = 0.0;  // flop model of one launch: sum over instances and rows of len^2 (x4 complex)
+    } csb;
+    DBuf<double> csbKim;
+    DBuf<int> csbKimDof;
     DBuf<int> conn, elMat, rank, perm;
     DBuf<uint8_t> elPos, thPos;
     std::vector<int> colorStart;
@@ -1179,6 +1196,10 @@ double astro_ctx_scalar_f(const astro_ctx* c, const char* name) {
     if (s == "gmresRes0") return c->gm.res0;
     if (s == "thermGmresRes") return c->tgm.res;
     if (s == "dynMs") return c->dynMs;
+    if (s == "csBatchMs") return c->csb.lastMs;
+    if (s == "csBatchFlops") return c->csb.flops;
+    if (s == "csBatchEntries") return (double)c->csb.total;
+    if (s == "csBatchMaxRow") return c->csb.maxRow;
     return NAN;
 }
 
@@ -1781,6 +1802,7 @@ int astro_ctx_set_design(astro_ctx* c, int nDV, const int32_t* elDV, const doubl
     }
     c->nDV = nDV;
     c->csH = h;
+    c->csb.ready = false;
     ASTRO_TRY(c->elDV.upload(c->hElDV, c->s));
     ASTRO_TRY(c->elABDim.upload(im, c->s));
     ASTRO_TRY(c->dRdD.alloc((size_t)c->prep.elMatDim * nDV));
@@ -1800,7 +1822,8 @@ static int ctx_sensitivities(astro_ctx* c) {
         if (cnt <= 0) continue;
         hipLaunchKernelGGL(k_shell4_dtangent_cs, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
                            c->conn.p, c->X.p, c->elABD.p, c->elABDim.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
-                           c->prep.numSectionNds, c->nodeDisp.p, c->elDV.p, 1.0 / c->csH, n, c->dRdD.p, c->gPart.p);
+                           c->prep.numSectionNds, c->nodeDisp.p, c->elDV.p, 1.0 / c->csH, n, c->dRdD.p, c->gPart.p,
+                           (double*)nullptr, (int*)nullptr);
     }
     HIP_TRY(hipGetLastError());
     return ASTRO_OK;
@@ -1830,6 +1853,283 @@ int astro_ctx_compliance_gradient(astro_ctx* c, double* grad) {
     for (size_t e = 0; e < g.size(); ++e)
         if (c->hElDV[e] >= 0) grad[c->hElDV[e]] += g[e];
     return ASTRO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched complex-step sensitivities: every design variable's perturbed
+// system as a complex128 instance (the kernel and its algorithm are in
+// astro_batch.hip).  Set-up, once per design: the node-major DOF order
+// (nodes by level rank, the DOFs of a node together) and its profile over
+// the CSR and penalty patterns, the loads in that order, the element
+// stage's DOF lists, and per variable its elements and first row r_k.
+static int csb_element_stage(astro_ctx* c) {
+    const int64_t n = c->prep.elMatDim;
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_shell4_dtangent_cs, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
+                           c->conn.p, c->X.p, c->elABD.p, c->elABDim.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
+                           c->prep.numSectionNds, c->nodeDisp.p, c->elDV.p, 1.0 / c->csH, n, c->dRdD.p, c->gPart.p,
+                           c->csbKim.p, c->csbKimDof.p);
+    }
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+static int csb_setup(astro_ctx* c) {
+    auto& B = c->csb;
+    if (B.ready) return ASTRO_OK;
+    const Prep& p = c->prep;
+    const int n = p.elMatDim, N = p.numNodes, E = p.numEls, nDV = c->nDV;
+    std::vector<int> nodes(N);
+    for (int i = 0; i < N; ++i) nodes[i] = i;
+    std::stable_sort(nodes.begin(), nodes.end(),
+                     [&](int a, int b) { return p.currentRank[a] < p.currentRank[b]; });
+    std::vector<int> order;
+    order.reserve(n);
+    for (int nd : nodes)
+        for (int q = 0; q < 6; ++q) {
+            const int d = p.nDofIndex[6 * (size_t)nd + q];
+            if (d > 0) order.push_back(d - 1);
+        }
+    if ((int)order.size() != n) return fail(ASTRO_ERR_ARG, "DOF map does not cover the system");
+    B.inv.assign(n, -1);
+    for (int k = 0; k < n; ++k) {
+        if (order[k] < 0 || order[k] >= n || B.inv[order[k]] >= 0) return fail(ASTRO_ERR_ARG, "DOF map is not a permutation");
+        B.inv[order[k]] = k;
+    }
+    B.first.resize(n);
+    for (int i = 0; i < n; ++i) B.first[i] = i;
+    auto touch = [&](int a, int b) {
+        const int I = std::max(a, b), J = std::min(a, b);
+        B.first[I] = std::min(B.first[I], J);
+    };
+    for (int r = 0; r < n; ++r)
+        for (int q = p.elMatRange[r]; q < p.elMatRange[r + 1]; ++q) {
+            const int col = p.elMatCols[q];
+            if (col >= 1 && col <= n) touch(B.inv[r], B.inv[col - 1]);
+        }
+    for (int r = 0; r < p.elMPCDim; ++r)
+        for (int a = p.elMPCMatRange[r]; a < p.elMPCMatRange[r + 1]; ++a)
+            for (int b = p.elMPCMatRange[r]; b < p.elMPCMatRange[r + 1]; ++b) {
+                const int ca = p.elMPCMatCols[a], cb = p.elMPCMatCols[b];
+                if (ca >= 1 && ca <= n && cb >= 1 && cb <= n) touch(B.inv[ca - 1], B.inv[cb - 1]);
+            }
+    B.rowoff.assign(n + 1, 0);
+    B.maxRow = 0;
+    for (int i = 0; i < n; ++i) {
+        const int len = i - B.first[i] + 1;
+        B.maxRow = std::max(B.maxRow, len);
+        B.rowoff[i + 1] = B.rowoff[i] + len;
+    }
+    B.total = B.rowoff[n];
+    const size_t ldsMax = 160 * 1024;
+    if (csb_lds_bytes(0, B.maxRow) > ldsMax)
+        return fail(ASTRO_ERR_UNSUPPORTED, "node-major profile rows of " + std::to_string(B.maxRow) +
+                                               " entries exceed the batched kernel's LDS tiles");
+    // loads in the node-major order
+    std::vector<int> ld(c->loadDofs.n);
+    std::vector<double> lv(c->loadVals.n), F(n, 0.0);
+    if (!ld.empty()) {
+        HIP_TRY(hipMemcpyAsync(ld.data(), c->loadDofs.p, sizeof(int) * ld.size(), hipMemcpyDeviceToHost, c->s));
+        HIP_TRY(hipMemcpyAsync(lv.data(), c->loadVals.p, sizeof(double) * lv.size(), hipMemcpyDeviceToHost, c->s));
+        HIP_TRY(hipStreamSynchronize(c->s));
+    }
+    for (size_t k = 0; k < ld.size(); ++k) F[B.inv[ld[k] - 1]] += lv[k];
+    // element DOF lists from the element stage
+    ASTRO_TRY(c->csbKim.alloc(576 * (size_t)E));
+    ASTRO_TRY(c->csbKimDof.alloc(24 * (size_t)E));
+    std::vector<int> kd(24 * (size_t)E, 0);
+    HIP_TRY(hipMemcpyAsync(c->csbKimDof.p, kd.data(), sizeof(int) * kd.size(), hipMemcpyHostToDevice, c->s));
+    ASTRO_TRY(csb_element_stage(c));
+    HIP_TRY(hipMemcpyAsync(kd.data(), c->csbKimDof.p, sizeof(int) * kd.size(), hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    std::vector<int> elStart(nDV + 1, 0), elList, instStart(nDV, n);
+    for (int e = 0; e < E; ++e)
+        if (c->hElDV[e] >= 0) ++elStart[c->hElDV[e] + 1];
+    for (int k = 0; k < nDV; ++k) elStart[k + 1] += elStart[k];
+    elList.resize(elStart[nDV]);
+    {
+        std::vector<int> fill(elStart.begin(), elStart.end() - 1);
+        for (int e = 0; e < E; ++e) {
+            const int k = c->hElDV[e];
+            if (k < 0) continue;
+            elList[fill[k]++] = e;
+            for (int a = 0; a < 24; ++a) {
+                const int d = kd[24 * (size_t)e + a];
+                if (d < 1 || d > n) return fail(ASTRO_ERR_ARG, "element DOF out of range in the batched set-up");
+                instStart[k] = std::min(instStart[k], B.inv[d - 1]);
+            }
+        }
+    }
+    std::vector<int> instOrder(nDV);
+    for (int k = 0; k < nDV; ++k) instOrder[k] = k;
+    std::stable_sort(instOrder.begin(), instOrder.end(), [&](int a, int b) { return instStart[a] < instStart[b]; });
+    B.instStart = instStart;
+    B.nInst = nDV;
+    B.flops = 0.0;
+    for (int k = 0; k < nDV; ++k)
+        for (int i = 0; i < n; ++i) {
+            const double len = (double)(i - B.first[i]);
+            B.flops += (i >= instStart[k] ? 4.0 : 1.0) * len * len;
+        }
+    ASTRO_TRY(B.dInv.upload(B.inv, c->s));
+    ASTRO_TRY(B.dDofOrder.upload(order, c->s));
+    ASTRO_TRY(B.dFirst.upload(B.first, c->s));
+    ASTRO_TRY(B.dRowoff.upload(B.rowoff, c->s));
+    ASTRO_TRY(B.dOrder.upload(instOrder, c->s));
+    ASTRO_TRY(B.dInstStart.upload(instStart, c->s));
+    ASTRO_TRY(B.dElStart.upload(elStart, c->s));
+    ASTRO_TRY(B.dElList.upload(elList, c->s));
+    ASTRO_TRY(B.F.upload(F, c->s));
+    ASTRO_TRY(B.profRe.alloc((size_t)B.total));
+    ASTRO_TRY(B.work.alloc(2 * (size_t)B.total * nDV));
+    ASTRO_TRY(B.vec.alloc(2 * (size_t)n * nDV));
+    ASTRO_TRY(B.sol.alloc(2 * (size_t)n * nDV));
+    ASTRO_TRY(B.ed.alloc(4 * (size_t)n * nDV));
+    ASTRO_TRY(B.grad.alloc(nDV));
+    ASTRO_TRY(B.comp.alloc(nDV));
+    ASTRO_TRY(B.status.alloc(1));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    B.ready = true;
+    return ASTRO_OK;
+}
+
+// One batched evaluation with tile t: the real assembly and penalty scaling
+// of the current design, the element stage, A_re in the node-major profile,
+// then the instances' launch and kCsbRefine refinement steps (residual
+// launches and the kernel in solve-only mode), timed with events on the
+// context stream.  The direct solve leaves an error of about cond eps (1e-9
+// on the 128 x 16 plate); a double-double residual step takes it to the
+// rounding of the system's own entries.
+static constexpr int kCsbRefine = 1;
+static int csb_run(astro_ctx* c, int t) {
+    auto& B = c->csb;
+    const Prep& p = c->prep;
+    if (csb_lds_bytes(t, B.maxRow) > 160 * 1024) return fail(ASTRO_ERR_ARG, "tile does not fit in LDS");
+    ASTRO_TRY(astro_ctx_assemble_elastic(c, 0.0));
+    ASTRO_TRY(astro_ctx_scale_mpc(c));
+    ASTRO_TRY(csb_element_stage(c));
+    HIP_TRY(csb_real_profile(c->csrRange.p, c->csrCols.p, c->csrVal.p, p.elMatDim, c->mpcMat.p, c->mpcCols.p,
+                             c->mpcRange.p, p.elMPCDim, B.dInv.p, B.dFirst.p, B.dRowoff.p, B.profRe.p, B.total, c->s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->s, B.status.p, (int64_t)1, 0);
+    CsbArgs a;
+    a.n = p.elMatDim;
+    a.W = csb_stride(t, B.maxRow);
+    a.maxRow = B.maxRow;
+    a.total = B.total;
+    a.first = B.dFirst.p;
+    a.rowoff = B.dRowoff.p;
+    a.inv = B.dInv.p;
+    a.profRe = B.profRe.p;
+    a.F = B.F.p;
+    a.order = B.dOrder.p;
+    a.instStart = B.dInstStart.p;
+    a.elStart = B.dElStart.p;
+    a.elList = B.dElList.p;
+    a.kim = c->csbKim.p;
+    a.kimDof = c->csbKimDof.p;
+    a.work = B.work.p;
+    a.vec = B.vec.p;
+    a.sol = B.sol.p;
+    a.invH = 1.0 / c->csH;
+    a.grad = B.grad.p;
+    a.compliance = B.comp.p;
+    a.status = B.status.p;
+    CsbRes R;
+    R.n = p.elMatDim;
+    R.order = B.dDofOrder.p;
+    R.inv = B.dInv.p;
+    R.range = c->csrRange.p;
+    R.cols = c->csrCols.p;
+    R.val = c->csrVal.p;
+    R.cmat = c->mpcMat.p;
+    R.ccols = c->mpcCols.p;
+    R.crange = c->mpcRange.p;
+    R.ctRange = p.elMPCDim > 0 ? c->mpcT.range.p : nullptr;
+    R.ctPos = c->mpcT.pos.p;
+    R.ctRow = c->mpcT.row.p;
+    R.F = B.F.p;
+    R.elStart = B.dElStart.p;
+    R.elList = B.dElList.p;
+    R.kim = c->csbKim.p;
+    R.kimDof = c->csbKimDof.p;
+    R.sol = B.sol.p;
+    R.ed = B.ed.p;
+    R.vec = B.vec.p;
+    HIP_TRY(hipEventRecord(c->ev[6], c->s));
+    a.mode = 0;
+    HIP_TRY(csb_launch(a, t, B.nInst, c->s));
+    for (int it = 0; it < kCsbRefine; ++it) {
+        HIP_TRY(csb_residual(R, B.nInst, c->s));
+        a.mode = 1;
+        HIP_TRY(csb_launch(a, t, B.nInst, c->s));
+    }
+    HIP_TRY(hipEventRecord(c->ev[7], c->s));
+    HIP_TRY(hipEventSynchronize(c->ev[7]));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+    B.lastMs = ms;
+    int st = 0;
+    HIP_TRY(hipMemcpy(&st, B.status.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) return fail(ASTRO_ERR_ZERO_PIVOT, "zero pivot in a batched complex factor");
+    return ASTRO_OK;
+}
+
+// Compliance gradient by direct complex-step solves, all design variables in
+// one launch: grad[k] = Im(F . u(d + i h e_k)) / h; compliance (may be null)
+// receives Re(F . u_k), each instance's compliance.
+int astro_ctx_cs_batch_gradient(astro_ctx* c, double* grad, double* compliance) {
+    if (!c || !grad) return fail(ASTRO_ERR_ARG, "null argument");
+    if (c->nDV < 1) return fail(ASTRO_ERR_ARG, "astro_ctx_set_design has not been called");
+    ASTRO_TRY(csb_setup(c));
+    ASTRO_TRY(csb_run(c, c->csb.tile));
+    HIP_TRY(hipMemcpy(grad, c->csb.grad.p, sizeof(double) * c->nDV, hipMemcpyDeviceToHost));
+    if (compliance) HIP_TRY(hipMemcpy(compliance, c->csb.comp.p, sizeof(double) * c->nDV, hipMemcpyDeviceToHost));
+    return ASTRO_OK;
+}
+
+// The autotuner: times every tile shape that fits in LDS on the context's
+// own batch (after one untimed run of each) and keeps the fastest for later
+// calls.  tiles / ms (capacity count): RB * 100 + CB and the kernel time
+// of each tile (ms < 0: does not fit).
+int astro_ctx_cs_batch_autotune(astro_ctx* c, int32_t* tiles, float* ms, int32_t count) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (c->nDV < 1) return fail(ASTRO_ERR_ARG, "astro_ctx_set_design has not been called");
+    ASTRO_TRY(csb_setup(c));
+    auto& B = c->csb;
+    const int nt = csb_num_tiles();
+    int best = -1;
+    for (int t = 0; t < nt; ++t) {
+        B.tileMs[t] = -1.0f;
+        if (csb_lds_bytes(t, B.maxRow) > 160 * 1024) continue;
+        ASTRO_TRY(csb_run(c, t));
+        ASTRO_TRY(csb_run(c, t));
+        B.tileMs[t] = B.lastMs;
+        if (best < 0 || B.tileMs[t] < B.tileMs[best]) best = t;
+    }
+    if (best < 0) return fail(ASTRO_ERR_UNSUPPORTED, "no tile shape fits");
+    B.tile = best;
+    for (int t = 0; t < nt && t < count; ++t) {
+        int rb = 0, cb = 0;
+        csb_tile(t, &rb, &cb);
+        if (tiles) tiles[t] = rb * 100 + cb;
+        if (ms) ms[t] = B.tileMs[t];
+    }
+    return ASTRO_OK;
+}
+
+int astro_ctx_cs_batch_tile(astro_ctx* c, int32_t tile) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    for (int t = 0; t < csb_num_tiles(); ++t) {
+        int rb = 0, cb = 0;
+        csb_tile(t, &rb, &cb);
+        if (rb * 100 + cb == tile) {
+            c->csb.tile = t;
+            return ASTRO_OK;
+        }
+    }
+    return fail(ASTRO_ERR_ARG, "unknown tile " + std::to_string(tile));
 }
 
 double astro_ctx_phase_ms(const astro_ctx* c, const char* phase) {

@@ -60,7 +60,7 @@ __global__ void k_shell4_tangent(const int* perm, int count, const int* conn, co
 __global__ void k_shell4_dtangent_cs(const int* perm, int count, const int* conn, const double* X, const double* elABD,
                                      const double* elABDim, const double* elAlpha, const int* rank, int nSolid, int nSec,
                                      const double* u, const int* elDV, double invH, int64_t n, double* dRdD,
-                                     double* gPart);
+                                     double* gPart, double* kim, int* kimDof);
 __global__ void k_brick8_residual(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                   const double* elScale, const double* matC, const double* elRho, const int* rank,
                                   int nS, const double* u, const double* pu, const double* pv, const double* pa,

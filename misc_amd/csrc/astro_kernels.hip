@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "astro_dd.h"
 #include "astro_kernels.h"
 
 namespace astro {
@@ -358,6 +359,8 @@ __global__ __launch_bounds__(64) void k_shell4_tangent(
 // the host), in complex arithmetic through the condensation, and contributes
 // Im(Kc u_e) / h = coef dKc/dt u_e to column elDV[e] of dRdD (n x nDV,
 // column-major) and -u_e . that to gPart[e] (the adjoint compliance term).
+// kim (may be null): Im(Kc) of the element, 24 x 24 row-major at 576 e, and
+// its DOFs (1-based) at kimDof[24 e], for the batched complex solves.
 // Elements of one launch share no node; elements of different design
 // variables write different columns.  Working/AStrO_commandFunctions.f90:
 // 1172-1375 (getdRelasticdD) and the AStrO_c_elementEqns twin of the shell
@@ -376,7 +379,8 @@ __global__ __launch_bounds__(64) void k_shell4_dtangent_cs(
     const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
     const double* __restrict__ elABD, const double* __restrict__ elABDim, const double* __restrict__ elAlpha,
     const int* __restrict__ rank, int nSolid, int nSec, const double* __restrict__ u, const int* __restrict__ elDV,
-    double invH, int64_t n, double* __restrict__ dRdD, double* __restrict__ gPart) {
+    double invH, int64_t n, double* __restrict__ dRdD, double* __restrict__ gPart, double* __restrict__ kim,
+    int* __restrict__ kimDof) {
     const int idx = blockIdx.x;
     if (idx >= count) return;
     const int e = perm[idx];
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(64) void k_shell4_dtangent_cs(
     __shared__ double sB[4][9][32], sDBr[4][9][32], sDBi[4][9][32];
     __shared__ cdbl sK[32][33];
     __shared__ cdbl sXs[24][8];
-    __shared__ double sU[24], sV[24];
+    __shared__ double sU[24], sVh[24], sVl[24], sIm[24][24];
     __shared__ int sNode[4], sDof[24];
     for (int k = lane; k < 81; k += 64) {
         sDr[k % 9][k / 9] = elABD[81 * (size_t)e + k];
@@ -451,25 +455,35 @@ __global__ __launch_bounds__(64) void k_shell4_dtangent_cs(
     }
     __syncthreads();
     const int dv = elDV[e];
-    if (lane < 24) {
-        // Im(Kc) u with Kc = Kcc - Kci X
-        const int i = lane;
-        double v = 0.0;
-        for (int j = 0; j < 24; ++j) {
-            double im = sK[i][j].i;
+    // Im(Kc) with Kc = Kcc - Kci X, once per entry
+    for (int k = lane; k < 576; k += 64) {
+        const int i = k / 24, j = k - (k / 24) * 24;
+        double im = sK[i][j].i;
 #pragma unroll
-            for (int m = 0; m < 8; ++m) im -= cmul(sK[i][24 + m], sXs[j][m]).i;
-            v += im * sU[j];
-        }
-        v *= invH;
-        sV[i] = v;
-        if (dv >= 0) dRdD[(size_t)dv * n + sDof[i] - 1] += v;
+        for (int m = 0; m < 8; ++m) im -= cmul(sK[i][24 + m], sXs[j][m]).i;
+        sIm[i][j] = im;
+        if (kim && dv >= 0) kim[576 * (size_t)e + k] = im;
+    }
+    if (kim && dv >= 0 && lane < 24) kimDof[24 * (size_t)e + lane] = sDof[lane];
+    __syncthreads();
+    // Im(Kc) u and u . Im(Kc) u in double-double: the displacement's
+    // rigid-body part cancels in both, terms much larger than the result
+    if (lane < 24) {
+        const int i = lane;
+        double vh = 0.0, vl = 0.0;
+        for (int j = 0; j < 24; ++j) dd_add_prod(vh, vl, sIm[i][j], sU[j]);
+        sVh[i] = vh * invH;
+        sVl[i] = vl * invH;
+        if (dv >= 0) dRdD[(size_t)dv * n + sDof[i] - 1] += (vh + vl) * invH;
     }
     __syncthreads();
     if (lane == 0) {
-        double g = 0.0;
-        for (int i = 0; i < 24; ++i) g -= sU[i] * sV[i];
-        gPart[e] = dv >= 0 ? g : 0.0;
+        double gh = 0.0, gl = 0.0;
+        for (int i = 0; i < 24; ++i) {
+            dd_add_prod(gh, gl, -sU[i], sVh[i]);
+            dd_add_prod(gh, gl, -sU[i], sVl[i]);
+        }
+        gPart[e] = dv >= 0 ? gh + gl : 0.0;
     }
 }
 
@@ -2260,31 +2274,6 @@ __global__ __launch_bounds__(256) void k_spmv(const double* __restrict__ val, co
     }
     for (int off = 16; off > 0; off >>= 1) s += __shfl_xor(s, off, 32);
     if (l == 0) y[row] = s;
-}
-
-// Double-double arithmetic (value = hi + lo, |lo| <= ulp(hi) / 2) for true
-// residuals: TwoSum (Knuth) and TwoProduct through fma give the rounding
-// error of each sum and product exactly.  Both need every product and sum
-// rounded on its own: contraction (which would fuse a*b into the sum that
-// follows, s = fma(a, b, hi)) is switched off in these helpers.
-__device__ __forceinline__ void dd_two_sum(double a, double b, double& s, double& e) {
-#pragma clang fp contract(off)
-    s = a + b;
-    const double bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-__device__ __forceinline__ void dd_add(double& hi, double& lo, double bh, double bl) {
-#pragma clang fp contract(off)
-    double s, e;
-    dd_two_sum(hi, bh, s, e);
-    e += lo + bl;
-    hi = s + e;
-    lo = e - (hi - s);
-}
-__device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double a, double b) {
-#pragma clang fp contract(off)
-    const double p = a * b;
-    dd_add(hi, lo, p, __builtin_fma(a, b, -p));
 }
 
 // t[r] = (C x)_r as a double-double pair (t[2r], t[2r + 1]).
