@@ -27,6 +27,9 @@ program astro_ref_driver
     use AStrO_bookKeeping
     use AStrO_commandFunctions
     use AStrO_solvers
+#ifdef ASTRO_HIP
+    use AStrO_gpuElastic
+#endif
     implicit none
 
     character(len=128) :: modelFile
@@ -139,8 +142,20 @@ program astro_ref_driver
     call loadInitialState()
 
     if (solveElastic .eq. 1) then
+#ifdef ASTRO_HIP
+        ! drop-in build: the reference's own assembly on the same arrays is
+        ! dumped for comparison, then the GPU assembly replaces it
+        call getElasticSolnLoad(1)
+        call dumpd(elasticMat, elMatSize, 'elasticMat_host')
+        call dumpi(elMatCols, elMatSize, 'elMatCols_host')
+        elasticMat(:) = r_0
+        elMatCols(:) = 0
+        call system_clock(ticks0)
+        call gpuGetElasticSolnLoad(1)
+#else
         call system_clock(ticks0)
         call getElasticSolnLoad(1)
+#endif
         call system_clock(ticks1)
         tAsm = dble(ticks1 - ticks0)/dble(rate)
         call dumpd(elasticMat, elMatSize, 'elasticMat')
@@ -170,6 +185,17 @@ program astro_ref_driver
         tSolve = dble(ticks1 - ticks0)/dble(rate)
         call dumpd(elasticLoad, elMatDim, 'elasticLoad')
         call dumpd(nodeDisp, elMatDim, 'nodeDisp')
+#ifdef ASTRO_HIP
+        ! the solution load at the converged displacement (the reactionForce
+        ! output's call, Working/AStrO_runJob.f90:228-233), host and GPU
+        elasticLoad(:) = r_0
+        call getElasticSolnLoad(0)
+        call dumpd(elasticLoad, elMatDim, 'solnLoad_host')
+        elasticLoad(:) = r_0
+        call gpuGetElasticSolnLoad(0)
+        call dumpd(elasticLoad, elMatDim, 'solnLoad')
+        call gpuElasticRelease()
+#endif
     else
         call system_clock(ticks0)
         call getThermalSolnLoad(1)
@@ -233,7 +259,11 @@ contains
         if (itArg .gt. 0) maxIt = itArg
         elasticLoad(:) = r_0
         intElasticLoad(:) = r_0
+#ifdef ASTRO_HIP
+        call gpuGetElasticSolnLoad(nLGeom)
+#else
         call getElasticSolnLoad(nLGeom)
+#endif
         call nodalLoads()
         call getElasticConstraintLoad()
         call gMRes(delDisp,elasticMat,elMatCols,elMatRange,elMatSize,elMatDim,elMPCMat,elMPCMatCols, &

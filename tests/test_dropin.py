@@ -1,15 +1,17 @@
 """Drop-in: the reference's own Fortran host (readModelInput, analysisPrep,
-getElasticSolnLoad, scaleElasticMPC, convertToLTri) with its solver module
-replaced by misc_amd/fortran/astro_solvers_hip.f90, which calls
-libastro_hip.so through ISO_C_BINDING.  The factor and the solution it
-produces are compared with the oracle."""
+scaleElasticMPC, convertToLTri) with its solver module replaced by
+misc_amd/fortran/astro_solvers_hip.f90 and its element assembly
+(getElasticSolnLoad) by misc_amd/fortran/astro_elastic_hip.f90, both calling
+libastro_hip.so through ISO_C_BINDING.  In the same process the host's own
+getElasticSolnLoad runs on the same arrays for comparison; the factor and the
+solution are compared with the oracle."""
 
 import os
 
 import numpy as np
 import pytest
 
-from helpers import entry_rel, golden, rel_norm
+from helpers import csr_sparse, entry_rel, golden, rel_norm
 from misc_amd.meshes import cantilever_block
 from oracle import oracle as O
 
@@ -24,6 +26,20 @@ def test_reference_host_with_gpu_solvers(dims):
     # the relative stop, so that the result is the converged solution
     r = O.run_reference(m, "elastic", driver=O.DROPIN_DRIVER, env={"ASTRO_GMRES_RTOL": "1e-12"})
     om = O.OracleModel(m).prep().static_elastic(stop_after="profile")
+    # GPU assembly against the host's own routine on the same arrays
+    np.testing.assert_array_equal(r["elMatCols"], r["elMatCols_host"])
+    Kh = r["elasticMat_host"]
+    e_gpu = entry_rel(r["elasticMat"], Kh, floor=1e-3 * np.abs(Kh).max())
+    # the solution load at the converged u: -K u with the host's own matrix
+    # (the host's getElasticSolnLoad(0) gathers element displacements by
+    # local node number, so its own dump is not that product)
+    n = len(r["nodeDisp"])
+    K = csr_sparse(Kh, r["elMatCols_host"], r["elMatRange"], n)
+    Rk = -(K @ r["nodeDisp"])
+    e_load = np.max(np.abs(r["solnLoad"] - Rk)) / np.max(np.abs(Rk))
+    print(f"GPU vs host assembly: csr {e_gpu:.3e}, solution load vs -K u {e_load:.3e} "
+          f"(host routine's own: max {np.abs(r['solnLoad_host']).max():.3e} vs {np.abs(Rk).max():.3e})", flush=True)
+    assert e_gpu < 1e-12 and e_load < 1e-12
     np.testing.assert_array_equal(r["elMatCols"], om.geti("elMatCols"))
     Kr = om.getd("elasticMat")
     e_csr = entry_rel(r["elasticMat"], Kr, floor=1e-3 * np.abs(Kr).max())
