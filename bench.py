@@ -42,22 +42,48 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 
 def factor_flops(lt_range):
     """Exact LDL^T flop count of the profile: sum over rows i and stored
-    columns j < i of 2 (j - max(s_i, s_j)).  When the first stored column s
-    never decreases down the rows (true for the level-sorted meshes here) the
-    inner sum is L (L - 1) / 2 with L = i - s_i; returns None otherwise."""
+    columns j < i of 2 (j - max(s_i, s_j)), s the first stored column.
+    When s never decreases down the rows (true for the level-sorted meshes
+    under the 64-bit rule) the inner sum is L (L - 1) / 2 with L = i - s_i;
+    otherwise the pairs are summed directly (profiles up to ~1e8 entries)."""
     import numpy as np
     R = np.asarray(lt_range, dtype=np.int64)
     rowlen = np.diff(R)
     s = np.arange(len(rowlen)) - rowlen + 1
-    if len(s) > 1 and np.any(np.diff(s) < 0):
+    if len(s) <= 1 or not np.any(np.diff(s) < 0):
+        L = (rowlen - 1).astype(np.float64)
+        return float(np.sum(L * (L - 1.0)))
+    if R[-1] > 2e8:
         return None
-    L = (rowlen - 1).astype(np.float64)
-    return float(np.sum(L * (L - 1.0)))
+    total = 0.0
+    n = len(rowlen)
+    for a in range(0, n, 65536):  # rows in chunks, to bound memory
+        rows = np.arange(a, min(a + 65536, n))
+        cnt = rowlen[rows] - 1
+        ri = np.repeat(rows, cnt)
+        off = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+        j = s[ri] + off
+        total += 2.0 * float(np.sum(j - np.maximum(s[ri], s[j])))
+    return total
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(sample=(16, 16, 8)):
     """Time the reference pipeline (assemble, scale, profile, factor, GMRES)
-    on a sample block with one host core; returns the cpu_baseline object."""
+    on a sample block with one host core; returns the cpu_baseline object
+    with the phase times and the sample's profile statistics (for the
+    extrapolation in extrapolate_cpu)."""
     from misc_amd.meshes import cantilever_block
     from oracle import oracle as O
 
@@ -65,21 +91,51 @@ def cpu_baseline(sample=(16, 16, 8)):
     if O.ref_available():
         r = O.run_reference(model, "elastic")
         t = r["times"]
-        total = t["assemble"] + t["scale"] + t["profile"] + t["factor"] + t["solve"]
+        phases = {k: t[k] for k in ("assemble", "scale", "profile", "factor", "solve")}
         n = int(r["scalars"][1])
+        lt_range = r["elMatLTRange"]
+        nnz = int(r["elMatRange"][-1])
         kind = "reference"
         what = "literal reference (flang -O2, its own GMRES stopping rule)"
     else:
         om = O.OracleModel(model).prep()
         t0 = time.perf_counter()
         om.static_elastic(mode=0)
-        total = time.perf_counter() - t0
+        phases = {"total": time.perf_counter() - t0}
         n = om.scalar("elMatDim")
+        lt_range = om.geti("elMatLTRange")
+        nnz = int(om.geti("elMatRange")[-1])
         kind = "port"
         what = "C++ restatement of the reference algorithm (-O2)"
-    return {"value": n / total, "unit": "DOF/s", "cores": 1, "kind": kind,
+    total = sum(phases.values())
+    return {"value": n / total, "unit": "DOF/s", "cores": 1, "kind": kind, "cpu_model": cpu_model(),
             "sample": f"brick8 block {sample[0]}x{sample[1]}x{sample[2]} ({n} DOF), same physics and "
-                      f"solve sequence, {what}, {total:.1f} s"}
+                      f"solve sequence, {what}, {total:.1f} s",
+            "phases_s": phases, "sample_dims": list(sample), "sample_dof": n, "sample_nnz": nnz,
+            "sample_profile_entries": int(lt_range[-1]), "sample_factor_flops": factor_flops(lt_range)}
+
+
+def extrapolate_cpu(base, steps_sample, target):
+    """Flop-model extrapolation of the literal reference's time from its
+    sample to the target workload (same rule, literal solverMaxBW): the factor
+    by the exact profile flop ratio, GMRES by the work per step (two sweeps
+    over the profile and one CSR product: 4 prof + 2 nnz) times the ratio of
+    GMRES steps (both counted by the GPU solver under the same rule, as a
+    proxy for the reference's own counts), the rest by the DOF ratio."""
+    ph = base["phases_s"]
+    if "factor" not in ph or not base["sample_factor_flops"] or not target["factor_flops"]:
+        return None
+    w_s = 4.0 * base["sample_profile_entries"] + 2.0 * base["sample_nnz"]
+    w_t = 4.0 * target["profile_entries"] + 2.0 * target["nnz"]
+    t_fact = ph["factor"] * target["factor_flops"] / base["sample_factor_flops"]
+    t_solve = ph["solve"] * (w_t / w_s) * (target["gmres_steps"] / max(1.0, steps_sample))
+    t_rest = (ph["assemble"] + ph["scale"] + ph["profile"]) * target["dof"] / base["sample_dof"]
+    total = t_fact + t_solve + t_rest
+    return {"seconds": total, "dof_per_s": target["dof"] / total,
+            "factor_s": t_fact, "gmres_s": t_solve, "rest_s": t_rest,
+            "model": "factor by exact profile flops; GMRES by (4 profile + 2 nnz) per step x steps "
+                     "(the GPU solver's counts under the literal rule at both sizes); the rest by DOF; "
+                     "one core, literal solverMaxBW rule"}
 
 
 def load_traffic():
@@ -587,10 +643,44 @@ def main():
                                   "bound": "mfma", "achieved": flops / fac_s / 1e12, "peak": FP64_MFMA_PEAK_TFLOPS,
                                   "unit": "TFLOP/s", "frac": flops / fac_s / 1e12 / FP64_MFMA_PEAK_TFLOPS,
                                   "flops_per_factor": flops}
+    # an error estimate beside the residual: a second solve from the
+    # converged state (a restart of the refinement) moves u by about its error
+    u1 = ctx.getd("nodeDisp")
+    ctx.static_solve()
+    u2 = ctx.getd("nodeDisp")
+    out["config"]["error_estimate_rel"] = float(np.linalg.norm(u2 - u1) / np.linalg.norm(u2))
     ctx.close()
+    # the same workload under the reference's literal (32-bit) solverMaxBW rule
+    lit = None
+    if args.maxbw == "int64":
+        lctx = astro.Context(model, astro.Options(literalMaxBW=True))
+        lctx.static_solve()
+        lctx.reset()
+        a = time.perf_counter()
+        lctx.static_solve()
+        lctx.sync()
+        lit_s = time.perf_counter() - a
+        lit = {"value": n / lit_s, "unit": "DOF/s", "ms_per_step": 1e3 * lit_s,
+               "solverMaxBW": lctx.scalar("solverMaxBW"), "profile_entries": lctx.scalar("elMatLTSize"),
+               "gmres_steps": lctx.scalar("gmresSteps"), "true_rel_residual": lctx.true_residual(),
+               "phases_ms": {p: lctx.phase_ms(p) for p in phases},
+               "factor_flops": factor_flops(lctx.geti("elMatLTRange"))}
+        lctx.close()
+        out["literal_rule"] = {k: v for k, v in lit.items() if k != "factor_flops"}
+        print("bench: literal-rule solve done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline()
+            base = cpu_baseline()
+            out["cpu_baseline"] = base
+            if lit is not None:
+                # the GPU solver's step count on the sample under the same rule
+                sctx = astro.Context(cantilever_block(*base["sample_dims"]), astro.Options(literalMaxBW=True))
+                sctx.static_solve()
+                steps_s = sctx.scalar("gmresSteps")
+                sctx.close()
+                base["extrapolated_to_workload"] = extrapolate_cpu(
+                    base, steps_s, {"dof": n, "nnz": nnz, "profile_entries": lit["profile_entries"],
+                                    "factor_flops": lit["factor_flops"], "gmres_steps": lit["gmres_steps"]})
         except Exception as e:  # the baseline is a report, not the measurement
             print(f"bench: cpu baseline failed: {e}", file=sys.stderr)
     print(json.dumps(out))
