@@ -169,7 +169,7 @@ def bench_newmark(args):
     scales = half_sine(n_steps, dt, pulse)
     print(f"bench: Newmark brick8 {nx}x{ny}x{nz}, {n_steps} steps", file=sys.stderr, flush=True)
     ctx = astro.Context(model, astro.Options(literalMaxBW=(args.maxbw == "literal"), numVecs=50, rtol=1e-8,
-                                             stepGraphs=True))
+                                             stepGraphs=not args.eager_steps))
     n = ctx.scalar("elMatDim")
     for w in range(args.warmup):
         ctx.dynamic_init(0.25, 0.5, dt)
@@ -246,7 +246,8 @@ def bench_learned(args):
     nx, ny, nz = 48, 48, 24
     n_steps, dt, pulse = 200, 1e-5, 5e-4
     n_fields, n_rhs, epochs = args.fields, 31, args.epochs
-    opts = astro.Options(literalMaxBW=(args.maxbw == "literal"), numVecs=50, rtol=1e-8, stepGraphs=True)
+    opts = astro.Options(literalMaxBW=(args.maxbw == "literal"), numVecs=50, rtol=1e-8,
+                         stepGraphs=not args.eager_steps)
     gnn = astro.Gnn(64, 3, seed=0)
     t0 = time.perf_counter()
     for f in range(n_fields):
@@ -552,6 +553,8 @@ def main():
                          "graph-network preconditioner trained on solver-generated pairs vs plain LDL^T")
     ap.add_argument("--fields", type=int, default=64, help="learned: training fields")
     ap.add_argument("--epochs", type=int, default=40, help="learned: training epochs")
+    ap.add_argument("--eager-steps", action="store_true",
+                    help="newmark / learned: launch every time step eagerly instead of replaying HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported. "
