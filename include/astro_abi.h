@@ -214,6 +214,14 @@ int astro_ctx_static_solve(astro_ctx* ctx);
  * model).  Reactions are -elasticLoad at the constrained nodes
  * (Working/AStrO_output.f90:122-141).  Synchronous. */
 int astro_ctx_elastic_soln_load(astro_ctx* ctx);
+/* Element stress and strain recovery (r_getElStress, solid branch,
+ * Working/AStrO_r_elementEqns.f90:924-976, 1700-1713) at the natural point
+ * sp[3] (each in [-1, 1]) of every element for the current nodeDisp:
+ * Green-Lagrange strain of the element's own nodal displacements and
+ * stress = C strain (no thermal strain), Voigt (11, 22, 33, 12, 13, 23) with
+ * engineering shear; stress, strain: 6 * numEls doubles, element order.
+ * brick8 models (ASTRO_ERR_UNSUPPORTED for shell4).  Synchronous. */
+int astro_ctx_element_stress(astro_ctx* ctx, const double* sp, double* stress, double* strain);
 
 /* Newmark dynamics, the dynamic branch of solve()
  * (Working/AStrO_commandFunctions.f90:762-810; element residual

@@ -37,7 +37,7 @@ EXPORTS = [
     "astro_ctx_dynamic_step", "astro_ctx_dynamic_run", "astro_ctx_set_design", "astro_ctx_dRdD",
     "astro_ctx_compliance_gradient", "astro_ctx_cs_batch_gradient", "astro_ctx_cs_batch_autotune",
     "astro_ctx_cs_batch_tile",
-    "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load",
+    "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load", "astro_ctx_element_stress",
     "astro_ctx_true_residual_noalloc", "astro_gnn_create", "astro_gnn_destroy", "astro_gnn_num_params",
     "astro_gnn_num_fields", "astro_gnn_get_params", "astro_gnn_set_params", "astro_ctx_gnn_collect",
     "astro_gnn_train", "astro_gnn_loss_grad", "astro_gnn_field_size", "astro_gnn_get_field",
@@ -114,6 +114,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_cs_batch_gradient.argtypes = [C.c_void_p, _dp, C.c_void_p]
     L.astro_ctx_cs_batch_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
     L.astro_ctx_cs_batch_tile.argtypes = [C.c_void_p, C.c_int32]
+    L.astro_ctx_element_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_ctx_operator_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_prep_create.argtypes = [C.POINTER(_Model), C.POINTER(_Options), C.POINTER(C.c_void_p)]
@@ -473,6 +474,14 @@ class Context:
         g = np.zeros(self.n_dv)
         _check(load_library().astro_ctx_compliance_gradient(self.h, g))
         return g
+
+    def element_stress(self, point=(0.0, 0.0, 0.0)):
+        """Stress and strain (numEls x 6, Voigt 11, 22, 33, 12, 13, 23) of every
+        brick8 at the natural point for the current nodeDisp."""
+        E = self.model.num_els
+        st, sn = np.zeros(6 * E), np.zeros(6 * E)
+        _check(load_library().astro_ctx_element_stress(self.h, np.ascontiguousarray(point, np.float64), st, sn))
+        return st.reshape(E, 6), sn.reshape(E, 6)
 
     def cs_batch_gradient(self):
         """The compliance gradient by direct complex-step solves of all design

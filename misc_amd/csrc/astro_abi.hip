@@ -1422,6 +1422,29 @@ int astro_ctx_reset(astro_ctx* c) {
     return ASTRO_OK;
 }
 
+// Element stress and strain at the natural point sp of every element for the
+// current nodeDisp (r_getElStress with layer = 0, the solid branch,
+// Working/AStrO_r_elementEqns.f90:924-976, 1700-1713).  stress, strain:
+// host arrays of 6 numEls (Voigt 11, 22, 33, 12, 13, 23).  brick8 models.
+int astro_ctx_element_stress(astro_ctx* c, const double* sp, double* stress, double* strain) {
+    if (!c || !sp || !stress || !strain) return fail(ASTRO_ERR_ARG, "null argument");
+    if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "stress recovery covers brick8 elements");
+    for (int k = 0; k < 3; ++k)
+        if (!(std::fabs(sp[k]) <= 1.0)) return fail(ASTRO_ERR_ARG, "natural coordinates must lie in [-1, 1]");
+    const int E = c->prep.numEls;
+    DBuf<double> ds, de;
+    ASTRO_TRY(ds.alloc(6 * (size_t)E));
+    ASTRO_TRY(de.alloc(6 * (size_t)E));
+    hipLaunchKernelGGL(k_brick8_stress, dim3(grid_for(E, 256)), dim3(256), 0, c->s, E, c->conn.p, c->X.p, c->elMat.p,
+                       c->elScale.p, c->matC.p, c->rank.p, c->prep.numSolidNds, c->nodeDisp.p, sp[0], sp[1], sp[2],
+                       ds.p, de.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(stress, ds.p, sizeof(double) * 6 * (size_t)E, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipMemcpyAsync(strain, de.p, sizeof(double) * 6 * (size_t)E, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
 int astro_ctx_elastic_soln_load(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     ctx_minus_internal_force(c->s, c->csrVal, c->csrCols, c->csrRange, c->prep.elMatDim, c->nodeDisp.p, c->load.p);

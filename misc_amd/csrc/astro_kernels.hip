@@ -16,14 +16,13 @@ namespace astro {
 // pattern; arithmetic never yields it.
 #define ASTRO_SENTINEL 0x7ff4d1c3a5b6c7e9LL
 
-// Gauss point gp (0..7, signs of the natural coordinates from its bits) of a
-// brick8 with nodal coordinates X: shape functions N, their x-derivatives Nx
-// and detJ (the 2x2x2 weights are 1).  Working/AStrO_r_elementEqns.f90:27-36,
+// A brick8 with nodal coordinates X at the natural point (s0, s1, s2): shape
+// functions N, their x-derivatives Nx and detJ; brick8_gauss_point takes Gauss
+// point gp (0..7, signs of the natural coordinates from its bits; the 2x2x2
+// weights are 1).  Working/AStrO_r_elementEqns.f90:27-36,
 // 110-135 (N, Ns), 212-238 (adjugate inverse), 580-620 (Nx = Ns J^-1).
-__device__ __forceinline__ void brick8_gauss_point(int gp, const double (&sX)[8][3], double (&N)[8],
-                                                   double (&Nx)[8][3], double& w) {
-    const double g = 0.57735026918962576451;  // 1/sqrt(3)
-    const double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
+__device__ __forceinline__ void brick8_point(double s0, double s1, double s2, const double (&sX)[8][3],
+                                             double (&N)[8], double (&Nx)[8][3], double& w) {
     double Ns[8][3];
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
@@ -60,7 +59,12 @@ __device__ __forceinline__ void brick8_gauss_point(int gp, const double (&sX)[8]
     for (int n = 0; n < 8; ++n)
 #pragma unroll
         for (int j = 0; j < 3; ++j) Nx[n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j] + Ns[n][2] * I[2][j];
-    w = det;  // Gauss weight 1
+    w = det;
+}
+__device__ __forceinline__ void brick8_gauss_point(int gp, const double (&sX)[8][3], double (&N)[8],
+                                                   double (&Nx)[8][3], double& w) {
+    const double g = 0.57735026918962576451;  // 1/sqrt(3)
+    brick8_point((gp & 1) ? g : -g, (gp & 2) ? g : -g, (gp & 4) ? g : -g, sX, N, Nx, w);  // weight 1
 }
 
 // ---------------------------------------------------------------------------
@@ -484,6 +488,65 @@ __global__ __launch_bounds__(64) void k_shell4_dtangent_cs(
             dd_add_prod(gh, gl, -sU[i], sVl[i]);
         }
         gPart[e] = dv >= 0 ? gh + gl : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stress and strain recovery, brick8 (r_getSolidStress,
+// Working/AStrO_r_elementEqns.f90:924-976): at the natural point sp of every
+// element, ux = sum_n u_n Nx_n of the element's own nodes (the reference's
+// gather indexes by local node number, D4), the Green-Lagrange strain of
+// r_getNLStrain (:622-648, orient = identity) and stress = C strain (no
+// thermal strain: the model carries no expansion coefficients).  Voigt order
+// (11, 22, 33, 12, 13, 23), engineering shear.  One thread per element.
+__global__ __launch_bounds__(256) void k_brick8_stress(int E, const int* __restrict__ conn,
+                                                       const double* __restrict__ X, const int* __restrict__ elMat,
+                                                       const double* __restrict__ elScale,
+                                                       const double* __restrict__ matC, const int* __restrict__ rank,
+                                                       int nS, const double* __restrict__ u, double s0, double s1,
+                                                       double s2, double* __restrict__ stress,
+                                                       double* __restrict__ strain) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double xe[8][3], ue[3][8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const int nd = conn[8 * (size_t)e + n] - 1;
+        const int r = rank[nd];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xe[n][c] = X[3 * (size_t)nd + c];
+            ue[c][n] = u[r + c * nS - 1];
+        }
+    }
+    double N[8], Nx[8][3], det;
+    brick8_point(s0, s1, s2, xe, N, Nx, det);
+    double ux[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int n = 0; n < 8; ++n) acc += ue[i][n] * Nx[n][j];
+            ux[i][j] = acc;
+        }
+    double Ev[6];
+    Ev[0] = ux[0][0] + 0.5 * (ux[0][0] * ux[0][0] + ux[1][0] * ux[1][0] + ux[2][0] * ux[2][0]);
+    Ev[1] = ux[1][1] + 0.5 * (ux[0][1] * ux[0][1] + ux[1][1] * ux[1][1] + ux[2][1] * ux[2][1]);
+    Ev[2] = ux[2][2] + 0.5 * (ux[0][2] * ux[0][2] + ux[1][2] * ux[1][2] + ux[2][2] * ux[2][2]);
+    Ev[3] = ux[0][1] + ux[1][0] + ux[0][0] * ux[0][1] + ux[1][0] * ux[1][1] + ux[2][0] * ux[2][1];
+    Ev[4] = ux[0][2] + ux[2][0] + ux[0][0] * ux[0][2] + ux[1][0] * ux[1][2] + ux[2][0] * ux[2][2];
+    Ev[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
+    const double sc = elScale[e];
+    const double* Cm = matC + 36 * (size_t)elMat[e];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc += Cm[r + 6 * k] * Ev[k];
+        stress[6 * (size_t)e + r] = sc * acc;
+        strain[6 * (size_t)e + r] = Ev[r];
     }
 }
 

@@ -630,6 +630,34 @@ void orc_brick8_internal_force(const double* xyz, const double* C, const double*
     }
 }
 
+// f4: brick8 stress and strain at the natural point s (r_getSolidStress,
+// Working/AStrO_r_elementEqns.f90:924-976): ux of the element DOFs ue
+// (component-major) at s, the Green-Lagrange strain of r_getNLStrain
+// (:622-648, orient = identity) and stress = C (strain - T alpha) with T alpha
+// = 0 (the models carry no expansion coefficients).  C: 6x6 column-major.
+void orc_brick8_stress(const double* xyz, const double* C, const double* ue, const double* s, double* stress,
+                       double* strain) {
+    ElProfile p;
+    element_profile(8, p);
+    double N[8], Nx[8][3];
+    brick_ipdata(xyz, s, N, Nx);
+    double ux[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int d = 0; d < 24; ++d) {
+        int var = p.dofTable[0][d] - 1, nd = p.dofTable[1][d] - 1;
+        for (int j = 0; j < 3; ++j) ux[var][j] += ue[d] * Nx[nd][j];
+    }
+    strain[0] = ux[0][0] + 0.5 * (ux[0][0] * ux[0][0] + ux[1][0] * ux[1][0] + ux[2][0] * ux[2][0]);
+    strain[1] = ux[1][1] + 0.5 * (ux[0][1] * ux[0][1] + ux[1][1] * ux[1][1] + ux[2][1] * ux[2][1]);
+    strain[2] = ux[2][2] + 0.5 * (ux[0][2] * ux[0][2] + ux[1][2] * ux[1][2] + ux[2][2] * ux[2][2]);
+    strain[3] = ux[0][1] + ux[1][0] + ux[0][0] * ux[0][1] + ux[1][0] * ux[1][1] + ux[2][0] * ux[2][1];
+    strain[4] = ux[0][2] + ux[2][0] + ux[0][0] * ux[0][2] + ux[1][0] * ux[1][2] + ux[2][0] * ux[2][2];
+    strain[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
+    for (int r = 0; r < 6; ++r) {
+        stress[r] = 0.0;
+        for (int k = 0; k < 6; ++k) stress[r] += C[r + 6 * k] * strain[k];
+    }
+}
+
 // a13: brick8 conductivity sum_gp detJ*w * Nx k Nx^T.
 // Working/AStrO_r_elementEqns.f90:2062-2124.  k: 3x3 column-major; Kt: 8x8.
 void orc_brick8_conductivity(const double* xyz, const double* k, double* Kt) {
@@ -1842,6 +1870,25 @@ void orc_compliance_gradient(OrcModel* mp, double h, double* grad) {
     }
 }
 
+// Every element of a model at s with its own nodes' displacements (the
+// reference's r_getElementSolution gathers by local node number, D4).
+// stress, strain: 6 x numEls.
+void orc_element_stress(OrcModel* mp, const double* s, double* stress, double* strain) {
+    OrcModel& m = *mp;
+    ElProfile p;
+    element_profile(8, p);
+    double xyz[24], ue[24];
+    for (int e = 0; e < m.numEls; ++e) {
+        orc_element_coords(&m, e, xyz);
+        for (int d = 0; d < 24; ++d) {
+            int node = m.elementList[8 * (size_t)e + p.dofTable[1][d] - 1];
+            int dof = m.nDofIndex[6 * (size_t)(node - 1) + p.dofTable[0][d] - 1];
+            ue[d] = m.nodeDisp[dof - 1];
+        }
+        orc_brick8_stress(xyz, &m.elC[36 * (size_t)e], ue, s, stress + 6 * (size_t)e, strain + 6 * (size_t)e);
+    }
+}
+
 // Array getters for the Python side of the tests.
 const int* orc_geti(OrcModel* m, const char* name, long long* len) {
     std::string s(name);
@@ -1894,6 +1941,7 @@ double* orc_getd(OrcModel* m, const char* name, long long* len) {
 int orc_geti_scalar(OrcModel* m, const char* name) {
     std::string s(name);
     if (s == "elMatDim") return m->elMatDim;
+    if (s == "numEls") return m->numEls;
     if (s == "solverMaxBW") return m->solverMaxBW;
     if (s == "numSolidNds") return m->numSolidNds;
     if (s == "elMPCDim") return m->elMPCDim;

@@ -70,6 +70,8 @@ def lib():
         L.orc_brick8_stiffness.argtypes = [_dp, _dp, _dp]
         L.orc_brick8_mass.argtypes = [_dp, C.c_double, _dp]
         L.orc_brick8_internal_force.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_brick8_stress.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_element_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_shell_frame.argtypes = [_dp, _dp, _dp]
         L.orc_shell_abd.argtypes = [C.c_int, _dp, C.c_double, _dp]
         L.orc_shell4_stiffness.argtypes = [_dp, _dp, _dp, C.c_void_p, _dp]
@@ -133,6 +135,15 @@ def brick8_internal_force(xyz38, Cmat, ue24):
     lib().orc_brick8_internal_force(_f64(np.asarray(xyz38).reshape(3, 8).ravel(order="F")),
                                     _f64(np.asarray(Cmat).ravel(order="F")), _f64(ue24), R)
     return R
+
+
+def brick8_stress(xyz38, Cmat, ue24, s3):
+    """(stress, strain) of one brick8 at the natural point s3 for element DOFs
+    ue (component-major), Voigt (11, 22, 33, 12, 13, 23)."""
+    st, sn = np.zeros(6), np.zeros(6)
+    lib().orc_brick8_stress(_f64(np.asarray(xyz38).reshape(3, 8).ravel(order="F")),
+                            _f64(np.asarray(Cmat).ravel(order="F")), _f64(ue24), _f64(s3), st, sn)
+    return st, sn
 
 
 def shell_frame(orient6, xyz34):
@@ -345,6 +356,14 @@ class OracleModel:
         if bad:
             raise RuntimeError(f"zero pivot at row {bad}")
         return self
+
+    def element_stress(self, s3=(0.0, 0.0, 0.0)):
+        """(stress, strain), numEls x 6 each, at the natural point s3 for the
+        current nodeDisp."""
+        E = self.scalar("numEls")
+        st, sn = np.zeros(6 * E), np.zeros(6 * E)
+        lib().orc_element_stress(self.h, _f64(s3), st, sn)
+        return st.reshape(E, 6), sn.reshape(E, 6)
 
     def dynamic_step(self, load_scale: float, num_vecs: int = 0, mode: int = 1, rtol: float = 1e-12):
         lib().orc_dyn_step(self.h, float(load_scale), int(num_vecs), int(mode), float(rtol))

@@ -31,7 +31,11 @@ def test_factor_flops_closed_form_matches_brute_force():
         assert bench.factor_flops(R) == brute_flops(R)
 
 
-def test_factor_flops_rejects_decreasing_first_column():
+def test_factor_flops_with_decreasing_first_column():
     # rows 0..2: lengths 1, 2, 1 -> first columns 0, 0, 2; row 3 length 4 -> first column 0 (decreases)
     R = np.array([0, 1, 3, 4, 8])
-    assert bench.factor_flops(R) is None
+    assert bench.factor_flops(R) == brute_flops(R) == 2
+    rng = np.random.default_rng(4)
+    s = np.array([max(0, i - int(rng.integers(0, 30))) for i in range(200)])
+    R = np.concatenate([[0], np.cumsum(np.arange(200) - s + 1)])
+    assert bench.factor_flops(R) == brute_flops(R)
