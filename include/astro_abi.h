@@ -120,6 +120,9 @@ typedef struct {
     const double* layers;     /* numEls * numLayers * 6 */
     const double* zOffset;    /* numEls */
     const double* orient;     /* numEls * 6 */
+    /* Specific heat per element (may be null: no capacitance); the transient
+     * thermal path uses density * specHeat. */
+    const double* specHeat;   /* numEls */
 } astro_model;
 
 /* Options of the analysis (solver parameters of the *solve command). */
@@ -278,6 +281,17 @@ int astro_ctx_thermal_solve(astro_ctx* ctx);
  * "thermalLoad" = -K T for the current nodeTemp (reactionHeatGen is
  * -thermalLoad).  Synchronous; needs astro_ctx_assemble_thermal first. */
 int astro_ctx_thermal_soln_load(astro_ctx* ctx);
+/* Transient heat conduction, the thermal branch of solve() with dynamic = 1
+ * (Working/AStrO_commandFunctions.f90:763-769, 689-703, 791-803; element
+ * residual r_getElRt / r_getElRtm, Working/AStrO_r_elementEqns.f90:2126-2205):
+ * Tdot = (1 / gamma) ((T - Tp) / dt - (1 - gamma) Tdp), element residual
+ * K T + C Tdot with the capacitance C = rho c N N^T, tangent
+ * K + C / (gamma dt), factored once by thermal_dynamic_init (the state
+ * starts at T = Tdot = 0).  A step: RHS -R(T) + C^T (rhs - C T) of the
+ * temperature constraints, GMRES, T += delta, then Tdot and the previous
+ * state.  brick8 models; "nodeTemp", "nodeTdot" readable.  Synchronous. */
+int astro_ctx_thermal_dynamic_init(astro_ctx* ctx, double gamma, double dt);
+int astro_ctx_thermal_dynamic_run(astro_ctx* ctx, int nSteps);
 
 /* Preconditioner apply on device vectors of length elMatDim:
  * x = P^-1 b via the two triangular sweeps.  b and x must be distinct buffers

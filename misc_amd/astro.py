@@ -38,6 +38,7 @@ EXPORTS = [
     "astro_ctx_compliance_gradient", "astro_ctx_cs_batch_gradient", "astro_ctx_cs_batch_autotune",
     "astro_ctx_cs_batch_tile",
     "astro_ctx_elastic_soln_load", "astro_ctx_thermal_soln_load", "astro_ctx_element_stress",
+    "astro_ctx_thermal_dynamic_init", "astro_ctx_thermal_dynamic_run",
     "astro_ctx_true_residual_noalloc", "astro_gnn_create", "astro_gnn_destroy", "astro_gnn_num_params",
     "astro_gnn_num_fields", "astro_gnn_get_params", "astro_gnn_set_params", "astro_ctx_gnn_collect",
     "astro_gnn_train", "astro_gnn_loss_grad", "astro_gnn_field_size", "astro_gnn_get_field",
@@ -66,7 +67,7 @@ class _Model(C.Structure):
                 ("mpcNodeStart", C.c_void_p), ("mpcNodes", C.c_void_p), ("numMPC", C.c_int32),
                 ("mpcRHS", C.c_void_p), ("numLoads", C.c_int32), ("loadNode", C.c_void_p),
                 ("loadVal", C.c_void_p), ("numLayers", C.c_int32), ("layers", C.c_void_p), ("zOffset", C.c_void_p),
-                ("orient", C.c_void_p)]
+                ("orient", C.c_void_p), ("specHeat", C.c_void_p)]
 
 
 class _Options(C.Structure):
@@ -115,6 +116,8 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_cs_batch_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
     L.astro_ctx_cs_batch_tile.argtypes = [C.c_void_p, C.c_int32]
     L.astro_ctx_element_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.astro_ctx_thermal_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    L.astro_ctx_thermal_dynamic_run.argtypes = [C.c_void_p, C.c_int]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_ctx_operator_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.astro_prep_create.argtypes = [C.POINTER(_Model), C.POINTER(_Options), C.POINTER(C.c_void_p)]
@@ -275,6 +278,8 @@ def _marshal(model, o: "Options"):
         M.layers = ptr(model.layers.reshape(-1), np.float64)
         M.zOffset = ptr(model.z_offset, np.float64)
         M.orient = ptr(model.orient.reshape(-1), np.float64)
+    if getattr(model, "spec_heat", None) is not None:
+        M.specHeat = ptr(model.spec_heat, np.float64)
     O = _Options(int(o.maxBW), 1 if o.literalMaxBW else 0, 1 if o.thermLiteralD8 else 0, int(o.numVecs),
                  float(o.rtol), int(o.maxCycles), 1 if o.coupledProfile else 0, int(o.precision),
                  1 if o.stepGraphs else 0)
@@ -359,7 +364,7 @@ class Context:
             "prevDisp": n("elMatDim"), "thermMatRange": self.model.num_nodes + 1,
             "thermMatLTRange": self.model.num_nodes + 1, "thermMatCols": n("thermMatSize"),
             "thermMat": n("thermMatSize"), "thermMatLT": n("thermMatLTSize"), "thermalLoad": self.model.num_nodes,
-            "nodeTemp": self.model.num_nodes, "elMPCMatRange": n("elMPCDim") + 1,
+            "nodeTemp": self.model.num_nodes, "nodeTdot": self.model.num_nodes, "elMPCMatRange": n("elMPCDim") + 1,
             "elMPCMatCols": n("elMPCSize"), "elMPCMat": n("elMPCSize"), "elMPCRHS": n("elMPCDim"),
         }
         return sizes[name]
@@ -474,6 +479,14 @@ class Context:
         g = np.zeros(self.n_dv)
         _check(load_library().astro_ctx_compliance_gradient(self.h, g))
         return g
+
+    def thermal_dynamic_init(self, gamma: float = 0.5, dt: float = 1.0):
+        """Transient conduction set-up: tangent K + C/(gamma dt), penalty
+        scaling, profile, factor; T = Tdot = 0."""
+        _check(load_library().astro_ctx_thermal_dynamic_init(self.h, float(gamma), float(dt)))
+
+    def thermal_dynamic_run(self, n_steps: int):
+        _check(load_library().astro_ctx_thermal_dynamic_run(self.h, int(n_steps)))
 
     def element_stress(self, point=(0.0, 0.0, 0.0)):
         """Stress and strain (numEls x 6, Voigt 11, 22, 33, 12, 13, 23) of every

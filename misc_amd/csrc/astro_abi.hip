@@ -776,7 +776,7 @@ struct astro_ctx {
     astro_options opt{};
     int numVecs = 100;
     // model on device
-    DBuf<double> X, matC, elScale, elRho, elCond;
+    DBuf<double> X, matC, elScale, elRho, elCond, elCap;
     DBuf<double> elABD, elAlpha;  // shell4 models: layup stiffness 81, frame 9 per element
     int nLayers = 0;
     std::vector<double> hLayers, hZOff;  // host copy of the shell sections
@@ -818,6 +818,11 @@ struct astro_ctx {
     // thermal system
     DBuf<int> tRange, tCols, tMpcRange, tMpcCols;
     DBuf<double> tVal, tMpcMat, tMpcRhs, tLoad, tDel, nodeTemp;
+    // transient thermal: capacitance CSR, the rate state, a work vector
+    DBuf<double> prevTemp, prevTdot, nodeTdot, tWork;
+    bool thDynReady = false;
+    double thGamma = 0.5, thDt = 0.0;
+    int64_t thDynSteps = 0, thDynGmresSteps = 0;
     DeviceProfile tprof;
     DeviceProfile32 tprof32;
     DeviceOperator top;
@@ -915,7 +920,7 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
     // element keeps E1 as a scale (C is linear in the moduli at fixed ratios)
     std::vector<double> mats;
     std::vector<int> elMat(E);
-    std::vector<double> scale(E), rho(E, 0.0), cond(6 * (size_t)E, 0.0);
+    std::vector<double> scale(E), rho(E, 0.0), cond(6 * (size_t)E, 0.0), cap(E, 0.0);
     std::vector<std::vector<double>> keys;
     for (int e = 0; e < E; ++e) {
         const double* ep = m->eprops ? m->eprops + 9 * (size_t)e : nullptr;
@@ -947,6 +952,7 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
         elMat[e] = id;
         scale[e] = E1;
         if (m->density) rho[e] = m->density[e];
+        if (m->specHeat) cap[e] = rho[e] * m->specHeat[e];
         if (m->cond)
             for (int i = 0; i < 6; ++i) cond[6 * (size_t)e + i] = m->cond[6 * (size_t)e + i];
     }
@@ -977,6 +983,7 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
     ASTRO_TRY(c->elScale.upload(scale, s));
     ASTRO_TRY(c->elRho.upload(rho, s));
     ASTRO_TRY(c->elCond.upload(cond, s));
+    ASTRO_TRY(c->elCap.upload(cap, s));
     ASTRO_TRY(c->rank.upload(p.currentRank, s));
     ASTRO_TRY(c->elPos.upload(p.elPos, s));
     ASTRO_TRY(c->thPos.upload(p.thPos, s));
@@ -1182,6 +1189,8 @@ int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
     if (s == "numVecs") return c->numVecs;
     if (s == "dynSteps") return c->dynSteps;
     if (s == "dynGmresSteps") return c->dynGmresSteps;
+    if (s == "thDynSteps") return c->thDynSteps;
+    if (s == "thDynGmresSteps") return c->thDynGmresSteps;
     if (s == "dynGraphSteps") return c->dynGraphSteps;
     if (s == "dynEagerSteps") return c->dynEagerSteps;
     if (s == "dynGraphs") return c->dynGraphs;
@@ -1250,6 +1259,7 @@ static double* dev_double_array(astro_ctx* c, const std::string& s, int64_t& len
     }
     if (s == "thermalLoad") return pick(c->tLoad, p.numNodes);
     if (s == "nodeTemp") return pick(c->nodeTemp, p.numNodes);
+    if (s == "nodeTdot") return pick(c->nodeTdot, p.numNodes);
     if (s == "nodeVel") return pick(c->nodeVel, p.elMatDim);
     if (s == "nodeAcc") return pick(c->nodeAcc, p.elMatDim);
     if (s == "prevDisp") return pick(c->prevDisp, p.elMatDim);
@@ -2208,7 +2218,8 @@ int astro_ctx_assemble_thermal(astro_ctx* c) {
         const int cnt = c->colorStart[k + 1] - c->colorStart[k];
         if (cnt <= 0) continue;
         hipLaunchKernelGGL(k_brick8_conduct, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
-                           c->conn.p, c->X.p, c->elCond.p, c->rank.p, c->tRange.p, c->thPos.p, c->tVal.p);
+                           c->conn.p, c->X.p, c->elCond.p, 1.0, (const double*)nullptr, 0.0, c->rank.p, c->tRange.p,
+                           c->thPos.p, c->tVal.p);
     }
     HIP_TRY(hipGetLastError());
     return ASTRO_OK;
@@ -2240,6 +2251,89 @@ int astro_ctx_thermal_solve(astro_ctx* c) {
         hipLaunchKernelGGL(k_axpby, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->nodeTemp.p, c->tDel.p,
                            c->nodeTemp.p, N, 1.0, 1.0, (const int*)nullptr);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));
+        return ASTRO_OK;
+    });
+}
+
+// Transient thermal set-up: the tangent K + C / (gamma dt) into the thermal
+// CSR (the steps recompute K and C element by element), penalty scaling by the
+// tangent, profile and factor; the state starts at T = Tdot = 0.
+int astro_ctx_thermal_dynamic_init(astro_ctx* c, double gamma, double dt) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (!(gamma > 0.0) || !(dt > 0.0)) return fail(ASTRO_ERR_ARG, "need gamma > 0 and dt > 0");
+    if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "transient thermal covers brick8 meshes");
+    ASTRO_TRY(ctx_thermal_setup(c));
+    hipStream_t s = c->s;
+    const Prep& p = c->prep;
+    const int64_t nnz = (int64_t)p.thermMatCols.size();
+    const int N = p.numNodes, cd = p.thermMPCDim;
+    const double cm = 1.0 / (gamma * dt);
+    for (DBuf<double>* b : {&c->prevTemp, &c->prevTdot, &c->nodeTdot, &c->tWork})
+        if (b->n != (size_t)N) ASTRO_TRY(b->alloc(N));
+    for (DBuf<double>* b : {&c->prevTemp, &c->prevTdot, &c->nodeTdot, &c->nodeTemp, &c->tDel})
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, b->p, (int64_t)N, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, c->tVal.p, nnz, 0.0);
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_brick8_conduct, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[k], cnt, c->conn.p,
+                           c->X.p, c->elCond.p, 1.0, (const double*)c->elCap.p, cm, c->rank.p, c->tRange.p,
+                           c->thPos.p, c->tVal.p);
+    }
+    HIP_TRY(hipMemsetAsync(c->maxBits.p, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_max_abs, dim3(grid_for(nnz, 256, 2048)), dim3(256), 0, s, c->tVal.p, nnz, c->maxBits.p);
+    if (cd > 0)
+        hipLaunchKernelGGL(k_scale_mpc, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p, c->tMpcRhs.p,
+                           c->tMpcRange.p, cd, c->maxBits.p);
+    HIP_TRY(hipGetLastError());
+    ASTRO_TRY(with_tprof(c, [&](auto& P) -> int {
+        ASTRO_TRY(profile_from_csr(P, s, c->tVal.p, c->tCols.p, c->tRange.p, N, c->tMpcMat.p, c->tMpcCols.p,
+                                   c->tMpcRange.p, cd));
+        return P.factor(s);
+    }));
+    c->thGamma = gamma;
+    c->thDt = dt;
+    c->thDynSteps = c->thDynGmresSteps = 0;
+    c->thDynReady = true;
+    return ASTRO_OK;
+}
+
+// nSteps transient steps (stepSolve's thermal branch, then the rate update).
+int astro_ctx_thermal_dynamic_run(astro_ctx* c, int nSteps) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (!c->thDynReady) return fail(ASTRO_ERR_ARG, "astro_ctx_thermal_dynamic_init has not been called");
+    if (nSteps < 0) return fail(ASTRO_ERR_ARG, "bad step count");
+    hipStream_t s = c->s;
+    const Prep& p = c->prep;
+    const int N = p.numNodes, cd = p.thermMPCDim;
+    const double g = c->thGamma, dt = c->thDt;
+    return with_tprof(c, [&](auto& P) -> int {
+        for (int k = 0; k < nSteps; ++k) {
+            // load = -(K T + C Tdot(T)) element by element + C^T (rhs - C T)
+            hipLaunchKernelGGL(k_thermal_rate, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, N, c->nodeTemp.p,
+                               c->prevTemp.p, c->prevTdot.p, g, dt, 0.0, c->tWork.p);
+            hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->tLoad.p, (int64_t)N, 0.0);
+            for (size_t q = 0; q + 1 < c->colorStart.size(); ++q) {
+                const int cnt = c->colorStart[q + 1] - c->colorStart[q];
+                if (cnt <= 0) continue;
+                hipLaunchKernelGGL(k_brick8_thermal_residual, dim3(cnt), dim3(64), 0, s,
+                                   c->perm.p + c->colorStart[q], cnt, c->conn.p, c->X.p, c->elCond.p,
+                                   (const double*)c->elCap.p, c->rank.p, c->nodeTemp.p, c->tWork.p, c->tLoad.p);
+            }
+            if (cd > 0)
+                hipLaunchKernelGGL(k_constraint_load, dim3(grid_for(cd, 256)), dim3(256), 0, s, c->tMpcMat.p,
+                                   c->tMpcCols.p, c->tMpcRange.p, c->tMpcRhs.p, cd, c->nodeTemp.p, c->tLoad.p,
+                                   (const int*)nullptr);
+            HIP_TRY(hipGetLastError());
+            // GMRES from the previous step's increment, as stepSolve passes delDisp
+            ASTRO_TRY(c->tgm.solve(c->top, P, c->tLoad.p, c->tDel.p, c->opt.rtol, 0.0, c->opt.maxCycles, s));
+            hipLaunchKernelGGL(k_thermal_update, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, N, c->nodeTemp.p,
+                               c->tDel.p, c->prevTemp.p, c->prevTdot.p, c->nodeTdot.p, g, dt);
+            HIP_TRY(hipGetLastError());
+            c->thDynSteps += 1;
+            c->thDynGmresSteps += c->tgm.steps;
+        }
         HIP_TRY(hipStreamSynchronize(s));
         return ASTRO_OK;
     });

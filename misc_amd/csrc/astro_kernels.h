@@ -70,8 +70,16 @@ __global__ void k_brick8_residual(const int* perm, int count, const int* conn, c
                                   double c1, double c2, double dt, double beta, double* load, const int* gate);
 __global__ void k_newmark_update(int n, double* u, const double* du, double* pu, double* pv, double* pa, double* v,
                                  double* a, double c1, double c2, double dt, double beta, double gamma, const int* gate);
+__global__ void k_thermal_rate(int n, const double* T, const double* Tp, const double* Tdp, double gamma, double dt,
+                               double cm, double* w);
+__global__ void k_thermal_update(int n, double* T, const double* dT, double* Tp, double* Tdp, double* Tdot,
+                                 double gamma, double dt);
 __global__ void k_brick8_conduct(const int* perm, int count, const int* conn, const double* X, const double* elCond,
-                                 const int* rank, const int* csrRange, const uint8_t* thPos, double* val);
+                                 double condFactor, const double* elCap, double capFactor, const int* rank,
+                                 const int* csrRange, const uint8_t* thPos, double* val);
+__global__ void k_brick8_thermal_residual(const int* perm, int count, const int* conn, const double* X,
+                                          const double* elCond, const double* elCap, const int* rank, const double* T,
+                                          const double* Tdot, double* load);
 // k ranges of step U: all columns before the panel, only those before the
 // previous panel ("old", which does not wait for the previous panel's
 // factor), or only the previous panel's 64 ("new").

@@ -678,19 +678,49 @@ __global__ void k_newmark_update(int n, double* __restrict__ u, const double* __
     }
 }
 
-// Brick8 conductivity (Working/AStrO_r_elementEqns.f90:2062-2124) scattered
-// into the thermal CSR by node rank; slot 255 marks a column the reference's
-// probe never stores (its `.lt.` bound, Working/AStrO_commandFunctions.f90:57).
+// Transient thermal vectors (r_getElRt, Working/AStrO_r_elementEqns.f90:
+// 2190-2200, and the update of solve, Working/AStrO_commandFunctions.f90:791-803):
+// w = Tdot(T) - cm T with Tdot(T) = (1 / gamma) ((T - Tp) / dt - (1 - gamma) Tdp),
+// so that K T + C Tdot = A T + C w for the tangent A = K + cm C, cm = 1 / (gamma dt).
+__global__ void k_thermal_rate(int n, const double* __restrict__ T, const double* __restrict__ Tp,
+                               const double* __restrict__ Tdp, double gamma, double dt, double cm,
+                               double* __restrict__ w) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double td = (1.0 / gamma) * ((1.0 / dt) * (T[i] - Tp[i]) - (1.0 - gamma) * Tdp[i]);
+        w[i] = td - cm * T[i];
+    }
+}
+// After the step: T += dT, Tdot(T), then Tp = T, Tdp = Tdot.
+__global__ void k_thermal_update(int n, double* __restrict__ T, const double* __restrict__ dT,
+                                 double* __restrict__ Tp, double* __restrict__ Tdp, double* __restrict__ Tdot,
+                                 double gamma, double dt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double t = T[i] + dT[i];
+        const double td = (1.0 / gamma) * ((1.0 / dt) * (t - Tp[i]) - (1.0 - gamma) * Tdp[i]);
+        T[i] = t;
+        Tdot[i] = td;
+        Tp[i] = t;
+        Tdp[i] = td;
+    }
+}
+
+// Brick8 conductivity (Working/AStrO_r_elementEqns.f90:2062-2124) times
+// condFactor plus capFactor times the capacitance, scattered into the
+// thermal CSR by node rank; slot 255 marks a column the reference's probe
+// never stores (its `.lt.` bound, Working/AStrO_commandFunctions.f90:57).
+// elCap: rho c per element (read only when capFactor != 0).
 __global__ __launch_bounds__(64) void k_brick8_conduct(
     const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
-    const double* __restrict__ elCond, const int* __restrict__ rank, const int* __restrict__ csrRange,
-    const uint8_t* __restrict__ thPos, double* __restrict__ val) {
+    const double* __restrict__ elCond, double condFactor, const double* __restrict__ elCap, double capFactor,
+    const int* __restrict__ rank, const int* __restrict__ csrRange, const uint8_t* __restrict__ thPos,
+    double* __restrict__ val) {
     const int idx = blockIdx.x;
     if (idx >= count) return;
     const int e = perm[idx];
     const int lane = threadIdx.x;
     __shared__ double sX[8][3];
     __shared__ double sNx[8][8][3];
+    __shared__ double sN[8][8];
     __shared__ double sW[8];
     __shared__ int sNode[8];
     if (lane < 8) {
@@ -699,58 +729,81 @@ __global__ __launch_bounds__(64) void k_brick8_conduct(
         for (int c = 0; c < 3; ++c) sX[lane][c] = X[3 * (size_t)nd + c];
     }
     __syncthreads();
-    if (lane < 8) {
-        const double g = 0.57735026918962576451;
-        const int gp = lane;
-        const double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
-        double Ns[8][3];
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int n = 0; n < 8; ++n) {
-            const double sg0 = ((n + 1) & 2) ? 1.0 : -1.0, sg1 = (n & 2) ? 1.0 : -1.0, sg2 = (n & 4) ? 1.0 : -1.0;
-            const double f0 = 1.0 + sg0 * s0, f1 = 1.0 + sg1 * s1, f2 = 1.0 + sg2 * s2;
-            Ns[n][0] = 0.125 * sg0 * f1 * f2;
-            Ns[n][1] = 0.125 * sg1 * f0 * f2;
-            Ns[n][2] = 0.125 * sg2 * f0 * f1;
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) J[i][j] += sX[n][i] * Ns[n][j];
-        }
-        const double a11 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        const double a12 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        const double a13 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        const double det = J[0][0] * a11 + J[0][1] * a12 + J[0][2] * a13;
-        const double di = 1.0 / det;
-        double I[3][3];
-        I[0][0] = di * a11;
-        I[1][0] = di * a12;
-        I[2][0] = di * a13;
-        I[0][1] = di * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
-        I[1][1] = di * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
-        I[2][1] = di * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
-        I[0][2] = di * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
-        I[1][2] = di * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
-        I[2][2] = di * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-        for (int n = 0; n < 8; ++n)
-            for (int j = 0; j < 3; ++j) sNx[gp][n][j] = Ns[n][0] * I[0][j] + Ns[n][1] * I[1][j] + Ns[n][2] * I[2][j];
-        sW[gp] = det;
-    }
+    if (lane < 8) brick8_gauss_point(lane, sX, sN[lane], sNx[lane], sW[lane]);
     __syncthreads();
     const int a = lane >> 3, b = lane & 7;
-    const double* kc = elCond + 6 * (size_t)e;
-    const double k00 = kc[0], k11 = kc[1], k22 = kc[2], k01 = kc[3], k02 = kc[4], k12 = kc[5];
     double acc = 0.0;
-    for (int g = 0; g < 8; ++g) {
-        const double ax = sNx[g][a][0], ay = sNx[g][a][1], az = sNx[g][a][2];
-        const double bx = sNx[g][b][0], by = sNx[g][b][1], bz = sNx[g][b][2];
-        const double kbx = k00 * bx + k01 * by + k02 * bz;
-        const double kby = k01 * bx + k11 * by + k12 * bz;
-        const double kbz = k02 * bx + k12 * by + k22 * bz;
-        acc += sW[g] * (ax * kbx + ay * kby + az * kbz);
+    if (condFactor != 0.0) {
+        const double* kc = elCond + 6 * (size_t)e;
+        const double k00 = kc[0], k11 = kc[1], k22 = kc[2], k01 = kc[3], k02 = kc[4], k12 = kc[5];
+        for (int g = 0; g < 8; ++g) {
+            const double ax = sNx[g][a][0], ay = sNx[g][a][1], az = sNx[g][a][2];
+            const double bx = sNx[g][b][0], by = sNx[g][b][1], bz = sNx[g][b][2];
+            const double kbx = k00 * bx + k01 * by + k02 * bz;
+            const double kby = k01 * bx + k11 * by + k12 * bz;
+            const double kbz = k02 * bx + k12 * by + k22 * bz;
+            acc += sW[g] * (ax * kbx + ay * kby + az * kbz);
+        }
+        acc *= condFactor;
+    }
+    if (capFactor != 0.0) {
+        // the capacitance rho c N N^T of r_getElRtm (Working/AStrO_r_elementEqns.f90:2126-2165)
+        double m = 0.0;
+        for (int g = 0; g < 8; ++g) m += sW[g] * sN[g][a] * sN[g][b];
+        acc += capFactor * elCap[e] * m;
     }
     const int slot = thPos[64 * (size_t)e + lane];
     if (slot != 255) {
         const int row = rank[sNode[a]];
         val[csrRange[row - 1] + slot] += acc;
     }
+}
+
+// The transient thermal load -Rt of getThermalSolnLoad, element by element:
+// load_a -= sum_b K_ab T_b + C_ab Tdot_b with the whole element matrices of
+// r_getElRt (Working/AStrO_r_elementEqns.f90:2166-2200), whatever the CSR
+// stores (the reference's probe drops entries of the assembled matrix, not of
+// the residual).  T, Tdot, load: by node rank.  One colour per launch.
+__global__ __launch_bounds__(64) void k_brick8_thermal_residual(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ elCond, const double* __restrict__ elCap, const int* __restrict__ rank,
+    const double* __restrict__ T, const double* __restrict__ Tdot, double* __restrict__ load) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[8][3];
+    __shared__ double sNx[8][8][3];
+    __shared__ double sN[8][8];
+    __shared__ double sW[8];
+    __shared__ int sRow[8];
+    if (lane < 8) {
+        int nd = conn[8 * (size_t)e + lane] - 1;
+        sRow[lane] = rank[nd] - 1;
+        for (int c = 0; c < 3; ++c) sX[lane][c] = X[3 * (size_t)nd + c];
+    }
+    __syncthreads();
+    if (lane < 8) brick8_gauss_point(lane, sX, sN[lane], sNx[lane], sW[lane]);
+    __syncthreads();
+    const int a = lane >> 3, b = lane & 7;
+    const double* kc = elCond + 6 * (size_t)e;
+    const double k00 = kc[0], k11 = kc[1], k22 = kc[2], k01 = kc[3], k02 = kc[4], k12 = kc[5];
+    double kab = 0.0, mab = 0.0;
+    for (int g = 0; g < 8; ++g) {
+        const double ax = sNx[g][a][0], ay = sNx[g][a][1], az = sNx[g][a][2];
+        const double bx = sNx[g][b][0], by = sNx[g][b][1], bz = sNx[g][b][2];
+        const double kbx = k00 * bx + k01 * by + k02 * bz;
+        const double kby = k01 * bx + k11 * by + k12 * bz;
+        const double kbz = k02 * bx + k12 * by + k22 * bz;
+        kab += sW[g] * (ax * kbx + ay * kby + az * kbz);
+        mab += sW[g] * sN[g][a] * sN[g][b];
+    }
+    const int rb = sRow[b];
+    double r = kab * T[rb] + elCap[e] * mab * Tdot[rb];
+    r += __shfl_xor(r, 1);
+    r += __shfl_xor(r, 2);
+    r += __shfl_xor(r, 4);
+    if (b == 0) load[sRow[a]] -= r;
 }
 
 // max |v| over a vector, combined into *out as the bit pattern of a

@@ -40,6 +40,7 @@ module AStrO_gpuElastic
         type(c_ptr) :: loadNode, loadVal
         integer(c_int32_t) :: numLayers
         type(c_ptr) :: layers, zOffset, orient
+        type(c_ptr) :: specHeat
     end type
 
     type, bind(C) :: astro_options_t
@@ -154,6 +155,7 @@ contains
         type(astro_model_t) :: m
         type(astro_options_t) :: o
         real(c_double), allocatable, target :: xyz(:), ep(:), den(:), cnd(:), lay(:), zo(:), ori(:), coef(:), rhs(:)
+        real(c_double), allocatable, target :: sh(:)
         integer(c_int32_t), allocatable, target :: conn(:), et(:), eqn(:), dof(:), isSet(:), nstart(:), nodes(:)
         integer(c_int32_t), allocatable :: tn(:)
         integer :: e, sec, mat, i, j, l, d, nL, nLay, cnt, iset, t, nTerms, k
@@ -165,7 +167,7 @@ contains
             call gpuElasticRelease()
         endif
         allocate(tn(max(1, numNodes)))
-        allocate(xyz(3*numNodes), conn(8*numEls), et(numEls), ep(9*numEls), den(numEls), cnd(6*numEls))
+        allocate(xyz(3*numNodes), conn(8*numEls), et(numEls), ep(9*numEls), den(numEls), cnd(6*numEls), sh(numEls))
         xyz = reshape(nodeList(1:3, 1:numNodes), [3*numNodes])
         conn = reshape(elementList(1:8, 1:numEls), [8*numEls])
         et = elementType(1:numEls)
@@ -191,6 +193,7 @@ contains
             ep(9*(e-1)+1:9*e) = materialElastic(1:9, mat)
             den(e) = materialDensity(mat)
             cnd(6*(e-1)+1:6*e) = materialThermCond(1:6, mat)
+            sh(e) = materialSpecHeat(mat)
             if (numDVar .gt. 0) then
                 do i = elToDRange(e-1) + 1, elToDRange(e)
                     d = elToD(i)
@@ -293,6 +296,7 @@ contains
         m%numLoads = 0
         m%loadNode = c_null_ptr
         m%loadVal = c_null_ptr
+        m%specHeat = c_loc(sh)
         m%numLayers = nLay
         if (nLay .gt. 0) then
             m%layers = c_loc(lay)

@@ -59,6 +59,7 @@ class Model:
     layers: Optional[np.ndarray] = None   # (E, nL, 6)
     z_offset: Optional[np.ndarray] = None  # (E,)
     orient: Optional[np.ndarray] = None   # (E, 6)
+    spec_heat: Optional[np.ndarray] = None  # (E,) specific heat (transient thermal)
 
     @property
     def num_nodes(self) -> int:
@@ -150,9 +151,11 @@ def cantilever_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, E=200e9, nu=0.3, pressu
     return m
 
 
-def thermal_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, k=50.0, t_top=100.0, t_bottom=0.0, name=None):
-    """Steady conduction on a brick8 block: temperature fixed on the top and
-    bottom faces through dof-7 constraint equations on node sets."""
+def thermal_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, k=50.0, t_top=100.0, t_bottom=0.0, rho=0.0, c=0.0,
+                  name=None):
+    """Conduction on a brick8 block: temperature fixed on the top and bottom
+    faces through dof-7 constraint equations on node sets; density rho and
+    specific heat c give the capacitance of the transient path."""
     nodes, conn, colors = brick_block(nx, ny, nz, lx, ly, lz)
     E = conn.shape[0]
     cond = np.tile(np.array([k, k, k, 0.0, 0.0, 0.0]), (E, 1))
@@ -161,7 +164,7 @@ def thermal_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, k=50.0, t_top=100.0, t_bot
     top = bottom + NX * NY * nz
     m = Model(name=name or f"thermal{nx}x{ny}x{nz}", nodes=nodes, elements=conn,
               etype=np.full(E, BRICK8, np.int32), eprops=np.tile(iso_props(200e9, 0.3), (E, 1)),
-              density=np.zeros(E), cond=cond, colors=colors)
+              density=np.full(E, float(rho)), cond=cond, colors=colors, spec_heat=np.full(E, float(c)))
     m.mpc = [MPCTerm(eqn=1, nodes=top, dof=7, coef=1.0, is_set=True, set_name="top"),
              MPCTerm(eqn=2, nodes=bottom, dof=7, coef=1.0, is_set=True, set_name="bottom")]
     m.mpc_rhs = [t_top, t_bottom]

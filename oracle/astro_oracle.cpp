@@ -684,6 +684,20 @@ void orc_brick8_conductivity(const double* xyz, const double* k, double* Kt) {
     }
 }
 
+// f3: brick8 capacitance sum_gp detJ*w * rhoc N N^T, the dRdTdot of r_getElRtm
+// (Working/AStrO_r_elementEqns.f90:2126-2165).  Mt: 8x8.
+void orc_brick8_capacitance(const double* xyz, double rhoc, double* Mt) {
+    ElProfile p;
+    element_profile(8, p);
+    std::memset(Mt, 0, sizeof(double) * 64);
+    for (int g = 0; g < 8; ++g) {
+        double N[8], Nx[8][3];
+        double wd = brick_ipdata(xyz, p.intPts[g], N, Nx) * p.ipWt[g] * rhoc;
+        for (int b = 0; b < 8; ++b)
+            for (int a = 0; a < 8; ++a) Mt[a + 8 * b] += N[a] * wd * N[b];
+    }
+}
+
 // Volume sum_gp detJ*w of one brick (used by the closed-form checks).
 double orc_brick8_volume(const double* xyz) {
     ElProfile p;
@@ -1037,6 +1051,7 @@ struct OrcModel {
     std::vector<double> elC;        // 36*E
     std::vector<double> elDen;      // E
     std::vector<double> elCond;     // 9*E
+    std::vector<double> elCap;      // E: density * specific heat
     std::vector<double> elABD;      // 81*E (shell elements)
     std::vector<double> elOrient;   // 6*E section orientation vectors (shell elements)
     std::vector<double> elLayers;   // nLayers*6*E (shell elements)
@@ -1074,6 +1089,10 @@ struct OrcModel {
     double gmresRes = 0;
     // Newmark state (solve's dynamic branch)
     double nmBeta = 0.25, nmGamma = 0.5, delT = 0.0;
+    // transient thermal: capacitance on the thermal CSR's columns, the state
+    std::vector<double> capMat, nodeTemp, prevTemp, prevTdot, nodeTdot;
+    std::vector<int> capCols;
+    double thGamma = 0.5, thDt = 0.0;
     std::vector<double> prevDisp, prevVel, prevAcc, nodeVel, nodeAcc;
     std::vector<double> timings;  // seconds per phase of the last static solve
 };
@@ -1248,6 +1267,7 @@ OrcModel* orc_create(int numNodes, const double* nodeList, int numEls, const int
     m->elC.assign(36 * (size_t)numEls, 0.0);
     m->elDen.assign(numEls, 0.0);
     m->elCond.assign(9 * (size_t)numEls, 0.0);
+    m->elCap.assign(numEls, 0.0);
     return m;
 }
 
@@ -1290,6 +1310,11 @@ void orc_set_properties(OrcModel* m, const double* eprops, const double* den, co
             K[7] = K[5] = k[5];
         }
     }
+}
+
+// Specific heat per element (after orc_set_properties): capacitance rho c.
+void orc_set_spec_heat(OrcModel* m, const double* sh) {
+    for (int e = 0; e < m->numEls; ++e) m->elCap[e] = m->elDen[e] * sh[e];
 }
 
 void orc_set_mpc(OrcModel* m, int nTerms, const int* eqn, const int* dof, const double* coef,
@@ -1889,6 +1914,100 @@ void orc_element_stress(OrcModel* mp, const double* s, double* stress, double* s
     }
 }
 
+// f3: transient conduction, the thermal branch of solve() with dynamic = 1
+// (Working/AStrO_commandFunctions.f90:763-769, 689-703, 791-803).  Set-up:
+// getThermalSolnLoad(1), whose element tangent r_getElRt adds
+// dRdTdot / (gamma dt) to the conductivity (Working/AStrO_r_elementEqns.f90:
+// 2190-2200), probed into the thermal CSR in the same order; the capacitance
+// alone goes into capMat on the same columns.  Then scaleThermalMPC, the
+// profile and the factor; the state starts at T = Tdot = 0.
+void orc_thermal_dyn_init(OrcModel* mp, double gamma, double dt, int literalD8) {
+    OrcModel& m = *mp;
+    const int N = m.numNodes;
+    const double cm = 1.0 / (gamma * dt);
+    m.thGamma = gamma;
+    m.thDt = dt;
+    std::fill(m.thermMat.begin(), m.thermMat.end(), 0.0);
+    std::fill(m.thermMatCols.begin(), m.thermMatCols.end(), 0);
+    m.capMat.assign(m.thermMat.size(), 0.0);
+    m.capCols.assign(m.thermMat.size(), 0);
+    double xyz[24], Kt[64], Mt[64];
+    for (int e = 0; e < m.numEls; ++e) {
+        orc_element_coords(&m, e, xyz);
+        orc_brick8_conductivity(xyz, &m.elCond[9 * (size_t)e], Kt);
+        orc_brick8_capacitance(xyz, m.elCap[e], Mt);
+        for (int a = 0; a < 8; ++a) {
+            int kGi = m.currentRank[m.elementList[8 * (size_t)e + a] - 1];
+            for (int b = 0; b < 8; ++b) {
+                int kGj = m.currentRank[m.elementList[8 * (size_t)e + b] - 1];
+                probe_add(m.thermMat, m.thermMatCols, m.thermMatRange, kGi, kGj, Kt[a + 8 * b] + cm * Mt[a + 8 * b],
+                          literalD8 != 0);
+                probe_add(m.capMat, m.capCols, m.thermMatRange, kGi, kGj, Mt[a + 8 * b], literalD8 != 0);
+            }
+        }
+    }
+    orc_scale_thermal_mpc(mp);
+    orc_build_thermal_profile(mp);
+    orc_factor_thermal(mp);
+    for (auto* v : {&m.nodeTemp, &m.prevTemp, &m.prevTdot, &m.nodeTdot, &m.delTemp}) v->assign(N, 0.0);
+}
+
+// One step (stepSolve's thermal branch): thermalLoad = -Rt(T) element by
+// element, Rt = K_e T_e + C_e Tdot_e with Tdot = (1 / gamma) ((T - Tp) / dt
+// - (1 - gamma) Tdp) per node (r_getElRt), then the constraint load
+// C^T (rhs - C T); GMRES from the previous increment; T += dT; Tdot and the
+// previous state (:791-803).  mode / rtol as orc_solve_elastic.
+void orc_thermal_dyn_step(OrcModel* mp, int numVecs, int mode, double rtol) {
+    OrcModel& m = *mp;
+    const int N = m.numNodes;
+    const double g = m.thGamma, dt = m.thDt;
+    std::vector<double> Td(N);
+    for (int i = 0; i < N; ++i)
+        Td[i] = (1.0 / g) * ((1.0 / dt) * (m.nodeTemp[i] - m.prevTemp[i]) - (1.0 - g) * m.prevTdot[i]);
+    std::fill(m.thermalLoad.begin(), m.thermalLoad.end(), 0.0);
+    double xyz[24], Kt[64], Mt[64];
+    for (int e = 0; e < m.numEls; ++e) {
+        orc_element_coords(&m, e, xyz);
+        orc_brick8_conductivity(xyz, &m.elCond[9 * (size_t)e], Kt);
+        orc_brick8_capacitance(xyz, m.elCap[e], Mt);
+        int r[8];
+        for (int a = 0; a < 8; ++a) r[a] = m.currentRank[m.elementList[8 * (size_t)e + a] - 1];
+        for (int a = 0; a < 8; ++a) {
+            double acc = 0.0;
+            for (int b = 0; b < 8; ++b) acc += Kt[a + 8 * b] * m.nodeTemp[r[b] - 1] + Mt[a + 8 * b] * Td[r[b] - 1];
+            m.thermalLoad[r[a] - 1] -= acc;
+        }
+    }
+    std::vector<double> t(m.thermMPCDim);
+    for (int i = 1; i <= m.thermMPCDim; ++i) {
+        double v = m.thermMPCRHS[i - 1];
+        for (int s = m.thermMPCMatRange[i - 1] + 1; s <= m.thermMPCMatRange[i]; ++s) {
+            int c = m.thermMPCMatCols[s - 1];
+            if (c != 0) v -= m.thermMPCMat[s - 1] * m.nodeTemp[c - 1];
+        }
+        t[i - 1] = v;
+    }
+    for (int i = 1; i <= m.thermMPCDim; ++i)
+        for (int s = m.thermMPCMatRange[i - 1] + 1; s <= m.thermMPCMatRange[i]; ++s) {
+            int c = m.thermMPCMatCols[s - 1];
+            if (c != 0) m.thermalLoad[c - 1] += m.thermMPCMat[s - 1] * t[i - 1];
+        }
+    if (numVecs <= 0) numVecs = orc_num_vecs(N);
+    int maxIt = mode == 0 ? N : 1000 * numVecs;
+    m.gmresCycles = orc_gmres_mode(m.delTemp.data(), m.thermMat.data(), m.thermMatCols.data(), m.thermMatRange.data(),
+                                   N, m.thermMPCMat.data(), m.thermMPCMatCols.data(), m.thermMPCMatRange.data(),
+                                   m.thermMPCDim, m.thermMatLT.data(), m.thermMatLTRange.data(), m.thermalLoad.data(),
+                                   numVecs, maxIt, mode, rtol, &m.gmresRes, &m.gmresSteps);
+    for (int i = 0; i < N; ++i) {
+        const double T = m.nodeTemp[i] + m.delTemp[i];
+        const double td = (1.0 / g) * ((1.0 / dt) * (T - m.prevTemp[i]) - (1.0 - g) * m.prevTdot[i]);
+        m.nodeTemp[i] = T;
+        m.nodeTdot[i] = td;
+        m.prevTemp[i] = T;
+        m.prevTdot[i] = td;
+    }
+}
+
 // Array getters for the Python side of the tests.
 const int* orc_geti(OrcModel* m, const char* name, long long* len) {
     std::string s(name);
@@ -1927,6 +2046,9 @@ double* orc_getd(OrcModel* m, const char* name, long long* len) {
     else if (s == "thermMatLT") v = &m->thermMatLT;
     else if (s == "thermalLoad") v = &m->thermalLoad;
     else if (s == "delTemp") v = &m->delTemp;
+    else if (s == "nodeTemp") v = &m->nodeTemp;
+    else if (s == "nodeTdot") v = &m->nodeTdot;
+    else if (s == "capMat") v = &m->capMat;
     else if (s == "thermMPCMat") v = &m->thermMPCMat;
     else if (s == "thermMPCRHS") v = &m->thermMPCRHS;
     else if (s == "elC") v = &m->elC;

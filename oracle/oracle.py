@@ -85,6 +85,10 @@ def lib():
         L.orc_dyn_residual_load.argtypes = [C.c_void_p, C.c_double]
         L.orc_dyn_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double]
         L.orc_brick8_conductivity.argtypes = [_dp, _dp, _dp]
+        L.orc_brick8_capacitance.argtypes = [_dp, C.c_double, _dp]
+        L.orc_set_spec_heat.argtypes = [C.c_void_p, _dp]
+        L.orc_thermal_dyn_init.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.orc_thermal_dyn_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
         L.orc_brick8_volume.restype = C.c_double
         L.orc_brick8_volume.argtypes = [_dp]
         L.orc_convert_ltri.argtypes = [_dp, _ip, C.c_longlong, _dp, _ip, _ip, C.c_int, _dp, _ip, _ip, C.c_int]
@@ -252,6 +256,8 @@ class OracleModel:
         self.h = L.orc_create(model.num_nodes, self._nodes, model.num_els, self._els, self._et)
         ep, den, cond = _f64(model.eprops.reshape(-1)), _f64(model.density), _f64(model.cond.reshape(-1))
         L.orc_set_properties(self.h, ep.ctypes.data, den.ctypes.data, cond.ctypes.data)
+        if getattr(model, "spec_heat", None) is not None:
+            L.orc_set_spec_heat(self.h, _f64(model.spec_heat))
         if model.mpc:
             eqn = _i32([t.eqn for t in model.mpc])
             dof = _i32([t.dof for t in model.mpc])
@@ -377,6 +383,24 @@ class OracleModel:
         L.orc_factor_thermal(self.h)
         L.orc_solve_thermal(self.h, int(num_vecs), int(mode), float(rtol))
         return self
+
+
+    def thermal_dynamic_init(self, gamma: float = 0.5, dt: float = 1.0, literal_d8: bool = True):
+        """The transient thermal set-up: K + C/(gamma dt) into the thermal CSR,
+        the capacitance C on the same columns (capMat), penalty scaling,
+        profile and factor; T = Tdot = 0."""
+        lib().orc_thermal_dyn_init(self.h, float(gamma), float(dt), 1 if literal_d8 else 0)
+        return self
+
+    def thermal_dynamic_step(self, num_vecs: int = 0, mode: int = 1, rtol: float = 1e-12):
+        lib().orc_thermal_dyn_step(self.h, int(num_vecs), int(mode), float(rtol))
+        return self
+
+
+def brick8_capacitance(xyz38, rhoc):
+    M = np.zeros(64)
+    lib().orc_brick8_capacitance(_f64(np.asarray(xyz38).reshape(3, 8).ravel(order="F")), float(rhoc), M)
+    return M.reshape(8, 8, order="F")
 
 
 # ---------------------------------------------------------------------------
