@@ -273,6 +273,90 @@ int astro_ctx_cs_batch_autotune(astro_ctx* ctx, int32_t* tiles, float* ms, int32
  * one of the kernel's tiles). */
 int astro_ctx_cs_batch_tile(astro_ctx* ctx, int32_t tile);
 
+/* ---- Design variables, objectives and the adjoint gradient (brick8) ------
+ * Design variables as the reference's readDesignVarInput leaves them
+ * (Working/AStrO_input.f90:1099-1408, Working/AStrO_globalData.f90): per
+ * variable a category and 1-based component, the element map elToDRange
+ * (numEls + 1 prefix offsets) / elToD (1-based variable ids) / elToCoef and
+ * the node map ndToDRange (numNodes + 1) / ndToD / ndToCoef.  A property is
+ * its model value plus sum(coef * d) over the variables that list the element
+ * or node (r_getMaterialStiffness, r_getMaterialThermCond, r_getMaterialDensity,
+ * r_getNodeCoord, r_getNodeDLoad, Working/AStrO_r_designPropertyFunctions.f90:
+ * 8-80, 129-160, 82-102, 945-991): the model passed to astro_ctx_create holds
+ * the properties at the current design, and derivatives are taken there by a
+ * complex step of size h on every property the variable moves.
+ * Categories: ASTRO_DV_MODULUS / _POISSON / _SHEAR ('elastic' modulus,
+ * poissonRatio, shearModulus; component 1..3 = E1..E3, nu12 nu13 nu23,
+ * G12 G13 G23), _DENSITY, _THERMAL_COND (component 1..6 = k11 k22 k33 k12 k13
+ * k23), _NODE_COORD (component 1..3), _NODE_LOAD ('nodeLoad' elastic,
+ * component 1..6; R = K u - F, so dR/dd = -coef at the node's DOF). */
+enum {
+    ASTRO_DV_MODULUS = 1, ASTRO_DV_POISSON = 2, ASTRO_DV_SHEAR = 3, ASTRO_DV_DENSITY = 4,
+    ASTRO_DV_THERMAL_COND = 5, ASTRO_DV_NODE_COORD = 6, ASTRO_DV_NODE_LOAD = 7
+};
+typedef struct {
+    int32_t numDV;
+    const int32_t* category;   /* numDV */
+    const int32_t* component;  /* numDV, 1-based */
+    const int32_t* elToDRange; /* numEls + 1, may be null (no element map) */
+    const int32_t* elToD;
+    const double* elToCoef;
+    const int32_t* ndToDRange; /* numNodes + 1, may be null (no node map) */
+    const int32_t* ndToD;
+    const double* ndToCoef;
+    double h;                  /* complex step (the reference's compStep is 1e-14i) */
+} astro_design;
+/* Sets (replaces) the context's design variables.  brick8 models. */
+int astro_ctx_set_design_vars(astro_ctx* ctx, const astro_design* design);
+/* dR/dd of the linear static residual R = K(d) u - F(d) at the current
+ * nodeDisp (getdRelasticdD, Working/AStrO_commandFunctions.f90:1172-1375,
+ * with each element scattering into its own DOFs, D9 corrected): dRdD is
+ * elMatDim x numDV column-major.  Element terms by complex step on the
+ * device, one thread per (element, variable) pair.  Synchronous. */
+int astro_ctx_dRdD_vars(astro_ctx* ctx, double* dRdD);
+/* dRt/dd of the thermal residual Rt = K_t(d) T at the current nodeTemp
+ * (getdRthermaldD, Working/AStrO_commandFunctions.f90:1016-1170):
+ * numNodes x numDV column-major (node rank order).  Needs the thermal system
+ * (astro_ctx_assemble_thermal).  Synchronous. */
+int astro_ctx_dRtdD_vars(astro_ctx* ctx, double* dRtdD);
+
+/* Objective terms (Working/AStrO_objective.f90:9-470; input
+ * Working/AStrO_input.f90:1410-1576): term value = coef * sum over the set of
+ * (q - tgt)^expo ('powerNorm'), coef * (sum q - tgt)^expo ('volumeIntegral')
+ * or coef * (sum q v / sum v - tgt)^expo ('volumeAverage'), v the element
+ * volume (2x2x2 Gauss).  q per category: displacement (component of nodeDisp
+ * at the set's nodes), temperature (nodeTemp), stress / strain (component of
+ * the Green-Lagrange strain, or C times it, at the element centre:
+ * r_getSolidStress, Working/AStrO_r_elementEqns.f90:924-976), strain energy
+ * (sum over the Gauss points of stress . strain / 2 times the weight;
+ * 'powerNorm' uses it per unit volume), mass (rho v), volume.  items: node ids
+ * (displacement, temperature) or element ids, 1-based, no repeats; tgt: numTgt
+ * = 0 ('none'), 1 ('single') or numItems ('vector'; the volume operators read
+ * its first entry). */
+enum {
+    ASTRO_OBJ_DISPLACEMENT = 1, ASTRO_OBJ_TEMPERATURE = 2, ASTRO_OBJ_STRESS = 3, ASTRO_OBJ_STRAIN = 4,
+    ASTRO_OBJ_STRAIN_ENERGY = 5, ASTRO_OBJ_MASS = 6, ASTRO_OBJ_VOLUME = 7
+};
+enum { ASTRO_OP_POWER_NORM = 1, ASTRO_OP_VOLUME_INTEGRAL = 2, ASTRO_OP_VOLUME_AVERAGE = 3 };
+typedef struct {
+    int32_t category, op, component;
+    double coef, expo;
+    int32_t numItems;
+    const int32_t* items;
+    int32_t numTgt;
+    const double* tgt;
+} astro_objective_term;
+/* getObjective: objVal[t] per term at the current nodeDisp / nodeTemp.
+ * Element terms on the device, one thread per element.  Synchronous. */
+int astro_ctx_objective(astro_ctx* ctx, int32_t numTerms, const astro_objective_term* terms, double* objVal);
+/* getTotaldLdD for a static step (Working/AStrO_commandFunctions.f90:
+ * 1377-1412): dL/dd = dObj/dd - lambda . dR/dd - lambda_t . dRt/dd with the
+ * adjoints (K + C^T C) lambda = dObj/du and the thermal twin
+ * (solveDynamicAdjoint, :917-1014; GMRES on the factored system of the last
+ * static / thermal solve).  dObj/du (getdObjdU) and the explicit dObj/dd
+ * (getdObjdD, complex step) on the device.  dLdD: numDV.  Synchronous. */
+int astro_ctx_objective_gradient(astro_ctx* ctx, int32_t numTerms, const astro_objective_term* terms, double* dLdD);
+
 /* Thermal twins (Working/AStrO_commandFunctions.f90:16-74, 229-258, 689-703,
  * 763-769). */
 int astro_ctx_assemble_thermal(astro_ctx* ctx);

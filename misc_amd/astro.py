@@ -43,7 +43,16 @@ EXPORTS = [
     "astro_gnn_num_fields", "astro_gnn_get_params", "astro_gnn_set_params", "astro_ctx_gnn_collect",
     "astro_gnn_train", "astro_gnn_loss_grad", "astro_gnn_field_size", "astro_gnn_get_field",
     "astro_gnn_field_blocks", "astro_ctx_gnn_attach", "astro_ctx_gnn_detach", "astro_ctx_gnn_get",
+    "astro_ctx_set_design_vars", "astro_ctx_dRdD_vars", "astro_ctx_dRtdD_vars", "astro_ctx_objective",
+    "astro_ctx_objective_gradient",
 ]
+
+# Design-variable categories and objective terms (include/astro_abi.h).
+DV_CATEGORY = {"modulus": 1, "poissonRatio": 2, "shearModulus": 3, "density": 4, "thermalCond": 5, "nodeCoord": 6,
+               "nodeLoad": 7}
+OBJ_CATEGORY = {"displacement": 1, "temperature": 2, "stress": 3, "strain": 4, "strainEnergy": 5, "mass": 6,
+                "volume": 7}
+OBJ_OPERATOR = {"powerNorm": 1, "volumeIntegral": 2, "volumeAverage": 3}
 
 
 class AstroError(RuntimeError):
@@ -68,6 +77,86 @@ class _Model(C.Structure):
                 ("mpcRHS", C.c_void_p), ("numLoads", C.c_int32), ("loadNode", C.c_void_p),
                 ("loadVal", C.c_void_p), ("numLayers", C.c_int32), ("layers", C.c_void_p), ("zOffset", C.c_void_p),
                 ("orient", C.c_void_p), ("specHeat", C.c_void_p)]
+
+
+class _Design(C.Structure):
+    _fields_ = [("numDV", C.c_int32), ("category", C.c_void_p), ("component", C.c_void_p),
+                ("elToDRange", C.c_void_p), ("elToD", C.c_void_p), ("elToCoef", C.c_void_p),
+                ("ndToDRange", C.c_void_p), ("ndToD", C.c_void_p), ("ndToCoef", C.c_void_p), ("h", C.c_double)]
+
+
+class _ObjTerm(C.Structure):
+    _fields_ = [("category", C.c_int32), ("op", C.c_int32), ("component", C.c_int32), ("coef", C.c_double),
+                ("expo", C.c_double), ("numItems", C.c_int32), ("items", C.c_void_p), ("numTgt", C.c_int32),
+                ("tgt", C.c_void_p)]
+
+
+@dataclass
+class ObjectiveTerm:
+    """One objective term as the reference's objective file states it
+    (Working/objectiveTemplate.yaml; Working/AStrO_objective.f90:9-470):
+    category ('displacement', 'temperature', 'stress', 'strain',
+    'strainEnergy', 'mass', 'volume'), operator ('powerNorm',
+    'volumeIntegral', 'volumeAverage'), 1-based component, coefficient,
+    exponent, the 1-based node or element ids of its set, and the target
+    (None: 'none'; a scalar: 'single'; one value per item: 'vector')."""
+
+    category: str
+    items: object
+    operator: str = "powerNorm"
+    component: int = 1
+    coef: float = 1.0
+    expo: float = 1.0
+    target: object = None
+
+
+@dataclass
+class DesignVariables:
+    """Design variables as readDesignVarInput leaves them
+    (Working/AStrO_input.f90:1099-1408): per variable its category (a key of
+    DV_CATEGORY) and 1-based component, the element map (elToDRange of
+    numEls + 1 offsets, 1-based variable ids elToD, coefficients elToCoef)
+    and the node map (ndToDRange, ndToD, ndToCoef)."""
+
+    category: list
+    component: list
+    elToDRange: object = None
+    elToD: object = None
+    elToCoef: object = None
+    ndToDRange: object = None
+    ndToD: object = None
+    ndToCoef: object = None
+
+    @property
+    def num(self):
+        return len(self.category)
+
+
+def _marshal_terms(terms):
+    keep = []
+
+    def ptr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    arr = (_ObjTerm * len(terms))()
+    for t, T in zip(arr, terms):
+        items = np.ascontiguousarray(T.items, dtype=np.int32).reshape(-1)
+        t.category = OBJ_CATEGORY[T.category]
+        t.op = OBJ_OPERATOR[T.operator]
+        t.component = int(T.component)
+        t.coef = float(T.coef)
+        t.expo = float(T.expo)
+        t.numItems = int(items.size)
+        t.items = ptr(items, np.int32)
+        if T.target is None:
+            t.numTgt = 0
+        else:
+            tg = np.atleast_1d(np.asarray(T.target, dtype=np.float64))
+            t.numTgt = int(tg.size)
+            t.tgt = ptr(tg, np.float64)
+    return arr, keep
 
 
 class _Options(C.Structure):
@@ -151,6 +240,11 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_gnn_attach.argtypes = [C.c_void_p, C.c_void_p]
     L.astro_ctx_gnn_detach.argtypes = [C.c_void_p]
     L.astro_ctx_gnn_get.argtypes = [C.c_void_p, C.c_char_p, _dp, _i64]
+    L.astro_ctx_set_design_vars.argtypes = [C.c_void_p, C.POINTER(_Design)]
+    L.astro_ctx_dRdD_vars.argtypes = [C.c_void_p, _dp]
+    L.astro_ctx_dRtdD_vars.argtypes = [C.c_void_p, _dp]
+    L.astro_ctx_objective.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, _dp]
+    L.astro_ctx_objective_gradient.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, _dp]
     _lib = L
     return L
 
@@ -479,6 +573,62 @@ class Context:
         g = np.zeros(self.n_dv)
         _check(load_library().astro_ctx_compliance_gradient(self.h, g))
         return g
+
+    def set_design_vars(self, dv: "DesignVariables", h: float = 1e-30):
+        """Design variables of the brick8 path (material constants, density,
+        conductivity, node coordinates, node loads); derivatives by complex
+        step h at the model's current properties."""
+        keep = []
+
+        def ptr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        D = _Design()
+        D.numDV = dv.num
+        D.category = ptr([DV_CATEGORY[c] if isinstance(c, str) else int(c) for c in dv.category], np.int32)
+        D.component = ptr(dv.component, np.int32)
+        D.elToDRange = ptr(dv.elToDRange, np.int32)
+        D.elToD = ptr(dv.elToD, np.int32)
+        D.elToCoef = ptr(dv.elToCoef, np.float64)
+        D.ndToDRange = ptr(dv.ndToDRange, np.int32)
+        D.ndToD = ptr(dv.ndToD, np.int32)
+        D.ndToCoef = ptr(dv.ndToCoef, np.float64)
+        D.h = float(h)
+        _check(load_library().astro_ctx_set_design_vars(self.h, C.byref(D)))
+        self.n_dv = dv.num
+
+    def dRdD_vars(self):
+        """dR/dd (elMatDim x numDV) of the linear static residual at nodeDisp."""
+        n = self.scalar("elMatDim")
+        out = np.zeros(n * self.n_dv)
+        _check(load_library().astro_ctx_dRdD_vars(self.h, out))
+        return out.reshape(self.n_dv, n).T
+
+    def dRtdD_vars(self):
+        """dRt/dd (numNodes x numDV, rank order) of the thermal residual at nodeTemp."""
+        n = self.model.num_nodes
+        out = np.zeros(n * self.n_dv)
+        _check(load_library().astro_ctx_dRtdD_vars(self.h, out))
+        return out.reshape(self.n_dv, n).T
+
+    def objective(self, terms):
+        """Value of each ObjectiveTerm at the current state."""
+        arr, keep = _marshal_terms(terms)
+        out = np.zeros(len(terms))
+        _check(load_library().astro_ctx_objective(self.h, len(terms), C.cast(arr, C.c_void_p), out))
+        return out
+
+    def objective_gradient(self, terms):
+        """dL/dd of the summed terms by the adjoint (after static_solve and,
+        for temperature terms, thermal_solve)."""
+        arr, keep = _marshal_terms(terms)
+        out = np.zeros(self.n_dv)
+        _check(load_library().astro_ctx_objective_gradient(self.h, len(terms), C.cast(arr, C.c_void_p), out))
+        return out
 
     def thermal_dynamic_init(self, gamma: float = 0.5, dt: float = 1.0):
         """Transient conduction set-up: tangent K + C/(gamma dt), penalty

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libastro_hip.so")
-SOURCES = ["astro_kernels.hip", "astro_abi.hip", "astro_gnn.hip", "astro_batch.hip", "astro_prep.cpp",
+SOURCES = ["astro_kernels.hip", "astro_abi.hip", "astro_gnn.hip", "astro_batch.hip", "astro_design.hip", "astro_prep.cpp",
            "astro_shell.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

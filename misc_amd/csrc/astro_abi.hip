@@ -15,6 +15,7 @@
 
 #include "../../include/astro_abi.h"
 #include "astro_batch.h"
+#include "astro_design.h"
 #include "astro_gnn.h"
 #include "astro_internal.h"
 #include "astro_kernels.h"
@@ -802,6 +803,27 @@ struct astro_ctx {
     } csb;
     DBuf<double> csbKim;
     DBuf<int> csbKimDof;
+    // design variables of astro_ctx_set_design_vars (brick8): the (element,
+    // variable) pairs sorted by element colour, then variable, then element,
+    // with their direction records; per element its pairs; the node-load
+    // entries (0-based DOF, coefficient, variable); the adjoint vectors
+    struct DesignVars {
+        bool ready = false;
+        int nDV = 0;
+        double h = 1e-30;
+        std::vector<int> cat, comp;
+        std::vector<int> pairEl, pairDV, colorPairStart, elPairStart, elPairList;
+        std::vector<int> loadDof, loadDV;
+        std::vector<double> loadCoef;
+        DBuf<int> dPairEl, dPairDV;
+        DBuf<double> dDir, dEprops;
+    } dvs;
+    std::vector<double> hEprops;         // per element engineering constants as created
+    std::vector<int> hConn;              // elementList(8, numEls) as created
+    std::vector<int> elColor;            // colour of each element
+    DBuf<double> adjU, adjT, dLdu, dLdt;
+    double csrMassFactor = 0.0;          // massFactor of the last elastic assembly
+    bool thermalSteady = false;          // the thermal CSR holds the conductivity alone
     DBuf<int> conn, elMat, rank, perm;
     DBuf<uint8_t> elPos, thPos;
     std::vector<int> colorStart;
@@ -978,6 +1000,11 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
         c->hZOff.assign(E, 0.0);
         if (m->zOffset) c->hZOff.assign(m->zOffset, m->zOffset + E);
     }
+    if (m->eprops) c->hEprops.assign(m->eprops, m->eprops + 9 * (size_t)E);
+    c->hConn.assign(m->elementList, m->elementList + 8 * (size_t)E);
+    c->elColor.assign(E, 0);
+    for (size_t k = 0; k + 1 < p.colorStart.size(); ++k)
+        for (int q = p.colorStart[k]; q < p.colorStart[k + 1]; ++q) c->elColor[p.colorPerm[q]] = (int)k;
     ASTRO_TRY(c->matC.upload(mats, s));
     ASTRO_TRY(c->elMat.upload(elMat, s));
     ASTRO_TRY(c->elScale.upload(scale, s));
@@ -1306,6 +1333,7 @@ int astro_ctx_set_f64(astro_ctx* c, const char* name, const double* host, int64_
 int astro_ctx_assemble_elastic(astro_ctx* c, double massFactor) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     const int64_t nnz = (int64_t)c->prep.elMatCols.size();
+    c->csrMassFactor = massFactor;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, c->s, c->csrVal.p, nnz, 0.0);
     const int nS = c->prep.numSolidNds;
     if (c->prep.shellMesh) {
@@ -1889,6 +1917,488 @@ int astro_ctx_compliance_gradient(astro_ctx* c, double* grad) {
 }
 
 // ---------------------------------------------------------------------------
+// Design variables, objective terms and the adjoint gradient (brick8).
+//
+// set_design_vars turns the reference's maps (elToD over elements, ndToD over
+// nodes) into (element, variable) pairs: for every element, each variable
+// that moves one of its properties or node coordinates, with a direction
+// record holding d(property)/d(variable) (the coefficients, summed where a
+// variable is listed twice).  Pairs are sorted by element colour, then
+// variable, then element, so that one launch per colour can add pair
+// contributions into a dense dR/dd without two writers on one entry.
+int astro_ctx_set_design_vars(astro_ctx* c, const astro_design* d) {
+    if (!c || !d) return fail(ASTRO_ERR_ARG, "null argument");
+    if (c->prep.shellMesh)
+        return fail(ASTRO_ERR_UNSUPPORTED,
+                    "astro_ctx_set_design_vars covers brick8 models (shell4 layer thickness: astro_ctx_set_design)");
+    if (c->hEprops.empty()) return fail(ASTRO_ERR_ARG, "the model carries no engineering constants");
+    const int nDV = d->numDV;
+    if (nDV < 1 || !d->category || !d->component || !(d->h > 0.0))
+        return fail(ASTRO_ERR_ARG, "bad design-variable count, arrays or step");
+    if ((d->elToDRange && (!d->elToD)) || (d->ndToDRange && !d->ndToD))
+        return fail(ASTRO_ERR_ARG, "a design map has a range array but no variable ids");
+    const int E = c->prep.numEls, N = c->prep.numNodes;
+    auto& V = c->dvs;
+    V.ready = false;
+    V.cat.assign(d->category, d->category + nDV);
+    V.comp.assign(d->component, d->component + nDV);
+    for (int k = 0; k < nDV; ++k) {
+        int maxc = 0;
+        switch (V.cat[k]) {
+            case ASTRO_DV_MODULUS:
+            case ASTRO_DV_POISSON:
+            case ASTRO_DV_SHEAR:
+            case ASTRO_DV_NODE_COORD: maxc = 3; break;
+            case ASTRO_DV_THERMAL_COND:
+            case ASTRO_DV_NODE_LOAD: maxc = 6; break;
+            case ASTRO_DV_DENSITY: maxc = INT_MAX; break;
+            default: return fail(ASTRO_ERR_ARG, "unknown design-variable category " + std::to_string(V.cat[k]));
+        }
+        if (V.cat[k] != ASTRO_DV_DENSITY && (V.comp[k] < 1 || V.comp[k] > maxc))
+            return fail(ASTRO_ERR_ARG, "design variable " + std::to_string(k + 1) + ": component out of range");
+    }
+    if (d->elToDRange && (d->elToDRange[0] != 0 || d->elToDRange[E] < d->elToDRange[0]))
+        return fail(ASTRO_ERR_ARG, "elToDRange must start at 0 and not decrease");
+    if (d->ndToDRange && (d->ndToDRange[0] != 0 || d->ndToDRange[N] < d->ndToDRange[0]))
+        return fail(ASTRO_ERR_ARG, "ndToDRange must start at 0 and not decrease");
+    std::vector<int> slot(nDV, -1);
+    std::vector<std::pair<int, std::vector<double>>> loc;
+    struct Rec {
+        int color, k, e;
+        size_t at;
+    };
+    std::vector<Rec> recs;
+    std::vector<double> dirs;  // DVD_STRIDE per record, in discovery order
+    for (int e = 0; e < E; ++e) {
+        loc.clear();
+        auto get = [&](int k) -> double* {
+            if (slot[k] < 0) {
+                slot[k] = (int)loc.size();
+                loc.emplace_back(k, std::vector<double>(DVD_STRIDE, 0.0));
+            }
+            return loc[slot[k]].second.data();
+        };
+        if (d->elToDRange) {
+            for (int j = d->elToDRange[e]; j < d->elToDRange[e + 1]; ++j) {
+                const int k = d->elToD[j] - 1;
+                if (k < 0 || k >= nDV) return fail(ASTRO_ERR_ARG, "elToD holds a variable id out of range");
+                const double cf = d->elToCoef ? d->elToCoef[j] : 1.0;
+                const int cp = V.comp[k] - 1;
+                switch (V.cat[k]) {
+                    case ASTRO_DV_MODULUS: get(k)[DVD_EP + cp] += cf; break;
+                    case ASTRO_DV_POISSON: get(k)[DVD_EP + 3 + cp] += cf; break;
+                    case ASTRO_DV_SHEAR: get(k)[DVD_EP + 6 + cp] += cf; break;
+                    case ASTRO_DV_DENSITY: get(k)[DVD_RHO] += cf; break;
+                    case ASTRO_DV_THERMAL_COND: get(k)[DVD_COND + cp] += cf; break;
+                    default: break;  // node categories act through the node map only
+                }
+            }
+        }
+        if (d->ndToDRange) {
+            for (int a = 0; a < 8; ++a) {
+                const int nd = c->hConn[8 * (size_t)e + a] - 1;
+                for (int j = d->ndToDRange[nd]; j < d->ndToDRange[nd + 1]; ++j) {
+                    const int k = d->ndToD[j] - 1;
+                    if (k < 0 || k >= nDV) return fail(ASTRO_ERR_ARG, "ndToD holds a variable id out of range");
+                    if (V.cat[k] == ASTRO_DV_NODE_COORD)
+                        get(k)[DVD_X + 3 * a + V.comp[k] - 1] += d->ndToCoef ? d->ndToCoef[j] : 1.0;
+                }
+            }
+        }
+        for (auto& kv : loc) {
+            slot[kv.first] = -1;
+            bool any = false;
+            for (double x : kv.second) any = any || x != 0.0;
+            if (!any) continue;
+            recs.push_back({c->elColor[e], kv.first, e, dirs.size()});
+            dirs.insert(dirs.end(), kv.second.begin(), kv.second.end());
+        }
+    }
+    std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+        if (a.color != b.color) return a.color < b.color;
+        if (a.k != b.k) return a.k < b.k;
+        return a.e < b.e;
+    });
+    const int nP = (int)recs.size();
+    const int nColors = (int)c->colorStart.size() - 1;
+    V.pairEl.resize(nP);
+    V.pairDV.resize(nP);
+    V.colorPairStart.assign(nColors + 1, 0);
+    std::vector<double> dsorted((size_t)nP * DVD_STRIDE);
+    for (int p = 0; p < nP; ++p) {
+        V.pairEl[p] = recs[p].e;
+        V.pairDV[p] = recs[p].k;
+        V.colorPairStart[recs[p].color + 1]++;
+        std::copy(dirs.begin() + recs[p].at, dirs.begin() + recs[p].at + DVD_STRIDE,
+                  dsorted.begin() + (size_t)p * DVD_STRIDE);
+    }
+    for (int q = 0; q < nColors; ++q) V.colorPairStart[q + 1] += V.colorPairStart[q];
+    V.elPairStart.assign(E + 1, 0);
+    for (int p = 0; p < nP; ++p) V.elPairStart[V.pairEl[p] + 1]++;
+    for (int e = 0; e < E; ++e) V.elPairStart[e + 1] += V.elPairStart[e];
+    V.elPairList.assign(nP, 0);
+    {
+        std::vector<int> fill(V.elPairStart.begin(), V.elPairStart.end() - 1);
+        for (int p = 0; p < nP; ++p) V.elPairList[fill[V.pairEl[p]]++] = p;
+    }
+    // node loads: R = K u - F(d), F(dof) += coef d (r_getNodeDLoad)
+    V.loadDof.clear();
+    V.loadDV.clear();
+    V.loadCoef.clear();
+    if (d->ndToDRange) {
+        for (int nd = 0; nd < N; ++nd)
+            for (int j = d->ndToDRange[nd]; j < d->ndToDRange[nd + 1]; ++j) {
+                const int k = d->ndToD[j] - 1;
+                if (V.cat[k] != ASTRO_DV_NODE_LOAD) continue;
+                const int dof = c->prep.nDofIndex[6 * (size_t)nd + V.comp[k] - 1];
+                if (dof <= 0) continue;
+                V.loadDof.push_back(dof - 1);
+                V.loadDV.push_back(k);
+                V.loadCoef.push_back(d->ndToCoef ? d->ndToCoef[j] : 1.0);
+            }
+    }
+    V.nDV = nDV;
+    V.h = d->h;
+    ASTRO_TRY(V.dPairEl.upload(V.pairEl, c->s));
+    ASTRO_TRY(V.dPairDV.upload(V.pairDV, c->s));
+    ASTRO_TRY(V.dDir.upload(dsorted, c->s));
+    ASTRO_TRY(V.dEprops.upload(c->hEprops, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    V.ready = true;
+    return ASTRO_OK;
+}
+
+static int dv_require(astro_ctx* c) {
+    if (!c) return fail(ASTRO_ERR_ARG, "null context");
+    if (!c->dvs.ready) return fail(ASTRO_ERR_ARG, "astro_ctx_set_design_vars has not been called");
+    return ASTRO_OK;
+}
+
+// Pair products of the elastic (force) or thermal (flux) kernel: with col,
+// the dense n x nDV derivative (one launch per colour); with lam, per-pair
+// lambda . f into out (one launch).
+static int dv_pairs_launch(astro_ctx* c, bool thermal, const double* state, const double* lam, double* out,
+                           double* col, int64_t n) {
+    auto& V = c->dvs;
+    auto launch = [&](int p0, int cnt) {
+        if (cnt <= 0) return;
+        if (thermal)
+            hipLaunchKernelGGL(k_dv_pair_flux, dim3(grid_for(cnt, 128, INT_MAX)), dim3(128), 0, c->s, cnt,
+                               V.dPairEl.p + p0, V.dPairDV.p + p0, V.dDir.p + (size_t)p0 * DVD_STRIDE, c->conn.p,
+                               c->X.p, c->elCond.p, c->rank.p, state, lam, V.h, out ? out + p0 : nullptr, col, n);
+        else
+            hipLaunchKernelGGL(k_dv_pair_force, dim3(grid_for(cnt, 128, INT_MAX)), dim3(128), 0, c->s, cnt,
+                               V.dPairEl.p + p0, V.dPairDV.p + p0, V.dDir.p + (size_t)p0 * DVD_STRIDE, c->conn.p,
+                               c->X.p, V.dEprops.p, c->rank.p, c->prep.numSolidNds, state, lam, V.h,
+                               out ? out + p0 : nullptr, col, n);
+    };
+    if (col) {
+        for (size_t q = 0; q + 1 < V.colorPairStart.size(); ++q)
+            launch(V.colorPairStart[q], V.colorPairStart[q + 1] - V.colorPairStart[q]);
+    } else {
+        launch(0, (int)V.pairEl.size());
+    }
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
+int astro_ctx_dRdD_vars(astro_ctx* c, double* dRdD) {
+    ASTRO_TRY(dv_require(c));
+    if (!dRdD) return fail(ASTRO_ERR_ARG, "null argument");
+    const int64_t n = c->prep.elMatDim, cols = c->dvs.nDV;
+    DBuf<double> D;
+    ASTRO_TRY(D.alloc((size_t)(n * cols)));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n * cols, 256, 8192)), dim3(256), 0, c->s, D.p, n * cols, 0.0);
+    ASTRO_TRY(dv_pairs_launch(c, false, c->nodeDisp.p, nullptr, nullptr, D.p, n));
+    HIP_TRY(hipMemcpyAsync(dRdD, D.p, sizeof(double) * n * cols, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    for (size_t i = 0; i < c->dvs.loadDof.size(); ++i)
+        dRdD[(int64_t)c->dvs.loadDV[i] * n + c->dvs.loadDof[i]] -= c->dvs.loadCoef[i];
+    return ASTRO_OK;
+}
+
+int astro_ctx_dRtdD_vars(astro_ctx* c, double* dRtdD) {
+    ASTRO_TRY(dv_require(c));
+    if (!dRtdD) return fail(ASTRO_ERR_ARG, "null argument");
+    if (!c->thermalReady) return fail(ASTRO_ERR_ARG, "the thermal system has not been assembled");
+    const int64_t n = c->prep.numNodes, cols = c->dvs.nDV;
+    DBuf<double> D;
+    ASTRO_TRY(D.alloc((size_t)(n * cols)));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n * cols, 256, 8192)), dim3(256), 0, c->s, D.p, n * cols, 0.0);
+    ASTRO_TRY(dv_pairs_launch(c, true, c->nodeTemp.p, nullptr, nullptr, D.p, n));
+    HIP_TRY(hipMemcpyAsync(dRtdD, D.p, sizeof(double) * n * cols, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
+// One objective term evaluated on the device: its value and, with grad, its
+// contributions to dL/du (or dL/dt) on the device and to the explicit dL/dd
+// (host accumulation in pair order, so the result is deterministic).
+static int obj_term(astro_ctx* c, const astro_objective_term& t, double& value, bool grad, std::vector<double>* dLdD,
+                    bool& usesU, bool& usesT) {
+    const int E = c->prep.numEls, N = c->prep.numNodes;
+    const int ni = t.numItems;
+    if (ni < 1 || !t.items) return fail(ASTRO_ERR_ARG, "an objective term needs a non-empty set");
+    if (t.numTgt != 0 && t.numTgt != 1 && t.numTgt != ni)
+        return fail(ASTRO_ERR_ARG, "objective target count must be 0, 1 or the set size");
+    if (t.numTgt > 0 && !t.tgt) return fail(ASTRO_ERR_ARG, "objective targets missing");
+    const int tgtMode = t.numTgt == 0 ? 0 : (t.numTgt == 1 ? 1 : 2);
+    auto tgtAt = [&](int i) { return tgtMode == 0 ? 0.0 : (tgtMode == 1 ? t.tgt[0] : t.tgt[i]); };
+    hipStream_t s = c->s;
+    value = 0.0;
+    if (t.category == ASTRO_OBJ_DISPLACEMENT || t.category == ASTRO_OBJ_TEMPERATURE) {
+        if (t.op != ASTRO_OP_POWER_NORM) return fail(ASTRO_ERR_ARG, "node terms take the powerNorm operator");
+        const bool temp = t.category == ASTRO_OBJ_TEMPERATURE;
+        if (temp && !c->thermalReady) return fail(ASTRO_ERR_ARG, "temperature term without a thermal system");
+        if (!temp && (t.component < 1 || t.component > 6)) return fail(ASTRO_ERR_ARG, "displacement component 1..6");
+        std::vector<int> dofs(ni);
+        std::vector<char> seen(N, 0);
+        for (int i = 0; i < ni; ++i) {
+            const int nd = t.items[i] - 1;
+            if (nd < 0 || nd >= N) return fail(ASTRO_ERR_ARG, "objective node out of range");
+            if (seen[nd]) return fail(ASTRO_ERR_ARG, "an objective node set lists a node twice");
+            seen[nd] = 1;
+            if (temp) {
+                dofs[i] = c->prep.currentRank[nd] - 1;
+            } else {
+                const int d1 = c->prep.nDofIndex[6 * (size_t)nd + t.component - 1];
+                if (d1 <= 0) return fail(ASTRO_ERR_ARG, "objective node has no such displacement component");
+                dofs[i] = d1 - 1;
+            }
+        }
+        DBuf<int> dd;
+        DBuf<double> tg, val;
+        ASTRO_TRY(dd.upload(dofs, s));
+        if (tgtMode) ASTRO_TRY(tg.upload(t.tgt, (size_t)t.numTgt, s));
+        ASTRO_TRY(val.alloc(ni));
+        hipLaunchKernelGGL(k_obj_nodes, dim3(grid_for(ni, 256, INT_MAX)), dim3(256), 0, s, ni, dd.p, tg.p, tgtMode,
+                           t.coef, t.expo, temp ? c->nodeTemp.p : c->nodeDisp.p, val.p,
+                           grad ? (temp ? c->dLdt.p : c->dLdu.p) : nullptr);
+        HIP_TRY(hipGetLastError());
+        std::vector<double> hv(ni);
+        HIP_TRY(hipMemcpyAsync(hv.data(), val.p, sizeof(double) * ni, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        double sum = 0.0;
+        for (double x : hv) sum += x;
+        value = t.coef * sum;
+        (temp ? usesT : usesU) = true;
+        return ASTRO_OK;
+    }
+    const int q = t.category;
+    if (q != ASTRO_OBJ_STRESS && q != ASTRO_OBJ_STRAIN && q != ASTRO_OBJ_STRAIN_ENERGY && q != ASTRO_OBJ_MASS &&
+        q != ASTRO_OBJ_VOLUME)
+        return fail(ASTRO_ERR_ARG, "unknown objective category " + std::to_string(q));
+    if ((q == ASTRO_OBJ_STRESS || q == ASTRO_OBJ_STRAIN) && (t.component < 1 || t.component > 6))
+        return fail(ASTRO_ERR_ARG, "stress / strain component 1..6");
+    const bool pn = t.op == ASTRO_OP_POWER_NORM, vi = t.op == ASTRO_OP_VOLUME_INTEGRAL,
+               va = t.op == ASTRO_OP_VOLUME_AVERAGE;
+    const bool opOk = (q == ASTRO_OBJ_MASS || q == ASTRO_OBJ_VOLUME) ? vi
+                      : (q == ASTRO_OBJ_STRAIN_ENERGY)              ? (pn || vi || va)
+                                                                    : (pn || va);
+    if (!opOk) return fail(ASTRO_ERR_ARG, "operator not defined for this objective category");
+    std::vector<int> items(ni);
+    std::vector<char> seen(E, 0);
+    for (int i = 0; i < ni; ++i) {
+        const int e = t.items[i] - 1;
+        if (e < 0 || e >= E) return fail(ASTRO_ERR_ARG, "objective element out of range");
+        if (seen[e]) return fail(ASTRO_ERR_ARG, "an objective element set lists an element twice");
+        seen[e] = 1;
+        items[i] = e;
+    }
+    const bool needU = q != ASTRO_OBJ_MASS && q != ASTRO_OBJ_VOLUME;
+    DBuf<int> di;
+    DBuf<double> dq, dv, dqdu;
+    ASTRO_TRY(di.upload(items, s));
+    ASTRO_TRY(dq.alloc(ni));
+    ASTRO_TRY(dv.alloc(ni));
+    if (grad && needU) ASTRO_TRY(dqdu.alloc(24 * (size_t)ni));
+    hipLaunchKernelGGL(k_obj_elements, dim3(grid_for(ni, 128, INT_MAX)), dim3(128), 0, s, q, t.component, ni, di.p,
+                       c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p,
+                       c->prep.numSolidNds, c->nodeDisp.p, dq.p, dv.p, grad && needU ? dqdu.p : nullptr);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> hq(ni), hv(ni);
+    HIP_TRY(hipMemcpyAsync(hq.data(), dq.p, sizeof(double) * ni, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hv.data(), dv.p, sizeof(double) * ni, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const bool perVol = pn && q == ASTRO_OBJ_STRAIN_ENERGY;  // strain energy per unit volume
+    double S = 0.0, Vt = 0.0;
+    std::vector<double> w(ni, 0.0);
+    if (pn) {
+        for (int i = 0; i < ni; ++i) {
+            const double x = perVol ? hq[i] / hv[i] : hq[i];
+            S += std::pow(x - tgtAt(i), t.expo);
+            w[i] = t.coef * t.expo * std::pow(x - tgtAt(i), t.expo - 1.0) / (perVol ? hv[i] : 1.0);
+        }
+        value = t.coef * S;
+    } else if (vi) {
+        for (int i = 0; i < ni; ++i) S += hq[i];
+        value = t.coef * std::pow(S - tgtAt(0), t.expo);
+        const double f = t.coef * t.expo * std::pow(S - tgtAt(0), t.expo - 1.0);
+        for (int i = 0; i < ni; ++i) w[i] = f;
+    } else {
+        for (int i = 0; i < ni; ++i) {
+            S += hq[i] * hv[i];
+            Vt += hv[i];
+        }
+        value = t.coef * std::pow(S / Vt - tgtAt(0), t.expo);
+        const double f = t.coef * t.expo * std::pow(S / Vt - tgtAt(0), t.expo - 1.0);
+        for (int i = 0; i < ni; ++i) w[i] = f * hv[i] / Vt;
+    }
+    if (!grad) return ASTRO_OK;
+    if (needU) {
+        usesU = true;
+        // dL/du += w_i dq_i/du, one launch per element colour
+        std::vector<int> slots;
+        std::vector<int> start(c->colorStart.size(), 0);
+        for (int i = 0; i < ni; ++i) start[c->elColor[items[i]] + 1]++;
+        for (size_t k = 0; k + 1 < start.size(); ++k) start[k + 1] += start[k];
+        slots.assign(ni, 0);
+        {
+            std::vector<int> fill(start.begin(), start.end() - 1);
+            for (int i = 0; i < ni; ++i) slots[fill[c->elColor[items[i]]]++] = i;
+        }
+        DBuf<int> dslot;
+        DBuf<double> dw;
+        ASTRO_TRY(dslot.upload(slots, s));
+        ASTRO_TRY(dw.upload(w, s));
+        for (size_t k = 0; k + 1 < start.size(); ++k) {
+            const int cnt = start[k + 1] - start[k];
+            if (cnt <= 0) continue;
+            hipLaunchKernelGGL(k_obj_scatter_du, dim3(grid_for((int64_t)cnt * 24, 256, INT_MAX)), dim3(256), 0, s,
+                               cnt, dslot.p + start[k], di.p, dw.p, dqdu.p, c->conn.p, c->rank.p, c->prep.numSolidNds,
+                               c->dLdu.p);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));  // the launches read dslot and dw, freed at scope end
+    }
+    if (!c->dvs.ready || !dLdD) {
+        HIP_TRY(hipStreamSynchronize(s));
+        return ASTRO_OK;
+    }
+    // explicit dL/dd: the pairs of the set's elements, in item order
+    auto& V = c->dvs;
+    std::vector<int> pidx, pitem;
+    for (int i = 0; i < ni; ++i)
+        for (int j = V.elPairStart[items[i]]; j < V.elPairStart[items[i] + 1]; ++j) {
+            pidx.push_back(V.elPairList[j]);
+            pitem.push_back(i);
+        }
+    const int np = (int)pidx.size();
+    if (np > 0) {
+        DBuf<int> dp;
+        DBuf<double> ddq, ddv;
+        ASTRO_TRY(dp.upload(pidx, s));
+        ASTRO_TRY(ddq.alloc(np));
+        ASTRO_TRY(ddv.alloc(np));
+        hipLaunchKernelGGL(k_obj_pairs, dim3(grid_for(np, 128, INT_MAX)), dim3(128), 0, s, q, t.component, np, dp.p,
+                           V.dPairEl.p, V.dDir.p, c->conn.p, c->X.p, V.dEprops.p, c->elRho.p, c->rank.p,
+                           c->prep.numSolidNds, c->nodeDisp.p, V.h, ddq.p, ddv.p);
+        HIP_TRY(hipGetLastError());
+        std::vector<double> gq(np), gv(np);
+        HIP_TRY(hipMemcpyAsync(gq.data(), ddq.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(gv.data(), ddv.p, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        std::vector<double>& g = *dLdD;
+        if (pn) {
+            for (int j = 0; j < np; ++j) {
+                const int i = pitem[j], k = V.pairDV[pidx[j]];
+                // w_i already carries 1 / v for the per-volume strain energy
+                const double dx = perVol ? (gq[j] - hq[i] / hv[i] * gv[j]) : gq[j];
+                g[k] += w[i] * dx;
+            }
+        } else if (vi) {
+            for (int j = 0; j < np; ++j) g[V.pairDV[pidx[j]]] += w[pitem[j]] * gq[j];
+        } else {
+            const double f = t.coef * t.expo * std::pow(S / Vt - tgtAt(0), t.expo - 1.0);
+            std::vector<double> dS(V.nDV, 0.0), dV(V.nDV, 0.0);
+            for (int j = 0; j < np; ++j) {
+                const int i = pitem[j], k = V.pairDV[pidx[j]];
+                dS[k] += gq[j] * hv[i] + hq[i] * gv[j];
+                dV[k] += gv[j];
+            }
+            for (int k = 0; k < V.nDV; ++k)
+                if (dS[k] != 0.0 || dV[k] != 0.0) g[k] += f * (dS[k] * Vt - S * dV[k]) / (Vt * Vt);
+        }
+    }
+    return ASTRO_OK;
+}
+
+int astro_ctx_objective(astro_ctx* c, int32_t numTerms, const astro_objective_term* terms, double* objVal) {
+    if (!c || !terms || !objVal || numTerms < 1) return fail(ASTRO_ERR_ARG, "null or empty argument");
+    if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "objective terms cover brick8 models");
+    bool uu = false, ut = false;
+    for (int t = 0; t < numTerms; ++t) ASTRO_TRY(obj_term(c, terms[t], objVal[t], false, nullptr, uu, ut));
+    return ASTRO_OK;
+}
+
+int astro_ctx_objective_gradient(astro_ctx* c, int32_t numTerms, const astro_objective_term* terms, double* dLdD) {
+    ASTRO_TRY(dv_require(c));
+    if (!terms || !dLdD || numTerms < 1) return fail(ASTRO_ERR_ARG, "null or empty argument");
+    const int n = c->prep.elMatDim, N = c->prep.numNodes, nDV = c->dvs.nDV;
+    hipStream_t s = c->s;
+    if (c->dLdu.n != (size_t)n) ASTRO_TRY(c->dLdu.alloc(n));
+    if (c->adjU.n != (size_t)n) ASTRO_TRY(c->adjU.alloc(n));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, c->dLdu.p, (int64_t)n, 0.0);
+    if (c->thermalReady) {
+        if (c->dLdt.n != (size_t)N) ASTRO_TRY(c->dLdt.alloc(N));
+        if (c->adjT.n != (size_t)N) ASTRO_TRY(c->adjT.alloc(N));
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->dLdt.p, (int64_t)N, 0.0);
+    }
+    std::vector<double> g(nDV, 0.0);
+    bool usesU = false, usesT = false;
+    for (int t = 0; t < numTerms; ++t) {
+        double v = 0.0;
+        ASTRO_TRY(obj_term(c, terms[t], v, true, &g, usesU, usesT));
+    }
+    auto& V = c->dvs;
+    const int nP = (int)V.pairEl.size();
+    DBuf<double> lamDot;
+    ASTRO_TRY(lamDot.alloc(std::max(nP, 1)));
+    std::vector<double> hl(nP);
+    if (usesU) {
+        // solveDynamicAdjoint, static branch: (K + C^T C) lambda = dL/du on the
+        // factored tangent of the last static solve
+        bool ok = false;
+        ASTRO_TRY(with_prof(c, [&](auto& P) {
+            ok = P.factored;
+            return ASTRO_OK;
+        }));
+        if (!ok || c->csrMassFactor != 0.0)
+            return fail(ASTRO_ERR_ARG, "the elastic adjoint needs the factored static system (static_solve first)");
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, c->adjU.p, (int64_t)n, 0.0);
+        const int rc = ctx_gmres(c, c->dLdu.p, c->adjU.p);
+        if (rc != ASTRO_OK) return rc;
+        ASTRO_TRY(dv_pairs_launch(c, false, c->nodeDisp.p, c->adjU.p, lamDot.p, nullptr, n));
+        HIP_TRY(hipMemcpyAsync(hl.data(), lamDot.p, sizeof(double) * nP, hipMemcpyDeviceToHost, s));
+        std::vector<double> lam(n);
+        HIP_TRY(hipMemcpyAsync(lam.data(), c->adjU.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int p = 0; p < nP; ++p) g[V.pairDV[p]] -= hl[p];
+        for (size_t i = 0; i < V.loadDof.size(); ++i) g[V.loadDV[i]] += V.loadCoef[i] * lam[V.loadDof[i]];
+    }
+    if (usesT) {
+        bool ok = false;
+        ASTRO_TRY(with_tprof(c, [&](auto& P) {
+            ok = P.factored;
+            return ASTRO_OK;
+        }));
+        if (!ok || !c->thermalSteady)
+            return fail(ASTRO_ERR_ARG, "the thermal adjoint needs the factored steady system (thermal_solve first)");
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, c->adjT.p, (int64_t)N, 0.0);
+        const int rc = with_tprof(c, [&](auto& P) {
+            return c->tgm.solve(c->top, P, c->dLdt.p, c->adjT.p, c->opt.rtol, 0.0, c->opt.maxCycles, s);
+        });
+        if (rc != ASTRO_OK) return rc;
+        ASTRO_TRY(dv_pairs_launch(c, true, c->nodeTemp.p, c->adjT.p, lamDot.p, nullptr, N));
+        HIP_TRY(hipMemcpyAsync(hl.data(), lamDot.p, sizeof(double) * nP, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int p = 0; p < nP; ++p) g[V.pairDV[p]] -= hl[p];
+    }
+    for (int k = 0; k < nDV; ++k) dLdD[k] = g[k];
+    return ASTRO_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Batched complex-step sensitivities: every design variable's perturbed
 // system as a complex128 instance (the kernel and its algorithm are in
 // astro_batch.hip).  Set-up, once per design: the node-major DOF order
@@ -2213,6 +2723,7 @@ int astro_ctx_assemble_thermal(astro_ctx* c) {
     if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "thermal analysis of shell4 meshes is not part of the device path");
     ASTRO_TRY(ctx_thermal_setup(c));
     const int64_t nnz = (int64_t)c->prep.thermMatCols.size();
+    c->thermalSteady = true;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, c->s, c->tVal.p, nnz, 0.0);
     for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
         const int cnt = c->colorStart[k + 1] - c->colorStart[k];
@@ -2273,6 +2784,7 @@ int astro_ctx_thermal_dynamic_init(astro_ctx* c, double gamma, double dt) {
         if (b->n != (size_t)N) ASTRO_TRY(b->alloc(N));
     for (DBuf<double>* b : {&c->prevTemp, &c->prevTdot, &c->nodeTdot, &c->nodeTemp, &c->tDel})
         hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, b->p, (int64_t)N, 0.0);
+    c->thermalSteady = false;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s, c->tVal.p, nnz, 0.0);
     for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
         const int cnt = c->colorStart[k + 1] - c->colorStart[k];
