@@ -15,6 +15,20 @@
 ! constraint terms with their node or node-set names -- and later calls reuse
 ! it until the design vector r_dVec changes.
 !
+! gpuGetdRelasticdD(time, dVarNum) has the effect of the reference's
+! getdRelasticdD (Working/AStrO_commandFunctions.f90:1172-1375) for brick8
+! meshes: dRudD(1:elMatDim) = dR/dd of variable dVarNum at the current
+! nodeDisp, with R = K u - F.  The host's design maps (dCategory, dSubCat,
+! dComponent, elToDRange / elToD / elToCoef, ndToDRange / ndToD / ndToCoef)
+! go to astro_ctx_set_design_vars; all variables' columns are formed on the
+! device by complex step in one call (astro_ctx_dRdD_vars) and kept until
+! nodeDisp or time changes, so the host's per-variable loop (getTotaldLdD)
+! reads one column per call.  Element terms scatter into the element's own
+! DOFs (the reference's D9 indexing is not reproduced).  Categories: elastic
+! modulus / poissonRatio / shearModulus, density, thermalCond, nodeCoord and
+! elastic nodeLoad (active when time lies in its activeTime window); another
+! category stops the run when its column is asked for.
+!
 ! Scope: brick8 (8) and shell4 (41) meshes of one element type; isotropic or
 ! orthotropic engineering constants (a material given by its 21-entry
 ! stiffness matrix stops the run).  For shell4 the condensed element matrix
@@ -27,7 +41,7 @@ module AStrO_gpuElastic
     use AStrO_globalData
     implicit none
     private
-    public :: gpuGetElasticSolnLoad, gpuElasticRelease
+    public :: gpuGetElasticSolnLoad, gpuGetdRelasticdD, gpuElasticRelease
 
     type, bind(C) :: astro_model_t
         integer(c_int32_t) :: numNodes, numEls
@@ -41,6 +55,12 @@ module AStrO_gpuElastic
         integer(c_int32_t) :: numLayers
         type(c_ptr) :: layers, zOffset, orient
         type(c_ptr) :: specHeat
+    end type
+
+    type, bind(C) :: astro_design_t
+        integer(c_int32_t) :: numDV
+        type(c_ptr) :: category, component, elToDRange, elToD, elToCoef, ndToDRange, ndToD, ndToCoef
+        real(c_double) :: h
     end type
 
     type, bind(C) :: astro_options_t
@@ -90,6 +110,16 @@ module AStrO_gpuElastic
             real(c_double), intent(in) :: host(*)
             integer(c_int64_t), value :: count
         end function
+        integer(c_int) function astro_ctx_set_design_vars(ctx, design) bind(C, name='astro_ctx_set_design_vars')
+            import :: c_int, c_ptr, astro_design_t
+            type(c_ptr), value :: ctx
+            type(astro_design_t), intent(in) :: design
+        end function
+        integer(c_int) function astro_ctx_dRdD_vars(ctx, dRdD) bind(C, name='astro_ctx_dRdD_vars')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: dRdD(*)
+        end function
         type(c_ptr) function astro_last_error() bind(C, name='astro_last_error')
             import :: c_ptr
         end function
@@ -102,6 +132,12 @@ module AStrO_gpuElastic
     type(c_ptr), save :: ctx = c_null_ptr
     logical, save :: assembled = .false.
     real*8, allocatable, save :: dVecAtSetup(:)
+    ! design maps handed to the context, and the dR/dd columns of the last
+    ! (nodeDisp, time) they were formed for
+    logical, save :: designSet = .false., haveCols = .false.
+    integer, allocatable, save :: dvKind(:)
+    real*8, allocatable, save :: dRdDCols(:), uAtCols(:)
+    real*8, save :: tAtCols = 0d0
 
 contains
 
@@ -169,6 +205,18 @@ contains
         allocate(tn(max(1, numNodes)))
         allocate(xyz(3*numNodes), conn(8*numEls), et(numEls), ep(9*numEls), den(numEls), cnd(6*numEls), sh(numEls))
         xyz = reshape(nodeList(1:3, 1:numNodes), [3*numNodes])
+        ! node coordinates with their design variables (r_getNodeCoord)
+        if (numDVar .gt. 0) then
+            do i = 1, numNodes
+                do j = ndToDRange(i-1) + 1, ndToDRange(i)
+                    d = ndToD(j)
+                    if (d .gt. 0) then
+                        if (dCategory(d) .eq. 'nodeCoord') xyz(3*(i-1)+dComponent(d)) = &
+                            xyz(3*(i-1)+dComponent(d)) + ndToCoef(j)*r_dVec(d)
+                    endif
+                enddo
+            enddo
+        endif
         conn = reshape(elementList(1:8, 1:numEls), [8*numEls])
         et = elementType(1:numEls)
         nLay = 0
@@ -320,6 +368,8 @@ contains
         o%stepGraphs = 0
         call check(astro_ctx_create(m, o, ctx), 'astro_ctx_create')
         assembled = .false.
+        designSet = .false.
+        haveCols = .false.
         if (numDVar .gt. 0) then
             if (allocated(dVecAtSetup)) deallocate(dVecAtSetup)
             allocate(dVecAtSetup(numDVar))
@@ -350,12 +400,127 @@ contains
         elasticLoad(1:elMatDim) = elasticLoad(1:elMatDim) + ld
     end subroutine gpuGetElasticSolnLoad
 
+    ! The host's design maps in the library's form: category codes of
+    ! include/astro_abi.h (0 where the library has no counterpart; such a
+    ! variable enters as a density variable without map entries, which moves
+    ! nothing in the static residual), the node map with the coefficients of
+    ! elastic nodeLoad variables outside their activeTime window set to 0.
+    subroutine design_setup(time)
+        real*8, intent(in) :: time
+        type(astro_design_t) :: dsn
+        integer(c_int32_t), allocatable, target :: cat(:), comp(:), er(:), eid(:), nr(:), nid(:)
+        real(c_double), allocatable, target :: ec(:), nc(:)
+        integer :: k, j, ne, nn
+        if (allocated(dvKind)) deallocate(dvKind)
+        allocate(dvKind(numDVar), cat(numDVar), comp(numDVar))
+        do k = 1, numDVar
+            dvKind(k) = 0
+            if (dCategory(k) .eq. 'elastic') then
+                if (dSubCat(k) .eq. 'modulus') dvKind(k) = 1
+                if (dSubCat(k) .eq. 'poissonRatio') dvKind(k) = 2
+                if (dSubCat(k) .eq. 'shearModulus') dvKind(k) = 3
+            elseif (dCategory(k) .eq. 'density') then
+                dvKind(k) = 4
+            elseif (dCategory(k) .eq. 'thermalCond') then
+                dvKind(k) = 5
+            elseif (dCategory(k) .eq. 'nodeCoord') then
+                dvKind(k) = 6
+            elseif (dCategory(k) .eq. 'nodeLoad' .and. dSubCat(k) .eq. 'elastic') then
+                dvKind(k) = 7
+            endif
+            cat(k) = dvKind(k)
+            comp(k) = dComponent(k)
+            if (dvKind(k) .eq. 0) then
+                cat(k) = 4
+                comp(k) = 1
+            endif
+        enddo
+        ne = elToDRange(numEls)
+        nn = ndToDRange(numNodes)
+        allocate(er(numEls+1), eid(max(1, ne)), ec(max(1, ne)), nr(numNodes+1), nid(max(1, nn)), nc(max(1, nn)))
+        er(1) = 0
+        j = 0
+        do k = 1, numEls
+            do ne = elToDRange(k-1) + 1, elToDRange(k)
+                if (elToD(ne) .gt. 0) then
+                    if (dvKind(elToD(ne)) .ne. 0) then
+                        j = j + 1
+                        eid(j) = elToD(ne)
+                        ec(j) = elToCoef(ne)
+                    endif
+                endif
+            enddo
+            er(k+1) = j
+        enddo
+        nr(1) = 0
+        j = 0
+        do k = 1, numNodes
+            do nn = ndToDRange(k-1) + 1, ndToDRange(k)
+                if (ndToD(nn) .gt. 0) then
+                    if (dvKind(ndToD(nn)) .ne. 0) then
+                        j = j + 1
+                        nid(j) = ndToD(nn)
+                        nc(j) = ndToCoef(nn)
+                        if (dvKind(ndToD(nn)) .eq. 7) then
+                            if (time .lt. dActTime(1, ndToD(nn)) .or. time .gt. dActTime(2, ndToD(nn))) nc(j) = 0d0
+                        endif
+                    endif
+                endif
+            enddo
+            nr(k+1) = j
+        enddo
+        dsn%numDV = numDVar
+        dsn%category = c_loc(cat)
+        dsn%component = c_loc(comp)
+        dsn%elToDRange = c_loc(er)
+        dsn%elToD = c_loc(eid)
+        dsn%elToCoef = c_loc(ec)
+        dsn%ndToDRange = c_loc(nr)
+        dsn%ndToD = c_loc(nid)
+        dsn%ndToCoef = c_loc(nc)
+        dsn%h = 1d-30
+        call check(astro_ctx_set_design_vars(ctx, dsn), 'astro_ctx_set_design_vars')
+        designSet = .true.
+        haveCols = .false.
+    end subroutine design_setup
+
+    subroutine gpuGetdRelasticdD(time, dVarNum)
+        real*8, intent(in) :: time
+        integer, intent(in) :: dVarNum
+        integer :: n
+        call setup()
+        n = elMatDim
+        if (.not. designSet .or. time .ne. tAtCols) call design_setup(time)
+        if (dvKind(dVarNum) .eq. 0) then
+            write(*,'(A,A,A,A)') 'gpuElastic: getdRelasticdD has no device form for design category ', &
+                trim(dCategory(dVarNum)), ' / ', trim(dSubCat(dVarNum))
+            stop 1
+        endif
+        if (haveCols) then
+            if (any(nodeDisp(1:n) .ne. uAtCols)) haveCols = .false.
+        endif
+        if (.not. haveCols) then
+            if (allocated(dRdDCols)) deallocate(dRdDCols)
+            if (allocated(uAtCols)) deallocate(uAtCols)
+            allocate(dRdDCols(n*numDVar), uAtCols(n))
+            uAtCols = nodeDisp(1:n)
+            call check(astro_ctx_set_f64(ctx, 'nodeDisp'//c_null_char, nodeDisp, int(n, c_int64_t)), &
+                       'writing nodeDisp')
+            call check(astro_ctx_dRdD_vars(ctx, dRdDCols), 'astro_ctx_dRdD_vars')
+            tAtCols = time
+            haveCols = .true.
+        endif
+        dRudD(1:n) = dRdDCols((dVarNum-1)*n+1 : dVarNum*n)
+    end subroutine gpuGetdRelasticdD
+
     ! Frees the device context (a later call builds it again).
     subroutine gpuElasticRelease()
         integer(c_int) :: rc
         if (c_associated(ctx)) rc = astro_ctx_destroy(ctx)
         ctx = c_null_ptr
         assembled = .false.
+        designSet = .false.
+        haveCols = .false.
     end subroutine gpuElasticRelease
 
 end module AStrO_gpuElastic

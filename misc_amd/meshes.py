@@ -52,6 +52,7 @@ class Model:
     load_nodes: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     load_vals: np.ndarray = field(default_factory=lambda: np.zeros((0, 7)))
     node_sets: dict = field(default_factory=dict)
+    element_sets: dict = field(default_factory=dict)  # name -> 1-based element ids
     colors: Optional[np.ndarray] = None  # (E,) element colour, conflict-free scatter
     # shell sections (shell elements only): per element nL layers of
     # (E1, E2, nu12, G12, thickness, angle in radians), z offset, and the two

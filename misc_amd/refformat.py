@@ -63,6 +63,10 @@ def write_model(model: Model, path: str) -> None:
         lines.append(f"        - name: es{q + 1}")
         lines.append("          labels:")
         lines.extend(_label_lines(np.nonzero(inv == q)[0] + 1))
+    for name, ids in getattr(model, "element_sets", {}).items():
+        lines.append(f"        - name: {name}")
+        lines.append("          labels:")
+        lines.extend(_label_lines(np.asarray(ids)))
     lines.append("sections:")
     for q in range(uniq.shape[0]):
         lines.append("    - type: solid")
@@ -126,6 +130,39 @@ def _run(a: int, b: int) -> str:
     if a == b:
         return f"              - {a}"
     return f"              - [{a}, {b}]"
+
+
+def write_design_vars(path: str, variables) -> None:
+    """Design-variable file read by readDesignVarInput
+    (Working/AStrO_input.f90:1099-1408; layout of Working/DVarInputTemplate.yaml).
+    Each variable is a dict with category, subCategory (optional), component,
+    elementSet or nodeSet (a set name of the model file), coefficients (a
+    scalar for every member, or one value per member in set order) and
+    activeTime (optional, a scalar or a [start, end] pair)."""
+    lines = ["designVariables:"]
+    for v in variables:
+        lines.append(f"    - category: {v['category']}")
+        if v.get("subCategory"):
+            lines.append(f"      subCategory: {v['subCategory']}")
+        lines.append(f"      component: {int(v.get('component', 1))}")
+        if v.get("activeTime") is not None:
+            at = v["activeTime"]
+            if np.ndim(at):
+                lines.append(f"      activeTime: [{_r(at[0])}, {_r(at[1])}]")
+            else:
+                lines.append(f"      activeTime: {_r(at)}")
+        if v.get("elementSet"):
+            lines.append(f"      elementSet: {v['elementSet']}")
+        else:
+            lines.append(f"      nodeSet: {v['nodeSet']}")
+        c = v.get("coefficients", 1.0)
+        if np.ndim(c):
+            lines.append("      coefficients:")
+            lines.extend(f"          - {_r(x)}" for x in c)
+        else:
+            lines.append(f"      coefficients: {_r(c)}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
 
 
 def write_static_script(path: str, model_file: str) -> None:

@@ -410,12 +410,14 @@ def ref_available() -> bool:
 
 def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
                   driver: Optional[str] = None, gmres: Optional[tuple] = None,
-                  env: Optional[Dict[str, str]] = None) -> Dict[str, np.ndarray]:
+                  env: Optional[Dict[str, str]] = None, design=None) -> Dict[str, np.ndarray]:
     """Write `model` in the reference's format, run the literal reference driver
     on it and return every dumped array (1-based values as the reference holds
     them).  gmres = (numVecs, maxIt, solverMaxBW) overrides the restart length,
     the iteration cap and the profile bandwidth of the job (0 keeps the
-    reference's rule)."""
+    reference's rule).  design: design variables for
+    misc_amd.refformat.write_design_vars, read by the driver through
+    ASTRO_DV_FILE."""
     from misc_amd.refformat import write_model
 
     exe = driver or REF_DRIVER
@@ -435,6 +437,12 @@ def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
     tun = os.environ.get("GLIBC_TUNABLES", "")
     tun = (tun + ":" if tun else "") + "glibc.malloc.tcache_count=0"
     env = dict(os.environ, MALLOC_PERTURB_="255", GLIBC_TUNABLES=tun, **(env or {}))
+    if design:
+        from misc_amd.refformat import write_design_vars
+
+        df = os.path.join(wd, "design.yaml")
+        write_design_vars(df, design)
+        env["ASTRO_DV_FILE"] = df
     extra = [str(int(v)) for v in gmres] if gmres else []
     subprocess.run([exe, mf, os.path.join(wd, "o_"), mode] + extra, cwd=wd, check=True, env=env)
     out = {}

@@ -14,6 +14,11 @@
 ! the *solve block, so that a job can stop GMRES after a chosen number of full
 ! restart cycles.
 !
+! With ASTRO_DV_FILE set, the design-variable file it names is read after the
+! model (readDesignVarInput, Working/AStrO_input.f90:1099-1408); the drop-in
+! build then dumps getdRelasticdD's column of every variable at the solution
+! (dRudD, elMatDim x numDVar), formed by gpuGetdRelasticdD.
+!
 ! Solver parameters follow the *solve block: solverBlockDim = numNodes and
 ! solverMaxBW from 27*numNodes*numNodes in default INTEGER arithmetic
 ! (Working/AStrO_runJob.f90:107-114).  Thermal mode sets thermMatDim =
@@ -35,6 +40,9 @@ program astro_ref_driver
     character(len=128) :: modelFile
     character(len=256) :: prefix
     character(len=16) :: mode, argv
+    character(len=256) :: dvFile
+    integer :: dvStat, kdv
+    real*8, allocatable :: dRAll(:)
     integer :: nnd2, bw3, ticks0, ticks1, rate, tu
     integer :: nvArg, itArg, bwArg
     real*8 :: tPrep, tAsm, tScale, tProf, tFact, tSolve
@@ -58,6 +66,8 @@ program astro_ref_driver
     open(unit=lfUnit, file=trim(prefix)//'log.txt', action='write', status='replace')
 
     call readModelInput(modelFile)
+    call get_environment_variable('ASTRO_DV_FILE', dvFile, status=dvStat)
+    if (dvStat .eq. 0) call readDesignVarInput(dvFile)
 
     solverBlockDim = numNodes
     nnd2 = 27*numNodes*numNodes
@@ -194,6 +204,14 @@ program astro_ref_driver
         elasticLoad(:) = r_0
         call gpuGetElasticSolnLoad(0)
         call dumpd(elasticLoad, elMatDim, 'solnLoad')
+        if (numDVar .gt. 0) then
+            allocate(dRAll(elMatDim*numDVar))
+            do kdv = 1, numDVar
+                call gpuGetdRelasticdD(loadTime, kdv)
+                dRAll((kdv-1)*elMatDim+1 : kdv*elMatDim) = dRudD(1:elMatDim)
+            enddo
+            call dumpd(dRAll, elMatDim*numDVar, 'dRudD')
+        endif
         call gpuElasticRelease()
 #endif
     else
