@@ -225,6 +225,11 @@ int astro_ctx_elastic_soln_load(astro_ctx* ctx);
  * engineering shear; stress, strain: 6 * numEls doubles, element order.
  * brick8 models (ASTRO_ERR_UNSUPPORTED for shell4).  Synchronous. */
 int astro_ctx_element_stress(astro_ctx* ctx, const double* sp, double* stress, double* strain);
+/* Heat flux (-k grad T) and temperature gradient of every element at the
+ * natural point sp[3] for the current nodeTemp (r_getElFlux,
+ * Working/AStrO_r_elementEqns.f90:1746-1799); flux, grad: 3 * numEls doubles.
+ * brick8 models with a thermal system.  Synchronous. */
+int astro_ctx_element_flux(astro_ctx* ctx, const double* sp, double* flux, double* grad);
 
 /* Newmark dynamics, the dynamic branch of solve()
  * (Working/AStrO_commandFunctions.f90:762-810; element residual

@@ -44,7 +44,7 @@ EXPORTS = [
     "astro_gnn_train", "astro_gnn_loss_grad", "astro_gnn_field_size", "astro_gnn_get_field",
     "astro_gnn_field_blocks", "astro_ctx_gnn_attach", "astro_ctx_gnn_detach", "astro_ctx_gnn_get",
     "astro_ctx_set_design_vars", "astro_ctx_dRdD_vars", "astro_ctx_dRtdD_vars", "astro_ctx_objective",
-    "astro_ctx_objective_gradient",
+    "astro_ctx_objective_gradient", "astro_ctx_element_flux",
 ]
 
 # Design-variable categories and objective terms (include/astro_abi.h).
@@ -205,6 +205,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_cs_batch_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
     L.astro_ctx_cs_batch_tile.argtypes = [C.c_void_p, C.c_int32]
     L.astro_ctx_element_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.astro_ctx_element_flux.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.astro_ctx_thermal_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.astro_ctx_thermal_dynamic_run.argtypes = [C.c_void_p, C.c_int]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -645,6 +646,14 @@ class Context:
         st, sn = np.zeros(6 * E), np.zeros(6 * E)
         _check(load_library().astro_ctx_element_stress(self.h, np.ascontiguousarray(point, np.float64), st, sn))
         return st.reshape(E, 6), sn.reshape(E, 6)
+
+    def element_flux(self, point=(0.0, 0.0, 0.0)):
+        """Heat flux and temperature gradient (numEls x 3 each) of every brick8
+        at the natural point for the current nodeTemp."""
+        E = self.model.num_els
+        fl, gr = np.zeros(3 * E), np.zeros(3 * E)
+        _check(load_library().astro_ctx_element_flux(self.h, np.ascontiguousarray(point, np.float64), fl, gr))
+        return fl.reshape(E, 3), gr.reshape(E, 3)
 
     def cs_batch_gradient(self):
         """The compliance gradient by direct complex-step solves of all design

@@ -1483,6 +1483,29 @@ int astro_ctx_element_stress(astro_ctx* c, const double* sp, double* stress, dou
     return ASTRO_OK;
 }
 
+// Heat flux -k grad T and the temperature gradient of every element at the
+// natural point sp for the current nodeTemp (r_getElFlux,
+// Working/AStrO_r_elementEqns.f90:1746-1799).  flux, grad: host arrays of
+// 3 numEls.  brick8 models with a thermal system.
+int astro_ctx_element_flux(astro_ctx* c, const double* sp, double* flux, double* grad) {
+    if (!c || !sp || !flux || !grad) return fail(ASTRO_ERR_ARG, "null argument");
+    if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "heat flux recovery covers brick8 elements");
+    if (!c->thermalReady) return fail(ASTRO_ERR_ARG, "the thermal system has not been set up");
+    for (int k = 0; k < 3; ++k)
+        if (!(std::fabs(sp[k]) <= 1.0)) return fail(ASTRO_ERR_ARG, "natural coordinates must lie in [-1, 1]");
+    const int E = c->prep.numEls;
+    DBuf<double> df, dg;
+    ASTRO_TRY(df.alloc(3 * (size_t)E));
+    ASTRO_TRY(dg.alloc(3 * (size_t)E));
+    hipLaunchKernelGGL(k_brick8_flux, dim3(grid_for(E, 256, INT_MAX)), dim3(256), 0, c->s, E, c->conn.p, c->X.p,
+                       c->elCond.p, c->rank.p, c->nodeTemp.p, sp[0], sp[1], sp[2], df.p, dg.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(flux, df.p, sizeof(double) * 3 * (size_t)E, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipMemcpyAsync(grad, dg.p, sizeof(double) * 3 * (size_t)E, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
 int astro_ctx_elastic_soln_load(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     ctx_minus_internal_force(c->s, c->csrVal, c->csrCols, c->csrRange, c->prep.elMatDim, c->nodeDisp.p, c->load.p);

@@ -36,6 +36,8 @@ __global__ void k_obj_pairs(int quantity, int comp, int nItems, const int* pairI
                             double* dv);
 __global__ void k_obj_nodes(int nItems, const int* dofs, const double* tgt, int tgtMode, double coef, double expo,
                             const double* x, double* val, double* dLdx);
+__global__ void k_brick8_flux(int E, const int* conn, const double* X, const double* cond, const int* rank,
+                              const double* T, double s0, double s1, double s2, double* flux, double* grad);
 __global__ void k_obj_scatter_du(int count, const int* slot, const int* items, const double* w, const double* dqdu,
                                  const int* conn, const int* rank, int nS, double* dLdu);
 

@@ -552,4 +552,40 @@ __global__ __launch_bounds__(256) void k_obj_scatter_du(int count, const int* __
     dLdu[rank[nd] + c * nS - 1] += w[i] * dqdu[24 * (size_t)i + j];
 }
 
+// Heat flux and temperature gradient of every brick8 at the natural point
+// (s0, s1, s2): Tx = Nx^T T_e, flux = -k Tx (r_getElFlux,
+// Working/AStrO_r_elementEqns.f90:1746-1799).  Thermal DOF = rank(node).
+__global__ __launch_bounds__(256) void k_brick8_flux(int E, const int* __restrict__ conn, const double* __restrict__ X,
+                                                     const double* __restrict__ cond, const int* __restrict__ rank,
+                                                     const double* __restrict__ T, double s0, double s1, double s2,
+                                                     double* __restrict__ flux, double* __restrict__ grad) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double xe[8][3], te[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int nd = conn[8 * (size_t)e + a] - 1;
+        te[a] = T[rank[nd] - 1];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xe[a][c] = X[3 * (size_t)nd + c];
+    }
+    double N[8], Nx[8][3], det;
+    b8_point(s0, s1, s2, xe, N, Nx, det);
+    double g[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) acc += Nx[a][j] * te[a];
+        g[j] = acc;
+    }
+    const double* k = cond + 6 * (size_t)e;
+    const double K[3][3] = {{k[0], k[3], k[4]}, {k[3], k[1], k[5]}, {k[4], k[5], k[2]}};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        flux[3 * (size_t)e + i] = -(K[i][0] * g[0] + K[i][1] * g[1] + K[i][2] * g[2]);
+        grad[3 * (size_t)e + i] = g[i];
+    }
+}
+
 }  // namespace astro

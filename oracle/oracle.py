@@ -410,14 +410,18 @@ def ref_available() -> bool:
 
 def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
                   driver: Optional[str] = None, gmres: Optional[tuple] = None,
-                  env: Optional[Dict[str, str]] = None, design=None) -> Dict[str, np.ndarray]:
+                  env: Optional[Dict[str, str]] = None, design=None, objective: Optional[str] = None,
+                  write_results: bool = False) -> Dict[str, np.ndarray]:
     """Write `model` in the reference's format, run the literal reference driver
     on it and return every dumped array (1-based values as the reference holds
     them).  gmres = (numVecs, maxIt, solverMaxBW) overrides the restart length,
     the iteration cap and the profile bandwidth of the job (0 keeps the
     reference's rule).  design: design variables for
     misc_amd.refformat.write_design_vars, read by the driver through
-    ASTRO_DV_FILE."""
+    ASTRO_DV_FILE; objective: the text of an objective file (ASTRO_OBJ_FILE);
+    write_results: run the reference's writers after the solve (their files
+    come back as text under the keys 'nodes.yaml', 'elements.yaml',
+    'objective.yaml', 'dvars.yaml')."""
     from misc_amd.refformat import write_model
 
     exe = driver or REF_DRIVER
@@ -443,10 +447,21 @@ def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
         df = os.path.join(wd, "design.yaml")
         write_design_vars(df, design)
         env["ASTRO_DV_FILE"] = df
+    if objective:
+        of = os.path.join(wd, "objective_in.yaml")
+        with open(of, "w") as f:
+            f.write(objective)
+        env["ASTRO_OBJ_FILE"] = of
+    if write_results:
+        env["ASTRO_WRITE_PREFIX"] = os.path.join(wd, "w_")
     extra = [str(int(v)) for v in gmres] if gmres else []
     subprocess.run([exe, mf, os.path.join(wd, "o_"), mode] + extra, cwd=wd, check=True, env=env)
     out = {}
     for fn in os.listdir(wd):
+        if fn.startswith("w_"):
+            with open(os.path.join(wd, fn)) as f:
+                out[fn[2:]] = f.read()
+            continue
         if not fn.startswith("o_"):
             continue
         key, ext = os.path.splitext(fn[2:])

@@ -19,6 +19,12 @@
 ! build then dumps getdRelasticdD's column of every variable at the solution
 ! (dRudD, elMatDim x numDVar), formed by gpuGetdRelasticdD.
 !
+! With ASTRO_OBJ_FILE set, the objective file it names is read too
+! (readObjectiveInput); with ASTRO_WRITE_PREFIX set, the reference's own
+! writers run after the solve on its arrays (writeNodeResults,
+! writeElementResults, writeObjective after getObjective, writeDVarValues;
+! Working/AStrO_output.f90), into files named by that prefix.
+!
 ! Solver parameters follow the *solve block: solverBlockDim = numNodes and
 ! solverMaxBW from 27*numNodes*numNodes in default INTEGER arithmetic
 ! (Working/AStrO_runJob.f90:107-114).  Thermal mode sets thermMatDim =
@@ -32,6 +38,8 @@ program astro_ref_driver
     use AStrO_bookKeeping
     use AStrO_commandFunctions
     use AStrO_solvers
+    use AStrO_output
+    use AStrO_objective
 #ifdef ASTRO_HIP
     use AStrO_gpuElastic
 #endif
@@ -41,7 +49,10 @@ program astro_ref_driver
     character(len=256) :: prefix
     character(len=16) :: mode, argv
     character(len=256) :: dvFile
-    integer :: dvStat, kdv
+    character(len=128) :: objFile, wrPrefix, wrName
+    character(len=16) :: wf(9)
+    integer :: dvStat, kdv, objStat, wrStat, q
+    integer, allocatable :: allIds(:)
     real*8, allocatable :: dRAll(:)
     integer :: nnd2, bw3, ticks0, ticks1, rate, tu
     integer :: nvArg, itArg, bwArg
@@ -68,6 +79,9 @@ program astro_ref_driver
     call readModelInput(modelFile)
     call get_environment_variable('ASTRO_DV_FILE', dvFile, status=dvStat)
     if (dvStat .eq. 0) call readDesignVarInput(dvFile)
+    call get_environment_variable('ASTRO_OBJ_FILE', objFile, status=objStat)
+    if (objStat .eq. 0) call readObjectiveInput(objFile)
+    call get_environment_variable('ASTRO_WRITE_PREFIX', wrPrefix, status=wrStat)
 
     solverBlockDim = numNodes
     nnd2 = 27*numNodes*numNodes
@@ -248,6 +262,8 @@ program astro_ref_driver
         call dumpd(nodeTemp, numNodes, 'nodeTemp')
     endif
 
+    if (wrStat .eq. 0) call writeAll()
+
     open(newunit=tu, file=trim(prefix)//'times.txt', action='write', status='replace')
     write(tu,'(A,ES14.6)') 'prep ', tPrep
     write(tu,'(A,ES14.6)') 'assemble ', tAsm
@@ -338,6 +354,63 @@ contains
             endif
         enddo
     end subroutine nodalLoads
+
+    ! The reference's writers on its state after the solve.
+    subroutine writeAll()
+        allocate(allIds(max(numNodes, numEls)))
+        do q = 1, size(allIds)
+            allIds(q) = q
+        enddo
+        if (solveElastic .eq. 1) then
+            wf(1) = 'displacement'
+            wf(2) = 'reactionForce'
+            wrName = trim(wrPrefix)//'nodes.yaml'
+            call writeNodeResults(wrName, wf, 2, allIds(1:numNodes), numNodes, 0d0, 0)
+            wf(1) = 'stress'
+            wf(2) = 'strain'
+            wf(3) = 'strainEnergyDen'
+            wrName = trim(wrPrefix)//'elements.yaml'
+            call writeElementResults(wrName, wf, 3, allIds(1:numEls), numEls, 0d0, 0)
+        else
+            wf(1) = 'temperature'
+            wf(2) = 'reactionHeatGen'
+            thermalLoad(:) = r_0
+            call getThermalSolnLoad(0)
+            call dumpd(thermalLoad, numNodes, 'thermalLoadSoln')
+            wrName = trim(wrPrefix)//'nodes.yaml'
+            call writeNodeResults(wrName, wf, 2, allIds(1:numNodes), numNodes, 0d0, 0)
+            wf(1) = 'heatFlux'
+            wf(2) = 'tempGradient'
+            wrName = trim(wrPrefix)//'elements.yaml'
+            call writeElementResults(wrName, wf, 2, allIds(1:numEls), numEls, 0d0, 0)
+        endif
+        if (objStat .eq. 0) then
+            if (numObjTerms .gt. 0) then
+                objVal(:) = r_0
+                call getObjective(loadTime)
+                wf(1) = 'category'
+                wf(2) = 'operator'
+                wf(3) = 'component'
+                wf(4) = 'layer'
+                wf(5) = 'coefficient'
+                wf(6) = 'exponent'
+                wf(7) = 'elementSet'
+                wf(8) = 'nodeSet'
+                wrName = trim(wrPrefix)//'objective.yaml'
+                call writeObjective(wrName, wf, 8, 0)
+                call dumpd(objVal, numObjTerms, 'objVal')
+            endif
+        endif
+        if (numDVar .gt. 0) then
+            wf(1) = 'category'
+            wf(2) = 'subCategory'
+            wf(3) = 'component'
+            wf(4) = 'layer'
+            wf(5) = 'activeTime'
+            wrName = trim(wrPrefix)//'dvars.yaml'
+            call writeDVarValues(wrName, wf, 5)
+        endif
+    end subroutine writeAll
 
     subroutine dumpi(a, n, name)
         integer, intent(in) :: n
