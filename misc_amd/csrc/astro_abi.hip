@@ -35,6 +35,62 @@ int fail(int code, const std::string& msg) {
 // double (the FP64 path) or float (the mixed-precision factor: FP32 L and
 // step-U partials on FP32 matrix cores; pivots, the diagonal-block factor,
 // the sweep blocks and every vector stay FP64).
+// The penalty C^T C of one constraint structure in profile order: the
+// entries (row, column) inside the profile that some constraint row reaches,
+// each with its products in convertToLTri's order (constraint row, then its
+// term pairs; Working/AStrO_bookKeeping.f90:776-790).  k_penalty_plan adds
+// them with one thread per entry.  Built on the host from the constraint
+// structure and the profile's first columns.
+struct PenaltyPlan {
+    int entries = 0;
+    DBuf<int> row, col, start, pa, pb;
+    int build(const int* crange, const int* ccols, int cd, const std::vector<int>& first, hipStream_t s) {
+        struct Term {
+            int64_t key;
+            int order, a, b;
+        };
+        std::vector<Term> t;
+        const int n = (int)first.size();
+        int order = 0;
+        for (int r = 0; r < cd; ++r)
+            for (int q2 = crange[r]; q2 < crange[r + 1]; ++q2) {
+                const int i4 = ccols[q2] - 1;
+                if (i4 < 0 || i4 >= n) continue;
+                for (int q3 = crange[r]; q3 < crange[r + 1]; ++q3) {
+                    const int i5 = ccols[q3] - 1;
+                    if (i5 >= first[i4] && i5 <= i4) t.push_back({(int64_t)i4 * n + i5, order++, q2, q3});
+                }
+            }
+        std::sort(t.begin(), t.end(), [](const Term& x, const Term& y) {
+            return x.key != y.key ? x.key < y.key : x.order < y.order;
+        });
+        std::vector<int> hr, hc, hs = {0}, ha, hb;
+        for (size_t q = 0; q < t.size(); ++q) {
+            if (q == 0 || t[q].key != t[q - 1].key) {
+                if (q > 0) hs.push_back((int)q);
+                hr.push_back((int)(t[q].key / n));
+                hc.push_back((int)(t[q].key % n));
+            }
+            ha.push_back(t[q].a);
+            hb.push_back(t[q].b);
+        }
+        if (!t.empty()) hs.push_back((int)t.size());
+        entries = (int)hr.size();
+        ASTRO_TRY(row.upload(hr, s));
+        ASTRO_TRY(col.upload(hc, s));
+        ASTRO_TRY(start.upload(hs, s));
+        ASTRO_TRY(pa.upload(ha, s));
+        ASTRO_TRY(pb.upload(hb, s));
+        return ASTRO_OK;
+    }
+    template <typename LT>
+    void launch(const double* cmat, const int64_t* rowoff, LT* lt, hipStream_t s) const {
+        if (entries > 0)
+            hipLaunchKernelGGL(k_penalty_plan<LT>, dim3(grid_for(entries, 256, INT_MAX)), dim3(256), 0, s, cmat,
+                               entries, row.p, col.p, start.p, pa.p, pb.p, rowoff, lt);
+    }
+};
+
 template <typename LT>
 struct DeviceProfileT {
     // rows start on 128-byte lines: 16 columns of FP64, 32 of FP32
@@ -44,6 +100,7 @@ struct DeviceProfileT {
     int64_t packedSize = 0;  // entries of the reference's packed profile
     bool aligned = false;    // rows padded: see setup
     std::vector<int> hS;     // first column per row
+    PenaltyPlan pen;         // the constraint penalty of the owning system (plan_penalty)
     std::vector<int64_t> hOff;  // device row offsets: entry (r, c) at lt[hOff[r] + c]
     DBuf<LT> lt;
     DBuf<int64_t> rowoff;
@@ -1047,7 +1104,10 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
     HIP_TRY(hipMemsetAsync(c->delDisp.p, 0, sizeof(double) * n, s));
     HIP_TRY(hipMemsetAsync(c->nodeDisp.p, 0, sizeof(double) * n, s));
     ASTRO_TRY(c->maxBits.alloc(1));
-    ASTRO_TRY(with_prof(c, [&](auto& P) { return P.setup(p.elMatLTRange.data(), n, s, true); }));
+    ASTRO_TRY(with_prof(c, [&](auto& P) {
+        ASTRO_TRY(P.setup(p.elMatLTRange.data(), n, s, true));
+        return P.pen.build(p.elMPCMatRange.data(), p.elMPCMatCols.data(), p.elMPCDim, P.hS, s);
+    }));
     c->op.n = n;
     c->op.cdim = p.elMPCDim;
     c->op.val = c->csrVal.p;
@@ -1384,9 +1444,7 @@ static int profile_from_csr(DeviceProfileT<LT>& P, hipStream_t s, const double* 
     hipLaunchKernelGGL(k_fill_t<LT>, dim3(grid_for(P.size, 256, 16384)), dim3(256), 0, s, P.lt.p, P.size, (LT)0);
     hipLaunchKernelGGL(k_csr_to_profile<LT>, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val,
                        cols, range, n, (const int*)P.sfirst.p, (const int64_t*)P.rowoff.p, P.lt.p);
-    if (cd > 0)
-        hipLaunchKernelGGL(k_penalty_to_profile<LT>, dim3(grid_for(cd, 256)), dim3(256), 0, s, cmat, ccols, crange,
-                           cd, (const int*)P.sfirst.p, (const int64_t*)P.rowoff.p, P.lt.p);
+    if (cd > 0) P.pen.launch(cmat, (const int64_t*)P.rowoff.p, P.lt.p, s);
     HIP_TRY(hipGetLastError());
     return ASTRO_OK;
 }
@@ -2724,7 +2782,10 @@ static int ctx_thermal_setup(astro_ctx* c) {
     ASTRO_TRY(c->nodeTemp.alloc(N));
     HIP_TRY(hipMemsetAsync(c->tDel.p, 0, sizeof(double) * N, s));
     HIP_TRY(hipMemsetAsync(c->nodeTemp.p, 0, sizeof(double) * N, s));
-    ASTRO_TRY(with_tprof(c, [&](auto& P) { return P.setup(p.thermMatLTRange.data(), N, s, true); }));
+    ASTRO_TRY(with_tprof(c, [&](auto& P) {
+        ASTRO_TRY(P.setup(p.thermMatLTRange.data(), N, s, true));
+        return P.pen.build(p.thermMPCMatRange.data(), p.thermMPCMatCols.data(), p.thermMPCDim, P.hS, s);
+    }));
     c->top.n = N;
     c->top.cdim = p.thermMPCDim;
     c->top.val = c->tVal.p;
@@ -3406,13 +3467,13 @@ int astro_convertToLTri(double* lTri, const int32_t* lTRange, int64_t lTSize, co
         if (rc == ASTRO_OK && cDim > 0) rc = cv.upload(cMat, cRange[cDim], s);
         if (rc == ASTRO_OK && cDim > 0) rc = cc.upload(cCols, cRange[cDim], s);
         if (rc == ASTRO_OK && cDim > 0) rc = cr.upload(cRange, cDim + 1, s);
+        PenaltyPlan pen;
+        if (rc == ASTRO_OK && cDim > 0) rc = pen.build(cRange, cCols, cDim, sf, s);
         if (rc == ASTRO_OK) {
             hipLaunchKernelGGL(k_fill, dim3(grid_for(sz, 256, 16384)), dim3(256), 0, s, lt.p, sz, 0.0);
             hipLaunchKernelGGL(k_csr_to_profile<double>, dim3(grid_for((int64_t)mainDim * 32, 256, 1 << 30)), dim3(256), 0, s,
                                mv.p, mc.p, mr.p, mainDim, sfd.p, offd.p, lt.p);
-            if (cDim > 0)
-                hipLaunchKernelGGL(k_penalty_to_profile<double>, dim3(grid_for(cDim, 256)), dim3(256), 0, s, cv.p, cc.p, cr.p,
-                                   cDim, sfd.p, offd.p, lt.p);
+            if (cDim > 0) pen.launch(cv.p, offd.p, lt.p, s);
             if (hipGetLastError() != hipSuccess ||
                 hipMemcpyAsync(lTri, lt.p, sizeof(double) * sz, hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess)

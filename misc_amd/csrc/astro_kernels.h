@@ -116,8 +116,8 @@ template <typename LT>
 __global__ void k_csr_to_profile(const double* val, const int* cols, const int* range, int n, const int* sfirst,
                                  const int64_t* rowoff, LT* lt);
 template <typename LT>
-__global__ void k_penalty_to_profile(const double* cmat, const int* ccols, const int* crange, int cdim,
-                                     const int* sfirst, const int64_t* rowoff, LT* lt);
+__global__ void k_penalty_plan(const double* cmat, int nEnt, const int* entRow, const int* entCol, const int* start,
+                               const int* pa, const int* pb, const int64_t* rowoff, LT* lt);
 template <typename LT>
 __global__ void k_sweep_blocks(const SweepBlock* blocks, const LT* lt, const int64_t* rowoff, const int* sfirst,
                                double* out);

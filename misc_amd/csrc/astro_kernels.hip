@@ -879,27 +879,29 @@ template __global__ void k_csr_to_profile<double>(const double*, const int*, con
 template __global__ void k_csr_to_profile<float>(const double*, const int*, const int*, int, const int*,
                                                  const int64_t*, float*);
 
-// Penalty C^T C into the profile (Working/AStrO_bookKeeping.f90:776-790); one
-// thread per constraint row, atomics because rows may share a DOF.
+// Penalty C^T C into the profile (Working/AStrO_bookKeeping.f90:776-790), one
+// thread per profile entry that a constraint row reaches: entry k is
+// (row entRow[k], column entCol[k]), its products cmat[pa[q]] * cmat[pb[q]]
+// for q in [start[k], start[k + 1]) in the order convertToLTri adds them
+// (constraint row, then term pairs), added one by one to the entry.  No two
+// threads share an entry, so the sum does not depend on scheduling; the
+// host drops pairs outside the profile.
 template <typename LT>
-__global__ void k_penalty_to_profile(const double* __restrict__ cmat, const int* __restrict__ ccols,
-                                     const int* __restrict__ crange, int cdim, const int* __restrict__ sfirst,
-                                     const int64_t* __restrict__ rowoff, LT* __restrict__ lt) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= cdim) return;
-    for (int s2 = crange[r]; s2 < crange[r + 1]; ++s2) {
-        const int i4 = ccols[s2] - 1;
-        if (i4 < 0) continue;
-        for (int s3 = crange[r]; s3 < crange[r + 1]; ++s3) {
-            const int i5 = ccols[s3] - 1;
-            if (i5 >= sfirst[i4] && i5 <= i4) atomicAdd(&lt[rowoff[i4] + i5], (LT)(cmat[s2] * cmat[s3]));
-        }
-    }
+__global__ void k_penalty_plan(const double* __restrict__ cmat, int nEnt, const int* __restrict__ entRow,
+                               const int* __restrict__ entCol, const int* __restrict__ start,
+                               const int* __restrict__ pa, const int* __restrict__ pb,
+                               const int64_t* __restrict__ rowoff, LT* __restrict__ lt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nEnt) return;
+    const int64_t idx = rowoff[entRow[k]] + entCol[k];
+    LT v = lt[idx];
+    for (int q = start[k]; q < start[k + 1]; ++q) v += (LT)(cmat[pa[q]] * cmat[pb[q]]);
+    lt[idx] = v;
 }
-template __global__ void k_penalty_to_profile<double>(const double*, const int*, const int*, int, const int*,
-                                                      const int64_t*, double*);
-template __global__ void k_penalty_to_profile<float>(const double*, const int*, const int*, int, const int*,
-                                                     const int64_t*, float*);
+template __global__ void k_penalty_plan<double>(const double*, int, const int*, const int*, const int*, const int*,
+                                                const int*, const int64_t*, double*);
+template __global__ void k_penalty_plan<float>(const double*, int, const int*, const int*, const int*, const int*,
+                                               const int*, const int64_t*, float*);
 
 // Fill of a profile array (either storage precision).
 template <typename T>

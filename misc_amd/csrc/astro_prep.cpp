@@ -388,19 +388,19 @@ std::string analysis_prep(const ModelDesc& m, int maxBW, bool literalMaxBW, bool
             }
         }
         // greedy colouring: smallest colour unused by every node of the element
-        std::vector<uint32_t> nodeMask(N + 1, 0);
+        std::vector<uint64_t> nodeMask(N + 1, 0);
         std::vector<int> color(E);
         int ncol = 0;
         for (int e = 0; e < E; ++e) {
             const int* el = m.elementList + 8 * (size_t)e;
-            uint32_t busy = 0;
+            uint64_t busy = 0;
             for (int a = 0; a < enn; ++a) busy |= nodeMask[el[a]];
             int c = 0;
-            while (c < 32 && (busy >> c) & 1u) ++c;
-            if (c == 32) return "element colouring needs more than 32 colours";
+            while (c < 64 && (busy >> c) & 1ull) ++c;
+            if (c == 64) return "element colouring needs more than 64 colours";
             color[e] = c;
             ncol = std::max(ncol, c + 1);
-            for (int a = 0; a < enn; ++a) nodeMask[el[a]] |= 1u << c;
+            for (int a = 0; a < enn; ++a) nodeMask[el[a]] |= 1ull << c;
         }
         p.colorStart.assign(ncol + 1, 0);
         for (int e = 0; e < E; ++e) p.colorStart[color[e] + 1]++;
