@@ -230,6 +230,18 @@ int astro_ctx_element_stress(astro_ctx* ctx, const double* sp, double* stress, d
  * Working/AStrO_r_elementEqns.f90:1746-1799); flux, grad: 3 * numEls doubles.
  * brick8 models with a thermal system.  Synchronous. */
 int astro_ctx_element_flux(astro_ctx* ctx, const double* sp, double* flux, double* grad);
+/* Co-rotational internal force of a shell4 model (the shell branch of
+ * r_getElRuk with nonLinearGeom, Working/AStrO_r_elementEqns.f90:2245-2423,
+ * kinematics of r_getInstOrient / r_getInstDof / r_getShellDef, :681-891):
+ * the element frame follows the mean nodal rotation by a finite rotation,
+ * R = dE/du by complex step on the element's 32 DOFs.  u: elMatDim doubles
+ * (null: the current nodeDisp); iters (0..64) chord steps on the element's
+ * 8 internal DOFs, which start from the linear recovery; force: elMatDim
+ * doubles, the condensed element forces at the nodal DOFs; elEnergy:
+ * numEls doubles (or null).  No counterpart to the element's own internal
+ * DOF recovery in the reference (it carries them at zero).  Synchronous. */
+int astro_ctx_shell_internal_force(astro_ctx* ctx, const double* u, int32_t iters, double* force,
+                                   double* elEnergy);
 
 /* Newmark dynamics, the dynamic branch of solve()
  * (Working/AStrO_commandFunctions.f90:762-810; element residual

@@ -44,7 +44,7 @@ EXPORTS = [
     "astro_gnn_train", "astro_gnn_loss_grad", "astro_gnn_field_size", "astro_gnn_get_field",
     "astro_gnn_field_blocks", "astro_ctx_gnn_attach", "astro_ctx_gnn_detach", "astro_ctx_gnn_get",
     "astro_ctx_set_design_vars", "astro_ctx_dRdD_vars", "astro_ctx_dRtdD_vars", "astro_ctx_objective",
-    "astro_ctx_objective_gradient", "astro_ctx_element_flux",
+    "astro_ctx_objective_gradient", "astro_ctx_element_flux", "astro_ctx_shell_internal_force",
 ]
 
 # Design-variable categories and objective terms (include/astro_abi.h).
@@ -206,6 +206,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_cs_batch_tile.argtypes = [C.c_void_p, C.c_int32]
     L.astro_ctx_element_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.astro_ctx_element_flux.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.astro_ctx_shell_internal_force.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _dp, _dp]
     L.astro_ctx_thermal_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.astro_ctx_thermal_dynamic_run.argtypes = [C.c_void_p, C.c_int]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -654,6 +655,22 @@ class Context:
         fl, gr = np.zeros(3 * E), np.zeros(3 * E)
         _check(load_library().astro_ctx_element_flux(self.h, np.ascontiguousarray(point, np.float64), fl, gr))
         return fl.reshape(E, 3), gr.reshape(E, 3)
+
+    def shell_internal_force(self, u=None, iters=16):
+        """Co-rotational internal force (elMatDim) and per-element strain
+        energy (numEls) of a shell4 model at the displacement u (elMatDim;
+        None: the current solution)."""
+        n = self.scalar("elMatDim")
+        E = self.model.num_els
+        f, en = np.zeros(n), np.zeros(E)
+        up = None
+        if u is not None:
+            u = np.ascontiguousarray(u, np.float64)
+            if u.shape != (n,):
+                raise ValueError(f"u must hold {n} values")
+            up = u.ctypes.data
+        _check(load_library().astro_ctx_shell_internal_force(self.h, up, int(iters), f, en))
+        return f, en
 
     def cs_batch_gradient(self):
         """The compliance gradient by direct complex-step solves of all design

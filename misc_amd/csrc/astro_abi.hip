@@ -1564,6 +1564,42 @@ int astro_ctx_element_flux(astro_ctx* c, const double* sp, double* flux, double*
     return ASTRO_OK;
 }
 
+// Co-rotational internal force of a shell4 model at the displacement u
+// (host, elMatDim; null: the current nodeDisp): the shell branch of
+// r_getElRuk with nonlinear geometry, Working/AStrO_r_elementEqns.f90:2245-2423.
+// force: host elMatDim, the condensed element forces summed at the nodal DOFs;
+// elEnergy: host numEls (or null), each element's strain energy.  iters chord
+// steps bring the element's internal DOFs to equilibrium.
+int astro_ctx_shell_internal_force(astro_ctx* c, const double* u, int32_t iters, double* force, double* elEnergy) {
+    if (!c || !force) return fail(ASTRO_ERR_ARG, "null argument");
+    if (!c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "the co-rotational force covers shell4 elements");
+    if (iters < 0 || iters > 64) return fail(ASTRO_ERR_ARG, "internal-DOF iterations must lie in [0, 64]");
+    const int64_t n = c->prep.elMatDim;
+    const int E = c->prep.numEls;
+    DBuf<double> du, df, de;
+    const double* ud = c->nodeDisp.p;
+    if (u) {
+        ASTRO_TRY(du.alloc((size_t)n));
+        HIP_TRY(hipMemcpyAsync(du.p, u, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+        ud = du.p;
+    }
+    ASTRO_TRY(df.alloc((size_t)n));
+    ASTRO_TRY(de.alloc((size_t)E));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->s, df.p, n, 0.0);
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_shell4_corot, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
+                           c->conn.p, c->X.p, c->elABD.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
+                           c->prep.numSectionNds, ud, 1e-30, (int)iters, df.p, de.p);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(force, df.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+    if (elEnergy) HIP_TRY(hipMemcpyAsync(elEnergy, de.p, sizeof(double) * E, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
 int astro_ctx_elastic_soln_load(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     ctx_minus_internal_force(c->s, c->csrVal, c->csrCols, c->csrRange, c->prep.elMatDim, c->nodeDisp.p, c->load.p);

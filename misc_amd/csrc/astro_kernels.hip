@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "astro_cx.h"
 #include "astro_dd.h"
 #include "astro_kernels.h"
 
@@ -354,6 +355,293 @@ __global__ __launch_bounds__(64) void k_shell4_tangent(
         const int slot = elPos[192 * (size_t)e + (a * 4 + b) * 6 + vj];
         val[csrRange[row - 1] + slot] += v;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Co-rotational shell4 strains at the four Gauss points for the 32 element
+// DOFs of sU (24 corner DOFs component-major u v w tx ty tz over the 4 nodes,
+// 8 internal) with DOF `pert` (< 0: none) carrying the imaginary part h:
+// Working/AStrO_r_elementEqns.f90:681-891 (r_getShellDef, r_getInstOrient,
+// r_getInstDof) with the finite rotation of r_rotateAlpha
+// (Working/AStrO_r_designPropertyFunctions.f90:1075-1463).  The frame
+// follows the mean nodal rotation t: instOri = alpha R(t)^T with
+// R = I + a K + b K^2, K = [t]x, a = sin r / r, b = (1 - cos r) / r^2, r^2 = s =
+// t.t, as power series in s (so R and its first and second derivatives are
+// analytic in the DOFs; exact sin / cos, the reference's r_sin / r_cos are
+// not reproduced, D1); local displacements instOri (u + X) - alpha X, local
+// rotations drIdrG (theta - t) + 1/2 d2rIdrG (theta - t)^2 with
+// drIdrG(:, i) = axial(d instOri / dt_i instOri^T).
+__device__ void shell4_corot_strains(int pert, double h, const double (&sU)[32], const double (&sX)[4][3],
+                                     const double (&sAl)[3][3], const double (&sN)[4][10],
+                                     const double (&sNx)[4][10][2], cx (&ep)[4][9]) {
+    cx ue[32];
+    for (int k = 0; k < 32; ++k) ue[k] = mkc(sU[k], k == pert ? h : 0.0);
+    cx tb[3];
+    for (int c = 0; c < 3; ++c)
+        tb[c] = 0.25 * (ue[12 + 4 * c] + ue[12 + 4 * c + 1] + ue[12 + 4 * c + 2] + ue[12 + 4 * c + 3]);
+    const cx s = tb[0] * tb[0] + tb[1] * tb[1] + tb[2] * tb[2];
+    // a, b, a', b', a'', b'' as power series in s (24 terms)
+    cx a = mkc(0, 0), b = mkc(0, 0), da = mkc(0, 0), db = mkc(0, 0), d2a = mkc(0, 0), d2b = mkc(0, 0);
+    cx pk = mkc(1, 0), pk1 = mkc(0, 0), pk2 = mkc(0, 0);
+    double ca = 1.0, cb = 0.5;
+    for (int k = 0; k < 24; ++k) {
+        a = a + ca * pk;
+        b = b + cb * pk;
+        da = da + (k * ca) * pk1;
+        db = db + (k * cb) * pk1;
+        d2a = d2a + (k * (k - 1) * ca) * pk2;
+        d2b = d2b + (k * (k - 1) * cb) * pk2;
+        pk2 = pk1;
+        pk1 = pk;
+        pk = pk * s;
+        ca *= -1.0 / ((2.0 * k + 2.0) * (2.0 * k + 3.0));
+        cb *= -1.0 / ((2.0 * k + 3.0) * (2.0 * k + 4.0));
+    }
+    auto skew = [](const cx (&v)[3], cx (&M)[3][3]) {
+        const cx z = mkc(0, 0);
+        M[0][0] = z; M[0][1] = -v[2]; M[0][2] = v[1];
+        M[1][0] = v[2]; M[1][1] = z; M[1][2] = -v[0];
+        M[2][0] = -v[1]; M[2][1] = v[0]; M[2][2] = z;
+    };
+    auto mm = [](const cx (&A)[3][3], const cx (&B)[3][3], cx (&C)[3][3]) {
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) C[p][q] = A[p][0] * B[0][q] + A[p][1] * B[1][q] + A[p][2] * B[2][q];
+    };
+    cx K[3][3], K2[3][3], Em[3][3][3];
+    skew(tb, K);
+    mm(K, K, K2);
+    for (int i = 0; i < 3; ++i) {
+        cx ev[3] = {mkc(i == 0, 0), mkc(i == 1, 0), mkc(i == 2, 0)};
+        skew(ev, Em[i]);
+    }
+    // instOri = alpha R^T
+    cx I[3][3];
+    {
+        cx R[3][3];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) R[p][q] = a * K[p][q] + b * K2[p][q] + (p == q ? 1.0 : 0.0);
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q)
+                I[p][q] = sAl[p][0] * R[q][0] + sAl[p][1] * R[q][1] + sAl[p][2] * R[q][2];
+    }
+    // axial(alpha M^T I^T) for a derivative M of R
+    auto axial_of = [&](const cx (&M)[3][3], cx (&w)[3]) {
+        cx A[3][3];  // alpha M^T
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) A[p][q] = sAl[p][0] * M[q][0] + sAl[p][1] * M[q][1] + sAl[p][2] * M[q][2];
+        auto P = [&](int p, int q) { return A[p][0] * I[q][0] + A[p][1] * I[q][1] + A[p][2] * I[q][2]; };
+        w[0] = P(1, 2);
+        w[1] = P(2, 0);
+        w[2] = P(0, 1);
+    };
+    cx d1[3][3], d2[3][3][3];
+    for (int i = 0; i < 3; ++i) {
+        cx M[3][3], EK[3][3], KE[3][3];
+        mm(Em[i], K, EK);
+        mm(K, Em[i], KE);
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q)
+                M[p][q] = (2.0 * tb[i]) * (da * K[p][q] + db * K2[p][q]) + a * Em[i][p][q] + b * (EK[p][q] + KE[p][q]);
+        cx w[3];
+        axial_of(M, w);
+        for (int c = 0; c < 3; ++c) d1[c][i] = w[c];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j) {
+            cx M[3][3], EiK[3][3], KEi[3][3], EjK[3][3], KEj[3][3], EiEj[3][3], EjEi[3][3];
+            mm(Em[i], K, EiK);
+            mm(K, Em[i], KEi);
+            mm(Em[j], K, EjK);
+            mm(K, Em[j], KEj);
+            mm(Em[i], Em[j], EiEj);
+            mm(Em[j], Em[i], EjEi);
+            const double dij = i == j ? 1.0 : 0.0;
+            const cx fK = 4.0 * (d2a * tb[i] * tb[j]) + (2.0 * dij) * da;
+            const cx fK2 = 4.0 * (d2b * tb[i] * tb[j]) + (2.0 * dij) * db;
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q)
+                    M[p][q] = fK * K[p][q] + (2.0 * da) * (tb[i] * Em[j][p][q] + tb[j] * Em[i][p][q]) + fK2 * K2[p][q] +
+                              (2.0 * db) * (tb[i] * (EjK[p][q] + KEj[p][q]) + tb[j] * (EiK[p][q] + KEi[p][q])) +
+                              b * (EiEj[p][q] + EjEi[p][q]);
+            cx w[3];
+            axial_of(M, w);
+            for (int c = 0; c < 3; ++c) {
+                d2[c][i][j] = w[c];
+                d2[c][j][i] = w[c];
+            }
+        }
+    // local displacements (corner nodes and bubbles) and rotations
+    cx disp[3][10], rot[3][4];
+    for (int c = 0; c < 3; ++c)
+        for (int n = 4; n < 10; ++n) disp[c][n] = mkc(0, 0);
+    for (int nd = 0; nd < 4; ++nd) {
+        cx ux[3], th[3];
+        for (int c = 0; c < 3; ++c) {
+            ux[c] = ue[4 * c + nd] + sX[nd][c];
+            th[c] = ue[12 + 4 * c + nd] - tb[c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            disp[c][nd] = I[c][0] * ux[0] + I[c][1] * ux[1] + I[c][2] * ux[2] -
+                          mkc(sAl[c][0] * sX[nd][0] + sAl[c][1] * sX[nd][1] + sAl[c][2] * sX[nd][2], 0.0);
+            cx r = d1[c][0] * th[0] + d1[c][1] * th[1] + d1[c][2] * th[2];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) r = r + 0.5 * (d2[c][i][j] * th[i] * th[j]);
+            rot[c][nd] = r;
+        }
+    }
+    // internal DOF k: (component, bubble function) = (0, 4) (0, 5) (1, 4) (1, 5) (2, 6..9)
+    for (int k = 0; k < 8; ++k) {
+        const int c = k < 2 ? 0 : (k < 4 ? 1 : 2);
+        const int fn = k < 4 ? 4 + (k & 1) : 6 + (k - 4);
+        disp[c][fn] = disp[c][fn] + ue[24 + k];
+    }
+    for (int g = 0; g < 4; ++g) {
+        cx ux[3][2], rx[3][2], pr[3];
+        for (int c = 0; c < 3; ++c) {
+            for (int j = 0; j < 2; ++j) {
+                cx acc = mkc(0, 0);
+                for (int n = 0; n < 10; ++n) acc = acc + sNx[g][n][j] * disp[c][n];
+                ux[c][j] = acc;
+                cx ar = mkc(0, 0);
+                for (int n = 0; n < 4; ++n) ar = ar + sNx[g][n][j] * rot[c][n];
+                rx[c][j] = ar;
+            }
+            cx ap = mkc(0, 0);
+            for (int n = 0; n < 4; ++n) ap = ap + sN[g][n] * rot[c][n];
+            pr[c] = ap;
+        }
+        ep[g][0] = ux[0][0];
+        ep[g][1] = ux[1][1];
+        ep[g][2] = ux[0][1] + ux[1][0];
+        ep[g][3] = rx[1][0];
+        ep[g][4] = -rx[0][1];
+        ep[g][5] = rx[1][1] - rx[0][0];
+        ep[g][6] = ux[2][0] + pr[1];
+        ep[g][7] = ux[2][1] - pr[0];
+        ep[g][8] = 2.0 * pr[2] - ux[1][0] + ux[0][1];
+    }
+}
+
+// Co-rotational internal force of shell4 elements at the displacement u
+// (the shell branch of r_getElRuk, Working/AStrO_r_elementEqns.f90:2245-2423):
+// R = dE/du of E = 1/2 sum over the Gauss points of w e^T ABD e, lane d taking
+// the derivative along DOF d by complex step (one strain evaluation per lane,
+// step h).  The 8 internal DOFs start from the linear recovery
+// -K_ii^-1 K_ie u_e and are brought to element equilibrium R_i = 0 by `iters`
+// chord steps with K_ii at rest (the reference never recovers them, D7); the
+// condensed force R_e - K_ei K_ii^-1 R_i is added into force at the element's
+// DOFs (elements of one launch share no node) and E into energy[e].  At rest
+// the derivative of this force is the condensed tangent of k_shell4_tangent;
+// a finite rigid rotation gives zero force.
+__global__ __launch_bounds__(64) void k_shell4_corot(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ elABD, const double* __restrict__ elAlpha, const int* __restrict__ rank, int nSolid,
+    int nSec, const double* __restrict__ u, double h, int iters, double* __restrict__ force,
+    double* __restrict__ energy) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[4][3], sAl[3][3], sD[9][9];
+    __shared__ double sN[4][10], sNx[4][10][2], sW[4];
+    __shared__ double sB[4][9][32], sDB[4][9][32];
+    __shared__ double sK[32][33];
+    __shared__ double sU[32], sR[32], sEn;
+    __shared__ int sNode[4], sDof[24];
+    for (int k = lane; k < 81; k += 64) sD[k % 9][k / 9] = elABD[81 * (size_t)e + k];
+    shell4_stage_B(e, lane, conn, X, elAlpha, sX, sAl, sN, sNx, sW, sB, sNode);
+    __syncthreads();
+    for (int k = lane; k < 4 * 9 * 32; k += 64) {
+        const int g = k / 288, r = (k / 32) % 9, d = k % 32;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) acc += sD[r][m] * sB[g][m][d];
+        sDB[g][r][d] = sW[g] * acc;
+    }
+    __syncthreads();
+    for (int k = lane; k < 32 * 32; k += 64) {
+        const int i = k >> 5, j = k & 31;
+        double acc = 0.0;
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) acc += sB[g][m][i] * sDB[g][m][j];
+        sK[i][j] = acc;
+    }
+    if (lane < 24) {
+        const int vi = lane >> 2, nd = lane & 3;
+        const int dof = rank[sNode[nd]] + vi * nSec + 2 * nSolid;
+        sDof[lane] = dof;
+        sU[lane] = u[dof - 1];
+    }
+    __syncthreads();
+    // LU (no pivoting) of the internal block in place, as k_shell4_tangent;
+    // then the linear recovery of the internal DOFs
+    auto solve_ii = [&](const double* rhs, double* x) {
+        double z[8];
+        for (int i = 0; i < 8; ++i) {
+            double v = rhs[i];
+            for (int k = 0; k < i; ++k) v -= sK[24 + i][24 + k] * z[k];
+            z[i] = v;
+        }
+        for (int i = 7; i >= 0; --i) {
+            double v = z[i];
+            for (int k = i + 1; k < 8; ++k) v -= sK[24 + i][24 + k] * x[k];
+            x[i] = v / sK[24 + i][24 + i];
+        }
+    };
+    if (lane == 0) {
+        for (int i = 24; i < 32; ++i) {
+            for (int j = i; j < 32; ++j)
+                for (int k = 24; k < i; ++k) sK[i][j] -= sK[i][k] * sK[k][j];
+            const double f = 1.0 / sK[i][i];
+            for (int j = i + 1; j < 32; ++j) {
+                for (int k = 24; k < i; ++k) sK[j][i] -= sK[j][k] * sK[k][i];
+                sK[j][i] *= f;
+            }
+        }
+        double rhs[8], x[8];
+        for (int i = 0; i < 8; ++i) {
+            double v = 0.0;
+            for (int j = 0; j < 24; ++j) v -= sK[24 + i][j] * sU[j];
+            rhs[i] = v;
+        }
+        solve_ii(rhs, x);
+        for (int i = 0; i < 8; ++i) sU[24 + i] = x[i];
+    }
+    __syncthreads();
+    for (int it = 0; it <= iters; ++it) {
+        if (lane < 32) {
+            cx ep[4][9];
+            shell4_corot_strains(lane, h, sU, sX, sAl, sN, sNx, ep);
+            double r = 0.0, en = 0.0;
+            for (int g = 0; g < 4; ++g)
+                for (int m = 0; m < 9; ++m) {
+                    double s = 0.0;
+                    for (int k = 0; k < 9; ++k) s += sD[m][k] * ep[g][k].r;
+                    r += sW[g] * (ep[g][m].i / h) * s;
+                    en += 0.5 * sW[g] * ep[g][m].r * s;
+                }
+            sR[lane] = r;
+            if (lane == 0) sEn = en;
+        }
+        __syncthreads();
+        if (it == iters) break;
+        if (lane == 0) {
+            double x[8];
+            solve_ii(&sR[24], x);
+            for (int i = 0; i < 8; ++i) sU[24 + i] -= x[i];
+        }
+        __syncthreads();
+    }
+    __shared__ double sY[8];
+    if (lane == 0) solve_ii(&sR[24], sY);
+    __syncthreads();
+    if (lane < 24) {
+        double v = sR[lane];
+        for (int m = 0; m < 8; ++m) v -= sK[lane][24 + m] * sY[m];
+        force[sDof[lane] - 1] += v;
+    }
+    if (lane == 0 && energy) energy[e] = sEn;
 }
 
 // ---------------------------------------------------------------------------
