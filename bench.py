@@ -543,6 +543,8 @@ def main():
     ap.add_argument("--nz", type=int, default=32)
     ap.add_argument("--maxbw", choices=["literal", "int64"], default="int64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-literal", action="store_true",
+                    help="static: skip the second solve under the literal solverMaxBW rule (for profiling)")
     ap.add_argument("--workload", choices=["static", "precision", "newmark", "learned", "sensitivity"],
                     default="static",
                     help="static: the 64x64x32 linear-static block (default); newmark: the 48x48x24 "
@@ -655,7 +657,7 @@ def main():
     ctx.close()
     # the same workload under the reference's literal (32-bit) solverMaxBW rule
     lit = None
-    if args.maxbw == "int64":
+    if args.maxbw == "int64" and not args.no_literal:
         lctx = astro.Context(model, astro.Options(literalMaxBW=True))
         lctx.static_solve()
         lctx.reset()
