@@ -165,7 +165,9 @@ int astro_ctx_destroy(astro_ctx* ctx);
 
 /* Scalars: "elMatDim", "elMatSize", "elMatLTSize", "numSolidNds", "solverMaxBW",
  * "elMPCDim", "thermMatSize", "thermMatLTSize", "gmresSteps", "gmresCycles",
- * "numColors".  Returns -1 for an unknown name. */
+ * "numColors", "nodeBlockNeighbours" (neighbour slots per node of the
+ * node-block form of the elastic operator that GMRES applies; 0 when it
+ * applies the CSR).  Returns -1 for an unknown name. */
 int64_t astro_ctx_scalar(const astro_ctx* ctx, const char* name);
 double astro_ctx_scalar_f(const astro_ctx* ctx, const char* name);
 

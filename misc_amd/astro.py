@@ -706,6 +706,11 @@ class Context:
         _check(load_library().astro_ctx_true_residual(self.h, C.byref(r)))
         return r.value
 
+    def operator_apply(self, d_x: int, d_y: int):
+        """y = (A + C^T C) x on device vectors at addresses d_x, d_y: the
+        operator GMRES applies (node-block form for brick8 models)."""
+        _check(load_library().astro_ctx_operator_apply(self.h, C.c_void_p(d_x), C.c_void_p(d_y)))
+
     def precond_apply(self, d_b: int, d_x: int):
         """x = P^-1 b on device vectors at addresses d_b, d_x (distinct)."""
         _check(load_library().astro_ctx_precond_apply(self.h, C.c_void_p(d_b), C.c_void_p(d_x)))

@@ -580,6 +580,11 @@ struct DeviceOperator {
     const int* ctPos = nullptr;
     const int* ctRow = nullptr;
     double* ctT = nullptr;
+    // node-block form of A (brick8 models, k_spmv_nodes): nbK neighbour
+    // slots per node over nbN node slots; 0 = the CSR form
+    int nbK = 0, nbN = 0;
+    const int* nb = nullptr;
+    const double* bv = nullptr;
     void set_transpose(DeviceMpcT& T) {
         ctRange = T.range.p;
         ctPos = T.pos.p;
@@ -588,8 +593,11 @@ struct DeviceOperator {
     }
     // gate (device int, may be null): every kernel returns at once while *gate == 0
     int apply(const double* x, double* y, hipStream_t s, const int* gate = nullptr) const {
-        hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val, cols,
-                           range, n, x, y, gate);
+        if (nbK > 0)
+            hipLaunchKernelGGL(k_spmv_nodes, dim3((nbN + 255) / 256), dim3(256), 0, s, bv, nb, nbK, nbN, x, y, gate);
+        else
+            hipLaunchKernelGGL(k_spmv, dim3(grid_for((int64_t)n * 32, 256, 1 << 30)), dim3(256), 0, s, val, cols,
+                               range, n, x, y, gate);
         if (cdim > 0)
             hipLaunchKernelGGL(k_mpc_apply, dim3(grid_for(cdim, 256)), dim3(256), 0, s, cmat, ccols, crange, cdim, x,
                                y, gate);
@@ -892,6 +900,12 @@ struct astro_ctx {
     DeviceProfile32 prof32;  // the elastic profile in FP32 storage (precision = 1)
     bool mixed = false;      // precision = 1: the FP32 profiles are the ones in use
     DeviceOperator op;
+    // node-block operator: neighbour slots; per CSR slot its block position
+    // 3 j + d (0xff: unused) for the tiled copy (K <= 32), else per block
+    // entry its CSR position
+    DBuf<int> nbIdx, nbMap;
+    DBuf<uint8_t> nbJD;
+    DBuf<double> nbVal;
     DeviceMpcT mpcT;
     Gmres gm;
     // thermal system
@@ -988,6 +1002,110 @@ extern "C" {
 
 const char* astro_last_error(void) { return g_lastError.c_str(); }
 const char* astro_version(void) { return "astro-mi355x 0.1 (gfx950, FP64)"; }
+
+// The node-block form of the elastic operator (k_spmv_nodes) for models whose
+// DOFs are three components per solid node, DOF = rank + c nS: from the CSR
+// structure, each node's neighbours (ascending node slot) and, per block
+// entry, the CSR position that holds it.  Left in
+// the CSR form when a node's three rows do not share one neighbour list, a
+// node has more than 64 neighbours, or the padded blocks would hold more than
+// twice the CSR's slots (boundary nodes have fewer neighbours: 1.98x on a
+// 3x2x2 block, 1.04x on 64x64x32).
+static int build_node_operator(astro_ctx* c) {
+    const auto& p = c->prep;
+    const int n = p.elMatDim, N = p.numSolidNds;
+    c->op.nbK = 0;
+    if (p.shellMesh || N <= 0 || n != 3 * N) return ASTRO_OK;
+    const int* R = p.elMatRange.data();
+    const int* Cc = p.elMatCols.data();
+    std::vector<int> pos(N, -1);
+    int K = 0;
+    for (int a = 0; a < N; ++a) {
+        int m = 0;
+        for (int q = R[a]; q < R[a + 1]; ++q) {
+            if (Cc[q] <= 0) continue;
+            const int b = (Cc[q] - 1) % N;
+            if (pos[b] < 0) pos[b] = m++;
+        }
+        for (int q = R[a]; q < R[a + 1]; ++q)
+            if (Cc[q] > 0) pos[(Cc[q] - 1) % N] = -1;
+        K = std::max(K, m);
+    }
+    if (K == 0 || K > 64 || 9.0 * K * N > 2.0 * (double)p.elMatCols.size()) return ASTRO_OK;
+    std::vector<int> nbh((size_t)K * N, -1), map((size_t)9 * K * N, -1), lst;
+    for (int a = 0; a < N; ++a) {
+        // neighbours in ascending slot order: lanes of consecutive slots then
+        // read x of neighbouring nodes
+        lst.clear();
+        for (int q = R[a]; q < R[a + 1]; ++q) {
+            if (Cc[q] <= 0) continue;
+            const int b = (Cc[q] - 1) % N;
+            if (pos[b] < 0) {
+                pos[b] = 0;
+                lst.push_back(b);
+            }
+        }
+        std::sort(lst.begin(), lst.end());
+        const int m = (int)lst.size();
+        for (int j = 0; j < m; ++j) {
+            pos[lst[j]] = j;
+            nbh[(size_t)j * N + a] = lst[j];
+        }
+        bool ok = true;
+        for (int cc = 0; cc < 3 && ok; ++cc) {
+            const int row = a + cc * N;
+            for (int q = R[row]; q < R[row + 1]; ++q) {
+                if (Cc[q] <= 0) continue;
+                const int b = (Cc[q] - 1) % N, d = (Cc[q] - 1) / N;
+                const int j = pos[b];
+                if (j < 0 || map[(size_t)(9 * j + 3 * cc + d) * N + a] >= 0) {
+                    ok = false;
+                    break;
+                }
+                map[(size_t)(9 * j + 3 * cc + d) * N + a] = q;
+            }
+        }
+        for (int j = 0; j < m; ++j) pos[nbh[(size_t)j * N + a]] = -1;
+        if (!ok) return ASTRO_OK;  // rows of one node with different neighbours: CSR form
+    }
+    ASTRO_TRY(c->nbIdx.upload(nbh, c->s));
+    if (K <= 32) {
+        std::vector<uint8_t> jd(p.elMatCols.size(), 0xff);
+        for (size_t t = 0; t < map.size(); ++t) {
+            if (map[t] < 0) continue;
+            const size_t blockRow = t / N;  // 9 j + 3 c + d
+            const int j = (int)(blockRow / 9), d = (int)(blockRow % 3);
+            jd[map[t]] = (uint8_t)(3 * j + d);
+        }
+        ASTRO_TRY(c->nbJD.upload(jd, c->s));
+        HIP_TRY(hipStreamSynchronize(c->s));  // jd is a local array
+    } else {
+        ASTRO_TRY(c->nbMap.upload(map, c->s));
+    }
+    ASTRO_TRY(c->nbVal.alloc(map.size()));
+    HIP_TRY(hipStreamSynchronize(c->s));  // nbh and map are local arrays
+    c->op.nbK = K;
+    c->op.nbN = N;
+    c->op.nb = c->nbIdx.p;
+    c->op.bv = c->nbVal.p;
+    return ASTRO_OK;
+}
+
+// Copies the CSR values into the node-block form after every write to them.
+static int refresh_node_operator(astro_ctx* c) {
+    const int K = c->op.nbK, N = c->op.nbN;
+    if (K <= 0) return ASTRO_OK;
+    if (K <= 32) {
+        hipLaunchKernelGGL(k_node_values, dim3((N + 63) / 64, 3), dim3(256), 0, c->s, c->csrVal.p, c->csrRange.p,
+                           c->nbJD.p, K, N, c->nbVal.p);
+    } else {
+        const int64_t cnt = (int64_t)c->nbMap.n;
+        hipLaunchKernelGGL(k_gather_values, dim3(grid_for(cnt, 256, 8192)), dim3(256), 0, c->s, c->csrVal.p,
+                           c->nbMap.p, cnt, c->nbVal.p);
+    }
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
 
 static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
     hipStream_t s = c->s;
@@ -1118,6 +1236,7 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
     c->op.crange = c->mpcRange.p;
     ASTRO_TRY(c->mpcT.upload(p.elMPCMatRange.data(), p.elMPCMatCols.data(), p.elMPCDim, n, s));
     c->op.set_transpose(c->mpcT);
+    ASTRO_TRY(build_node_operator(c));
     ASTRO_TRY(c->gm.setup(n, c->numVecs));
     return ASTRO_OK;
 }
@@ -1282,6 +1401,7 @@ int64_t astro_ctx_scalar(const astro_ctx* c, const char* name) {
     if (s == "dynEagerSteps") return c->dynEagerSteps;
     if (s == "dynGraphs") return c->dynGraphs;
     if (s == "dynGraphCap") return c->dynGraphCap;
+    if (s == "nodeBlockNeighbours") return c->op.nbK;
     return -1;
 }
 
@@ -1386,6 +1506,7 @@ int astro_ctx_set_f64(astro_ctx* c, const char* name, const double* host, int64_
     }
     if (!d) return fail(ASTRO_ERR_ARG, std::string("unknown double array ") + name);
     HIP_TRY(hipMemcpyAsync(d, host, sizeof(double) * count, hipMemcpyHostToDevice, c->s));
+    if (nm == "elasticMat") ASTRO_TRY(refresh_node_operator(c));
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
 }
@@ -1416,7 +1537,7 @@ int astro_ctx_assemble_elastic(astro_ctx* c, double massFactor) {
                            nS, c->csrRange.p, c->elPos.p, c->csrVal.p);
     }
     HIP_TRY(hipGetLastError());
-    return ASTRO_OK;
+    return refresh_node_operator(c);
 }
 
 int astro_ctx_scale_mpc(astro_ctx* c) {

@@ -137,6 +137,10 @@ __global__ void k_sweep_backward(const SweepTile* tiles, int numTiles, const LT*
                                  int* flags, int* ticket, int epoch, long spinLimit, int* timeout, const double* diag, const int* gate);
 __global__ void k_reset_ticket(int* ticket, const int* gate);
 __global__ void k_spmv(const double* val, const int* cols, const int* range, int n, const double* x, double* y, const int* gate);
+__global__ void k_spmv_nodes(const double* bv, const int* nb, int K, int N, const double* x, double* y,
+                             const int* gate);
+__global__ void k_gather_values(const double* val, const int* map, int64_t count, double* bv);
+__global__ void k_node_values(const double* val, const int* range, const uint8_t* jd, int K, int N, double* bv);
 __global__ void k_mpc_dd(const double* cmat, const int* ccols, const int* crange, int cdim, const double* x,
                          double* t, const int* gate);
 __global__ void k_residual_dd(const double* val, const int* cols, const int* range, int n, const int* ctRange,

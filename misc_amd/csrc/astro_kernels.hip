@@ -2686,6 +2686,90 @@ __global__ __launch_bounds__(256) void k_spmv(const double* __restrict__ val, co
     if (l == 0) y[row] = s;
 }
 
+// The same operator's A x in the node-block form of a brick8 model (three
+// displacement components per node, DOF d of node slot b = b + d N, 0-based):
+// one lane per node slot a forms its three rows a, a + N, a + 2 N.  Neighbour
+// j of slot a is nb[j N + a] (-1: none); the entry (row a + c N, column
+// b + d N) of neighbour b = nb[j N + a] sits at bv[(9 j + 3 c + d) N + a].
+// Index and value loads are coalesced across lanes (slot-major), the column
+// indices are shared by the node's three rows and nine entries (4 bytes per
+// nine values instead of per value), and each neighbour's three x values
+// serve all three rows.  Per row the products are summed in neighbour order.
+__global__ __launch_bounds__(256) void k_spmv_nodes(const double* __restrict__ bv, const int* __restrict__ nb, int K,
+                                                    int N, const double* __restrict__ x, double* __restrict__ y,
+                                                    const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < K; ++j) {
+        const int b = nb[(size_t)j * N + a];
+        const double* v = bv + (size_t)9 * j * N + a;
+        double w[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = v[(size_t)k * N];
+        if (b >= 0) {
+            const double x0 = x[b], x1 = x[b + N], x2 = x[b + 2 * N];
+            s0 += w[0] * x0 + w[1] * x1 + w[2] * x2;
+            s1 += w[3] * x0 + w[4] * x1 + w[5] * x2;
+            s2 += w[6] * x0 + w[7] * x1 + w[8] * x2;
+        }
+    }
+    y[a] = s0;
+    y[a + N] = s1;
+    y[a + 2 * N] = s2;
+}
+
+// bv[q] = val[map[q]] (0 where map[q] < 0): the node-block copy of the CSR
+// values, refreshed after every assembly.
+__global__ void k_gather_values(const double* __restrict__ val, const int* __restrict__ map, int64_t count,
+                                double* __restrict__ bv) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
+        const int m = map[q];
+        bv[q] = m >= 0 ? val[m] : 0.0;
+    }
+}
+
+// The node-block copy by tiles: workgroup (t, c) takes node slots
+// a0 = 64 t .. a0 + 63 and component c, i.e. CSR rows a0 + c N .. a0 + 63 + c N,
+// which are consecutive, so their slots form one contiguous range read once
+// and coalesced.  Each slot's block position 3 j + d (jd[q], from the CSR
+// column) and its row (a search over the tile's 65 row starts) place the
+// value into an LDS tile T[3 j + d][a - a0]; the tile's rows are then
+// written to bv[(9 j + 3 c + d) N + a] for 64 consecutive a.  3 K <= 96.
+__global__ __launch_bounds__(256) void k_node_values(const double* __restrict__ val, const int* __restrict__ range,
+                                                     const uint8_t* __restrict__ jd, int K, int N,
+                                                     double* __restrict__ bv) {
+    __shared__ double T[96][65];
+    __shared__ int rs[65];
+    const int a0 = 64 * blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+    const int na = min(64, N - a0);
+    const int r0 = a0 + c * N;
+    if (tid <= na) rs[tid] = range[r0 + tid];
+    for (int id = tid; id < 3 * K * 64; id += 256) T[id >> 6][id & 63] = 0.0;
+    __syncthreads();
+    const int q0 = rs[0], q1 = rs[na];
+    for (int q = q0 + tid; q < q1; q += 256) {
+        const int k = jd[q];
+        if (k == 0xff) continue;  // unused slot
+        int lo = 0, hi = na;      // rs[lo] <= q < rs[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rs[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        T[k][lo] = val[q];
+    }
+    __syncthreads();
+    for (int id = tid; id < 3 * K * 64; id += 256) {
+        const int k = id >> 6, a = id & 63;
+        if (a < na) {
+            const int j = k / 3, d = k - 3 * j;
+            bv[(size_t)(9 * j + 3 * c + d) * N + a0 + a] = T[k][a];
+        }
+    }
+}
+
 // t[r] = (C x)_r as a double-double pair (t[2r], t[2r + 1]).
 __global__ void k_mpc_dd(const double* __restrict__ cmat, const int* __restrict__ ccols,
                          const int* __restrict__ crange, int cdim, const double* __restrict__ x,
