@@ -123,6 +123,14 @@ typedef struct {
     /* Specific heat per element (may be null: no capacitance); the transient
      * thermal path uses density * specHeat. */
     const double* specHeat;   /* numEls */
+    /* Thermal expansion per element (may be null: none), the material's
+     * "expansion" (Working/AStrO_input.f90:751-760, r_getMaterialThermExp,
+     * Working/AStrO_r_designPropertyFunctions.f90:105-127) in the strain's
+     * Voigt order 11, 22, 33, 12, 13, 23 with engineering shear.  brick8:
+     * the element stress is C (strain - T alpha) with T the nodal
+     * temperature interpolated to the point (nodeTemp, e.g. from a thermal
+     * solve on the same context). */
+    const double* thermExp;   /* numEls * 6 */
 } astro_model;
 
 /* Options of the analysis (solver parameters of the *solve command). */
@@ -232,6 +240,13 @@ int astro_ctx_element_stress(astro_ctx* ctx, const double* sp, double* stress, d
  * Working/AStrO_r_elementEqns.f90:1746-1799); flux, grad: 3 * numEls doubles.
  * brick8 models with a thermal system.  Synchronous. */
 int astro_ctx_element_flux(astro_ctx* ctx, const double* sp, double* flux, double* grad);
+/* (dR/dT)^T lam of a brick8 model with thermal expansion: lam elMatDim
+ * doubles, out numNodes doubles indexed like nodeTemp (by rank).  The
+ * thermal adjoint's coupling term, dLdt -= dRdT^T dispAdj, of
+ * solveDynamicAdjoint (Working/AStrO_commandFunctions.f90:960-999), with the
+ * element's intended dRdT = -sum_g detJ B0^T C alpha N^T (r_getElRuk's loop
+ * writes a stale row, D10).  Synchronous. */
+int astro_ctx_dRdT_product(astro_ctx* ctx, const double* lam, double* out);
 /* Co-rotational internal force of a shell4 model (the shell branch of
  * r_getElRuk with nonLinearGeom, Working/AStrO_r_elementEqns.f90:2245-2423,
  * kinematics of r_getInstOrient / r_getInstDof / r_getShellDef, :681-891):

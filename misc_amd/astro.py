@@ -45,6 +45,7 @@ EXPORTS = [
     "astro_gnn_field_blocks", "astro_ctx_gnn_attach", "astro_ctx_gnn_detach", "astro_ctx_gnn_get",
     "astro_ctx_set_design_vars", "astro_ctx_dRdD_vars", "astro_ctx_dRtdD_vars", "astro_ctx_objective",
     "astro_ctx_objective_gradient", "astro_ctx_element_flux", "astro_ctx_shell_internal_force",
+    "astro_ctx_dRdT_product",
 ]
 
 # Design-variable categories and objective terms (include/astro_abi.h).
@@ -76,7 +77,7 @@ class _Model(C.Structure):
                 ("mpcNodeStart", C.c_void_p), ("mpcNodes", C.c_void_p), ("numMPC", C.c_int32),
                 ("mpcRHS", C.c_void_p), ("numLoads", C.c_int32), ("loadNode", C.c_void_p),
                 ("loadVal", C.c_void_p), ("numLayers", C.c_int32), ("layers", C.c_void_p), ("zOffset", C.c_void_p),
-                ("orient", C.c_void_p), ("specHeat", C.c_void_p)]
+                ("orient", C.c_void_p), ("specHeat", C.c_void_p), ("thermExp", C.c_void_p)]
 
 
 class _Design(C.Structure):
@@ -207,6 +208,7 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_element_stress.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.astro_ctx_element_flux.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.astro_ctx_shell_internal_force.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _dp, _dp]
+    L.astro_ctx_dRdT_product.argtypes = [C.c_void_p, _dp, _dp]
     L.astro_ctx_thermal_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.astro_ctx_thermal_dynamic_run.argtypes = [C.c_void_p, C.c_int]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -376,6 +378,8 @@ def _marshal(model, o: "Options"):
         M.orient = ptr(model.orient.reshape(-1), np.float64)
     if getattr(model, "spec_heat", None) is not None:
         M.specHeat = ptr(model.spec_heat, np.float64)
+    if getattr(model, "therm_exp", None) is not None:
+        M.thermExp = ptr(np.asarray(model.therm_exp).reshape(-1), np.float64)
     O = _Options(int(o.maxBW), 1 if o.literalMaxBW else 0, 1 if o.thermLiteralD8 else 0, int(o.numVecs),
                  float(o.rtol), int(o.maxCycles), 1 if o.coupledProfile else 0, int(o.precision),
                  1 if o.stepGraphs else 0)
@@ -655,6 +659,14 @@ class Context:
         fl, gr = np.zeros(3 * E), np.zeros(3 * E)
         _check(load_library().astro_ctx_element_flux(self.h, np.ascontiguousarray(point, np.float64), fl, gr))
         return fl.reshape(E, 3), gr.reshape(E, 3)
+
+    def dRdT_product(self, lam):
+        """(dR/dT)^T lam (numNodes, by rank) for a displacement-sized lam of a
+        brick8 model with thermal expansion."""
+        lam = np.ascontiguousarray(lam, np.float64)
+        out = np.zeros(self.model.num_nodes)
+        _check(load_library().astro_ctx_dRdT_product(self.h, lam, out))
+        return out
 
     def shell_internal_force(self, u=None, iters=16):
         """Co-rotational internal force (elMatDim) and per-element strain

@@ -843,6 +843,8 @@ struct astro_ctx {
     int numVecs = 100;
     // model on device
     DBuf<double> X, matC, elScale, elRho, elCond, elCap;
+    DBuf<double> elTExp;  // thermal expansion, 6 per element (brick8), empty when the model has none
+    bool hasTExp = false;
     DBuf<double> elABD, elAlpha;  // shell4 models: layup stiffness 81, frame 9 per element
     int nLayers = 0;
     std::vector<double> hLayers, hZOff;  // host copy of the shell sections
@@ -1174,6 +1176,16 @@ static int ctx_upload_model(astro_ctx* c, const astro_model* m) {
         c->hLayers.assign(m->layers, m->layers + 6 * (size_t)m->numLayers * E);
         c->hZOff.assign(E, 0.0);
         if (m->zOffset) c->hZOff.assign(m->zOffset, m->zOffset + E);
+    }
+    if (m->thermExp && !p.shellMesh) {
+        for (size_t k = 0; k < 6 * (size_t)E && !c->hasTExp; ++k) c->hasTExp = m->thermExp[k] != 0.0;
+        if (c->hasTExp) {
+            ASTRO_TRY(c->elTExp.upload(m->thermExp, 6 * (size_t)E, s));
+            // the elastic residual reads nodal temperatures: zero until a
+            // thermal solve or the caller sets them
+            ASTRO_TRY(c->nodeTemp.alloc(N));
+            HIP_TRY(hipMemsetAsync(c->nodeTemp.p, 0, sizeof(double) * N, s));
+        }
     }
     if (m->eprops) c->hEprops.assign(m->eprops, m->eprops + 9 * (size_t)E);
     c->hConn.assign(m->elementList, m->elementList + 8 * (size_t)E);
@@ -1598,9 +1610,26 @@ static void ctx_minus_internal_force(hipStream_t s, const DBuf<double>& val, con
 // Right-hand side of a static step (stepSolve, Working/AStrO_commandFunctions.f90:
 // 716-723): -R(u) + nodal loads + C^T (rhs - C u).  With the internal force in
 // it, a second solve from a converged state leaves the state unchanged.
+// load += scale * the thermal load of the current nodeTemp (models with
+// thermal expansion; F - R(u, T) at u = 0 holds it).
+static int ctx_thermal_load(astro_ctx* c, double scale, double* load) {
+    if (!c->hasTExp) return ASTRO_OK;
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_brick8_thermal_strain, dim3((cnt + 127) / 128), dim3(128), 0, c->s, 0,
+                           c->perm.p + c->colorStart[k], cnt, c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p,
+                           c->elTExp.p, c->rank.p, c->prep.numSolidNds, (const double*)c->nodeTemp.p,
+                           (const double*)nullptr, scale, load);
+    }
+    HIP_TRY(hipGetLastError());
+    return ASTRO_OK;
+}
+
 static int ctx_rhs(astro_ctx* c) {
     const int n = c->prep.elMatDim;
     ctx_minus_internal_force(c->s, c->csrVal, c->csrCols, c->csrRange, n, c->nodeDisp.p, c->load.p);
+    ASTRO_TRY(ctx_thermal_load(c, 1.0, c->load.p));
     const int nl = (int)c->loadDofs.n;
     if (nl > 0)
         hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, c->s, c->loadDofs.p, c->loadVals.p,
@@ -1653,8 +1682,9 @@ int astro_ctx_element_stress(astro_ctx* c, const double* sp, double* stress, dou
     ASTRO_TRY(ds.alloc(6 * (size_t)E));
     ASTRO_TRY(de.alloc(6 * (size_t)E));
     hipLaunchKernelGGL(k_brick8_stress, dim3(grid_for(E, 256)), dim3(256), 0, c->s, E, c->conn.p, c->X.p, c->elMat.p,
-                       c->elScale.p, c->matC.p, c->rank.p, c->prep.numSolidNds, c->nodeDisp.p, sp[0], sp[1], sp[2],
-                       ds.p, de.p);
+                       c->elScale.p, c->matC.p, c->rank.p, c->prep.numSolidNds, c->nodeDisp.p,
+                       c->hasTExp ? (const double*)c->elTExp.p : nullptr, (const double*)c->nodeTemp.p, sp[0], sp[1],
+                       sp[2], ds.p, de.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(stress, ds.p, sizeof(double) * 6 * (size_t)E, hipMemcpyDeviceToHost, c->s));
     HIP_TRY(hipMemcpyAsync(strain, de.p, sizeof(double) * 6 * (size_t)E, hipMemcpyDeviceToHost, c->s));
@@ -1721,9 +1751,37 @@ int astro_ctx_shell_internal_force(astro_ctx* c, const double* u, int32_t iters,
     return ASTRO_OK;
 }
 
+// (dR/dT)^T lam for a displacement-sized lam (host, elMatDim) into out
+// (host, numNodes, indexed by rank like nodeTemp): the coupling term of the
+// thermal adjoint (solveDynamicAdjoint's dLdt -= dRdT^T dispAdj,
+// Working/AStrO_commandFunctions.f90:960-999) with the element's intended
+// dRdT (D10).  brick8 models with thermal expansion.
+int astro_ctx_dRdT_product(astro_ctx* c, const double* lam, double* out) {
+    if (!c || !lam || !out) return fail(ASTRO_ERR_ARG, "null argument");
+    if (!c->hasTExp) return fail(ASTRO_ERR_UNSUPPORTED, "the model has no thermal expansion");
+    const int n = c->prep.elMatDim, N = c->prep.numNodes;
+    DBuf<double> dl, dout;
+    ASTRO_TRY(dl.upload(lam, (size_t)n, c->s));
+    ASTRO_TRY(dout.alloc((size_t)N));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256, 4096)), dim3(256), 0, c->s, dout.p, (int64_t)N, 0.0);
+    for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+        const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_brick8_thermal_strain, dim3((cnt + 127) / 128), dim3(128), 0, c->s, 1,
+                           c->perm.p + c->colorStart[k], cnt, c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p,
+                           c->elTExp.p, c->rank.p, c->prep.numSolidNds, (const double*)nullptr,
+                           (const double*)dl.p, 1.0, dout.p);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->s));
+    HIP_TRY(hipStreamSynchronize(c->s));
+    return ASTRO_OK;
+}
+
 int astro_ctx_elastic_soln_load(astro_ctx* c) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
     ctx_minus_internal_force(c->s, c->csrVal, c->csrCols, c->csrRange, c->prep.elMatDim, c->nodeDisp.p, c->load.p);
+    ASTRO_TRY(ctx_thermal_load(c, 1.0, c->load.p));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
@@ -1760,6 +1818,7 @@ int astro_ctx_dynamic_init(astro_ctx* c, double beta, double gamma, double dt) {
     if (!(beta > 0.0) || !(dt > 0.0) || !(gamma >= 0.0))
         return fail(ASTRO_ERR_ARG, "Newmark parameters need beta > 0, gamma >= 0, dt > 0");
     if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "Newmark dynamics of shell4 meshes is not part of the device path");
+    if (c->hasTExp) return fail(ASTRO_ERR_UNSUPPORTED, "Newmark dynamics with thermal expansion is not part of the device path");
     const int n = c->prep.elMatDim;
     c->nmBeta = beta;
     c->nmGamma = gamma;
@@ -2342,6 +2401,7 @@ static int dv_pairs_launch(astro_ctx* c, bool thermal, const double* state, cons
 
 int astro_ctx_dRdD_vars(astro_ctx* c, double* dRdD) {
     ASTRO_TRY(dv_require(c));
+    if (c->hasTExp) return fail(ASTRO_ERR_UNSUPPORTED, "design derivatives of a model with thermal expansion");
     if (!dRdD) return fail(ASTRO_ERR_ARG, "null argument");
     const int64_t n = c->prep.elMatDim, cols = c->dvs.nDV;
     DBuf<double> D;
@@ -2561,9 +2621,21 @@ static int obj_term(astro_ctx* c, const astro_objective_term& t, double& value, 
     return ASTRO_OK;
 }
 
+// Objective terms whose element quantity holds the stress (stress, strain
+// energy) of a model with thermal expansion: C (strain - T alpha) is not in
+// the objective kernels.
+static int texp_terms_check(const astro_ctx* c, int32_t numTerms, const astro_objective_term* terms) {
+    if (!c->hasTExp) return ASTRO_OK;
+    for (int t = 0; t < numTerms; ++t)
+        if (terms[t].category == ASTRO_OBJ_STRESS || terms[t].category == ASTRO_OBJ_STRAIN_ENERGY)
+            return fail(ASTRO_ERR_UNSUPPORTED, "stress and strain-energy terms of a model with thermal expansion");
+    return ASTRO_OK;
+}
+
 int astro_ctx_objective(astro_ctx* c, int32_t numTerms, const astro_objective_term* terms, double* objVal) {
     if (!c || !terms || !objVal || numTerms < 1) return fail(ASTRO_ERR_ARG, "null or empty argument");
     if (c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "objective terms cover brick8 models");
+    ASTRO_TRY(texp_terms_check(c, numTerms, terms));
     bool uu = false, ut = false;
     for (int t = 0; t < numTerms; ++t) ASTRO_TRY(obj_term(c, terms[t], objVal[t], false, nullptr, uu, ut));
     return ASTRO_OK;
@@ -2572,6 +2644,8 @@ int astro_ctx_objective(astro_ctx* c, int32_t numTerms, const astro_objective_te
 int astro_ctx_objective_gradient(astro_ctx* c, int32_t numTerms, const astro_objective_term* terms, double* dLdD) {
     ASTRO_TRY(dv_require(c));
     if (!terms || !dLdD || numTerms < 1) return fail(ASTRO_ERR_ARG, "null or empty argument");
+    if (c->hasTExp)
+        return fail(ASTRO_ERR_UNSUPPORTED, "design gradients of a model with thermal expansion");
     const int n = c->prep.elMatDim, N = c->prep.numNodes, nDV = c->dvs.nDV;
     hipStream_t s = c->s;
     if (c->dLdu.n != (size_t)n) ASTRO_TRY(c->dLdu.alloc(n));

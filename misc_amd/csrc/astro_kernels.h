@@ -65,8 +65,12 @@ __global__ void k_shell4_dtangent_cs(const int* perm, int count, const int* conn
                                      const double* u, const int* elDV, double invH, int64_t n, double* dRdD,
                                      double* gPart, double* kim, int* kimDof);
 __global__ void k_brick8_stress(int E, const int* conn, const double* X, const int* elMat, const double* elScale,
-                                const double* matC, const int* rank, int nS, const double* u, double s0, double s1,
-                                double s2, double* stress, double* strain);
+                                const double* matC, const int* rank, int nS, const double* u, const double* tExp,
+                                const double* T, double s0, double s1, double s2, double* stress, double* strain);
+__global__ void k_brick8_thermal_strain(int mode, const int* perm, int count, const int* conn, const double* X,
+                                        const int* elMat, const double* elScale, const double* matC,
+                                        const double* tExp, const int* rank, int nS, const double* T,
+                                        const double* lam, double scale, double* out);
 __global__ void k_brick8_residual(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                   const double* elScale, const double* matC, const double* elRho, const int* rank,
                                   int nS, const double* u, const double* pu, const double* pv, const double* pa,

@@ -784,14 +784,17 @@ __global__ __launch_bounds__(64) void k_shell4_dtangent_cs(
 // Working/AStrO_r_elementEqns.f90:924-976): at the natural point sp of every
 // element, ux = sum_n u_n Nx_n of the element's own nodes (the reference's
 // gather indexes by local node number, D4), the Green-Lagrange strain of
-// r_getNLStrain (:622-648, orient = identity) and stress = C strain (no
-// thermal strain: the model carries no expansion coefficients).  Voigt order
-// (11, 22, 33, 12, 13, 23), engineering shear.  One thread per element.
+// r_getNLStrain (:622-648, orient = identity) and stress = C (strain -
+// T alpha) with T the nodal temperatures (rank-indexed) interpolated to the
+// point (:962-973; tExp null: no expansion).  Voigt order (11, 22, 33, 12,
+// 13, 23), engineering shear.  One thread per element.
 __global__ __launch_bounds__(256) void k_brick8_stress(int E, const int* __restrict__ conn,
                                                        const double* __restrict__ X, const int* __restrict__ elMat,
                                                        const double* __restrict__ elScale,
                                                        const double* __restrict__ matC, const int* __restrict__ rank,
-                                                       int nS, const double* __restrict__ u, double s0, double s1,
+                                                       int nS, const double* __restrict__ u,
+                                                       const double* __restrict__ tExp,
+                                                       const double* __restrict__ T, double s0, double s1,
                                                        double s2, double* __restrict__ stress,
                                                        double* __restrict__ strain) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -828,13 +831,111 @@ __global__ __launch_bounds__(256) void k_brick8_stress(int E, const int* __restr
     Ev[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
     const double sc = elScale[e];
     const double* Cm = matC + 36 * (size_t)elMat[e];
+    double Em[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Em[k] = Ev[k];
+    if (tExp) {
+        double Tp = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) Tp += N[n] * T[rank[conn[8 * (size_t)e + n] - 1] - 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Em[k] -= Tp * tExp[6 * (size_t)e + k];
+    }
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) acc += Cm[r + 6 * k] * Ev[k];
+        for (int k = 0; k < 6; ++k) acc += Cm[r + 6 * k] * Em[k];
         stress[6 * (size_t)e + r] = sc * acc;
         strain[6 * (size_t)e + r] = Ev[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Thermal expansion of brick8 elements at u = 0 (the solid branch of
+// r_getElRuk with temperatures, Working/AStrO_r_elementEqns.f90:2433-2447,
+// stress = C (strain - T_g alpha), T_g = sum_n N_n(g) T_n): the residual is
+// R = -f with f_(a,c) = sum_g detJ T_g (Nx_a . S)_c, S = C alpha in Voigt
+// form, so that
+//  - mode 0 adds scale * f to load at the element's DOFs (the thermal load
+//    of the static right-hand side, F - R(0, T));
+//  - mode 1 forms out[rank(b) - 1] += (dR/dT)^T lam at node b, i.e.
+//    -sum_g detJ N_b(g) sum_(a,c) lam_(a,c) (Nx_a . S)_c (the intended dRdT
+//    of r_getElRuk, whose loop writes a stale row, D10).
+// One thread per element; elements of one launch share no node.
+__global__ __launch_bounds__(128) void k_brick8_thermal_strain(int mode, const int* __restrict__ perm, int count,
+                                                              const int* __restrict__ conn,
+                                                              const double* __restrict__ X,
+                                                              const int* __restrict__ elMat,
+                                                              const double* __restrict__ elScale,
+                                                              const double* __restrict__ matC,
+                                                              const double* __restrict__ tExp,
+                                                              const int* __restrict__ rank, int nS,
+                                                              const double* __restrict__ T,
+                                                              const double* __restrict__ lam, double scale,
+                                                              double* __restrict__ out) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    double xe[8][3], tn[8], lm[3][8];
+    int rk[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const int nd = conn[8 * (size_t)e + n] - 1;
+        rk[n] = rank[nd];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xe[n][c] = X[3 * (size_t)nd + c];
+            lm[c][n] = mode == 1 ? lam[rk[n] + c * nS - 1] : 0.0;
+        }
+        tn[n] = mode == 0 ? T[rk[n] - 1] : 0.0;
+    }
+    const double sc = elScale[e];
+    const double* Cm = matC + 36 * (size_t)elMat[e];
+    double S[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc += Cm[r + 6 * k] * tExp[6 * (size_t)e + k];
+        S[r] = sc * acc;
+    }
+    double f[3][8], g[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        g[n] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[c][n] = 0.0;
+    }
+    for (int gp = 0; gp < 8; ++gp) {
+        double N[8], Nx[8][3], w;
+        brick8_gauss_point(gp, xe, N, Nx, w);
+        double Tg = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) Tg += N[n] * tn[n];
+        double s = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            // (Nx_n . S)_c: the B0 column of DOF (n, c) against S
+            const double b0 = Nx[n][0] * S[0] + Nx[n][1] * S[3] + Nx[n][2] * S[4];
+            const double b1 = Nx[n][1] * S[1] + Nx[n][0] * S[3] + Nx[n][2] * S[5];
+            const double b2 = Nx[n][2] * S[2] + Nx[n][0] * S[4] + Nx[n][1] * S[5];
+            f[0][n] += w * Tg * b0;
+            f[1][n] += w * Tg * b1;
+            f[2][n] += w * Tg * b2;
+            s += lm[0][n] * b0 + lm[1][n] * b1 + lm[2][n] * b2;
+        }
+#pragma unroll
+        for (int n = 0; n < 8; ++n) g[n] -= w * N[n] * s;
+    }
+    if (mode == 0) {
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[rk[n] + c * nS - 1] += scale * f[c][n];
+    } else {
+#pragma unroll
+        for (int n = 0; n < 8; ++n) out[rk[n] - 1] += g[n];
     }
 }
 

@@ -55,6 +55,7 @@ module AStrO_gpuElastic
         integer(c_int32_t) :: numLayers
         type(c_ptr) :: layers, zOffset, orient
         type(c_ptr) :: specHeat
+        type(c_ptr) :: thermExp
     end type
 
     type, bind(C) :: astro_design_t
@@ -191,7 +192,7 @@ contains
         type(astro_model_t) :: m
         type(astro_options_t) :: o
         real(c_double), allocatable, target :: xyz(:), ep(:), den(:), cnd(:), lay(:), zo(:), ori(:), coef(:), rhs(:)
-        real(c_double), allocatable, target :: sh(:)
+        real(c_double), allocatable, target :: sh(:), tex(:)
         integer(c_int32_t), allocatable, target :: conn(:), et(:), eqn(:), dof(:), isSet(:), nstart(:), nodes(:)
         integer(c_int32_t), allocatable :: tn(:)
         integer :: e, sec, mat, i, j, l, d, nL, nLay, cnt, iset, t, nTerms, k
@@ -203,7 +204,8 @@ contains
             call gpuElasticRelease()
         endif
         allocate(tn(max(1, numNodes)))
-        allocate(xyz(3*numNodes), conn(8*numEls), et(numEls), ep(9*numEls), den(numEls), cnd(6*numEls), sh(numEls))
+        allocate(xyz(3*numNodes), conn(8*numEls), et(numEls), ep(9*numEls), den(numEls), cnd(6*numEls), sh(numEls), &
+                 tex(6*numEls))
         xyz = reshape(nodeList(1:3, 1:numNodes), [3*numNodes])
         ! node coordinates with their design variables (r_getNodeCoord)
         if (numDVar .gt. 0) then
@@ -242,6 +244,7 @@ contains
             den(e) = materialDensity(mat)
             cnd(6*(e-1)+1:6*e) = materialThermCond(1:6, mat)
             sh(e) = materialSpecHeat(mat)
+            tex(6*e-5:6*e) = materialThermExp(1:6, mat)
             if (numDVar .gt. 0) then
                 do i = elToDRange(e-1) + 1, elToDRange(e)
                     d = elToD(i)
@@ -345,6 +348,7 @@ contains
         m%loadNode = c_null_ptr
         m%loadVal = c_null_ptr
         m%specHeat = c_loc(sh)
+        m%thermExp = c_loc(tex)
         m%numLayers = nLay
         if (nLay .gt. 0) then
             m%layers = c_loc(lay)
@@ -394,6 +398,11 @@ contains
         allocate(ld(elMatDim))
         call check(astro_ctx_set_f64(ctx, 'nodeDisp'//c_null_char, nodeDisp, int(elMatDim, c_int64_t)), &
                    'writing nodeDisp')
+        ! the element stress holds C (strain - T alpha): the host's nodal
+        ! temperatures (rank-indexed, as nodeTemp on the device)
+        if (any(materialThermExp .ne. 0d0)) &
+            call check(astro_ctx_set_f64(ctx, 'nodeTemp'//c_null_char, nodeTemp, int(numNodes, c_int64_t)), &
+                       'writing nodeTemp')
         call check(astro_ctx_elastic_soln_load(ctx), 'astro_ctx_elastic_soln_load')
         call check(astro_ctx_get_f64(ctx, 'elasticLoad'//c_null_char, ld, int(elMatDim, c_int64_t)), &
                    'reading elasticLoad')

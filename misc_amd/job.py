@@ -306,11 +306,14 @@ def read_model(path: str):
         th = mdef.get("thermal") or {}
         k = th.get("conductivity", [0.0] * 6)
         k = [float(v) for v in (k if isinstance(k, list) else [k, k, k, 0.0, 0.0, 0.0])] + [0.0] * 6
+        # expansion: six components, or one value for 11, 22, 33 (Working/AStrO_input.f90:751-760)
+        x = th.get("expansion", [0.0] * 6)
+        x = [float(v) for v in (x if isinstance(x, list) else [x, x, x, 0.0, 0.0, 0.0])] + [0.0] * 6
         fl = mdef.get("failure") or {}
         ms = fl.get("maxStress") or {}
         mn = fl.get("maxStrain") or {}
         mats.append(dict(name=str(mdef.get("name")), eprops=np.array(Ev + nu + G), density=float(mdef.get("density", 0.0)),
-                         cond=np.array(k[:6]), spec_heat=float(th.get("specHeat", 0.0)),
+                         cond=np.array(k[:6]), spec_heat=float(th.get("specHeat", 0.0)), therm_exp=np.array(x[:6]),
                          max_stress=np.array([float(v) for v in (ms.get("tensile", [0] * 3) + ms.get("compressive", [0] * 3)
                                                                  + ms.get("shear", [0] * 3))]),
                          max_strain=np.array([float(v) for v in (mn.get("tensile", [0] * 3) + mn.get("compressive", [0] * 3)
@@ -358,6 +361,9 @@ def read_model(path: str):
               density=np.array([mats[q]["density"] for q in el_mat]),
               cond=np.stack([mats[q]["cond"] for q in el_mat]),
               spec_heat=np.array([mats[q]["spec_heat"] for q in el_mat]))
+    tex = np.stack([mats[q]["therm_exp"] for q in el_mat])
+    if np.any(tex != 0.0):
+        m.therm_exp = tex
     if nL:
         m.layers, m.z_offset, m.orient = layers, z_off, orient
     m.node_sets = dict(info.node_sets)
@@ -715,16 +721,19 @@ class Job:
         if not dynamic:
             self.num_steps = 0
             m = self._model_at(self.load_time)
-            if elastic:
-                self.ctx = astro.Context(m, self._options())
-                self._initial(self.ctx)
-                self.ctx.static_solve()
-                self._grab_elastic()
+            # stepSolve's order (Working/AStrO_commandFunctions.f90:689-747): the
+            # temperatures first, then the displacements, whose element stress
+            # C (strain - T alpha) reads them (one context holds both)
             if thermal:
-                self.tctx = self.ctx if self.ctx is not None else astro.Context(m, self._options())
+                self.tctx = astro.Context(m, self._options())
                 self.tctx.assemble_thermal()
                 self.tctx.thermal_solve()
                 self._grab_thermal()
+            if elastic:
+                self.ctx = self.tctx if self.tctx is not None else astro.Context(m, self._options())
+                self._initial(self.ctx)
+                self.ctx.static_solve()
+                self._grab_elastic()
             return
         if self.delT <= 0.0 or period <= 0.0:
             raise JobError("a dynamic *solve needs timeStep and simPeriod")

@@ -61,6 +61,7 @@ class Model:
     z_offset: Optional[np.ndarray] = None  # (E,)
     orient: Optional[np.ndarray] = None   # (E, 6)
     spec_heat: Optional[np.ndarray] = None  # (E,) specific heat (transient thermal)
+    therm_exp: Optional[np.ndarray] = None  # (E, 6) thermal expansion, Voigt 11 22 33 12 13 23
 
     @property
     def num_nodes(self) -> int:
@@ -171,6 +172,28 @@ def thermal_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, k=50.0, t_top=100.0, t_bot
     m.mpc_rhs = [t_top, t_bottom]
     m.node_sets["top"] = top
     m.node_sets["bottom"] = bottom
+    return m
+
+
+def thermoelastic_block(nx, ny, nz, lx=1.0, ly=1.0, lz=0.5, E=200e9, nu=0.3, k=50.0, alpha=1.2e-5,
+                        t_top=100.0, t_bottom=0.0, pressure=0.0, e_field=None, name=None):
+    """The cantilever block (clamped on z = 0, top pressure as nodal loads)
+    with conduction and thermal expansion: temperature fixed on the top and
+    bottom faces (dof-7 equations after the three clamp equations), k on
+    the diagonal, and isotropic expansion alpha (alpha may also be an (E, 6)
+    array of Voigt components)."""
+    m = cantilever_block(nx, ny, nz, lx, ly, lz, E, nu, pressure, e_field, name or f"thermoel{nx}x{ny}x{nz}")
+    Ne = m.num_els
+    m.cond = np.tile(np.array([k, k, k, 0.0, 0.0, 0.0]), (Ne, 1))
+    a = np.asarray(alpha, np.float64)
+    m.therm_exp = np.tile([a, a, a, 0.0, 0.0, 0.0], (Ne, 1)) if a.ndim == 0 else a.reshape(Ne, 6).copy()
+    NX, NY = nx + 1, ny + 1
+    bottom = np.arange(1, NX * NY + 1, dtype=np.int32)
+    top = bottom + NX * NY * nz
+    m.mpc = m.mpc + [MPCTerm(eqn=4, nodes=top, dof=7, coef=1.0, is_set=True, set_name="top"),
+                     MPCTerm(eqn=5, nodes=bottom, dof=7, coef=1.0, is_set=True, set_name="bottom")]
+    m.mpc_rhs = m.mpc_rhs + [t_top, t_bottom]
+    m.node_sets["top"] = top
     return m
 
 

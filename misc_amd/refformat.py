@@ -48,7 +48,9 @@ def write_model(model: Model, path: str) -> None:
             ids = ", ".join(str(int(v)) for v in model.elements[e, :nn])
             lines.append(f"          - [{e + 1}, {ids}]")
     # one material / section / element set per distinct property row
-    key = np.concatenate([model.eprops, model.density[:, None], model.cond], axis=1)
+    tex = getattr(model, "therm_exp", None)
+    tex = np.zeros((model.num_els, 6)) if tex is None else np.asarray(tex, np.float64).reshape(-1, 6)
+    key = np.concatenate([model.eprops, model.density[:, None], model.cond, tex], axis=1)
     uniq, inv = np.unique(key, axis=0, return_inverse=True)
     inv = inv.ravel()
     lines.append("sets:")
@@ -82,9 +84,12 @@ def write_model(model: Model, path: str) -> None:
         lines.append(f"          nu: [{_r(ep[3])}, {_r(ep[4])}, {_r(ep[5])}]")
         lines.append(f"          G: [{_r(ep[6])}, {_r(ep[7])}, {_r(ep[8])}]")
         k = uniq[q, 10:16]
-        if np.any(k != 0):
+        x = uniq[q, 16:22]
+        if np.any(k != 0) or np.any(x != 0):
             lines.append("      thermal:")
             lines.append("          conductivity: [" + ", ".join(_r(v) for v in k) + "]")
+            if np.any(x != 0):
+                lines.append("          expansion: [" + ", ".join(_r(v) for v in x) + "]")
     if model.mpc:
         lines.append("constraints:")
         lines.append("    matrix:")

@@ -110,6 +110,18 @@ def element_force(X, C, ue):
     return f
 
 
+def element_thermal_load(X, C, alpha, te):
+    """f_e of the thermal expansion at u = 0, the solid branch of r_getElRuk
+    with temperatures (Working/AStrO_r_elementEqns.f90:2433-2447): the stress
+    there is -C alpha T_g with T_g = N te at each Gauss point, so R_e = -f_e
+    with f_e = sum_g detJ B0^T C alpha T_g; component-major (24,)."""
+    f = np.zeros(24, dtype=np.result_type(X, C, alpha, te))
+    for s in GAUSS:
+        N, Nx, det = point(X, s)
+        f = f + det * (N @ te) * (lin_B(Nx).T @ (C @ alpha))
+    return f
+
+
 def element_flux(X, k6, te):
     """K_t,e T_e of the conduction residual (r_getElRtk)."""
     K = np.array([[k6[0], k6[3], k6[4]], [k6[3], k6[1], k6[5]], [k6[4], k6[5], k6[2]]])
@@ -193,6 +205,42 @@ class DesignOracle:
         self.A = A
         self.u = spl.spsolve(A, b)
         return self.u
+
+    def thermal_load(self, T):
+        """The assembled thermal-expansion load for nodal temperatures T
+        (by rank): F - R(0, T) - F."""
+        m = self.model
+        f = np.zeros(self.n)
+        for e in range(m.num_els):
+            X, dofs, _ = self._el(e, np.zeros(self.n))
+            te = T[self.rank[m.elements[e] - 1] - 1]
+            f[dofs] += element_thermal_load(X, stiffness(m.eprops[e]), m.therm_exp[e], te)
+        return f
+
+    def dRdT_product(self, lam):
+        """(dR/dT)^T lam by rank: R = K u - f_th(T) is linear in T, so the
+        product is -(df_th/dT)^T lam, formed column by column of the unit
+        temperatures on each element."""
+        m = self.model
+        out = np.zeros(m.num_nodes)
+        for e in range(m.num_els):
+            X, dofs, _ = self._el(e, np.zeros(self.n))
+            C = stiffness(m.eprops[e])
+            rk = self.rank[m.elements[e] - 1] - 1
+            for b in range(8):
+                te = np.zeros(8)
+                te[b] = 1.0
+                out[rk[b]] -= lam[dofs] @ element_thermal_load(X, C, m.therm_exp[e], te)
+        return out
+
+    def solve_thermoelastic(self):
+        """Temperatures, then displacements with the thermal load (stepSolve's
+        order, Working/AStrO_commandFunctions.f90:689-747)."""
+        T = self.solve_thermal()
+        A, b = self._system("elastic")
+        self.A = A
+        self.u = spl.spsolve(A, b + self.thermal_load(T))
+        return T, self.u
 
     def solve_thermal(self):
         A, b = self._system("thermal")

@@ -411,7 +411,7 @@ def ref_available() -> bool:
 def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
                   driver: Optional[str] = None, gmres: Optional[tuple] = None,
                   env: Optional[Dict[str, str]] = None, design=None, objective: Optional[str] = None,
-                  write_results: bool = False) -> Dict[str, np.ndarray]:
+                  write_results: bool = False, temperatures=None) -> Dict[str, np.ndarray]:
     """Write `model` in the reference's format, run the literal reference driver
     on it and return every dumped array (1-based values as the reference holds
     them).  gmres = (numVecs, maxIt, solverMaxBW) overrides the restart length,
@@ -421,7 +421,9 @@ def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
     ASTRO_DV_FILE; objective: the text of an objective file (ASTRO_OBJ_FILE);
     write_results: run the reference's writers after the solve (their files
     come back as text under the keys 'nodes.yaml', 'elements.yaml',
-    'objective.yaml', 'dvars.yaml')."""
+    'objective.yaml', 'dvars.yaml'); temperatures: nodal temperatures by rank
+    for mode 'thermoelastic' (ASTRO_TEMP_FILE: the elastic step with these
+    temperatures in its residual)."""
     from misc_amd.refformat import write_model
 
     exe = driver or REF_DRIVER
@@ -454,6 +456,10 @@ def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
         env["ASTRO_OBJ_FILE"] = of
     if write_results:
         env["ASTRO_WRITE_PREFIX"] = os.path.join(wd, "w_")
+    if temperatures is not None:
+        tf = os.path.join(wd, "temps.f64")
+        np.ascontiguousarray(temperatures, "<f8").tofile(tf)
+        env["ASTRO_TEMP_FILE"] = tf
     extra = [str(int(v)) for v in gmres] if gmres else []
     subprocess.run([exe, mf, os.path.join(wd, "o_"), mode] + extra, cwd=wd, check=True, env=env)
     out = {}

@@ -48,7 +48,8 @@ program astro_ref_driver
     character(len=128) :: modelFile
     character(len=256) :: prefix
     character(len=16) :: mode, argv
-    character(len=256) :: dvFile
+    character(len=256) :: dvFile, tFile
+    integer :: tStat
     character(len=128) :: objFile, wrPrefix, wrName
     character(len=16) :: wf(9)
     integer :: dvStat, kdv, objStat, wrStat, q
@@ -117,7 +118,7 @@ program astro_ref_driver
     ! kind this job has no constraints of an empty array up front.
     if (trim(mode) .eq. 'thermal') then
         allocate(elMPCMatRange(0:0))
-    else
+    elseif (trim(mode) .ne. 'thermoelastic') then
         allocate(thermMPCMatRange(0:0))
     endif
 
@@ -164,6 +165,34 @@ program astro_ref_driver
     endif
 
     call loadInitialState()
+
+    if (trim(mode) .eq. 'thermoelastic') then
+        ! solve()'s elastic sequence (Working/AStrO_commandFunctions.f90:751-780):
+        ! the tangent is assembled and factored at the zero nodeTemp the
+        ! allocation left; then the nodal temperatures a thermal step would
+        ! have produced are read from ASTRO_TEMP_FILE (numNodes doubles by
+        ! rank; the reference's own thermal step is not used: D8 drops a
+        ! coupling of every row) and stepSolve's elastic step runs, its
+        ! residual holding the thermal strain
+        call getElasticSolnLoad(1)
+        call scaleElasticMPC()
+        call convertToLTri(elMatLT,elMatLTRange,elMatLTSize,elasticMat,elMatCols,elMatRange, &
+             elMatSize,elMatDim,elMPCMat,elMPCMatCols,elMPCMatRange,elMPCSize,elMPCDim)
+        call getSparseLDLFact(elMatLT,elMatLTSize,elMatLTRange,elMatDim)
+        call get_environment_variable('ASTRO_TEMP_FILE', tFile, status=tStat)
+        if (tStat .ne. 0) then
+            write(*,'(A)') 'ref_driver: thermoelastic needs ASTRO_TEMP_FILE'
+            stop 1
+        endif
+        open(newunit=tu, file=trim(tFile), access='stream', form='unformatted', action='read')
+        read(tu) nodeTemp(1:numNodes)
+        close(tu)
+        call elasticStep()
+        call dumpd(elasticLoad, elMatDim, 'elasticLoad')
+        call dumpd(nodeDisp, elMatDim, 'nodeDisp')
+        close(lfUnit)
+        stop
+    endif
 
     if (solveElastic .eq. 1) then
 #ifdef ASTRO_HIP
