@@ -775,6 +775,21 @@ class Job:
             else:
                 self.tctx.thermal_dynamic_run(n)
                 self._grab_thermal()
+        if save:
+            self._write_history(n)
+
+    def _write_history(self, n):
+        """writeBinarySolution after every step and for the start (solve,
+        Working/AStrO_commandFunctions.f90:786-812): the previous-state
+        vectors of step k are the solution of step k (zero at the start)."""
+        c = self.ctx or self.tctx
+        N, dim = self.model.num_nodes, c.scalar("elMatDim")
+        z, zn = np.zeros(dim), np.zeros(N)
+        for k in range(n + 1):
+            h = self.history[k - 1] if k else {}
+            write_binary_solution(self.path(history_file_name(k)),
+                                  [h.get("temperature", zn), h.get("tdot", zn), h.get("displacement", z),
+                                   h.get("velocity", z), h.get("acceleration", z)])
 
     def _initial(self, ctx):
         disp = self.info.initial.get("displacement")
@@ -1046,6 +1061,42 @@ def write_objective(fn, terms, values, include, grad=None):
             lines.append(ld_line(("    - [" + str(k).ljust(32) + ", " + g18(g).ljust(32) + "]").ljust(128)))
     with open(fn, "w") as fh:
         fh.write("\n".join(lines) + "\n")
+
+
+def history_file_name(step: int) -> str:
+    """The name writeBinarySolution opens (Working/AStrO_output.f90:766-769):
+    'SolutionHistory' // stepString // '.out' with stepString a 16-character
+    variable holding the step left-justified, into a 32-character name, so the
+    step's trailing blanks stay inside and '.out' is cut to '.' (readBinarySolution
+    builds the same name, D33)."""
+    return ("SolutionHistory" + f"{int(step):<16d}" + ".out")[:32].rstrip()
+
+
+def write_binary_solution(path: str, arrays) -> None:
+    """One record per array in Fortran unformatted sequential form (4-byte
+    little-endian length before and after the doubles), as writeBinarySolution
+    writes prevTemp, prevTdot, prevDisp, [prevIntDisp,] prevVel, prevAcc."""
+    with open(path, "wb") as f:
+        for a in arrays:
+            b = np.ascontiguousarray(a, "<f8").tobytes()
+            n = np.array([len(b)], "<i4").tobytes()
+            f.write(n + b + n)
+
+
+def read_binary_solution(path: str):
+    """The records of a binary solution history file (readBinarySolution,
+    Working/AStrO_input.f90:1740-1768) as float64 arrays."""
+    out = []
+    with open(path, "rb") as f:
+        data = f.read()
+    i = 0
+    while i < len(data):
+        n = int(np.frombuffer(data[i:i + 4], "<i4")[0])
+        out.append(np.frombuffer(data[i + 4:i + 4 + n], "<f8").copy())
+        if int(np.frombuffer(data[i + 4 + n:i + 8 + n], "<i4")[0]) != n:
+            raise JobError(f"{path}: record markers disagree")
+        i += 8 + n
+    return out
 
 
 def write_dvar_values(fn, design, values, include):

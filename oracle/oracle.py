@@ -468,6 +468,10 @@ def run_reference(model, mode: str = "elastic", workdir: Optional[str] = None,
             with open(os.path.join(wd, fn)) as f:
                 out[fn[2:]] = f.read()
             continue
+        if fn.startswith("SolutionHistory"):
+            with open(os.path.join(wd, fn), "rb") as f:
+                out[fn] = f.read()
+            continue
         if not fn.startswith("o_"):
             continue
         key, ext = os.path.splitext(fn[2:])

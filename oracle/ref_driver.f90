@@ -439,6 +439,18 @@ contains
             wrName = trim(wrPrefix)//'dvars.yaml'
             call writeDVarValues(wrName, wf, 5)
         endif
+        ! the binary solution history record of step 3 (writeBinarySolution,
+        ! Working/AStrO_output.f90:760-782) for a state with distinct values
+        if (solveElastic .eq. 1) then
+            do q = 1, numNodes
+                prevTemp(q) = 0.5d0*q
+                prevTdot(q) = -0.25d0*q
+            enddo
+            prevDisp(:) = nodeDisp(:)
+            prevVel(:) = 2d0*nodeDisp(:)
+            prevAcc(:) = -3d0*nodeDisp(:)
+            call writeBinarySolution(3)
+        endif
     end subroutine writeAll
 
     subroutine dumpi(a, n, name)

@@ -1,6 +1,7 @@
 """Writes tests/golden/ref_writers_{elastic,thermal}.npz: the reference's own
 writers (writeNodeResults, writeElementResults, writeObjective after
-getObjective, writeDVarValues; Working/AStrO_output.f90) run by the
+getObjective, writeDVarValues, and writeBinarySolution for a state of
+distinct values; Working/AStrO_output.f90) run by the
 instrumented driver (oracle/ref_driver.f90, ASTRO_WRITE_PREFIX) after a solve,
 with the input files and the arrays they printed.
 
@@ -90,7 +91,9 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ref_writers_elastic.npz"), model=model_txt, design=dv_txt, objective=OBJECTIVE,
              nodes=r["nodes.yaml"], elements=r["elements.yaml"], objective_out=r["objective.yaml"],
              dvars=r["dvars.yaml"], nodeDisp=r["nodeDisp"], elasticLoad=r["elasticLoad"],
-             nDofIndex=r["nDofIndex"], currentRank=r["currentRank"], objVal=r["objVal"])
+             nDofIndex=r["nDofIndex"], currentRank=r["currentRank"], objVal=r["objVal"],
+             history3=np.frombuffer(r[[k for k in r if k.startswith("SolutionHistory")][0]], np.uint8),
+             history3_name=[k for k in r if k.startswith("SolutionHistory")][0])
     mt = thermal_case()
     r = O.run_reference(mt, "thermal", write_results=True)
     model_txt, _ = texts(mt)

@@ -196,3 +196,22 @@ def test_volume_and_mass_terms_match_the_reference_values():
     vals = o.objective([ObjectiveTerm("volume", alls, operator="volumeIntegral"),
                         ObjectiveTerm("mass", alls, operator="volumeIntegral", coef=1e-3, expo=2.0, target=1.0)])
     np.testing.assert_allclose(vals, EL["objVal"][2:], rtol=1e-13)
+
+
+def test_binary_history_matches_the_reference():
+    """writeBinarySolution's file for a state of distinct values (the
+    reference driver fills prevTemp = i / 2, prevTdot = -i / 4 by rank and
+    prevDisp, prevVel, prevAcc = u, 2 u, -3 u): same name, same bytes, and the
+    reader returns the arrays."""
+    u = EL["nodeDisp"]
+    N = len(EL["currentRank"])
+    i = np.arange(1, N + 1, dtype=np.float64)
+    arrays = [0.5 * i, -0.25 * i, u, 2.0 * u, -3.0 * u]
+    assert J.history_file_name(3) == str(EL["history3_name"])
+    with tempfile.TemporaryDirectory() as tmp:
+        fn = os.path.join(tmp, J.history_file_name(3))
+        J.write_binary_solution(fn, arrays)
+        assert open(fn, "rb").read() == EL["history3"].tobytes()
+        back = J.read_binary_solution(fn)
+    assert len(back) == 5 and all(np.array_equal(a, b) for a, b in zip(back, arrays))
+    assert J.history_file_name(12) == "SolutionHistory12" + " " * 14 + "."

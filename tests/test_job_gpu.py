@@ -221,3 +221,46 @@ def test_thermal_job_end_to_end():
     # heat balance: the reactions at the two faces cancel
     rh = np.array([float(r[1]) for r in _rows(nodes, "reactionHeatGen")])
     assert abs(rh.sum()) <= 1e-8 * np.abs(rh).sum()
+
+
+DYN_SCRIPT = """*readModelInput:
+    fileName: model.yaml
+*solve:
+    elastic: yes
+    thermal: no
+    dynamic: yes
+    timeStep: 1.0e-5
+    simPeriod: 3.0e-5
+    saveSolnHist: yes
+"""
+
+
+def test_dynamic_job_writes_the_solution_history():
+    m = cantilever_block(3, 2, 2, pressure=1e6)
+    m.density = np.full(m.num_els, 2700.0)
+    with tempfile.TemporaryDirectory() as wd:
+        write_model(m, os.path.join(wd, "model.yaml"))
+        with open(os.path.join(wd, "job.txt"), "w") as f:
+            f.write(DYN_SCRIPT)
+        print("job: Newmark 3x2x2, 3 steps with history", flush=True)
+        job = J.Job(wd)
+        try:
+            job.run_script(os.path.join(wd, "job.txt"))
+            hist = [dict(h) for h in job.history]
+            n = job.num_steps
+        finally:
+            job.close()
+        assert n == 3 and len(hist) == 3
+        recs = [J.read_binary_solution(os.path.join(wd, J.history_file_name(k))) for k in range(n + 1)]
+    assert all(len(r) == 5 for r in recs)
+    assert all(np.abs(a).max() == 0.0 for a in recs[0])
+    for k in range(1, n + 1):
+        np.testing.assert_array_equal(recs[k][2], hist[k - 1]["displacement"])
+        np.testing.assert_array_equal(recs[k][3], hist[k - 1]["velocity"])
+        np.testing.assert_array_equal(recs[k][4], hist[k - 1]["acceleration"])
+    # the Newmark relations between consecutive records (average acceleration)
+    dt = 1.0e-5
+    for k in range(1, n + 1):
+        u0, v0, a0 = recs[k - 1][2:5]
+        u1, v1, a1 = recs[k][2:5]
+        np.testing.assert_allclose(v1, v0 + 0.5 * dt * (a0 + a1), rtol=1e-12, atol=1e-12 * np.abs(v1).max())
