@@ -390,6 +390,18 @@ int astro_ctx_objective(astro_ctx* ctx, int32_t numTerms, const astro_objective_
  * static / thermal solve).  dObj/du (getdObjdU) and the explicit dObj/dd
  * (getdObjdD, complex step) on the device.  dLdD: numDV.  Synchronous. */
 int astro_ctx_objective_gradient(astro_ctx* ctx, int32_t numTerms, const astro_objective_term* terms, double* dLdD);
+/* Gradient of a Newmark objective J = sum over steps i = 1..nSteps of the
+ * terms active at t_i = i dt (termWindow: 2 doubles per term, the activeTime
+ * window; null: every step), the dynamic branch of getTotaldLdD with
+ * augmentdLdu and solveDynamicAdjoint (Working/AStrO_commandFunctions.f90:
+ * 819-956, 1416-1464).  U, A: the forward's displacements and accelerations,
+ * (nSteps + 1) x elMatDim each, row 0 the start (host); loadScale: the nodal
+ * load factor of each step (null: 1).  Needs astro_ctx_dynamic_init (its
+ * factored tangent K + M / (beta dt^2) serves every adjoint step) and
+ * astro_ctx_set_design_vars.  objSum (may be null): J.  Synchronous. */
+int astro_ctx_dynamic_objective_gradient(astro_ctx* ctx, int32_t nSteps, const double* U, const double* A,
+                                         const double* loadScale, const double* termWindow, int32_t numTerms,
+                                         const astro_objective_term* terms, double* objSum, double* dLdD);
 
 /* Thermal twins (Working/AStrO_commandFunctions.f90:16-74, 229-258, 689-703,
  * 763-769). */

@@ -45,7 +45,7 @@ EXPORTS = [
     "astro_gnn_field_blocks", "astro_ctx_gnn_attach", "astro_ctx_gnn_detach", "astro_ctx_gnn_get",
     "astro_ctx_set_design_vars", "astro_ctx_dRdD_vars", "astro_ctx_dRtdD_vars", "astro_ctx_objective",
     "astro_ctx_objective_gradient", "astro_ctx_element_flux", "astro_ctx_shell_internal_force",
-    "astro_ctx_dRdT_product",
+    "astro_ctx_dRdT_product", "astro_ctx_dynamic_objective_gradient",
 ]
 
 # Design-variable categories and objective terms (include/astro_abi.h).
@@ -209,6 +209,8 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_element_flux.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.astro_ctx_shell_internal_force.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _dp, _dp]
     L.astro_ctx_dRdT_product.argtypes = [C.c_void_p, _dp, _dp]
+    L.astro_ctx_dynamic_objective_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, C.c_void_p, C.c_void_p,
+                                                       C.c_int32, C.c_void_p, _dp, _dp]
     L.astro_ctx_thermal_dynamic_init.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.astro_ctx_thermal_dynamic_run.argtypes = [C.c_void_p, C.c_int]
     L.astro_ctx_precond_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -635,6 +637,28 @@ class Context:
         out = np.zeros(self.n_dv)
         _check(load_library().astro_ctx_objective_gradient(self.h, len(terms), C.cast(arr, C.c_void_p), out))
         return out
+
+    def dynamic_objective_gradient(self, terms, U, A, load_scale=None, windows=None):
+        """(J, dJ/dd) of a Newmark objective, J = sum over steps 1..nSteps of
+        the terms active at i dt, from the forward's states U, A ((nSteps + 1)
+        x elMatDim, row 0 the start).  windows: (numTerms, 2) activeTime
+        windows or None."""
+        U = np.ascontiguousarray(U, np.float64)
+        A = np.ascontiguousarray(A, np.float64)
+        ns = U.shape[0] - 1
+        if A.shape != U.shape or ns < 1:
+            raise ValueError("U and A must hold nSteps + 1 rows of elMatDim values")
+        ls = None if load_scale is None else np.ascontiguousarray(load_scale, np.float64)
+        if ls is not None and ls.size != ns:
+            raise ValueError("load_scale must hold nSteps values")
+        w = None if windows is None else np.ascontiguousarray(windows, np.float64).reshape(-1)
+        arr, keep = _marshal_terms(terms)
+        out = np.zeros(self.n_dv)
+        J = np.zeros(1)
+        _check(load_library().astro_ctx_dynamic_objective_gradient(
+            self.h, ns, U.reshape(-1), A.reshape(-1), None if ls is None else ls.ctypes.data,
+            None if w is None else w.ctypes.data, len(terms), C.cast(arr, C.c_void_p), J, out))
+        return float(J[0]), out
 
     def thermal_dynamic_init(self, gamma: float = 0.5, dt: float = 1.0):
         """Transient conduction set-up: tangent K + C/(gamma dt), penalty

@@ -2375,7 +2375,7 @@ static int dv_require(astro_ctx* c) {
 // the dense n x nDV derivative (one launch per colour); with lam, per-pair
 // lambda . f into out (one launch).
 static int dv_pairs_launch(astro_ctx* c, bool thermal, const double* state, const double* lam, double* out,
-                           double* col, int64_t n) {
+                           double* col, int64_t n, const double* acc = nullptr) {
     auto& V = c->dvs;
     auto launch = [&](int p0, int cnt) {
         if (cnt <= 0) return;
@@ -2386,8 +2386,8 @@ static int dv_pairs_launch(astro_ctx* c, bool thermal, const double* state, cons
         else
             hipLaunchKernelGGL(k_dv_pair_force, dim3(grid_for(cnt, 128, INT_MAX)), dim3(128), 0, c->s, cnt,
                                V.dPairEl.p + p0, V.dPairDV.p + p0, V.dDir.p + (size_t)p0 * DVD_STRIDE, c->conn.p,
-                               c->X.p, V.dEprops.p, c->rank.p, c->prep.numSolidNds, state, lam, V.h,
-                               out ? out + p0 : nullptr, col, n);
+                               c->X.p, V.dEprops.p, c->rank.p, c->prep.numSolidNds, state, acc,
+                               (const double*)c->elRho.p, lam, V.h, out ? out + p0 : nullptr, col, n);
     };
     if (col) {
         for (size_t q = 0; q + 1 < V.colorPairStart.size(); ++q)
@@ -2707,6 +2707,111 @@ int astro_ctx_objective_gradient(astro_ctx* c, int32_t numTerms, const astro_obj
         for (int p = 0; p < nP; ++p) g[V.pairDV[p]] -= hl[p];
     }
     for (int k = 0; k < nDV; ++k) dLdD[k] = g[k];
+    return ASTRO_OK;
+}
+
+
+// Gradient of a Newmark objective, the dynamic branch of getTotaldLdD
+// (Working/AStrO_commandFunctions.f90:1416-1464) with augmentdLdu (:819-915)
+// and solveDynamicAdjoint (:917-956): J = sum over steps i = 1..nSteps of
+// the terms active at t_i = i dt, on the states U_i, A_i the forward left
+// (host, (nSteps + 1) x elMatDim each, row 0 the start).  Backward from the
+// last step: dL/du_i = dObj/du(U_i) plus, from step i + 1, c1 c2 M lam +
+// accAdj; dL/dv_i and dL/da_i likewise (M lam by the residual kernel with a
+// zero displacement and lam in the previous-state slot, as augmentdLdu does);
+// velAdj = dL/dv, accAdj = -(dL/da + dt gamma velAdj) / (dt^2 beta),
+// (K + M / (beta dt^2) + C^T C) lam_i = dL/du - accAdj on the factored
+// Newmark tangent; dL/dd += dObj/dd(U_i) - lam_i . dR_i/dd with the inertia
+// at A_i and the nodal loads scaled by loadScale[i - 1].  termWindow: 2 per
+// term, the activeTime window (null: always active).
+int astro_ctx_dynamic_objective_gradient(astro_ctx* c, int32_t nSteps, const double* U, const double* A,
+                                         const double* loadScale, const double* termWindow, int32_t numTerms,
+                                         const astro_objective_term* terms, double* objSum, double* dLdD) {
+    ASTRO_TRY(dv_require(c));
+    if (!U || !A || !terms || !dLdD || numTerms < 1 || nSteps < 1) return fail(ASTRO_ERR_ARG, "null or empty argument");
+    if (!c->dynReady) return fail(ASTRO_ERR_ARG, "astro_ctx_dynamic_init has not been called");
+    const int n = c->prep.elMatDim, nS = c->prep.numSolidNds, nDV = c->dvs.nDV;
+    const double beta = c->nmBeta, gamma = c->nmGamma, dt = c->delT, c1 = 0.5 / beta, c2 = 2.0 / (dt * dt);
+    hipStream_t s = c->s;
+    if (c->dLdu.n != (size_t)n) ASTRO_TRY(c->dLdu.alloc(n));
+    DBuf<double> lam, lamN, accAdj, accAdjN, velAdj, velAdjN, dLdv, dLda, zero, accBuf, lamDot;
+    for (DBuf<double>* b : {&lam, &lamN, &accAdj, &accAdjN, &velAdj, &velAdjN, &dLdv, &dLda, &zero, &accBuf}) {
+        ASTRO_TRY(b->alloc(n));
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, b->p, (int64_t)n, 0.0);
+    }
+    // the current and the next step's adjoint vectors (swapped per step)
+    double *pLam = lam.p, *pLamN = lamN.p, *pAcc = accAdj.p, *pAccN = accAdjN.p, *pVel = velAdj.p,
+           *pVelN = velAdjN.p;
+    auto& V = c->dvs;
+    const int nP = (int)V.pairEl.size();
+    ASTRO_TRY(lamDot.alloc(std::max(nP, 1)));
+    std::vector<double> g(nDV, 0.0), hl(nP), hlam(n);
+    double total = 0.0;
+    // load -= R_e(u = 0, previous state (pu, pv, pa)) over the element colours
+    auto inertia = [&](const double* pu, const double* pv, const double* pa, double* load) {
+        for (size_t k = 0; k + 1 < c->colorStart.size(); ++k) {
+            const int cnt = c->colorStart[k + 1] - c->colorStart[k];
+            if (cnt <= 0) continue;
+            hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[k], cnt,
+                               c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p, nS,
+                               (const double*)zero.p, pu, pv, pa, c1, c2, dt, beta, load, (const int*)nullptr);
+        }
+    };
+    auto axpby = [&](double* y, const double* x, double a, double b) {  // y = a y + b x
+        hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, y, x, y, n, a, b,
+                           (const int*)nullptr);
+    };
+    for (int i = nSteps; i >= 1; --i) {
+        const double t = dt * i;
+        HIP_TRY(hipMemcpyAsync(c->nodeDisp.p, U + (size_t)i * n, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(accBuf.p, A + (size_t)i * n, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, c->dLdu.p, (int64_t)n, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, dLdv.p, (int64_t)n, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, dLda.p, (int64_t)n, 0.0);
+        bool usesU = false, usesT = false;
+        for (int q = 0; q < numTerms; ++q) {
+            if (termWindow && !(termWindow[2 * q] <= t && t <= termWindow[2 * q + 1])) continue;
+            double v = 0.0;
+            ASTRO_TRY(obj_term(c, terms[q], v, true, &g, usesU, usesT));
+            total += v;
+        }
+        if (usesT) return fail(ASTRO_ERR_UNSUPPORTED, "temperature terms of a Newmark objective");
+        if (i < nSteps) {
+            // augmentdLdu: the next step's residual through this step's state
+            inertia(pLamN, zero.p, zero.p, c->dLdu.p);  // + c1 c2 M lam
+            axpby(c->dLdu.p, pAccN, 1.0, 1.0);
+            inertia(zero.p, pLamN, zero.p, dLdv.p);  // + c1 c2 dt M lam
+            axpby(dLdv.p, pAccN, 1.0, dt);
+            axpby(dLdv.p, pVelN, 1.0, 1.0);
+            inertia(zero.p, zero.p, pLamN, dLda.p);  // + c1 (1 - 2 beta) M lam
+            axpby(dLda.p, pAccN, 1.0, 0.5 * dt * dt * (1.0 - 2.0 * beta));
+            axpby(dLda.p, pVelN, 1.0, dt * (1.0 - gamma));
+        }
+        // solveDynamicAdjoint
+        HIP_TRY(hipMemcpyAsync(pVel, dLdv.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(pAcc, dLda.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        axpby(pAcc, pVel, -1.0 / (dt * dt * beta), -gamma / (dt * beta));
+        axpby(c->dLdu.p, pAcc, 1.0, -1.0);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, pLam, (int64_t)n, 0.0);
+        const int rc = ctx_gmres(c, c->dLdu.p, pLam);
+        if (rc != ASTRO_OK) return rc;
+        // - lam . dR_i/dd: stiffness and inertia per pair, the scaled node loads
+        ASTRO_TRY(dv_pairs_launch(c, false, c->nodeDisp.p, pLam, lamDot.p, nullptr, n, accBuf.p));
+        HIP_TRY(hipMemcpyAsync(hl.data(), lamDot.p, sizeof(double) * nP, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(hlam.data(), pLam, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int p = 0; p < nP; ++p) g[V.pairDV[p]] -= hl[p];
+        const double sc = loadScale ? loadScale[i - 1] : 1.0;
+        for (size_t k = 0; k < V.loadDof.size(); ++k) g[V.loadDV[k]] += sc * V.loadCoef[k] * hlam[V.loadDof[k]];
+        std::swap(pLam, pLamN);
+        std::swap(pAcc, pAccN);
+        std::swap(pVel, pVelN);
+    }
+    // the context keeps the forward's last state
+    HIP_TRY(hipMemcpyAsync(c->nodeDisp.p, U + (size_t)nSteps * n, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < nDV; ++k) dLdD[k] = g[k];
+    if (objSum) *objSum = total;
     return ASTRO_OK;
 }
 

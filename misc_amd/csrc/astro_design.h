@@ -23,7 +23,8 @@ enum { OBJQ_STRESS = 3, OBJQ_STRAIN = 4, OBJQ_STRAIN_ENERGY = 5, OBJQ_MASS = 6, 
 
 __global__ void k_dv_pair_force(int nPairs, const int* pairEl, const int* pairDV, const double* dir, const int* conn,
                                 const double* X, const double* eprops, const int* rank, int nS, const double* u,
-                                const double* lam, double h, double* lamDot, double* dRdD, int64_t n);
+                                const double* acc, const double* rho, const double* lam, double h, double* lamDot,
+                                double* dRdD, int64_t n);
 __global__ void k_dv_pair_flux(int nPairs, const int* pairEl, const int* pairDV, const double* dir, const int* conn,
                                const double* X, const double* cond, const int* rank, const double* T,
                                const double* lam, double h, double* lamDot, double* dRtdD, int64_t n);

@@ -225,12 +225,18 @@ __device__ void element_quantity(int quantity, int comp, const T (&X)[8][3], con
 // (lambda may be null); with dRdD (n x nDV column-major) the pair adds f into
 // its variable's column at the element's DOFs, so a launch must not hold two
 // pairs of one variable whose elements share a node (one colour per launch).
+// With acc (a Newmark step's acceleration) the element's inertia
+// M_e(d) a_e = sum_g rho detJ N N^T a_e is part of the residual as well
+// (r_getElRum, Working/AStrO_r_elementEqns.f90:2628-2653), its density and
+// coordinates perturbed with the rest.
 __global__ __launch_bounds__(128) void k_dv_pair_force(int nPairs, const int* __restrict__ pairEl,
                                                        const int* __restrict__ pairDV, const double* __restrict__ dir,
                                                        const int* __restrict__ conn, const double* __restrict__ X,
                                                        const double* __restrict__ eprops,
                                                        const int* __restrict__ rank, int nS,
-                                                       const double* __restrict__ u, const double* __restrict__ lam,
+                                                       const double* __restrict__ u, const double* __restrict__ acc,
+                                                       const double* __restrict__ rho,
+                                                       const double* __restrict__ lam,
                                                        double h, double* __restrict__ lamDot,
                                                        double* __restrict__ dRdD, int64_t n) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -261,10 +267,29 @@ __global__ __launch_bounds__(128) void k_dv_pair_force(int nPairs, const int* __
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int a = 0; a < 8; ++a) f[c][a] = mkc(0.0, 0.0);
+    double ae[3][8];
+    if (acc) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ae[c][a] = acc[dof[c][a]];
+    }
+    const cx rh = mkc(rho ? rho[e] : 0.0, h * dv[DVD_RHO]);
     for (int gp = 0; gp < 8; ++gp) {
         double N[8];
         cx Nx[8][3], det;
         b8_point(gauss_coord(gp, 0), gauss_coord(gp, 1), gauss_coord(gp, 2), xe, N, Nx, det);
+        if (acc) {
+            const cx rd = rh * det;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double na = 0.0;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) na += N[b] * ae[c][b];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) f[c][a] = f[c][a] + (N[a] * na) * rd;
+            }
+        }
         cx ux[3][3];
         grad_u(ue, Nx, ux);
         const cx eps[6] = {ux[0][0], ux[1][1], ux[2][2], ux[0][1] + ux[1][0], ux[0][2] + ux[2][0],
@@ -285,17 +310,17 @@ __global__ __launch_bounds__(128) void k_dv_pair_force(int nPairs, const int* __
         }
     }
     const double invH = 1.0 / h;
-    double acc = 0.0;
+    double ld = 0.0;
     double* col = dRdD ? dRdD + (int64_t)pairDV[p] * n : nullptr;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
             const double g = f[c][a].i * invH;
-            if (lam) acc += lam[dof[c][a]] * g;
+            if (lam) ld += lam[dof[c][a]] * g;
             if (col) col[dof[c][a]] += g;
         }
-    if (lamDot) lamDot[p] = acc;
+    if (lamDot) lamDot[p] = ld;
 }
 
 // dRt/dd of the thermal residual for one (element, variable) pair per thread:

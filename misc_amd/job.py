@@ -750,6 +750,7 @@ class Job:
                 raise JobError("dynamic loads with more than one activeTime window are not on the device path")
             win = wins.pop() if wins else (0.0, 1e100)
             scales = np.array([1.0 if win[0] <= (k + 1) * self.delT <= win[1] else 0.0 for k in range(n)])
+            self.dyn_scales = scales
             mm = self._model_at(win[0])
             self.ctx = astro.Context(mm, self._options())
             self.ctx.dynamic_init(beta, gamma, self.delT)
@@ -826,9 +827,11 @@ class Job:
 
     # -- objective ----------------------------------------------------------------
     def _lib_terms(self, time):
+        """The library's terms of those active at time (time None: every
+        term, for the Newmark gradient, which applies the windows per step)."""
         out = []
         for t in self.terms:
-            if not (t.act_time[0] <= time <= t.act_time[1]):
+            if time is not None and not (t.act_time[0] <= time <= t.act_time[1]):
                 continue
             if t.tgt_tag.startswith("max"):
                 raise JobError(f"target {t.tgt_tag} is not on the device path")
@@ -862,12 +865,31 @@ class Job:
         if self.design is None:
             raise JobError("*calcObjGradient without design variables")
         if self.num_steps:
-            raise JobError("*calcObjGradient of a dynamic solve is not on the device path")
+            return self._dynamic_gradient()
         time = float(a.get("staticLoadTime", self.load_time) or self.load_time)
         c = self._term_ctx()
         c.set_design_vars(design_for_library(self.design, self.model.num_els, self.model.num_nodes, time))
         lib = self._lib_terms(time)
         self.dLdD = c.objective_gradient(lib) if lib else np.zeros(len(self.design.category))
+
+    def _dynamic_gradient(self):
+        """getTotaldLdD's dynamic branch (Working/AStrO_commandFunctions.f90:
+        1416-1464) on the device from the solution history the solve kept
+        (saveSolnHist; the reference reads it back from its binary files)."""
+        if self.ctx is None or not self.history or "displacement" not in self.history[0]:
+            raise JobError("*calcObjGradient of a dynamic solve needs the elastic solution history (saveSolnHist)")
+        c = self.ctx
+        # node-load variables follow the loads' first step of activity
+        c.set_design_vars(design_for_library(self.design, self.model.num_els, self.model.num_nodes, self.delT))
+        lib = self._lib_terms(None)
+        if not lib:
+            self.dLdD = np.zeros(len(self.design.category))
+            return
+        n = c.scalar("elMatDim")
+        U = np.vstack([np.zeros(n)] + [h["displacement"] for h in self.history])
+        A = np.vstack([np.zeros(n)] + [h["acceleration"] for h in self.history])
+        windows = np.array([t.act_time for t in self.terms], np.float64)
+        _, self.dLdD = c.dynamic_objective_gradient(lib, U, A, load_scale=self.dyn_scales, windows=windows)
 
     # -- writers ----------------------------------------------------------------
     def cmd_writeNodeResults(self, a):

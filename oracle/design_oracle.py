@@ -374,6 +374,18 @@ class DesignOracle:
 
     def gradient(self, terms):
         """dL/dd = dObj/dd - lambda . dR/dd - lambda_t . dRt/dd."""
+        dLdu, dLdt, g, uses_u, uses_t = self.explicit_parts(terms)
+        if uses_u:
+            lam = spl.spsolve(self.A, dLdu)
+            g -= lam @ self.dRdD()
+        if uses_t:
+            lam_t = spl.spsolve(self.At, dLdt)
+            g -= lam_t @ self.dRtdD()
+        return g
+
+    def explicit_parts(self, terms):
+        """dObj/du, dObj/dT and the explicit dObj/dd of the terms at the
+        current state (getdObjdU, getdObjdD), and whether u / T enter."""
         m, h = self.model, self.h
         nDV = len(self.dv.category)
         g = np.zeros(nDV)
@@ -447,13 +459,7 @@ class DesignOracle:
                         dS[k] += a * v[i] + q[i] * b
                         dV[k] += b
                 g += f * (dS * V - S * dV) / V ** 2
-        if uses_u:
-            lam = spl.spsolve(self.A, dLdu)
-            g -= lam @ self.dRdD()
-        if uses_t:
-            lam_t = spl.spsolve(self.At, dLdt)
-            g -= lam_t @ self.dRtdD()
-        return g
+        return dLdu, dLdt, g, uses_u, uses_t
 
 
 def _csr(vals, cols, rng, n):

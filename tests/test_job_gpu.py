@@ -264,3 +264,83 @@ def test_dynamic_job_writes_the_solution_history():
         u0, v0, a0 = recs[k - 1][2:5]
         u1, v1, a1 = recs[k][2:5]
         np.testing.assert_allclose(v1, v0 + 0.5 * dt * (a0 + a1), rtol=1e-12, atol=1e-12 * np.abs(v1).max())
+
+
+DYN_GRAD_SCRIPT = """*readModelInput:
+    fileName: model.yaml
+*readDesignVarInput:
+    fileName: design.yaml
+*readObjectiveInput:
+    fileName: objective.yaml
+*solve:
+    elastic: yes
+    thermal: no
+    dynamic: yes
+    timeStep: 2.0e-5
+    simPeriod: 1.0e-4
+    saveSolnHist: yes
+*calcObjGradient:
+"""
+
+DYN_OBJECTIVE = """objectiveTerms:
+    - category: displacement
+      operator: powerNorm
+      component: 3
+      coefficient: 1.0e6
+      exponent: 2.0
+      nodeSet: topn
+      activeTime: [0.0, 1.0]
+    - category: strainEnergy
+      operator: volumeIntegral
+      coefficient: 1.0
+      exponent: 1.0
+      elementSet: allel
+      activeTime: [5.0e-5, 1.0]
+"""
+
+
+def test_dynamic_job_gradient():
+    from oracle.newmark_adjoint import NewmarkAdjointOracle
+
+    m = cantilever_block(3, 2, 2, e_field=np.random.default_rng(3).uniform(150e9, 250e9, 12), pressure=0.0)
+    m.density = np.random.default_rng(4).uniform(2000.0, 3000.0, 12)
+    top = np.arange(4 * 3 * 2 + 1, 4 * 3 * 3 + 1, dtype=np.int32)
+    m.load_nodes = top
+    m.load_vals = np.zeros((top.size, 7))
+    m.load_vals[:, 2] = -2.0e5
+    m.node_sets["topn"] = top
+    m.element_sets = {"allel": np.arange(1, 13), "half": np.arange(1, 7)}
+    design = [dict(category="elastic", subCategory="modulus", component=1, elementSet="half", coefficients=1e9),
+              dict(category="density", elementSet="allel", coefficients=100.0),
+              dict(category="nodeCoord", component=1, nodeSet="topn", coefficients=0.01),
+              dict(category="nodeLoad", subCategory="elastic", component=3, nodeSet="topn", coefficients=50.0,
+                   activeTime=[0.0, 1.0])]
+    with tempfile.TemporaryDirectory() as wd:
+        write_model(m, os.path.join(wd, "model.yaml"))
+        write_design_vars(os.path.join(wd, "design.yaml"), design)
+        with open(os.path.join(wd, "objective.yaml"), "w") as f:
+            f.write(DYN_OBJECTIVE)
+        with open(os.path.join(wd, "job.txt"), "w") as f:
+            f.write(DYN_GRAD_SCRIPT)
+        print("job: Newmark gradient 3x2x2, 5 steps", flush=True)
+        job = J.Job(wd)
+        try:
+            job.run_script(os.path.join(wd, "job.txt"))
+            g = job.dLdD.copy()
+            hist = [dict(h) for h in job.history]
+            scales = job.dyn_scales.copy()
+        finally:
+            job.close()
+        mj, info = J.read_model(os.path.join(wd, "model.yaml"))
+        d = J.read_design_vars(os.path.join(wd, "design.yaml"), info, mj.num_els, mj.num_nodes)
+    lib = J.design_for_library(d, mj.num_els, mj.num_nodes, 2.0e-5)
+    mj = J.model_at(mj, info, d, None, 0.0)
+    o = NewmarkAdjointOracle(mj, lib, dt=2.0e-5)
+    n = o.n
+    U = np.vstack([np.zeros(n)] + [h["displacement"] for h in hist])
+    A = np.vstack([np.zeros(n)] + [h["acceleration"] for h in hist])
+    terms = [ObjectiveTerm("displacement", top, component=3, coef=1.0e6, expo=2.0),
+             ObjectiveTerm("strainEnergy", np.arange(1, 13), operator="volumeIntegral")]
+    gr = o.gradient(terms, U, A, scales, windows=[(0.0, 1.0), (5.0e-5, 1.0)])
+    np.testing.assert_allclose(g, gr, rtol=1e-9, atol=1e-12 * np.abs(gr).max())
+    assert np.abs(gr).min() > 0

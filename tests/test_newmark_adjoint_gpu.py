@@ -1,0 +1,37 @@
+"""The Newmark objective gradient on the device
+(astro_ctx_dynamic_objective_gradient) against the CPU oracle
+(oracle/newmark_adjoint.py, pinned by finite differences) on the device's own
+forward states, and the objective sum against the oracle's."""
+
+import numpy as np
+import pytest
+
+from misc_amd import astro
+from oracle.newmark_adjoint import NewmarkAdjointOracle
+from test_newmark_adjoint import DT, STEPS, case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_newmark_gradient_matches_oracle():
+    m, dv, terms, windows, scales = case()
+    ctx = astro.Context(m, astro.Options(rtol=1e-12))
+    ctx.dynamic_init(0.25, 0.5, DT)
+    n = ctx.scalar("elMatDim")
+    U, A = np.zeros((STEPS + 1, n)), np.zeros((STEPS + 1, n))
+    for k in range(STEPS):
+        ctx.dynamic_step(float(scales[k]))
+        U[k + 1], A[k + 1] = ctx.getd("nodeDisp"), ctx.getd("nodeAcc")
+    ctx.set_design_vars(dv)
+    print("newmark adjoint", n, STEPS, flush=True)
+    J, G = ctx.dynamic_objective_gradient(terms, U, A, load_scale=scales, windows=windows)
+    ctx.close()
+    o = NewmarkAdjointOracle(m, dv, dt=DT)
+    Gr = o.gradient(terms, U, A, scales, windows)
+    Jr = o.objective(terms, U, windows)
+    assert abs(J - Jr) <= 1e-12 * abs(Jr)
+    np.testing.assert_allclose(G, Gr, rtol=1e-9, atol=1e-12 * np.abs(Gr).max())
+    # the device forward is the linear one up to the Green-Lagrange terms of
+    # its residual (relative size about the strain, 2e-5 with these loads)
+    Ul = o.forward(scales)[0]
+    assert np.linalg.norm(U - Ul) <= 1e-4 * np.linalg.norm(Ul)
