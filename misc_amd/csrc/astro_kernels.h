@@ -60,6 +60,10 @@ __global__ void k_shell4_tangent(const int* perm, int count, const int* conn, co
 __global__ void k_shell4_corot(const int* perm, int count, const int* conn, const double* X, const double* elABD,
                                const double* elAlpha, const int* rank, int nSolid, int nSec, const double* u, double h,
                                int iters, double* force, double* energy);
+__global__ void k_shell4_corot_tangent(const int* perm, int count, const int* conn, const double* X,
+                                       const double* elABD, const double* elAlpha, const int* rank, int nSolid,
+                                       int nSec, const double* u, double h, int iters, double eps,
+                                       const int* csrRange, const uint8_t* elPos, double* val);
 __global__ void k_shell4_dtangent_cs(const int* perm, int count, const int* conn, const double* X, const double* elABD,
                                      const double* elABDim, const double* elAlpha, const int* rank, int nSolid, int nSec,
                                      const double* u, const int* elDV, double invH, int64_t n, double* dRdD,

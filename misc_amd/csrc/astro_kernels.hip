@@ -282,6 +282,66 @@ __device__ __forceinline__ void shell4_stage_B(int e, int lane, const int* __res
     __syncthreads();
 }
 
+// LU (no pivoting) in place of the internal block of the element matrix,
+// rows and columns 24..31 (r_condenseMat, Working/AStrO_r_elementEqns.f90:
+// 2722-2743).  One lane.
+__device__ __forceinline__ void shell4_lu_internal(double (&sK)[32][33]) {
+    for (int i = 24; i < 32; ++i) {
+        for (int j = i; j < 32; ++j)
+            for (int k = 24; k < i; ++k) sK[i][j] -= sK[i][k] * sK[k][j];
+        const double f = 1.0 / sK[i][i];
+        for (int j = i + 1; j < 32; ++j) {
+            for (int k = 24; k < i; ++k) sK[j][i] -= sK[j][k] * sK[k][i];
+            sK[j][i] *= f;
+        }
+    }
+}
+
+// x = K_ii^-1 rhs with the factors of shell4_lu_internal
+__device__ __forceinline__ void shell4_solve_internal(const double (&sK)[32][33], const double* rhs, double* x) {
+    double z[8];
+    for (int i = 0; i < 8; ++i) {
+        double v = rhs[i];
+        for (int k = 0; k < i; ++k) v -= sK[24 + i][24 + k] * z[k];
+        z[i] = v;
+    }
+    for (int i = 7; i >= 0; --i) {
+        double v = z[i];
+        for (int k = i + 1; k < 8; ++k) v -= sK[24 + i][24 + k] * x[k];
+        x[i] = v / sK[24 + i][24 + i];
+    }
+}
+
+// The 24 x 24 condensed matrix K_ee - K_ei K_ii^-1 K_ie of the 32 x 32
+// element matrix sK (its internal block factored in place) added into the CSR
+// values at the element's slots.  sXs: 24 x 8 scratch.  Starts with a barrier.
+__device__ __forceinline__ void shell4_condense_scatter(int e, int lane, double (&sK)[32][33], double (&sXs)[24][8],
+                                                        const int (&sNode)[4], const int* __restrict__ rank,
+                                                        int nSolid, int nSec, const int* __restrict__ csrRange,
+                                                        const uint8_t* __restrict__ elPos, double* __restrict__ val) {
+    __syncthreads();
+    if (lane == 0) shell4_lu_internal(sK);
+    __syncthreads();
+    if (lane < 24) {
+        // x = K_ii^-1 K_i,c for nodal column c = lane
+        double rhs[8], x[8];
+        for (int i = 0; i < 8; ++i) rhs[i] = sK[24 + i][lane];
+        shell4_solve_internal(sK, rhs, x);
+        for (int i = 0; i < 8; ++i) sXs[lane][i] = x[i];
+    }
+    __syncthreads();
+    for (int k = lane; k < 24 * 24; k += 64) {
+        const int i = k / 24, j = k % 24;
+        double v = sK[i][j];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v -= sK[i][24 + m] * sXs[j][m];
+        const int vi = i >> 2, a = i & 3, vj = j >> 2, b = j & 3;
+        const int row = rank[sNode[a]] + vi * nSec + 2 * nSolid;
+        const int slot = elPos[192 * (size_t)e + (a * 4 + b) * 6 + vj];
+        val[csrRange[row - 1] + slot] += v;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_shell4_tangent(
     const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
     const double* __restrict__ elABD, const double* __restrict__ elAlpha, const int* __restrict__ rank, int nSolid,
@@ -315,46 +375,7 @@ __global__ __launch_bounds__(64) void k_shell4_tangent(
             for (int m = 0; m < 9; ++m) acc += sB[g][m][i] * sDB[g][m][j];
         sK[i][j] = acc;
     }
-    __syncthreads();
-    // condensation: LU (no pivoting) of the internal block, rows/cols 24..31
-    if (lane == 0) {
-        for (int i = 24; i < 32; ++i) {
-            for (int j = i; j < 32; ++j)
-                for (int k = 24; k < i; ++k) sK[i][j] -= sK[i][k] * sK[k][j];
-            const double f = 1.0 / sK[i][i];
-            for (int j = i + 1; j < 32; ++j) {
-                for (int k = 24; k < i; ++k) sK[j][i] -= sK[j][k] * sK[k][i];
-                sK[j][i] *= f;
-            }
-        }
-    }
-    __syncthreads();
-    if (lane < 24) {
-        // x = K_ii^-1 K_i,c for nodal column c = lane
-        double z[8], x[8];
-        for (int i = 0; i < 8; ++i) {
-            double v = sK[24 + i][lane];
-            for (int k = 0; k < i; ++k) v -= sK[24 + i][24 + k] * z[k];
-            z[i] = v;
-        }
-        for (int i = 7; i >= 0; --i) {
-            double v = z[i];
-            for (int k = i + 1; k < 8; ++k) v -= sK[24 + i][24 + k] * x[k];
-            x[i] = v / sK[24 + i][24 + i];
-        }
-        for (int i = 0; i < 8; ++i) sXs[lane][i] = x[i];
-    }
-    __syncthreads();
-    for (int k = lane; k < 24 * 24; k += 64) {
-        const int i = k / 24, j = k % 24;
-        double v = sK[i][j];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) v -= sK[i][24 + m] * sXs[j][m];
-        const int vi = i >> 2, a = i & 3, vj = j >> 2, b = j & 3;
-        const int row = rank[sNode[a]] + vi * nSec + 2 * nSolid;
-        const int slot = elPos[192 * (size_t)e + (a * 4 + b) * 6 + vj];
-        val[csrRange[row - 1] + slot] += v;
-    }
+    shell4_condense_scatter(e, lane, sK, sXs, sNode, rank, nSolid, nSec, csrRange, elPos, val);
 }
 
 // ---------------------------------------------------------------------------
@@ -522,17 +543,106 @@ __device__ void shell4_corot_strains(int pert, double h, const double (&sU)[32],
     }
 }
 
-// Co-rotational internal force of shell4 elements at the displacement u
-// (the shell branch of r_getElRuk, Working/AStrO_r_elementEqns.f90:2245-2423):
-// R = dE/du of E = 1/2 sum over the Gauss points of w e^T ABD e, lane d taking
-// the derivative along DOF d by complex step (one strain evaluation per lane,
-// step h).  The 8 internal DOFs start from the linear recovery
-// -K_ii^-1 K_ie u_e and are brought to element equilibrium R_i = 0 by `iters`
-// chord steps with K_ii at rest (the reference never recovers them, D7); the
-// condensed force R_e - K_ei K_ii^-1 R_i is added into force at the element's
-// DOFs (elements of one launch share no node) and E into energy[e].  At rest
-// the derivative of this force is the condensed tangent of k_shell4_tangent;
-// a finite rigid rotation gives zero force.
+// LDS state of one element in the co-rotational shell4 kernels.
+struct Shell4CorotLds {
+    double X[4][3], Al[3][3], D[9][9];
+    double N[4][10], Nx[4][10][2], W[4];
+    double B[4][9][32], DB[4][9][32];
+    double K[32][33];
+    double U[32], R[32], En;
+    int Node[4], Dof[24];
+};
+
+// dE/dU_d at the element DOFs U by complex step along DOF d (step h), and
+// en = E = 1/2 sum over the Gauss points of w e^T ABD e.
+__device__ __forceinline__ double shell4_corot_grad(int d, double h, const double (&U)[32], const Shell4CorotLds& L,
+                                                    double& en) {
+    cx ep[4][9];
+    shell4_corot_strains(d, h, U, L.X, L.Al, L.N, L.Nx, ep);
+    double r = 0.0;
+    en = 0.0;
+    for (int g = 0; g < 4; ++g)
+        for (int m = 0; m < 9; ++m) {
+            double s = 0.0;
+            for (int k = 0; k < 9; ++k) s += L.D[m][k] * ep[g][k].r;
+            r += L.W[g] * (ep[g][m].i / h) * s;
+            en += 0.5 * L.W[g] * ep[g][m].r * s;
+        }
+    return r;
+}
+
+// Co-rotational state of shell4 element e at the displacement u (the shell
+// branch of r_getElRuk, Working/AStrO_r_elementEqns.f90:2245-2423): the rest
+// tangent K (internal block LU-factored in place), the element DOFs U (24
+// from u, the 8 internal ones from the linear recovery -K_ii^-1 K_ie u_e
+// brought to element equilibrium R_i = 0 by `iters` chord steps with K_ii at
+// rest; the reference never recovers them, D7), R = dE/dU at U (lane d of
+// 0..31 by complex step) and the strain energy En.  Ends with a barrier.
+__device__ void shell4_corot_state(int e, int lane, const int* __restrict__ conn, const double* __restrict__ X,
+                                   const double* __restrict__ elABD, const double* __restrict__ elAlpha,
+                                   const int* __restrict__ rank, int nSolid, int nSec, const double* __restrict__ u,
+                                   double h, int iters, Shell4CorotLds& L) {
+    for (int k = lane; k < 81; k += 64) L.D[k % 9][k / 9] = elABD[81 * (size_t)e + k];
+    shell4_stage_B(e, lane, conn, X, elAlpha, L.X, L.Al, L.N, L.Nx, L.W, L.B, L.Node);
+    __syncthreads();
+    for (int k = lane; k < 4 * 9 * 32; k += 64) {
+        const int g = k / 288, r = (k / 32) % 9, d = k % 32;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) acc += L.D[r][m] * L.B[g][m][d];
+        L.DB[g][r][d] = L.W[g] * acc;
+    }
+    __syncthreads();
+    for (int k = lane; k < 32 * 32; k += 64) {
+        const int i = k >> 5, j = k & 31;
+        double acc = 0.0;
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) acc += L.B[g][m][i] * L.DB[g][m][j];
+        L.K[i][j] = acc;
+    }
+    if (lane < 24) {
+        const int vi = lane >> 2, nd = lane & 3;
+        const int dof = rank[L.Node[nd]] + vi * nSec + 2 * nSolid;
+        L.Dof[lane] = dof;
+        L.U[lane] = u[dof - 1];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        shell4_lu_internal(L.K);
+        double rhs[8], x[8];
+        for (int i = 0; i < 8; ++i) {
+            double v = 0.0;
+            for (int j = 0; j < 24; ++j) v -= L.K[24 + i][j] * L.U[j];
+            rhs[i] = v;
+        }
+        shell4_solve_internal(L.K, rhs, x);
+        for (int i = 0; i < 8; ++i) L.U[24 + i] = x[i];
+    }
+    __syncthreads();
+    for (int it = 0; it <= iters; ++it) {
+        if (lane < 32) {
+            double en;
+            L.R[lane] = shell4_corot_grad(lane, h, L.U, L, en);
+            if (lane == 0) L.En = en;
+        }
+        __syncthreads();
+        if (it == iters) break;
+        if (lane == 0) {
+            double x[8];
+            shell4_solve_internal(L.K, &L.R[24], x);
+            for (int i = 0; i < 8; ++i) L.U[24 + i] -= x[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Co-rotational internal force of shell4 elements at the displacement u: the
+// condensed force R_e - K_ei K_ii^-1 R_i of shell4_corot_state's element
+// state added into force at the element's DOFs (elements of one launch share
+// no node) and E into energy[e].  At rest the derivative of this force is the
+// condensed tangent of k_shell4_tangent; a finite rigid rotation gives zero
+// force.
 __global__ __launch_bounds__(64) void k_shell4_corot(
     const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
     const double* __restrict__ elABD, const double* __restrict__ elAlpha, const int* __restrict__ rank, int nSolid,
@@ -542,106 +652,74 @@ __global__ __launch_bounds__(64) void k_shell4_corot(
     if (idx >= count) return;
     const int e = perm[idx];
     const int lane = threadIdx.x;
-    __shared__ double sX[4][3], sAl[3][3], sD[9][9];
-    __shared__ double sN[4][10], sNx[4][10][2], sW[4];
-    __shared__ double sB[4][9][32], sDB[4][9][32];
-    __shared__ double sK[32][33];
-    __shared__ double sU[32], sR[32], sEn;
-    __shared__ int sNode[4], sDof[24];
-    for (int k = lane; k < 81; k += 64) sD[k % 9][k / 9] = elABD[81 * (size_t)e + k];
-    shell4_stage_B(e, lane, conn, X, elAlpha, sX, sAl, sN, sNx, sW, sB, sNode);
+    __shared__ Shell4CorotLds L;
+    __shared__ double sY[8];
+    shell4_corot_state(e, lane, conn, X, elABD, elAlpha, rank, nSolid, nSec, u, h, iters, L);
+    if (lane == 0) shell4_solve_internal(L.K, &L.R[24], sY);
     __syncthreads();
-    for (int k = lane; k < 4 * 9 * 32; k += 64) {
-        const int g = k / 288, r = (k / 32) % 9, d = k % 32;
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < 9; ++m) acc += sD[r][m] * sB[g][m][d];
-        sDB[g][r][d] = sW[g] * acc;
+    if (lane < 24) {
+        double v = L.R[lane];
+        for (int m = 0; m < 8; ++m) v -= L.K[lane][24 + m] * sY[m];
+        force[L.Dof[lane] - 1] += v;
+    }
+    if (lane == 0 && energy) energy[e] = L.En;
+}
+
+// Co-rotational tangent of shell4 elements at the displacement u: the dRdU
+// of r_getElRuk's shell branch that getElasticSolnLoad(1) assembles at every
+// Newton step of a nonlinear solve (Working/AStrO_r_elementEqns.f90:2245-2423,
+// Working/AStrO_commandFunctions.f90:260-351).  With the internal DOFs at
+// element equilibrium (shell4_corot_state), column j of the 32 x 32 Hessian of
+// E is the central difference of the complex-step gradient,
+// (R(U + s e_j) - R(U - s e_j)) / 2 s (lanes 0..31 take +, 32..63 take -),
+// s = eps for the rotations and eps times the longest element edge for the
+// translations and internal DOFs; symmetrised and condensed over the internal
+// DOFs it is the derivative of k_shell4_corot's force (internal equilibrium
+// makes the condensation exact), added into the CSR values as
+// k_shell4_tangent's.  Elements of one launch share no node.
+__global__ __launch_bounds__(64) void k_shell4_corot_tangent(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ elABD, const double* __restrict__ elAlpha, const int* __restrict__ rank, int nSolid,
+    int nSec, const double* __restrict__ u, double h, int iters, double eps, const int* __restrict__ csrRange,
+    const uint8_t* __restrict__ elPos, double* __restrict__ val) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ Shell4CorotLds L;
+    __shared__ double sUp[2][32], sRp[2][32], sH[32][33], sXs[24][8], sLen;
+    shell4_corot_state(e, lane, conn, X, elABD, elAlpha, rank, nSolid, nSec, u, h, iters, L);
+    if (lane == 0) {
+        double m = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            const int b = (a + 1) & 3;
+            double d2 = 0.0;
+            for (int c = 0; c < 3; ++c) d2 += (L.X[b][c] - L.X[a][c]) * (L.X[b][c] - L.X[a][c]);
+            m = fmax(m, d2);
+        }
+        sLen = sqrt(m);
+    }
+    __syncthreads();
+    const int d = lane & 31, sg = lane >> 5;
+    for (int j = 0; j < 32; ++j) {
+        const double s = (j >= 12 && j < 24) ? eps : eps * sLen;
+        if (lane < 32) {
+            const double p = lane == j ? s : 0.0;
+            sUp[0][lane] = L.U[lane] + p;
+            sUp[1][lane] = L.U[lane] - p;
+        }
+        __syncthreads();
+        double en;
+        sRp[sg][d] = shell4_corot_grad(d, h, sUp[sg], L, en);
+        __syncthreads();
+        if (lane < 32) sH[lane][j] = (sRp[0][lane] - sRp[1][lane]) / (sUp[0][j] - sUp[1][j]);
     }
     __syncthreads();
     for (int k = lane; k < 32 * 32; k += 64) {
         const int i = k >> 5, j = k & 31;
-        double acc = 0.0;
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int m = 0; m < 9; ++m) acc += sB[g][m][i] * sDB[g][m][j];
-        sK[i][j] = acc;
+        L.K[i][j] = 0.5 * (sH[i][j] + sH[j][i]);
     }
-    if (lane < 24) {
-        const int vi = lane >> 2, nd = lane & 3;
-        const int dof = rank[sNode[nd]] + vi * nSec + 2 * nSolid;
-        sDof[lane] = dof;
-        sU[lane] = u[dof - 1];
-    }
-    __syncthreads();
-    // LU (no pivoting) of the internal block in place, as k_shell4_tangent;
-    // then the linear recovery of the internal DOFs
-    auto solve_ii = [&](const double* rhs, double* x) {
-        double z[8];
-        for (int i = 0; i < 8; ++i) {
-            double v = rhs[i];
-            for (int k = 0; k < i; ++k) v -= sK[24 + i][24 + k] * z[k];
-            z[i] = v;
-        }
-        for (int i = 7; i >= 0; --i) {
-            double v = z[i];
-            for (int k = i + 1; k < 8; ++k) v -= sK[24 + i][24 + k] * x[k];
-            x[i] = v / sK[24 + i][24 + i];
-        }
-    };
-    if (lane == 0) {
-        for (int i = 24; i < 32; ++i) {
-            for (int j = i; j < 32; ++j)
-                for (int k = 24; k < i; ++k) sK[i][j] -= sK[i][k] * sK[k][j];
-            const double f = 1.0 / sK[i][i];
-            for (int j = i + 1; j < 32; ++j) {
-                for (int k = 24; k < i; ++k) sK[j][i] -= sK[j][k] * sK[k][i];
-                sK[j][i] *= f;
-            }
-        }
-        double rhs[8], x[8];
-        for (int i = 0; i < 8; ++i) {
-            double v = 0.0;
-            for (int j = 0; j < 24; ++j) v -= sK[24 + i][j] * sU[j];
-            rhs[i] = v;
-        }
-        solve_ii(rhs, x);
-        for (int i = 0; i < 8; ++i) sU[24 + i] = x[i];
-    }
-    __syncthreads();
-    for (int it = 0; it <= iters; ++it) {
-        if (lane < 32) {
-            cx ep[4][9];
-            shell4_corot_strains(lane, h, sU, sX, sAl, sN, sNx, ep);
-            double r = 0.0, en = 0.0;
-            for (int g = 0; g < 4; ++g)
-                for (int m = 0; m < 9; ++m) {
-                    double s = 0.0;
-                    for (int k = 0; k < 9; ++k) s += sD[m][k] * ep[g][k].r;
-                    r += sW[g] * (ep[g][m].i / h) * s;
-                    en += 0.5 * sW[g] * ep[g][m].r * s;
-                }
-            sR[lane] = r;
-            if (lane == 0) sEn = en;
-        }
-        __syncthreads();
-        if (it == iters) break;
-        if (lane == 0) {
-            double x[8];
-            solve_ii(&sR[24], x);
-            for (int i = 0; i < 8; ++i) sU[24 + i] -= x[i];
-        }
-        __syncthreads();
-    }
-    __shared__ double sY[8];
-    if (lane == 0) solve_ii(&sR[24], sY);
-    __syncthreads();
-    if (lane < 24) {
-        double v = sR[lane];
-        for (int m = 0; m < 8; ++m) v -= sK[lane][24 + m] * sY[m];
-        force[sDof[lane] - 1] += v;
-    }
-    if (lane == 0 && energy) energy[e] = sEn;
+    shell4_condense_scatter(e, lane, L.K, sXs, L.Node, rank, nSolid, nSec, csrRange, elPos, val);
 }
 
 // ---------------------------------------------------------------------------

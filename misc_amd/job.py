@@ -696,8 +696,8 @@ class Job:
     def _model_at(self, time):
         return model_at(self.model, self.info, self.design, self.dvec, time)
 
-    def _options(self):
-        return astro.Options(literalMaxBW=False, thermLiteralD8=False, rtol=1e-12)
+    def _options(self, coupled=False):
+        return astro.Options(literalMaxBW=False, thermLiteralD8=False, rtol=1e-12, coupledProfile=coupled)
 
     def cmd_solve(self, a):
         self._need_model()
@@ -711,8 +711,10 @@ class Job:
         gamma = float(a.get("newmarkGamma", 0.5) or 0.5)
         period = float(a.get("simPeriod", 0.0) or 0.0)
         save = yes("saveSolnHist", "no")
-        if yes("nonLinearGeom", "no"):
-            self.log(" Warning: nonLinearGeom: the device path solves the linear step")
+        nonlinear = yes("nonLinearGeom", "no")
+        shell = self.model is not None and bool(np.all(self.model.etype == SHELL4))
+        if nonlinear and not shell:
+            self.log(" Warning: nonLinearGeom: the device path solves the linear step of brick8 models")
         self.history = []
         for c in (self.ctx, self.tctx):
             if c is not None:
@@ -730,9 +732,18 @@ class Job:
                 self.tctx.thermal_solve()
                 self._grab_thermal()
             if elastic:
-                self.ctx = self.tctx if self.tctx is not None else astro.Context(m, self._options())
+                # shells factor over the coupled profile: the reference's
+                # component-block factor leaves GMRES short of the tolerance on
+                # thin shells (1000 cycles on a 1 x 0.1 x 0.002 strip)
+                nl_shell = nonlinear and shell
+                self.ctx = self.tctx if self.tctx is not None else astro.Context(m, self._options(shell))
                 self._initial(self.ctx)
-                self.ctx.static_solve()
+                if nl_shell:
+                    # stepSolve's Newton loop with nonLinearGeom: at most 20 steps
+                    # (Working/AStrO_commandFunctions.f90:706-747)
+                    self.ctx.shell_nonlinear_solve(max_it=20, tol=1e-12)
+                else:
+                    self.ctx.static_solve()
                 self._grab_elastic()
             return
         if self.delT <= 0.0 or period <= 0.0:

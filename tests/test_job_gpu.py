@@ -344,3 +344,62 @@ def test_dynamic_job_gradient():
     gr = o.gradient(terms, U, A, scales, windows=[(0.0, 1.0), (5.0e-5, 1.0)])
     np.testing.assert_allclose(g, gr, rtol=1e-9, atol=1e-12 * np.abs(gr).max())
     assert np.abs(gr).min() > 0
+
+
+SHELL_SCRIPT = """*readModelInput:
+    fileName: model.yaml
+*solve:
+    elastic: yes
+    thermal: no
+    nonLinearGeom: {nl}
+    dynamic: no
+*writeNodeResults:
+    fileName: nodes.yaml
+    fields:
+        - displacement
+"""
+
+
+def test_nonlinear_shell_job_bends_the_strip_into_an_arc():
+    """*solve with nonLinearGeom on a shell4 model runs stepSolve's Newton
+    loop (at most 20 steps, Working/AStrO_commandFunctions.f90:706-747) on
+    the co-rotational force and tangent: a 1 x 0.1 strip (t = 0.002, nu = 0)
+    clamped at x = 0 under the end moment 0.05 E I about y takes the arc of
+    curvature 0.05, whose tip moves in x by -L^3 k^2 / 6 to second order; the
+    same job without nonLinearGeom moves the tip only in z."""
+    from misc_amd.meshes import shell_plate
+
+    nx, ny, kl = 24, 2, 0.05
+    m = shell_plate(nx, ny, nu=0.0, tip_load=0.0)
+    EI = 70e9 * 0.1 * 0.002 ** 3 / 12
+    tip = m.node_sets["tip"]
+    m.load_nodes = tip
+    m.load_vals = np.zeros((tip.size, 7))
+    m.load_vals[:, 4] = kl * EI / ny
+    m.load_vals[[0, -1], 4] *= 0.5
+    out = {}
+    for nl in ("yes", "no"):
+        with tempfile.TemporaryDirectory() as wd:
+            write_model(m, os.path.join(wd, "model.yaml"))
+            txt = open(os.path.join(wd, "model.yaml")).read()
+            solid = "    - type: solid\n      material: mat1\n      elementSet: es1\n"
+            assert txt.count(solid) == 1
+            txt = txt.replace(solid, "    - type: shell\n      elementSet: es1\n"
+                              "      orientation: [1.0, 0.0, 0.0, 0.0, 1.0, 0.0]\n      layup:\n"
+                              "          zOffset: 0.0\n          layers:\n              - [mat1, 0.002, 0.0]\n")
+            with open(os.path.join(wd, "model.yaml"), "w") as f:
+                f.write(txt)
+            with open(os.path.join(wd, "job.txt"), "w") as f:
+                f.write(SHELL_SCRIPT.format(nl=nl))
+            print(f"job: shell strip nonLinearGeom={nl}", flush=True)
+            assert J.main([os.path.join(wd, "job.txt"), wd]) == 0
+            nodes = open(os.path.join(wd, "nodes.yaml")).read()
+        out[nl] = np.array([[float(v) for v in r[1:]] for r in _rows(nodes, "displacement")])
+    mid = tip[ny // 2] - 1
+    ux_e, uz_e = np.sin(kl) / kl - 1.0, -(1.0 - np.cos(kl)) / kl
+    d = out["yes"][mid]
+    assert abs(d[2] - uz_e) <= 2e-4 * abs(uz_e)
+    assert abs(d[0] - ux_e) <= 1e-3 * abs(ux_e)
+    assert abs(d[4] - kl) <= 1e-9 * kl
+    lin = out["no"][mid]
+    assert abs(lin[0]) <= 1e-6 * abs(ux_e) and abs(lin[2] + kl / 2) <= 1e-3 * kl / 2

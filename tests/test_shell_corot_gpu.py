@@ -131,3 +131,144 @@ def test_corot_force_at_the_solution_matches_the_linear_residual():
             free[nd[i] - 1] = False
         errs.append(np.abs(F[free] - load[free]).max() / load_n)
     assert errs[0] < 2e-3 and 7 < errs[1] / errs[0] < 13
+
+
+def _csr_dense(ctx, n):
+    K = ctx.getd("elasticMat")
+    rng, cols = ctx.geti("elMatRange"), ctx.geti("elMatCols")
+    A = np.zeros((n, n))
+    for i in range(n):
+        for s in range(rng[i], rng[i + 1]):
+            if cols[s]:
+                A[i, cols[s] - 1] += K[s]
+    return A
+
+
+def test_corot_tangent_is_the_derivative_of_the_force():
+    """The assembled co-rotational tangent at a displacement with finite
+    rotations equals the central difference of the device force, column by
+    column; at rest it is the linear tangent of k_shell4_tangent.  Entries
+    are compared scaled by sqrt(K0_ii K0_jj), so that the bending block,
+    3e4 times softer than the membrane block on this plate, is held to the
+    same relative bar (host step 1e-7: ~1e-8 scaled error)."""
+    m = tilted_plate()
+    ctx = astro.Context(m)
+    n = ctx.scalar("elMatDim")
+    nd = ctx.geti("nDofIndex").reshape(-1, 6)
+    ctx.assemble()
+    ctx.sync()
+    K0 = _csr_dense(ctx, n)
+    print("corot tangent: at rest", flush=True)
+    ctx.shell_tangent(np.zeros(n), iters=16)
+    Kr = _csr_dense(ctx, n)
+    dg = np.sqrt(np.abs(np.diag(K0)))
+    S = np.outer(1.0 / dg, 1.0 / dg)
+    assert np.abs((Kr - K0) * S).max() <= 1e-8
+    u = _plate_u(m, nd, n, 0.3, 3e-3, 7)
+    print("corot tangent: finite rotations", flush=True)
+    ctx.shell_tangent(u, iters=16)
+    Kt = _csr_dense(ctx, n)
+    rot = np.zeros(n, bool)
+    for i in range(m.num_nodes):
+        for v in range(3, 6):
+            if nd[i, v]:
+                rot[nd[i, v] - 1] = True
+    Kfd = np.zeros((n, n))
+    for j in range(n):
+        d = 1e-7 if rot[j] else 1e-7 * 0.05
+        up, um = u.copy(), u.copy()
+        up[j] += d
+        um[j] -= d
+        Fp, _ = ctx.shell_internal_force(up, iters=16)
+        Fm, _ = ctx.shell_internal_force(um, iters=16)
+        Kfd[:, j] = (Fp - Fm) / (up[j] - um[j])
+    ctx.close()
+    assert np.abs((Kt - Kt.T) * S).max() <= 1e-9
+    # the geometric terms are of the order of the finite rotations
+    assert np.abs((Kt - K0) * S).max() > 1e-2
+    assert np.abs((Kt - Kfd) * S).max() <= 1e-6
+
+
+def _strip_under_end_moment(kl, nx=24, ny=2):
+    """A 1 x 0.1 strip (nu = 0, t = 0.002) clamped at x = 0 with the end moment
+    kl E I about y spread over the tip nodes, in a context whose coupled
+    profile makes the factor a direct solver of the shell tangent (each
+    Newton step's GMRES ends in a few iterations)."""
+    m = shell_plate(nx, ny, nu=0.0, tip_load=0.0)
+    EI = 70e9 * 0.1 * 0.002 ** 3 / 12
+    tip = m.node_sets["tip"]
+    vals = np.zeros((tip.size, 7))
+    w = np.full(tip.size, 1.0 / ny)
+    w[0] = w[-1] = 0.5 / ny
+    vals[:, 4] = kl * EI * w
+    m.load_nodes, m.load_vals = tip, vals
+    return m, astro.Context(m, astro.Options(coupledProfile=True, rtol=1e-12, maxCycles=50))
+
+
+def _check_arc(m, ctx, kl, ny=2):
+    """The elastica of a constant moment is a circular arc of curvature
+    k = M / (E I): the tip goes to x = sin(k L) / k, z = -(1 - cos(k L)) / k
+    and turns by k L (L = 1).  24 elements put the tip within 2e-4 of its
+    travel in z, 1e-3 in x."""
+    u = ctx.getd("nodeDisp")
+    nd = ctx.geti("nDofIndex").reshape(-1, 6)
+    ux_e, uz_e = np.sin(kl) / kl - 1.0, -(1.0 - np.cos(kl)) / kl
+    mid = m.node_sets["tip"][ny // 2] - 1
+    ux, uz, ry = u[nd[mid, 0] - 1], u[nd[mid, 2] - 1], u[nd[mid, 4] - 1]
+    assert abs(uz - uz_e) <= 2e-4 * abs(uz_e)
+    assert abs(ux - ux_e) <= 1e-3 * abs(ux_e)
+    assert abs(ry - kl) <= 1e-9 * kl
+    return u
+
+
+@pytest.mark.parametrize("kl", [0.2, 1.0])
+def test_newton_from_the_arc_converges_quadratically(kl):
+    """Started on the closed-form arc, the Newton solve of the co-rotational
+    force with its assembled tangent closes the discretisation error in
+    quadratically shrinking steps (3e-6, 2e-9, 6e-15 at kL = 0.2): the
+    tangent is the residual's derivative.  A linear solve from the arc
+    would move it by the size of the membrane strain of the bending."""
+    m, ctx = _strip_under_end_moment(kl)
+    nd = ctx.geti("nDofIndex").reshape(-1, 6)
+    u = np.zeros(ctx.scalar("elMatDim"))
+    for i in range(m.num_nodes):
+        x = m.nodes[i, 0]
+        u[nd[i, 0] - 1] = np.sin(kl * x) / kl - x
+        u[nd[i, 2] - 1] = -(1 - np.cos(kl * x)) / kl
+        u[nd[i, 4] - 1] = kl * x
+    ctx.setd("nodeDisp", u)
+    print(f"newton from the arc kL={kl}", flush=True)
+    steps = []
+    for _ in range(3):
+        u0 = ctx.getd("nodeDisp")
+        assert ctx.shell_nonlinear_solve(max_it=1, tol=1e-300) == 1
+        steps.append(np.abs(ctx.getd("nodeDisp") - u0).max())
+    assert steps[0] < 1e-3 * kl
+    assert steps[1] <= 1e4 * steps[0] ** 2
+    assert steps[2] <= 1e-12
+    assert ctx.shell_nonlinear_solve(max_it=5, tol=1e-10) == 1
+    _check_arc(m, ctx, kl)
+    ctx.close()
+
+
+@pytest.mark.parametrize("kl, steps", [(0.05, 1), (0.5, 10)])
+def test_nonlinear_plate_under_end_moment_bends_into_a_circle(kl, steps):
+    """The Newton solve from rest reaches the arc under the end moment in one
+    load step for a small turn, as the reference's nonlinear solve does; a
+    turn of 0.5 rad takes ten load steps, each solve starting from the last
+    one's solution (larger steps send Newton on this thin strip, whose
+    membrane stiffness is 3e5 times its bending stiffness, to other
+    equilibria).  The linear solve would move the tip only in z."""
+    u = None
+    for q in range(1, steps + 1):
+        m, ctx = _strip_under_end_moment(kl * q / steps)
+        if u is not None:
+            ctx.setd("nodeDisp", u)
+        print(f"end moment kL={kl} load step {q}", flush=True)
+        it = ctx.shell_nonlinear_solve(max_it=30, tol=1e-10)
+        assert it < 30
+        u = ctx.getd("nodeDisp")
+        if q < steps:
+            ctx.close()
+    _check_arc(m, ctx, kl)
+    ctx.close()
