@@ -259,24 +259,26 @@ int astro_ctx_dRdT_product(astro_ctx* ctx, const double* lam, double* out);
  * DOF recovery in the reference (it carries them at zero).  Synchronous. */
 int astro_ctx_shell_internal_force(astro_ctx* ctx, const double* u, int32_t iters, double* force,
                                    double* elEnergy);
-/* Co-rotational tangent of a shell4 model at the displacement u (elMatDim
- * doubles; null: the current nodeDisp) into the CSR values elasticMat: the
- * assembly of getElasticSolnLoad(1) with nonLinearGeom
- * (Working/AStrO_commandFunctions.f90:260-351, element dRdU of r_getElRuk,
- * Working/AStrO_r_elementEqns.f90:2245-2423).  Each element matrix is the
- * derivative of astro_ctx_shell_internal_force's condensed force (central
- * differences of the complex-step gradient, internal DOFs at element
- * equilibrium after intIters (0..64) chord steps).  Synchronous. */
-int astro_ctx_shell_tangent(astro_ctx* ctx, const double* u, int32_t intIters);
-/* Nonlinear static solve of a shell4 model, stepSolve's elastic loop with
- * nonLinearGeom (Working/AStrO_commandFunctions.f90:704-747): Newton steps
- * (K(u) + C^T C) du = F - R(u) + C^T (rhs - C u), R the co-rotational
- * internal force above (intIters internal-DOF steps) and K(u) the tangent
- * above, assembled and factored at every step, until max|du| <= tol max|u|
- * or maxIt steps (ASTRO_ERR_NO_CONVERGE if du stops being finite).  Starts
+/* Tangent at the displacement u (elMatDim doubles; null: the current
+ * nodeDisp) into the CSR values elasticMat: the assembly of
+ * getElasticSolnLoad(1) with nonLinearGeom (Working/AStrO_commandFunctions.f90:
+ * 260-351, element dRdU of r_getElRuk, Working/AStrO_r_elementEqns.f90:
+ * 2206-2533).  Each element matrix is the derivative of the element's
+ * internal force at u: shell4, astro_ctx_shell_internal_force's condensed
+ * force (central differences of the complex-step gradient, internal DOFs at
+ * element equilibrium after intIters (0..64) chord steps); brick8, the
+ * Green-Lagrange force (material and geometric parts, exact).  Synchronous. */
+int astro_ctx_nonlinear_tangent(astro_ctx* ctx, const double* u, int32_t intIters);
+/* Nonlinear static solve, stepSolve's elastic loop with nonLinearGeom
+ * (Working/AStrO_commandFunctions.f90:704-747): Newton steps
+ * (K(u) + C^T C) du = F - R(u) + C^T (rhs - C u), R the internal force at u
+ * (co-rotational shell4 with intIters internal-DOF steps, Green-Lagrange
+ * brick8) and K(u) the tangent above, assembled and factored at every step,
+ * until max|du| <= tol max|u| or maxIt steps (ASTRO_ERR_NO_CONVERGE if du
+ * stops being finite).  Models with thermal expansion are refused.  Starts
  * from the current nodeDisp; *iters (may be null): the steps taken.
  * Synchronous. */
-int astro_ctx_shell_nonlinear_solve(astro_ctx* ctx, int32_t maxIt, double tol, int32_t intIters, int32_t* iters);
+int astro_ctx_nonlinear_solve(astro_ctx* ctx, int32_t maxIt, double tol, int32_t intIters, int32_t* iters);
 
 /* Newmark dynamics, the dynamic branch of solve()
  * (Working/AStrO_commandFunctions.f90:762-810; element residual

@@ -45,8 +45,8 @@ EXPORTS = [
     "astro_gnn_field_blocks", "astro_ctx_gnn_attach", "astro_ctx_gnn_detach", "astro_ctx_gnn_get",
     "astro_ctx_set_design_vars", "astro_ctx_dRdD_vars", "astro_ctx_dRtdD_vars", "astro_ctx_objective",
     "astro_ctx_objective_gradient", "astro_ctx_element_flux", "astro_ctx_shell_internal_force",
-    "astro_ctx_dRdT_product", "astro_ctx_dynamic_objective_gradient", "astro_ctx_shell_nonlinear_solve",
-    "astro_ctx_shell_tangent",
+    "astro_ctx_dRdT_product", "astro_ctx_dynamic_objective_gradient", "astro_ctx_nonlinear_solve",
+    "astro_ctx_nonlinear_tangent",
 ]
 
 # Design-variable categories and objective terms (include/astro_abi.h).
@@ -210,8 +210,8 @@ def load_library(path: str = LIB_PATH):
     L.astro_ctx_element_flux.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.astro_ctx_shell_internal_force.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _dp, _dp]
     L.astro_ctx_dRdT_product.argtypes = [C.c_void_p, _dp, _dp]
-    L.astro_ctx_shell_tangent.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
-    L.astro_ctx_shell_nonlinear_solve.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+    L.astro_ctx_nonlinear_tangent.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+    L.astro_ctx_nonlinear_solve.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                                                   C.POINTER(C.c_int32)]
     L.astro_ctx_dynamic_objective_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, C.c_void_p, C.c_void_p,
                                                        C.c_int32, C.c_void_p, _dp, _dp]
@@ -688,12 +688,13 @@ class Context:
         _check(load_library().astro_ctx_element_flux(self.h, np.ascontiguousarray(point, np.float64), fl, gr))
         return fl.reshape(E, 3), gr.reshape(E, 3)
 
-    def shell_nonlinear_solve(self, max_it=100, tol=1e-10, int_iters=16):
-        """Newton static solve of a shell4 model with the co-rotational
-        internal force and its tangent; returns the steps taken."""
+    def nonlinear_solve(self, max_it=100, tol=1e-10, int_iters=16):
+        """Newton static solve with the internal force at u and its tangent
+        (co-rotational shell4, Green-Lagrange brick8); returns the steps
+        taken."""
         it = C.c_int32(0)
-        _check(load_library().astro_ctx_shell_nonlinear_solve(self.h, int(max_it), float(tol), int(int_iters),
-                                                               C.byref(it)))
+        _check(load_library().astro_ctx_nonlinear_solve(self.h, int(max_it), float(tol), int(int_iters),
+                                                         C.byref(it)))
         return it.value
 
     def dRdT_product(self, lam):
@@ -704,17 +705,17 @@ class Context:
         _check(load_library().astro_ctx_dRdT_product(self.h, lam, out))
         return out
 
-    def shell_tangent(self, u=None, iters=16):
-        """Assemble the co-rotational tangent of a shell4 model at the
-        displacement u (elMatDim; None: the current solution) into the CSR
-        values (``getd("elasticMat")``)."""
+    def nonlinear_tangent(self, u=None, iters=16):
+        """Assemble the tangent at the displacement u (elMatDim; None: the
+        current solution) into the CSR values (``getd("elasticMat")``):
+        co-rotational for shell4, Green-Lagrange for brick8."""
         up = None
         if u is not None:
             u = np.ascontiguousarray(u, np.float64)
             if u.shape != (self.scalar("elMatDim"),):
                 raise ValueError("u must hold elMatDim values")
             up = u.ctypes.data
-        _check(load_library().astro_ctx_shell_tangent(self.h, up, int(iters)))
+        _check(load_library().astro_ctx_nonlinear_tangent(self.h, up, int(iters)))
 
     def shell_internal_force(self, u=None, iters=16):
         """Co-rotational internal force (elMatDim) and per-element strain

@@ -1756,27 +1756,64 @@ int astro_ctx_shell_internal_force(astro_ctx* c, const double* u, int32_t iters,
 // 1e-16 |R| over it; both stay near 1e-10 of the tangent at this step.
 static constexpr double kShellTangentStep = 1e-6;
 
-static void launch_shell_corot_tangent(astro_ctx* c, const double* u, int intIters) {
+// The tangent at u into the CSR values: co-rotational for shell4 models,
+// Green-Lagrange for brick8 models (whose node-block operator is refreshed).
+static int launch_nonlinear_tangent(astro_ctx* c, const double* u, int intIters) {
     const int64_t nnz = (int64_t)c->prep.elMatCols.size();
     hipLaunchKernelGGL(k_fill, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, c->s, c->csrVal.p, nnz, 0.0);
     for (size_t q = 0; q + 1 < c->colorStart.size(); ++q) {
         const int cnt = c->colorStart[q + 1] - c->colorStart[q];
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_shell4_corot_tangent, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[q], cnt,
-                           c->conn.p, c->X.p, c->elABD.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
-                           c->prep.numSectionNds, u, 1e-30, intIters, kShellTangentStep, c->csrRange.p, c->elPos.p,
-                           c->csrVal.p);
+        if (c->prep.shellMesh)
+            hipLaunchKernelGGL(k_shell4_corot_tangent, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[q],
+                               cnt, c->conn.p, c->X.p, c->elABD.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
+                               c->prep.numSectionNds, u, 1e-30, intIters, kShellTangentStep, c->csrRange.p,
+                               c->elPos.p, c->csrVal.p);
+        else
+            hipLaunchKernelGGL(k_brick8_tangent_gl, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[q], cnt,
+                               c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->rank.p,
+                               c->prep.numSolidNds, u, c->csrRange.p, c->elPos.p, c->csrVal.p);
+    }
+    HIP_TRY(hipGetLastError());
+    return c->prep.shellMesh ? ASTRO_OK : refresh_node_operator(c);
+}
+
+// load = -R(u): the co-rotational shell force (intIters chord steps on the
+// internal DOFs; rv: elMatDim scratch) or the Green-Lagrange brick force
+static void launch_minus_internal_force(astro_ctx* c, const double* u, int intIters, double* rv, double* load) {
+    const int n = c->prep.elMatDim;
+    hipStream_t s = c->s;
+    if (c->prep.shellMesh) {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rv, (int64_t)n, 0.0);
+        for (size_t q = 0; q + 1 < c->colorStart.size(); ++q) {
+            const int cnt = c->colorStart[q + 1] - c->colorStart[q];
+            if (cnt <= 0) continue;
+            hipLaunchKernelGGL(k_shell4_corot, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[q], cnt,
+                               c->conn.p, c->X.p, c->elABD.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
+                               c->prep.numSectionNds, u, 1e-30, intIters, rv, (double*)nullptr);
+        }
+        hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)rv,
+                           (const double*)rv, load, n, -1.0, 0.0, (const int*)nullptr);
+        return;
+    }
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, load, (int64_t)n, 0.0);
+    for (size_t q = 0; q + 1 < c->colorStart.size(); ++q) {
+        const int cnt = c->colorStart[q + 1] - c->colorStart[q];
+        if (cnt <= 0) continue;
+        // c1 = 0: no inertia term
+        hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[q], cnt,
+                           c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p,
+                           c->prep.numSolidNds, u, u, u, u, 0.0, 0.0, 0.0, 0.0, load, (const int*)nullptr);
     }
 }
 
-// Co-rotational tangent of a shell4 model at the displacement u (host,
-// elMatDim; null: the current nodeDisp) into the CSR values (elasticMat):
-// getElasticSolnLoad(1)'s assembly with nonlinear geometry
-// (Working/AStrO_commandFunctions.f90:260-351), the element matrix the
-// derivative of k_shell4_corot's condensed force (k_shell4_corot_tangent).
-int astro_ctx_shell_tangent(astro_ctx* c, const double* u, int32_t intIters) {
+// Tangent at the displacement u (host, elMatDim; null: the current nodeDisp)
+// into the CSR values (elasticMat): getElasticSolnLoad(1)'s assembly with
+// nonlinear geometry (Working/AStrO_commandFunctions.f90:260-351), the
+// element matrix the derivative of the element's internal force at u
+// (k_shell4_corot_tangent, k_brick8_tangent_gl).
+int astro_ctx_nonlinear_tangent(astro_ctx* c, const double* u, int32_t intIters) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
-    if (!c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "the co-rotational tangent covers shell4 models");
     if (intIters < 0 || intIters > 64) return fail(ASTRO_ERR_ARG, "internal-DOF iterations must lie in [0, 64]");
     const int64_t n = c->prep.elMatDim;
     DBuf<double> du;
@@ -1785,23 +1822,22 @@ int astro_ctx_shell_tangent(astro_ctx* c, const double* u, int32_t intIters) {
         ASTRO_TRY(du.upload(u, (size_t)n, c->s));
         ud = du.p;
     }
-    launch_shell_corot_tangent(c, ud, intIters);
-    HIP_TRY(hipGetLastError());
+    ASTRO_TRY(launch_nonlinear_tangent(c, ud, intIters));
     HIP_TRY(hipStreamSynchronize(c->s));
     return ASTRO_OK;
 }
 
-// Nonlinear static shell solve, stepSolve's elastic loop with nonLinearGeom
+// Nonlinear static solve, stepSolve's elastic loop with nonLinearGeom
 // (Working/AStrO_commandFunctions.f90:704-747): up to maxIt Newton steps
-// u += du with (K(u) + C^T C) du = F - R(u) + C^T (rhs - C u), R the
-// co-rotational internal force (k_shell4_corot, intIters chord steps on the
-// internal DOFs) and K(u) its derivative (k_shell4_corot_tangent), assembled
-// and factored at every step as the reference does, until
-// max|du| <= tol max|u|.  The penalty rows are scaled once, from the tangent at
-// rest.  Starts from the current nodeDisp; *iters: the steps taken.
-int astro_ctx_shell_nonlinear_solve(astro_ctx* c, int32_t maxIt, double tol, int32_t intIters, int32_t* iters) {
+// u += du with (K(u) + C^T C) du = F - R(u) + C^T (rhs - C u), R the internal
+// force at u (co-rotational shell4 with intIters chord steps on the internal
+// DOFs, Green-Lagrange brick8) and K(u) its derivative, assembled and
+// factored at every step as the reference does, until max|du| <= tol max|u|.
+// The penalty rows are scaled once, from the tangent at rest.  Starts from the
+// current nodeDisp; *iters: the steps taken.
+int astro_ctx_nonlinear_solve(astro_ctx* c, int32_t maxIt, double tol, int32_t intIters, int32_t* iters) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
-    if (!c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "the nonlinear solve covers shell4 models");
+    if (c->hasTExp) return fail(ASTRO_ERR_UNSUPPORTED, "the nonlinear solve has no thermal strain");
     if (maxIt < 1 || !(tol > 0.0) || intIters < 0 || intIters > 64) return fail(ASTRO_ERR_ARG, "bad iteration limits");
     const int n = c->prep.elMatDim;
     hipStream_t s = c->s;
@@ -1812,22 +1848,11 @@ int astro_ctx_shell_nonlinear_solve(astro_ctx* c, int32_t maxIt, double tol, int
     std::vector<double> hdu(n), hu(n);
     int k = 0;
     for (; k < maxIt; ++k) {
-        launch_shell_corot_tangent(c, c->nodeDisp.p, intIters);
-        HIP_TRY(hipGetLastError());
+        ASTRO_TRY(launch_nonlinear_tangent(c, c->nodeDisp.p, intIters));
         ASTRO_TRY(astro_ctx_build_profile(c));
         ASTRO_TRY(astro_ctx_factor(c));
         // load = F - R(u) + C^T (rhs - C u)
-        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rv.p, (int64_t)n, 0.0);
-        for (size_t q = 0; q + 1 < c->colorStart.size(); ++q) {
-            const int cnt = c->colorStart[q + 1] - c->colorStart[q];
-            if (cnt <= 0) continue;
-            hipLaunchKernelGGL(k_shell4_corot, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[q], cnt,
-                               c->conn.p, c->X.p, c->elABD.p, c->elAlpha.p, c->rank.p, c->prep.numSolidNds,
-                               c->prep.numSectionNds, (const double*)c->nodeDisp.p, 1e-30, (int)intIters, rv.p,
-                               (double*)nullptr);
-        }
-        hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (const double*)rv.p,
-                           (const double*)rv.p, c->load.p, n, -1.0, 0.0, (const int*)nullptr);
+        launch_minus_internal_force(c, c->nodeDisp.p, intIters, rv.p, c->load.p);
         const int nl = (int)c->loadDofs.n;
         if (nl > 0)
             hipLaunchKernelGGL(k_nodal_loads, dim3(grid_for(nl, 256)), dim3(256), 0, s, c->loadDofs.p,
@@ -1851,7 +1876,7 @@ int astro_ctx_shell_nonlinear_solve(astro_ctx* c, int32_t maxIt, double tol, int
             mdu = std::max(mdu, std::fabs(hdu[i]));
             mu = std::max(mu, std::fabs(hu[i]));
         }
-        if (!std::isfinite(mdu)) return fail(ASTRO_ERR_NO_CONVERGE, "nonlinear shell solve diverged");
+        if (!std::isfinite(mdu)) return fail(ASTRO_ERR_NO_CONVERGE, "nonlinear solve diverged");
         if (mdu <= tol * mu) {
             ++k;
             break;

@@ -54,6 +54,9 @@ enum {
 __global__ void k_brick8_tangent(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                  const double* elScale, const double* matC, const double* elRho, double massFactor,
                                  const int* rank, int nS, const int* csrRange, const uint8_t* elPos, double* val);
+__global__ void k_brick8_tangent_gl(const int* perm, int count, const int* conn, const double* X, const int* elMat,
+                                    const double* elScale, const double* matC, const int* rank, int nS, const double* u,
+                                    const int* csrRange, const uint8_t* elPos, double* val);
 __global__ void k_shell4_tangent(const int* perm, int count, const int* conn, const double* X, const double* elABD,
                                  const double* elAlpha, const int* rank, int nSolid, int nSec, const int* csrRange,
                                  const uint8_t* elPos, double* val);

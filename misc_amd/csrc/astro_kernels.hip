@@ -151,6 +151,124 @@ __global__ __launch_bounds__(64) void k_brick8_tangent(
     }
 }
 
+// Green-Lagrange tangent of brick8 elements at the displacement u, the
+// derivative of k_brick8_residual's internal force (the solid branch of
+// r_getElRuk with the strain of r_getNLStrain, Working/AStrO_r_elementEqns.f90:
+// 2206-2533, 622-680, which getElasticSolnLoad(1) assembles at every Newton
+// step with nonLinearGeom): per Gauss point, with F = I + grad u and
+// S = C E, the material part dE/du^T C dE/du, dE/du_(a,c) = sym(F^T e_c
+// grad N_a) in engineering Voigt order (11, 22, 33, 12, 13, 23), and the
+// geometric part grad N_a . S grad N_b on the diagonal of each 3x3 block.  At
+// u = 0 it is k_brick8_tangent's B^T C B.  Lane (a, b) owns the 3x3 block of
+// node pair (a, b), scattered as k_brick8_tangent's.
+__global__ __launch_bounds__(64) void k_brick8_tangent_gl(
+    const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
+    const int* __restrict__ elMat, const double* __restrict__ elScale, const double* __restrict__ matC,
+    const int* __restrict__ rank, int nS, const double* __restrict__ u, const int* __restrict__ csrRange,
+    const uint8_t* __restrict__ elPos, double* __restrict__ val) {
+    const int idx = blockIdx.x;
+    if (idx >= count) return;
+    const int e = perm[idx];
+    const int lane = threadIdx.x;
+    __shared__ double sX[8][3];
+    __shared__ double sNx[8][8][3];
+    __shared__ double sN[8][8];
+    __shared__ double sW[8];
+    __shared__ double sU[3][8], sF[8][3][3], sS[8][6];
+    __shared__ int sNode[8];
+    if (lane < 8) {
+        const int nd = conn[8 * (size_t)e + lane] - 1;
+        sNode[lane] = nd;
+        for (int c = 0; c < 3; ++c) sX[lane][c] = X[3 * (size_t)nd + c];
+    }
+    __syncthreads();
+    if (lane < 24) {
+        const int comp = lane >> 3, nd = lane & 7;
+        sU[comp][nd] = u[rank[sNode[nd]] + comp * nS - 1];
+    }
+    if (lane < 8) brick8_gauss_point(lane, sX, sN[lane], sNx[lane], sW[lane]);
+    __syncthreads();
+    const double sc = elScale[e];
+    const double* Cm = matC + 36 * (size_t)elMat[e];
+    if (lane < 8) {
+        const int g = lane;
+        double ux[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double acc = 0.0;
+                for (int n = 0; n < 8; ++n) acc += sU[i][n] * sNx[g][n][j];
+                ux[i][j] = acc;
+                sF[g][i][j] = (i == j ? 1.0 : 0.0) + acc;
+            }
+        double E[6];
+        E[0] = ux[0][0] + 0.5 * (ux[0][0] * ux[0][0] + ux[1][0] * ux[1][0] + ux[2][0] * ux[2][0]);
+        E[1] = ux[1][1] + 0.5 * (ux[0][1] * ux[0][1] + ux[1][1] * ux[1][1] + ux[2][1] * ux[2][1]);
+        E[2] = ux[2][2] + 0.5 * (ux[0][2] * ux[0][2] + ux[1][2] * ux[1][2] + ux[2][2] * ux[2][2]);
+        E[3] = ux[0][1] + ux[1][0] + ux[0][0] * ux[0][1] + ux[1][0] * ux[1][1] + ux[2][0] * ux[2][1];
+        E[4] = ux[0][2] + ux[2][0] + ux[0][0] * ux[0][2] + ux[1][0] * ux[1][2] + ux[2][0] * ux[2][2];
+        E[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
+        for (int r = 0; r < 6; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 6; ++k) acc += Cm[r + 6 * k] * E[k];
+            sS[g][r] = sc * acc;
+        }
+    }
+    __syncthreads();
+    const int a = lane >> 3, b = lane & 7;
+    double C[6][6];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) C[r][c] = sc * Cm[r + 6 * c];
+    double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int g = 0; g < 8; ++g) {
+        const double w = sW[g];
+        const double* xa = sNx[g][a];
+        const double* xb = sNx[g][b];
+        double Ba[3][6], Bb[3][6];
+        for (int c = 0; c < 3; ++c) {
+            const double f0 = sF[g][c][0], f1 = sF[g][c][1], f2 = sF[g][c][2];
+            Ba[c][0] = xa[0] * f0;
+            Ba[c][1] = xa[1] * f1;
+            Ba[c][2] = xa[2] * f2;
+            Ba[c][3] = xa[1] * f0 + xa[0] * f1;
+            Ba[c][4] = xa[2] * f0 + xa[0] * f2;
+            Ba[c][5] = xa[2] * f1 + xa[1] * f2;
+            Bb[c][0] = xb[0] * f0;
+            Bb[c][1] = xb[1] * f1;
+            Bb[c][2] = xb[2] * f2;
+            Bb[c][3] = xb[1] * f0 + xb[0] * f1;
+            Bb[c][4] = xb[2] * f0 + xb[0] * f2;
+            Bb[c][5] = xb[2] * f1 + xb[1] * f2;
+        }
+        for (int d = 0; d < 3; ++d) {
+            double CB[6];
+            for (int r = 0; r < 6; ++r) {
+                double acc = 0.0;
+                for (int k = 0; k < 6; ++k) acc += C[r][k] * Bb[d][k];
+                CB[r] = acc;
+            }
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0.0;
+                for (int r = 0; r < 6; ++r) acc += Ba[c][r] * CB[r];
+                K[c][d] += w * acc;
+            }
+        }
+        const double* S = sS[g];
+        const double geo = S[0] * xa[0] * xb[0] + S[1] * xa[1] * xb[1] + S[2] * xa[2] * xb[2] +
+                           S[3] * (xa[0] * xb[1] + xa[1] * xb[0]) + S[4] * (xa[0] * xb[2] + xa[2] * xb[0]) +
+                           S[5] * (xa[1] * xb[2] + xa[2] * xb[1]);
+        for (int c = 0; c < 3; ++c) K[c][c] += w * geo;
+    }
+    const int ra = rank[sNode[a]];
+    const uint8_t* pos = elPos + 192 * (size_t)e + 3 * lane;
+    const int p0 = pos[0], p1 = pos[1], p2 = pos[2];
+    for (int c = 0; c < 3; ++c) {
+        double* v = val + csrRange[ra + c * nS - 1];
+        v[p0] += K[c][0];
+        v[p1] += K[c][1];
+        v[p2] += K[c][2];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Shell4 tangent at rest with static condensation, one 64-lane workgroup per
 // element (colour classes: elements of one launch share no node).

@@ -713,8 +713,6 @@ class Job:
         save = yes("saveSolnHist", "no")
         nonlinear = yes("nonLinearGeom", "no")
         shell = self.model is not None and bool(np.all(self.model.etype == SHELL4))
-        if nonlinear and not shell:
-            self.log(" Warning: nonLinearGeom: the device path solves the linear step of brick8 models")
         self.history = []
         for c in (self.ctx, self.tctx):
             if c is not None:
@@ -735,13 +733,12 @@ class Job:
                 # shells factor over the coupled profile: the reference's
                 # component-block factor leaves GMRES short of the tolerance on
                 # thin shells (1000 cycles on a 1 x 0.1 x 0.002 strip)
-                nl_shell = nonlinear and shell
                 self.ctx = self.tctx if self.tctx is not None else astro.Context(m, self._options(shell))
                 self._initial(self.ctx)
-                if nl_shell:
+                if nonlinear:
                     # stepSolve's Newton loop with nonLinearGeom: at most 20 steps
                     # (Working/AStrO_commandFunctions.f90:706-747)
-                    self.ctx.shell_nonlinear_solve(max_it=20, tol=1e-12)
+                    self.ctx.nonlinear_solve(max_it=20, tol=1e-12)
                 else:
                     self.ctx.static_solve()
                 self._grab_elastic()

@@ -166,6 +166,19 @@ program astro_ref_driver
 
     call loadInitialState()
 
+    if (trim(mode) .eq. 'nonlinear') then
+        ! solve() with nonLinearGeom (Working/AStrO_commandFunctions.f90:770-778):
+        ! the tangent at rest scales the penalty rows, then stepSolve's elastic
+        ! loop (:704-747) assembles and factors the tangent at u every step
+        call getElasticSolnLoad(1)
+        call scaleElasticMPC()
+        nLGeom = 1
+        call nonlinearElasticSteps()
+        call dumpd(nodeDisp, elMatDim, 'nodeDisp')
+        close(lfUnit)
+        stop
+    endif
+
     if (trim(mode) .eq. 'thermoelastic') then
         ! solve()'s elastic sequence (Working/AStrO_commandFunctions.f90:751-780):
         ! the tangent is assembled and factored at the zero nodeTemp the
@@ -333,6 +346,41 @@ contains
             elMPCMatRange,elMPCSize,elMPCDim,elMatLT,elMatLTRange,elMatLTSize,elasticLoad,numVecs,maxIt)
         nodeDisp(:) = nodeDisp(:) + delDisp(:)
     end subroutine elasticStep
+
+    ! Elastic branch of stepSolve with nonLinearGeom (Working/AStrO_commandFunctions.f90:
+    ! 704-747): at most 20 steps, each assembling the tangent and the residual at
+    ! u, factoring, solving and adding the increment, until max|du| <= 1e-12.
+    ! Nodal loads as elasticStep; the step count is dumped as 'nlSteps'.
+    subroutine nonlinearElasticSteps()
+        integer :: numVecs, maxIt, it
+        real*8 :: delUNorm
+        if (elMatDim .lt. 200) then
+            numVecs = elMatDim/2
+        else
+            numVecs = 100
+        endif
+        maxIt = elMatDim
+        if (nvArg .gt. 0) numVecs = nvArg
+        if (itArg .gt. 0) maxIt = itArg
+        it = 0
+        delUNorm = 1d0
+        do while (it .lt. 20 .and. delUNorm .gt. 1d-12)
+            elasticLoad(:) = r_0
+            intElasticLoad(:) = r_0
+            call getElasticSolnLoad(1)
+            call nodalLoads()
+            call getElasticConstraintLoad()
+            call convertToLTri(elMatLT,elMatLTRange,elMatLTSize,elasticMat,elMatCols,elMatRange, &
+                 elMatSize,elMatDim,elMPCMat,elMPCMatCols,elMPCMatRange,elMPCSize,elMPCDim)
+            call getSparseLDLFact(elMatLT,elMatLTSize,elMatLTRange,elMatDim)
+            call gMRes(delDisp,elasticMat,elMatCols,elMatRange,elMatSize,elMatDim,elMPCMat,elMPCMatCols, &
+                elMPCMatRange,elMPCSize,elMPCDim,elMatLT,elMatLTRange,elMatLTSize,elasticLoad,numVecs,maxIt)
+            nodeDisp(:) = nodeDisp(:) + delDisp(:)
+            delUNorm = maxval(abs(delDisp))
+            it = it + 1
+        enddo
+        call dumpi((/it/), 1, 'nlSteps')
+    end subroutine nonlinearElasticSteps
 
     ! Thermal branch of stepSolve (Working/AStrO_commandFunctions.f90:689-703); the
     ! models driven here carry no thermal loads, only temperature constraints.

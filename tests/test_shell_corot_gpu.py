@@ -159,14 +159,14 @@ def test_corot_tangent_is_the_derivative_of_the_force():
     ctx.sync()
     K0 = _csr_dense(ctx, n)
     print("corot tangent: at rest", flush=True)
-    ctx.shell_tangent(np.zeros(n), iters=16)
+    ctx.nonlinear_tangent(np.zeros(n), iters=16)
     Kr = _csr_dense(ctx, n)
     dg = np.sqrt(np.abs(np.diag(K0)))
     S = np.outer(1.0 / dg, 1.0 / dg)
     assert np.abs((Kr - K0) * S).max() <= 1e-8
     u = _plate_u(m, nd, n, 0.3, 3e-3, 7)
     print("corot tangent: finite rotations", flush=True)
-    ctx.shell_tangent(u, iters=16)
+    ctx.nonlinear_tangent(u, iters=16)
     Kt = _csr_dense(ctx, n)
     rot = np.zeros(n, bool)
     for i in range(m.num_nodes):
@@ -241,12 +241,12 @@ def test_newton_from_the_arc_converges_quadratically(kl):
     steps = []
     for _ in range(3):
         u0 = ctx.getd("nodeDisp")
-        assert ctx.shell_nonlinear_solve(max_it=1, tol=1e-300) == 1
+        assert ctx.nonlinear_solve(max_it=1, tol=1e-300) == 1
         steps.append(np.abs(ctx.getd("nodeDisp") - u0).max())
     assert steps[0] < 1e-3 * kl
     assert steps[1] <= 1e4 * steps[0] ** 2
     assert steps[2] <= 1e-12
-    assert ctx.shell_nonlinear_solve(max_it=5, tol=1e-10) == 1
+    assert ctx.nonlinear_solve(max_it=5, tol=1e-10) == 1
     _check_arc(m, ctx, kl)
     ctx.close()
 
@@ -265,7 +265,7 @@ def test_nonlinear_plate_under_end_moment_bends_into_a_circle(kl, steps):
         if u is not None:
             ctx.setd("nodeDisp", u)
         print(f"end moment kL={kl} load step {q}", flush=True)
-        it = ctx.shell_nonlinear_solve(max_it=30, tol=1e-10)
+        it = ctx.nonlinear_solve(max_it=30, tol=1e-10)
         assert it < 30
         u = ctx.getd("nodeDisp")
         if q < steps:
