@@ -275,7 +275,8 @@ int astro_ctx_nonlinear_tangent(astro_ctx* ctx, const double* u, int32_t intIter
  * (co-rotational shell4 with intIters internal-DOF steps, Green-Lagrange
  * brick8) and K(u) the tangent above, assembled and factored at every step,
  * until max|du| <= tol max|u| or maxIt steps (ASTRO_ERR_NO_CONVERGE if du
- * stops being finite).  Models with thermal expansion are refused.  Starts
+ * stops being finite).  brick8 models with thermal expansion take the
+ * thermal strain at the current nodeTemp in the force and the tangent.  Starts
  * from the current nodeDisp; *iters (may be null): the steps taken.
  * Synchronous. */
 int astro_ctx_nonlinear_solve(astro_ctx* ctx, int32_t maxIt, double tol, int32_t intIters, int32_t* iters);

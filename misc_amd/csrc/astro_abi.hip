@@ -1772,7 +1772,8 @@ static int launch_nonlinear_tangent(astro_ctx* c, const double* u, int intIters)
         else
             hipLaunchKernelGGL(k_brick8_tangent_gl, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[q], cnt,
                                c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->rank.p,
-                               c->prep.numSolidNds, u, c->csrRange.p, c->elPos.p, c->csrVal.p);
+                               c->prep.numSolidNds, u, c->hasTExp ? (const double*)c->elTExp.p : nullptr,
+                               (const double*)c->nodeTemp.p, c->csrRange.p, c->elPos.p, c->csrVal.p);
     }
     HIP_TRY(hipGetLastError());
     return c->prep.shellMesh ? ASTRO_OK : refresh_node_operator(c);
@@ -1803,7 +1804,8 @@ static void launch_minus_internal_force(astro_ctx* c, const double* u, int intIt
         // c1 = 0: no inertia term
         hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[q], cnt,
                            c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p,
-                           c->prep.numSolidNds, u, u, u, u, 0.0, 0.0, 0.0, 0.0, load, (const int*)nullptr);
+                           c->prep.numSolidNds, u, u, u, u, 0.0, 0.0, 0.0, 0.0, c->hasTExp ? (const double*)c->elTExp.p : nullptr,
+                           (const double*)c->nodeTemp.p, load, (const int*)nullptr);
     }
 }
 
@@ -1837,7 +1839,7 @@ int astro_ctx_nonlinear_tangent(astro_ctx* c, const double* u, int32_t intIters)
 // current nodeDisp; *iters: the steps taken.
 int astro_ctx_nonlinear_solve(astro_ctx* c, int32_t maxIt, double tol, int32_t intIters, int32_t* iters) {
     if (!c) return fail(ASTRO_ERR_ARG, "null context");
-    if (c->hasTExp) return fail(ASTRO_ERR_UNSUPPORTED, "the nonlinear solve has no thermal strain");
+    if (c->hasTExp && c->prep.shellMesh) return fail(ASTRO_ERR_UNSUPPORTED, "shell4 models carry no expansion");
     if (maxIt < 1 || !(tol > 0.0) || intIters < 0 || intIters > 64) return fail(ASTRO_ERR_ARG, "bad iteration limits");
     const int n = c->prep.elMatDim;
     hipStream_t s = c->s;
@@ -1987,7 +1989,8 @@ int astro_ctx_dynamic_step(astro_ctx* c, double loadScale) {
         if (cnt <= 0) continue;
         hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, c->s, c->perm.p + c->colorStart[k], cnt,
                            c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p, nS,
-                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, c->load.p, (const int*)nullptr);
+                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, (const double*)nullptr, (const double*)nullptr, c->load.p,
+                           (const int*)nullptr);
     }
     const int nl = (int)c->loadDofs.n;
     if (nl > 0)
@@ -2045,7 +2048,7 @@ static int record_newmark_step(astro_ctx* c, Prof& P, Pc& Q, bool judge, int K, 
         if (cnt <= 0) continue;
         hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[k], cnt,
                            c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p, nS,
-                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, c->load.p, run);
+                           c->nodeDisp.p, c->prevDisp.p, c->prevVel.p, c->prevAcc.p, c1, c2, dt, beta, (const double*)nullptr, (const double*)nullptr, c->load.p, run);
     }
     const int nl = (int)c->loadDofs.n;
     if (nl > 0)
@@ -2889,7 +2892,8 @@ int astro_ctx_dynamic_objective_gradient(astro_ctx* c, int32_t nSteps, const dou
             if (cnt <= 0) continue;
             hipLaunchKernelGGL(k_brick8_residual, dim3(cnt), dim3(64), 0, s, c->perm.p + c->colorStart[k], cnt,
                                c->conn.p, c->X.p, c->elMat.p, c->elScale.p, c->matC.p, c->elRho.p, c->rank.p, nS,
-                               (const double*)zero.p, pu, pv, pa, c1, c2, dt, beta, load, (const int*)nullptr);
+                               (const double*)zero.p, pu, pv, pa, c1, c2, dt, beta, (const double*)nullptr,
+                               (const double*)nullptr, load, (const int*)nullptr);
         }
     };
     auto axpby = [&](double* y, const double* x, double a, double b) {  // y = a y + b x

@@ -56,7 +56,8 @@ __global__ void k_brick8_tangent(const int* perm, int count, const int* conn, co
                                  const int* rank, int nS, const int* csrRange, const uint8_t* elPos, double* val);
 __global__ void k_brick8_tangent_gl(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                     const double* elScale, const double* matC, const int* rank, int nS, const double* u,
-                                    const int* csrRange, const uint8_t* elPos, double* val);
+                                    const double* tExp, const double* T, const int* csrRange, const uint8_t* elPos,
+                                    double* val);
 __global__ void k_shell4_tangent(const int* perm, int count, const int* conn, const double* X, const double* elABD,
                                  const double* elAlpha, const int* rank, int nSolid, int nSec, const int* csrRange,
                                  const uint8_t* elPos, double* val);
@@ -81,7 +82,8 @@ __global__ void k_brick8_thermal_strain(int mode, const int* perm, int count, co
 __global__ void k_brick8_residual(const int* perm, int count, const int* conn, const double* X, const int* elMat,
                                   const double* elScale, const double* matC, const double* elRho, const int* rank,
                                   int nS, const double* u, const double* pu, const double* pv, const double* pa,
-                                  double c1, double c2, double dt, double beta, double* load, const int* gate);
+                                  double c1, double c2, double dt, double beta, const double* tExp, const double* T,
+                                  double* load, const int* gate);
 __global__ void k_newmark_update(int n, double* u, const double* du, double* pu, double* pv, double* pa, double* v,
                                  double* a, double c1, double c2, double dt, double beta, double gamma, const int* gate);
 __global__ void k_thermal_rate(int n, const double* T, const double* Tp, const double* Tdp, double gamma, double dt,

@@ -151,6 +151,19 @@ __global__ __launch_bounds__(64) void k_brick8_tangent(
     }
 }
 
+// E -= T alpha at a Gauss point with shape functions N: the thermal strain
+// of the element's expansion tExp[6 e..] at the interpolated nodal
+// temperature (T by rank; tExp null: none), stress = C (E - T alpha) as in
+// Working/AStrO_r_elementEqns.f90:2433-2447.
+__device__ __forceinline__ void brick8_subtract_thermal(int e, const double (&N)[8], const int (&sNode)[8],
+                                                        const int* __restrict__ rank, const double* __restrict__ tExp,
+                                                        const double* __restrict__ T, double (&E)[6]) {
+    if (!tExp) return;
+    double Tp = 0.0;
+    for (int n = 0; n < 8; ++n) Tp += N[n] * T[rank[sNode[n]] - 1];
+    for (int k = 0; k < 6; ++k) E[k] -= Tp * tExp[6 * (size_t)e + k];
+}
+
 // Green-Lagrange tangent of brick8 elements at the displacement u, the
 // derivative of k_brick8_residual's internal force (the solid branch of
 // r_getElRuk with the strain of r_getNLStrain, Working/AStrO_r_elementEqns.f90:
@@ -159,13 +172,15 @@ __global__ __launch_bounds__(64) void k_brick8_tangent(
 // S = C E, the material part dE/du^T C dE/du, dE/du_(a,c) = sym(F^T e_c
 // grad N_a) in engineering Voigt order (11, 22, 33, 12, 13, 23), and the
 // geometric part grad N_a . S grad N_b on the diagonal of each 3x3 block.  At
-// u = 0 it is k_brick8_tangent's B^T C B.  Lane (a, b) owns the 3x3 block of
+// u = 0 it is k_brick8_tangent's B^T C B.  With tExp (may be null) S is
+// C (E - T alpha), as in k_brick8_residual.  Lane (a, b) owns the 3x3 block of
 // node pair (a, b), scattered as k_brick8_tangent's.
 __global__ __launch_bounds__(64) void k_brick8_tangent_gl(
     const int* __restrict__ perm, int count, const int* __restrict__ conn, const double* __restrict__ X,
     const int* __restrict__ elMat, const double* __restrict__ elScale, const double* __restrict__ matC,
-    const int* __restrict__ rank, int nS, const double* __restrict__ u, const int* __restrict__ csrRange,
-    const uint8_t* __restrict__ elPos, double* __restrict__ val) {
+    const int* __restrict__ rank, int nS, const double* __restrict__ u, const double* __restrict__ tExp,
+    const double* __restrict__ T, const int* __restrict__ csrRange, const uint8_t* __restrict__ elPos,
+    double* __restrict__ val) {
     const int idx = blockIdx.x;
     if (idx >= count) return;
     const int e = perm[idx];
@@ -207,6 +222,7 @@ __global__ __launch_bounds__(64) void k_brick8_tangent_gl(
         E[3] = ux[0][1] + ux[1][0] + ux[0][0] * ux[0][1] + ux[1][0] * ux[1][1] + ux[2][0] * ux[2][1];
         E[4] = ux[0][2] + ux[2][0] + ux[0][0] * ux[0][2] + ux[1][0] * ux[1][2] + ux[2][0] * ux[2][2];
         E[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
+        brick8_subtract_thermal(e, sN[g], sNode, rank, tExp, T, E);
         for (int r = 0; r < 6; ++r) {
             double acc = 0.0;
             for (int k = 0; k < 6; ++k) acc += Cm[r + 6 * k] * E[k];
@@ -1142,7 +1158,9 @@ __global__ __launch_bounds__(128) void k_brick8_thermal_strain(int mode, const i
 // the consistent-mass inertia of a_e = c1 (c2 (u - pu - dt pv) - (1 - 2 beta) pa)
 // (r_getElRu :2684-2690, r_getElRum :2628-2653).  Every element gathers its own
 // nodes' values (the reference's r_getElementSolution indexes by local node
-// number, D4).  load[dof] -= R_e; elements of one launch share no node.
+// number, D4).  tExp (may be null) and T: the thermal strain subtracted from
+// E before the stress (brick8_subtract_thermal).  load[dof] -= R_e; elements
+// of one launch share no node.
 // One wave per element: lanes 0..7 own a Gauss point, lanes 0..23 a DOF
 // (component c = d / 8, node d % 8).
 __global__ __launch_bounds__(64) void k_brick8_residual(
@@ -1150,7 +1168,8 @@ __global__ __launch_bounds__(64) void k_brick8_residual(
     const int* __restrict__ elMat, const double* __restrict__ elScale, const double* __restrict__ matC,
     const double* __restrict__ elRho, const int* __restrict__ rank, int nS, const double* __restrict__ u,
     const double* __restrict__ pu, const double* __restrict__ pv, const double* __restrict__ pa, double c1, double c2,
-    double dt, double beta, double* __restrict__ load, const int* __restrict__ gate) {
+    double dt, double beta, const double* __restrict__ tExp, const double* __restrict__ T, double* __restrict__ load,
+    const int* __restrict__ gate) {
     if (gate && *gate == 0) return;
     const int idx = blockIdx.x;
     if (idx >= count) return;
@@ -1202,6 +1221,7 @@ __global__ __launch_bounds__(64) void k_brick8_residual(
         E[3] = ux[0][1] + ux[1][0] + ux[0][0] * ux[0][1] + ux[1][0] * ux[1][1] + ux[2][0] * ux[2][1];
         E[4] = ux[0][2] + ux[2][0] + ux[0][0] * ux[0][2] + ux[1][0] * ux[1][2] + ux[2][0] * ux[2][2];
         E[5] = ux[1][2] + ux[2][1] + ux[0][2] * ux[0][1] + ux[1][2] * ux[1][1] + ux[2][2] * ux[2][1];
+        brick8_subtract_thermal(e, sN[g], sNode, rank, tExp, T, E);
         const double sc = elScale[e];
         const double* Cm = matC + 36 * (size_t)elMat[e];
 #pragma unroll

@@ -132,3 +132,28 @@ def test_nonlinear_column_reaches_the_oracle_equilibrium():
     print(f"tip dx {dx:.6e} (linear {dxl:.6e}), dz {dz:.6e} (linear {dzl:.6e})", flush=True)
     assert 1e-5 < abs(dx - dxl) / abs(dxl) < 1e-2
     assert dz - dzl < -0.2 * dxl ** 2 / 2.0
+
+
+@pytest.mark.gpu
+def test_nonlinear_free_expansion_is_the_exact_stretch():
+    """A uniform temperature rise on statically determinate supports: with
+    the Green-Lagrange strain, E = a + a^2 / 2 = T alpha on the diagonal for
+    u = a x, so the nonlinear solve must stretch the block by
+    a = sqrt(1 + 2 T alpha) - 1 exactly (zero stress), where the linear
+    solve gives T alpha.  At T alpha = 1e-2 the two differ by 5e-5."""
+    from thermoelastic_cases import free_block
+
+    alpha, T0 = 1.0e-4, 100.0
+    m = free_block(alpha=alpha)
+    ctx = astro.Context(m, astro.Options(rtol=1e-12))
+    ctx.setd("nodeTemp", np.full(m.num_nodes, T0))
+    print("nonlinear free expansion", flush=True)
+    it = ctx.nonlinear_solve(max_it=20, tol=1e-13)
+    u = ctx.getd("nodeDisp")
+    nd = ctx.geti("nDofIndex").reshape(-1, 6)
+    ctx.close()
+    assert it <= 8
+    got = np.array([[u[nd[i, c] - 1] for c in range(3)] for i in range(m.num_nodes)])
+    a = np.sqrt(1.0 + 2.0 * T0 * alpha) - 1.0
+    assert abs(a - T0 * alpha) > 4e-5
+    assert np.abs(got - a * m.nodes).max() <= 1e-11 * np.abs(a * m.nodes).max()
